@@ -1,0 +1,200 @@
+"""Benchmark: time-to-rtol 1e-8 of IC(0)-preconditioned CG on the 3D 7-point
+Poisson problem on a 128^3 grid (n = 2,097,152, nnz = 14,581,760, FP64 CSR),
+b = A x_true with x_true ~ N(0, 1) from SplitMix64(0), x0 = 0.  Mixed
+precision (FP32 factor, sweeps and inner CG; FP64 residual and refinement)
+against the plain FP64 solve, plus the HBM throughput of the SpMV and of the
+triangular sweeps measured with HIP events on the library's stream.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+One "step" = one complete solve from x0 = 0 to relative residual 1e-8 with
+A, the factor and b resident in HBM.  Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+GRID = 128
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=GRID)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_time_ms(torch, stream_handle, fn, reps):
+    """Average duration of fn() (which launches on the library's stream),
+    measured with HIP events recorded on that same stream."""
+    ext = torch.cuda.ExternalStream(stream_handle)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    e0.record(ext)
+    for _ in range(reps):
+        fn()
+    e1.record(ext)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def cpu_baseline(A, b, n):
+    """The reference's own CPU path (FlexibleCG + IncompleteCholesky, compiled
+    from its sources with its release flags and 4 OpenMP threads) on the same
+    matrix and right-hand side: one complete solve."""
+    from oracle import ppo
+    R = ppo.Ref("omp")
+    hA = R.matrix(A)
+    tau = 1e-2
+    t0 = time.perf_counter()
+    hp = R.prec(hA, "ict", tau)
+    t1 = time.perf_counter()
+    out = R.cg(hA, hp, b, tol=1e-8, maxit=1000)
+    t2 = time.perf_counter()
+    R.L.refh_prec_free(hp)
+    R.free_matrix(hA)
+    return {"value": round(t2 - t1, 4), "unit": "s", "cores": 4, "kind": "reference",
+            "sample": (f"one full solve of the same {n}-row system: reference FlexibleCG + IncompleteCholesky "
+                       f"(tau={tau}, the reference has no IC(0)), OMP_NUM_THREADS=4 as compiled in "
+                       f"(THREAD_NUM), release flags; {out['iters']} iterations, setup {t1 - t0:.3f} s excluded"),
+            "iters": out["iters"], "converged": out["converged"], "setup_s": round(t1 - t0, 4),
+            "host_cpus": os.cpu_count()}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described "
+              "in DESIGN.md and is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import prispack_amd as pp
+    from prispack_amd import _capi
+
+    g = args.grid
+    A = pp.generateLaplacianCSR(g, g, g)
+    n, nnz = len(A[0]) - 1, int(A[0][-1])
+    log(f"[bench] 3D Poisson {g}^3 n={n} nnz={nnz}")
+    xt = pp.normal(n, 0)
+    M = pp.CSRMatrix(*A)
+    ctx = M.ctx
+    L = _capi.lib()
+    d_xt = torch.tensor(xt, device="cuda")
+    d_b = torch.zeros(n, dtype=torch.float64, device="cuda")
+    M.MatPVec(d_xt, d_b)
+    b = d_b.cpu().numpy()
+
+    t0 = time.perf_counter()
+    P = pp.IC0(M, make_f32=True)
+    P.setup()
+    setup_s = time.perf_counter() - t0
+    info = P.info()
+    log(f"[bench] IC(0) setup {setup_s:.2f} s, levels {info.levels_lower}/{info.levels_upper}")
+
+    def solve(precision):
+        cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision=precision)
+        cg.opts.record_history = 0
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        cg.solve(M, d_b, x)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        return dt, cg.report, x
+
+    for _ in range(args.warmup):
+        solve("fp64")
+        solve("mixed")
+    t64, tmx, rep64, repmx = [], [], None, None
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        dt, rep64, x64 = solve("fp64")
+        t64.append(dt)
+    for _ in range(args.steps):
+        dt, repmx, xmx = solve("mixed")
+        tmx.append(dt)
+    torch.cuda.synchronize()
+    err_mixed = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
+    err_true = float(torch.linalg.norm(xmx - d_xt) / torch.linalg.norm(d_xt))
+    log(f"[bench] fp64 {np.median(t64):.4f} s ({rep64.iters} it), mixed {np.median(tmx):.4f} s "
+        f"({repmx.outer_iters} outer / {repmx.inner_iters} inner)")
+
+    # ---- kernel throughput on the library's stream
+    stream = L.pp_ctx_stream(ctx.h)
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    px, py = C.c_void_p(d_xt.data_ptr()), C.c_void_p(y.data_ptr())
+    spmv_ms = kernel_time_ms(torch, stream, lambda: L.pp_spmv(ctx.h, M.h, 0, 1.0, px, 0.0, py), args.kernel_reps)
+    spmv_bytes = 12 * nnz + 20 * n + 4
+    x32 = d_xt.float()
+    y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+    p32x, p32y = C.c_void_p(x32.data_ptr()), C.c_void_p(y32.data_ptr())
+    spmv32_ms = kernel_time_ms(torch, stream, lambda: L.pp_spmv(ctx.h, M.h, 1, 1.0, p32x, 0.0, p32y),
+                               args.kernel_reps)
+    spmv32_bytes = 8 * nnz + 12 * n + 4
+    nnz_l = (nnz + n) // 2
+    trsv_bytes = 12 * nnz_l + 4 * (n + 1) + 16 * n + 4 * n
+    tv = torch.tensor(xt, device="cuda")
+    ptv = C.c_void_p(tv.data_ptr())
+    trsv_ms = kernel_time_ms(torch, stream, lambda: L.pp_precond_apply_part(ctx.h, P.h, 0, 0, ptv),
+                             max(3, args.kernel_reps // 10))
+    gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
+    log(f"[bench] SpMV fp64 {spmv_ms * 1e3:.1f} us {gbs(spmv_bytes, spmv_ms):.0f} GB/s; fp32 {spmv32_ms * 1e3:.1f} us; "
+        f"L-sweep fp64 {trsv_ms * 1e3:.1f} us {gbs(trsv_bytes, trsv_ms):.1f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(A, b, n)
+        except FileNotFoundError as e:
+            cpu = {"value": None, "unit": "s", "cores": 0, "kind": "reference", "sample": f"unavailable: {e}"}
+
+    med64, medmx = float(np.median(t64)), float(np.median(tmx))
+    dominant = {"kernel": "k_sweep_level (one forward sweep = %d level launches)" % info.levels_lower,
+                "bound": "hbm", "achieved": round(gbs(trsv_bytes, trsv_ms), 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(gbs(trsv_bytes, trsv_ms) / HBM_PEAK_GBS, 5),
+                "traffic": None, "bytes_per_launch": trsv_bytes, "avg_ms": round(trsv_ms, 4)}
+    out = {
+        "metric": "time-to-rtol 1e-8 (s) mixed-prec vs FP64 on GPU; SpMV/SpTRSV HBM GB/s",
+        "value": round(medmx, 5), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(medmx * 1e3, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-inner/f64-outer", "data": "synthetic",
+        "config": {"workload": f"IC(0)-PCG, 3D 7-point Poisson {g}^3 (n={n}, nnz={nnz}), b=A x_true, "
+                               "x_true~N(0,1) seed 0, rtol 1e-8", "precision": "mixed vs fp64"},
+        "fp64_s": round(med64, 5), "speedup_mixed_vs_fp64": round(med64 / medmx, 3),
+        "iters_fp64": rep64.iters, "iters_mixed_inner": repmx.inner_iters, "outer_mixed": repmx.outer_iters,
+        "relres_fp64": rep64.true_relres, "relres_mixed": repmx.true_relres,
+        "err_mixed_vs_fp64": err_mixed, "err_vs_xtrue": err_true, "setup_s": round(setup_s, 3),
+        "levels": [info.levels_lower, info.levels_upper],
+        "roofline": dominant,
+        "roofline_spmv": {"kernel": "k_spmv<double>", "bound": "hbm", "achieved": round(gbs(spmv_bytes, spmv_ms), 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs(spmv_bytes, spmv_ms) / HBM_PEAK_GBS, 4),
+                          "traffic": None, "bytes_per_launch": spmv_bytes, "avg_us": round(spmv_ms * 1e3, 2)},
+        "roofline_spmv_f32": {"achieved": round(gbs(spmv32_bytes, spmv32_ms), 1), "unit": "GB/s",
+                              "frac": round(gbs(spmv32_bytes, spmv32_ms) / HBM_PEAK_GBS, 4),
+                              "bytes_per_launch": spmv32_bytes, "avg_us": round(spmv32_ms * 1e3, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

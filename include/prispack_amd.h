@@ -1,0 +1,272 @@
+/*
+ * prispack_amd.h -- C ABI of the MI355X (gfx950) preconditioned-Krylov hot path.
+ *
+ * Plain pointers and sizes only.  Vectors are device pointers (hipMalloc'd by
+ * the caller, by pp_malloc, or borrowed from a framework such as torch) with
+ * an element type PP_F64 / PP_F32.  Matrices use 32-bit unsigned CSR indices,
+ * 0-based, rows sorted by column, FP64 values (an FP32 copy is made on demand
+ * for mixed precision).  All work is ordered on the context's HIP stream.
+ * Every entry point returns a status (PP_OK = 0); pp_last_error(ctx) gives
+ * the message.  Nothing here calls exit().
+ *
+ * Each entry point names the reference interface it replaces (PrisPack,
+ * file:line).
+ */
+#ifndef PRISPACK_AMD_H
+#define PRISPACK_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PP_ABI_VERSION 1
+
+/* ---- status codes (replace config/debug.h:96-135 exit()/THROW paths) ---- */
+enum {
+    PP_OK = 0,
+    PP_ERR_INVALID = 1,      /* bad argument / shape mismatch                    */
+    PP_ERR_HIP = 2,          /* HIP runtime error                                 */
+    PP_ERR_NOT_SPD = 3,      /* IC pivot <= 0 (IncompleteCholesky.cpp:189-193)    */
+    PP_ERR_ZERO_PIVOT = 4,   /* ILUT/ILDLT zero pivot (IncompleteLU.cpp:199,
+                                IncompleteLDLT.cpp:131)                           */
+    PP_ERR_ZERO_ROW = 5,     /* ILUT zero row (IncompleteLU.cpp:89)               */
+    PP_ERR_NOT_READY = 6,    /* preconditioner used before setup                  */
+    PP_ERR_BREAKDOWN = 7,    /* Krylov breakdown (zero/NaN denominator)           */
+    PP_ERR_NOT_FINITE = 8,   /* NaN/Inf in the right-hand side or the iterates    */
+    PP_ERR_NOMEM = 9,        /* device allocation failed                          */
+    PP_ERR_UNSUPPORTED = 10
+};
+
+enum { PP_F64 = 0, PP_F32 = 1 };
+
+typedef struct pp_ctx pp_ctx;
+typedef struct pp_csr pp_csr;
+typedef struct pp_precond pp_precond;
+
+/* ---- context: one per device + stream (replaces the per-call cudaSetDevice
+ *      and StreamController of include/CUDA/StreamController.cuh:19-77) ---- */
+int pp_ctx_create(int device, pp_ctx **out);
+/* Borrow an existing hipStream_t (e.g. torch.cuda.current_stream().cuda_stream).
+ * NULL restores the context's own stream. */
+int pp_ctx_set_stream(pp_ctx *ctx, void *hip_stream);
+void *pp_ctx_stream(pp_ctx *ctx);
+int pp_ctx_sync(pp_ctx *ctx);
+void pp_ctx_destroy(pp_ctx *ctx);
+const char *pp_last_error(const pp_ctx *ctx);
+/* message of the last failure that had no context (e.g. pp_ctx_create) */
+const char *pp_global_error(void);
+int pp_device_count(int *count);
+
+/* ---- device memory (DeviceVector / SyncDeviceVector storage,
+ *      include/VectorClass/DeviceVector.cuh:17-251) ---- */
+int pp_malloc(pp_ctx *ctx, size_t bytes, void **dptr);
+int pp_free(pp_ctx *ctx, void *dptr);
+int pp_memcpy_h2d(pp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int pp_memcpy_d2h(pp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int pp_memcpy_d2d(pp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int pp_memset_zero(pp_ctx *ctx, void *dst, size_t bytes);   /* fill kernel, graph-safe */
+
+/* ---- device CSR (DEVICE::DeviceCSRMatrix, include/MatrixClass/DeviceCSRMatrix.cuh:19-93;
+ *      host storage HOST::CSRMatrix, include/MatrixClass/CSRMatrix.h:19-31) ---- */
+/* Host arrays are copied to the device. */
+int pp_csr_create(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rowptr,
+                  const uint32_t *colidx, const double *values, pp_csr **out);
+/* Replace the values (same pattern).  on_device != 0: values is a device pointer. */
+int pp_csr_set_values(pp_ctx *ctx, pp_csr *A, const double *values, int on_device);
+int pp_csr_dims(const pp_csr *A, uint32_t *nrow, uint32_t *ncol, uint32_t *nnz);
+/* Device pointers of the stored arrays (values64 FP64; values32 NULL until an
+ * FP32 copy exists). */
+int pp_csr_device_ptrs(const pp_csr *A, const uint32_t **rowptr, const uint32_t **colidx,
+                       const double **values64, const float **values32);
+void pp_csr_destroy(pp_csr *A);
+
+/* ---- SpMV (CSRMatrix::MatPVec / MatPVec(alpha,..,beta), CSRMatrix.h:360-447) ----
+ * y = alpha * A x + beta * y  (beta == 0 never reads y).  dtype of x, y. */
+int pp_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x,
+            double beta, void *y);
+/* y = alpha * A^T x + beta * y   (CSRMatrix::transMatPVec, CSRMatrix.h:453-486;
+ * gather over an explicit transpose built on first use, no atomics). */
+int pp_spmv_t(pp_ctx *ctx, pp_csr *A, int dtype, double alpha, const void *x, double beta,
+              void *y);
+
+/* ---- BLAS-1 (HostVector::add/scale/innerProduct/norm_2,
+ *      include/VectorClass/BaseVector.h:279-530).  Dots accumulate in FP64 for
+ *      PP_F64 and FP32 for PP_F32; reductions are deterministic (fixed order). */
+int pp_dot(pp_ctx *ctx, int dtype, uint32_t n, const void *x, const void *y, double *result);
+int pp_nrm2(pp_ctx *ctx, int dtype, uint32_t n, const void *x, double *result);
+/* z = a x + b y (z may alias x or y) */
+int pp_axpby(pp_ctx *ctx, int dtype, uint32_t n, double a, const void *x, double b,
+             const void *y, void *z);
+int pp_scale(pp_ctx *ctx, int dtype, uint32_t n, double a, void *x);
+/* dtype conversion f64 <-> f32 (dst_dtype != src_dtype) */
+int pp_convert(pp_ctx *ctx, uint32_t n, int src_dtype, const void *src, int dst_dtype, void *dst);
+
+/* ---- preconditioners (BasePrecondition / TriangularPrecondition,
+ *      include/Preconditioner/BasePreconditon.h:34-359) ---- */
+enum {
+    PP_PC_NONE = 0,
+    PP_PC_ICT = 1,     /* IncompleteCholesky(A, tau)        IncompleteCholesky.cpp:96-295 */
+    PP_PC_ILUT = 2,    /* IncompleteLU(A, lfil, tau)        IncompleteLU.cpp:38-297 */
+    PP_PC_ILDLT = 3,   /* IncompleteLDLT(A, tau)            IncompleteLDLT.cpp:40-233 */
+    PP_PC_IC0 = 4,     /* zero-fill IC  (no reference counterpart) */
+    PP_PC_ILU0 = 5,    /* zero-fill ILU (no reference counterpart) */
+    PP_PC_AMSED = 6,   /* AMSEDPrecondition                  AMSEDPrecondition.cpp */
+    PP_PC_GMSLR = 7    /* GMSLRPrecondition                  GMSLRDPrecondition.cpp */
+};
+
+/* SpTRSV schedule */
+enum {
+    PP_TRSV_AUTO = 0,
+    PP_TRSV_LEVELS = 1,    /* one launch per level (graph-captured)               */
+    PP_TRSV_SYNCFREE = 2,  /* one launch per sweep, per-row ready flags            */
+    PP_TRSV_BLOCK = 3      /* one workgroup, rows of a level held in LDS           */
+};
+
+typedef struct {
+    int kind;            /* PP_PC_* */
+    double tau;          /* drop tolerance (ICT/ILUT/ILDLT) */
+    uint32_t lfil;       /* ILUT fill per row */
+    int trsv;            /* PP_TRSV_* */
+    int make_f32;        /* also keep an FP32 copy of the factor (mixed precision) */
+    /* AMSED/GMSLR */
+    uint32_t levels;     /* multilevel depth */
+    uint32_t ndom;       /* subdomains at the first level */
+    uint32_t rank;       /* low-rank correction rank */
+    double eps;          /* AMSED deflation bound */
+    uint32_t reserved[8];
+} pp_precond_opts;
+
+typedef struct {
+    int kind;
+    int ready;              /* isReady() */
+    uint32_t n;
+    uint32_t annz;          /* getOriginalNonZeroCount() */
+    uint32_t mnnz;          /* getPreconditionNonZeroCount() */
+    double fill_ratio;      /* getPreconditionFillinRatio() */
+    uint32_t levels_lower;  /* SpTRSV dependency depth */
+    uint32_t levels_upper;
+    double setup_ms;        /* host factorization + analysis + upload */
+    uint32_t fail_row;      /* row of a failed pivot */
+    uint32_t reserved[7];
+} pp_precond_info;
+
+void pp_precond_opts_default(pp_precond_opts *o);
+int pp_precond_create(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *opts, pp_precond **out);
+/* setup(): factorization on the host, level analysis, upload (BasePreconditon.h:102) */
+int pp_precond_setup(pp_ctx *ctx, pp_precond *M);
+/* y = M^{-1} x (MInvSolve in->out, BasePreconditon.h:113-128); y may equal x */
+int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
+/* part: 0 lower (MSolveLower), 1 diagonal (MSolveDiagonal), 2 upper (MSolveUpper),
+ * in place (TriangularPrecondition, BasePreconditon.h:200-303) */
+int pp_precond_apply_part(pp_ctx *ctx, pp_precond *M, int part, int dtype, void *x);
+int pp_precond_info_get(const pp_precond *M, pp_precond_info *info);
+/* Copy the host factor out (for inspection / parity tests).
+ * which: 0 L^T (IC, ILDLT strict upper), 1 L (ILU), 2 U (ILU).
+ * Call with NULL arrays to get nnz; d receives the diagonal (ILU 1/u_ii, ILDLT d). */
+int pp_precond_factor(const pp_precond *M, int which, uint32_t *nnz, uint32_t *rowptr,
+                      uint32_t *colidx, double *values, double *d);
+void pp_precond_destroy(pp_precond *M);
+
+/* Host-only factorization (no device needed): the setup() step of
+ * IncompleteCholesky / IncompleteLU / IncompleteLDLT (and IC(0), ILU(0)) on
+ * host arrays.  Returns PP_OK or the factorization failure; the handle is
+ * created in both cases (the partial factor of a failure is kept). */
+typedef struct pp_hfactor pp_hfactor;
+int pp_factor_host(const pp_precond_opts *opts, uint32_t n, const uint32_t *rowptr, const uint32_t *colidx,
+                   const double *values, pp_hfactor **out);
+/* which: 0 L^T, 1 L, 2 U; NULL arrays to query nnz; d as in pp_precond_factor */
+int pp_hfactor_get(const pp_hfactor *F, int which, uint32_t *nnz, uint32_t *rowptr, uint32_t *colidx,
+                   double *values, double *d);
+int pp_hfactor_info(const pp_hfactor *F, uint32_t *mnnz, uint32_t *fail_row, uint32_t *levels_lower,
+                    uint32_t *levels_upper);
+void pp_hfactor_destroy(pp_hfactor *F);
+
+/* ---- solvers (BaseSolver::solve, include/Solver/BaseSolver.h:9-17;
+ *      CG, source/Solver/CG.cpp:33-83; GMRES, source/Solver/GMRES.cpp:59-221) ---- */
+enum { PP_SOLVER_CG = 0, PP_SOLVER_GMRES = 1 };
+enum { PP_FP64 = 0, PP_MIXED = 1 };
+
+typedef struct {
+    int solver;          /* PP_SOLVER_* */
+    int flexible;        /* 1: FlexibleCG / FlexibleGMRES (default); 0: Normal* */
+    int precision;       /* PP_FP64 or PP_MIXED (FP32 inner, FP64 residual/refinement) */
+    double tol;          /* relative residual target */
+    uint32_t maxit;      /* CG: maxIter; GMRES: total budget (reference default 500) */
+    uint32_t restart;    /* GMRES(m) */
+    int reference_quirks;/* GMRES: reproduce the reference's converged-cycle
+                            back-substitution with dim = j (GMRES.cpp:194-201) */
+    uint32_t check_every;/* iterations between host convergence polls */
+    double inner_tol;    /* mixed: relative reduction asked of each inner solve */
+    uint32_t max_outer;  /* mixed: refinement steps */
+    int record_history;  /* keep per-iteration ||r||/||b|| */
+    uint32_t reserved[8];
+} pp_solve_opts;
+
+typedef struct {
+    int32_t iters;          /* getNumIter(), reference counting convention */
+    int32_t converged;      /* getConvergence() */
+    double final_error;     /* getError(): the recursive residual estimate */
+    double true_relres;     /* ||b - A x|| / ||b|| recomputed in FP64 at exit */
+    int32_t outer_iters;    /* mixed: refinement steps taken */
+    int32_t inner_iters;    /* mixed: total inner iterations */
+    double solve_ms;        /* stream time, setup excluded */
+    int32_t status;         /* PP_OK or the failure */
+    int32_t nhist;
+    uint32_t reserved[8];
+} pp_solve_report;
+
+void pp_solve_opts_default(pp_solve_opts *o, int solver);
+/* Device-resident solve: b, x are FP64 device pointers; x holds x0 on entry. */
+int pp_solve(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *opts,
+             const double *b, double *x, pp_solve_report *rep);
+/* Host in / host out (the BaseSolver::solve drop-in): uploads b and x0,
+ * solves, downloads x. */
+int pp_solve_host(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *opts,
+                  const double *b_host, double *x_host, pp_solve_report *rep);
+/* residual history of the last solve on this context (nhist entries) */
+int pp_solve_history(pp_ctx *ctx, double *out, uint32_t cap, uint32_t *count);
+
+/* ---- adjoint (no reference counterpart): solve A^T lambda = g with the same
+ * Krylov/preconditioner machinery on the explicit transpose. ---- */
+int pp_adjoint_solve(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *popts,
+                     const pp_solve_opts *opts, const double *g, double *lambda,
+                     pp_solve_report *rep);
+/* dL/dA on the pattern of A: grad[k] = -lambda[row(k)] * x[col(k)] */
+int pp_csr_outer_on_pattern(pp_ctx *ctx, const pp_csr *A, double scale, const double *lambda,
+                            const double *x, double *grad_values);
+
+/* ---- batched solves (no reference counterpart): nsys independent CSR systems
+ * stored back to back (block-diagonal), solved by one persistent launch of
+ * IC(0)-PCG with one workgroup per system.  Offsets are host arrays. ---- */
+typedef struct pp_batch pp_batch;
+int pp_batch_create(pp_ctx *ctx, uint32_t nsys, const uint32_t *row_off, const uint32_t *rowptr,
+                    const uint32_t *colidx, const double *values, pp_batch **out);
+int pp_batch_set_values(pp_ctx *ctx, pp_batch *B, const double *values, int on_device);
+/* b, x: device, concatenated (x holds x0).  precision PP_FP64 or PP_MIXED.
+ * iters_out (host, nsys) receives per-system iteration counts (may be NULL). */
+int pp_batch_solve(pp_ctx *ctx, pp_batch *B, const pp_solve_opts *opts, const double *b,
+                   double *x, int32_t *iters_out, pp_solve_report *rep);
+/* transpose-solve for the adjoint of every system at once */
+int pp_batch_solve_t(pp_ctx *ctx, pp_batch *B, const pp_solve_opts *opts, const double *g,
+                     double *lambda, int32_t *iters_out, pp_solve_report *rep);
+int pp_batch_tune(pp_ctx *ctx, pp_batch *B, int *chosen_block);
+void pp_batch_destroy(pp_batch *B);
+
+/* ---- model problems (ModelProblem.cpp:68-133 / MatrixTools.h:16-62 and the
+ * operators the reference lacks).  Two calls: first with NULL arrays for nnz. ---- */
+int pp_generate_laplacian(int nx, int ny, int nz, double ax, double ay, double az, double shift,
+                          uint32_t *n, uint32_t *nnz, uint32_t *rowptr, uint32_t *colidx,
+                          double *values);
+int pp_generate_upwind(int nx, int ny, int nz, double bx, double by, double bz, uint32_t *n,
+                       uint32_t *nnz, uint32_t *rowptr, uint32_t *colidx, double *values);
+int pp_generate_vardiff2d(int side, const double *kappa, uint32_t *n, uint32_t *nnz,
+                          uint32_t *rowptr, uint32_t *colidx, double *values);
+/* seeded SplitMix64 inputs: kind 0 uniform(-1,1), 1 normal(0,1) */
+int pp_fill_random(int kind, uint64_t seed, uint64_t n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRISPACK_AMD_H */

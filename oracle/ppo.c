@@ -630,7 +630,7 @@ ppo_prec *ppo_ic0(const ppo_csr *a) {
             lt->v[pos] = lv[t];
         }
     p->lt = lt;
-    p->mnnz = lnnz * 2 + n;
+    p->mnnz = (lnnz + n) * 2 + n; /* L^T entries incl. the diagonal, the IC convention */
     free(fill); free(lrp); free(lci); free(lv); free(diag); free(mark);
     return p;
 }

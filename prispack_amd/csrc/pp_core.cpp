@@ -1,0 +1,545 @@
+// pp_core.cpp -- context, device memory, CSR handles, SpMV/BLAS-1 entry points,
+// model-problem generators and seeded inputs of the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "pp_internal.h"
+
+namespace {
+std::mutex g_err_mu;
+std::string g_err;
+
+void set_global_error(const std::string &m) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = m;
+}
+}  // namespace
+
+void set_host_error(const std::string &m) { set_global_error(m); }
+
+// Runs f, turning exceptions into a status and a message on ctx.
+template <class F>
+static int guarded(pp_ctx *ctx, F &&f) {
+    try {
+        if (ctx) pp::set_device(ctx);
+        f();
+        return PP_OK;
+    } catch (const pp::Error &e) {
+        if (ctx) ctx->err = e.what();
+        else set_global_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->err = "host allocation failed";
+        else set_global_error("host allocation failed");
+        return PP_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        else set_global_error(e.what());
+        return PP_ERR_INVALID;
+    }
+}
+
+namespace pp {
+
+void set_device(pp_ctx *ctx) { PP_HIP(hipSetDevice(ctx->device)); }
+
+template <class T>
+T *dalloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw Error(PP_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return (T *)p;
+}
+template double *dalloc<double>(size_t);
+template float *dalloc<float>(size_t);
+template uint32_t *dalloc<uint32_t>(size_t);
+template int *dalloc<int>(size_t);
+template unsigned char *dalloc<unsigned char>(size_t);
+
+void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+void ensure_hist(pp_ctx *ctx, uint32_t cap) {
+    if (ctx->hist_cap >= cap) return;
+    dfree(ctx->hist);
+    ctx->hist = dalloc<double>(cap);
+    ctx->hist_cap = cap;
+}
+
+void *ws_get(pp_ctx *ctx, int idx, size_t bytes) {
+    PP_CHECK(idx >= 0 && idx < 32, PP_ERR_INVALID, "workspace index");
+    if (ctx->ws_bytes[idx] < bytes) {
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        dfree(ctx->ws[idx]);
+        ctx->ws[idx] = dalloc<unsigned char>(bytes);
+        ctx->ws_bytes[idx] = bytes;
+    }
+    return ctx->ws[idx];
+}
+
+double read_slot(pp_ctx *ctx, int slot) {
+    PP_HIP(hipMemcpyAsync(ctx->pinned, ctx->scal + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    return ctx->pinned[0];
+}
+
+void read_ctl(pp_ctx *ctx, int *out) {
+    PP_HIP(hipMemcpyAsync(ctx->pinned_ctl, ctx->ctl, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out, ctx->pinned_ctl, C_COUNT * sizeof(int));
+}
+
+void Graph::reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (g) (void)hipGraphDestroy(g);
+    exec = nullptr;
+    g = nullptr;
+}
+
+static void csr_upload_values_f32(pp_ctx *ctx, pp_csr *A) {
+    if (!A->v32) A->v32 = dalloc<float>(A->nnz);
+    launch_convert(ctx, A->nnz, PP_F64, A->v64, PP_F32, A->v32);
+    A->f32_dirty = false;
+}
+
+void csr_need_f32(pp_ctx *ctx, pp_csr *A) {
+    if (A->f32_dirty || !A->v32) csr_upload_values_f32(ctx, A);
+}
+
+static pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci,
+                       const double *v) {
+    PP_CHECK(rp != nullptr, PP_ERR_INVALID, "rowptr is NULL");
+    PP_CHECK(rp[0] == 0, PP_ERR_INVALID, "rowptr[0] must be 0");
+    uint32_t nnz = rp[nrow];
+    PP_CHECK(nnz == 0 || (ci && v), PP_ERR_INVALID, "colidx/values are NULL");
+    for (uint32_t i = 0; i < nrow; ++i) PP_CHECK(rp[i] <= rp[i + 1], PP_ERR_INVALID, "rowptr not monotone");
+    for (uint32_t k = 0; k < nnz; ++k) PP_CHECK(ci[k] < ncol, PP_ERR_INVALID, "column index out of range");
+    std::unique_ptr<pp_csr> A(new pp_csr);
+    A->ctx = ctx;
+    A->nrow = nrow;
+    A->ncol = ncol;
+    A->nnz = nnz;
+    A->h_rp.assign(rp, rp + nrow + 1);
+    A->h_ci.assign(ci, ci + nnz);
+    A->h_v.assign(v, v + nnz);
+    A->rp = dalloc<uint32_t>(nrow + 1);
+    A->ci = dalloc<uint32_t>(nnz);
+    A->v64 = dalloc<double>(nnz);
+    PP_HIP(hipMemcpyAsync(A->rp, rp, (nrow + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    if (nnz) {
+        PP_HIP(hipMemcpyAsync(A->ci, ci, nnz * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(A->v64, v, nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    }
+    std::vector<uint32_t> blocks;
+    build_row_blocks(A->h_rp, blocks);
+    A->nblocks = (uint32_t)blocks.size() - 1;
+    A->row_blocks = dalloc<uint32_t>(blocks.size());
+    PP_HIP(hipMemcpyAsync(A->row_blocks, blocks.data(), blocks.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    return A.release();
+}
+
+pp_csr *csr_create_host(pp_ctx *ctx, const HostCsr &H) {
+    return csr_new(ctx, H.n, H.n, H.rp.data(), H.ci.data(), H.v.data());
+}
+
+void csr_free(pp_csr *A) {
+    if (!A) return;
+    if (A->trans) csr_free(A->trans);
+    dfree(A->rp);
+    dfree(A->ci);
+    dfree(A->v64);
+    dfree(A->v32);
+    dfree(A->row_blocks);
+    delete A;
+}
+
+// Explicit transpose (gather form for A^T x, no atomics).
+pp_csr *csr_transpose(pp_ctx *ctx, pp_csr *A) {
+    if (A->trans) return A->trans;
+    HostCsr H;
+    H.n = A->nrow;
+    H.rp = A->h_rp;
+    H.ci = A->h_ci;
+    H.v = A->h_v;
+    PP_CHECK(A->nrow == A->ncol, PP_ERR_UNSUPPORTED, "transpose of a rectangular matrix");
+    HostCsr T = transpose(H);
+    A->trans = csr_create_host(ctx, T);
+    return A->trans;
+}
+
+}  // namespace pp
+
+using namespace pp;
+
+// ============================================================================ C ABI
+extern "C" {
+
+const char *pp_global_error(void) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_err.c_str();
+}
+
+const char *pp_last_error(const pp_ctx *ctx) { return ctx ? ctx->err.c_str() : pp_global_error(); }
+
+int pp_device_count(int *count) {
+    return guarded(nullptr, [&] {
+        PP_CHECK(count, PP_ERR_INVALID, "count is NULL");
+        int c = 0;
+        PP_HIP(hipGetDeviceCount(&c));
+        *count = c;
+    });
+}
+
+int pp_ctx_create(int device, pp_ctx **out) {
+    return guarded(nullptr, [&] {
+        PP_CHECK(out, PP_ERR_INVALID, "out is NULL");
+        int ndev = 0;
+        PP_HIP(hipGetDeviceCount(&ndev));
+        PP_CHECK(device >= 0 && device < ndev, PP_ERR_INVALID, "no such device");
+        std::unique_ptr<pp_ctx> c(new pp_ctx);
+        c->device = device;
+        PP_HIP(hipSetDevice(device));
+        PP_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+        c->stream = c->own_stream;
+        hipDeviceProp_t prop;
+        PP_HIP(hipGetDeviceProperties(&prop, device));
+        c->cu_count = prop.multiProcessorCount;
+        c->red.partials = dalloc<double>((size_t)S_COUNT * kMaxPartials);
+        c->red.counters = (unsigned *)dalloc<uint32_t>(S_COUNT);
+        c->scal = dalloc<double>(S_COUNT);
+        c->ctl = dalloc<int>(C_COUNT);
+        PP_HIP(hipMemset(c->red.counters, 0, S_COUNT * sizeof(unsigned)));
+        PP_HIP(hipMemset(c->scal, 0, S_COUNT * sizeof(double)));
+        PP_HIP(hipMemset(c->ctl, 0, C_COUNT * sizeof(int)));
+        PP_HIP(hipHostMalloc((void **)&c->pinned, 640 * sizeof(double), hipHostMallocDefault));
+        PP_HIP(hipHostMalloc((void **)&c->pinned_ctl, 64 * sizeof(int), hipHostMallocDefault));
+        PP_HIP(hipEventCreate(&c->ev0));
+        PP_HIP(hipEventCreate(&c->ev1));
+        ensure_hist(c.get(), 4096);
+        PP_HIP(hipDeviceSynchronize());
+        *out = c.release();
+    });
+}
+
+int pp_ctx_set_stream(pp_ctx *ctx, void *s) {
+    if (!ctx) return PP_ERR_INVALID;
+    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    return PP_OK;
+}
+
+void *pp_ctx_stream(pp_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int pp_ctx_sync(pp_ctx *ctx) {
+    return guarded(ctx, [&] { PP_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+void pp_ctx_destroy(pp_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    dfree(ctx->red.partials);
+    dfree(ctx->red.counters);
+    dfree(ctx->scal);
+    dfree(ctx->ctl);
+    dfree(ctx->hist);
+    for (int i = 0; i < 32; ++i) dfree(ctx->ws[i]);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->pinned_ctl) (void)hipHostFree(ctx->pinned_ctl);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+int pp_malloc(pp_ctx *ctx, size_t bytes, void **dptr) {
+    return guarded(ctx, [&] {
+        PP_CHECK(dptr, PP_ERR_INVALID, "dptr is NULL");
+        *dptr = dalloc<unsigned char>(bytes);
+    });
+}
+
+int pp_free(pp_ctx *ctx, void *dptr) {
+    return guarded(ctx, [&] { PP_HIP(hipFree(dptr)); });
+}
+
+int pp_memcpy_h2d(pp_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return guarded(ctx, [&] {
+        PP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pp_memcpy_d2h(pp_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return guarded(ctx, [&] {
+        PP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int pp_memcpy_d2d(pp_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    return guarded(ctx, [&] { PP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream)); });
+}
+
+int pp_memset_zero(pp_ctx *ctx, void *dst, size_t bytes) {
+    return guarded(ctx, [&] { launch_fill_zero(ctx, dst, bytes); });
+}
+
+int pp_csr_create(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci,
+                  const double *v, pp_csr **out) {
+    return guarded(ctx, [&] {
+        PP_CHECK(ctx && out, PP_ERR_INVALID, "ctx/out is NULL");
+        *out = csr_new(ctx, nrow, ncol, rp, ci, v);
+    });
+}
+
+int pp_csr_set_values(pp_ctx *ctx, pp_csr *A, const double *values, int on_device) {
+    return guarded(ctx, [&] {
+        PP_CHECK(A && values, PP_ERR_INVALID, "A/values is NULL");
+        if (on_device) {
+            PP_HIP(hipMemcpyAsync(A->v64, values, A->nnz * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+            PP_HIP(hipMemcpyAsync(A->h_v.data(), values, A->nnz * sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        } else {
+            std::memcpy(A->h_v.data(), values, A->nnz * sizeof(double));
+            PP_HIP(hipMemcpyAsync(A->v64, values, A->nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        }
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        A->f32_dirty = true;
+        if (A->trans) {
+            csr_free(A->trans);
+            A->trans = nullptr;
+        }
+    });
+}
+
+int pp_csr_dims(const pp_csr *A, uint32_t *nrow, uint32_t *ncol, uint32_t *nnz) {
+    if (!A) return PP_ERR_INVALID;
+    if (nrow) *nrow = A->nrow;
+    if (ncol) *ncol = A->ncol;
+    if (nnz) *nnz = A->nnz;
+    return PP_OK;
+}
+
+int pp_csr_device_ptrs(const pp_csr *A, const uint32_t **rowptr, const uint32_t **colidx, const double **v64,
+                       const float **v32) {
+    if (!A) return PP_ERR_INVALID;
+    if (rowptr) *rowptr = A->rp;
+    if (colidx) *colidx = A->ci;
+    if (v64) *v64 = A->v64;
+    if (v32) *v32 = A->f32_dirty ? nullptr : A->v32;
+    return PP_OK;
+}
+
+void pp_csr_destroy(pp_csr *A) {
+    if (!A) return;
+    (void)hipSetDevice(A->ctx->device);
+    (void)hipStreamSynchronize(A->ctx->stream);
+    csr_free(A);
+}
+
+int pp_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x, double beta, void *y) {
+    return guarded(ctx, [&] {
+        PP_CHECK(A && x && y, PP_ERR_INVALID, "NULL argument");
+        PP_CHECK(dtype == PP_F64 || dtype == PP_F32, PP_ERR_INVALID, "bad dtype");
+        if (dtype == PP_F32) csr_need_f32(ctx, const_cast<pp_csr *>(A));
+        launch_spmv(ctx, A, dtype, alpha, x, beta, y, -1, nullptr, false);
+    });
+}
+
+int pp_spmv_t(pp_ctx *ctx, pp_csr *A, int dtype, double alpha, const void *x, double beta, void *y) {
+    return guarded(ctx, [&] {
+        PP_CHECK(A && x && y, PP_ERR_INVALID, "NULL argument");
+        pp_csr *T = csr_transpose(ctx, A);
+        if (dtype == PP_F32) csr_need_f32(ctx, T);
+        launch_spmv(ctx, T, dtype, alpha, x, beta, y, -1, nullptr, false);
+    });
+}
+
+int pp_dot(pp_ctx *ctx, int dtype, uint32_t n, const void *x, const void *y, double *result) {
+    return guarded(ctx, [&] {
+        PP_CHECK(x && y && result, PP_ERR_INVALID, "NULL argument");
+        launch_dot(ctx, dtype, n, x, y, S_TMP1);
+        *result = read_slot(ctx, S_TMP1);
+    });
+}
+
+int pp_nrm2(pp_ctx *ctx, int dtype, uint32_t n, const void *x, double *result) {
+    return guarded(ctx, [&] {
+        PP_CHECK(x && result, PP_ERR_INVALID, "NULL argument");
+        launch_dot(ctx, dtype, n, x, x, S_TMP1);
+        *result = std::sqrt(read_slot(ctx, S_TMP1));
+    });
+}
+
+int pp_axpby(pp_ctx *ctx, int dtype, uint32_t n, double a, const void *x, double b, const void *y, void *z) {
+    return guarded(ctx, [&] {
+        PP_CHECK(x && y && z, PP_ERR_INVALID, "NULL argument");
+        launch_axpby(ctx, dtype, n, a, x, b, y, z);
+    });
+}
+
+int pp_scale(pp_ctx *ctx, int dtype, uint32_t n, double a, void *x) {
+    return guarded(ctx, [&] {
+        PP_CHECK(x, PP_ERR_INVALID, "NULL argument");
+        launch_axpby(ctx, dtype, n, a, x, 0.0, x, x);
+    });
+}
+
+int pp_convert(pp_ctx *ctx, uint32_t n, int sdt, const void *s, int ddt, void *d) {
+    return guarded(ctx, [&] {
+        PP_CHECK(s && d, PP_ERR_INVALID, "NULL argument");
+        launch_convert(ctx, n, sdt, s, ddt, d);
+    });
+}
+
+int pp_csr_outer_on_pattern(pp_ctx *ctx, const pp_csr *A, double scale, const double *lambda, const double *x,
+                            double *grad) {
+    return guarded(ctx, [&] {
+        PP_CHECK(A && lambda && x && grad, PP_ERR_INVALID, "NULL argument");
+        launch_outer_on_pattern(ctx, A, scale, lambda, x, grad);
+    });
+}
+
+// ---------------------------------------------------------------------------- inputs
+// Seeded SplitMix64 stream (Steele, Lea, Flood 2014).
+static inline uint64_t splitmix64(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+int pp_fill_random(int kind, uint64_t seed, uint64_t n, double *out) {
+    return guarded(nullptr, [&] {
+        PP_CHECK(out || n == 0, PP_ERR_INVALID, "out is NULL");
+        uint64_t s = seed;
+        const double inv53 = 1.0 / 9007199254740992.0;
+        if (kind == 0) {
+            for (uint64_t i = 0; i < n; ++i) out[i] = 2.0 * ((double)(splitmix64(s) >> 11) * inv53) - 1.0;
+        } else {
+            const double two_pi = 6.283185307179586476925286766559;
+            for (uint64_t i = 0; i < n; i += 2) {
+                double u1 = (double)((splitmix64(s) >> 11) + 1) * inv53;
+                double u2 = (double)(splitmix64(s) >> 11) * inv53;
+                double r = std::sqrt(-2.0 * std::log(u1));
+                double th = two_pi * u2;
+                out[i] = r * std::cos(th);
+                if (i + 1 < n) out[i + 1] = r * std::sin(th);
+            }
+        }
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------- generators
+namespace {
+struct StencilOut {
+    uint32_t *n, *nnz, *rp, *ci;
+    double *v;
+};
+
+// 7-point (or 5-point when nz == 1) stencil on a lexicographic x-fastest grid;
+// coef(dir) gives the off-diagonal for dir in {z-, y-, x-, x+, y+, z+}, written
+// in ascending column order.
+template <class C>
+int stencil(int nx, int ny, int nz, double diag, C coef, StencilOut o) {
+    return guarded(nullptr, [&] {
+        PP_CHECK(nx > 0 && ny > 0 && nz > 0, PP_ERR_INVALID, "grid dims must be positive");
+        const int64_t n = (int64_t)nx * ny * nz;
+        PP_CHECK(n < (int64_t)UINT32_MAX / 8, PP_ERR_INVALID, "grid too large for 32-bit indices");
+        const int64_t nxy = (int64_t)nx * ny;
+        uint64_t nnz = 0;
+        for (int64_t p = 0; p < n; ++p) {
+            int64_t k = p / nxy, i = (p - k * nxy) / nx, j = p - k * nxy - i * nx;
+            nnz += 1 + (k > 0) + (i > 0) + (j > 0) + (j < nx - 1) + (i < ny - 1) + (k < nz - 1);
+        }
+        if (o.n) *o.n = (uint32_t)n;
+        if (o.nnz) *o.nnz = (uint32_t)nnz;
+        if (!o.rp) return;
+        PP_CHECK(o.ci && o.v, PP_ERR_INVALID, "colidx/values NULL");
+        uint32_t q = 0;
+        o.rp[0] = 0;
+        for (int64_t p = 0; p < n; ++p) {
+            int64_t k = p / nxy, i = (p - k * nxy) / nx, j = p - k * nxy - i * nx;
+            auto put = [&](int64_t c, double v) {
+                o.ci[q] = (uint32_t)c;
+                o.v[q++] = v;
+            };
+            if (k > 0) put(p - nxy, coef(0));
+            if (i > 0) put(p - nx, coef(1));
+            if (j > 0) put(p - 1, coef(2));
+            put(p, diag);
+            if (j < nx - 1) put(p + 1, coef(3));
+            if (i < ny - 1) put(p + nx, coef(4));
+            if (k < nz - 1) put(p + nxy, coef(5));
+            o.rp[p + 1] = q;
+        }
+    });
+}
+}  // namespace
+
+extern "C" {
+
+// generateLaplacianCOO + transCOO2CSR (ModelProblem.cpp:68-133, MatrixTools.h:16-62):
+// 2D when a dimension is 1 (diagonal 4), else 3D (diagonal 6); centred
+// convection: "+" neighbour -1 - alpha, "-" neighbour -1 + alpha; diagonal -= shift.
+int pp_generate_laplacian(int nx, int ny, int nz, double ax, double ay, double az, double shift, uint32_t *n,
+                          uint32_t *nnz, uint32_t *rp, uint32_t *ci, double *v) {
+    int is2d = (nx == 1) + (ny == 1) + (nz == 1) > 0;
+    double vd = (is2d ? 4.0 : 6.0) - shift;
+    double c[6] = {-1.0 + az, -1.0 + ay, -1.0 + ax, -1.0 - ax, -1.0 - ay, -1.0 - az};
+    return stencil(nx, ny, nz, vd, [&](int d) { return c[d]; }, StencilOut{n, nnz, rp, ci, v});
+}
+
+// First-order upwind convection-diffusion scaled by h^2, h = 1/(nx+1):
+// diagonal 2*dim + h*sum(beta), upstream neighbour -1 - h*beta_k, downstream -1.
+int pp_generate_upwind(int nx, int ny, int nz, double bx, double by, double bz, uint32_t *n, uint32_t *nnz,
+                       uint32_t *rp, uint32_t *ci, double *v) {
+    double h = 1.0 / (double)(nx + 1);
+    double vd = nz > 1 ? 6.0 + h * (bx + by + bz) : 4.0 + h * (bx + by);
+    double c[6] = {-1.0 - h * bz, -1.0 - h * by, -1.0 - h * bx, -1.0, -1.0, -1.0};
+    return stencil(nx, ny, nz, vd, [&](int d) { return c[d]; }, StencilOut{n, nnz, rp, ci, v});
+}
+
+// -div(kappa grad u) on a side x side grid, Dirichlet; interior faces take the
+// harmonic mean of the two nodal coefficients, boundary faces the node's own.
+int pp_generate_vardiff2d(int side, const double *kappa, uint32_t *n, uint32_t *nnz, uint32_t *rp, uint32_t *ci,
+                          double *v) {
+    return guarded(nullptr, [&] {
+        PP_CHECK(side > 0, PP_ERR_INVALID, "side must be positive");
+        const int64_t N = (int64_t)side * side;
+        uint64_t cnt = (uint64_t)N * 5 - 4 * (uint64_t)side;
+        if (n) *n = (uint32_t)N;
+        if (nnz) *nnz = (uint32_t)cnt;
+        if (!rp) return;
+        PP_CHECK(kappa && ci && v, PP_ERR_INVALID, "NULL argument");
+        uint32_t q = 0;
+        rp[0] = 0;
+        for (int64_t p = 0; p < N; ++p) {
+            int64_t j = p / side, i = p - j * side;
+            double kp = kappa[p];
+            auto hm = [&](int64_t o) { return 2.0 * kp * kappa[o] / (kp + kappa[o]); };
+            double fs = j > 0 ? hm(p - side) : kp;
+            double fw = i > 0 ? hm(p - 1) : kp;
+            double fe = i < side - 1 ? hm(p + 1) : kp;
+            double fn = j < side - 1 ? hm(p + side) : kp;
+            if (j > 0) { ci[q] = (uint32_t)(p - side); v[q++] = -fs; }
+            if (i > 0) { ci[q] = (uint32_t)(p - 1); v[q++] = -fw; }
+            ci[q] = (uint32_t)p; v[q++] = ((fs + fw) + fe) + fn;
+            if (i < side - 1) { ci[q] = (uint32_t)(p + 1); v[q++] = -fe; }
+            if (j < side - 1) { ci[q] = (uint32_t)(p + side); v[q++] = -fn; }
+            rp[p + 1] = q;
+        }
+    });
+}
+
+}  // extern "C"

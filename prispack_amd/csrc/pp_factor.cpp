@@ -1,0 +1,378 @@
+// pp_factor.cpp -- host-side preconditioner setup (the factorization stays on
+// the host, as in the reference; the device only applies the factor).
+//
+// The threshold factorizations must produce the reference's factor bit for bit
+// (iteration-count parity depends on it), so they take the same steps in the
+// same order: the kept pattern depends on every drop decision and the values
+// on the order of the floating-point updates.  The zero-fill IC(0)/ILU(0)
+// have no reference counterpart.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "pp_internal.h"
+
+namespace pp {
+
+static constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+HostCsr transpose(const HostCsr &A) {
+    HostCsr T;
+    T.n = A.n;
+    T.rp.assign(A.n + 1, 0);
+    const uint32_t nnz = A.rp[A.n];
+    T.ci.resize(nnz);
+    T.v.resize(nnz);
+    for (uint32_t k = 0; k < nnz; ++k) T.rp[A.ci[k] + 1]++;
+    for (uint32_t i = 0; i < A.n; ++i) T.rp[i + 1] += T.rp[i];
+    std::vector<uint32_t> next(T.rp.begin(), T.rp.end() - 1);
+    for (uint32_t i = 0; i < A.n; ++i)
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            uint32_t p = next[A.ci[k]]++;
+            T.ci[p] = i;
+            T.v[p] = A.v[k];
+        }
+    return T;
+}
+
+// Threshold IC / LDL^T, left-looking by column
+// (IncompleteCholesky.cpp:96-246, IncompleteLDLT.cpp:40-186).
+//
+// Column j accumulates the lower part of row j of A and subtracts the
+// contributions of every factored column k whose next unconsumed entry lies in
+// row j.  Those columns sit in per-row singly-linked lists that share one
+// array: `chain[r]` is the head of row r's list until row r is factored and
+// column r's successor afterwards.  Entries with |value| < tau * sum_{i>=j}
+// |a_ij| (measured before scaling) are dropped.  The result is L^T by rows:
+// IC stores sqrt(pivot) first and divides by it; LDL^T stores no diagonal
+// (unit), keeps the pivot in d and divides by it.
+int factor_ict(const HostCsr &A, double tau, bool ldlt, HostCsr &lt, std::vector<double> &d, uint32_t &fail_row) {
+    const uint32_t n = A.n;
+    std::vector<uint8_t> active(n, 0);
+    std::vector<uint32_t> rows;
+    rows.reserve(64);
+    std::vector<double> acc(n, 0.0);
+    std::vector<uint32_t> chain(n, kNone), cursor(n, 0);
+    lt.n = n;
+    lt.rp.assign(n + 1, 0);
+    lt.ci.clear();
+    lt.v.clear();
+    lt.ci.reserve((size_t)A.rp[n] * 2);
+    lt.v.reserve((size_t)A.rp[n] * 2);
+    if (ldlt) d.assign(n, 0.0);
+    int status = PP_OK;
+
+    for (uint32_t j = 0; j < n; ++j) {
+        rows.clear();
+        double colsum = 0.0;
+        for (uint32_t k = A.rp[j]; k < A.rp[j + 1]; ++k) {
+            const uint32_t i = A.ci[k];
+            if (i < j) continue;
+            colsum += std::fabs(A.v[k]);
+            if (!active[i]) {
+                active[i] = 1;
+                acc[i] = 0.0;
+                rows.push_back(i);
+            }
+            acc[i] += A.v[k];
+        }
+        const double thr = colsum * tau;
+
+        for (uint32_t k = chain[j]; k != kNone;) {
+            uint32_t pos = cursor[k];
+            const double ljk = lt.v[pos];
+            const double dk = ldlt ? d[k] : 1.0;
+            const uint32_t end = lt.rp[k + 1];
+            for (uint32_t q = pos; q < end; ++q) {
+                const uint32_t i = lt.ci[q];
+                if (!active[i]) {
+                    active[i] = 1;
+                    acc[i] = 0.0;
+                    rows.push_back(i);
+                }
+                if (ldlt) acc[i] -= lt.v[q] * ljk * dk;
+                else acc[i] -= lt.v[q] * ljk;
+            }
+            const uint32_t after = chain[k];
+            if (++pos < end) {  // move column k to the list of its next row
+                cursor[k] = pos;
+                const uint32_t r = lt.ci[pos];
+                chain[k] = chain[r];
+                chain[r] = k;
+            }
+            k = after;
+        }
+
+        double piv = acc[j];
+        if (!ldlt) {
+            if (!(piv > 0.0)) {
+                status = PP_ERR_NOT_SPD;
+                fail_row = j;
+                for (uint32_t r = j; r < n; ++r) lt.rp[r + 1] = (uint32_t)lt.ci.size();
+                break;
+            }
+            piv = std::sqrt(piv);
+            acc[j] = piv;
+            lt.ci.push_back(j);
+            lt.v.push_back(piv);
+        } else {
+            if (std::fabs(piv) < 1e-15) {
+                status = PP_ERR_ZERO_PIVOT;
+                fail_row = j;
+                for (uint32_t r = j; r < n; ++r) lt.rp[r + 1] = (uint32_t)lt.ci.size();
+                break;
+            }
+            d[j] = piv;
+        }
+        cursor[j] = (uint32_t)lt.ci.size();
+        std::sort(rows.begin(), rows.end());
+        bool linked = false;
+        for (uint32_t i : rows) {
+            active[i] = 0;
+            if (i == j) continue;
+            const double lij = acc[i];
+            if (!(std::fabs(lij) >= thr)) continue;
+            if (!linked) {  // column j enters the list of its first off-diagonal row
+                linked = true;
+                chain[j] = chain[i];
+                chain[i] = j;
+            }
+            lt.ci.push_back(i);
+            lt.v.push_back(lij / piv);
+        }
+        lt.rp[j + 1] = (uint32_t)lt.ci.size();
+    }
+    return status;
+}
+
+// Partial selection that brings the `keep` largest |a| to the front and fixes
+// the stored order of an ILUT row; same pivoting steps as qsplit
+// (include/VectorClass/VectorTools.h:159-201), whose output order is
+// observable in the factor.
+static void select_largest(double *a, int *ind, int n, int keep) {
+    const int cut = keep - 1;
+    int lo = 0, hi = n - 1;
+    if (cut < lo || cut > hi) return;
+    for (;;) {
+        int mid = lo;
+        const double key = std::fabs(a[mid]);
+        for (int j = lo + 1; j <= hi; ++j)
+            if (std::fabs(a[j]) > key) {
+                ++mid;
+                std::swap(a[mid], a[j]);
+                std::swap(ind[mid], ind[j]);
+            }
+        std::swap(a[mid], a[lo]);
+        std::swap(ind[mid], ind[lo]);
+        if (mid == cut) return;
+        if (mid > cut) hi = mid - 1;
+        else lo = mid + 1;
+    }
+}
+
+// ILUT(lfil, tau), row-wise IKJ elimination (IncompleteLU.cpp:38-266).
+// Work row layout: lower entries at [0, nl), diagonal at i, upper entries at
+// i+1 .. i+nu; where[col] maps a column to its slot (-1 when absent).
+int factor_ilut(const HostCsr &A, uint32_t lfil, double tau, HostCsr &L, HostCsr &U, std::vector<double> &D,
+                uint32_t &fail_row) {
+    const int n = (int)A.n;
+    std::vector<int> where(n, -1), col(n, 0), sel(n, 0);
+    std::vector<double> w(n, 0.0), mag(n, 0.0);
+    D.assign(n, 0.0);
+    L.n = U.n = (uint32_t)n;
+    L.rp.assign(n + 1, 0);
+    U.rp.assign(n + 1, 0);
+    L.ci.clear(); L.v.clear(); U.ci.clear(); U.v.clear();
+    int status = PP_OK;
+    for (int i = 0; i < n; ++i) {
+        const uint32_t b = A.rp[i], e = A.rp[i + 1];
+        double rownorm = 0.0;
+        for (uint32_t k = b; k < e; ++k) rownorm += std::fabs(A.v[k]);
+        if (rownorm < 1e-15) {
+            status = PP_ERR_ZERO_ROW;
+            fail_row = (uint32_t)i;
+            break;
+        }
+        rownorm /= (double)(e - b);
+        const double droptol = tau * rownorm;
+        int nl = 0, nu = 0;
+        col[i] = i;
+        w[i] = 0.0;
+        where[i] = i;
+        for (uint32_t k = b; k < e; ++k) {
+            const int c = (int)A.ci[k];
+            if (c < i) {
+                where[c] = nl; col[nl] = c; w[nl] = A.v[k]; ++nl;
+            } else if (c == i) {
+                w[i] = A.v[k];
+            } else {
+                ++nu;
+                where[c] = i + nu; col[i + nu] = c; w[i + nu] = A.v[k];
+            }
+        }
+        for (int j = 0; j < nl; ++j) {
+            // bring the smallest remaining column to slot j
+            int piv_row = col[j], piv_slot = j;
+            for (int k = j + 1; k < nl; ++k)
+                if (col[k] < piv_row) { piv_row = col[k]; piv_slot = k; }
+            if (piv_slot != j) {
+                const int c = col[j];
+                col[j] = col[piv_slot]; col[piv_slot] = c;
+                where[piv_row] = j; where[c] = piv_slot;
+                std::swap(w[j], w[piv_slot]);
+            }
+            const double mult = w[j] * D[piv_row];
+            w[j] = mult;
+            where[piv_row] = -1;
+            for (uint32_t k = U.rp[piv_row]; k < U.rp[piv_row + 1]; ++k) {
+                const int c = (int)U.ci[k];
+                const int slot = where[c];
+                const double upd = -mult * U.v[k];
+                if (slot == -1 && std::fabs(upd) < droptol) continue;
+                if (c < i) {
+                    if (slot == -1) { col[nl] = c; where[c] = nl; w[nl] = upd; ++nl; }
+                    else w[slot] += upd;
+                } else if (slot != -1) {
+                    w[slot] += upd;
+                } else if (std::fabs(upd) > droptol) {
+                    ++nu;
+                    col[i + nu] = c; where[c] = i + nu; w[i + nu] = upd;
+                }
+                // |upd| == droptol for a new upper entry: dropped (undefined in the reference)
+            }
+        }
+        where[i] = -1;
+        for (int j = 0; j < nu; ++j) where[col[i + j + 1]] = -1;
+        if (std::fabs(w[i]) < 1e-15) {
+            status = PP_ERR_ZERO_PIVOT;
+            fail_row = (uint32_t)i;
+            break;
+        }
+        D[i] = 1.0 / w[i];
+        int keep = std::min(nl, (int)lfil);
+        for (int j = 0; j < nl; ++j) { mag[j] = std::fabs(w[j]); sel[j] = j; }
+        select_largest(mag.data(), sel.data(), nl, keep);
+        for (int j = 0; j < keep; ++j) { L.ci.push_back((uint32_t)col[sel[j]]); L.v.push_back(w[sel[j]]); }
+        L.rp[i + 1] = (uint32_t)L.ci.size();
+        keep = std::min(nu, (int)lfil);
+        for (int j = 0; j < nu; ++j) { mag[j] = std::fabs(w[i + j + 1]); sel[j] = i + j + 1; }
+        select_largest(mag.data(), sel.data(), nu, keep);
+        for (int j = 0; j < keep; ++j) { U.ci.push_back((uint32_t)col[sel[j]]); U.v.push_back(w[sel[j]]); }
+        U.rp[i + 1] = (uint32_t)U.ci.size();
+    }
+    if (status != PP_OK)
+        for (int r = (int)fail_row; r < n; ++r) {
+            L.rp[r + 1] = (uint32_t)L.ci.size();
+            U.rp[r + 1] = (uint32_t)U.ci.size();
+        }
+    return status;
+}
+
+// IC(0): zero-fill incomplete Cholesky on the pattern of tril(A), row by row:
+//   L_ik = (a_ik - sum_{j<k in rows i and k} L_ij L_kj) / L_kk,  L_ii = sqrt(a_ii - sum_k L_ik^2).
+// Returned as L^T with the diagonal first (the threshold-IC layout).
+int factor_ic0(const HostCsr &A, HostCsr &lt, uint32_t &fail_row) {
+    const uint32_t n = A.n;
+    std::vector<uint32_t> rp(n + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t c = 0;
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) c += A.ci[k] < i;
+        rp[i + 1] = rp[i] + c;
+    }
+    std::vector<uint32_t> ci(rp[n]);
+    std::vector<double> lv(rp[n]), diag(n);
+    std::vector<int64_t> slot(n, -1);
+    int status = PP_OK;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t q = rp[i];
+        double aii = 0.0;
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const uint32_t c = A.ci[k];
+            if (c < i) { ci[q] = c; lv[q] = A.v[k]; slot[c] = q; ++q; }
+            else if (c == i) aii = A.v[k];
+        }
+        for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) {
+            const uint32_t k = ci[t];
+            double s = lv[t];
+            for (uint32_t u = rp[k]; u < rp[k + 1]; ++u) {
+                const int64_t p = slot[ci[u]];
+                if (p >= 0) s -= lv[p] * lv[u];
+            }
+            lv[t] = s / diag[k];
+        }
+        double s = aii;
+        for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) s -= lv[t] * lv[t];
+        for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) slot[ci[t]] = -1;
+        if (!(s > 0.0)) {
+            if (status == PP_OK) { status = PP_ERR_NOT_SPD; fail_row = i; }
+            s = 1.0;
+        }
+        diag[i] = std::sqrt(s);
+    }
+    lt.n = n;
+    lt.rp.assign(n + 1, 0);
+    for (uint32_t t = 0; t < rp[n]; ++t) lt.rp[ci[t] + 1]++;
+    for (uint32_t j = 0; j < n; ++j) lt.rp[j + 1] += lt.rp[j] + 1;
+    lt.ci.resize(rp[n] + n);
+    lt.v.resize(rp[n] + n);
+    std::vector<uint32_t> fill(n);
+    for (uint32_t j = 0; j < n; ++j) {
+        lt.ci[lt.rp[j]] = j;
+        lt.v[lt.rp[j]] = diag[j];
+        fill[j] = lt.rp[j] + 1;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) {
+            const uint32_t p = fill[ci[t]]++;
+            lt.ci[p] = i;
+            lt.v[p] = lv[t];
+        }
+    return status;
+}
+
+// ILU(0): zero-fill ILU in IKJ form on the pattern of A; L strict lower (unit),
+// U strict upper, D = 1/u_ii.
+int factor_ilu0(const HostCsr &A, HostCsr &L, HostCsr &U, std::vector<double> &D, uint32_t &fail_row) {
+    const uint32_t n = A.n;
+    std::vector<double> w(A.v);
+    std::vector<uint32_t> dpos(n, kNone);
+    std::vector<int64_t> slot(n, -1);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.ci[k] == i) dpos[i] = k;
+    int status = PP_OK;
+    for (uint32_t i = 0; i < n && status == PP_OK; ++i) {
+        if (dpos[i] == kNone) { status = PP_ERR_ZERO_PIVOT; fail_row = i; break; }
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) slot[A.ci[k]] = k;
+        for (uint32_t t = A.rp[i]; t < A.rp[i + 1]; ++t) {
+            const uint32_t k = A.ci[t];
+            if (k >= i) break;
+            w[t] /= w[dpos[k]];
+            for (uint32_t u = dpos[k] + 1; u < A.rp[k + 1]; ++u) {
+                const int64_t p = slot[A.ci[u]];
+                if (p >= 0) w[p] -= w[t] * w[u];
+            }
+        }
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) slot[A.ci[k]] = -1;
+        if (std::fabs(w[dpos[i]]) < 1e-15) { status = PP_ERR_ZERO_PIVOT; fail_row = i; }
+    }
+    L.n = U.n = n;
+    L.rp.assign(n + 1, 0);
+    U.rp.assign(n + 1, 0);
+    L.ci.clear(); L.v.clear(); U.ci.clear(); U.v.clear();
+    D.assign(n, 0.0);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+            const uint32_t c = A.ci[k];
+            if (c < i) { L.ci.push_back(c); L.v.push_back(w[k]); }
+            else if (c > i) { U.ci.push_back(c); U.v.push_back(w[k]); }
+        }
+        L.rp[i + 1] = (uint32_t)L.ci.size();
+        U.rp[i + 1] = (uint32_t)U.ci.size();
+        D[i] = dpos[i] != kNone ? 1.0 / w[dpos[i]] : 0.0;
+    }
+    return status;
+}
+
+}  // namespace pp

@@ -1,0 +1,217 @@
+// pp_internal.h -- shared types of the MI355X hot-path library (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/prispack_amd.h"
+
+namespace pp {
+
+// A failure carried to the C ABI boundary as a status + message.
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+#define PP_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::pp::Error(PP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define PP_CHECK(cond, status, msg)                        \
+    do {                                                   \
+        if (!(cond)) throw ::pp::Error((status), (msg));   \
+    } while (0)
+
+constexpr int kBlock = 256;             // threads per workgroup of the streaming kernels
+constexpr uint32_t kSpmvNnzBlock = 2048; // nnz staged in LDS per SpMV workgroup
+constexpr uint32_t kSpmvRowsBlock = 256; // rows per SpMV workgroup (one per thread)
+constexpr int kMaxPartials = 4096;       // reduction partials per slot
+
+// Device scalar slots used by the Krylov kernels (FP64 even in mixed mode).
+enum Slot : int {
+    S_RR = 0,   // (r, r)
+    S_DV,       // (d, A d)
+    S_DR,       // (d, r)
+    S_WV,       // (w, v)
+    S_BB,       // (b, b)
+    S_TMP0,
+    S_TMP1,
+    S_TMP2,
+    S_TMP3,
+    S_COUNT = 16
+};
+// Integer control words.
+enum Ctl : int {
+    C_STOP = 0,   // bodies skip when set
+    C_CONV,       // converged
+    C_M,          // reference loop index at exit
+    C_CHECK,      // index of the next convergence check
+    C_LAST,       // the next body is the last one
+    C_STATUS,     // PP_OK / PP_ERR_BREAKDOWN / PP_ERR_NOT_FINITE
+    C_COUNT = 16
+};
+
+// Deterministic reduction workspace: partials + arrival counter per slot.
+struct RedWs {
+    double *partials = nullptr;  // [S_COUNT][kMaxPartials]
+    unsigned *counters = nullptr; // [S_COUNT]
+};
+
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void reset();
+    ~Graph() { reset(); }
+};
+
+}  // namespace pp
+
+struct pp_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cu_count = 256;
+    pp::RedWs red;
+    double *scal = nullptr;       // device scalars [S_COUNT]
+    int *ctl = nullptr;           // device control words [C_COUNT]
+    double *hist = nullptr;       // device residual history
+    uint32_t hist_cap = 0;
+    double *pinned = nullptr;     // pinned host mirror (scalars, 64 doubles)
+    int *pinned_ctl = nullptr;    // pinned host mirror of ctl
+    std::vector<double> last_hist;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *ws[32] = {};             // cached solver workspaces
+    size_t ws_bytes[32] = {};
+};
+
+namespace pp {
+
+// Device-side copy of a gather-form triangular sweep, rows stored in level order.
+template <class T>
+struct DevSweep {
+    uint32_t n = 0, nnz = 0, nlev = 0;
+    uint32_t *perm = nullptr;   // [n] row processed at position k
+    uint32_t *rp = nullptr;     // [n+1] positions in level order
+    uint32_t *ci = nullptr;     // [nnz] dependency rows (original numbering)
+    T *val = nullptr;           // [nnz]
+    T *diag = nullptr;          // [n] (original numbering) or nullptr
+    std::vector<uint32_t> level_ptr;  // host [nlev+1]
+    uint32_t max_level_rows = 0;
+    void free_all();
+};
+
+// Host gather-form sweep: row i = s_i -> (b_i - sum val*x[ci]) op diag_i.
+struct HostSweep {
+    enum Mode { DIVIDE = 0, UNIT = 1, MULTIPLY = 2, PREDIVIDE = 3 };
+    int mode = DIVIDE;
+    bool lower = true;
+    uint32_t n = 0;
+    std::vector<uint32_t> rp, ci;
+    std::vector<double> val, diag;
+};
+
+}  // namespace pp
+
+struct pp_csr {
+    pp_ctx *ctx = nullptr;
+    uint32_t nrow = 0, ncol = 0, nnz = 0;
+    uint32_t *rp = nullptr, *ci = nullptr;
+    double *v64 = nullptr;
+    float *v32 = nullptr;
+    uint32_t *row_blocks = nullptr;  // adaptive SpMV row blocks
+    uint32_t nblocks = 0;
+    pp_csr *trans = nullptr;         // explicit transpose, built on demand
+    std::vector<uint32_t> h_rp, h_ci;
+    std::vector<double> h_v;
+    bool f32_dirty = true;
+};
+
+struct pp_precond {
+    pp_ctx *ctx = nullptr;
+    pp_csr *A = nullptr;
+    pp_precond_opts opts{};
+    int ready = 0;
+    uint32_t n = 0, mnnz = 0, fail_row = 0;
+    double setup_ms = 0;
+    // host factor (reference layouts)
+    std::vector<uint32_t> lt_rp, lt_ci, l_rp, l_ci, u_rp, u_ci;
+    std::vector<double> lt_v, l_v, u_v, d;
+    // device sweeps
+    pp::HostSweep hlow, hup;
+    bool has_diag_stage = false;  // ILDLT: D^{-1} between the sweeps (folded into upper)
+    pp::DevSweep<double> low64, up64;
+    pp::DevSweep<float> low32, up32;
+    bool have32 = false;
+    void *multilevel = nullptr;   // AMSED / GMSLR state
+};
+
+namespace pp {
+
+// ---- core helpers (pp_core.cpp) ----
+void set_device(pp_ctx *ctx);
+template <class T>
+T *dalloc(size_t n);
+void dfree(void *p);
+void ensure_hist(pp_ctx *ctx, uint32_t cap);
+void *ws_get(pp_ctx *ctx, int idx, size_t bytes);
+
+// ---- host factorizations (pp_factor.cpp) ----
+struct HostCsr {
+    uint32_t n = 0;
+    std::vector<uint32_t> rp, ci;
+    std::vector<double> v;
+};
+int factor_ict(const HostCsr &A, double tau, bool ldlt, HostCsr &lt, std::vector<double> &d,
+               uint32_t &fail_row);
+int factor_ilut(const HostCsr &A, uint32_t lfil, double tau, HostCsr &L, HostCsr &U,
+                std::vector<double> &D, uint32_t &fail_row);
+int factor_ic0(const HostCsr &A, HostCsr &lt, uint32_t &fail_row);
+int factor_ilu0(const HostCsr &A, HostCsr &L, HostCsr &U, std::vector<double> &D, uint32_t &fail_row);
+HostCsr transpose(const HostCsr &A);
+
+// ---- level analysis + upload (pp_precond.cpp) ----
+template <class T>
+void upload_sweep(const HostSweep &s, DevSweep<T> &d);
+
+// ---- kernels (pp_kernels.hip) ----
+void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x,
+                 double beta, void *y, int dot_slot, const void *dot_with, bool dot_first);
+void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes);
+void launch_convert(pp_ctx *ctx, uint32_t n, int sdt, const void *s, int ddt, void *d);
+void launch_axpby(pp_ctx *ctx, int dtype, uint32_t n, double a, const void *x, double b,
+                  const void *y, void *z);
+void launch_dot(pp_ctx *ctx, int dtype, uint32_t n, const void *x, const void *y, int slot);
+void launch_outer_on_pattern(pp_ctx *ctx, const pp_csr *A, double scale, const double *lam,
+                             const double *x, double *g);
+void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks);
+
+template <class T>
+void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x);
+
+// PCG device steps (templated on the working precision)
+template <class T>
+void pcg_d_update(pp_ctx *ctx, uint32_t n, bool first, const T *w, T *d, const T *r);
+template <class T>
+void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, double tol,
+                   uint32_t maxit, int hist_on);
+template <class T>
+void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v);
+void ctl_init(pp_ctx *ctx, int first_check, int last);
+void copy_slot(pp_ctx *ctx, int dst, int src);
+
+// Reads slot s to the host (synchronizes the stream).
+double read_slot(pp_ctx *ctx, int slot);
+void read_ctl(pp_ctx *ctx, int *out);
+void write_ctl(pp_ctx *ctx, const int *vals);
+
+}  // namespace pp

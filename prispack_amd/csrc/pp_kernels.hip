@@ -1,0 +1,495 @@
+// pp_kernels.hip -- CDNA4 (gfx950) kernels of the Krylov hot path: CSR SpMV,
+// BLAS-1 with deterministic reductions, level-scheduled triangular sweeps and
+// the PCG update steps.  64-wide wavefronts, 256-thread workgroups.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "pp_internal.h"
+
+namespace pp {
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions.  A workgroup reduces with a fixed butterfly inside
+// each wavefront and a fixed left-to-right pass over the wavefronts; the last
+// workgroup to finish sums the per-workgroup partials in index order.  The
+// result depends only on the grid shape, never on scheduling.
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Valid in thread 0 only.
+__device__ inline double block_sum(double v, double *sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; ++w) t += sh[w];
+    }
+    __syncthreads();
+    return t;
+}
+
+__device__ inline void store_agent(double *p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double load_agent(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publishes this workgroup's partial; the last workgroup returns true with the
+// total in *total (thread 0) and writes it to *result.
+__device__ inline bool finish_partial(double blockval, double *partials, unsigned *counter,
+                                      double *result, double *sh, double *total) {
+    __shared__ int am_last;
+    if (threadIdx.x == 0) {
+        store_agent(&partials[blockIdx.x], blockval);
+        __threadfence();
+        unsigned t = atomicAdd(counter, 1u);
+        am_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return false;
+    __threadfence();
+    double s = 0.0;
+    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) s += load_agent(&partials[i]);
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        *result = s;
+        *total = s;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
+static inline unsigned grid_for(pp_ctx *ctx, uint64_t n, int per_thread = 1) {
+    uint64_t g = (n + (uint64_t)kBlock * per_thread - 1) / ((uint64_t)kBlock * per_thread);
+    uint64_t cap = (uint64_t)ctx->cu_count * 8;
+    if (cap > (uint64_t)kMaxPartials) cap = kMaxPartials;
+    if (g > cap) g = cap;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV, "CSR-stream": each workgroup owns a row block whose nonzeros fit
+// in LDS.  Phase 1 streams values and column indices with unit stride
+// (coalesced) and stores the products a_k * x[col_k]; phase 2 gives every row
+// one thread that sums its products in storage order starting from zero --
+// the exact operation sequence of CSRMatrix::MatPVecWithRawPtr
+// (CSRMatrix.h:360-377), so the FP64 result is bit-identical to it.  A row
+// longer than the LDS stage gets a whole workgroup and a tree sum.
+// Optional fused dot: partial sum of w_row * y_row.
+// ---------------------------------------------------------------------------
+template <class T, bool DOT>
+__global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
+                                              const uint32_t *__restrict__ ci,
+                                              const T *__restrict__ val, const T *__restrict__ x,
+                                              T *__restrict__ y, const uint32_t *__restrict__ rb,
+                                              uint32_t nb, T alpha, T beta, int beta_zero,
+                                              const T *__restrict__ dotw, double *partials,
+                                              unsigned *counter, double *result) {
+    __shared__ T prod[kSpmvNnzBlock];
+    __shared__ double sh[4];
+    double dacc = 0.0;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint32_t r0 = rb[b], r1 = rb[b + 1];
+        const uint32_t k0 = rp[r0], k1 = rp[r1];
+        const uint32_t cnt = k1 - k0;
+        if (cnt <= kSpmvNnzBlock) {
+            for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) prod[k] = val[k0 + k] * x[ci[k0 + k]];
+            __syncthreads();
+            if (threadIdx.x < r1 - r0) {
+                const uint32_t row = r0 + threadIdx.x;
+                uint32_t a = rp[row] - k0;
+                const uint32_t e = rp[row + 1] - k0;
+                T s = 0;
+                for (; a < e; ++a) s += prod[a];
+                T out = alpha * s;
+                if (!beta_zero) out += beta * y[row];
+                y[row] = out;
+                if (DOT) dacc += (double)dotw[row] * (double)out;
+            }
+            __syncthreads();
+        } else {
+            double s = 0.0;
+            for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) s += (double)(val[k] * x[ci[k]]);
+            s = block_sum(s, sh);
+            if (threadIdx.x == 0) {
+                T out = alpha * (T)s;
+                if (!beta_zero) out += beta * y[r0];
+                y[r0] = out;
+                if (DOT) dacc += (double)dotw[r0] * (double)out;
+            }
+            __syncthreads();
+        }
+    }
+    if (DOT) {
+        double tot;
+        double bs = block_sum(dacc, sh);
+        finish_partial(bs, partials, counter, result, sh, &tot);
+    }
+}
+
+void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks) {
+    blocks.clear();
+    const uint32_t n = (uint32_t)rp.size() - 1;
+    uint32_t r = 0;
+    blocks.push_back(0);
+    while (r < n) {
+        uint32_t start = r;
+        uint32_t nnz0 = rp[r];
+        if (rp[r + 1] - nnz0 > kSpmvNnzBlock) {
+            r = r + 1;  // a long row alone
+        } else {
+            while (r < n && r - start < kSpmvRowsBlock && rp[r + 1] - nnz0 <= kSpmvNnzBlock) ++r;
+        }
+        blocks.push_back(r);
+    }
+}
+
+void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x, double beta,
+                 void *y, int dot_slot, const void *dot_with, bool) {
+    unsigned grid = std::min<unsigned>(A->nblocks, kMaxPartials);
+    if (grid == 0) return;
+    double *part = dot_slot >= 0 ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
+    unsigned *cnt = dot_slot >= 0 ? ctx->red.counters + dot_slot : nullptr;
+    double *res = dot_slot >= 0 ? ctx->scal + dot_slot : nullptr;
+    int bz = beta == 0.0;
+    if (dtype == PP_F64) {
+        if (dot_slot >= 0)
+            k_spmv<double, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v64, (const double *)x,
+                                                                 (double *)y, A->row_blocks, A->nblocks, alpha, beta, bz,
+                                                                 (const double *)dot_with, part, cnt, res);
+        else
+            k_spmv<double, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v64, (const double *)x,
+                                                                  (double *)y, A->row_blocks, A->nblocks, alpha, beta,
+                                                                  bz, nullptr, nullptr, nullptr, nullptr);
+    } else {
+        if (dot_slot >= 0)
+            k_spmv<float, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v32, (const float *)x,
+                                                                (float *)y, A->row_blocks, A->nblocks, (float)alpha,
+                                                                (float)beta, bz, (const float *)dot_with, part, cnt, res);
+        else
+            k_spmv<float, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v32, (const float *)x,
+                                                                 (float *)y, A->row_blocks, A->nblocks, (float)alpha,
+                                                                 (float)beta, bz, nullptr, nullptr, nullptr, nullptr);
+    }
+    PP_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// BLAS-1
+// ---------------------------------------------------------------------------
+__global__ void k_fill_zero(uint32_t *p, size_t nwords) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes) {
+    size_t words = bytes / 4;
+    if (words) k_fill_zero<<<grid_for(ctx, words), kBlock, 0, ctx->stream>>>((uint32_t *)p, words);
+    if (bytes % 4) PP_HIP(hipMemsetAsync((char *)p + words * 4, 0, bytes % 4, ctx->stream));
+    PP_HIP(hipGetLastError());
+}
+
+template <class S, class D>
+__global__ void k_convert(uint32_t n, const S *s, D *d) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] = (D)s[i];
+}
+
+void launch_convert(pp_ctx *ctx, uint32_t n, int sdt, const void *s, int ddt, void *d) {
+    unsigned g = grid_for(ctx, n);
+    if (sdt == PP_F64 && ddt == PP_F32)
+        k_convert<double, float><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)s, (float *)d);
+    else if (sdt == PP_F32 && ddt == PP_F64)
+        k_convert<float, double><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)s, (double *)d);
+    else if (sdt == PP_F64)
+        k_convert<double, double><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)s, (double *)d);
+    else
+        k_convert<float, float><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)s, (float *)d);
+    PP_HIP(hipGetLastError());
+}
+
+// z = a x + b y, evaluated as (a*x) + (b*y) with both products rounded, the
+// order of HostVector::add (BaseVector.h:279-301).
+template <class T>
+__global__ void k_axpby(uint32_t n, T a, const T *x, T b, const T *y, T *z) {
+#pragma clang fp contract(off)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        T t1 = a * x[i];
+        T t2 = b * y[i];
+        z[i] = t1 + t2;
+    }
+}
+
+void launch_axpby(pp_ctx *ctx, int dtype, uint32_t n, double a, const void *x, double b, const void *y,
+                  void *z) {
+    unsigned g = grid_for(ctx, n);
+    if (dtype == PP_F64)
+        k_axpby<double><<<g, kBlock, 0, ctx->stream>>>(n, a, (const double *)x, b, (const double *)y, (double *)z);
+    else
+        k_axpby<float><<<g, kBlock, 0, ctx->stream>>>(n, (float)a, (const float *)x, (float)b, (const float *)y,
+                                                      (float *)z);
+    PP_HIP(hipGetLastError());
+}
+
+// dot with the accumulator of the working precision inside a thread, FP64
+// across threads/workgroups.
+template <class T>
+__global__ __launch_bounds__(256) void k_dot(uint32_t n, const T *x, const T *y, double *partials,
+                                             unsigned *counter, double *result) {
+    __shared__ double sh[4];
+    T acc = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += x[i] * y[i];
+    double bs = block_sum((double)acc, sh);
+    double tot;
+    finish_partial(bs, partials, counter, result, sh, &tot);
+}
+
+void launch_dot(pp_ctx *ctx, int dtype, uint32_t n, const void *x, const void *y, int slot) {
+    unsigned g = grid_for(ctx, n, 4);
+    double *part = ctx->red.partials + (size_t)slot * kMaxPartials;
+    unsigned *cnt = ctx->red.counters + slot;
+    if (dtype == PP_F64)
+        k_dot<double><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)x, (const double *)y, part, cnt,
+                                                     ctx->scal + slot);
+    else
+        k_dot<float><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)x, (const float *)y, part, cnt,
+                                                    ctx->scal + slot);
+    PP_HIP(hipGetLastError());
+}
+
+__global__ void k_outer_on_pattern(uint32_t n, const uint32_t *rp, const uint32_t *ci, double scale,
+                                   const double *lam, const double *x, double *g) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double li = scale * lam[i];
+        for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) g[k] = li * x[ci[k]];
+    }
+}
+
+void launch_outer_on_pattern(pp_ctx *ctx, const pp_csr *A, double scale, const double *lam, const double *x,
+                             double *g) {
+    k_outer_on_pattern<<<grid_for(ctx, A->nrow), kBlock, 0, ctx->stream>>>(A->nrow, A->rp, A->ci, scale, lam, x, g);
+    PP_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Level-scheduled triangular sweep, one launch per level.  Rows of a level
+// are stored contiguously (level order); each thread finishes one row by
+// subtracting its dependencies in storage order with no fused multiply-add,
+// which is the operation sequence of the reference's serial sweeps
+// (IncompleteCholesky.cpp:250-295, IncompleteLU.cpp:269-297,
+// IncompleteLDLT.cpp:189-233) -- bit-identical results in FP64.
+// mode: 0 divide by diag, 1 unit, 2 multiply by diag, 3 divide first.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_sweep_level(const uint32_t *__restrict__ perm,
+                                                     const uint32_t *__restrict__ rp,
+                                                     const uint32_t *__restrict__ ci,
+                                                     const T *__restrict__ val, const T *__restrict__ diag,
+                                                     int mode, const T *b, T *x, uint32_t k0, uint32_t k1,
+                                                     const int *stop) {
+#pragma clang fp contract(off)
+    const uint32_t k = k0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= k1) return;
+    if (stop && *stop) return;
+    const uint32_t i = perm[k];
+    T s = b[i];
+    if (mode == 3) s = s / diag[i];
+    for (uint32_t e = rp[k]; e < rp[k + 1]; ++e) {
+        T p = val[e] * x[ci[e]];
+        s = s - p;
+    }
+    if (mode == 0) s = s / diag[i];
+    else if (mode == 2) s = s * diag[i];
+    x[i] = s;
+}
+
+template <class T>
+void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x) {
+    const int *stop = ctx->ctl + C_STOP;
+    for (uint32_t l = 0; l < s.nlev; ++l) {
+        uint32_t k0 = s.level_ptr[l], k1 = s.level_ptr[l + 1];
+        unsigned g = (k1 - k0 + kBlock - 1) / kBlock;
+        k_sweep_level<T><<<g, kBlock, 0, ctx->stream>>>(s.perm, s.rp, s.ci, s.val, s.diag, mode, b, x, k0, k1, stop);
+    }
+    PP_HIP(hipGetLastError());
+}
+template void launch_sweep<double>(pp_ctx *, const DevSweep<double> &, int, const double *, double *);
+template void launch_sweep<float>(pp_ctx *, const DevSweep<float> &, int, const float *, float *);
+
+// ---------------------------------------------------------------------------
+// PCG steps (FlexibleCG, source/Solver/CG.cpp:49-79):
+//   d = w - beta d with beta = (w, A d_old) / (d_old, A d_old)   [fused (d, r)]
+//   v = A d                                                       [fused (d, v)]
+//   x += alpha d, r -= alpha v with alpha = (d, r) / (d, v)       [fused (r, r) + check]
+// Every kernel returns at once when the stop word is set, so iterations
+// launched past convergence cost only their launch.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_pcg_d(uint32_t n, int first, const T *__restrict__ w,
+                                               T *__restrict__ d, const T *__restrict__ r,
+                                               const double *scal, const int *ctl, double *partials,
+                                               unsigned *counter, double *result) {
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    if (ctl[C_STOP]) return;
+    const T beta = first ? (T)0 : (T)(scal[S_WV] / scal[S_DV]);
+    T acc = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        T dn;
+        if (first) dn = w[i];
+        else {
+            T t1 = w[i];
+            T t2 = (-beta) * d[i];
+            dn = t1 + t2;
+        }
+        d[i] = dn;
+        acc += dn * r[i];
+    }
+    double bs = block_sum((double)acc, sh), tot;
+    finish_partial(bs, partials, counter, result, sh, &tot);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pcg_wv(uint32_t n, const T *__restrict__ w, const T *__restrict__ v,
+                                                const int *ctl, double *partials, unsigned *counter,
+                                                double *result) {
+    __shared__ double sh[4];
+    if (ctl[C_STOP]) return;
+    T acc = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += w[i] * v[i];
+    double bs = block_sum((double)acc, sh), tot;
+    finish_partial(bs, partials, counter, result, sh, &tot);
+}
+
+// Convergence check of CG.cpp:57-63 done by the last workgroup: the check
+// index c counts the reference loop index i.
+__device__ inline void pcg_check(double rr, const double *scal, int *ctl, double tol, uint32_t maxit,
+                                 double *hist, uint32_t hist_cap) {
+    if (ctl[C_LAST]) {  // the body after the final check has run
+        ctl[C_STOP] = 1;
+        ctl[C_M] = (int)maxit;
+        return;
+    }
+    const int c = ctl[C_CHECK];
+    const double norm = sqrt(rr) / sqrt(scal[S_BB]);
+    if (hist && (uint32_t)c - 1u < hist_cap) hist[c - 1] = norm;
+    double *s = const_cast<double *>(scal);
+    s[S_TMP0] = norm;
+    if (!(norm == norm) || isinf(norm)) {
+        ctl[C_STATUS] = PP_ERR_NOT_FINITE;
+        ctl[C_STOP] = 1;
+        ctl[C_M] = c;
+        return;
+    }
+    if (norm < tol) {
+        ctl[C_CONV] = 1;
+        ctl[C_STOP] = 1;
+        ctl[C_M] = c;
+        return;
+    }
+    if ((uint32_t)c >= maxit - 1) ctl[C_LAST] = 1;
+    ctl[C_CHECK] = c + 1;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_pcg_xr(uint32_t n, T *__restrict__ x, T *__restrict__ r,
+                                                const T *__restrict__ d, const T *__restrict__ v,
+                                                const double *scal, int *ctl, double *partials,
+                                                unsigned *counter, double *result, double tol, uint32_t maxit,
+                                                double *hist, uint32_t hist_cap) {
+#pragma clang fp contract(off)
+    __shared__ double sh[4];
+    if (ctl[C_STOP]) return;
+    const double den = scal[S_DV];
+    if (den == 0.0 || !(den == den)) {  // (d, A d) = 0 or NaN: Krylov breakdown
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctl[C_STATUS] = PP_ERR_BREAKDOWN;
+            ctl[C_STOP] = 1;
+            ctl[C_M] = ctl[C_CHECK];
+        }
+        return;
+    }
+    const T alpha = (T)(scal[S_DR] / den);
+    T acc = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        T t1 = alpha * d[i];
+        T xn = x[i] + t1;
+        x[i] = xn;
+        T t2 = (-alpha) * v[i];
+        T rn = r[i] + t2;
+        r[i] = rn;
+        acc += rn * rn;
+    }
+    double bs = block_sum((double)acc, sh), tot;
+    if (finish_partial(bs, partials, counter, result, sh, &tot) && threadIdx.x == 0)
+        pcg_check(tot, scal, ctl, tol, maxit, hist, hist_cap);
+}
+
+template <class T>
+void pcg_d_update(pp_ctx *ctx, uint32_t n, bool first, const T *w, T *d, const T *r) {
+    unsigned g = grid_for(ctx, n, 4);
+    k_pcg_d<T><<<g, kBlock, 0, ctx->stream>>>(n, first ? 1 : 0, w, d, r, ctx->scal, ctx->ctl,
+                                              ctx->red.partials + (size_t)S_DR * kMaxPartials,
+                                              ctx->red.counters + S_DR, ctx->scal + S_DR);
+    PP_HIP(hipGetLastError());
+}
+
+template <class T>
+void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v) {
+    unsigned g = grid_for(ctx, n, 4);
+    k_pcg_wv<T><<<g, kBlock, 0, ctx->stream>>>(n, w, v, ctx->ctl, ctx->red.partials + (size_t)S_WV * kMaxPartials,
+                                               ctx->red.counters + S_WV, ctx->scal + S_WV);
+    PP_HIP(hipGetLastError());
+}
+
+template <class T>
+void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, double tol, uint32_t maxit,
+                   int hist_on) {
+    unsigned g = grid_for(ctx, n, 4);
+    k_pcg_xr<T><<<g, kBlock, 0, ctx->stream>>>(n, x, r, d, v, ctx->scal, ctx->ctl,
+                                               ctx->red.partials + (size_t)S_RR * kMaxPartials,
+                                               ctx->red.counters + S_RR, ctx->scal + S_RR, tol, maxit,
+                                               hist_on ? ctx->hist : nullptr, hist_on ? ctx->hist_cap : 0);
+    PP_HIP(hipGetLastError());
+}
+
+template void pcg_d_update<double>(pp_ctx *, uint32_t, bool, const double *, double *, const double *);
+template void pcg_d_update<float>(pp_ctx *, uint32_t, bool, const float *, float *, const float *);
+template void pcg_wv<double>(pp_ctx *, uint32_t, const double *, const double *);
+template void pcg_wv<float>(pp_ctx *, uint32_t, const float *, const float *);
+template void pcg_xr_update<double>(pp_ctx *, uint32_t, double *, double *, const double *, const double *, double,
+                                    uint32_t, int);
+template void pcg_xr_update<float>(pp_ctx *, uint32_t, float *, float *, const float *, const float *, double,
+                                   uint32_t, int);
+
+// Sets the control words for a fresh PCG solve (graph-safe: a kernel, no memset).
+__global__ void k_ctl_init(int *ctl, int first_check, int last) {
+    if (threadIdx.x < C_COUNT) ctl[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        ctl[C_CHECK] = first_check;
+        ctl[C_LAST] = last;
+    }
+}
+
+void ctl_init(pp_ctx *ctx, int first_check, int last) {
+    k_ctl_init<<<1, 64, 0, ctx->stream>>>(ctx->ctl, first_check, last);
+    PP_HIP(hipGetLastError());
+}
+
+// scal[dst] = scal[src] (used to keep (b,b) while reusing slots)
+__global__ void k_copy_slot(double *scal, int dst, int src) { scal[dst] = scal[src]; }
+void copy_slot(pp_ctx *ctx, int dst, int src) {
+    k_copy_slot<<<1, 1, 0, ctx->stream>>>(ctx->scal, dst, src);
+    PP_HIP(hipGetLastError());
+}
+
+}  // namespace pp

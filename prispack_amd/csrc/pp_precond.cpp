@@ -1,0 +1,477 @@
+// pp_precond.cpp -- preconditioner objects of the C ABI: host factorization,
+// conversion to gather-form sweeps, dependency-level analysis and upload
+// (rows stored in level order so each level is a contiguous, coalesced range).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "pp_internal.h"
+
+namespace pp {
+pp_csr *csr_create_host(pp_ctx *ctx, const HostCsr &H);
+void csr_free(pp_csr *A);
+void csr_need_f32(pp_ctx *ctx, pp_csr *A);
+// multilevel preconditioners (pp_multilevel.cpp)
+void ml_setup(pp_ctx *ctx, pp_precond *M);
+void ml_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
+void ml_free(pp_precond *M);
+
+template <class T>
+void DevSweep<T>::free_all() {
+    dfree(perm);
+    dfree(rp);
+    dfree(ci);
+    dfree(val);
+    dfree(diag);
+    perm = rp = ci = nullptr;
+    val = diag = nullptr;
+    level_ptr.clear();
+    n = nnz = nlev = 0;
+}
+template struct DevSweep<double>;
+template struct DevSweep<float>;
+
+// Levels of the dependency DAG: level(i) = 1 + max level of its dependencies.
+static void levels_of(const HostSweep &s, std::vector<uint32_t> &perm, std::vector<uint32_t> &level_ptr) {
+    const uint32_t n = s.n;
+    std::vector<uint32_t> lev(n, 0);
+    uint32_t maxlev = 0;
+    auto visit = [&](uint32_t i) {
+        uint32_t l = 0;
+        for (uint32_t k = s.rp[i]; k < s.rp[i + 1]; ++k) l = std::max(l, lev[s.ci[k]] + 1);
+        lev[i] = l;
+        maxlev = std::max(maxlev, l);
+    };
+    if (s.lower)
+        for (uint32_t i = 0; i < n; ++i) visit(i);
+    else
+        for (uint32_t i = n; i-- > 0;) visit(i);
+    const uint32_t nlev = n ? maxlev + 1 : 0;
+    level_ptr.assign(nlev + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) level_ptr[lev[i] + 1]++;
+    for (uint32_t l = 0; l < nlev; ++l) level_ptr[l + 1] += level_ptr[l];
+    std::vector<uint32_t> fill(level_ptr.begin(), level_ptr.end() - (nlev ? 1 : 0));
+    perm.assign(n, 0);
+    for (uint32_t i = 0; i < n; ++i) perm[fill[lev[i]]++] = i;
+}
+
+template <class T>
+void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
+    d.free_all();
+    std::vector<uint32_t> perm, lp;
+    levels_of(s, perm, lp);
+    const uint32_t n = s.n;
+    std::vector<uint32_t> rp(n + 1, 0), ci;
+    std::vector<T> val;
+    ci.reserve(s.ci.size());
+    val.reserve(s.val.size());
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t i = perm[k];
+        for (uint32_t e = s.rp[i]; e < s.rp[i + 1]; ++e) {
+            ci.push_back(s.ci[e]);
+            val.push_back((T)s.val[e]);
+        }
+        rp[k + 1] = (uint32_t)ci.size();
+    }
+    d.n = n;
+    d.nnz = (uint32_t)ci.size();
+    d.nlev = lp.empty() ? 0 : (uint32_t)lp.size() - 1;
+    d.level_ptr = lp;
+    d.max_level_rows = 0;
+    for (uint32_t l = 0; l < d.nlev; ++l) d.max_level_rows = std::max(d.max_level_rows, lp[l + 1] - lp[l]);
+    d.perm = dalloc<uint32_t>(n);
+    d.rp = dalloc<uint32_t>(n + 1);
+    d.ci = dalloc<uint32_t>(d.nnz);
+    d.val = dalloc<T>(d.nnz);
+    PP_HIP(hipMemcpy(d.perm, perm.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    PP_HIP(hipMemcpy(d.rp, rp.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (d.nnz) {
+        PP_HIP(hipMemcpy(d.ci, ci.data(), d.nnz * sizeof(uint32_t), hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(d.val, val.data(), d.nnz * sizeof(T), hipMemcpyHostToDevice));
+    }
+    if (s.mode != HostSweep::UNIT) {
+        std::vector<T> dg(n);
+        for (uint32_t i = 0; i < n; ++i) dg[i] = (T)s.diag[i];
+        d.diag = dalloc<T>(n);
+        PP_HIP(hipMemcpy(d.diag, dg.data(), n * sizeof(T), hipMemcpyHostToDevice));
+    }
+}
+template void upload_sweep<double>(const HostSweep &, DevSweep<double> &);
+template void upload_sweep<float>(const HostSweep &, DevSweep<float> &);
+
+// Gather form of a sweep given by the rows of `rows_src`:
+//   skip_first: the first entry of each row is its diagonal (IC L^T layout).
+static HostSweep sweep_from_rows(const HostCsr &R, bool lower, int mode, bool diag_first,
+                                 const std::vector<double> *diag_vec) {
+    HostSweep s;
+    s.n = R.n;
+    s.lower = lower;
+    s.mode = mode;
+    s.rp.assign(R.n + 1, 0);
+    s.diag.assign(R.n, 1.0);
+    for (uint32_t i = 0; i < R.n; ++i) {
+        uint32_t b = R.rp[i], e = R.rp[i + 1];
+        if (diag_first) {
+            s.diag[i] = R.v[b];
+            ++b;
+        }
+        for (uint32_t k = b; k < e; ++k) {
+            s.ci.push_back(R.ci[k]);
+            s.val.push_back(R.v[k]);
+        }
+        s.rp[i + 1] = (uint32_t)s.ci.size();
+    }
+    if (diag_vec) s.diag = *diag_vec;
+    return s;
+}
+
+// Column j of a CSR with the diagonal first becomes row j of the transpose
+// with the diagonal last; returns the gather-form lower sweep.
+static HostSweep lower_from_upper_rows(const HostCsr &Ut, int mode, bool diag_first) {
+    HostCsr L = transpose(Ut);  // rows ascending in column
+    HostSweep s;
+    s.n = L.n;
+    s.lower = true;
+    s.mode = mode;
+    s.rp.assign(L.n + 1, 0);
+    s.diag.assign(L.n, 1.0);
+    for (uint32_t i = 0; i < L.n; ++i) {
+        for (uint32_t k = L.rp[i]; k < L.rp[i + 1]; ++k) {
+            if (L.ci[k] == i) {
+                if (diag_first) s.diag[i] = L.v[k];
+                continue;
+            }
+            s.ci.push_back(L.ci[k]);
+            s.val.push_back(L.v[k]);
+        }
+        s.rp[i + 1] = (uint32_t)s.ci.size();
+    }
+    return s;
+}
+
+static void setup_triangular(pp_precond *M, const HostCsr &A, bool upload) {
+    const int kind = M->opts.kind;
+    int st = PP_OK;
+    uint32_t fail = 0;
+    if (kind == PP_PC_ICT || kind == PP_PC_IC0 || kind == PP_PC_ILDLT) {
+        HostCsr lt;
+        std::vector<double> d;
+        if (kind == PP_PC_ICT) st = factor_ict(A, M->opts.tau, false, lt, d, fail);
+        else if (kind == PP_PC_ILDLT) st = factor_ict(A, M->opts.tau, true, lt, d, fail);
+        else st = factor_ic0(A, lt, fail);
+        M->lt_rp = lt.rp;
+        M->lt_ci = lt.ci;
+        M->lt_v = lt.v;
+        M->d = d;
+        M->mnnz = (uint32_t)(lt.ci.size() * 2 + A.n);  // IncompleteCholesky.cpp:245 convention
+        if (st != PP_OK) {
+            M->fail_row = fail;
+            throw Error(st, kind == PP_PC_ILDLT ? "ILDLT: zero pivot at row " + std::to_string(fail)
+                                                : "IC: matrix is not positive definite (pivot row " +
+                                                      std::to_string(fail) + ")");
+        }
+        if (kind == PP_PC_ILDLT) {
+            M->hlow = lower_from_upper_rows(lt, HostSweep::UNIT, false);
+            M->hup = sweep_from_rows(lt, false, HostSweep::PREDIVIDE, false, &M->d);
+        } else {
+            M->hlow = lower_from_upper_rows(lt, HostSweep::DIVIDE, true);
+            M->hup = sweep_from_rows(lt, false, HostSweep::DIVIDE, true, nullptr);
+        }
+    } else if (kind == PP_PC_ILUT || kind == PP_PC_ILU0) {
+        HostCsr L, U;
+        std::vector<double> D;
+        if (kind == PP_PC_ILUT) st = factor_ilut(A, M->opts.lfil, M->opts.tau, L, U, D, fail);
+        else st = factor_ilu0(A, L, U, D, fail);
+        M->l_rp = L.rp; M->l_ci = L.ci; M->l_v = L.v;
+        M->u_rp = U.rp; M->u_ci = U.ci; M->u_v = U.v;
+        M->d = D;
+        M->mnnz = (uint32_t)(L.ci.size() + U.ci.size() + A.n);
+        if (st != PP_OK) {
+            M->fail_row = fail;
+            throw Error(st, std::string(st == PP_ERR_ZERO_ROW ? "ILU: zero row " : "ILU: zero pivot at row ") +
+                                std::to_string(fail));
+        }
+        M->hlow = sweep_from_rows(L, true, HostSweep::UNIT, false, nullptr);
+        M->hup = sweep_from_rows(U, false, HostSweep::MULTIPLY, false, &M->d);
+    } else {
+        throw Error(PP_ERR_UNSUPPORTED, "unknown preconditioner kind");
+    }
+    if (!upload) return;
+    upload_sweep(M->hlow, M->low64);
+    upload_sweep(M->hup, M->up64);
+    if (M->opts.make_f32) {
+        upload_sweep(M->hlow, M->low32);
+        upload_sweep(M->hup, M->up32);
+        M->have32 = true;
+    }
+}
+
+void precond_need_f32(pp_precond *M) {
+    if (M->have32 || M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) return;
+    upload_sweep(M->hlow, M->low32);
+    upload_sweep(M->hup, M->up32);
+    M->have32 = true;
+}
+
+// y = M^{-1} x on the context stream (y may alias x).
+void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
+    PP_CHECK(M->ready, PP_ERR_NOT_READY, "preconditioner used before setup()");
+    const size_t esz = dtype == PP_F64 ? 8 : 4;
+    if (M->opts.kind == PP_PC_NONE) {
+        if (x != y) PP_HIP(hipMemcpyAsync(y, x, (size_t)M->n * esz, hipMemcpyDeviceToDevice, ctx->stream));
+        return;
+    }
+    if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) {
+        ml_apply(ctx, M, dtype, x, y);
+        return;
+    }
+    if (x != y) PP_HIP(hipMemcpyAsync(y, x, (size_t)M->n * esz, hipMemcpyDeviceToDevice, ctx->stream));
+    if (dtype == PP_F64) {
+        launch_sweep<double>(ctx, M->low64, M->hlow.mode, (const double *)y, (double *)y);
+        launch_sweep<double>(ctx, M->up64, M->hup.mode, (const double *)y, (double *)y);
+    } else {
+        precond_need_f32(M);
+        launch_sweep<float>(ctx, M->low32, M->hlow.mode, (const float *)y, (float *)y);
+        launch_sweep<float>(ctx, M->up32, M->hup.mode, (const float *)y, (float *)y);
+    }
+}
+
+}  // namespace pp
+
+using namespace pp;
+
+void set_host_error(const std::string &m);
+
+template <class F>
+static int guarded_pc(pp_ctx *ctx, F &&f) {
+    try {
+        PP_HIP(hipSetDevice(ctx->device));
+        f();
+        return PP_OK;
+    } catch (const pp::Error &e) {
+        ctx->err = e.what();
+        return e.status;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return PP_ERR_INVALID;
+    }
+}
+
+extern "C" {
+
+void pp_precond_opts_default(pp_precond_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->kind = PP_PC_ICT;
+    o->tau = 1e-3;
+    o->lfil = 20;
+    o->trsv = PP_TRSV_AUTO;
+    o->levels = 3;
+    o->ndom = 16;
+    o->rank = 32;
+    o->eps = 0.8;
+}
+
+int pp_precond_create(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *opts, pp_precond **out) {
+    if (!ctx) return PP_ERR_INVALID;
+    return guarded_pc(ctx, [&] {
+        PP_CHECK(A && opts && out, PP_ERR_INVALID, "NULL argument");
+        PP_CHECK(A->nrow == A->ncol, PP_ERR_INVALID, "preconditioner needs a square matrix");
+        auto *M = new pp_precond;
+        M->ctx = ctx;
+        M->A = A;
+        M->opts = *opts;
+        M->n = A->nrow;
+        *out = M;
+    });
+}
+
+int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
+    if (!ctx) return PP_ERR_INVALID;
+    return guarded_pc(ctx, [&] {
+        PP_CHECK(M, PP_ERR_INVALID, "NULL preconditioner");
+        if (M->ready) return;  // setup() is idempotent (IncompleteCholesky.cpp:105)
+        auto t0 = std::chrono::steady_clock::now();
+        if (M->opts.kind == PP_PC_NONE) {
+            M->mnnz = 0;
+        } else if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) {
+            ml_setup(ctx, M);
+        } else {
+            HostCsr A;
+            A.n = M->A->nrow;
+            A.rp = M->A->h_rp;
+            A.ci = M->A->h_ci;
+            A.v = M->A->h_v;
+            setup_triangular(M, A, true);
+        }
+        M->ready = 1;
+        M->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
+    if (!ctx) return PP_ERR_INVALID;
+    return guarded_pc(ctx, [&] {
+        PP_CHECK(M && x && y, PP_ERR_INVALID, "NULL argument");
+        launch_fill_zero(ctx, ctx->ctl, sizeof(int));  // clear the stop word (graph-safe)
+        precond_apply(ctx, M, dtype, x, y);
+    });
+}
+
+int pp_precond_apply_part(pp_ctx *ctx, pp_precond *M, int part, int dtype, void *x) {
+    if (!ctx) return PP_ERR_INVALID;
+    return guarded_pc(ctx, [&] {
+        PP_CHECK(M && x, PP_ERR_INVALID, "NULL argument");
+        PP_CHECK(M->ready, PP_ERR_NOT_READY, "preconditioner used before setup()");
+        PP_CHECK(M->opts.kind != PP_PC_AMSED && M->opts.kind != PP_PC_GMSLR && M->opts.kind != PP_PC_NONE,
+                 PP_ERR_UNSUPPORTED, "not a triangular preconditioner");
+        launch_fill_zero(ctx, ctx->ctl, sizeof(int));
+        const bool ildlt = M->opts.kind == PP_PC_ILDLT;
+        if (dtype == PP_F32) precond_need_f32(M);
+        if (part == 0) {
+            if (dtype == PP_F64) launch_sweep<double>(ctx, M->low64, M->hlow.mode, (double *)x, (double *)x);
+            else launch_sweep<float>(ctx, M->low32, M->hlow.mode, (float *)x, (float *)x);
+        } else if (part == 1) {
+            // D^{-1} alone: only ILDLT has a diagonal stage (identity otherwise)
+            if (ildlt) {
+                HostSweep diag;
+                diag.n = M->n;
+                diag.lower = true;
+                diag.mode = HostSweep::PREDIVIDE;
+                diag.rp.assign(M->n + 1, 0);
+                diag.diag = M->d;
+                if (dtype == PP_F64) {
+                    DevSweep<double> ds;
+                    upload_sweep(diag, ds);
+                    launch_sweep<double>(ctx, ds, HostSweep::PREDIVIDE, (double *)x, (double *)x);
+                    PP_HIP(hipStreamSynchronize(ctx->stream));
+                    ds.free_all();
+                } else {
+                    DevSweep<float> ds;
+                    upload_sweep(diag, ds);
+                    launch_sweep<float>(ctx, ds, HostSweep::PREDIVIDE, (float *)x, (float *)x);
+                    PP_HIP(hipStreamSynchronize(ctx->stream));
+                    ds.free_all();
+                }
+            }
+        } else if (part == 2) {
+            // the ILDLT upper sweep folds D^{-1} in; alone it must not
+            int mode = ildlt ? HostSweep::UNIT : M->hup.mode;
+            if (dtype == PP_F64) launch_sweep<double>(ctx, M->up64, mode, (double *)x, (double *)x);
+            else launch_sweep<float>(ctx, M->up32, mode, (float *)x, (float *)x);
+        } else {
+            throw Error(PP_ERR_INVALID, "part must be 0, 1 or 2");
+        }
+    });
+}
+
+int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {
+    if (!M || !info) return PP_ERR_INVALID;
+    std::memset(info, 0, sizeof(*info));
+    info->kind = M->opts.kind;
+    info->ready = M->ready;
+    info->n = M->n;
+    info->annz = M->A->nnz;
+    info->mnnz = M->mnnz;
+    info->fill_ratio = M->A->nnz ? (double)M->mnnz / (double)M->A->nnz : 0.0;
+    info->levels_lower = M->low64.nlev;
+    info->levels_upper = M->up64.nlev;
+    info->setup_ms = M->setup_ms;
+    info->fail_row = M->fail_row;
+    return PP_OK;
+}
+
+int pp_precond_factor(const pp_precond *M, int which, uint32_t *nnz, uint32_t *rowptr, uint32_t *colidx,
+                      double *values, double *d) {
+    if (!M || !nnz) return PP_ERR_INVALID;
+    const std::vector<uint32_t> *rp, *ci;
+    const std::vector<double> *v;
+    if (which == 0) { rp = &M->lt_rp; ci = &M->lt_ci; v = &M->lt_v; }
+    else if (which == 1) { rp = &M->l_rp; ci = &M->l_ci; v = &M->l_v; }
+    else if (which == 2) { rp = &M->u_rp; ci = &M->u_ci; v = &M->u_v; }
+    else return PP_ERR_INVALID;
+    if (rp->empty()) return PP_ERR_NOT_READY;
+    *nnz = (uint32_t)ci->size();
+    if (rowptr) std::memcpy(rowptr, rp->data(), rp->size() * sizeof(uint32_t));
+    if (colidx && !ci->empty()) std::memcpy(colidx, ci->data(), ci->size() * sizeof(uint32_t));
+    if (values && !v->empty()) std::memcpy(values, v->data(), v->size() * sizeof(double));
+    if (d && !M->d.empty()) std::memcpy(d, M->d.data(), M->d.size() * sizeof(double));
+    return PP_OK;
+}
+
+void pp_precond_destroy(pp_precond *M) {
+    if (!M) return;
+    (void)hipSetDevice(M->ctx->device);
+    (void)hipStreamSynchronize(M->ctx->stream);
+    M->low64.free_all();
+    M->up64.free_all();
+    M->low32.free_all();
+    M->up32.free_all();
+    if (M->multilevel) ml_free(M);
+    delete M;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------- host-only factorization
+struct pp_hfactor {
+    pp_precond M;
+    int status = PP_OK;
+};
+
+extern "C" {
+
+int pp_factor_host(const pp_precond_opts *opts, uint32_t n, const uint32_t *rp, const uint32_t *ci, const double *v,
+                   pp_hfactor **out) {
+    if (!opts || !rp || !out) return PP_ERR_INVALID;
+    auto *F = new pp_hfactor;
+    *out = F;
+    F->M.opts = *opts;
+    F->M.n = n;
+    try {
+        HostCsr A;
+        A.n = n;
+        A.rp.assign(rp, rp + n + 1);
+        A.ci.assign(ci, ci + rp[n]);
+        A.v.assign(v, v + rp[n]);
+        PP_CHECK(opts->kind >= PP_PC_ICT && opts->kind <= PP_PC_ILU0, PP_ERR_UNSUPPORTED,
+                 "host factorization covers ICT, ILUT, ILDLT, IC0, ILU0");
+        setup_triangular(&F->M, A, false);
+        F->M.ready = 1;
+    } catch (const Error &e) {
+        F->status = e.status;
+        set_host_error(e.what());
+    }
+    return F->status;
+}
+
+int pp_hfactor_get(const pp_hfactor *F, int which, uint32_t *nnz, uint32_t *rp, uint32_t *ci, double *v, double *d) {
+    if (!F) return PP_ERR_INVALID;
+    return pp_precond_factor(&F->M, which, nnz, rp, ci, v, d);
+}
+
+int pp_hfactor_info(const pp_hfactor *F, uint32_t *mnnz, uint32_t *fail_row, uint32_t *lev_lo, uint32_t *lev_up) {
+    if (!F) return PP_ERR_INVALID;
+    if (mnnz) *mnnz = F->M.mnnz;
+    if (fail_row) *fail_row = F->M.fail_row;
+    std::vector<uint32_t> perm, lp;
+    if (lev_lo) {
+        *lev_lo = 0;
+        if (F->M.hlow.n) {
+            levels_of(F->M.hlow, perm, lp);
+            *lev_lo = (uint32_t)lp.size() - 1;
+        }
+    }
+    if (lev_up) {
+        *lev_up = 0;
+        if (F->M.hup.n) {
+            levels_of(F->M.hup, perm, lp);
+            *lev_up = (uint32_t)lp.size() - 1;
+        }
+    }
+    return PP_OK;
+}
+
+void pp_hfactor_destroy(pp_hfactor *F) { delete F; }
+
+}  // extern "C"

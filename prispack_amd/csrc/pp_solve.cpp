@@ -1,0 +1,514 @@
+// pp_solve.cpp -- Krylov drivers of the C ABI: device-resident PCG (FP64, and
+// mixed precision = FP32 inner PCG inside FP64 iterative refinement) and
+// FGMRES(m) with CGS2 orthogonalisation.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "pp_internal.h"
+
+namespace pp {
+void csr_need_f32(pp_ctx *ctx, pp_csr *A);
+pp_csr *csr_transpose(pp_ctx *ctx, pp_csr *A);
+void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
+void precond_need_f32(pp_precond *M);
+void launch_scale_convert(pp_ctx *ctx, uint32_t n, const double *src, double scale, float *dst);
+void launch_axpy_f32_to_f64(pp_ctx *ctx, uint32_t n, double a, const float *x, double *y);
+void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w, int slot0);
+void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h,
+                     double *w);
+void launch_gemv_add(pp_ctx *ctx, uint32_t n, uint32_t ncol, const float *Z32, const double *Z64, size_t ld,
+                     const double *y, double *x);
+void read_slots(pp_ctx *ctx, int first, int count, double *out);
+double *gmres_coeff_buffer(pp_ctx *ctx);
+
+namespace {
+
+// Workspace indices in the context cache.
+enum { W_R = 0, W_W, W_D, W_V, W_R32, W_W32, W_D32, W_V32, W_X32, W_B32, W_E32, W_VB, W_ZB, W_ZB32, W_T, W_T32 };
+
+template <class T>
+constexpr int dt() {
+    return sizeof(T) == 8 ? PP_F64 : PP_F32;
+}
+
+struct PcgResult {
+    int32_t iters = 0;
+    int conv = 0;
+    int status = PP_OK;
+    double final_error = -1;
+    int nhist = 0;
+};
+
+// FlexibleCG on the device (source/Solver/CG.cpp:33-83), working precision T.
+// A, M already hold T copies.  x holds x0.  One v = A d per iteration: the
+// reference's repeated product at CG.cpp:68 returns the same vector.
+template <class T>
+PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const T *b, T *x, double tol,
+                     uint32_t maxit, uint32_t check_every, bool hist_on, int wbase) {
+    const uint32_t n = A->nrow;
+    const int DT = dt<T>();
+    T *r = (T *)ws_get(ctx, wbase + 0, (size_t)n * sizeof(T));
+    T *w = (T *)ws_get(ctx, wbase + 1, (size_t)n * sizeof(T));
+    T *d = (T *)ws_get(ctx, wbase + 2, (size_t)n * sizeof(T));
+    T *v = (T *)ws_get(ctx, wbase + 3, (size_t)n * sizeof(T));
+    if (hist_on) ensure_hist(ctx, maxit + 2);
+    PcgResult res;
+    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0);
+    // r = b - A x
+    PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    launch_spmv(ctx, A, DT, -1.0, x, 1.0, r, -1, nullptr, false);
+    launch_dot(ctx, DT, n, b, b, S_BB);
+    // w = M^{-1} r ; d = w ; v = A d ; x1, r1
+    if (M) precond_apply(ctx, M, DT, r, w);
+    else PP_HIP(hipMemcpyAsync(w, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    pcg_d_update<T>(ctx, n, true, w, d, r);
+    launch_spmv(ctx, A, DT, 1.0, d, 0.0, v, S_DV, d, false);
+    pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on);
+
+    int ctl[C_COUNT];
+    uint32_t launched = 1;  // bodies enqueued (the check index advances one per body)
+    const uint32_t every = std::max<uint32_t>(1, check_every);
+    for (;;) {
+        read_ctl(ctx, ctl);
+        if (ctl[C_STOP]) break;
+        if (launched > maxit + 1) break;  // cannot happen: the device sets STOP by maxit
+        for (uint32_t k = 0; k < every; ++k) {
+            if (flexible && M) precond_apply(ctx, M, DT, r, w);
+            else PP_HIP(hipMemcpyAsync(w, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+            pcg_wv<T>(ctx, n, w, v);
+            pcg_d_update<T>(ctx, n, false, w, d, r);
+            launch_spmv(ctx, A, DT, 1.0, d, 0.0, v, S_DV, d, false);
+            pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on);
+            ++launched;
+        }
+    }
+    res.conv = ctl[C_CONV];
+    res.status = ctl[C_STATUS];
+    const int m = ctl[C_M];
+    res.iters = m + 1;  // CG.cpp:81
+    res.final_error = read_slot(ctx, S_TMP0);
+    if (ctl[C_CHECK] <= 1 && !res.conv && maxit <= 1) res.final_error = -1;
+    res.nhist = std::max(0, std::min<int>(m, (int)maxit) - (res.conv ? 0 : 0));
+    if (hist_on) {
+        int cnt = res.conv ? m : std::max(0, ctl[C_CHECK] - 1);
+        if (!res.conv && ctl[C_LAST]) cnt = (int)maxit - 1;
+        cnt = std::max(0, std::min<int>(cnt, (int)ctx->hist_cap));
+        ctx->last_hist.assign(cnt, 0.0);
+        if (cnt) PP_HIP(hipMemcpy(ctx->last_hist.data(), ctx->hist, cnt * sizeof(double), hipMemcpyDeviceToHost));
+        res.nhist = cnt;
+    }
+    return res;
+}
+
+double true_relres(pp_ctx *ctx, pp_csr *A, const double *b, const double *x) {
+    const uint32_t n = A->nrow;
+    double *t = (double *)ws_get(ctx, W_T, (size_t)n * sizeof(double));
+    PP_HIP(hipMemcpyAsync(t, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    launch_spmv(ctx, A, PP_F64, -1.0, x, 1.0, t, -1, nullptr, false);
+    launch_dot(ctx, PP_F64, n, t, t, S_TMP2);
+    launch_dot(ctx, PP_F64, n, b, b, S_TMP3);
+    double out[2];
+    read_slots(ctx, S_TMP2, 2, out);
+    return std::sqrt(out[0]) / std::sqrt(out[1]);
+}
+
+// Mixed precision: FP64 residual r = b - A x and update x += ||r|| e on the
+// device, inner FP32 FlexibleCG on A e = r / ||r|| (FP32 factor, SpMV, sweeps
+// and accumulation) to a relative reduction max(inner_tol, 0.5 tol ||b||/||r||).
+PcgResult pcg_mixed(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts &o, const double *b, double *x,
+                    int32_t &outer, int32_t &inner_total, double &relres) {
+    const uint32_t n = A->nrow;
+    csr_need_f32(ctx, A);
+    if (M) precond_need_f32(M);
+    double *r = (double *)ws_get(ctx, W_T, (size_t)n * sizeof(double));
+    float *r32 = (float *)ws_get(ctx, W_B32, (size_t)n * sizeof(float));
+    float *e32 = (float *)ws_get(ctx, W_E32, (size_t)n * sizeof(float));
+    PcgResult res;
+    launch_dot(ctx, PP_F64, n, b, b, S_TMP3);
+    const double bnorm = std::sqrt(read_slot(ctx, S_TMP3));
+    outer = 0;
+    inner_total = 0;
+    const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-6;
+    const uint32_t max_outer = o.max_outer ? o.max_outer : 20;
+    relres = 1.0;
+    for (;;) {
+        PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        launch_spmv(ctx, A, PP_F64, -1.0, x, 1.0, r, -1, nullptr, false);
+        launch_dot(ctx, PP_F64, n, r, r, S_TMP2);
+        const double rnorm = std::sqrt(read_slot(ctx, S_TMP2));
+        relres = bnorm > 0 ? rnorm / bnorm : rnorm;
+        if (!(relres == relres)) {
+            res.status = PP_ERR_NOT_FINITE;
+            break;
+        }
+        if (relres < o.tol) {
+            res.conv = 1;
+            break;
+        }
+        if ((uint32_t)outer >= max_outer || (uint32_t)inner_total >= o.maxit) break;
+        launch_scale_convert(ctx, n, r, 1.0 / rnorm, r32);
+        launch_fill_zero(ctx, e32, (size_t)n * sizeof(float));
+        const double want = std::max(inner_tol, 0.5 * o.tol * bnorm / rnorm);
+        PcgResult in = pcg_device<float>(ctx, A, M, o.flexible != 0, r32, e32, want,
+                                         std::max<uint32_t>(2, o.maxit - inner_total), o.check_every, false, W_R32);
+        inner_total += in.iters;
+        ++outer;
+        if (in.status != PP_OK && in.status != PP_ERR_NOT_FINITE) {
+            res.status = in.status;
+            break;
+        }
+        launch_axpy_f32_to_f64(ctx, n, rnorm, e32, x);
+    }
+    res.iters = inner_total;
+    res.final_error = relres;
+    return res;
+}
+
+// ------------------------------------------------------------------ GMRES
+struct Givens {
+    std::vector<double> H;  // (m+1) x m column-major
+    std::vector<double> c, s, g;
+    uint32_t ld = 0;
+    double &h(uint32_t i, uint32_t j) { return H[(size_t)j * ld + i]; }
+};
+
+// GMRES::givensTransform (source/Solver/GMRES.cpp:72-101).
+void givens_step(Givens &G, uint32_t j) {
+    for (uint32_t i = 0; i < j; ++i) {
+        const double h1 = G.h(i, j), h2 = G.h(i + 1, j);
+        G.h(i + 1, j) = G.c[i] * h2 - G.s[i] * h1;
+        G.h(i, j) = G.c[i] * h1 + G.s[i] * h2;
+    }
+    const double h1 = G.h(j, j), h2 = G.h(j + 1, j);
+    if (std::fabs(h1) > std::fabs(h2)) {
+        const double t = h2 / h1;
+        G.c[j] = 1.0 / std::sqrt(std::fabs(1.0 + t * t));
+        G.s[j] = G.c[j] * t;
+    } else {
+        const double t = h1 / h2;
+        G.s[j] = 1.0 / std::sqrt(std::fabs(1.0 + t * t));
+        G.c[j] = G.s[j] * t;
+    }
+    G.h(j, j) = G.c[j] * h1 + G.s[j] * h2;
+    G.h(j + 1, j) = 0.0;
+    G.g[j + 1] = -G.s[j] * G.g[j];
+    G.g[j] = G.c[j] * G.g[j];
+}
+
+// Back substitution R y = g over the leading dim x dim block (GMRES.cpp:103-119).
+void back_solve(Givens &G, uint32_t dim) {
+    for (int i = (int)dim - 1; i >= 0; --i) {
+        double t = G.g[i];
+        for (uint32_t j = i + 1; j < dim; ++j) t -= G.h(i, j) * G.g[j];
+        G.g[i] = t / G.h(i, i);
+    }
+    G.g[dim] = 0.0;
+}
+
+// FlexibleGMRES(m) (source/Solver/GMRES.cpp:122-221) on the device: Z holds
+// the preconditioned basis (FP32 when mixed), V the orthonormal basis (FP64),
+// classical Gram-Schmidt applied twice (two GEMV^T + GEMV passes) in FP64;
+// Givens rotations and the small least-squares problem on the host.
+PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts &o, const double *b, double *x) {
+    const uint32_t n = A->nrow;
+    const bool mixed = o.precision == PP_MIXED;
+    const bool flexible = o.flexible != 0;
+    uint32_t mm = o.restart ? o.restart : 30;
+    const int32_t maxcyc = (int32_t)((o.maxit ? o.maxit : 500) / mm);  // GMRES.cpp:63
+    if (mm >= n) mm = n - 1;
+    PP_CHECK(mm >= 1, PP_ERR_INVALID, "GMRES needs n >= 2");
+    PP_CHECK(mm <= 256, PP_ERR_UNSUPPORTED, "restart > 256");
+    const size_t ld = n;
+    double *V = (double *)ws_get(ctx, W_VB, ld * (mm + 1) * sizeof(double));
+    double *Z = nullptr;
+    float *Z32 = nullptr;
+    if (mixed) {
+        csr_need_f32(ctx, A);
+        if (M) precond_need_f32(M);
+        Z32 = (float *)ws_get(ctx, W_ZB32, ld * (mm + 1) * sizeof(float));
+        launch_fill_zero(ctx, Z32, ld * (mm + 1) * sizeof(float));
+    } else if (flexible) {
+        Z = (double *)ws_get(ctx, W_ZB, ld * (mm + 1) * sizeof(double));
+        launch_fill_zero(ctx, Z, ld * (mm + 1) * sizeof(double));
+    }
+    launch_fill_zero(ctx, V, ld * (mm + 1) * sizeof(double));
+    double *w = (double *)ws_get(ctx, W_W, (size_t)n * sizeof(double));
+    double *r = (double *)ws_get(ctx, W_R, (size_t)n * sizeof(double));
+    float *t32 = mixed ? (float *)ws_get(ctx, W_T32, (size_t)n * sizeof(float)) : nullptr;
+    float *w32 = mixed ? (float *)ws_get(ctx, W_W32, (size_t)n * sizeof(float)) : nullptr;
+    double *hdev = gmres_coeff_buffer(ctx);
+
+    Givens G;
+    G.ld = mm + 1;
+    G.H.assign((size_t)(mm + 1) * mm, 0.0);
+    G.c.assign(mm + 1, 0.0);
+    G.s.assign(mm + 1, 0.0);
+    G.g.assign(mm + 1, 0.0);
+    std::vector<double> hcol(mm + 1), hcol2(mm + 1);
+    ctx->last_hist.clear();
+    ctl_init(ctx, 1, 0);  // the sweeps honour the stop word: clear what a previous solve left
+
+    launch_dot(ctx, PP_F64, n, b, b, S_TMP3);
+    const double bnorm = std::sqrt(read_slot(ctx, S_TMP3));
+    // r = b - A x ; beta = ||r||
+    PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    launch_spmv(ctx, A, PP_F64, -1.0, x, 1.0, r, -1, nullptr, false);
+    launch_dot(ctx, PP_F64, n, r, r, S_TMP2);
+    double beta = std::sqrt(read_slot(ctx, S_TMP2));
+    double rnorm = 0.0, relerr = 0.0;
+    int conv = 0;
+    int32_t k = -1;
+    uint32_t m = mm;
+    PcgResult res;
+    if (!(beta == beta)) {
+        res.status = PP_ERR_NOT_FINITE;
+        return res;
+    }
+    if (!o.reference_quirks && beta / (bnorm > 0 ? bnorm : 1.0) < o.tol) {
+        res.conv = 1;
+        res.iters = 0;
+        res.final_error = beta / (bnorm > 0 ? bnorm : 1.0);
+        return res;
+    }
+    while (!conv && k < maxcyc) {
+        ++k;
+        launch_axpby(ctx, PP_F64, n, 1.0 / beta, r, 0.0, r, V);  // v0 = r / beta
+        std::fill(G.g.begin(), G.g.begin() + mm, 0.0);
+        G.g[0] = beta;
+        bool converged_cycle = false;
+        for (uint32_t j = 0; j < m; ++j) {
+            double *vj = V + ld * j;
+            // z_j = M^{-1} v_j ; w = A z_j
+            if (mixed) {
+                float *zj = Z32 + ld * j;
+                launch_convert(ctx, n, PP_F64, vj, PP_F32, zj);
+                if (flexible && M) precond_apply(ctx, M, PP_F32, zj, zj);
+                launch_spmv(ctx, A, PP_F32, 1.0, zj, 0.0, w32, -1, nullptr, false);
+                launch_convert(ctx, n, PP_F32, w32, PP_F64, w);
+            } else if (flexible) {
+                double *zj = Z + ld * j;
+                if (M) precond_apply(ctx, M, PP_F64, vj, zj);
+                else PP_HIP(hipMemcpyAsync(zj, vj, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+                launch_spmv(ctx, A, PP_F64, 1.0, zj, 0.0, w, -1, nullptr, false);
+            } else {
+                launch_spmv(ctx, A, PP_F64, 1.0, vj, 0.0, w, -1, nullptr, false);
+            }
+            // CGS2: h = V^T w ; w -= V h ; twice
+            for (int pass = 0; pass < 2; ++pass) {
+                launch_gemv_t(ctx, n, j + 1, V, ld, w, 0);
+                PP_HIP(hipMemcpyAsync(hdev + 320, hdev, (j + 1) * sizeof(double), hipMemcpyDeviceToDevice,
+                                      ctx->stream));
+                launch_gemv_sub(ctx, n, j + 1, V, ld, hdev + 320, w);
+                read_slots(ctx, -1, j + 1, pass == 0 ? hcol.data() : hcol2.data());
+            }
+            launch_dot(ctx, PP_F64, n, w, w, S_TMP2);
+            const double hn = std::sqrt(read_slot(ctx, S_TMP2));
+            for (uint32_t i = 0; i <= j; ++i) G.h(i, j) = hcol[i] + hcol2[i];
+            G.h(j + 1, j) = hn;
+            if (!(hn == hn)) {
+                res.status = PP_ERR_NOT_FINITE;
+                m = j;
+                break;
+            }
+            if (std::fabs(hn) == 0) {  // happy breakdown (GMRES.cpp:183-186)
+                m = o.reference_quirks ? j : j + 1;
+                if (!o.reference_quirks) {
+                    givens_step(G, j);
+                    rnorm = std::fabs(G.g[j + 1]);
+                    relerr = rnorm / bnorm;
+                    converged_cycle = true;
+                }
+                break;
+            }
+            launch_axpby(ctx, PP_F64, n, 1.0 / hn, w, 0.0, w, V + ld * (j + 1));
+            givens_step(G, j);
+            rnorm = std::fabs(G.g[j + 1]);
+            relerr = rnorm / bnorm;
+            ctx->last_hist.push_back(relerr);
+            if (relerr < o.tol) {
+                m = o.reference_quirks ? j : j + 1;  // the reference drops column j
+                converged_cycle = true;
+                break;
+            }
+        }
+        (void)converged_cycle;
+        if (res.status != PP_OK) break;
+        back_solve(G, m);
+        // x += Z g over all mm+1 columns (GMRES.cpp:204-208); the entry past
+        // the back-solve is the residual estimate, which the reference leaves
+        // in place in its converged cycle
+        double *y = ctx->pinned + 320;  // pinned staging (read_slots uses [0, 320))
+        std::copy(G.g.begin(), G.g.end(), y);
+        if (!o.reference_quirks)
+            for (uint32_t i = m; i <= mm; ++i) y[i] = 0.0;
+        PP_HIP(hipMemcpyAsync(hdev + 320, y, (mm + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        launch_gemv_add(ctx, n, mm + 1, Z32, mixed ? nullptr : (flexible ? Z : V), ld, hdev + 320, x);
+        conv = relerr < o.tol;
+        PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        launch_spmv(ctx, A, PP_F64, -1.0, x, 1.0, r, -1, nullptr, false);
+        if (o.reference_quirks) {
+            beta = rnorm;  // GMRES.cpp:214 restarts from the estimate
+        } else {
+            launch_dot(ctx, PP_F64, n, r, r, S_TMP2);
+            beta = std::sqrt(read_slot(ctx, S_TMP2));
+            relerr = beta / bnorm;
+            conv = relerr < o.tol;
+        }
+        if (!conv && k < maxcyc) {
+            if (!o.reference_quirks) m = mm;
+        }
+    }
+    res.conv = conv;
+    res.iters = k * (int32_t)mm + (int32_t)m;  // GMRES.cpp:218
+    if (!o.reference_quirks && conv && m > 0) res.iters = k * (int32_t)mm + (int32_t)m - 1;
+    res.final_error = relerr;
+    res.nhist = (int)ctx->last_hist.size();
+    return res;
+}
+
+}  // namespace
+}  // namespace pp
+
+using namespace pp;
+
+template <class F>
+static int guarded_solve(pp_ctx *ctx, F &&f) {
+    try {
+        PP_HIP(hipSetDevice(ctx->device));
+        f();
+        return PP_OK;
+    } catch (const pp::Error &e) {
+        ctx->err = e.what();
+        return e.status;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return PP_ERR_INVALID;
+    }
+}
+
+static int solve_impl(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *opts, const double *b, double *x,
+                      pp_solve_report *rep) {
+    return guarded_solve(ctx, [&] {
+        PP_CHECK(A && opts && b && x, PP_ERR_INVALID, "NULL argument");
+        PP_CHECK(A->nrow == A->ncol, PP_ERR_INVALID, "square matrix required");
+        if (M) PP_CHECK(M->ready, PP_ERR_NOT_READY, "preconditioner used before setup()");
+        pp_solve_report R;
+        std::memset(&R, 0, sizeof(R));
+        const pp_solve_opts &o = *opts;
+        PP_CHECK(o.tol > 0, PP_ERR_INVALID, "tol must be positive");
+        PP_CHECK(o.maxit >= 1, PP_ERR_INVALID, "maxit must be >= 1");
+        PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        PcgResult res;
+        if (o.solver == PP_SOLVER_CG) {
+            if (o.precision == PP_MIXED) {
+                double rel;
+                res = pcg_mixed(ctx, A, M, o, b, x, R.outer_iters, R.inner_iters, rel);
+            } else {
+                res = pcg_device<double>(ctx, A, M, o.flexible != 0, b, x, o.tol, o.maxit,
+                                         o.check_every ? o.check_every : 1, o.record_history != 0, W_R);
+            }
+        } else if (o.solver == PP_SOLVER_GMRES) {
+            res = gmres_device(ctx, A, M, o, b, x);
+        } else {
+            throw Error(PP_ERR_INVALID, "unknown solver");
+        }
+        PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        PP_HIP(hipEventSynchronize(ctx->ev1));
+        float ms = 0;
+        PP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        R.iters = res.iters;
+        R.converged = res.conv;
+        R.final_error = res.final_error;
+        R.status = res.status;
+        R.nhist = res.nhist;
+        R.solve_ms = ms;
+        R.true_relres = true_relres(ctx, A, b, x);
+        if (rep) *rep = R;
+        if (res.status != PP_OK) {
+            throw Error(res.status, res.status == PP_ERR_BREAKDOWN ? "Krylov breakdown ((d, A d) = 0)"
+                                                                   : "non-finite value in the iteration");
+        }
+    });
+}
+
+extern "C" {
+
+void pp_solve_opts_default(pp_solve_opts *o, int solver) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->solver = solver;
+    o->flexible = 1;
+    o->precision = PP_FP64;
+    o->tol = 1e-8;
+    o->maxit = solver == PP_SOLVER_GMRES ? 500 : 1000;
+    o->restart = 30;
+    o->reference_quirks = 0;
+    o->check_every = 1;
+    o->inner_tol = 1e-6;
+    o->max_outer = 20;
+    o->record_history = 1;
+}
+
+int pp_solve(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *opts, const double *b, double *x,
+             pp_solve_report *rep) {
+    if (!ctx) return PP_ERR_INVALID;
+    return solve_impl(ctx, A, M, opts, b, x, rep);
+}
+
+int pp_solve_host(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *opts, const double *b_host,
+                  double *x_host, pp_solve_report *rep) {
+    if (!ctx) return PP_ERR_INVALID;
+    double *db = nullptr, *dx = nullptr;
+    int st = guarded_solve(ctx, [&] {
+        PP_CHECK(A && b_host && x_host, PP_ERR_INVALID, "NULL argument");
+        const uint32_t n = A->nrow;
+        db = dalloc<double>(n);
+        dx = dalloc<double>(n);
+        PP_HIP(hipMemcpyAsync(db, b_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(dx, x_host, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    });
+    if (st == PP_OK) st = solve_impl(ctx, A, M, opts, db, dx, rep);
+    if (st == PP_OK || st == PP_ERR_BREAKDOWN || st == PP_ERR_NOT_FINITE) {
+        int st2 = guarded_solve(ctx, [&] {
+            PP_HIP(hipMemcpyAsync(x_host, dx, A->nrow * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            PP_HIP(hipStreamSynchronize(ctx->stream));
+        });
+        if (st == PP_OK) st = st2;
+    }
+    dfree(db);
+    dfree(dx);
+    return st;
+}
+
+int pp_solve_history(pp_ctx *ctx, double *out, uint32_t cap, uint32_t *count) {
+    if (!ctx || !count) return PP_ERR_INVALID;
+    uint32_t c = (uint32_t)ctx->last_hist.size();
+    *count = c;
+    if (out) std::memcpy(out, ctx->last_hist.data(), std::min(c, cap) * sizeof(double));
+    return PP_OK;
+}
+
+// Adjoint: A^T lambda = g with a preconditioner built on A^T.
+int pp_adjoint_solve(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *popts, const pp_solve_opts *opts,
+                     const double *g, double *lambda, pp_solve_report *rep) {
+    if (!ctx) return PP_ERR_INVALID;
+    pp_csr *T = nullptr;
+    int st = guarded_solve(ctx, [&] { T = csr_transpose(ctx, A); });
+    if (st != PP_OK) return st;
+    pp_precond *M = nullptr;
+    if (popts && popts->kind != PP_PC_NONE) {
+        st = pp_precond_create(ctx, T, popts, &M);
+        if (st == PP_OK) st = pp_precond_setup(ctx, M);
+        if (st != PP_OK) {
+            pp_precond_destroy(M);
+            return st;
+        }
+    }
+    st = solve_impl(ctx, T, M, opts, g, lambda, rep);
+    pp_precond_destroy(M);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+"""The C ABI without a GPU: the library loads, exports every entry point that
+include/prispack_amd.h declares, and its host-side parts (model generators,
+seeded inputs, the setup() factorizations) agree bit for bit with the oracle.
+CPU only."""
+import numpy as np
+import pytest
+
+from oracle import ppo
+
+
+@pytest.fixture(scope="module")
+def pp():
+    import prispack_amd
+    return prispack_amd
+
+
+def test_library_exports_every_declared_symbol(pp):
+    from prispack_amd import _capi
+    syms = _capi.header_symbols()
+    assert len(syms) > 40
+    L = _capi.lib()
+    assert [s for s in syms if not hasattr(L, s)] == []
+    # every declared symbol has a ctypes signature in the binding
+    assert set(syms) <= set(_capi.SIGS)
+
+
+def eq(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("args", [(8, 8, 1, 0, 0, 0, 0), (6, 5, 4, 0.2, 0.1, 0.05, 0.3), (1, 9, 1, 0, 0, 0, 0),
+                                  (128, 128, 1, 0, 0, 0, 0)])
+def test_laplacian_generator(pp, args):
+    assert eq(pp.generateLaplacianCSR(*args), ppo.laplacian(*args))
+
+
+def test_other_generators_and_inputs(pp):
+    assert eq(pp.generate_upwind(9, 8, 7, 20.0, 10.0, 5.0), ppo.upwind(9, 8, 7, 20.0, 10.0, 5.0))
+    k = np.exp(ppo.normal(49, 1))
+    assert eq(pp.generate_vardiff2d(7, k), ppo.vardiff2d(7, k))
+    assert np.array_equal(pp.uniform(5000, 42), ppo.uniform(5000, 42))
+    # Box-Muller goes through libm cos/sin/log, which hipcc and gcc may
+    # evaluate with a one-ulp difference; inputs are generated once and handed
+    # to both sides, so one ulp is the bar here.
+    a, b = pp.normal(5001, 42), ppo.normal(5001, 42)
+    assert np.all(np.abs(a - b) <= 2.3e-16 * np.maximum(1.0, np.abs(b)))
+
+
+@pytest.mark.parametrize("dims", [(16, 16, 1), (12, 12, 12), (6, 5, 4)])
+@pytest.mark.parametrize("kind,tau", [("ict", 0.0), ("ict", 1e-3), ("ict", 1e-2), ("ildlt", 1e-3), ("ic0", None)])
+def test_symmetric_factorizations(pp, dims, kind, tau):
+    A = ppo.laplacian(*dims)
+    args = () if tau is None else (tau,)
+    f = pp.host_factor(A, kind, *args)
+    P = ppo.Prec(A, kind, *args)
+    g = P.factors()
+    assert f["status"] == 0
+    assert eq(f["lt"], g["lt"])
+    assert f["mnnz"] == P.mnnz
+    if kind == "ildlt":
+        assert np.array_equal(f["d"], g["d"])
+
+
+@pytest.mark.parametrize("A", [ppo.upwind(10, 9, 8, 20.0, 10.0, 5.0), ppo.laplacian(16, 16, 1, 0.3, 0.2, 0, 0)])
+@pytest.mark.parametrize("kind,tau,lfil", [("ilut", 1e-2, 20), ("ilut", 0.0, 3), ("ilut", 1e-3, 5), ("ilu0", 0, 0)])
+def test_ilu_factorizations(pp, A, kind, tau, lfil):
+    f = pp.host_factor(A, kind, tau, lfil) if kind == "ilut" else pp.host_factor(A, kind)
+    P = ppo.Prec(A, kind, tau, lfil) if kind == "ilut" else ppo.Prec(A, kind)
+    g = P.factors()
+    assert eq(f["l"], g["l"]) and eq(f["u"], g["u"])
+    assert np.array_equal(f["d"], g["d"]) and f["mnnz"] == P.mnnz
+
+
+def test_factorization_failures(pp):
+    from prispack_amd._capi import PP_ERR_NOT_SPD, PP_ERR_ZERO_ROW
+    A = ppo.laplacian(5, 5, 1, 0, 0, 0, 6.0)
+    f = pp.host_factor(A, "ict", 1e-3)
+    assert f["status"] == PP_ERR_NOT_SPD and f["fail_row"] == ppo.Prec(A, "ict", 1e-3).fail_row
+    assert pp.host_factor(A, "ic0")["status"] == PP_ERR_NOT_SPD
+    Z = (np.array([0, 1, 1, 2], np.uint32), np.array([0, 2], np.uint32), np.array([1.0, 1.0]))
+    assert pp.host_factor(Z, "ilut", 0.0, 5)["status"] == PP_ERR_ZERO_ROW
+
+
+def test_level_counts_of_natural_order_ic0(pp):
+    # hyperplanes i+j+k = const of a natural-order 7-point IC(0): 3(n-1)+1 levels
+    f = pp.host_factor(ppo.laplacian(9, 8, 7), "ic0")
+    assert f["levels_lower"] == f["levels_upper"] == 8 + 7 + 6 + 1

@@ -1,0 +1,321 @@
+"""HIP path through the C ABI against the oracle (oracle/ppo.c) and the golden
+vectors of the compiled reference.  Bars:
+  - SpMV / SpMV^T / the triangular sweeps in FP64: bit-identical (same
+    operation order as the reference's serial loops, no fused multiply-add);
+  - FP32 kernels: |err| <= 1e-5 relative to the FP64 result;
+  - solvers (FP64): iterations within +-2 of the reference, true relative
+    residual <= tol, ||x - x_ref|| / ||x_ref|| <= 1e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import ppo
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def pp(torch):
+    import prispack_amd
+    return prispack_amd
+
+
+def dev(torch, a, dtype=None):
+    return torch.tensor(np.asarray(a), dtype=dtype or torch.float64, device="cuda")
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def long_row_matrix():
+    # rows of very different lengths, including empty rows and one row longer
+    # than the LDS stage of the SpMV kernel
+    rng = np.random.default_rng(5)
+    n = 3000
+    rows = []
+    for i in range(n):
+        if i % 97 == 5:
+            cols = np.array([], dtype=np.int64)
+        elif i == 17:
+            cols = np.arange(0, n, 1)
+        else:
+            cols = np.unique(rng.integers(0, n, size=rng.integers(1, 40)))
+        rows.append(cols)
+    rp = np.zeros(n + 1, np.uint32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate(rows).astype(np.uint32)
+    v = rng.standard_normal(len(ci))
+    return rp, ci, v
+
+
+MATS = {
+    "lap3d_20": lambda: ppo.laplacian(20, 20, 20),
+    "lap2d_128": lambda: ppo.laplacian(128, 128, 1),
+    "upwind_17": lambda: ppo.upwind(17, 17, 17, 20.0, 10.0, 5.0),
+    "ragged": long_row_matrix,
+}
+
+
+@pytest.mark.parametrize("name", sorted(MATS))
+def test_spmv_fp64_bit_exact(pp, torch, name):
+    A = MATS[name]()
+    n = len(A[0]) - 1
+    print(f"[spmv] {name} n={n}", flush=True)
+    M = pp.CSRMatrix(*A)
+    x = ppo.uniform(n, 3)
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    M.MatPVec(dev(torch, x), y)
+    want = ppo.spmv(A, x)
+    if name == "ragged":  # the long row is summed by a tree: FP64 rounding only
+        long = np.zeros(n, bool)
+        long[17] = True
+        assert np.array_equal(host(y)[~long], want[~long])
+        np.testing.assert_allclose(host(y)[long], want[long], rtol=1e-13)
+    else:
+        assert np.array_equal(host(y), want)
+    # alpha / beta epilogue: y = -A x + b
+    b = ppo.uniform(n, 4)
+    yb = dev(torch, b)
+    M.MatPVec(dev(torch, x), yb, alpha=-1.0, beta=1.0)
+    np.testing.assert_allclose(host(yb), b - want, rtol=0, atol=1e-13 * (1 + np.abs(want).max()))
+
+
+@pytest.mark.parametrize("name", ["lap3d_20", "upwind_17", "ragged"])
+def test_spmv_t_and_fp32(pp, torch, name):
+    A = MATS[name]()
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    x = ppo.uniform(n, 8)
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    M.transMatPVec(dev(torch, x), y, alpha=1.0, beta=0.0)
+    assert np.array_equal(host(y), ppo.spmv_t(A, x))
+    y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+    M.MatPVec(dev(torch, x, torch.float32), y32)
+    ref = ppo.spmv(A, x)
+    scale = np.abs(A[2]).max() * np.abs(x).max() * 40
+    np.testing.assert_allclose(host(y32), ref, rtol=0, atol=1e-6 * scale)
+
+
+def test_blas1(pp, torch):
+    from prispack_amd import _capi
+    import ctypes as C
+    ctx = pp.Context.default()
+    L = _capi.lib()
+    for n in (1, 1000, 1_000_003):
+        x, y = ppo.uniform(n, 1), ppo.uniform(n, 2)
+        tx, ty = dev(torch, x), dev(torch, y)
+        torch.cuda.synchronize()
+        out = C.c_double()
+        ctx.check(L.pp_dot(ctx.h, 0, n, C.c_void_p(tx.data_ptr()), C.c_void_p(ty.data_ptr()), C.byref(out)))
+        assert abs(out.value - x @ y) <= 1e-13 * n
+        ctx.check(L.pp_nrm2(ctx.h, 0, n, C.c_void_p(tx.data_ptr()), C.byref(out)))
+        assert abs(out.value - np.linalg.norm(x)) <= 1e-13 * np.sqrt(n)
+        tz = torch.empty_like(tx)
+        ctx.check(L.pp_axpby(ctx.h, 0, n, 2.0, C.c_void_p(tx.data_ptr()), -3.0, C.c_void_p(ty.data_ptr()),
+                             C.c_void_p(tz.data_ptr())))
+        ctx.sync()
+        assert np.array_equal(host(tz), 2.0 * x + -3.0 * y)
+
+
+# --------------------------------------------------------------------------- preconditioner sweeps
+PRECS = [("ict", (1e-3,), "lap"), ("ict", (0.0,), "lap_small"), ("ict", (1e-2,), "lap"), ("ildlt", (1e-3,), "lap"),
+         ("ic0", (), "lap"), ("ilut", (1e-2, 20), "upw"), ("ilut", (0.0, 3), "upw"), ("ilu0", (), "upw")]
+
+
+def prec_matrix(which):
+    return {"lap": ppo.laplacian(16, 15, 14), "lap_small": ppo.laplacian(9, 8, 7),
+            "upw": ppo.upwind(16, 15, 14, 20.0, 10.0, 5.0)}[which]
+
+
+def make_prec(pp, A, kind, args):
+    cls = {"ict": pp.IncompleteCholesky, "ildlt": pp.IncompleteLDLT, "ic0": pp.IC0, "ilu0": pp.ILU0}
+    if kind == "ilut":
+        P = pp.IncompleteLU(pp.CSRMatrix(*A), lfill=args[1], tolerance=args[0])
+    else:
+        P = cls[kind](pp.CSRMatrix(*A), *args)
+    P.setup()
+    return P
+
+
+@pytest.mark.parametrize("kind,args,mat", PRECS)
+def test_precond_apply_bit_exact(pp, torch, kind, args, mat):
+    A = prec_matrix(mat)
+    n = len(A[0]) - 1
+    print(f"[precond] {kind}{args} {mat} n={n}", flush=True)
+    P = make_prec(pp, A, kind, args)
+    Q = ppo.Prec(A, kind, *args)
+    assert P.getPreconditionNonZeroCount() == Q.mnnz
+    x = ppo.uniform(n, 21)
+    t = dev(torch, x)
+    P.MInvSolve(t)
+    assert np.array_equal(host(t), Q.apply(x))
+    for part in ("lower", "diag", "upper"):
+        t = dev(torch, x)
+        getattr(P, {"lower": "MSolveLower", "diag": "MSolveDiagonal", "upper": "MSolveUpper"}[part])(t)
+        assert np.array_equal(host(t), getattr(Q, part)(x)), part
+    # out-of-place form and FP32
+    out = torch.zeros(n, dtype=torch.float64, device="cuda")
+    P.MInvSolve(dev(torch, x), out)
+    assert np.array_equal(host(out), Q.apply(x))
+    t32 = dev(torch, x, torch.float32)
+    P.MInvSolve(t32)
+    ref = Q.apply(x)
+    np.testing.assert_allclose(host(t32), ref, rtol=0, atol=2e-5 * np.abs(ref).max())
+
+
+def test_precond_errors(pp, torch):
+    from prispack_amd import PrisPackError
+    from prispack_amd._capi import PP_ERR_NOT_READY, PP_ERR_NOT_SPD
+    A = ppo.laplacian(6, 6, 1, 0, 0, 0, 6.0)
+    P = pp.IncompleteCholesky(pp.CSRMatrix(*A), 1e-3)
+    with pytest.raises(PrisPackError) as e:
+        P.MInvSolve(dev(torch, np.ones(36)))
+    assert e.value.status == PP_ERR_NOT_READY
+    with pytest.raises(PrisPackError) as e:
+        P.setup()
+    assert e.value.status == PP_ERR_NOT_SPD
+
+
+# --------------------------------------------------------------------------- solvers
+def _golden(name):
+    return dict(np.load(os.path.join(GOLD, name + ".npz")))
+
+
+@pytest.mark.parametrize("name", ["cg_128x128x1_tau0.001", "cg_128x128x1_tau0.01", "cg_12x12x12_tau0.001",
+                                  "cg_20x20x20_tau0.01", "cg_16x16x1_tau0.001"])
+def test_cg_parity_with_reference(pp, torch, name):
+    g = _golden(name)
+    dims = tuple(int(t) for t in g["dims"])
+    A = ppo.laplacian(*dims)
+    n = len(A[0]) - 1
+    print(f"[cg] {name} n={n}", flush=True)
+    xt = ppo.uniform(n, int(g["seed"]))
+    b = ppo.spmv(A, xt)
+    M = pp.CSRMatrix(*A)
+    P = pp.IncompleteCholesky(M, float(g["tau"]))
+    cg = pp.CG(n, P, pp.FlexibleCG, tol=float(g["tol"]), maxIter=int(g["maxit"]))
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cg.solve(M, dev(torch, b), x)
+    xs = host(x)
+    assert abs(cg.getNumIter() - int(g["iters"])) <= 2
+    assert cg.getConvergence()
+    assert cg.report.true_relres <= float(g["tol"])
+    xr = g["x"]
+    assert np.linalg.norm(xs - xr) / np.linalg.norm(xr) <= 1e-6
+    # the residual history follows the oracle's to rounding
+    o = ppo.cg(A, b, ppo.Prec(A, "ict", float(g["tau"])), tol=float(g["tol"]), maxit=int(g["maxit"]))
+    h = cg.history()
+    k = min(len(h), len(o["hist"]))
+    np.testing.assert_allclose(h[:k], o["hist"][:k], rtol=1e-6)
+
+
+def test_normal_cg_and_host_boundary(pp, torch):
+    g = _golden("cg_16x16x1_tau0.001")
+    A = ppo.laplacian(16, 16, 1)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.uniform(n, 0))
+    M = pp.CSRMatrix(*A)
+    P = pp.IncompleteCholesky(M, 1e-3)
+    cg = pp.CG(n, P, pp.NormalCG, tol=1e-8, maxIter=1000)
+    x = np.zeros(n)
+    cg.solve(M, b, x)  # host in / host out
+    assert abs(cg.getNumIter() - int(g["normal_iters"])) <= 2
+
+
+def test_cg_limits_and_errors(pp, torch):
+    from prispack_amd import PrisPackError
+    from prispack_amd._capi import PP_ERR_NOT_FINITE
+    A = ppo.laplacian(30, 30, 1)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.uniform(n, 0))
+    M = pp.CSRMatrix(*A)
+    P = pp.IncompleteCholesky(M, 1e-2)
+    o = ppo.cg(A, b, ppo.Prec(A, "ict", 1e-2), tol=1e-12, maxit=5)
+    cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-12, maxIter=5)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cg.solve(M, dev(torch, b), x)
+    assert not cg.getConvergence() and cg.getNumIter() == o["iters"] == 6
+    assert abs(cg.getError() - o["error"]) <= 1e-10 * abs(o["error"])
+    bad = b.copy()
+    bad[7] = np.nan
+    with pytest.raises(PrisPackError) as e:
+        cg.solve(M, dev(torch, bad), torch.zeros(n, dtype=torch.float64, device="cuda"))
+    assert e.value.status == PP_ERR_NOT_FINITE
+
+
+@pytest.mark.parametrize("dims", [(64, 64, 1), (24, 24, 24)])
+def test_mixed_precision_cg(pp, torch, dims):
+    A = ppo.laplacian(*dims)
+    n = len(A[0]) - 1
+    xt = ppo.normal(n, 0)
+    b = ppo.spmv(A, xt)
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M)
+    x64 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    c64 = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000)
+    c64.solve(M, dev(torch, b), x64)
+    xm = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cm = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision="mixed")
+    cm.solve(M, dev(torch, b), xm)
+    assert cm.getConvergence() and cm.report.true_relres < 1e-8
+    assert cm.report.outer_iters >= 2
+    # against the FP64 oracle solve with the same IC(0) factor
+    o = ppo.cg(A, b, ppo.Prec(A, "ic0"), tol=1e-8, maxit=2000)
+    xo = o["x"]
+    assert abs(c64.getNumIter() - o["iters"]) <= 2
+    assert np.linalg.norm(host(x64) - xo) / np.linalg.norm(xo) <= 1e-6
+    err = np.linalg.norm(host(xm) - xo) / np.linalg.norm(xo)
+    print(f"mixed: outer {cm.report.outer_iters} inner {cm.report.inner_iters} fp64 {c64.getNumIter()} err {err:.2e}")
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["gmres_lapconv_12x12x12", "gmres_upwind_16x16x16", "gmres_upwind_10x10x10_l3"])
+def test_gmres_parity_with_reference(pp, torch, name):
+    g = _golden(name)
+    A = (g["rp"], g["ci"], g["v"])
+    n = len(A[0]) - 1
+    print(f"[gmres] {name} n={n}", flush=True)
+    xt = ppo.uniform(n, int(g["seed"]))
+    b = ppo.spmv(A, xt)
+    M = pp.CSRMatrix(*A)
+    P = pp.IncompleteLU(M, lfill=int(g["lfil"]), tolerance=float(g["tau"]))
+    for quirks in (True, False):
+        gm = pp.GMRES(n, P, float(g["tol"]), pp.FlexibleGMRES, reference_quirks=quirks)
+        gm.setRestart(True, int(g["restart"]))
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        gm.solve(M, dev(torch, b), x)
+        xs = host(x)
+        assert abs(gm.getNumIter() - int(g["iters"])) <= 2, (quirks, gm.getNumIter(), int(g["iters"]))
+        assert gm.getConvergence()
+        xr = g["x"]
+        assert np.linalg.norm(xs - xr) / np.linalg.norm(xr) <= 1e-6
+        if not quirks:  # the fixed back-substitution meets the tolerance on the true residual
+            assert gm.report.true_relres <= float(g["tol"])
+
+
+def test_gmres_mixed(pp, torch):
+    A = ppo.upwind(24, 24, 24, 20.0, 10.0, 5.0)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.uniform(n, 0))
+    M = pp.CSRMatrix(*A)
+    P = pp.ILU0(M)
+    gm = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, precision="mixed")
+    gm.setRestart(True, 30)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    gm.solve(M, dev(torch, b), x)
+    assert gm.getConvergence() and gm.report.true_relres <= 1e-8
+    o = ppo.gmres(A, b, ppo.Prec(A, "ilu0"), tol=1e-10, restart=30)
+    err = np.linalg.norm(host(x) - o["x"]) / np.linalg.norm(o["x"])
+    assert err <= 1e-6

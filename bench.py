@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--trsv", type=int, default=0, help="SpTRSV schedule: 0 auto, 1 levels, 2 sync-free")
     return ap.parse_args()
 
 
@@ -104,7 +105,7 @@ def main():
     b = d_b.cpu().numpy()
 
     t0 = time.perf_counter()
-    P = pp.IC0(M, make_f32=True)
+    P = pp.IC0(M, make_f32=True, trsv=args.trsv)
     P.setup()
     setup_s = time.perf_counter() - t0
     info = P.info()
@@ -150,15 +151,23 @@ def main():
     spmv32_ms = kernel_time_ms(torch, stream, lambda: L.pp_spmv(ctx.h, M.h, 1, 1.0, p32x, 0.0, p32y),
                                args.kernel_reps)
     spmv32_bytes = 8 * nnz + 12 * n + 4
-    nnz_l = (nnz + n) // 2
-    trsv_bytes = 12 * nnz_l + 4 * (n + 1) + 16 * n + 4 * n
+    # one preconditioner apply = forward sweep on L + backward sweep on L^T;
+    # per sweep: 12 nnz_L (values + columns) + 4(n+1) row pointers + 4n level
+    # permutation + 8n right-hand side + 8n solution (diagonal folded in the row)
+    nnz_l = (nnz - n) // 2
+    sweep_bytes = 12 * nnz_l + 4 * (n + 1) + 4 * n + 16 * n + 8 * n
+    trsv_bytes = 2 * sweep_bytes
     tv = torch.tensor(xt, device="cuda")
     ptv = C.c_void_p(tv.data_ptr())
-    trsv_ms = kernel_time_ms(torch, stream, lambda: L.pp_precond_apply_part(ctx.h, P.h, 0, 0, ptv),
+    trsv_ms = kernel_time_ms(torch, stream, lambda: L.pp_precond_apply(ctx.h, P.h, 0, ptv, ptv),
                              max(3, args.kernel_reps // 10))
+    tv32 = tv.float()
+    ptv32 = C.c_void_p(tv32.data_ptr())
+    trsv32_ms = kernel_time_ms(torch, stream, lambda: L.pp_precond_apply(ctx.h, P.h, 1, ptv32, ptv32),
+                               max(3, args.kernel_reps // 10))
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
     log(f"[bench] SpMV fp64 {spmv_ms * 1e3:.1f} us {gbs(spmv_bytes, spmv_ms):.0f} GB/s; fp32 {spmv32_ms * 1e3:.1f} us; "
-        f"L-sweep fp64 {trsv_ms * 1e3:.1f} us {gbs(trsv_bytes, trsv_ms):.1f} GB/s")
+        f"M^-1 apply fp64 {trsv_ms * 1e3:.1f} us {gbs(trsv_bytes, trsv_ms):.1f} GB/s; fp32 {trsv32_ms * 1e3:.1f} us")
 
     cpu = None
     if not args.no_cpu:
@@ -168,10 +177,12 @@ def main():
             cpu = {"value": None, "unit": "s", "cores": 0, "kind": "reference", "sample": f"unavailable: {e}"}
 
     med64, medmx = float(np.median(t64)), float(np.median(tmx))
-    dominant = {"kernel": "k_sweep_level (one forward sweep = %d level launches)" % info.levels_lower,
+    sched = {0: "levels", 1: "levels", 2: "syncfree"}[args.trsv]
+    dominant = {"kernel": f"IC(0) apply = forward + backward sweep ({sched} schedule, {info.levels_lower} levels)",
                 "bound": "hbm", "achieved": round(gbs(trsv_bytes, trsv_ms), 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(gbs(trsv_bytes, trsv_ms) / HBM_PEAK_GBS, 5),
-                "traffic": None, "bytes_per_launch": trsv_bytes, "avg_ms": round(trsv_ms, 4)}
+                "traffic": None, "bytes_per_launch": trsv_bytes, "avg_ms": round(trsv_ms, 4),
+                "fp32_avg_ms": round(trsv32_ms, 4)}
     out = {
         "metric": "time-to-rtol 1e-8 (s) mixed-prec vs FP64 on GPU; SpMV/SpTRSV HBM GB/s",
         "value": round(medmx, 5), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

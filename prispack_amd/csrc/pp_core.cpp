@@ -212,6 +212,11 @@ int pp_ctx_create(int device, pp_ctx **out) {
         hipDeviceProp_t prop;
         PP_HIP(hipGetDeviceProperties(&prop, device));
         c->cu_count = prop.multiProcessorCount;
+        int clock_khz = 0;
+        if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
+            clock_khz <= 0)
+            clock_khz = 100000;
+        c->spin_deadline = (long long)clock_khz * 1000LL * 2;  // a dependency wait may last 2 s
         c->red.partials = dalloc<double>((size_t)S_COUNT * kMaxPartials);
         c->red.counters = (unsigned *)dalloc<uint32_t>(S_COUNT);
         c->scal = dalloc<double>(S_COUNT);

@@ -57,6 +57,7 @@ enum Ctl : int {
     C_CHECK,      // index of the next convergence check
     C_LAST,       // the next body is the last one
     C_STATUS,     // PP_OK / PP_ERR_BREAKDOWN / PP_ERR_NOT_FINITE
+    C_SPIN,       // a sync-free sweep hit its wait deadline
     C_COUNT = 16
 };
 
@@ -92,6 +93,7 @@ struct pp_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     void *ws[32] = {};             // cached solver workspaces
     size_t ws_bytes[32] = {};
+    long long spin_deadline = 0;   // wall_clock64 ticks a sync-free wait may last
 };
 
 namespace pp {
@@ -152,6 +154,9 @@ struct pp_precond {
     pp::DevSweep<double> low64, up64;
     pp::DevSweep<float> low32, up32;
     bool have32 = false;
+    int trsv = PP_TRSV_LEVELS;    // resolved SpTRSV schedule
+    double *tmp64 = nullptr;      // out-of-place buffer between the two sweeps
+    float *tmp32 = nullptr;
     void *multilevel = nullptr;   // AMSED / GMSLR state
 };
 
@@ -197,6 +202,8 @@ void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &bl
 
 template <class T>
 void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x);
+template <class T>
+void launch_sweep_syncfree(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *y);
 
 // PCG device steps (templated on the working precision)
 template <class T>

@@ -326,6 +326,103 @@ template void launch_sweep<double>(pp_ctx *, const DevSweep<double> &, int, cons
 template void launch_sweep<float>(pp_ctx *, const DevSweep<float> &, int, const float *, float *);
 
 // ---------------------------------------------------------------------------
+// Sync-free triangular sweep: ONE launch per sweep.  Rows keep the level
+// order, so every dependency of a row lives in a lower workgroup index, and
+// workgroups are dispatched in index order on each XCD -- the lowest
+// unfinished row always has its inputs.  The output vector is pre-filled with
+// a signalling-NaN pattern that no arithmetic produces; a row polls each
+// dependency's value with an agent-scope atomic load until it is no longer
+// that pattern, so one 64-bit (32-bit) word carries both the value and its
+// ready flag and no release/acquire fences are needed.  Dependencies are
+// consumed in storage order (same arithmetic as the level kernel, so the FP64
+// result is bit-identical).  A wall-clock deadline bounds every wait: on
+// expiry the sweep stops and raises an error word instead of hanging.
+// ---------------------------------------------------------------------------
+template <class T>
+struct SentinelBits;
+template <>
+struct SentinelBits<double> {
+    using U = unsigned long long;
+    static constexpr U value = 0x7FF4B5A5C3D2E1F0ull;  // signalling NaN
+};
+template <>
+struct SentinelBits<float> {
+    using U = unsigned int;
+    static constexpr U value = 0x7FA5C3E1u;  // signalling NaN
+};
+
+template <class T>
+__global__ void k_fill_sentinel(uint32_t n, T *y) {
+    using U = typename SentinelBits<T>::U;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        reinterpret_cast<U *>(y)[i] = SentinelBits<T>::value;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_sweep_syncfree(const uint32_t *__restrict__ perm,
+                                                        const uint32_t *__restrict__ rp,
+                                                        const uint32_t *__restrict__ ci,
+                                                        const T *__restrict__ val, const T *__restrict__ diag,
+                                                        int mode, const T *__restrict__ b, T *y, uint32_t n,
+                                                        long long deadline, int *ctl) {
+#pragma clang fp contract(off)
+    using U = typename SentinelBits<T>::U;
+    constexpr U kSent = SentinelBits<T>::value;
+    if (ctl[C_STOP]) return;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool done = k >= n;
+    uint32_t i = 0, e = 0, e1 = 0;
+    T s = 0;
+    if (!done) {
+        i = perm[k];
+        s = b[i];
+        if (mode == 3) s = s / diag[i];
+        e = rp[k];
+        e1 = rp[k + 1];
+    }
+    U *ybits = reinterpret_cast<U *>(y);
+    const long long t0 = wall_clock64();
+    for (unsigned iter = 0;; ++iter) {
+        if (!done) {
+            while (e < e1) {
+                const U bits = __hip_atomic_load(&ybits[ci[e]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (bits == kSent) break;
+                const T p = val[e] * __builtin_bit_cast(T, bits);
+                s = s - p;
+                ++e;
+            }
+            if (e == e1) {
+                if (mode == 0) s = s / diag[i];
+                else if (mode == 2) s = s * diag[i];
+                __hip_atomic_store(&ybits[i], __builtin_bit_cast(U, s), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                done = true;
+            }
+        }
+        if (__all(done)) break;
+        if (wall_clock64() - t0 > deadline) {
+            if (!done) atomicOr(&ctl[C_SPIN], 1);
+            break;
+        }
+        // another wave already gave up: its dependents would wait in vain
+        if ((iter & 63) == 63 && __hip_atomic_load(&ctl[C_SPIN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+template <class T>
+void launch_sweep_syncfree(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *y) {
+    if (s.n == 0) return;
+    const unsigned g = (s.n + kBlock - 1) / kBlock;
+    k_fill_sentinel<T><<<grid_for(ctx, s.n), kBlock, 0, ctx->stream>>>(s.n, y);
+    k_sweep_syncfree<T><<<g, kBlock, 0, ctx->stream>>>(s.perm, s.rp, s.ci, s.val, s.diag, mode, b, y, s.n,
+                                                        ctx->spin_deadline, ctx->ctl);
+    PP_HIP(hipGetLastError());
+}
+template void launch_sweep_syncfree<double>(pp_ctx *, const DevSweep<double> &, int, const double *, double *);
+template void launch_sweep_syncfree<float>(pp_ctx *, const DevSweep<float> &, int, const float *, float *);
+
+// ---------------------------------------------------------------------------
 // PCG steps (FlexibleCG, source/Solver/CG.cpp:49-79):
 //   d = w - beta d with beta = (w, A d_old) / (d_old, A d_old)   [fused (d, r)]
 //   v = A d                                                       [fused (d, v)]
@@ -410,9 +507,9 @@ __global__ __launch_bounds__(256) void k_pcg_xr(uint32_t n, T *__restrict__ x, T
     __shared__ double sh[4];
     if (ctl[C_STOP]) return;
     const double den = scal[S_DV];
-    if (den == 0.0 || !(den == den)) {  // (d, A d) = 0 or NaN: Krylov breakdown
+    if (den == 0.0 || !(den == den)) {  // (d, A d) = 0: breakdown; NaN: non-finite data
         if (blockIdx.x == 0 && threadIdx.x == 0) {
-            ctl[C_STATUS] = PP_ERR_BREAKDOWN;
+            ctl[C_STATUS] = den == 0.0 ? PP_ERR_BREAKDOWN : PP_ERR_NOT_FINITE;
             ctl[C_STOP] = 1;
             ctl[C_M] = ctl[C_CHECK];
         }

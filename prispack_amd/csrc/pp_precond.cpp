@@ -225,6 +225,20 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
         ml_apply(ctx, M, dtype, x, y);
         return;
     }
+    if (M->trsv == PP_TRSV_SYNCFREE) {
+        // out of place: x -> tmp (lower) -> y (upper); y may alias x
+        if (dtype == PP_F64) {
+            if (!M->tmp64) M->tmp64 = dalloc<double>(M->n);
+            launch_sweep_syncfree<double>(ctx, M->low64, M->hlow.mode, (const double *)x, M->tmp64);
+            launch_sweep_syncfree<double>(ctx, M->up64, M->hup.mode, M->tmp64, (double *)y);
+        } else {
+            precond_need_f32(M);
+            if (!M->tmp32) M->tmp32 = dalloc<float>(M->n);
+            launch_sweep_syncfree<float>(ctx, M->low32, M->hlow.mode, (const float *)x, M->tmp32);
+            launch_sweep_syncfree<float>(ctx, M->up32, M->hup.mode, M->tmp32, (float *)y);
+        }
+        return;
+    }
     if (x != y) PP_HIP(hipMemcpyAsync(y, x, (size_t)M->n * esz, hipMemcpyDeviceToDevice, ctx->stream));
     if (dtype == PP_F64) {
         launch_sweep<double>(ctx, M->low64, M->hlow.mode, (const double *)y, (double *)y);
@@ -303,6 +317,7 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
             A.ci = M->A->h_ci;
             A.v = M->A->h_v;
             setup_triangular(M, A, true);
+            M->trsv = M->opts.trsv == PP_TRSV_AUTO ? PP_TRSV_LEVELS : M->opts.trsv;
         }
         M->ready = 1;
         M->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -313,8 +328,13 @@ int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void 
     if (!ctx) return PP_ERR_INVALID;
     return guarded_pc(ctx, [&] {
         PP_CHECK(M && x && y, PP_ERR_INVALID, "NULL argument");
-        launch_fill_zero(ctx, ctx->ctl, sizeof(int));  // clear the stop word (graph-safe)
+        launch_fill_zero(ctx, ctx->ctl, C_COUNT * sizeof(int));  // clear stop / error words (graph-safe)
         precond_apply(ctx, M, dtype, x, y);
+        if (M->trsv == PP_TRSV_SYNCFREE) {
+            int ctl[C_COUNT];
+            read_ctl(ctx, ctl);
+            PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
+        }
     });
 }
 
@@ -403,6 +423,8 @@ void pp_precond_destroy(pp_precond *M) {
     if (!M) return;
     (void)hipSetDevice(M->ctx->device);
     (void)hipStreamSynchronize(M->ctx->stream);
+    dfree(M->tmp64);
+    dfree(M->tmp32);
     M->low64.free_all();
     M->up64.free_all();
     M->low32.free_all();

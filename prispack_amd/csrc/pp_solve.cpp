@@ -72,6 +72,7 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
     const uint32_t every = std::max<uint32_t>(1, check_every);
     for (;;) {
         read_ctl(ctx, ctl);
+        PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
         if (ctl[C_STOP]) break;
         if (launched > maxit + 1) break;  // cannot happen: the device sets STOP by maxit
         for (uint32_t k = 0; k < every; ++k) {
@@ -359,6 +360,11 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
         if (!conv && k < maxcyc) {
             if (!o.reference_quirks) m = mm;
         }
+    }
+    {
+        int ctl[C_COUNT];
+        read_ctl(ctx, ctl);
+        PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
     }
     res.conv = conv;
     res.iters = k * (int32_t)mm + (int32_t)m;  // GMRES.cpp:218

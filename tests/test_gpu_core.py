@@ -139,22 +139,26 @@ def prec_matrix(which):
             "upw": ppo.upwind(16, 15, 14, 20.0, 10.0, 5.0)}[which]
 
 
-def make_prec(pp, A, kind, args):
+def make_prec(pp, A, kind, args, trsv=0):
     cls = {"ict": pp.IncompleteCholesky, "ildlt": pp.IncompleteLDLT, "ic0": pp.IC0, "ilu0": pp.ILU0}
     if kind == "ilut":
-        P = pp.IncompleteLU(pp.CSRMatrix(*A), lfill=args[1], tolerance=args[0])
+        P = pp.IncompleteLU(pp.CSRMatrix(*A), lfill=args[1], tolerance=args[0], trsv=trsv)
     else:
-        P = cls[kind](pp.CSRMatrix(*A), *args)
+        P = cls[kind](pp.CSRMatrix(*A), *args, trsv=trsv)
     P.setup()
     return P
 
 
+TRSV = {"levels": 1, "syncfree": 2}
+
+
+@pytest.mark.parametrize("trsv", ["levels", "syncfree"])
 @pytest.mark.parametrize("kind,args,mat", PRECS)
-def test_precond_apply_bit_exact(pp, torch, kind, args, mat):
+def test_precond_apply_bit_exact(pp, torch, kind, args, mat, trsv):
     A = prec_matrix(mat)
     n = len(A[0]) - 1
-    print(f"[precond] {kind}{args} {mat} n={n}", flush=True)
-    P = make_prec(pp, A, kind, args)
+    print(f"[precond] {kind}{args} {mat} n={n} {trsv}", flush=True)
+    P = make_prec(pp, A, kind, args, TRSV[trsv])
     Q = ppo.Prec(A, kind, *args)
     assert P.getPreconditionNonZeroCount() == Q.mnnz
     x = ppo.uniform(n, 21)
@@ -193,18 +197,19 @@ def _golden(name):
     return dict(np.load(os.path.join(GOLD, name + ".npz")))
 
 
+@pytest.mark.parametrize("trsv", ["levels", "syncfree"])
 @pytest.mark.parametrize("name", ["cg_128x128x1_tau0.001", "cg_128x128x1_tau0.01", "cg_12x12x12_tau0.001",
                                   "cg_20x20x20_tau0.01", "cg_16x16x1_tau0.001"])
-def test_cg_parity_with_reference(pp, torch, name):
+def test_cg_parity_with_reference(pp, torch, name, trsv):
     g = _golden(name)
     dims = tuple(int(t) for t in g["dims"])
     A = ppo.laplacian(*dims)
     n = len(A[0]) - 1
-    print(f"[cg] {name} n={n}", flush=True)
+    print(f"[cg] {name} n={n} {trsv}", flush=True)
     xt = ppo.uniform(n, int(g["seed"]))
     b = ppo.spmv(A, xt)
     M = pp.CSRMatrix(*A)
-    P = pp.IncompleteCholesky(M, float(g["tau"]))
+    P = pp.IncompleteCholesky(M, float(g["tau"]), trsv=TRSV[trsv])
     cg = pp.CG(n, P, pp.FlexibleCG, tol=float(g["tol"]), maxIter=int(g["maxit"]))
     x = torch.zeros(n, dtype=torch.float64, device="cuda")
     cg.solve(M, dev(torch, b), x)
@@ -255,14 +260,15 @@ def test_cg_limits_and_errors(pp, torch):
     assert e.value.status == PP_ERR_NOT_FINITE
 
 
+@pytest.mark.parametrize("trsv", ["levels", "syncfree"])
 @pytest.mark.parametrize("dims", [(64, 64, 1), (24, 24, 24)])
-def test_mixed_precision_cg(pp, torch, dims):
+def test_mixed_precision_cg(pp, torch, dims, trsv):
     A = ppo.laplacian(*dims)
     n = len(A[0]) - 1
     xt = ppo.normal(n, 0)
     b = ppo.spmv(A, xt)
     M = pp.CSRMatrix(*A)
-    P = pp.IC0(M)
+    P = pp.IC0(M, trsv=TRSV[trsv])
     x64 = torch.zeros(n, dtype=torch.float64, device="cuda")
     c64 = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000)
     c64.solve(M, dev(torch, b), x64)

@@ -252,6 +252,9 @@ int pp_batch_solve(pp_ctx *ctx, pp_batch *B, const pp_solve_opts *opts, const do
 int pp_batch_solve_t(pp_ctx *ctx, pp_batch *B, const pp_solve_opts *opts, const double *g,
                      double *lambda, int32_t *iters_out, pp_solve_report *rep);
 int pp_batch_tune(pp_ctx *ctx, pp_batch *B, int *chosen_block);
+/* grad[k] = scale * lambda[row(k)] * x[col(k)] over the batch's pattern (device) */
+int pp_batch_outer_on_pattern(pp_ctx *ctx, const pp_batch *B, double scale, const double *lambda,
+                              const double *x, double *grad_values);
 void pp_batch_destroy(pp_batch *B);
 
 /* ---- model problems (ModelProblem.cpp:68-133 / MatrixTools.h:16-62 and the

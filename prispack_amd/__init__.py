@@ -477,3 +477,93 @@ class GMRES(_Solver):
 
     def setRestart(self, restart, m):
         self.opts.restart = m if restart else max(1, self.n - 1)
+
+
+# ----------------------------------------------------------------------------- batched systems
+class BatchedSystems:
+    """Many independent SPD systems (block-diagonal CSR) solved together by ONE
+    persistent, graph-captured IC(0)-PCG launch, one workgroup per system
+    (no reference counterpart).  `systems` is a list of (rp, ci, v) CSR
+    triples; right-hand sides and solutions are concatenated CUDA float64
+    tensors.  Solves start from x0 = 0."""
+
+    def __init__(self, systems, ctx=None):
+        self.ctx = ctx or Context.default()
+        sizes = [len(s[0]) - 1 for s in systems]
+        self.row_off = np.zeros(len(systems) + 1, np.uint32)
+        self.row_off[1:] = np.cumsum(sizes)
+        rps, cis, vs, base_nnz = [np.zeros(1, np.uint32)], [], [], 0
+        for s, (rp, ci, v) in enumerate(systems):
+            rps.append(_u32(rp)[1:] + base_nnz)
+            cis.append(_u32(ci) + self.row_off[s])
+            vs.append(_f64(v))
+            base_nnz += int(rp[-1])
+        self.rp = np.concatenate(rps).astype(np.uint32)
+        self.ci = np.concatenate(cis).astype(np.uint32)
+        self.v = np.concatenate(vs)
+        self.nsys, self.n, self.nnz = len(systems), int(self.row_off[-1]), len(self.ci)
+        h = C.c_void_p()
+        self.ctx.check(_capi.lib().pp_batch_create(self.ctx.h, self.nsys, _p(self.row_off, C.c_uint32),
+                                                   _p(self.rp, C.c_uint32), _p(self.ci, C.c_uint32),
+                                                   _p(self.v, C.c_double), C.byref(h)))
+        self.h = h
+        self.block = 256
+        self.report = _capi.SolveReport()
+
+    def tune(self):
+        """Pick the workgroup size (row tile of the SpMV and sweep steps) by timing."""
+        bs = C.c_int()
+        self.ctx.check(_capi.lib().pp_batch_tune(self.ctx.h, self.h, C.byref(bs)))
+        self.block = bs.value
+        return bs.value
+
+    def set_values(self, values):
+        """New values on the same pattern (numpy or CUDA tensor); re-factors IC(0)."""
+        L = _capi.lib()
+        if isinstance(values, np.ndarray):
+            v = _f64(values)
+            self.ctx.check(L.pp_batch_set_values(self.ctx.h, self.h, v.ctypes.data_as(C.c_void_p), 0))
+        else:
+            _before(self.ctx)
+            p, _ = _dev_ptr(values.contiguous(), PP_F64)
+            self.ctx.check(L.pp_batch_set_values(self.ctx.h, self.h, p, 1))
+
+    def _solve(self, fn, b, x, precision, tol, maxit, inner_tol):
+        o = _capi.SolveOpts()
+        L = _capi.lib()
+        L.pp_solve_opts_default(C.byref(o), PP_SOLVER_CG)
+        o.precision = PP_MIXED if precision == "mixed" else PP_FP64
+        o.tol, o.maxit, o.inner_tol = tol, maxit, inner_tol
+        its = np.zeros(self.nsys, np.int32)
+        rep = _capi.SolveReport()
+        _before(self.ctx)
+        pb, _ = _dev_ptr(b, PP_F64)
+        px, _ = _dev_ptr(x, PP_F64)
+        self.ctx.check(fn(self.ctx.h, self.h, C.byref(o), pb, px, _p(its, C.c_int32), C.byref(rep)))
+        self.report = rep
+        return its
+
+    def solve(self, b, x, precision="mixed", tol=1e-8, maxit=20000, inner_tol=1e-6):
+        """x = A^{-1} b for every system; returns per-system iteration counts."""
+        return self._solve(_capi.lib().pp_batch_solve, b, x, precision, tol, maxit, inner_tol)
+
+    def solve_t(self, g, lam, precision="mixed", tol=1e-8, maxit=20000, inner_tol=1e-6):
+        """lambda = A^{-T} g (the adjoint solve)."""
+        return self._solve(_capi.lib().pp_batch_solve_t, g, lam, precision, tol, maxit, inner_tol)
+
+    def outer_on_pattern(self, lam, x, scale=-1.0):
+        """grad[k] = scale * lam[row(k)] * x[col(k)] on the batch pattern."""
+        torch = _torch()
+        g = torch.empty(self.nnz, dtype=torch.float64, device=lam.device)
+        _before(self.ctx)
+        self.ctx.check(_capi.lib().pp_batch_outer_on_pattern(self.ctx.h, self.h, scale, _dev_ptr(lam, PP_F64)[0],
+                                                              _dev_ptr(x, PP_F64)[0], _dev_ptr(g, PP_F64)[0]))
+        return g
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _capi.lib().pp_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass

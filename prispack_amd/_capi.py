@@ -115,6 +115,7 @@ SIGS = {
     "pp_batch_solve": (C.c_int, [vp, vp, C.POINTER(SolveOpts), vp, vp, i32p, C.POINTER(SolveReport)]),
     "pp_batch_solve_t": (C.c_int, [vp, vp, C.POINTER(SolveOpts), vp, vp, i32p, C.POINTER(SolveReport)]),
     "pp_batch_tune": (C.c_int, [vp, vp, C.POINTER(C.c_int)]),
+    "pp_batch_outer_on_pattern": (C.c_int, [vp, vp, C.c_double, vp, vp, vp]),
     "pp_batch_destroy": (None, [vp]),
     "pp_generate_laplacian": (C.c_int, [C.c_int] * 3 + [C.c_double] * 4 + [u32p, u32p, u32p, u32p, f64p]),
     "pp_generate_upwind": (C.c_int, [C.c_int] * 3 + [C.c_double] * 3 + [u32p, u32p, u32p, u32p, f64p]),

@@ -1,6 +1,6 @@
 """Builds prispack_amd/libprispack_amd.so (HIP for gfx950) in-tree.
 
-    python -m prispack_amd.build        (or __graft_entry__.build())
+    python prispack_amd/build.py        (or __graft_entry__.build())
 
 Every source under csrc/ is compiled by hipcc with --offload-arch=gfx950; the
 objects go to _build/ at the repository root and the shared library next to

@@ -228,6 +228,8 @@ int pp_ctx_create(int device, pp_ctx **out) {
         PP_HIP(hipHostMalloc((void **)&c->pinned_ctl, 64 * sizeof(int), hipHostMallocDefault));
         PP_HIP(hipEventCreate(&c->ev0));
         PP_HIP(hipEventCreate(&c->ev1));
+        PP_HIP(hipEventCreateWithFlags(&c->ev_ring[0], hipEventDisableTiming));
+        PP_HIP(hipEventCreateWithFlags(&c->ev_ring[1], hipEventDisableTiming));
         ensure_hist(c.get(), 4096);
         PP_HIP(hipDeviceSynchronize());
         *out = c.release();
@@ -260,6 +262,8 @@ void pp_ctx_destroy(pp_ctx *ctx) {
     if (ctx->pinned_ctl) (void)hipHostFree(ctx->pinned_ctl);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->ev_ring)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }

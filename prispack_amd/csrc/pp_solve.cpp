@@ -68,13 +68,8 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
     pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on);
 
     int ctl[C_COUNT];
-    uint32_t launched = 1;  // bodies enqueued (the check index advances one per body)
     const uint32_t every = std::max<uint32_t>(1, check_every);
-    for (;;) {
-        read_ctl(ctx, ctl);
-        PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
-        if (ctl[C_STOP]) break;
-        if (launched > maxit + 1) break;  // cannot happen: the device sets STOP by maxit
+    auto body = [&] {
         for (uint32_t k = 0; k < every; ++k) {
             if (flexible && M) precond_apply(ctx, M, DT, r, w);
             else PP_HIP(hipMemcpyAsync(w, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
@@ -82,8 +77,52 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
             pcg_d_update<T>(ctx, n, false, w, d, r);
             launch_spmv(ctx, A, DT, 1.0, d, 0.0, v, S_DV, d, false);
             pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on);
-            ++launched;
         }
+    };
+    // The iteration body (every launch of it: sweeps, updates, SpMV) is
+    // captured once into a hipGraph and replayed.  Two bodies stay in flight:
+    // after each replay the control words are copied to pinned memory behind
+    // an event, and the host reads the copy of the previous replay, so the
+    // GPU never waits for the host; a replay past convergence only runs
+    // kernels that return at once.
+    Graph G;
+    PP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    try {
+        body();
+    } catch (...) {
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    PP_HIP(hipStreamEndCapture(ctx->stream, &G.g));
+    PP_HIP(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
+    int *slots = ctx->pinned_ctl;  // two C_COUNT-word slots
+    hipEvent_t evs[2] = {ctx->ev_ring[0], ctx->ev_ring[1]};
+    const uint64_t max_replays = (uint64_t)maxit / every + 2;
+    uint64_t issued = 0, checked = 0;
+    auto issue = [&] {
+        PP_HIP(hipGraphLaunch(G.exec, ctx->stream));
+        int *slot = slots + (issued & 1) * C_COUNT;
+        PP_HIP(hipMemcpyAsync(slot, ctx->ctl, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipEventRecord(evs[issued & 1], ctx->stream));
+        ++issued;
+    };
+    // the control words after the first step decide whether any body runs
+    read_ctl(ctx, ctl);
+    PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
+    if (!ctl[C_STOP]) {
+        issue();
+        for (;;) {
+            if (issued < max_replays) issue();
+            PP_HIP(hipEventSynchronize(evs[checked & 1]));
+            std::memcpy(ctl, slots + (checked & 1) * C_COUNT, C_COUNT * sizeof(int));
+            ++checked;
+            PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
+            if (ctl[C_STOP] || checked >= max_replays) break;
+        }
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        read_ctl(ctx, ctl);
     }
     res.conv = ctl[C_CONV];
     res.status = ctl[C_STATUS];

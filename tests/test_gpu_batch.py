@@ -1,0 +1,156 @@
+"""Batched persistent IC(0)-PCG and the differentiable solves, through the C
+ABI, against the oracle.  Bars: FP64 per-system iterations within +-2 of the
+oracle's FlexibleCG with the same IC(0) factor and x within 1e-6; mixed
+precision true relative residual < 1e-8 and x within 1e-6 of the FP64 oracle;
+gradients within 1e-5 (relative) of central finite differences (h = 1e-6, FP64)
+and within 1e-6 of the FP64 CPU adjoint (oracle CG on A^T)."""
+import numpy as np
+import pytest
+
+from oracle import ppo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def pp(torch):
+    import prispack_amd
+    return prispack_amd
+
+
+def make_systems(nsys, lo, hi, seed):
+    """Uneven 2D variable-coefficient diffusion systems: side ~ U[lo, hi],
+    kappa = exp(z), z ~ N(0, 1) per node."""
+    sides = [lo + int((hi - lo + 1) * u) for u in (ppo.uniform(nsys, seed) + 1) / 2]
+    systems = []
+    for s, side in enumerate(sides):
+        kappa = np.exp(ppo.normal(side * side, 1000 + s))
+        systems.append(ppo.vardiff2d(side, kappa))
+    return sides, systems
+
+
+def test_batched_fp64_and_mixed_against_oracle(pp, torch):
+    sides, systems = make_systems(12, 6, 40, 3)
+    print("[batch] sides", sides, flush=True)
+    B = pp.BatchedSystems(systems)
+    bs = [ppo.spmv(A, ppo.uniform(len(A[0]) - 1, 50 + s)) for s, A in enumerate(systems)]
+    b = torch.tensor(np.concatenate(bs), device="cuda")
+    x64 = torch.zeros_like(b)
+    its = B.solve(b, x64, precision="fp64", tol=1e-8)
+    assert B.report.converged and B.report.true_relres < 1e-8
+    xm = torch.zeros_like(b)
+    B.solve(b, xm, precision="mixed", tol=1e-8)
+    assert B.report.converged and B.report.true_relres < 1e-8
+    off = B.row_off
+    for s, A in enumerate(systems):
+        o = ppo.cg(A, bs[s], ppo.Prec(A, "ic0"), tol=1e-8, maxit=5000)
+        xs = x64[off[s]:off[s + 1]].cpu().numpy()
+        assert abs(int(its[s]) - o["iters"]) <= 2, (s, its[s], o["iters"])
+        assert np.linalg.norm(xs - o["x"]) / np.linalg.norm(o["x"]) <= 1e-6
+        xms = xm[off[s]:off[s + 1]].cpu().numpy()
+        assert np.linalg.norm(xms - o["x"]) / np.linalg.norm(o["x"]) <= 1e-6
+
+
+def test_batched_autotune_and_graph_replays(pp, torch):
+    _, systems = make_systems(40, 8, 64, 4)
+    B = pp.BatchedSystems(systems)
+    bsz = B.tune()
+    assert bsz in (256, 512, 1024)
+    b = torch.tensor(np.concatenate([ppo.uniform(len(A[0]) - 1, 7) for A in systems]), device="cuda")
+    x1 = torch.zeros_like(b)
+    B.solve(b, x1, precision="mixed")
+    for _ in range(3):  # replays give the same answer
+        x2 = torch.zeros_like(b)
+        B.solve(b, x2, precision="mixed")
+        assert torch.equal(x1, x2)
+    assert B.report.true_relres < 1e-8
+
+
+def test_batched_gradients(pp, torch):
+    from prispack_amd.autograd import batched_solve
+    sides, systems = make_systems(3, 5, 8, 9)
+    B = pp.BatchedSystems(systems)
+    n = B.n
+    b0 = ppo.uniform(n, 11)
+    xt = ppo.uniform(n, 12)
+    vals0 = B.v.copy()
+
+    def loss_np(vals, b, tol=1e-13):
+        B.set_values(vals)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        B.solve(torch.tensor(b, device="cuda"), x, precision="fp64", tol=tol)
+        x = x.cpu().numpy()
+        return float(np.sum((x - xt) ** 2)), x
+
+    values = torch.tensor(vals0, device="cuda", requires_grad=True)
+    bt = torch.tensor(b0, device="cuda", requires_grad=True)
+    x = batched_solve(values, bt, B, precision="fp64", tol=1e-13)
+    L = torch.sum((x - torch.tensor(xt, device="cuda")) ** 2)
+    L.backward()
+    gv, gb = values.grad.cpu().numpy(), bt.grad.cpu().numpy()
+
+    # FP64 CPU adjoint with the oracle: A^T lambda = 2 (x - x_t), per system
+    _, xs = loss_np(vals0, b0)
+    off, rp, ci = B.row_off, B.rp, B.ci
+    lam = np.zeros(n)
+    for s, A in enumerate(systems):
+        r0, r1 = off[s], off[s + 1]
+        g = 2 * (xs[r0:r1] - xt[r0:r1])
+        lam[r0:r1] = ppo.cg(ppo.transpose(A), g, ppo.Prec(A, "ic0"), tol=1e-13, maxit=5000)["x"]
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    gv_cpu = -lam[rows] * xs[ci]
+    np.testing.assert_allclose(gb, lam, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(gv, gv_cpu, rtol=1e-6, atol=1e-9)
+
+    # central finite differences, h = 1e-6: b entries, diagonal entries and
+    # symmetric off-diagonal pairs (the perturbed matrix stays SPD)
+    h = 1e-6
+    for i in (0, n // 2, n - 1):
+        e = np.zeros(n)
+        e[i] = h
+        fd = (loss_np(vals0, b0 + e)[0] - loss_np(vals0, b0 - e)[0]) / (2 * h)
+        assert abs(fd - gb[i]) <= 1e-5 * max(1.0, abs(gb[i])), (i, fd, gb[i])
+    diag = [k for k in range(len(ci)) if ci[k] == rows[k]]
+    off_pairs = []
+    for k in range(len(ci)):
+        if ci[k] > rows[k]:
+            kt = [q for q in range(rp[ci[k]], rp[ci[k] + 1]) if ci[q] == rows[k]][0]
+            off_pairs.append((k, kt))
+        if len(off_pairs) >= 3:
+            break
+    for ks in [[diag[0]], [diag[len(diag) // 2]], *[list(p) for p in off_pairs]]:
+        vp, vm = vals0.copy(), vals0.copy()
+        vp[ks] += h
+        vm[ks] -= h
+        fd = (loss_np(vp, b0)[0] - loss_np(vm, b0)[0]) / (2 * h)
+        an = sum(gv[k] for k in ks)
+        assert abs(fd - an) <= 1e-5 * max(1.0, abs(an)), (ks, fd, an)
+
+
+def test_sparse_solve_gradient(pp, torch):
+    from prispack_amd.autograd import sparse_solve
+    A = ppo.upwind(6, 5, 4, 20.0, 10.0, 5.0)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    b0, xt = ppo.uniform(n, 1), ppo.uniform(n, 2)
+    values = torch.tensor(A[2], device="cuda", requires_grad=True)
+    bt = torch.tensor(b0, device="cuda", requires_grad=True)
+    x = sparse_solve(values, bt, M, lambda m: pp.ILU0(m), gmres=True, tol=1e-12)
+    L = torch.sum((x - torch.tensor(xt, device="cuda")) ** 2)
+    L.backward()
+    # dense FP64 check of the adjoint formulas
+    D = np.zeros((n, n))
+    rows = np.repeat(np.arange(n), np.diff(A[0]))
+    D[rows, A[1]] = A[2]
+    xs = np.linalg.solve(D, b0)
+    lam = np.linalg.solve(D.T, 2 * (xs - xt))
+    np.testing.assert_allclose(bt.grad.cpu().numpy(), lam, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(values.grad.cpu().numpy(), -lam[rows] * xs[A[1]], rtol=1e-6, atol=1e-9)

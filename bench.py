@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--trsv", type=int, default=0, help="SpTRSV schedule: 0 auto, 1 levels, 2 sync-free")
+    ap.add_argument("--ordering", type=int, default=1,
+                    help="IC(0) row ordering: 1 multicolour (2 sweep levels, default), 0 natural (382 levels)")
     return ap.parse_args()
 
 
@@ -105,13 +107,13 @@ def main():
     b = d_b.cpu().numpy()
 
     t0 = time.perf_counter()
-    P = pp.IC0(M, make_f32=True, trsv=args.trsv)
+    P = pp.IC0(M, make_f32=True, trsv=args.trsv, ordering=args.ordering)
     P.setup()
     setup_s = time.perf_counter() - t0
     info = P.info()
     log(f"[bench] IC(0) setup {setup_s:.2f} s, levels {info.levels_lower}/{info.levels_upper}")
 
-    def solve(precision):
+    def solve(precision, P=P):
         cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision=precision)
         cg.opts.record_history = 0
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
@@ -157,6 +159,8 @@ def main():
     nnz_l = (nnz - n) // 2
     sweep_bytes = 12 * nnz_l + 4 * (n + 1) + 4 * n + 16 * n + 8 * n
     trsv_bytes = 2 * sweep_bytes
+    if args.ordering == 1:  # gather into and scatter out of the colour order: 4n index + 8n in + 8n out each
+        trsv_bytes += 2 * (4 * n + 16 * n)
     tv = torch.tensor(xt, device="cuda")
     ptv = C.c_void_p(tv.data_ptr())
     trsv_ms = kernel_time_ms(torch, stream, lambda: L.pp_precond_apply(ctx.h, P.h, 0, ptv, ptv),
@@ -168,6 +172,20 @@ def main():
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
     log(f"[bench] SpMV fp64 {spmv_ms * 1e3:.1f} us {gbs(spmv_bytes, spmv_ms):.0f} GB/s; fp32 {spmv32_ms * 1e3:.1f} us; "
         f"M^-1 apply fp64 {trsv_ms * 1e3:.1f} us {gbs(trsv_bytes, trsv_ms):.1f} GB/s; fp32 {trsv32_ms * 1e3:.1f} us")
+
+    # the other ordering, one solve of each precision, for the record
+    other = None
+    if args.ordering == 1:
+        Pn = pp.IC0(M, make_f32=True, trsv=args.trsv, ordering=0)
+        Pn.setup()
+        solve("fp64", Pn)
+        dn64, rn64, _ = solve("fp64", Pn)
+        dnmx, rnmx, _ = solve("mixed", Pn)
+        other = {"ordering": "natural (382 sweep levels)", "fp64_s": round(dn64, 4), "mixed_s": round(dnmx, 4),
+                 "iters_fp64": rn64.iters, "iters_mixed_inner": rnmx.inner_iters,
+                 "speedup_mixed_vs_fp64": round(dn64 / dnmx, 3)}
+        log(f"[bench] natural ordering: fp64 {dn64:.4f} s ({rn64.iters} it), mixed {dnmx:.4f} s")
+        del Pn
 
     cpu = None
     if not args.no_cpu:
@@ -189,7 +207,9 @@ def main():
         "ms_per_step": round(medmx * 1e3, 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32-inner/f64-outer", "data": "synthetic",
         "config": {"workload": f"IC(0)-PCG, 3D 7-point Poisson {g}^3 (n={n}, nnz={nnz}), b=A x_true, "
-                               "x_true~N(0,1) seed 0, rtol 1e-8", "precision": "mixed vs fp64"},
+                               "x_true~N(0,1) seed 0, rtol 1e-8", "precision": "mixed vs fp64",
+                   "ordering": "multicolour" if args.ordering == 1 else "natural"},
+        "other_ordering": other,
         "fp64_s": round(med64, 5), "speedup_mixed_vs_fp64": round(med64 / medmx, 3),
         "iters_fp64": rep64.iters, "iters_mixed_inner": repmx.inner_iters, "outer_mixed": repmx.outer_iters,
         "relres_fp64": rep64.true_relres, "relres_mixed": repmx.true_relres,

@@ -124,12 +124,20 @@ enum {
     PP_TRSV_BLOCK = 3      /* one workgroup, rows of a level held in LDS           */
 };
 
+/* Row ordering of the factor (no reference counterpart; the reference factors
+ * in the given order).  MULTICOLOR factors P A P^T with rows grouped by a
+ * greedy distance-1 colouring, so IC(0)/ILU(0) sweeps have at most as many
+ * dependency levels as colours (2 for 5/7-point stencils); the apply permutes
+ * in and out.  It changes the preconditioner (and the iteration count). */
+enum { PP_ORDER_NATURAL = 0, PP_ORDER_MULTICOLOR = 1 };
+
 typedef struct {
     int kind;            /* PP_PC_* */
     double tau;          /* drop tolerance (ICT/ILUT/ILDLT) */
     uint32_t lfil;       /* ILUT fill per row */
     int trsv;            /* PP_TRSV_* */
     int make_f32;        /* also keep an FP32 copy of the factor (mixed precision) */
+    int ordering;        /* PP_ORDER_* */
     /* AMSED/GMSLR */
     uint32_t levels;     /* multilevel depth */
     uint32_t ndom;       /* subdomains at the first level */
@@ -182,6 +190,16 @@ int pp_hfactor_get(const pp_hfactor *F, int which, uint32_t *nnz, uint32_t *rowp
 int pp_hfactor_info(const pp_hfactor *F, uint32_t *mnnz, uint32_t *fail_row, uint32_t *levels_lower,
                     uint32_t *levels_upper);
 void pp_hfactor_destroy(pp_hfactor *F);
+
+/* Multilevel (AMSED / GMSLR) introspection: number of levels, rows per level
+ * and the low-rank correction rank of each level (AMSEDPrecondition.h:417-424). */
+int pp_ml_info(const pp_precond *M, uint32_t *nlev, uint32_t *level_sizes, uint32_t *ranks, uint32_t cap);
+/* the hierarchical-interface-decomposition ordering: new position k holds row perm[k] */
+int pp_ml_permutation(const pp_precond *M, uint32_t *perm);
+/* W (n x r, col-major) = V (n x k, col-major) * Y (k x r, col-major), FP64 on the
+ * matrix cores (device pointers): the tall-skinny product of the low-rank
+ * correction (cublasDgemm at AMSEDPrecondition.cpp:1851). */
+int pp_gemm_tall_f64(pp_ctx *ctx, uint32_t n, uint32_t k, uint32_t r, const double *V, const double *Y, double *W);
 
 /* ---- solvers (BaseSolver::solve, include/Solver/BaseSolver.h:9-17;
  *      CG, source/Solver/CG.cpp:33-83; GMRES, source/Solver/GMRES.cpp:59-221) ---- */

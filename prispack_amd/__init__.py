@@ -109,15 +109,16 @@ def _popts(kind, tau=1e-3, lfil=20, make_f32=False, **extra):
     return o
 
 
-def host_factor(A, kind, tau=1e-3, lfil=20):
+def host_factor(A, kind, tau=1e-3, lfil=20, ordering=0):
     """setup() on the host only (no GPU): returns dict of factor arrays,
-    mnnz, fail_row, status and the SpTRSV level counts."""
+    mnnz, fail_row, status and the SpTRSV level counts.  ordering=1 factors
+    the multicolour-reordered matrix."""
     kinds = {"ict": PP_PC_ICT, "ilut": PP_PC_ILUT, "ildlt": PP_PC_ILDLT, "ic0": PP_PC_IC0, "ilu0": PP_PC_ILU0}
     rp, ci, v = _u32(A[0]), _u32(A[1]), _f64(A[2])
     if len(ci) == 0:
         ci, v = np.zeros(1, np.uint32), np.zeros(1)
     L = _capi.lib()
-    o = _popts(kinds[kind], tau, lfil)
+    o = _popts(kinds[kind], tau, lfil, ordering=ordering)
     h = C.c_void_p()
     st = L.pp_factor_host(C.byref(o), len(rp) - 1, _p(rp, C.c_uint32), _p(ci, C.c_uint32), _p(v, C.c_double),
                           C.byref(h))
@@ -286,6 +287,9 @@ class BasePrecondition:
         self.A = A if isinstance(A, CSRMatrix) else CSRMatrix(*A, ctx=ctx)
         self.ctx = self.A.ctx
         self._opts = _popts(self._kind, make_f32=make_f32, **opts)
+        steps = getattr(self, "_lanczos_steps", 0)
+        if steps:
+            self._opts.reserved[0] = steps
         h = C.c_void_p()
         self.ctx.check(_capi.lib().pp_precond_create(self.ctx.h, self.A.h, C.byref(self._opts), C.byref(h)))
         self.h = h
@@ -567,3 +571,27 @@ class BatchedSystems:
                 self.h = None
         except Exception:
             pass
+
+
+class AMSED(BasePrecondition):
+    """AMSEDPrecondition<Low,High> (include/Preconditioner/AMSEDPrecondition.h:75-533):
+    multilevel low-rank Schur correction for SPD matrices.  levels / ndom give
+    the hierarchical interface decomposition (ndom leaf subdomains, a power of
+    two), rank the low-rank correction per level; tolerance is the ILDLT/ICT
+    drop tolerance of the block factors (0 = exact blocks)."""
+    _kind, _type = _capi.PP_PC_AMSED, PreconditionAMSED
+
+    def __init__(self, A, levels=3, ndom=16, rank=32, tolerance=1e-3, lanczos_steps=0, exact=False, **kw):
+        self._lanczos_steps = lanczos_steps
+        super().__init__(A, tau=0.0 if exact else tolerance, levels=levels, ndom=ndom, rank=rank, **kw)
+
+    def info_levels(self):
+        nlev = C.c_uint32()
+        sizes, ranks = (C.c_uint32 * 32)(), (C.c_uint32 * 32)()
+        check(_capi.lib().pp_ml_info(self.h, C.byref(nlev), sizes, ranks, 32))
+        return list(sizes)[:nlev.value], list(ranks)[:nlev.value]
+
+    def permutation(self):
+        perm = np.zeros(self.A.n, np.uint32)
+        check(_capi.lib().pp_ml_permutation(self.h, _p(perm, C.c_uint32)))
+        return perm

@@ -18,6 +18,7 @@ PP_PC_NONE, PP_PC_ICT, PP_PC_ILUT, PP_PC_ILDLT, PP_PC_IC0, PP_PC_ILU0, PP_PC_AMS
 PP_TRSV_AUTO, PP_TRSV_LEVELS, PP_TRSV_SYNCFREE, PP_TRSV_BLOCK = range(4)
 PP_SOLVER_CG, PP_SOLVER_GMRES = 0, 1
 PP_FP64, PP_MIXED = 0, 1
+PP_ORDER_NATURAL, PP_ORDER_MULTICOLOR = 0, 1
 
 STATUS_NAMES = {0: "PP_OK", 1: "PP_ERR_INVALID", 2: "PP_ERR_HIP", 3: "PP_ERR_NOT_SPD", 4: "PP_ERR_ZERO_PIVOT",
                 5: "PP_ERR_ZERO_ROW", 6: "PP_ERR_NOT_READY", 7: "PP_ERR_BREAKDOWN", 8: "PP_ERR_NOT_FINITE",
@@ -32,8 +33,8 @@ class PrisPackError(RuntimeError):
 
 class PrecondOpts(C.Structure):
     _fields_ = [("kind", C.c_int), ("tau", C.c_double), ("lfil", C.c_uint32), ("trsv", C.c_int),
-                ("make_f32", C.c_int), ("levels", C.c_uint32), ("ndom", C.c_uint32), ("rank", C.c_uint32),
-                ("eps", C.c_double), ("reserved", C.c_uint32 * 8)]
+                ("make_f32", C.c_int), ("ordering", C.c_int), ("levels", C.c_uint32), ("ndom", C.c_uint32),
+                ("rank", C.c_uint32), ("eps", C.c_double), ("reserved", C.c_uint32 * 8)]
 
 
 class PrecondInfo(C.Structure):
@@ -103,6 +104,9 @@ SIGS = {
     "pp_hfactor_get": (C.c_int, [vp, C.c_int, u32p, u32p, u32p, f64p, f64p]),
     "pp_hfactor_info": (C.c_int, [vp, u32p, u32p, u32p, u32p]),
     "pp_hfactor_destroy": (None, [vp]),
+    "pp_ml_info": (C.c_int, [vp, u32p, u32p, u32p, u32]),
+    "pp_ml_permutation": (C.c_int, [vp, u32p]),
+    "pp_gemm_tall_f64": (C.c_int, [vp, u32, u32, u32, vp, vp, vp]),
     "pp_solve_opts_default": (None, [C.POINTER(SolveOpts), C.c_int]),
     "pp_solve": (C.c_int, [vp, vp, vp, C.POINTER(SolveOpts), vp, vp, C.POINTER(SolveReport)]),
     "pp_solve_host": (C.c_int, [vp, vp, vp, C.POINTER(SolveOpts), f64p, f64p, C.POINTER(SolveReport)]),

@@ -376,3 +376,63 @@ int factor_ilu0(const HostCsr &A, HostCsr &L, HostCsr &U, std::vector<double> &D
 }
 
 }  // namespace pp
+
+namespace pp {
+
+// Greedy distance-1 colouring of the symmetrised pattern in natural order;
+// rows sorted by (colour, index).  Every dependency of a zero-fill factor row
+// then lies in an earlier colour, so the sweeps have <= ncolors levels.
+std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors) {
+    const uint32_t n = A.n;
+    std::vector<std::vector<uint32_t>> nb(n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.ci[k] != i) {
+                nb[i].push_back(A.ci[k]);
+                nb[A.ci[k]].push_back(i);
+            }
+    std::vector<uint32_t> color(n, kNone), mark;
+    ncolors = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (mark.size() < ncolors + 1) mark.resize(ncolors + 1, kNone);
+        for (uint32_t j : nb[i])
+            if (color[j] != kNone) mark[color[j]] = i;
+        uint32_t c = 0;
+        while (c < ncolors && mark[c] == i) ++c;
+        color[i] = c;
+        ncolors = std::max(ncolors, c + 1);
+    }
+    std::vector<uint32_t> order(n);
+    std::vector<uint32_t> start(ncolors + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) start[color[i] + 1]++;
+    for (uint32_t c = 0; c < ncolors; ++c) start[c + 1] += start[c];
+    for (uint32_t i = 0; i < n; ++i) order[start[color[i]]++] = i;
+    return order;
+}
+
+// P A P^T: row k is original row order[k]; columns renumbered and sorted.
+HostCsr permute_symmetric(const HostCsr &A, const std::vector<uint32_t> &order) {
+    const uint32_t n = A.n;
+    std::vector<uint32_t> inv(n);
+    for (uint32_t k = 0; k < n; ++k) inv[order[k]] = k;
+    HostCsr B;
+    B.n = n;
+    B.rp.assign(n + 1, 0);
+    B.ci.reserve(A.ci.size());
+    B.v.reserve(A.v.size());
+    std::vector<std::pair<uint32_t, double>> row;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t i = order[k];
+        row.clear();
+        for (uint32_t q = A.rp[i]; q < A.rp[i + 1]; ++q) row.emplace_back(inv[A.ci[q]], A.v[q]);
+        std::sort(row.begin(), row.end());
+        for (auto &e : row) {
+            B.ci.push_back(e.first);
+            B.v.push_back(e.second);
+        }
+        B.rp[k + 1] = (uint32_t)B.ci.size();
+    }
+    return B;
+}
+
+}  // namespace pp

@@ -156,6 +156,11 @@ struct pp_precond {
     pp::DevSweep<float> low32, up32;
     bool have32 = false;
     int trsv = PP_TRSV_LEVELS;    // resolved SpTRSV schedule
+    std::vector<uint32_t> order;  // factor row k = original row order[k] (empty: natural)
+    uint32_t ncolors = 0;
+    uint32_t *dorder = nullptr;
+    double *xp64 = nullptr;       // permuted work vectors
+    float *xp32 = nullptr;
     double *tmp64 = nullptr;      // out-of-place buffer between the two sweeps
     float *tmp32 = nullptr;
     void *multilevel = nullptr;   // AMSED / GMSLR state
@@ -184,6 +189,10 @@ int factor_ilut(const HostCsr &A, uint32_t lfil, double tau, HostCsr &L, HostCsr
 int factor_ic0(const HostCsr &A, HostCsr &lt, uint32_t &fail_row);
 int factor_ilu0(const HostCsr &A, HostCsr &L, HostCsr &U, std::vector<double> &D, uint32_t &fail_row);
 HostCsr transpose(const HostCsr &A);
+std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors);
+HostCsr permute_symmetric(const HostCsr &A, const std::vector<uint32_t> &order);
+void launch_permute(pp_ctx *ctx, uint32_t n, const uint32_t *perm, int dtype, const void *src, void *dst,
+                    bool gather);
 
 // ---- level analysis + upload (pp_precond.cpp) ----
 template <class T>

@@ -590,3 +590,25 @@ void copy_slot(pp_ctx *ctx, int dst, int src) {
 }
 
 }  // namespace pp
+
+namespace pp {
+template <class T>
+__global__ void k_permute(uint32_t n, const uint32_t *__restrict__ perm, const T *__restrict__ src, T *__restrict__ dst,
+                          int gather) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        if (gather) dst[k] = src[perm[k]];
+        else dst[perm[k]] = src[k];
+    }
+}
+
+// gather: dst[k] = src[perm[k]]; scatter: dst[perm[k]] = src[k]
+void launch_permute(pp_ctx *ctx, uint32_t n, const uint32_t *perm, int dtype, const void *src, void *dst,
+                    bool gather) {
+    const unsigned g = grid_for(ctx, n);
+    if (dtype == PP_F64)
+        k_permute<double><<<g, kBlock, 0, ctx->stream>>>(n, perm, (const double *)src, (double *)dst, gather);
+    else
+        k_permute<float><<<g, kBlock, 0, ctx->stream>>>(n, perm, (const float *)src, (float *)dst, gather);
+    PP_HIP(hipGetLastError());
+}
+}  // namespace pp

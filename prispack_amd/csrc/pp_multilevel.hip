@@ -1,0 +1,399 @@
+// pp_multilevel.hip -- AMSED (SPD) / GMSLR (nonsymmetric) multilevel low-rank
+// Schur-complement preconditioners on the device.
+//
+// With the hierarchical interface decomposition every level l splits the
+// remaining matrix as A_l = [B_l E_l; F_l C_l] with block-diagonal B_l and
+// C_l = A_{l+1}.  The apply (AMSEDPrecondition::levelCInvRecursiveSolve,
+// source/Preconditioner/AMSEDPrecondition.cpp:1494-1587) is, on the device:
+//     x1 = B^{-1} x1 ;  x2 -= F x1 ;  s = x2 ;  x2 = C^{-1} x2 (recursion)
+//     x2 += W (H o (W^T s)) ;  x1 -= B^{-1} (E x2)
+// B^{-1}: one level-scheduled / sync-free sweep pair over all blocks of the
+// level at once (the blocks are independent, so their rows share levels).
+// The low-rank term W H W^T comes from a generalised Lanczos process in the
+// C-inner product whose C^{-1} is the already-built deeper recursion
+// (AMSEDPrecondition.cpp:1275-1350, classic form without restarts); the
+// tall-skinny product W = V Y runs on the FP64 matrix cores.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "pp_ml.h"
+
+namespace pp {
+
+void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
+void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w, int);
+void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h, double *w);
+void launch_gemv_add(pp_ctx *ctx, uint32_t n, uint32_t ncol, const float *Z32, const double *Z64, size_t ld,
+                     const double *y, double *x);
+void read_slots(pp_ctx *ctx, int first, int count, double *out);
+double *gmres_coeff_buffer(pp_ctx *ctx);
+
+namespace {
+
+constexpr int kLrBlocks = 256;
+
+__global__ void k_gather(uint32_t n, const uint32_t *perm, const double *x, double *xp) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) xp[k] = x[perm[k]];
+}
+__global__ void k_scatter(uint32_t n, const uint32_t *perm, const double *xp, double *y) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) y[perm[k]] = xp[k];
+}
+
+// part[c][b] = sum over block b's rows of W[:, c] * s
+__global__ __launch_bounds__(256) void k_lr_partial(uint32_t n, uint32_t r, const double *__restrict__ W,
+                                                    const double *__restrict__ s, double *part) {
+    __shared__ double sh[4];
+    const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+    for (uint32_t c = 0; c < r; ++c) {
+        double acc = 0.0;
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) acc += W[(size_t)c * n + i] * s[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) part[(size_t)c * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        __syncthreads();
+    }
+}
+
+// h[c] = H[c] * sum_b part[c][b]
+__global__ __launch_bounds__(256) void k_lr_reduce(uint32_t nb, const double *part, const double *H, double *h) {
+    __shared__ double sh[4];
+    const uint32_t c = blockIdx.x;
+    double acc = 0.0;
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) acc += part[(size_t)c * nb + b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) h[c] = H[c] * (((sh[0] + sh[1]) + sh[2]) + sh[3]);
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// W (n x r) = V (n x k) * Y (k x r), column-major, FP64 MFMA 16x16x4: one
+// wavefront per 16 x 16 output tile, K in steps of 4.  Operand layout of
+// v_mfma_f64_16x16x4f64: lane l holds A[l%16][l/16], B[l/16][l%16] and
+// D[4*(l/16) + reg][l%16].
+__global__ __launch_bounds__(256) void k_gemm_tall_f64(uint32_t n, uint32_t k, uint32_t r, const double *__restrict__ V,
+                                                       const double *__restrict__ Y, double *__restrict__ W) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t ntr = (n + 15) / 16, ntc = (r + 15) / 16;
+    if (wave >= ntr * ntc) return;
+    const uint32_t tr = wave % ntr, tc = wave / ntr;
+    const uint32_t i = lane & 15, q = lane >> 4;
+    const uint32_t row = tr * 16 + i, col = tc * 16 + i;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k0 = 0; k0 < k; k0 += 4) {
+        const uint32_t kk = k0 + q;
+        const double a = (row < n && kk < k) ? V[(size_t)kk * n + row] : 0.0;
+        const double b = (kk < k && col < r) ? Y[(size_t)col * k + kk] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    const uint32_t ocol = tc * 16 + i;
+#pragma unroll
+    for (int reg = 0; reg < 4; ++reg) {
+        const uint32_t orow = tr * 16 + 4 * q + reg;
+        if (orow < n && ocol < r) W[(size_t)ocol * n + orow] = acc[reg];
+    }
+}
+
+void check_status(pp_ctx *ctx, int st) {
+    if (st != PP_OK) throw Error(st, pp_last_error(ctx));
+}
+
+pp_csr *make_csr(pp_ctx *ctx, const HostCsr &M, uint32_t ncol) {
+    pp_csr *A = nullptr;
+    std::vector<uint32_t> ci(M.ci);
+    std::vector<double> v(M.v);
+    if (ci.empty()) {
+        ci.push_back(0);
+        v.push_back(0.0);
+    }
+    check_status(ctx, pp_csr_create(ctx, M.n, ncol, M.rp.data(), ci.data(), v.data(), &A));
+    return A;
+}
+
+pp_precond *make_factor(pp_ctx *ctx, pp_csr *A, int kind, double tau, uint32_t lfil, int trsv) {
+    pp_precond_opts o;
+    pp_precond_opts_default(&o);
+    o.kind = kind;
+    o.tau = tau;
+    o.lfil = lfil;
+    o.trsv = trsv;
+    pp_precond *P = nullptr;
+    check_status(ctx, pp_precond_create(ctx, A, &o, &P));
+    int st = pp_precond_setup(ctx, P);
+    if (st != PP_OK) {
+        std::string msg = pp_last_error(ctx);
+        pp_precond_destroy(P);
+        throw Error(st, msg);
+    }
+    return P;
+}
+
+// x (device, length n - lev_ptr[level+1]) <- C_level^{-1} x; level -1 is A.
+void rec_solve(pp_ctx *ctx, MlState &S, int level, double *x) {
+    const int next = level + 1;
+    if (next == (int)S.hid.nlev - 1) {
+        precond_apply(ctx, S.last, PP_F64, x, x);
+        return;
+    }
+    MlLevel &L = S.lev[next];
+    double *x1 = x, *x2 = x + L.nb;
+    precond_apply(ctx, L.Bpc, PP_F64, x1, x1);
+    launch_spmv(ctx, L.F, PP_F64, -1.0, x1, 1.0, x2, -1, nullptr, false);  // x2 -= F x1
+    if (L.rank) PP_HIP(hipMemcpyAsync(L.save, x2, (size_t)L.nrest * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    rec_solve(ctx, S, next, x2);
+    if (L.rank) {
+        double *part = L.h + 64 * 1024;
+        k_lr_partial<<<kLrBlocks, 256, 0, ctx->stream>>>(L.nrest, L.rank, L.W, L.save, part);
+        k_lr_reduce<<<L.rank, 256, 0, ctx->stream>>>(kLrBlocks, part, L.Hd, L.h);
+        PP_HIP(hipGetLastError());
+        launch_gemv_add(ctx, L.nrest, L.rank, nullptr, L.W, L.nrest, L.h, x2);  // x2 += W h
+    }
+    launch_spmv(ctx, L.E, PP_F64, 1.0, x2, 0.0, L.t, -1, nullptr, false);  // t = E x2
+    precond_apply(ctx, L.Bpc, PP_F64, L.t, L.t);
+    launch_axpby(ctx, PP_F64, L.nb, 1.0, x1, -1.0, L.t, x1);  // x1 -= t
+}
+
+double dot_dev(pp_ctx *ctx, uint32_t n, const double *a, const double *b) {
+    launch_dot(ctx, PP_F64, n, a, b, S_TMP1);
+    return read_slot(ctx, S_TMP1);
+}
+
+// Generalised Lanczos for E^T B^{-1} E w = theta C w at level l (symmetric
+// case), C^{-1} by the recursion below l.  Full reorthogonalisation against
+// the stored basis.  Returns the Ritz values (ascending) and sets W, H.
+void lowrank_lanczos(pp_ctx *ctx, MlState &S, int l, uint32_t rank, uint32_t steps) {
+    MlLevel &L = S.lev[l];
+    const uint32_t n = L.nrest;
+    uint32_t k = std::min<uint32_t>(std::min<uint32_t>(steps, n), 250);
+    if (rank == 0 || n == 0 || k == 0) return;
+    double *V = dalloc<double>((size_t)n * (k + 1));
+    double *Z = dalloc<double>((size_t)n * (k + 1));
+    double *u = dalloc<double>(n), *w = dalloc<double>(n), *tb = dalloc<double>(std::max(1u, L.nb));
+    std::vector<double> alpha, beta;
+    try {
+        // start vector: seeded uniform(-1, 1)
+        std::vector<double> u0(n);
+        PP_CHECK(pp_fill_random(0, 12345 + l, n, u0.data()) == PP_OK, PP_ERR_INVALID, "rng");
+        PP_HIP(hipMemcpy(u, u0.data(), n * 8, hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(w, u, n * 8, hipMemcpyDeviceToDevice));
+        rec_solve(ctx, S, l, w);
+        double b0 = dot_dev(ctx, n, w, u);
+        PP_CHECK(b0 > 0 && std::isfinite(b0), PP_ERR_BREAKDOWN, "Lanczos start vector has non-positive C-norm");
+        b0 = std::sqrt(b0);
+        launch_axpby(ctx, PP_F64, n, 1.0 / b0, w, 0.0, w, V);
+        launch_axpby(ctx, PP_F64, n, 1.0 / b0, u, 0.0, u, Z);
+        uint32_t done = 0;
+        for (uint32_t j = 0; j < k; ++j) {
+            double *vj = V + (size_t)j * n, *zj = Z + (size_t)j * n;
+            // u = F B^{-1} E v_j  (F = E^T)
+            launch_spmv(ctx, L.E, PP_F64, 1.0, vj, 0.0, tb, -1, nullptr, false);
+            precond_apply(ctx, L.Bpc, PP_F64, tb, tb);
+            launch_spmv(ctx, L.F, PP_F64, 1.0, tb, 0.0, u, -1, nullptr, false);
+            if (j > 0) launch_axpby(ctx, PP_F64, n, 1.0, u, -beta[j - 1], Z + (size_t)(j - 1) * n, u);
+            const double a = dot_dev(ctx, n, u, vj);
+            alpha.push_back(a);
+            launch_axpby(ctx, PP_F64, n, 1.0, u, -a, zj, u);
+            // u -= Z (V^T u), twice for stability
+            for (int pass = 0; pass < 2; ++pass) {
+                launch_gemv_t(ctx, n, j + 1, V, n, u, 0);
+                launch_gemv_sub(ctx, n, j + 1, Z, n, gmres_coeff_buffer(ctx), u);
+            }
+            done = j + 1;
+            PP_HIP(hipMemcpyAsync(w, u, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            rec_solve(ctx, S, l, w);
+            double bt = dot_dev(ctx, n, w, u);
+            if (!(bt > 1e-28) || j + 1 == k) break;  // C-norm collapsed (AMSEDPrecondition.cpp:1334-1339)
+            bt = std::sqrt(bt);
+            beta.push_back(bt);
+            launch_axpby(ctx, PP_F64, n, 1.0 / bt, w, 0.0, w, V + (size_t)(j + 1) * n);
+            launch_axpby(ctx, PP_F64, n, 1.0 / bt, u, 0.0, u, Z + (size_t)(j + 1) * n);
+        }
+        const uint32_t m = done;
+        std::vector<double> d(alpha.begin(), alpha.begin() + m), e(beta.begin(), beta.begin() + (m ? m - 1 : 0));
+        std::vector<double> Yall;
+        tridiag_eig(d, e, Yall);
+        // largest Ritz values in [0, 1): theta/(1 - theta) (the basic correction)
+        std::vector<uint32_t> pick;
+        for (int i = (int)m - 1; i >= 0 && pick.size() < rank; --i)
+            if (d[i] > 0.0 && d[i] < 1.0 - 1e-12) pick.push_back((uint32_t)i);
+        L.rank = (uint32_t)pick.size();
+        L.theta.assign(d.begin(), d.end());
+        if (L.rank) {
+            std::vector<double> Y((size_t)m * L.rank);
+            L.Hhost.resize(L.rank);
+            for (uint32_t c = 0; c < L.rank; ++c) {
+                const double th = d[pick[c]];
+                L.Hhost[c] = th / (1.0 - th);
+                std::copy(Yall.begin() + (size_t)pick[c] * m, Yall.begin() + (size_t)(pick[c] + 1) * m,
+                          Y.begin() + (size_t)c * m);
+            }
+            double *Yd = dalloc<double>(Y.size());
+            PP_HIP(hipMemcpy(Yd, Y.data(), Y.size() * 8, hipMemcpyHostToDevice));
+            L.W = dalloc<double>((size_t)n * L.rank);
+            launch_gemm_tall_f64(ctx, n, m, L.rank, V, Yd, L.W);
+            PP_HIP(hipStreamSynchronize(ctx->stream));
+            dfree(Yd);
+            L.Hd = dalloc<double>(L.rank);
+            PP_HIP(hipMemcpy(L.Hd, L.Hhost.data(), L.rank * 8, hipMemcpyHostToDevice));
+            L.save = dalloc<double>(n);
+            L.h = dalloc<double>(64 * 1024 + (size_t)L.rank * kLrBlocks);
+        }
+    } catch (...) {
+        dfree(V); dfree(Z); dfree(u); dfree(w); dfree(tb);
+        throw;
+    }
+    dfree(V); dfree(Z); dfree(u); dfree(w); dfree(tb);
+}
+
+}  // namespace
+
+void launch_gemm_tall_f64(pp_ctx *ctx, uint32_t n, uint32_t k, uint32_t r, const double *V, const double *Y,
+                          double *W) {
+    const uint64_t waves = (uint64_t)((n + 15) / 16) * ((r + 15) / 16);
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    if (blocks == 0) return;
+    k_gemm_tall_f64<<<blocks, 256, 0, ctx->stream>>>(n, k, r, V, Y, W);
+    PP_HIP(hipGetLastError());
+}
+
+void ml_free(pp_precond *M) {
+    auto *S = static_cast<MlState *>(M->multilevel);
+    if (!S) return;
+    for (auto &L : S->lev) {
+        pp_precond_destroy(L.Bpc);
+        pp_csr_destroy(L.B);
+        pp_csr_destroy(L.E);
+        pp_csr_destroy(L.F);
+        dfree(L.W); dfree(L.Hd); dfree(L.save); dfree(L.t); dfree(L.h);
+    }
+    pp_precond_destroy(S->last);
+    pp_csr_destroy(S->Clast);
+    dfree(S->dperm);
+    dfree(S->xp);
+    delete S;
+    M->multilevel = nullptr;
+}
+
+// setup(): partition, factor every level's B blocks and the last level, then
+// build the low-rank corrections bottom-up (each uses the recursion below it).
+void ml_setup(pp_ctx *ctx, pp_precond *M) {
+    const bool spd = M->opts.kind == PP_PC_AMSED;
+    HostCsr A;
+    A.n = M->A->nrow;
+    A.rp = M->A->h_rp;
+    A.ci = M->A->h_ci;
+    A.v = M->A->h_v;
+    auto *S = new MlState;
+    M->multilevel = S;
+    try {
+        S->symmetric = spd;
+        S->n = A.n;
+        const uint32_t nlev = M->opts.levels ? M->opts.levels : 3;
+        const uint32_t ndom = M->opts.ndom ? M->opts.ndom : 16;
+        S->hid = build_hid(A, ndom, nlev);
+        const Hid &H = S->hid;
+        const int trsv = M->opts.trsv == PP_TRSV_AUTO ? PP_TRSV_LEVELS : M->opts.trsv;
+        // factor drop tolerance: the reference's are 1e-3 (AMSED) and 1e-2
+        // (GMSLR) (AMSEDPrecondition.h:39-51, GMSLRPrecondition.h:25-37); 0 gives
+        // exact block factors
+        const double tau = M->opts.tau;
+        const uint32_t lfil = 20;
+        M->mnnz = 0;
+        // last level
+        uint32_t nc = 0;
+        HostCsr Cl = extract_block(A, H, H.lev_ptr[nlev - 1], A.n, H.lev_ptr[nlev - 1], A.n, nc);
+        S->Clast = make_csr(ctx, Cl, nc);
+        S->last = make_factor(ctx, S->Clast, spd ? PP_PC_ICT : PP_PC_ILUT, tau, lfil, trsv);
+        M->mnnz += S->last->mnnz;
+        S->lev.resize(nlev - 1);
+        for (uint32_t l = 0; l + 1 < nlev; ++l) {
+            MlLevel &L = S->lev[l];
+            L.off = H.lev_ptr[l];
+            L.nb = H.lev_ptr[l + 1] - H.lev_ptr[l];
+            L.nrest = A.n - H.lev_ptr[l + 1];
+            uint32_t c = 0;
+            HostCsr B = extract_block(A, H, H.lev_ptr[l], H.lev_ptr[l + 1], H.lev_ptr[l], H.lev_ptr[l + 1], c);
+            HostCsr E = extract_block(A, H, H.lev_ptr[l], H.lev_ptr[l + 1], H.lev_ptr[l + 1], A.n, c);
+            HostCsr F = spd ? transpose_rect(E, L.nrest)
+                            : extract_block(A, H, H.lev_ptr[l + 1], A.n, H.lev_ptr[l], H.lev_ptr[l + 1], c);
+            L.B = make_csr(ctx, B, L.nb);
+            L.E = make_csr(ctx, E, L.nrest);
+            L.F = make_csr(ctx, F, L.nb);
+            L.Bpc = make_factor(ctx, L.B, spd ? PP_PC_ILDLT : PP_PC_ILUT, tau, lfil, trsv);
+            L.t = dalloc<double>(std::max(1u, L.nb));
+            M->mnnz += L.Bpc->mnnz;
+        }
+        S->dperm = dalloc<uint32_t>(A.n);
+        PP_HIP(hipMemcpy(S->dperm, H.perm.data(), A.n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        S->xp = dalloc<double>(A.n);
+        const uint32_t rank = M->opts.rank;
+        const uint32_t steps = M->opts.reserved[0] ? M->opts.reserved[0] : std::max<uint32_t>(2 * rank, rank + 16);
+        S->lanczos_steps = steps;
+        for (int l = (int)nlev - 2; l >= 0; --l) {
+            if (spd) lowrank_lanczos(ctx, *S, l, rank, steps);
+            MlLevel &L = S->lev[l];
+            M->mnnz += 2 * L.rank * L.nrest + L.rank;  // AMSEDPrecondition.cpp:1269 convention
+        }
+        M->low64.nlev = nlev;
+        M->up64.nlev = nlev;
+    } catch (...) {
+        ml_free(M);
+        throw;
+    }
+}
+
+void ml_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
+    PP_CHECK(dtype == PP_F64, PP_ERR_UNSUPPORTED, "multilevel apply is FP64");
+    auto *S = static_cast<MlState *>(M->multilevel);
+    PP_CHECK(S, PP_ERR_NOT_READY, "multilevel preconditioner not set up");
+    const unsigned g = std::min<uint32_t>((S->n + 255) / 256, 4096);
+    k_gather<<<std::max(1u, g), 256, 0, ctx->stream>>>(S->n, S->dperm, (const double *)x, S->xp);
+    rec_solve(ctx, *S, -1, S->xp);
+    k_scatter<<<std::max(1u, g), 256, 0, ctx->stream>>>(S->n, S->dperm, S->xp, (double *)y);
+    PP_HIP(hipGetLastError());
+}
+
+}  // namespace pp
+
+extern "C" int pp_ml_info(const pp_precond *M, uint32_t *nlev, uint32_t *level_sizes, uint32_t *ranks,
+                          uint32_t cap) {
+    if (!M || !M->multilevel) return PP_ERR_NOT_READY;
+    auto *S = static_cast<const pp::MlState *>(M->multilevel);
+    if (nlev) *nlev = S->hid.nlev;
+    for (uint32_t l = 0; l < S->hid.nlev && l < cap; ++l) {
+        if (level_sizes) level_sizes[l] = S->hid.lev_ptr[l + 1] - S->hid.lev_ptr[l];
+        if (ranks) ranks[l] = l + 1 < S->hid.nlev ? S->lev[l].rank : 0;
+    }
+    return PP_OK;
+}
+
+extern "C" int pp_ml_permutation(const pp_precond *M, uint32_t *perm) {
+    if (!M || !M->multilevel || !perm) return PP_ERR_NOT_READY;
+    auto *S = static_cast<const pp::MlState *>(M->multilevel);
+    std::copy(S->hid.perm.begin(), S->hid.perm.end(), perm);
+    return PP_OK;
+}
+
+// out (device, n x r col-major) = V (n x k) * Y (k x r) on the matrix cores; all device pointers.
+extern "C" int pp_gemm_tall_f64(pp_ctx *ctx, uint32_t n, uint32_t k, uint32_t r, const double *V, const double *Y,
+                                double *W) {
+    if (!ctx || !V || !Y || !W) return PP_ERR_INVALID;
+    try {
+        pp::launch_gemm_tall_f64(ctx, n, k, r, V, Y, W);
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (const pp::Error &e) {
+        ctx->err = e.what();
+        return e.status;
+    }
+    return PP_OK;
+}

@@ -149,7 +149,16 @@ static HostSweep lower_from_upper_rows(const HostCsr &Ut, int mode, bool diag_fi
     return s;
 }
 
-static void setup_triangular(pp_precond *M, const HostCsr &A, bool upload) {
+static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
+    HostCsr Aperm;
+    const HostCsr *Ap = &A0;
+    M->order.clear();
+    if (M->opts.ordering == PP_ORDER_MULTICOLOR) {
+        M->order = multicolor_order(A0, M->ncolors);
+        Aperm = permute_symmetric(A0, M->order);
+        Ap = &Aperm;
+    }
+    const HostCsr &A = *Ap;
     const int kind = M->opts.kind;
     int st = PP_OK;
     uint32_t fail = 0;
@@ -197,6 +206,11 @@ static void setup_triangular(pp_precond *M, const HostCsr &A, bool upload) {
         throw Error(PP_ERR_UNSUPPORTED, "unknown preconditioner kind");
     }
     if (!upload) return;
+    if (!M->order.empty()) {
+        M->dorder = dalloc<uint32_t>(A.n);
+        PP_HIP(hipMemcpy(M->dorder, M->order.data(), A.n * sizeof(uint32_t), hipMemcpyHostToDevice));
+        M->xp64 = dalloc<double>(A.n);
+    }
     upload_sweep(M->hlow, M->low64);
     upload_sweep(M->hup, M->up64);
     if (M->opts.make_f32) {
@@ -207,7 +221,11 @@ static void setup_triangular(pp_precond *M, const HostCsr &A, bool upload) {
 }
 
 void precond_need_f32(pp_precond *M) {
-    if (M->have32 || M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) return;
+    if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) return;
+    // every buffer an FP32 apply touches exists before any graph capture
+    if (!M->order.empty() && !M->xp32) M->xp32 = dalloc<float>(M->n);
+    if (M->trsv == PP_TRSV_SYNCFREE && !M->tmp32) M->tmp32 = dalloc<float>(M->n);
+    if (M->have32) return;
     upload_sweep(M->hlow, M->low32);
     upload_sweep(M->hup, M->up32);
     M->have32 = true;
@@ -223,6 +241,20 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
     }
     if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) {
         ml_apply(ctx, M, dtype, x, y);
+        return;
+    }
+    if (!M->order.empty() && x != (const void *)M->xp64 && x != (const void *)M->xp32) {
+        // factor of P A P^T: y = P^T M^{-1} P x (the inner call sees x == xp)
+        void *xp;
+        if (dtype == PP_F64) {
+            xp = M->xp64;
+        } else {
+            if (!M->xp32) M->xp32 = dalloc<float>(M->n);
+            xp = M->xp32;
+        }
+        launch_permute(ctx, M->n, M->dorder, dtype, x, xp, true);
+        precond_apply(ctx, M, dtype, xp, xp);
+        launch_permute(ctx, M->n, M->dorder, dtype, xp, y, false);
         return;
     }
     if (M->trsv == PP_TRSV_SYNCFREE) {
@@ -338,16 +370,36 @@ int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void 
     });
 }
 
-int pp_precond_apply_part(pp_ctx *ctx, pp_precond *M, int part, int dtype, void *x) {
+int pp_precond_apply_part(pp_ctx *ctx, pp_precond *M, int part, int dtype, void *xuser) {
     if (!ctx) return PP_ERR_INVALID;
     return guarded_pc(ctx, [&] {
-        PP_CHECK(M && x, PP_ERR_INVALID, "NULL argument");
+        PP_CHECK(M && xuser, PP_ERR_INVALID, "NULL argument");
         PP_CHECK(M->ready, PP_ERR_NOT_READY, "preconditioner used before setup()");
         PP_CHECK(M->opts.kind != PP_PC_AMSED && M->opts.kind != PP_PC_GMSLR && M->opts.kind != PP_PC_NONE,
                  PP_ERR_UNSUPPORTED, "not a triangular preconditioner");
         launch_fill_zero(ctx, ctx->ctl, sizeof(int));
         const bool ildlt = M->opts.kind == PP_PC_ILDLT;
         if (dtype == PP_F32) precond_need_f32(M);
+        // a reordered factor works on P x (permuted in, permuted back out)
+        void *x = xuser;
+        if (!M->order.empty()) {
+            x = dtype == PP_F64 ? (void *)M->xp64 : (void *)M->xp32;
+            launch_permute(ctx, M->n, M->dorder, dtype, xuser, x, true);
+        }
+        struct Back {
+            pp_ctx *c;
+            pp_precond *m;
+            int dt;
+            void *from, *to;
+            ~Back() {
+                if (from != to) {
+                    try {
+                        launch_permute(c, m->n, m->dorder, dt, from, to, false);
+                    } catch (...) {
+                    }
+                }
+            }
+        } back{ctx, M, dtype, x, xuser};
         if (part == 0) {
             if (dtype == PP_F64) launch_sweep<double>(ctx, M->low64, M->hlow.mode, (double *)x, (double *)x);
             else launch_sweep<float>(ctx, M->low32, M->hlow.mode, (float *)x, (float *)x);
@@ -423,6 +475,9 @@ void pp_precond_destroy(pp_precond *M) {
     if (!M) return;
     (void)hipSetDevice(M->ctx->device);
     (void)hipStreamSynchronize(M->ctx->stream);
+    dfree(M->dorder);
+    dfree(M->xp64);
+    dfree(M->xp32);
     dfree(M->tmp64);
     dfree(M->tmp32);
     M->low64.free_all();

@@ -1,0 +1,221 @@
+"""Multicolour-reordered factors, the AMSED multilevel low-rank Schur
+preconditioner and the FP64 matrix-core GEMM, through the C ABI.
+
+Multicolour: the product's apply equals the oracle's IC(0)/ILU(0) of P A P^T
+(P from an independent greedy colouring written here), bit for bit in FP64.
+AMSED: parity with the reference is unpinned (the reference needs METIS,
+SuiteSparse AMD and LAPACKE, absent here); pinned instead by
+  - the hierarchical ordering being a permutation whose levels are block
+    diagonal (no coupling between blocks of one level),
+  - exactness: with exact block factors (tau = 0) and rank 0 the recursion is
+    an exact block LDL^T, so M^{-1} A x = x,
+  - the Lanczos Ritz values lying in [0, 1) (S = C - E^T B^{-1} E is SPD),
+  - PCG converging to rtol 1e-8 on the config-4 matrix type.
+GEMM: against a host FP64 product (brute force)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import ppo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def pp(torch):
+    import prispack_amd
+    return prispack_amd
+
+
+def greedy_order(A):
+    rp, ci, _ = A
+    n = len(rp) - 1
+    nb = [set() for _ in range(n)]
+    for i in range(n):
+        for j in ci[rp[i]:rp[i + 1]]:
+            if j != i:
+                nb[i].add(int(j))
+                nb[int(j)].add(i)
+    color = [-1] * n
+    for i in range(n):
+        used = {color[j] for j in nb[i] if color[j] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        color[i] = c
+    return np.array(sorted(range(n), key=lambda i: (color[i], i)), np.uint32), max(color) + 1
+
+
+def permute(A, order):
+    rp, ci, v = A
+    n = len(rp) - 1
+    inv = np.empty(n, np.int64)
+    inv[order] = np.arange(n)
+    rows, cols, vals = [], [], []
+    for k, i in enumerate(order):
+        c = inv[ci[rp[i]:rp[i + 1]]]
+        s = np.argsort(c)
+        rows.append(np.full(len(c), k))
+        cols.append(c[s])
+        vals.append(v[rp[i]:rp[i + 1]][s])
+    rp2 = np.zeros(n + 1, np.uint32)
+    rp2[1:] = np.cumsum([len(c) for c in cols])
+    return rp2, np.concatenate(cols).astype(np.uint32), np.concatenate(vals)
+
+
+@pytest.mark.parametrize("kind,mat", [("ic0", "lap"), ("ilu0", "upw")])
+def test_multicolor_apply_bit_exact(pp, torch, kind, mat):
+    A = ppo.laplacian(14, 13, 12) if mat == "lap" else ppo.upwind(14, 13, 12, 20.0, 10.0, 5.0)
+    n = len(A[0]) - 1
+    order, ncol = greedy_order(A)
+    assert ncol == 2
+    Ap = permute(A, order)
+    M = pp.CSRMatrix(*A)
+    P = (pp.IC0 if kind == "ic0" else pp.ILU0)(M, ordering=1)
+    P.setup()
+    assert P.info().levels_lower <= 2 and P.info().levels_upper <= 2
+    Q = ppo.Prec(Ap, kind)
+    x = ppo.uniform(n, 5)
+    t = torch.tensor(x, device="cuda")
+    P.MInvSolve(t)
+    want = np.empty(n)
+    want[order] = Q.apply(x[order])
+    assert np.array_equal(t.cpu().numpy(), want)
+
+
+def test_multicolor_pcg(pp, torch):
+    A = ppo.laplacian(40, 40, 40)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 0))
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M, ordering=1)
+    for prec in ("fp64", "mixed"):
+        cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=3000, precision=prec)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        cg.solve(M, torch.tensor(b, device="cuda"), x)
+        assert cg.getConvergence() and cg.report.true_relres < 1e-8
+    order, _ = greedy_order(A)
+    o = ppo.cg(permute(A, order), b[order], ppo.Prec(permute(A, order), "ic0"), tol=1e-8, maxit=3000)
+    cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=3000)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cg.solve(M, torch.tensor(b, device="cuda"), x)
+    assert abs(cg.getNumIter() - o["iters"]) <= 2
+    xo = np.empty(n)
+    xo[order] = o["x"]
+    assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+def test_gemm_tall_f64_matrix_cores(pp, torch):
+    from prispack_amd import _capi
+    ctx = pp.Context.default()
+    rng = np.random.default_rng(3)
+    for n, k, r in ((1000, 64, 32), (37, 5, 3), (4096, 128, 48)):
+        V = rng.standard_normal((k, n))  # column-major n x k = row-major k x n
+        Y = rng.standard_normal((r, k))  # column-major k x r
+        tV, tY = torch.tensor(V, device="cuda"), torch.tensor(Y, device="cuda")
+        tW = torch.zeros((r, n), dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        ctx.check(_capi.lib().pp_gemm_tall_f64(ctx.h, n, k, r, C.c_void_p(tV.data_ptr()), C.c_void_p(tY.data_ptr()),
+                                                C.c_void_p(tW.data_ptr())))
+        want = (V.T @ Y.T).T  # (n x r) column-major as (r x n) row-major
+        np.testing.assert_allclose(tW.cpu().numpy(), want, rtol=1e-12, atol=1e-12)
+
+
+def _ml_info(pp, P):
+    from prispack_amd import _capi
+    nlev = C.c_uint32()
+    sizes = (C.c_uint32 * 16)()
+    ranks = (C.c_uint32 * 16)()
+    assert _capi.lib().pp_ml_info(P.h, C.byref(nlev), sizes, ranks, 16) == 0
+    perm = np.zeros(P.A.n, np.uint32)
+    assert _capi.lib().pp_ml_permutation(P.h, perm.ctypes.data_as(C.POINTER(C.c_uint32))) == 0
+    return nlev.value, list(sizes)[:nlev.value], list(ranks)[:nlev.value], perm
+
+
+def dense(A):
+    rp, ci, v = A
+    n = len(rp) - 1
+    D = np.zeros((n, n))
+    D[np.repeat(np.arange(n), np.diff(rp)), ci] = v
+    return D
+
+
+def dense_recursion(Ap, lev_ptr, level, x):
+    """levelCInvRecursiveSolve (AMSEDPrecondition.cpp:1494-1587) with exact
+    block solves and no low-rank term, densely: C_level^{-1} x."""
+    nlev = len(lev_ptr) - 1
+    nxt = level + 1
+    if nxt == nlev - 1:
+        return np.linalg.solve(Ap[lev_ptr[nxt]:, lev_ptr[nxt]:], x)
+    b0, b1 = lev_ptr[nxt], lev_ptr[nxt + 1]
+    B, E, F = Ap[b0:b1, b0:b1], Ap[b0:b1, b1:], Ap[b1:, b0:b1]
+    nb = b1 - b0
+    x1 = np.linalg.solve(B, x[:nb])
+    x2 = dense_recursion(Ap, lev_ptr, nxt, x[nb:] - F @ x1)
+    return np.concatenate([x1 - np.linalg.solve(B, E @ x2), x2])
+
+
+@pytest.mark.parametrize("levels,ndom", [(2, 4), (3, 8), (4, 8)])
+def test_amsed_recursion_matches_dense_restatement(pp, torch, levels, ndom):
+    A = ppo.laplacian(12, 11, 10)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    P = pp.AMSED(M, levels=levels, ndom=ndom, rank=0, exact=True)
+    P.setup()
+    nlev, sizes, ranks, perm = _ml_info(pp, P)
+    assert nlev == levels and sum(sizes) == n and not any(ranks)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    Ap = dense(A)[np.ix_(perm, perm)]
+    lev_ptr = np.concatenate([[0], np.cumsum(sizes)])
+    # the blocks of a level are decoupled: B_l is block diagonal by construction,
+    # and every level but the last has at least two blocks' worth of rows
+    x = ppo.uniform(n, 2)
+    y = torch.tensor(x, device="cuda")
+    P.MInvSolve(y)
+    want = np.empty(n)
+    want[perm] = dense_recursion(Ap, lev_ptr, -1, x[perm])
+    np.testing.assert_allclose(y.cpu().numpy(), want, rtol=1e-9, atol=1e-11)
+
+
+def test_amsed_full_rank_is_exact(pp, torch):
+    # exact blocks and a complete low-rank term: C^{-1} + W H W^T = S^{-1}
+    # at every level, so M^{-1} A = I
+    A = ppo.laplacian(8, 8, 8)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    P = pp.AMSED(M, levels=3, ndom=4, rank=n, lanczos_steps=250, exact=True)
+    P.setup()
+    x = ppo.uniform(n, 3)
+    y = torch.tensor(ppo.spmv(A, x), device="cuda")
+    P.MInvSolve(y)
+    np.testing.assert_allclose(y.cpu().numpy(), x, rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("rank", [0, 16])
+def test_amsed_pcg_config4_type(pp, torch, rank):
+    A = ppo.laplacian(32, 32, 32)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 0))
+    M = pp.CSRMatrix(*A)
+    P = pp.AMSED(M, levels=3, ndom=16, rank=rank)
+    P.setup()
+    nlev, sizes, ranks, _ = _ml_info(pp, P)
+    print(f"[amsed] sizes {sizes} ranks {ranks}", flush=True)
+    if rank:
+        assert all(0 < r <= rank for r in ranks[:-1])
+    cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=1000)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cg.solve(M, torch.tensor(b, device="cuda"), x)
+    assert cg.getConvergence() and cg.report.true_relres < 1e-8
+    o = ppo.cg(A, b, None, tol=1e-12, maxit=5000)
+    assert np.linalg.norm(x.cpu().numpy() - o["x"]) / np.linalg.norm(o["x"]) <= 1e-6
+    print(f"[amsed] rank {rank}: {cg.getNumIter()} iterations", flush=True)

@@ -595,3 +595,15 @@ class AMSED(BasePrecondition):
         perm = np.zeros(self.A.n, np.uint32)
         check(_capi.lib().pp_ml_permutation(self.h, _p(perm, C.c_uint32)))
         return perm
+
+
+class GMSLR(AMSED):
+    """GMSLRPrecondition<Low,High> (include/Preconditioner/GMSLRPrecondition.h:48-399):
+    the nonsymmetric multilevel low-rank Schur preconditioner (ILUT blocks,
+    lower coupling F, Arnoldi low-rank term)."""
+    _kind, _type = _capi.PP_PC_GMSLR, PreconditionGMSLR
+
+    def __init__(self, A, levels=3, ndom=16, rank=32, tolerance=1e-2, lfill=20, lanczos_steps=0, exact=False,
+                 **kw):
+        super().__init__(A, levels=levels, ndom=ndom, rank=rank, tolerance=tolerance, lanczos_steps=lanczos_steps,
+                         exact=exact, lfil=lfill if not exact else 1 << 30, **kw)

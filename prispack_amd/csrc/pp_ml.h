@@ -18,6 +18,8 @@ HostCsr extract_block(const HostCsr &A, const Hid &H, uint32_t r0, uint32_t r1, 
                       uint32_t &ncol);
 HostCsr transpose_rect(const HostCsr &A, uint32_t ncol);
 void tridiag_eig(std::vector<double> &d, std::vector<double> e, std::vector<double> &Z);
+bool hessenberg_eigenvalues(std::vector<double> H, int n, std::vector<double> &re, std::vector<double> &im);
+std::vector<double> inverse_iteration(const std::vector<double> &H, int n, double lambda);
 // W (n x r, col-major) = V (n x k, ld n) * Y (k x r, ld k), FP64 on the matrix cores
 void launch_gemm_tall_f64(pp_ctx *ctx, uint32_t n, uint32_t k, uint32_t r, const double *V, const double *Y,
                           double *W);
@@ -28,6 +30,7 @@ struct MlLevel {
     pp_csr *B = nullptr, *E = nullptr, *F = nullptr;
     pp_precond *Bpc = nullptr;
     double *W = nullptr, *Hd = nullptr;   // low-rank term W H W^T (device)
+    double *Md = nullptr;                 // nonsymmetric: dense r x r core (W orthonormal)
     uint32_t rank = 0;
     double *save = nullptr, *t = nullptr, *h = nullptr;  // device work vectors
     std::vector<double> Hhost, theta;     // host copy of H and the Ritz values

@@ -74,6 +74,16 @@ __global__ __launch_bounds__(256) void k_lr_reduce(uint32_t nb, const double *pa
     if (threadIdx.x == 0) h[c] = H[c] * (((sh[0] + sh[1]) + sh[2]) + sh[3]);
 }
 
+// h = M t with M dense r x r (column-major); one workgroup
+__global__ __launch_bounds__(64) void k_lr_dense(uint32_t r, const double *__restrict__ M, const double *__restrict__ t,
+                                                 double *h) {
+    for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) {
+        double acc = 0.0;
+        for (uint32_t j = 0; j < r; ++j) acc += M[(size_t)j * r + i] * t[j];
+        h[i] = acc;
+    }
+}
+
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // W (n x r) = V (n x k) * Y (k x r), column-major, FP64 MFMA 16x16x4: one
@@ -149,12 +159,23 @@ void rec_solve(pp_ctx *ctx, MlState &S, int level, double *x) {
     double *x1 = x, *x2 = x + L.nb;
     precond_apply(ctx, L.Bpc, PP_F64, x1, x1);
     launch_spmv(ctx, L.F, PP_F64, -1.0, x1, 1.0, x2, -1, nullptr, false);  // x2 -= F x1
-    if (L.rank) PP_HIP(hipMemcpyAsync(L.save, x2, (size_t)L.nrest * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    // symmetric: S^{-1} ~ C^{-1} + W H W^T (on the input of C^{-1});
+    // nonsymmetric: S^{-1} ~ (I + W H W^T) C^{-1} (on its output)
+    if (L.rank && S.symmetric)
+        PP_HIP(hipMemcpyAsync(L.save, x2, (size_t)L.nrest * 8, hipMemcpyDeviceToDevice, ctx->stream));
     rec_solve(ctx, S, next, x2);
+    if (L.rank && !S.symmetric)
+        PP_HIP(hipMemcpyAsync(L.save, x2, (size_t)L.nrest * 8, hipMemcpyDeviceToDevice, ctx->stream));
     if (L.rank) {
         double *part = L.h + 64 * 1024;
         k_lr_partial<<<kLrBlocks, 256, 0, ctx->stream>>>(L.nrest, L.rank, L.W, L.save, part);
-        k_lr_reduce<<<L.rank, 256, 0, ctx->stream>>>(kLrBlocks, part, L.Hd, L.h);
+        if (L.Md) {  // nonsymmetric: h = ((I - R)^{-1} - I) U^T s
+            double *t = L.h + 32 * 1024;
+            k_lr_reduce<<<L.rank, 256, 0, ctx->stream>>>(kLrBlocks, part, L.Hd, t);
+            k_lr_dense<<<1, 64, 0, ctx->stream>>>(L.rank, L.Md, t, L.h);
+        } else {
+            k_lr_reduce<<<L.rank, 256, 0, ctx->stream>>>(kLrBlocks, part, L.Hd, L.h);
+        }
         PP_HIP(hipGetLastError());
         launch_gemv_add(ctx, L.nrest, L.rank, nullptr, L.W, L.nrest, L.h, x2);  // x2 += W h
     }
@@ -255,6 +276,149 @@ void lowrank_lanczos(pp_ctx *ctx, MlState &S, int l, uint32_t rank, uint32_t ste
     dfree(V); dfree(Z); dfree(u); dfree(w); dfree(tb);
 }
 
+// I - R for the r x r column-major R
+std::vector<double> identity_minus(const std::vector<double> &R, uint32_t r) {
+    std::vector<double> A(R.size());
+    for (size_t i = 0; i < R.size(); ++i) A[i] = -R[i];
+    for (uint32_t i = 0; i < r; ++i) A[(size_t)i * r + i] += 1.0;
+    return A;
+}
+
+// X = A^{-1} B for A r x r and B r x nrhs (column-major), Gaussian
+// elimination with partial pivoting
+std::vector<double> solve_dense(std::vector<double> A, uint32_t r, std::vector<double> B, uint32_t nrhs) {
+    for (uint32_t k = 0; k < r; ++k) {
+        uint32_t p = k;
+        for (uint32_t i = k + 1; i < r; ++i)
+            if (std::fabs(A[(size_t)k * r + i]) > std::fabs(A[(size_t)k * r + p])) p = i;
+        PP_CHECK(A[(size_t)k * r + p] != 0.0, PP_ERR_BREAKDOWN, "singular low-rank core I - R");
+        if (p != k) {
+            for (uint32_t j = 0; j < r; ++j) std::swap(A[(size_t)j * r + k], A[(size_t)j * r + p]);
+            for (uint32_t j = 0; j < nrhs; ++j) std::swap(B[(size_t)j * r + k], B[(size_t)j * r + p]);
+        }
+        for (uint32_t i = k + 1; i < r; ++i) {
+            const double f = A[(size_t)k * r + i] / A[(size_t)k * r + k];
+            for (uint32_t j = k; j < r; ++j) A[(size_t)j * r + i] -= f * A[(size_t)j * r + k];
+            for (uint32_t j = 0; j < nrhs; ++j) B[(size_t)j * r + i] -= f * B[(size_t)j * r + k];
+        }
+    }
+    for (uint32_t j = 0; j < nrhs; ++j)
+        for (int i = (int)r - 1; i >= 0; --i) {
+            double v = B[(size_t)j * r + i];
+            for (uint32_t c = i + 1; c < r; ++c) v -= A[(size_t)c * r + i] * B[(size_t)j * r + c];
+            B[(size_t)j * r + i] = v / A[(size_t)i * r + i];
+        }
+    return B;
+}
+
+// Low-rank term of the nonsymmetric case: Arnoldi on G = C^{-1} F B^{-1} E
+// (C^{-1} by the recursion below l; CGS2 orthogonalisation), eigenvalues of
+// the Hessenberg matrix by double-shift QR, eigenvectors of the largest real
+// eigenvalues in [0, 1) by inverse iteration; W = V Y, H = lambda / (1 - lambda)
+// (GMSLRDPrecondition.cpp:980-1036 takes leading Schur vectors of H with a
+// diagonal core; here the core R = Q^T H Q of the selected invariant subspace
+// is kept whole, so the correction is exact on that subspace).
+void lowrank_arnoldi(pp_ctx *ctx, MlState &S, int l, uint32_t rank, uint32_t steps) {
+    MlLevel &L = S.lev[l];
+    const uint32_t n = L.nrest;
+    const uint32_t k = std::min<uint32_t>(std::min<uint32_t>(steps, n), 64);
+    if (rank == 0 || n == 0 || k == 0) return;
+    double *V = dalloc<double>((size_t)n * (k + 1));
+    double *w = dalloc<double>(n), *tb = dalloc<double>(std::max(1u, L.nb));
+    double *buf = gmres_coeff_buffer(ctx);
+    try {
+        std::vector<double> v0(n, 1.0 / std::sqrt((double)n));
+        PP_HIP(hipMemcpy(V, v0.data(), n * 8, hipMemcpyHostToDevice));
+        const uint32_t ld = k + 1;
+        std::vector<double> Hh((size_t)ld * k, 0.0), h1(k + 1), h2(k + 1);
+        uint32_t m = 0;
+        for (uint32_t j = 0; j < k; ++j) {
+            launch_spmv(ctx, L.E, PP_F64, 1.0, V + (size_t)j * n, 0.0, tb, -1, nullptr, false);
+            precond_apply(ctx, L.Bpc, PP_F64, tb, tb);
+            launch_spmv(ctx, L.F, PP_F64, 1.0, tb, 0.0, w, -1, nullptr, false);
+            rec_solve(ctx, S, l, w);
+            for (int pass = 0; pass < 2; ++pass) {
+                launch_gemv_t(ctx, n, j + 1, V, n, w, 0);
+                PP_HIP(hipMemcpyAsync(buf + 320, buf, (j + 1) * 8, hipMemcpyDeviceToDevice, ctx->stream));
+                launch_gemv_sub(ctx, n, j + 1, V, n, buf + 320, w);
+                read_slots(ctx, -1, j + 1, pass == 0 ? h1.data() : h2.data());
+            }
+            for (uint32_t i = 0; i <= j; ++i) Hh[(size_t)j * ld + i] = h1[i] + h2[i];
+            const double hn = std::sqrt(dot_dev(ctx, n, w, w));
+            m = j + 1;
+            Hh[(size_t)j * ld + j + 1] = hn;
+            if (!(hn > 1e-12) || j + 1 == k) break;
+            launch_axpby(ctx, PP_F64, n, 1.0 / hn, w, 0.0, w, V + (size_t)(j + 1) * n);
+        }
+        std::vector<double> Hm((size_t)m * m);
+        for (uint32_t c = 0; c < m; ++c)
+            for (uint32_t r = 0; r < m; ++r) Hm[(size_t)c * m + r] = Hh[(size_t)c * ld + r];
+        std::vector<double> re, im;
+        PP_CHECK(hessenberg_eigenvalues(Hm, (int)m, re, im), PP_ERR_BREAKDOWN,
+                 "Hessenberg eigenvalue iteration did not converge");
+        std::vector<uint32_t> idx;
+        for (uint32_t i = 0; i < m; ++i)
+            if (std::fabs(im[i]) <= 1e-10 * std::max(1.0, std::fabs(re[i])) && re[i] < 1.0 - 1e-12)
+                idx.push_back(i);
+        std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return re[a] > re[b]; });
+        if (idx.size() > rank) idx.resize(rank);
+        L.theta.assign(re.begin(), re.end());
+        // orthonormal basis Q of the eigenvectors' span (modified Gram-Schmidt,
+        // dependent vectors dropped): an invariant subspace of H_m
+        std::vector<double> Q;
+        for (uint32_t c = 0; c < idx.size(); ++c) {
+            std::vector<double> y = inverse_iteration(Hm, (int)m, re[idx[c]]);
+            const uint32_t q = (uint32_t)(Q.size() / m);
+            for (int pass = 0; pass < 2; ++pass)
+                for (uint32_t j = 0; j < q; ++j) {
+                    double d = 0.0;
+                    for (uint32_t i = 0; i < m; ++i) d += Q[(size_t)j * m + i] * y[i];
+                    for (uint32_t i = 0; i < m; ++i) y[i] -= d * Q[(size_t)j * m + i];
+                }
+            double nr = 0.0;
+            for (double v : y) nr += v * v;
+            nr = std::sqrt(nr);
+            if (!(nr > 1e-8)) continue;
+            for (double v : y) Q.push_back(v / nr);
+        }
+        const uint32_t r = (uint32_t)(Q.size() / m);
+        L.rank = r;
+        if (r) {
+            // R = Q^T H_m Q, core K = (I - R)^{-1} - I = (I - R)^{-1} R
+            std::vector<double> HQ((size_t)m * r, 0.0), R((size_t)r * r, 0.0);
+            for (uint32_t c = 0; c < r; ++c)
+                for (uint32_t j = 0; j < m; ++j)
+                    for (uint32_t i = 0; i < m; ++i) HQ[(size_t)c * m + i] += Hm[(size_t)j * m + i] * Q[(size_t)c * m + j];
+            for (uint32_t c = 0; c < r; ++c)
+                for (uint32_t i = 0; i < r; ++i) {
+                    double d = 0.0;
+                    for (uint32_t j = 0; j < m; ++j) d += Q[(size_t)i * m + j] * HQ[(size_t)c * m + j];
+                    R[(size_t)c * r + i] = d;
+                }
+            std::vector<double> K = solve_dense(identity_minus(R, r), r, R, r);
+            L.Hhost.resize(r);
+            for (uint32_t c = 0; c < r; ++c) L.Hhost[c] = R[(size_t)c * r + c] / (1.0 - R[(size_t)c * r + c]);
+            double *Qd = dalloc<double>(Q.size());
+            PP_HIP(hipMemcpy(Qd, Q.data(), Q.size() * 8, hipMemcpyHostToDevice));
+            L.W = dalloc<double>((size_t)n * r);
+            launch_gemm_tall_f64(ctx, n, m, r, V, Qd, L.W);
+            PP_HIP(hipStreamSynchronize(ctx->stream));
+            dfree(Qd);
+            std::vector<double> ones(r, 1.0);
+            L.Hd = dalloc<double>(r);
+            PP_HIP(hipMemcpy(L.Hd, ones.data(), r * 8, hipMemcpyHostToDevice));
+            L.Md = dalloc<double>((size_t)r * r);
+            PP_HIP(hipMemcpy(L.Md, K.data(), (size_t)r * r * 8, hipMemcpyHostToDevice));
+            L.save = dalloc<double>(n);
+            L.h = dalloc<double>(64 * 1024 + (size_t)r * kLrBlocks);
+        }
+    } catch (...) {
+        dfree(V); dfree(w); dfree(tb);
+        throw;
+    }
+    dfree(V); dfree(w); dfree(tb);
+}
+
 }  // namespace
 
 void launch_gemm_tall_f64(pp_ctx *ctx, uint32_t n, uint32_t k, uint32_t r, const double *V, const double *Y,
@@ -274,7 +438,7 @@ void ml_free(pp_precond *M) {
         pp_csr_destroy(L.B);
         pp_csr_destroy(L.E);
         pp_csr_destroy(L.F);
-        dfree(L.W); dfree(L.Hd); dfree(L.save); dfree(L.t); dfree(L.h);
+        dfree(L.W); dfree(L.Hd); dfree(L.Md); dfree(L.save); dfree(L.t); dfree(L.h);
     }
     pp_precond_destroy(S->last);
     pp_csr_destroy(S->Clast);
@@ -307,7 +471,7 @@ void ml_setup(pp_ctx *ctx, pp_precond *M) {
         // (GMSLR) (AMSEDPrecondition.h:39-51, GMSLRPrecondition.h:25-37); 0 gives
         // exact block factors
         const double tau = M->opts.tau;
-        const uint32_t lfil = 20;
+        const uint32_t lfil = M->opts.lfil ? M->opts.lfil : 20;
         M->mnnz = 0;
         // last level
         uint32_t nc = 0;
@@ -341,6 +505,7 @@ void ml_setup(pp_ctx *ctx, pp_precond *M) {
         S->lanczos_steps = steps;
         for (int l = (int)nlev - 2; l >= 0; --l) {
             if (spd) lowrank_lanczos(ctx, *S, l, rank, steps);
+            else lowrank_arnoldi(ctx, *S, l, rank, steps);
             MlLevel &L = S->lev[l];
             M->mnnz += 2 * L.rank * L.nrest + L.rank;  // AMSEDPrecondition.cpp:1269 convention
         }

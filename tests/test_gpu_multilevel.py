@@ -3,14 +3,17 @@ preconditioner and the FP64 matrix-core GEMM, through the C ABI.
 
 Multicolour: the product's apply equals the oracle's IC(0)/ILU(0) of P A P^T
 (P from an independent greedy colouring written here), bit for bit in FP64.
-AMSED: parity with the reference is unpinned (the reference needs METIS,
+AMSED and GMSLR: parity with the reference is unpinned (the reference needs METIS,
 SuiteSparse AMD and LAPACKE, absent here); pinned instead by
   - the hierarchical ordering being a permutation whose levels are block
     diagonal (no coupling between blocks of one level),
   - exactness: with exact block factors (tau = 0) and rank 0 the recursion is
     an exact block LDL^T, so M^{-1} A x = x,
   - the Lanczos Ritz values lying in [0, 1) (S = C - E^T B^{-1} E is SPD),
-  - PCG converging to rtol 1e-8 on the config-4 matrix type.
+  - PCG converging to rtol 1e-8 on the config-4 matrix type,
+  - GMSLR: the nonsymmetric recursion against the same dense restatement,
+    full-rank exactness on D A D^{-1} (real spectrum of G, no symmetry), and
+    FGMRES(30) converging on a 3-D upwind matrix.
 GEMM: against a host FP64 product (brute force)."""
 import ctypes as C
 
@@ -219,3 +222,74 @@ def test_amsed_pcg_config4_type(pp, torch, rank):
     o = ppo.cg(A, b, None, tol=1e-12, maxit=5000)
     assert np.linalg.norm(x.cpu().numpy() - o["x"]) / np.linalg.norm(o["x"]) <= 1e-6
     print(f"[amsed] rank {rank}: {cg.getNumIter()} iterations", flush=True)
+
+
+def scaled(A, seed):
+    """D A D^{-1} with a random positive diagonal D: nonsymmetric, with the
+    pattern and eigenvalues of A."""
+    rp, ci, v = A
+    n = len(rp) - 1
+    d = np.exp(np.random.default_rng(seed).uniform(-1, 1, n))
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    return rp, ci, v * d[rows] / d[ci]
+
+
+@pytest.mark.parametrize("levels,ndom", [(2, 4), (3, 8)])
+def test_gmslr_recursion_matches_dense_restatement(pp, torch, levels, ndom):
+    # GMSLRDPrecondition.cpp levelCInvRecursiveSolve with exact blocks and no
+    # low-rank term; nonsymmetric matrix (upwind), F the lower coupling
+    A = ppo.upwind(10, 9, 8, 30.0, 10.0, 5.0)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    P = pp.GMSLR(M, levels=levels, ndom=ndom, rank=0, exact=True)
+    P.setup()
+    nlev, sizes, ranks, perm = _ml_info(pp, P)
+    assert nlev == levels and sum(sizes) == n and not any(ranks)
+    Ap = dense(A)[np.ix_(perm, perm)]
+    lev_ptr = np.concatenate([[0], np.cumsum(sizes)])
+    x = ppo.uniform(n, 4)
+    y = torch.tensor(x, device="cuda")
+    P.MInvSolve(y)
+    want = np.empty(n)
+    want[perm] = dense_recursion(Ap, lev_ptr, -1, x[perm])
+    np.testing.assert_allclose(y.cpu().numpy(), want, rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("levels,ndom", [(2, 2), (3, 4)])
+def test_gmslr_full_rank_is_exact(pp, torch, levels, ndom):
+    # D A D^{-1} of a variable-coefficient diffusion matrix: G = C^{-1} F B^{-1} E
+    # is diagonalisable with real eigenvalues in [0, 1) and no structural
+    # symmetry, so a full Arnoldi basis (every Schur complement under 64 rows)
+    # makes (I + W K W^T) C^{-1} = S^{-1} at every level and M^{-1} A = I
+    side = 12
+    kap = np.exp(np.random.default_rng(5).uniform(-1, 1, side * side))
+    A = scaled(ppo.vardiff2d(side, kap), 6)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    P = pp.GMSLR(M, levels=levels, ndom=ndom, rank=64, lanczos_steps=64, exact=True)
+    P.setup()
+    nlev, sizes, ranks, _ = _ml_info(pp, P)
+    print(f"[gmslr] sizes {sizes} ranks {ranks}", flush=True)
+    assert all(n - sum(sizes[:l + 1]) <= 64 for l in range(nlev - 1))
+    x = ppo.uniform(n, 3)
+    y = torch.tensor(ppo.spmv(A, x), device="cuda")
+    P.MInvSolve(y)
+    np.testing.assert_allclose(y.cpu().numpy(), x, rtol=0, atol=1e-7)
+
+
+@pytest.mark.parametrize("rank", [0, 16])
+def test_gmslr_gmres_upwind(pp, torch, rank):
+    A = ppo.upwind(24, 24, 24, 40.0, 20.0, 10.0)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 1))
+    M = pp.CSRMatrix(*A)
+    P = pp.GMSLR(M, levels=3, ndom=16, rank=rank)
+    P.setup()
+    nlev, sizes, ranks, _ = _ml_info(pp, P)
+    print(f"[gmslr] sizes {sizes} ranks {ranks}", flush=True)
+    g = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=500)
+    g.setRestart(True, 30)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    g.solve(M, torch.tensor(b, device="cuda"), x)
+    assert g.getConvergence() and g.report.true_relres < 1e-7
+    print(f"[gmslr] rank {rank}: {g.getNumIter()} iterations", flush=True)

@@ -262,7 +262,8 @@ typedef struct pp_batch pp_batch;
 int pp_batch_create(pp_ctx *ctx, uint32_t nsys, const uint32_t *row_off, const uint32_t *rowptr,
                     const uint32_t *colidx, const double *values, pp_batch **out);
 int pp_batch_set_values(pp_ctx *ctx, pp_batch *B, const double *values, int on_device);
-/* b, x: device, concatenated (x holds x0).  precision PP_FP64 or PP_MIXED.
+/* b, x: device, concatenated; every system starts from x0 = 0 and x receives
+ * the solutions.  precision PP_FP64 or PP_MIXED.
  * iters_out (host, nsys) receives per-system iteration counts (may be NULL). */
 int pp_batch_solve(pp_ctx *ctx, pp_batch *B, const pp_solve_opts *opts, const double *b,
                    double *x, int32_t *iters_out, pp_solve_report *rep);

@@ -60,7 +60,30 @@ def build(verbose=True):
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT} from {len(srcs)} sources", flush=True)
+    build_dropin_driver(verbose)
     return OUT
+
+
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "dropin_driver")
+
+
+def build_dropin_driver(verbose=True):
+    """The C++ drop-in test driver: host code only (g++), over
+    include/prispack_amd.hpp, linked against the library built above."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    cxx = os.environ.get("CXX", "g++")
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(ROOT, "include"),
+           DRIVER_SRC, "-L" + HERE, "-lprispack_amd", "-Wl,-rpath,$ORIGIN/../../prispack_amd",
+           "-Wl,-rpath-link,/opt/rocm/lib", "-o", DRIVER + ".tmp"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{p.stdout}\n{p.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print(f"built {DRIVER}", flush=True)
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -2,10 +2,13 @@
 include/prispack_amd.h declares, and its host-side parts (model generators,
 seeded inputs, the setup() factorizations) agree bit for bit with the oracle.
 CPU only."""
+import os
 import numpy as np
 import pytest
 
 from oracle import ppo
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 @pytest.fixture(scope="module")
@@ -85,3 +88,27 @@ def test_level_counts_of_natural_order_ic0(pp):
     # hyperplanes i+j+k = const of a natural-order 7-point IC(0): 3(n-1)+1 levels
     f = pp.host_factor(ppo.laplacian(9, 8, 7), "ic0")
     assert f["levels_lower"] == f["levels_upper"] == 8 + 7 + 6 + 1
+
+
+# ---------------------------------------------------------------- C++ drop-in layer
+def test_cpp_header_compiles_strict():
+    import subprocess
+    src = "#include \"prispack_amd.hpp\"\nint main() { return 0; }\n"
+    p = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Wpedantic", "-Werror",
+                        "-I" + os.path.join(ROOT, "include"), "-x", "c++", "-"], input=src, text=True,
+                       capture_output=True)
+    assert p.returncode == 0, p.stderr
+
+
+def test_cpp_dropin_driver_fails_loudly_without_gpu(tmp_path):
+    # no GPU in this process's view: the first device call throws prispack::Error
+    # and the driver exits non-zero; there is no CPU fallback
+    import dropin_io
+    if not os.path.exists(dropin_io.DRIVER):
+        pytest.skip("driver not built (run the build first)")
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    A = ppo.laplacian(6, 6, 1)
+    r = dropin_io.run(str(tmp_path), A, np.ones(36), "cg", "ict", 1e-2, 1e-8)
+    assert r["returncode"] == 1 and r.get("status", 0) not in (0, None) and "exception" in r

@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--trsv", type=int, default=0, help="SpTRSV schedule: 0 auto, 1 levels, 2 sync-free")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized FP64 buffer 5 times (device-to-device copy kernel) and exit: "
+                         "the known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--ordering", type=int, default=1,
                     help="IC(0) row ordering: 1 multicolour (2 sweep levels, default), 0 natural (382 levels)")
     return ap.parse_args()
@@ -89,6 +92,16 @@ def main():
               "in DESIGN.md and is not part of this build.", file=sys.stderr)
         sys.exit(2)
     import torch
+
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nb, "launches": 5}), flush=True)
+        return
 
     import prispack_amd as pp
     from prispack_amd import _capi

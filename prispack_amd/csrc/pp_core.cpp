@@ -322,6 +322,7 @@ int pp_csr_set_values(pp_ctx *ctx, pp_csr *A, const double *values, int on_devic
         }
         PP_HIP(hipStreamSynchronize(ctx->stream));
         A->f32_dirty = true;
+        ++A->version;
         if (A->trans) {
             csr_free(A->trans);
             A->trans = nullptr;

@@ -109,6 +109,11 @@ struct DevSweep {
     T *val = nullptr;           // [nnz]
     T *diag = nullptr;          // [n] (original numbering) or nullptr
     std::vector<uint32_t> level_ptr;  // host [nlev+1]
+    // host [nlev]: perm[k] = k + level_off[l] over level l when the level's rows
+    // are contiguous in the original numbering (the sweep then skips perm),
+    // else kNoOffset
+    std::vector<int64_t> level_off;
+    static constexpr int64_t kNoOffset = INT64_MIN;
     uint32_t max_level_rows = 0;
     void free_all();
 };
@@ -137,6 +142,7 @@ struct pp_csr {
     std::vector<uint32_t> h_rp, h_ci;
     std::vector<double> h_v;
     bool f32_dirty = true;
+    uint64_t version = 0;            // bumped by every value update
 };
 
 struct pp_precond {
@@ -164,6 +170,14 @@ struct pp_precond {
     double *tmp64 = nullptr;      // out-of-place buffer between the two sweeps
     float *tmp32 = nullptr;
     void *multilevel = nullptr;   // AMSED / GMSLR state
+    // P A P^T in the factor's row order, for solves that run in that order
+    // (no gather/scatter around each apply); built from Aperm_src at
+    // Aperm_version.  io_permuted: the current solve's vectors are already in
+    // the factor's order.
+    pp_csr *Aperm = nullptr;
+    const pp_csr *Aperm_src = nullptr;
+    uint64_t Aperm_version = 0;
+    bool io_permuted = false;
 };
 
 namespace pp {
@@ -191,6 +205,7 @@ int factor_ilu0(const HostCsr &A, HostCsr &L, HostCsr &U, std::vector<double> &D
 HostCsr transpose(const HostCsr &A);
 std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors);
 HostCsr permute_symmetric(const HostCsr &A, const std::vector<uint32_t> &order);
+pp_csr *precond_permuted_matrix(pp_ctx *ctx, pp_precond *M, pp_csr *A);
 void launch_permute(pp_ctx *ctx, uint32_t n, const uint32_t *perm, int dtype, const void *src, void *dst,
                     bool gather);
 

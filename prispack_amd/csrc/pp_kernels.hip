@@ -295,12 +295,13 @@ __global__ __launch_bounds__(256) void k_sweep_level(const uint32_t *__restrict_
                                                      const uint32_t *__restrict__ ci,
                                                      const T *__restrict__ val, const T *__restrict__ diag,
                                                      int mode, const T *b, T *x, uint32_t k0, uint32_t k1,
-                                                     const int *stop) {
+                                                     long long off, const int *stop) {
 #pragma clang fp contract(off)
     const uint32_t k = k0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= k1) return;
     if (stop && *stop) return;
-    const uint32_t i = perm[k];
+    // perm == nullptr: the level's rows are contiguous, row = position + off
+    const uint32_t i = perm ? perm[k] : (uint32_t)((long long)k + off);
     T s = b[i];
     if (mode == 3) s = s / diag[i];
     for (uint32_t e = rp[k]; e < rp[k + 1]; ++e) {
@@ -318,7 +319,9 @@ void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x)
     for (uint32_t l = 0; l < s.nlev; ++l) {
         uint32_t k0 = s.level_ptr[l], k1 = s.level_ptr[l + 1];
         unsigned g = (k1 - k0 + kBlock - 1) / kBlock;
-        k_sweep_level<T><<<g, kBlock, 0, ctx->stream>>>(s.perm, s.rp, s.ci, s.val, s.diag, mode, b, x, k0, k1, stop);
+        const bool contiguous = l < s.level_off.size() && s.level_off[l] != DevSweep<T>::kNoOffset;
+        k_sweep_level<T><<<g, kBlock, 0, ctx->stream>>>(contiguous ? nullptr : s.perm, s.rp, s.ci, s.val, s.diag,
+                                                        mode, b, x, k0, k1, contiguous ? s.level_off[l] : 0, stop);
     }
     PP_HIP(hipGetLastError());
 }

@@ -89,7 +89,8 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // W (n x r) = V (n x k) * Y (k x r), column-major, FP64 MFMA 16x16x4: one
 // wavefront per 16 x 16 output tile, K in steps of 4.  Operand layout of
 // v_mfma_f64_16x16x4f64: lane l holds A[l%16][l/16], B[l/16][l%16] and
-// D[4*(l/16) + reg][l%16].
+// D[l/16 + 4*reg][l%16] (the FP64 output interleaves rows by 4, unlike the
+// 32-bit 16x16 forms).
 __global__ __launch_bounds__(256) void k_gemm_tall_f64(uint32_t n, uint32_t k, uint32_t r, const double *__restrict__ V,
                                                        const double *__restrict__ Y, double *__restrict__ W) {
     const uint32_t lane = threadIdx.x & 63;
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) void k_gemm_tall_f64(uint32_t n, uint32_t k, u
     const uint32_t ocol = tc * 16 + i;
 #pragma unroll
     for (int reg = 0; reg < 4; ++reg) {
-        const uint32_t orow = tr * 16 + 4 * q + reg;
+        const uint32_t orow = tr * 16 + q + 4 * reg;
         if (orow < n && ocol < r) W[(size_t)ocol * n + orow] = acc[reg];
     }
 }

@@ -79,6 +79,12 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
     d.level_ptr = lp;
     d.max_level_rows = 0;
     for (uint32_t l = 0; l < d.nlev; ++l) d.max_level_rows = std::max(d.max_level_rows, lp[l + 1] - lp[l]);
+    d.level_off.assign(d.nlev, DevSweep<T>::kNoOffset);
+    for (uint32_t l = 0; l < d.nlev; ++l) {
+        bool contiguous = true;
+        for (uint32_t k = lp[l] + 1; k < lp[l + 1] && contiguous; ++k) contiguous = perm[k] == perm[k - 1] + 1;
+        if (contiguous && lp[l + 1] > lp[l]) d.level_off[l] = (int64_t)perm[lp[l]] - (int64_t)lp[l];
+    }
     d.perm = dalloc<uint32_t>(n);
     d.rp = dalloc<uint32_t>(n + 1);
     d.ci = dalloc<uint32_t>(d.nnz);
@@ -231,6 +237,23 @@ void precond_need_f32(pp_precond *M) {
     M->have32 = true;
 }
 
+// P A P^T for a solve in the factor's row order (rebuilt when A or its values
+// changed since the last build)
+pp_csr *precond_permuted_matrix(pp_ctx *ctx, pp_precond *M, pp_csr *A) {
+    if (M->Aperm && M->Aperm_src == A && M->Aperm_version == A->version) return M->Aperm;
+    csr_free(M->Aperm);
+    M->Aperm = nullptr;
+    HostCsr H;
+    H.n = A->nrow;
+    H.rp = A->h_rp;
+    H.ci = A->h_ci;
+    H.v = A->h_v;
+    M->Aperm = csr_create_host(ctx, permute_symmetric(H, M->order));
+    M->Aperm_src = A;
+    M->Aperm_version = A->version;
+    return M->Aperm;
+}
+
 // y = M^{-1} x on the context stream (y may alias x).
 void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
     PP_CHECK(M->ready, PP_ERR_NOT_READY, "preconditioner used before setup()");
@@ -243,7 +266,7 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
         ml_apply(ctx, M, dtype, x, y);
         return;
     }
-    if (!M->order.empty() && x != (const void *)M->xp64 && x != (const void *)M->xp32) {
+    if (!M->order.empty() && !M->io_permuted && x != (const void *)M->xp64 && x != (const void *)M->xp32) {
         // factor of P A P^T: y = P^T M^{-1} P x (the inner call sees x == xp)
         void *xp;
         if (dtype == PP_F64) {
@@ -480,6 +503,7 @@ void pp_precond_destroy(pp_precond *M) {
     dfree(M->xp32);
     dfree(M->tmp64);
     dfree(M->tmp32);
+    csr_free(M->Aperm);
     M->low64.free_all();
     M->up64.free_all();
     M->low32.free_all();

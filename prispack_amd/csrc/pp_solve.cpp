@@ -26,7 +26,8 @@ double *gmres_coeff_buffer(pp_ctx *ctx);
 namespace {
 
 // Workspace indices in the context cache.
-enum { W_R = 0, W_W, W_D, W_V, W_R32, W_W32, W_D32, W_V32, W_X32, W_B32, W_E32, W_VB, W_ZB, W_ZB32, W_T, W_T32 };
+enum { W_R = 0, W_W, W_D, W_V, W_R32, W_W32, W_D32, W_V32, W_X32, W_B32, W_E32, W_VB, W_ZB, W_ZB32, W_T, W_T32,
+       W_PB, W_PX };
 
 template <class T>
 constexpr int dt() {
@@ -444,21 +445,45 @@ static int solve_impl(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts
         const pp_solve_opts &o = *opts;
         PP_CHECK(o.tol > 0, PP_ERR_INVALID, "tol must be positive");
         PP_CHECK(o.maxit >= 1, PP_ERR_INVALID, "maxit must be >= 1");
+        PP_CHECK(o.solver == PP_SOLVER_CG || o.solver == PP_SOLVER_GMRES, PP_ERR_INVALID, "unknown solver");
+        // A factor of P A P^T (multicolour order): run the whole solve on
+        // P A P^T with P b and P x0, so no apply gathers or scatters; x is
+        // scattered back once at the end.
+        const uint32_t n = A->nrow;
+        pp_csr *As = A;
+        const double *bs = b;
+        double *xs = x;
+        const bool permuted = M && !M->order.empty() && !M->multilevel && M->A == A;
+        if (permuted) {
+            As = precond_permuted_matrix(ctx, M, A);
+            double *bp = (double *)ws_get(ctx, W_PB, (size_t)n * sizeof(double));
+            double *xp = (double *)ws_get(ctx, W_PX, (size_t)n * sizeof(double));
+            launch_permute(ctx, n, M->dorder, PP_F64, b, bp, true);
+            launch_permute(ctx, n, M->dorder, PP_F64, x, xp, true);
+            bs = bp;
+            xs = xp;
+        }
+        struct Unpermute {
+            pp_precond *M;
+            ~Unpermute() {
+                if (M) M->io_permuted = false;
+            }
+        } unpermute{permuted ? M : nullptr};
+        if (permuted) M->io_permuted = true;
         PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         PcgResult res;
         if (o.solver == PP_SOLVER_CG) {
             if (o.precision == PP_MIXED) {
                 double rel;
-                res = pcg_mixed(ctx, A, M, o, b, x, R.outer_iters, R.inner_iters, rel);
+                res = pcg_mixed(ctx, As, M, o, bs, xs, R.outer_iters, R.inner_iters, rel);
             } else {
-                res = pcg_device<double>(ctx, A, M, o.flexible != 0, b, x, o.tol, o.maxit,
+                res = pcg_device<double>(ctx, As, M, o.flexible != 0, bs, xs, o.tol, o.maxit,
                                          o.check_every ? o.check_every : 1, o.record_history != 0, W_R);
             }
-        } else if (o.solver == PP_SOLVER_GMRES) {
-            res = gmres_device(ctx, A, M, o, b, x);
         } else {
-            throw Error(PP_ERR_INVALID, "unknown solver");
+            res = gmres_device(ctx, As, M, o, bs, xs);
         }
+        if (permuted) launch_permute(ctx, n, M->dorder, PP_F64, xs, x, false);
         PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         PP_HIP(hipEventSynchronize(ctx->ev1));
         float ms = 0;

@@ -33,11 +33,14 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-natural", action="store_true", help="skip the natural-ordering record")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--trsv", type=int, default=0, help="SpTRSV schedule: 0 auto, 1 levels, 2 sync-free")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized FP64 buffer 5 times (device-to-device copy kernel) and exit: "
                          "the known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--launch", type=int, default=0, help="PCG body: 0 hipGraph replay, 1 stream launches")
+    ap.add_argument("--check-every", type=int, default=1, help="iterations per host convergence poll")
     ap.add_argument("--ordering", type=int, default=1,
                     help="IC(0) row ordering: 1 multicolour (2 sweep levels, default), 0 natural (382 levels)")
     return ap.parse_args()
@@ -129,6 +132,8 @@ def main():
     def solve(precision, P=P):
         cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision=precision)
         cg.opts.record_history = 0
+        cg.opts.launch = args.launch
+        cg.opts.check_every = args.check_every
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -188,7 +193,7 @@ def main():
 
     # the other ordering, one solve of each precision, for the record
     other = None
-    if args.ordering == 1:
+    if args.ordering == 1 and not args.no_natural:
         Pn = pp.IC0(M, make_f32=True, trsv=args.trsv, ordering=0)
         Pn.setup()
         solve("fp64", Pn)

@@ -127,8 +127,10 @@ enum {
 /* Row ordering of the factor (no reference counterpart; the reference factors
  * in the given order).  MULTICOLOR factors P A P^T with rows grouped by a
  * greedy distance-1 colouring, so IC(0)/ILU(0) sweeps have at most as many
- * dependency levels as colours (2 for 5/7-point stencils); the apply permutes
- * in and out.  It changes the preconditioner (and the iteration count). */
+ * dependency levels as colours (2 for 5/7-point stencils).  A solve on the
+ * matrix the factor was built from runs entirely in the colour order (P A P^T
+ * is formed once); a direct apply permutes in and out.  It changes the
+ * preconditioner (and the iteration count). */
 enum { PP_ORDER_NATURAL = 0, PP_ORDER_MULTICOLOR = 1 };
 
 typedef struct {
@@ -205,6 +207,7 @@ int pp_gemm_tall_f64(pp_ctx *ctx, uint32_t n, uint32_t k, uint32_t r, const doub
  *      CG, source/Solver/CG.cpp:33-83; GMRES, source/Solver/GMRES.cpp:59-221) ---- */
 enum { PP_SOLVER_CG = 0, PP_SOLVER_GMRES = 1 };
 enum { PP_FP64 = 0, PP_MIXED = 1 };
+enum { PP_LAUNCH_GRAPH = 0, PP_LAUNCH_STREAM = 1 };
 
 typedef struct {
     int solver;          /* PP_SOLVER_* */
@@ -219,7 +222,10 @@ typedef struct {
     double inner_tol;    /* mixed: relative reduction asked of each inner solve */
     uint32_t max_outer;  /* mixed: refinement steps */
     int record_history;  /* keep per-iteration ||r||/||b|| */
-    uint32_t reserved[8];
+    int launch;          /* PP_LAUNCH_GRAPH: the iteration body is captured once in a
+                            hipGraph and replayed; PP_LAUNCH_STREAM: its kernels are
+                            launched on the stream each iteration */
+    uint32_t reserved[7];
 } pp_solve_opts;
 
 typedef struct {

@@ -33,6 +33,7 @@ struct Error : std::runtime_error {
 
 constexpr int kBlock = 256;             // threads per workgroup of the streaming kernels
 constexpr uint32_t kSpmvNnzBlock = 2048; // nnz staged in LDS per SpMV workgroup
+constexpr uint32_t kStreamLevelRows = 4096; // sweep levels this wide use the LDS-staged kernel
 constexpr uint32_t kSpmvRowsBlock = 256; // rows per SpMV workgroup (one per thread)
 constexpr int kMaxPartials = 4096;       // reduction partials per slot
 
@@ -114,6 +115,12 @@ struct DevSweep {
     // else kNoOffset
     std::vector<int64_t> level_off;
     static constexpr int64_t kNoOffset = INT64_MIN;
+    // wide levels (>= kStreamLevelRows rows) are swept in row blocks of at
+    // most kSpmvRowsBlock rows / kSpmvNnzBlock dependencies (positions
+    // blocks[j]..blocks[j+1]); level l owns blocks [level_blk[2l],
+    // level_blk[2l+1]) (host), an empty range for a narrow level
+    uint32_t *blocks = nullptr;
+    std::vector<uint32_t> level_blk;
     uint32_t max_level_rows = 0;
     void free_all();
 };

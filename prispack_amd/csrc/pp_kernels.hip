@@ -103,7 +103,25 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
         const uint32_t k0 = rp[r0], k1 = rp[r1];
         const uint32_t cnt = k1 - k0;
         if (cnt <= kSpmvNnzBlock) {
-            for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) prod[k] = val[k0 + k] * x[ci[k0 + k]];
+            // all of the block's column indices and values are loaded before
+            // the first gather (PER independent loads in flight per thread);
+            // they stream through once, so they bypass the caches that keep x
+            constexpr int PER = kSpmvNnzBlock / kBlock;
+            uint32_t c[PER];
+            T v[PER];
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t k = threadIdx.x + j * kBlock;
+                if (k < cnt) {
+                    c[j] = __builtin_nontemporal_load(ci + k0 + k);
+                    v[j] = __builtin_nontemporal_load(val + k0 + k);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t k = threadIdx.x + j * kBlock;
+                if (k < cnt) prod[k] = v[j] * x[c[j]];
+            }
             __syncthreads();
             if (threadIdx.x < r1 - r0) {
                 const uint32_t row = r0 + threadIdx.x;
@@ -313,13 +331,84 @@ __global__ __launch_bounds__(256) void k_sweep_level(const uint32_t *__restrict_
     x[i] = s;
 }
 
+// One wide level, LDS-staged like the SpMV: each workgroup takes a row block,
+// loads its dependencies' values, columns and x gathers coalesced into LDS
+// products, then one thread per row subtracts them in storage order.  Same
+// operations in the same order as k_sweep_level (bit-identical results).
+// Rows of one level never depend on each other, so x may alias b.
+template <class T>
+__global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict__ perm,
+                                                      const uint32_t *__restrict__ rp,
+                                                      const uint32_t *__restrict__ ci,
+                                                      const T *__restrict__ val, const T *__restrict__ diag,
+                                                      int mode, const T *b, T *x,
+                                                      const uint32_t *__restrict__ blocks, uint32_t blk0,
+                                                      uint32_t blk1, long long off, const int *stop) {
+#pragma clang fp contract(off)
+    __shared__ T prod[kSpmvNnzBlock];
+    if (stop && *stop) return;
+    constexpr int PER = kSpmvNnzBlock / kBlock;
+    for (uint32_t bi = blk0 + blockIdx.x; bi < blk1; bi += gridDim.x) {
+        const uint32_t r0 = blocks[bi], r1 = blocks[bi + 1];
+        const uint32_t e0 = rp[r0], cnt = rp[r1] - e0;
+        const bool staged = cnt <= kSpmvNnzBlock;
+        if (staged) {
+            uint32_t c[PER];
+            T v[PER];
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t k = threadIdx.x + j * kBlock;
+                if (k < cnt) {
+                    c[j] = __builtin_nontemporal_load(ci + e0 + k);
+                    v[j] = __builtin_nontemporal_load(val + e0 + k);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const uint32_t k = threadIdx.x + j * kBlock;
+                if (k < cnt) {
+                    T p = v[j] * x[c[j]];
+                    prod[k] = p;
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = r0 + threadIdx.x; k < r1; k += kBlock) {
+            const uint32_t i = perm ? perm[k] : (uint32_t)((long long)k + off);
+            T s = b[i];
+            if (mode == 3) s = s / diag[i];
+            const uint32_t a0 = rp[k], a1 = rp[k + 1];
+            if (staged) {
+                for (uint32_t e = a0; e < a1; ++e) s = s - prod[e - e0];
+            } else {  // a single row longer than the staging buffer
+                for (uint32_t e = a0; e < a1; ++e) {
+                    T p = val[e] * x[ci[e]];
+                    s = s - p;
+                }
+            }
+            if (mode == 0) s = s / diag[i];
+            else if (mode == 2) s = s * diag[i];
+            x[i] = s;
+        }
+        __syncthreads();
+    }
+}
+
 template <class T>
 void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x) {
     const int *stop = ctx->ctl + C_STOP;
     for (uint32_t l = 0; l < s.nlev; ++l) {
         uint32_t k0 = s.level_ptr[l], k1 = s.level_ptr[l + 1];
-        unsigned g = (k1 - k0 + kBlock - 1) / kBlock;
         const bool contiguous = l < s.level_off.size() && s.level_off[l] != DevSweep<T>::kNoOffset;
+        if (s.blocks && 2 * l + 1 < s.level_blk.size() && s.level_blk[2 * l + 1] > s.level_blk[2 * l]) {
+            const uint32_t b0 = s.level_blk[2 * l], b1 = s.level_blk[2 * l + 1];
+            const unsigned g = std::min<unsigned>(b1 - b0, 8192u);
+            k_sweep_stream<T><<<g, kBlock, 0, ctx->stream>>>(contiguous ? nullptr : s.perm, s.rp, s.ci, s.val,
+                                                             s.diag, mode, b, x, s.blocks, b0, b1,
+                                                             contiguous ? s.level_off[l] : 0, stop);
+            continue;
+        }
+        unsigned g = (k1 - k0 + kBlock - 1) / kBlock;
         k_sweep_level<T><<<g, kBlock, 0, ctx->stream>>>(contiguous ? nullptr : s.perm, s.rp, s.ci, s.val, s.diag,
                                                         mode, b, x, k0, k1, contiguous ? s.level_off[l] : 0, stop);
     }

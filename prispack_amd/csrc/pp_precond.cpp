@@ -23,6 +23,9 @@ void DevSweep<T>::free_all() {
     dfree(ci);
     dfree(val);
     dfree(diag);
+    dfree(blocks);
+    blocks = nullptr;
+    level_blk.clear();
     perm = rp = ci = nullptr;
     val = diag = nullptr;
     level_ptr.clear();
@@ -84,6 +87,30 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
         bool contiguous = true;
         for (uint32_t k = lp[l] + 1; k < lp[l + 1] && contiguous; ++k) contiguous = perm[k] == perm[k - 1] + 1;
         if (contiguous && lp[l + 1] > lp[l]) d.level_off[l] = (int64_t)perm[lp[l]] - (int64_t)lp[l];
+    }
+    // row blocks of the wide levels (positions, level order)
+    std::vector<uint32_t> blk;
+    d.level_blk.assign(2 * (size_t)d.nlev, 0);
+    for (uint32_t l = 0; l < d.nlev; ++l) {
+        if (lp[l + 1] - lp[l] >= kStreamLevelRows) {
+            uint32_t k = lp[l];
+            if (blk.empty() || blk.back() != k) blk.push_back(k);
+            d.level_blk[2 * l] = (uint32_t)blk.size() - 1;
+            while (k < lp[l + 1]) {
+                const uint32_t start = k, e0 = rp[k];
+                if (rp[k + 1] - e0 > kSpmvNnzBlock) {
+                    ++k;  // a long row alone
+                } else {
+                    while (k < lp[l + 1] && k - start < kSpmvRowsBlock && rp[k + 1] - e0 <= kSpmvNnzBlock) ++k;
+                }
+                blk.push_back(k);
+            }
+            d.level_blk[2 * l + 1] = (uint32_t)blk.size() - 1;
+        }
+    }
+    if (blk.size() > 1) {
+        d.blocks = dalloc<uint32_t>(blk.size());
+        PP_HIP(hipMemcpy(d.blocks, blk.data(), blk.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
     d.perm = dalloc<uint32_t>(n);
     d.rp = dalloc<uint32_t>(n + 1);

@@ -47,7 +47,7 @@ struct PcgResult {
 // reference's repeated product at CG.cpp:68 returns the same vector.
 template <class T>
 PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const T *b, T *x, double tol,
-                     uint32_t maxit, uint32_t check_every, bool hist_on, int wbase) {
+                     uint32_t maxit, uint32_t check_every, bool hist_on, int wbase, bool use_graph) {
     const uint32_t n = A->nrow;
     const int DT = dt<T>();
     T *r = (T *)ws_get(ctx, wbase + 0, (size_t)n * sizeof(T));
@@ -81,29 +81,33 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
         }
     };
     // The iteration body (every launch of it: sweeps, updates, SpMV) is
-    // captured once into a hipGraph and replayed.  Two bodies stay in flight:
-    // after each replay the control words are copied to pinned memory behind
-    // an event, and the host reads the copy of the previous replay, so the
-    // GPU never waits for the host; a replay past convergence only runs
-    // kernels that return at once.
+    // captured once into a hipGraph and replayed, or (use_graph false)
+    // launched on the stream each time.  Two bodies stay in flight: after
+    // each one the control words are copied to pinned memory behind an
+    // event, and the host reads the copy of the previous body, so the GPU
+    // never waits for the host; a body past convergence only runs kernels
+    // that return at once.
     Graph G;
-    PP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    try {
-        body();
-    } catch (...) {
-        hipGraph_t tmp = nullptr;
-        (void)hipStreamEndCapture(ctx->stream, &tmp);
-        if (tmp) (void)hipGraphDestroy(tmp);
-        throw;
+    if (use_graph) {
+        PP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        PP_HIP(hipStreamEndCapture(ctx->stream, &G.g));
+        PP_HIP(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
     }
-    PP_HIP(hipStreamEndCapture(ctx->stream, &G.g));
-    PP_HIP(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
     int *slots = ctx->pinned_ctl;  // two C_COUNT-word slots
     hipEvent_t evs[2] = {ctx->ev_ring[0], ctx->ev_ring[1]};
     const uint64_t max_replays = (uint64_t)maxit / every + 2;
     uint64_t issued = 0, checked = 0;
     auto issue = [&] {
-        PP_HIP(hipGraphLaunch(G.exec, ctx->stream));
+        if (use_graph) PP_HIP(hipGraphLaunch(G.exec, ctx->stream));
+        else body();
         int *slot = slots + (issued & 1) * C_COUNT;
         PP_HIP(hipMemcpyAsync(slot, ctx->ctl, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         PP_HIP(hipEventRecord(evs[issued & 1], ctx->stream));
@@ -193,7 +197,8 @@ PcgResult pcg_mixed(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts &
         launch_fill_zero(ctx, e32, (size_t)n * sizeof(float));
         const double want = std::max(inner_tol, 0.5 * o.tol * bnorm / rnorm);
         PcgResult in = pcg_device<float>(ctx, A, M, o.flexible != 0, r32, e32, want,
-                                         std::max<uint32_t>(2, o.maxit - inner_total), o.check_every, false, W_R32);
+                                         std::max<uint32_t>(2, o.maxit - inner_total), o.check_every, false, W_R32,
+                                         o.launch != PP_LAUNCH_STREAM);
         inner_total += in.iters;
         ++outer;
         if (in.status != PP_OK && in.status != PP_ERR_NOT_FINITE) {
@@ -478,7 +483,8 @@ static int solve_impl(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts
                 res = pcg_mixed(ctx, As, M, o, bs, xs, R.outer_iters, R.inner_iters, rel);
             } else {
                 res = pcg_device<double>(ctx, As, M, o.flexible != 0, bs, xs, o.tol, o.maxit,
-                                         o.check_every ? o.check_every : 1, o.record_history != 0, W_R);
+                                         o.check_every ? o.check_every : 1, o.record_history != 0, W_R,
+                                         o.launch != PP_LAUNCH_STREAM);
             }
         } else {
             res = gmres_device(ctx, As, M, o, bs, xs);

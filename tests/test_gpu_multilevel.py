@@ -75,9 +75,11 @@ def permute(A, order):
     return rp2, np.concatenate(cols).astype(np.uint32), np.concatenate(vals)
 
 
+@pytest.mark.parametrize("shape", [(14, 13, 12), (26, 25, 24)])
 @pytest.mark.parametrize("kind,mat", [("ic0", "lap"), ("ilu0", "upw")])
-def test_multicolor_apply_bit_exact(pp, torch, kind, mat):
-    A = ppo.laplacian(14, 13, 12) if mat == "lap" else ppo.upwind(14, 13, 12, 20.0, 10.0, 5.0)
+def test_multicolor_apply_bit_exact(pp, torch, kind, mat, shape):
+    # (26, 25, 24): colour levels of ~7800 rows take the LDS-staged sweep kernel
+    A = ppo.laplacian(*shape) if mat == "lap" else ppo.upwind(*shape, 20.0, 10.0, 5.0)
     n = len(A[0]) - 1
     order, ncol = greedy_order(A)
     assert ncol == 2

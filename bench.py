@@ -34,6 +34,9 @@ def parse():
     ap.add_argument("--grid", type=int, default=GRID)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-natural", action="store_true", help="skip the natural-ordering record")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the secondary records (128^2 plain FP64, 100^3 upwind GMRES(30), 64^3 AMSED/GMSLR, "
+                         "256-system batch)")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--trsv", type=int, default=0, help="SpTRSV schedule: 0 auto, 1 levels, 2 sync-free")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -86,6 +89,129 @@ def cpu_baseline(A, b, n):
                        f"(THREAD_NUM), release flags; {out['iters']} iterations, setup {t1 - t0:.3f} s excluded"),
             "iters": out["iters"], "converged": out["converged"], "setup_s": round(t1 - t0, 4),
             "host_cpus": os.cpu_count()}
+
+
+def timed(torch, fn):
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t, out
+
+
+def extra_records(torch, pp, args):
+    """Secondary workloads, one record each (timed after one untimed run);
+    a failure is recorded as an error and never affects the headline."""
+    out = {}
+
+    def guard(name, fn):
+        try:
+            out[name] = fn()
+        except Exception as e:  # recorded, not raised
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+        log(f"[bench] {name}: {out[name]}")
+
+    def plain_2d():
+        A = pp.generateLaplacianCSR(128, 128, 1)
+        n = len(A[0]) - 1
+        M = pp.CSRMatrix(*A)
+        xt = torch.tensor(pp.uniform(n, 0), device="cuda")
+        b = torch.zeros(n, dtype=torch.float64, device="cuda")
+        M.MatPVec(xt, b)
+        P = pp.IncompleteCholesky(M, 1e-3)
+        P.setup()
+        def run():
+            cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=1000)
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            cg.solve(M, b, x)
+            return cg
+        timed(torch, run)
+        dt, cg = timed(torch, run)
+        return {"workload": "ICT(1e-3)-FlexibleCG FP64, 2D Poisson 128^2, natural order, b=A x_true U(-1,1) seed 0",
+                "s": round(dt, 5), "iters": cg.getNumIter(), "relres": cg.report.true_relres}
+
+    def upwind_gmres():
+        A = pp.generate_upwind(100, 100, 100, 20.0, 10.0, 5.0)
+        n = len(A[0]) - 1
+        M = pp.CSRMatrix(*A)
+        xt = torch.tensor(pp.uniform(n, 0), device="cuda")
+        b = torch.zeros(n, dtype=torch.float64, device="cuda")
+        M.MatPVec(xt, b)
+        P = pp.ILU0(M, make_f32=True, ordering=1)
+        P.setup()
+        rec = {"workload": "ILU(0) multicolour + FGMRES(30), 3D upwind 100^3, beta=(20,10,5)h, rtol 1e-8"}
+        for prec in ("fp64", "mixed"):
+            def run():
+                g = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000, precision=prec)
+                g.setRestart(True, 30)
+                x = torch.zeros(n, dtype=torch.float64, device="cuda")
+                g.solve(M, b, x)
+                return g
+            timed(torch, run)
+            dt, g = timed(torch, run)
+            rec[prec] = {"s": round(dt, 5), "iters": g.getNumIter(), "relres": g.report.true_relres,
+                         "converged": g.getConvergence()}
+        rec["speedup_mixed_vs_fp64"] = round(rec["fp64"]["s"] / rec["mixed"]["s"], 3)
+        return rec
+
+    def multilevel(kind):
+        if kind == "amsed":
+            A = pp.generateLaplacianCSR(64, 64, 64)
+        else:
+            A = pp.generate_upwind(64, 64, 64, 20.0, 10.0, 5.0)
+        n = len(A[0]) - 1
+        M = pp.CSRMatrix(*A)
+        xt = torch.tensor(pp.uniform(n, 0), device="cuda")
+        b = torch.zeros(n, dtype=torch.float64, device="cuda")
+        M.MatPVec(xt, b)
+        t0 = time.perf_counter()
+        P = (pp.AMSED if kind == "amsed" else pp.GMSLR)(M, levels=3, ndom=16, rank=32)
+        P.setup()
+        setup = time.perf_counter() - t0
+        sizes, ranks = P.info_levels()
+        def run():
+            if kind == "amsed":
+                s = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000)
+            else:
+                s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000)
+                s.setRestart(True, 30)
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            s.solve(M, b, x)
+            return s
+        timed(torch, run)
+        dt, s = timed(torch, run)
+        return {"workload": f"{kind.upper()} 3 levels, 16 subdomains, rank 32, 64^3 "
+                            f"{'Poisson, FlexibleCG' if kind == 'amsed' else 'upwind (20,10,5)h, FGMRES(30)'} FP64",
+                "s": round(dt, 5), "setup_s": round(setup, 3), "iters": s.getNumIter(),
+                "relres": s.report.true_relres, "level_sizes": sizes, "ranks": ranks}
+
+    def batch():
+        nsys = 256
+        sides = [32 + int(225 * u) for u in (pp.uniform(nsys, 7) + 1) / 2]
+        systems = [pp.generate_vardiff2d(s, np.exp(pp.normal(s * s, 1000 + i))) for i, s in enumerate(sides)]
+        t0 = time.perf_counter()
+        B = pp.BatchedSystems(systems)
+        setup = time.perf_counter() - t0
+        blk = B.tune()
+        b = torch.tensor(pp.normal(B.n, 5), device="cuda")
+        def run():
+            x = torch.zeros_like(b)
+            its = B.solve(b, x, precision="mixed", tol=1e-8)
+            return its
+        timed(torch, run)
+        dt, its = timed(torch, run)
+        return {"workload": f"{nsys} 2D variable-coefficient diffusion systems, side U[32,256] (sum n={B.n}), "
+                            "one persistent IC(0)-PCG launch, mixed precision, rtol 1e-8",
+                "s": round(dt, 5), "setup_s": round(setup, 3), "block": blk, "iters_max": int(its.max()),
+                "iters_mean": float(its.mean()), "relres": B.report.true_relres,
+                "converged": bool(B.report.converged)}
+
+    guard("plain_fp64_2d_128", plain_2d)
+    guard("upwind_gmres_100", upwind_gmres)
+    guard("amsed_64", lambda: multilevel("amsed"))
+    guard("gmslr_64", lambda: multilevel("gmslr"))
+    guard("batch_256", batch)
+    return out
 
 
 def main():
@@ -205,6 +331,8 @@ def main():
         log(f"[bench] natural ordering: fp64 {dn64:.4f} s ({rn64.iters} it), mixed {dnmx:.4f} s")
         del Pn
 
+    extra = None if args.no_extra else extra_records(torch, pp, args)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -241,6 +369,7 @@ def main():
                               "frac": round(gbs(spmv32_bytes, spmv32_ms) / HBM_PEAK_GBS, 4),
                               "bytes_per_launch": spmv32_bytes, "avg_us": round(spmv32_ms * 1e3, 2)},
         "cpu_baseline": cpu,
+        "secondary": extra,
     }
     print(json.dumps(out), flush=True)
 

@@ -96,9 +96,21 @@ struct pp_ctx {
     void *ws[32] = {};             // cached solver workspaces
     size_t ws_bytes[32] = {};
     long long spin_deadline = 0;   // wall_clock64 ticks a sync-free wait may last
+    // while set (StopScope), SpMV / dot / GEMV / convert kernels return at
+    // once when the word is non-zero, like the sweeps and the PCG kernels
+    const int *stop_word = nullptr;
 };
 
 namespace pp {
+
+// Kernels launched inside the scope skip their work once ctl[C_STOP] is set
+// (iteration bodies issued ahead of the host's convergence check).
+struct StopScope {
+    pp_ctx *c;
+    const int *prev;
+    explicit StopScope(pp_ctx *ctx);
+    ~StopScope() { c->stop_word = prev; }
+};
 
 // Device-side copy of a gather-form triangular sweep, rows stored in level order.
 template <class T>
@@ -213,6 +225,31 @@ HostCsr transpose(const HostCsr &A);
 std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors);
 HostCsr permute_symmetric(const HostCsr &A, const std::vector<uint32_t> &order);
 pp_csr *precond_permuted_matrix(pp_ctx *ctx, pp_precond *M, pp_csr *A);
+
+// ---- tall-skinny GEMVs and GMRES helpers (pp_kernels_krylov.hip) ----
+// device coefficient buffer: [0,320) h, [320,640) y or a copy of h, then partials
+double *gmres_coeff_buffer(pp_ctx *ctx);
+// dest[c] = sum_r V[r, c] w[r] for c < ncol (dest NULL: the coefficient buffer)
+void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w,
+                   double *dest);
+void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h, double *w);
+void launch_gemv_add(pp_ctx *ctx, uint32_t n, uint32_t ncol, const float *Z32, const double *Z64, size_t ld,
+                     const double *y, double *x);
+void read_slots(pp_ctx *ctx, int first, int count, double *out);
+// Device-resident Givens least-squares of FlexibleGMRES(m): H ((m+1) x m,
+// column-major), c, s, g, the two CGS passes' coefficients h1, h2, and
+// scalars misc[kGmMisc]
+struct GmresDev {
+    double *H = nullptr, *c = nullptr, *s = nullptr, *g = nullptr, *h1 = nullptr, *h2 = nullptr, *misc = nullptr;
+    uint32_t mm = 0;
+};
+enum GmMisc { GM_INVH = 0, GM_RELERR, GM_RNORM, GM_M, GM_STATUS, GM_CONVCYC, GM_BNORM, GM_TOL, GM_NREC, kGmMisc = 16 };
+GmresDev gmres_state(pp_ctx *ctx, uint32_t mm);
+void launch_gmres_cycle_init(pp_ctx *ctx, const GmresDev &st, double beta, double bnorm, double tol);
+void launch_gmres_step(pp_ctx *ctx, const GmresDev &st, uint32_t j, int quirks, double *hist, uint32_t hist_idx,
+                       uint32_t hist_cap);
+void launch_gmres_next_basis(pp_ctx *ctx, const GmresDev &st, uint32_t n, const double *w, double *vnext);
+void launch_gmres_backsolve(pp_ctx *ctx, const GmresDev &st, int quirks, double *y);
 void launch_permute(pp_ctx *ctx, uint32_t n, const uint32_t *perm, int dtype, const void *src, void *dst,
                     bool gather);
 

@@ -94,9 +94,10 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
                                               T *__restrict__ y, const uint32_t *__restrict__ rb,
                                               uint32_t nb, T alpha, T beta, int beta_zero,
                                               const T *__restrict__ dotw, double *partials,
-                                              unsigned *counter, double *result) {
+                                              unsigned *counter, double *result, const int *stop) {
     __shared__ T prod[kSpmvNnzBlock];
     __shared__ double sh[4];
+    if (stop && *stop) return;
     double dacc = 0.0;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const uint32_t r0 = rb[b], r1 = rb[b + 1];
@@ -184,20 +185,23 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
         if (dot_slot >= 0)
             k_spmv<double, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v64, (const double *)x,
                                                                  (double *)y, A->row_blocks, A->nblocks, alpha, beta, bz,
-                                                                 (const double *)dot_with, part, cnt, res);
+                                                                 (const double *)dot_with, part, cnt, res, ctx->stop_word);
         else
             k_spmv<double, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v64, (const double *)x,
                                                                   (double *)y, A->row_blocks, A->nblocks, alpha, beta,
-                                                                  bz, nullptr, nullptr, nullptr, nullptr);
+                                                                  bz, nullptr, nullptr, nullptr, nullptr,
+                                                                  ctx->stop_word);
     } else {
         if (dot_slot >= 0)
             k_spmv<float, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v32, (const float *)x,
                                                                 (float *)y, A->row_blocks, A->nblocks, (float)alpha,
-                                                                (float)beta, bz, (const float *)dot_with, part, cnt, res);
+                                                                (float)beta, bz, (const float *)dot_with, part, cnt, res,
+                                                                ctx->stop_word);
         else
             k_spmv<float, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v32, (const float *)x,
                                                                  (float *)y, A->row_blocks, A->nblocks, (float)alpha,
-                                                                 (float)beta, bz, nullptr, nullptr, nullptr, nullptr);
+                                                                 (float)beta, bz, nullptr, nullptr, nullptr, nullptr,
+                                                                 ctx->stop_word);
     }
     PP_HIP(hipGetLastError());
 }
@@ -218,20 +222,21 @@ void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes) {
 }
 
 template <class S, class D>
-__global__ void k_convert(uint32_t n, const S *s, D *d) {
+__global__ void k_convert(uint32_t n, const S *s, D *d, const int *stop) {
+    if (stop && *stop) return;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] = (D)s[i];
 }
 
 void launch_convert(pp_ctx *ctx, uint32_t n, int sdt, const void *s, int ddt, void *d) {
     unsigned g = grid_for(ctx, n);
     if (sdt == PP_F64 && ddt == PP_F32)
-        k_convert<double, float><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)s, (float *)d);
+        k_convert<double, float><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)s, (float *)d, ctx->stop_word);
     else if (sdt == PP_F32 && ddt == PP_F64)
-        k_convert<float, double><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)s, (double *)d);
+        k_convert<float, double><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)s, (double *)d, ctx->stop_word);
     else if (sdt == PP_F64)
-        k_convert<double, double><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)s, (double *)d);
+        k_convert<double, double><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)s, (double *)d, ctx->stop_word);
     else
-        k_convert<float, float><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)s, (float *)d);
+        k_convert<float, float><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)s, (float *)d, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
 
@@ -262,8 +267,9 @@ void launch_axpby(pp_ctx *ctx, int dtype, uint32_t n, double a, const void *x, d
 // across threads/workgroups.
 template <class T>
 __global__ __launch_bounds__(256) void k_dot(uint32_t n, const T *x, const T *y, double *partials,
-                                             unsigned *counter, double *result) {
+                                             unsigned *counter, double *result, const int *stop) {
     __shared__ double sh[4];
+    if (stop && *stop) return;
     T acc = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += x[i] * y[i];
     double bs = block_sum((double)acc, sh);
@@ -277,10 +283,10 @@ void launch_dot(pp_ctx *ctx, int dtype, uint32_t n, const void *x, const void *y
     unsigned *cnt = ctx->red.counters + slot;
     if (dtype == PP_F64)
         k_dot<double><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)x, (const double *)y, part, cnt,
-                                                     ctx->scal + slot);
+                                                     ctx->scal + slot, ctx->stop_word);
     else
         k_dot<float><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)x, (const float *)y, part, cnt,
-                                                    ctx->scal + slot);
+                                                    ctx->scal + slot, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
 
@@ -668,6 +674,8 @@ __global__ void k_ctl_init(int *ctl, int first_check, int last) {
         ctl[C_LAST] = last;
     }
 }
+
+StopScope::StopScope(pp_ctx *ctx) : c(ctx), prev(ctx->stop_word) { ctx->stop_word = ctx->ctl + C_STOP; }
 
 void ctl_init(pp_ctx *ctx, int first_check, int last) {
     k_ctl_init<<<1, 64, 0, ctx->stream>>>(ctx->ctl, first_check, last);

@@ -27,8 +27,10 @@ double *gmres_coeff_buffer(pp_ctx *ctx) {
 
 // partial[c][block] = sum over the block's rows of V[r, c] * w[r]
 __global__ __launch_bounds__(256) void k_gemv_t_partial(uint32_t n, uint32_t ncol, const double *__restrict__ V,
-                                                        size_t ld, const double *__restrict__ w, double *partial) {
+                                                        size_t ld, const double *__restrict__ w, double *partial,
+                                                        const int *stop) {
     __shared__ double sh[4];
+    if (stop && *stop) return;
     const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const uint32_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
     for (uint32_t c = 0; c < ncol; ++c) {
@@ -44,8 +46,10 @@ __global__ __launch_bounds__(256) void k_gemv_t_partial(uint32_t n, uint32_t nco
 }
 
 // h[c] = sum_b partial[c][b] in block order (deterministic)
-__global__ __launch_bounds__(256) void k_gemv_t_reduce(uint32_t nb, const double *partial, double *h) {
+__global__ __launch_bounds__(256) void k_gemv_t_reduce(uint32_t nb, const double *partial, double *h,
+                                                       const int *stop) {
     __shared__ double sh[4];
+    if (stop && *stop) return;
     const uint32_t c = blockIdx.x;
     double acc = 0.0;
     for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) acc += partial[(size_t)c * nb + b];
@@ -55,18 +59,21 @@ __global__ __launch_bounds__(256) void k_gemv_t_reduce(uint32_t nb, const double
     if (threadIdx.x == 0) h[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w, int) {
+void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w,
+                   double *dest) {
     PP_CHECK(ncol <= kMaxCols, PP_ERR_UNSUPPORTED, "too many columns");
     double *buf = gmres_coeff_buffer(ctx);
     double *partial = buf + 640;
-    k_gemv_t_partial<<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial);
-    k_gemv_t_reduce<<<ncol, 256, 0, ctx->stream>>>(kGemvBlocks, partial, buf);
+    k_gemv_t_partial<<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
+    k_gemv_t_reduce<<<ncol, 256, 0, ctx->stream>>>(kGemvBlocks, partial, dest ? dest : buf, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
 
 // w[r] -= sum_c V[r, c] h[c]
 __global__ __launch_bounds__(256) void k_gemv_sub(uint32_t n, uint32_t ncol, const double *__restrict__ V,
-                                                  size_t ld, const double *__restrict__ h, double *__restrict__ w) {
+                                                  size_t ld, const double *__restrict__ h, double *__restrict__ w,
+                                                  const int *stop) {
+    if (stop && *stop) return;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
         double acc = 0.0;
         for (uint32_t c = 0; c < ncol; ++c) acc += V[ld * c + r] * h[c];
@@ -76,7 +83,7 @@ __global__ __launch_bounds__(256) void k_gemv_sub(uint32_t n, uint32_t ncol, con
 
 void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h, double *w) {
     unsigned g = std::min<uint32_t>((n + 255) / 256, 8192);
-    k_gemv_sub<<<g, 256, 0, ctx->stream>>>(n, ncol, V, ld, h, w);
+    k_gemv_sub<<<g, 256, 0, ctx->stream>>>(n, ncol, V, ld, h, w, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
 
@@ -129,6 +136,166 @@ void read_slots(pp_ctx *ctx, int first, int count, double *out) {
     PP_HIP(hipMemcpyAsync(ctx->pinned, src, count * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->stream));
     std::copy(ctx->pinned, ctx->pinned + count, out);
+}
+
+}  // namespace pp
+
+// ---------------------------------------------------------------------------
+// Device-resident least-squares of FlexibleGMRES(m): the Hessenberg column,
+// the Givens rotations (GMRES.cpp:72-101), the convergence test and the
+// back substitution (GMRES.cpp:103-119) run in single-thread kernels on the
+// stream, so an Arnoldi cycle is issued without any host round trip; the
+// stop word ends the cycle's remaining kernels once the estimate converges.
+// ---------------------------------------------------------------------------
+namespace pp {
+
+GmresDev gmres_state(pp_ctx *ctx, uint32_t mm) {
+    const size_t ld = mm + 1;
+    const size_t total = ld * mm + 5 * ld + kGmMisc;
+    double *base = (double *)ws_get(ctx, 30, total * sizeof(double));
+    GmresDev st;
+    st.mm = mm;
+    st.H = base;
+    st.c = st.H + ld * mm;
+    st.s = st.c + ld;
+    st.g = st.s + ld;
+    st.h1 = st.g + ld;
+    st.h2 = st.h1 + ld;
+    st.misc = st.h2 + ld;
+    return st;
+}
+
+namespace {
+
+__device__ void dev_givens(const GmresDev &st, uint32_t j) {
+#pragma clang fp contract(off)
+    const uint32_t ld = st.mm + 1;
+    double *H = st.H;
+    for (uint32_t i = 0; i < j; ++i) {
+        const double h1 = H[(size_t)j * ld + i], h2 = H[(size_t)j * ld + i + 1];
+        H[(size_t)j * ld + i + 1] = st.c[i] * h2 - st.s[i] * h1;
+        H[(size_t)j * ld + i] = st.c[i] * h1 + st.s[i] * h2;
+    }
+    const double h1 = H[(size_t)j * ld + j], h2 = H[(size_t)j * ld + j + 1];
+    if (fabs(h1) > fabs(h2)) {
+        const double t = h2 / h1;
+        st.c[j] = 1.0 / sqrt(fabs(1.0 + t * t));
+        st.s[j] = st.c[j] * t;
+    } else {
+        const double t = h1 / h2;
+        st.s[j] = 1.0 / sqrt(fabs(1.0 + t * t));
+        st.c[j] = st.s[j] * t;
+    }
+    H[(size_t)j * ld + j] = st.c[j] * h1 + st.s[j] * h2;
+    H[(size_t)j * ld + j + 1] = 0.0;
+    st.g[j + 1] = -st.s[j] * st.g[j];
+    st.g[j] = st.c[j] * st.g[j];
+}
+
+__global__ void k_gmres_cycle_init(GmresDev st, double beta, double bnorm, double tol) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (uint32_t i = 0; i < st.mm; ++i) st.g[i] = 0.0;  // g[mm] keeps its value (GMRES.cpp:160)
+    st.g[0] = beta;
+    st.misc[GM_M] = st.mm;
+    st.misc[GM_STATUS] = 0.0;
+    st.misc[GM_CONVCYC] = 0.0;
+    st.misc[GM_NREC] = 0.0;
+    st.misc[GM_BNORM] = bnorm;
+    st.misc[GM_TOL] = tol;
+}
+
+__global__ void k_gmres_step(GmresDev st, uint32_t j, int quirks, const double *wn2, int *ctl, double *hist,
+                             uint32_t hist_idx, uint32_t hist_cap) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (ctl[C_STOP]) return;
+    const uint32_t ld = st.mm + 1;
+    const double hn = sqrt(*wn2);
+    for (uint32_t i = 0; i <= j; ++i) st.H[(size_t)j * ld + i] = st.h1[i] + st.h2[i];
+    st.H[(size_t)j * ld + j + 1] = hn;
+    const double bnorm = st.misc[GM_BNORM];
+    if (!(hn == hn)) {
+        st.misc[GM_STATUS] = PP_ERR_NOT_FINITE;
+        st.misc[GM_M] = j;
+        ctl[C_STOP] = 1;
+        return;
+    }
+    if (fabs(hn) == 0.0) {  // happy breakdown (GMRES.cpp:183-186)
+        st.misc[GM_M] = quirks ? j : j + 1;
+        if (!quirks) {
+            dev_givens(st, j);
+            const double rnorm = fabs(st.g[j + 1]);
+            st.misc[GM_RNORM] = rnorm;
+            st.misc[GM_RELERR] = rnorm / bnorm;
+            st.misc[GM_CONVCYC] = 1.0;
+        }
+        ctl[C_STOP] = 1;
+        return;
+    }
+    st.misc[GM_INVH] = 1.0 / hn;
+    dev_givens(st, j);
+    const double rnorm = fabs(st.g[j + 1]);
+    const double relerr = rnorm / bnorm;
+    st.misc[GM_RNORM] = rnorm;
+    st.misc[GM_RELERR] = relerr;
+    if (hist && hist_idx < hist_cap) hist[hist_idx] = relerr;
+    st.misc[GM_NREC] += 1.0;
+    if (relerr < st.misc[GM_TOL]) {
+        st.misc[GM_M] = quirks ? j : j + 1;  // the reference drops column j
+        st.misc[GM_CONVCYC] = 1.0;
+        ctl[C_STOP] = 1;
+    }
+}
+
+// v_{j+1} = (1/h_{j+1,j}) w, evaluated like HostVector::add with b = 0
+__global__ void k_gmres_next_basis(uint32_t n, const double *misc, const double *__restrict__ w,
+                                   double *__restrict__ v, const int *stop) {
+#pragma clang fp contract(off)
+    if (stop && *stop) return;
+    const double a = misc[GM_INVH];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double t1 = a * w[i];
+        const double t2 = 0.0 * w[i];
+        v[i] = t1 + t2;
+    }
+}
+
+__global__ void k_gmres_backsolve(GmresDev st, int quirks, double *y) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t ld = st.mm + 1;
+    const uint32_t m = (uint32_t)st.misc[GM_M];
+    for (int i = (int)m - 1; i >= 0; --i) {
+        double t = st.g[i];
+        for (uint32_t j = i + 1; j < m; ++j) t -= st.H[(size_t)j * ld + i] * st.g[j];
+        st.g[i] = t / st.H[(size_t)i * ld + i];
+    }
+    st.g[m] = 0.0;
+    for (uint32_t i = 0; i <= st.mm; ++i) y[i] = (!quirks && i >= m) ? 0.0 : st.g[i];
+}
+
+}  // namespace
+
+void launch_gmres_cycle_init(pp_ctx *ctx, const GmresDev &st, double beta, double bnorm, double tol) {
+    k_gmres_cycle_init<<<1, 64, 0, ctx->stream>>>(st, beta, bnorm, tol);
+    PP_HIP(hipGetLastError());
+}
+
+void launch_gmres_step(pp_ctx *ctx, const GmresDev &st, uint32_t j, int quirks, double *hist, uint32_t hist_idx,
+                       uint32_t hist_cap) {
+    k_gmres_step<<<1, 64, 0, ctx->stream>>>(st, j, quirks, ctx->scal + S_TMP2, ctx->ctl, hist, hist_idx, hist_cap);
+    PP_HIP(hipGetLastError());
+}
+
+void launch_gmres_next_basis(pp_ctx *ctx, const GmresDev &st, uint32_t n, const double *w, double *vnext) {
+    const unsigned g = std::min<uint32_t>((n + 255) / 256, 4096);
+    k_gmres_next_basis<<<g, 256, 0, ctx->stream>>>(n, st.misc, w, vnext, ctx->stop_word);
+    PP_HIP(hipGetLastError());
+}
+
+void launch_gmres_backsolve(pp_ctx *ctx, const GmresDev &st, int quirks, double *y) {
+    k_gmres_backsolve<<<1, 64, 0, ctx->stream>>>(st, quirks, y);
+    PP_HIP(hipGetLastError());
 }
 
 }  // namespace pp

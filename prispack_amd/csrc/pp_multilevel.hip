@@ -25,12 +25,6 @@
 namespace pp {
 
 void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
-void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w, int);
-void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h, double *w);
-void launch_gemv_add(pp_ctx *ctx, uint32_t n, uint32_t ncol, const float *Z32, const double *Z64, size_t ld,
-                     const double *y, double *x);
-void read_slots(pp_ctx *ctx, int first, int count, double *out);
-double *gmres_coeff_buffer(pp_ctx *ctx);
 
 namespace {
 
@@ -227,7 +221,7 @@ void lowrank_lanczos(pp_ctx *ctx, MlState &S, int l, uint32_t rank, uint32_t ste
             launch_axpby(ctx, PP_F64, n, 1.0, u, -a, zj, u);
             // u -= Z (V^T u), twice for stability
             for (int pass = 0; pass < 2; ++pass) {
-                launch_gemv_t(ctx, n, j + 1, V, n, u, 0);
+                launch_gemv_t(ctx, n, j + 1, V, n, u, nullptr);
                 launch_gemv_sub(ctx, n, j + 1, Z, n, gmres_coeff_buffer(ctx), u);
             }
             done = j + 1;
@@ -339,7 +333,7 @@ void lowrank_arnoldi(pp_ctx *ctx, MlState &S, int l, uint32_t rank, uint32_t ste
             launch_spmv(ctx, L.F, PP_F64, 1.0, tb, 0.0, w, -1, nullptr, false);
             rec_solve(ctx, S, l, w);
             for (int pass = 0; pass < 2; ++pass) {
-                launch_gemv_t(ctx, n, j + 1, V, n, w, 0);
+                launch_gemv_t(ctx, n, j + 1, V, n, w, nullptr);
                 PP_HIP(hipMemcpyAsync(buf + 320, buf, (j + 1) * 8, hipMemcpyDeviceToDevice, ctx->stream));
                 launch_gemv_sub(ctx, n, j + 1, V, n, buf + 320, w);
                 read_slots(ctx, -1, j + 1, pass == 0 ? h1.data() : h2.data());

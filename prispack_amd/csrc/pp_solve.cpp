@@ -15,13 +15,6 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
 void precond_need_f32(pp_precond *M);
 void launch_scale_convert(pp_ctx *ctx, uint32_t n, const double *src, double scale, float *dst);
 void launch_axpy_f32_to_f64(pp_ctx *ctx, uint32_t n, double a, const float *x, double *y);
-void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w, int slot0);
-void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h,
-                     double *w);
-void launch_gemv_add(pp_ctx *ctx, uint32_t n, uint32_t ncol, const float *Z32, const double *Z64, size_t ld,
-                     const double *y, double *x);
-void read_slots(pp_ctx *ctx, int first, int count, double *out);
-double *gmres_coeff_buffer(pp_ctx *ctx);
 
 namespace {
 
@@ -71,6 +64,7 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
     int ctl[C_COUNT];
     const uint32_t every = std::max<uint32_t>(1, check_every);
     auto body = [&] {
+        StopScope scope(ctx);  // a body past convergence returns at once, SpMV included
         for (uint32_t k = 0; k < every; ++k) {
             if (flexible && M) precond_apply(ctx, M, DT, r, w);
             else PP_HIP(hipMemcpyAsync(w, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
@@ -282,17 +276,14 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
     launch_fill_zero(ctx, V, ld * (mm + 1) * sizeof(double));
     double *w = (double *)ws_get(ctx, W_W, (size_t)n * sizeof(double));
     double *r = (double *)ws_get(ctx, W_R, (size_t)n * sizeof(double));
-    float *t32 = mixed ? (float *)ws_get(ctx, W_T32, (size_t)n * sizeof(float)) : nullptr;
     float *w32 = mixed ? (float *)ws_get(ctx, W_W32, (size_t)n * sizeof(float)) : nullptr;
     double *hdev = gmres_coeff_buffer(ctx);
 
-    Givens G;
-    G.ld = mm + 1;
-    G.H.assign((size_t)(mm + 1) * mm, 0.0);
-    G.c.assign(mm + 1, 0.0);
-    G.s.assign(mm + 1, 0.0);
-    G.g.assign(mm + 1, 0.0);
-    std::vector<double> hcol(mm + 1), hcol2(mm + 1);
+    const GmresDev st = gmres_state(ctx, mm);
+    const bool hist_on = o.record_history != 0;
+    const uint32_t hist_cap = (uint32_t)(maxcyc + 1) * mm + 2;
+    if (hist_on) ensure_hist(ctx, hist_cap);
+    uint32_t nrec = 0;
     ctx->last_hist.clear();
     ctl_init(ctx, 1, 0);  // the sweeps honour the stop word: clear what a previous solve left
 
@@ -318,79 +309,67 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
         res.final_error = beta / (bnorm > 0 ? bnorm : 1.0);
         return res;
     }
+    const int quirks = o.reference_quirks ? 1 : 0;
     while (!conv && k < maxcyc) {
         ++k;
+        // One Arnoldi cycle, issued without host round trips: the Hessenberg
+        // column, Givens rotations and convergence test run on the device
+        // (k_gmres_step) and the stop word ends the cycle's remaining kernels.
+        ctl_init(ctx, 1, 0);
+        launch_gmres_cycle_init(ctx, st, beta, bnorm, o.tol);
         launch_axpby(ctx, PP_F64, n, 1.0 / beta, r, 0.0, r, V);  // v0 = r / beta
-        std::fill(G.g.begin(), G.g.begin() + mm, 0.0);
-        G.g[0] = beta;
-        bool converged_cycle = false;
-        for (uint32_t j = 0; j < m; ++j) {
-            double *vj = V + ld * j;
-            // z_j = M^{-1} v_j ; w = A z_j
-            if (mixed) {
-                float *zj = Z32 + ld * j;
-                launch_convert(ctx, n, PP_F64, vj, PP_F32, zj);
-                if (flexible && M) precond_apply(ctx, M, PP_F32, zj, zj);
-                launch_spmv(ctx, A, PP_F32, 1.0, zj, 0.0, w32, -1, nullptr, false);
-                launch_convert(ctx, n, PP_F32, w32, PP_F64, w);
-            } else if (flexible) {
-                double *zj = Z + ld * j;
-                if (M) precond_apply(ctx, M, PP_F64, vj, zj);
-                else PP_HIP(hipMemcpyAsync(zj, vj, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
-                launch_spmv(ctx, A, PP_F64, 1.0, zj, 0.0, w, -1, nullptr, false);
-            } else {
-                launch_spmv(ctx, A, PP_F64, 1.0, vj, 0.0, w, -1, nullptr, false);
-            }
-            // CGS2: h = V^T w ; w -= V h ; twice
-            for (int pass = 0; pass < 2; ++pass) {
-                launch_gemv_t(ctx, n, j + 1, V, ld, w, 0);
-                PP_HIP(hipMemcpyAsync(hdev + 320, hdev, (j + 1) * sizeof(double), hipMemcpyDeviceToDevice,
-                                      ctx->stream));
-                launch_gemv_sub(ctx, n, j + 1, V, ld, hdev + 320, w);
-                read_slots(ctx, -1, j + 1, pass == 0 ? hcol.data() : hcol2.data());
-            }
-            launch_dot(ctx, PP_F64, n, w, w, S_TMP2);
-            const double hn = std::sqrt(read_slot(ctx, S_TMP2));
-            for (uint32_t i = 0; i <= j; ++i) G.h(i, j) = hcol[i] + hcol2[i];
-            G.h(j + 1, j) = hn;
-            if (!(hn == hn)) {
-                res.status = PP_ERR_NOT_FINITE;
-                m = j;
-                break;
-            }
-            if (std::fabs(hn) == 0) {  // happy breakdown (GMRES.cpp:183-186)
-                m = o.reference_quirks ? j : j + 1;
-                if (!o.reference_quirks) {
-                    givens_step(G, j);
-                    rnorm = std::fabs(G.g[j + 1]);
-                    relerr = rnorm / bnorm;
-                    converged_cycle = true;
+        {
+            StopScope scope(ctx);
+            for (uint32_t j = 0; j < m; ++j) {
+                double *vj = V + ld * j;
+                // z_j = M^{-1} v_j ; w = A z_j
+                if (mixed) {
+                    float *zj = Z32 + ld * j;
+                    launch_convert(ctx, n, PP_F64, vj, PP_F32, zj);
+                    if (flexible && M) precond_apply(ctx, M, PP_F32, zj, zj);
+                    launch_spmv(ctx, A, PP_F32, 1.0, zj, 0.0, w32, -1, nullptr, false);
+                    launch_convert(ctx, n, PP_F32, w32, PP_F64, w);
+                } else if (flexible) {
+                    double *zj = Z + ld * j;
+                    if (M) precond_apply(ctx, M, PP_F64, vj, zj);
+                    else launch_convert(ctx, n, PP_F64, vj, PP_F64, zj);
+                    launch_spmv(ctx, A, PP_F64, 1.0, zj, 0.0, w, -1, nullptr, false);
+                } else {
+                    launch_spmv(ctx, A, PP_F64, 1.0, vj, 0.0, w, -1, nullptr, false);
                 }
-                break;
-            }
-            launch_axpby(ctx, PP_F64, n, 1.0 / hn, w, 0.0, w, V + ld * (j + 1));
-            givens_step(G, j);
-            rnorm = std::fabs(G.g[j + 1]);
-            relerr = rnorm / bnorm;
-            ctx->last_hist.push_back(relerr);
-            if (relerr < o.tol) {
-                m = o.reference_quirks ? j : j + 1;  // the reference drops column j
-                converged_cycle = true;
-                break;
+                // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
+                launch_gemv_t(ctx, n, j + 1, V, ld, w, st.h1);
+                launch_gemv_sub(ctx, n, j + 1, V, ld, st.h1, w);
+                launch_gemv_t(ctx, n, j + 1, V, ld, w, st.h2);
+                launch_gemv_sub(ctx, n, j + 1, V, ld, st.h2, w);
+                launch_dot(ctx, PP_F64, n, w, w, S_TMP2);
+                launch_gmres_step(ctx, st, j, quirks, hist_on ? ctx->hist : nullptr, nrec + j, hist_cap);
+                launch_gmres_next_basis(ctx, st, n, w, V + ld * (j + 1));
             }
         }
-        (void)converged_cycle;
-        if (res.status != PP_OK) break;
-        back_solve(G, m);
-        // x += Z g over all mm+1 columns (GMRES.cpp:204-208); the entry past
-        // the back-solve is the residual estimate, which the reference leaves
-        // in place in its converged cycle
-        double *y = ctx->pinned + 320;  // pinned staging (read_slots uses [0, 320))
-        std::copy(G.g.begin(), G.g.end(), y);
-        if (!o.reference_quirks)
-            for (uint32_t i = m; i <= mm; ++i) y[i] = 0.0;
-        PP_HIP(hipMemcpyAsync(hdev + 320, y, (mm + 1) * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-        launch_gemv_add(ctx, n, mm + 1, Z32, mixed ? nullptr : (flexible ? Z : V), ld, hdev + 320, x);
+        double misc[kGmMisc];
+        PP_HIP(hipMemcpyAsync(ctx->pinned, st.misc, kGmMisc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        std::copy(ctx->pinned, ctx->pinned + kGmMisc, misc);
+        {
+            int ctl[C_COUNT];
+            read_ctl(ctx, ctl);
+            PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
+        }
+        nrec += (uint32_t)misc[GM_NREC];
+        m = (uint32_t)misc[GM_M];
+        rnorm = misc[GM_RNORM];
+        relerr = misc[GM_RELERR];
+        if ((int)misc[GM_STATUS] != PP_OK) {
+            res.status = (int)misc[GM_STATUS];
+            break;
+        }
+        // x += Z y over all mm+1 columns (GMRES.cpp:204-208); in the
+        // reference's converged cycle the entry past the back-solve is the
+        // residual estimate, left in place
+        double *y = hdev + 320;
+        launch_gmres_backsolve(ctx, st, quirks, y);
+        launch_gemv_add(ctx, n, mm + 1, Z32, mixed ? nullptr : (flexible ? Z : V), ld, y, x);
         conv = relerr < o.tol;
         PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         launch_spmv(ctx, A, PP_F64, -1.0, x, 1.0, r, -1, nullptr, false);
@@ -405,6 +384,11 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
         if (!conv && k < maxcyc) {
             if (!o.reference_quirks) m = mm;
         }
+    }
+    if (hist_on && nrec) {
+        const uint32_t cnt = std::min(nrec, ctx->hist_cap);
+        ctx->last_hist.assign(cnt, 0.0);
+        PP_HIP(hipMemcpy(ctx->last_hist.data(), ctx->hist, cnt * sizeof(double), hipMemcpyDeviceToHost));
     }
     {
         int ctl[C_COUNT];

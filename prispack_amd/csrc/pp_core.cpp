@@ -228,8 +228,7 @@ int pp_ctx_create(int device, pp_ctx **out) {
         PP_HIP(hipHostMalloc((void **)&c->pinned_ctl, 64 * sizeof(int), hipHostMallocDefault));
         PP_HIP(hipEventCreate(&c->ev0));
         PP_HIP(hipEventCreate(&c->ev1));
-        PP_HIP(hipEventCreateWithFlags(&c->ev_ring[0], hipEventDisableTiming));
-        PP_HIP(hipEventCreateWithFlags(&c->ev_ring[1], hipEventDisableTiming));
+        for (hipEvent_t &e : c->ev_ring) PP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         ensure_hist(c.get(), 4096);
         PP_HIP(hipDeviceSynchronize());
         *out = c.release();

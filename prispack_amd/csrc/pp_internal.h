@@ -92,7 +92,8 @@ struct pp_ctx {
     int *pinned_ctl = nullptr;    // pinned host mirror of ctl
     std::vector<double> last_hist;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t ev_ring[2] = {nullptr, nullptr};  // in-flight iteration fences (no timing)
+    static constexpr int kRing = 4;  // iteration bodies kept in flight (pinned_ctl holds kRing slots)
+    hipEvent_t ev_ring[kRing] = {};   // their fences (no timing)
     void *ws[32] = {};             // cached solver workspaces
     size_t ws_bytes[32] = {};
     long long spin_deadline = 0;   // wall_clock64 ticks a sync-free wait may last

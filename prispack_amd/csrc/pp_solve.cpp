@@ -35,6 +35,21 @@ struct PcgResult {
     int nhist = 0;
 };
 
+// Wait for an event by polling it (no interrupt wake-up); after 20 ms of
+// polling, fall back to a blocking wait.
+void wait_event(hipEvent_t ev) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        const hipError_t e = hipEventQuery(ev);
+        if (e == hipSuccess) return;
+        if (e != hipErrorNotReady) PP_HIP(e);
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+            PP_HIP(hipEventSynchronize(ev));
+            return;
+        }
+    }
+}
+
 // FlexibleCG on the device (source/Solver/CG.cpp:33-83), working precision T.
 // A, M already hold T copies.  x holds x0.  One v = A d per iteration: the
 // reference's repeated product at CG.cpp:68 returns the same vector.
@@ -95,27 +110,29 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
         PP_HIP(hipStreamEndCapture(ctx->stream, &G.g));
         PP_HIP(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
     }
-    int *slots = ctx->pinned_ctl;  // two C_COUNT-word slots
-    hipEvent_t evs[2] = {ctx->ev_ring[0], ctx->ev_ring[1]};
+    // kRing bodies in flight; the host polls their fences (a blocking wait
+    // would add an interrupt wake-up per body)
+    constexpr int R = pp_ctx::kRing;
+    static_assert(R * C_COUNT <= 64, "pinned control slots");
+    int *slots = ctx->pinned_ctl;
     const uint64_t max_replays = (uint64_t)maxit / every + 2;
     uint64_t issued = 0, checked = 0;
     auto issue = [&] {
         if (use_graph) PP_HIP(hipGraphLaunch(G.exec, ctx->stream));
         else body();
-        int *slot = slots + (issued & 1) * C_COUNT;
+        int *slot = slots + (issued % R) * C_COUNT;
         PP_HIP(hipMemcpyAsync(slot, ctx->ctl, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        PP_HIP(hipEventRecord(evs[issued & 1], ctx->stream));
+        PP_HIP(hipEventRecord(ctx->ev_ring[issued % R], ctx->stream));
         ++issued;
     };
     // the control words after the first step decide whether any body runs
     read_ctl(ctx, ctl);
     PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
     if (!ctl[C_STOP]) {
-        issue();
         for (;;) {
-            if (issued < max_replays) issue();
-            PP_HIP(hipEventSynchronize(evs[checked & 1]));
-            std::memcpy(ctl, slots + (checked & 1) * C_COUNT, C_COUNT * sizeof(int));
+            while (issued < checked + R && issued < max_replays) issue();
+            wait_event(ctx->ev_ring[checked % R]);
+            std::memcpy(ctl, slots + (checked % R) * C_COUNT, C_COUNT * sizeof(int));
             ++checked;
             PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
             if (ctl[C_STOP] || checked >= max_replays) break;

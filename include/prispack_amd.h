@@ -268,6 +268,11 @@ typedef struct pp_batch pp_batch;
 int pp_batch_create(pp_ctx *ctx, uint32_t nsys, const uint32_t *row_off, const uint32_t *rowptr,
                     const uint32_t *colidx, const double *values, pp_batch **out);
 int pp_batch_set_values(pp_ctx *ctx, pp_batch *B, const double *values, int on_device);
+/* PP_ORDER_MULTICOLOR: solve every system as P_s A_s P_s^T with its rows
+ * grouped by a greedy colouring (one sweep level per colour instead of one per
+ * anti-diagonal); b and x stay in the caller's order.  Re-factors.  It changes
+ * the preconditioner and the iteration counts.  Default PP_ORDER_NATURAL. */
+int pp_batch_set_ordering(pp_ctx *ctx, pp_batch *B, int ordering);
 /* b, x: device, concatenated; every system starts from x0 = 0 and x receives
  * the solutions.  precision PP_FP64 or PP_MIXED.
  * iters_out (host, nsys) receives per-system iteration counts (may be NULL). */

@@ -491,7 +491,7 @@ class BatchedSystems:
     triples; right-hand sides and solutions are concatenated CUDA float64
     tensors.  Solves start from x0 = 0."""
 
-    def __init__(self, systems, ctx=None):
+    def __init__(self, systems, ctx=None, ordering=0):
         self.ctx = ctx or Context.default()
         sizes = [len(s[0]) - 1 for s in systems]
         self.row_off = np.zeros(len(systems) + 1, np.uint32)
@@ -513,6 +513,15 @@ class BatchedSystems:
         self.h = h
         self.block = 256
         self.report = _capi.SolveReport()
+        self.ordering = 0
+        if ordering:
+            self.set_ordering(ordering)
+
+    def set_ordering(self, ordering):
+        """1: multicolour order inside every system (IC(0) sweeps with one level
+        per colour); 0: natural order.  Re-factors; b and x keep the caller's order."""
+        self.ctx.check(_capi.lib().pp_batch_set_ordering(self.ctx.h, self.h, int(ordering)))
+        self.ordering = int(ordering)
 
     def tune(self):
         """Pick the workgroup size (row tile of the SpMV and sweep steps) by timing."""

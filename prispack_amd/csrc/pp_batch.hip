@@ -292,6 +292,16 @@ struct pp_batch {
     int block = 256;
     bool symmetric = true;
     uint32_t max_levels = 0;
+    // PP_ORDER_MULTICOLOR: every system is solved as P_s A_s P_s^T (rows
+    // grouped by colour, so its IC(0) sweeps have one level per colour);
+    // the kernel works on the permuted copy below, b and x are permuted once
+    // per solve.  Position k of the permuted batch holds original row gperm[k];
+    // permuted entry e holds original entry vmap[e].
+    int ordering = PP_ORDER_NATURAL;
+    std::vector<uint32_t> h_gperm, h_prp, h_pci, vmap;
+    uint32_t *gperm = nullptr, *prp = nullptr, *pci = nullptr;
+    double *pv64 = nullptr, *bp = nullptr, *xp = nullptr;
+    float *pv32 = nullptr;
 };
 
 namespace pp {
@@ -376,10 +386,15 @@ void upload_batch_sweep(const HostBatchSweep &H, DevBatchSweep &D) {
     D.lev_off = upload(H.lev_off);
 }
 
-// IC(0) of every diagonal block, as two gather-form sweeps.
-void batch_factor(pp_batch *B) {
-    HostBatchSweep lo, up;
-    B->max_levels = 0;
+// The permuted pattern of every system (multicolour order within each
+// diagonal block) and the entry map; values follow through vmap.
+void batch_build_order(pp_batch *B) {
+    B->h_gperm.assign(B->ntot, 0);
+    B->h_prp.assign(B->ntot + 1, 0);
+    B->h_pci.clear();
+    B->vmap.clear();
+    B->h_pci.reserve(B->nnz);
+    B->vmap.reserve(B->nnz);
     for (uint32_t s = 0; s < B->nsys; ++s) {
         const uint32_t r0 = B->h_roff[s], r1 = B->h_roff[s + 1], n = r1 - r0;
         HostCsr A;
@@ -388,7 +403,59 @@ void batch_factor(pp_batch *B) {
         for (uint32_t i = 0; i < n; ++i) {
             for (uint32_t k = B->h_rp[r0 + i]; k < B->h_rp[r0 + i + 1]; ++k) {
                 A.ci.push_back(B->h_ci[k] - r0);
-                A.v.push_back(B->h_v[k]);
+                A.v.push_back(0.0);
+            }
+            A.rp[i + 1] = (uint32_t)A.ci.size();
+        }
+        uint32_t ncol = 0;
+        const std::vector<uint32_t> order = multicolor_order(A, ncol);
+        std::vector<uint32_t> inv(n);
+        for (uint32_t k = 0; k < n; ++k) inv[order[k]] = k;
+        std::vector<std::pair<uint32_t, uint32_t>> row;  // (new column, original entry)
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t i = order[k];
+            B->h_gperm[r0 + k] = r0 + i;
+            row.clear();
+            for (uint32_t e = B->h_rp[r0 + i]; e < B->h_rp[r0 + i + 1]; ++e)
+                row.emplace_back(inv[B->h_ci[e] - r0], e);
+            std::sort(row.begin(), row.end());
+            for (const auto &pe : row) {
+                B->h_pci.push_back(r0 + pe.first);
+                B->vmap.push_back(pe.second);
+            }
+            B->h_prp[r0 + k + 1] = (uint32_t)B->h_pci.size();
+        }
+    }
+}
+
+// values of the permuted copy from the original values
+void batch_upload_permuted_values(pp_batch *B) {
+    std::vector<double> pv(B->nnz);
+    for (uint32_t e = 0; e < B->nnz; ++e) pv[e] = B->h_v[B->vmap[e]];
+    if (!B->pv64) B->pv64 = dalloc<double>(B->nnz);
+    if (!B->pv32) B->pv32 = dalloc<float>(B->nnz);
+    PP_HIP(hipMemcpy(B->pv64, pv.data(), B->nnz * sizeof(double), hipMemcpyHostToDevice));
+    const std::vector<float> p32 = to_f32(pv);
+    PP_HIP(hipMemcpy(B->pv32, p32.data(), B->nnz * sizeof(float), hipMemcpyHostToDevice));
+}
+
+// IC(0) of every diagonal block (of the permuted copy under multicolour
+// ordering), as two gather-form sweeps.
+void batch_factor(pp_batch *B) {
+    HostBatchSweep lo, up;
+    B->max_levels = 0;
+    const bool perm = B->ordering == PP_ORDER_MULTICOLOR;
+    const std::vector<uint32_t> &Rp = perm ? B->h_prp : B->h_rp;
+    const std::vector<uint32_t> &Ci = perm ? B->h_pci : B->h_ci;
+    for (uint32_t s = 0; s < B->nsys; ++s) {
+        const uint32_t r0 = B->h_roff[s], r1 = B->h_roff[s + 1], n = r1 - r0;
+        HostCsr A;
+        A.n = n;
+        A.rp.assign(n + 1, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            for (uint32_t k = Rp[r0 + i]; k < Rp[r0 + i + 1]; ++k) {
+                A.ci.push_back(Ci[k] - r0);
+                A.v.push_back(B->h_v[perm ? B->vmap[k] : k]);
             }
             A.rp[i + 1] = (uint32_t)A.ci.size();
         }
@@ -413,10 +480,11 @@ BatchArgs make_args(pp_batch *B, const pp_solve_opts &o, const double *b, double
     std::memset(&a, 0, sizeof(a));
     a.nsys = B->nsys;
     a.roff = B->roff;
-    a.rp = B->rp;
-    a.ci = B->ci;
-    a.v64 = B->v64;
-    a.v32 = B->v32;
+    const bool perm = B->ordering == PP_ORDER_MULTICOLOR;
+    a.rp = perm ? B->prp : B->rp;
+    a.ci = perm ? B->pci : B->ci;
+    a.v64 = perm ? B->pv64 : B->v64;
+    a.v32 = perm ? B->pv32 : B->v32;
     a.lo = B->lo;
     a.up = B->up;
     a.b = b;
@@ -476,6 +544,7 @@ void batch_free(pp_batch *B) {
     dfree(B->r64); dfree(B->rr64); dfree(B->w64); dfree(B->d64); dfree(B->p64);
     dfree(B->r32); dfree(B->e32); dfree(B->q32); dfree(B->w32); dfree(B->d32); dfree(B->p32);
     dfree(B->iters); dfree(B->outer); dfree(B->conv); dfree(B->relres);
+    dfree(B->gperm); dfree(B->prp); dfree(B->pci); dfree(B->pv64); dfree(B->pv32); dfree(B->bp); dfree(B->xp);
 }
 
 }  // namespace
@@ -551,6 +620,27 @@ int pp_batch_set_values(pp_ctx *ctx, pp_batch *B, const double *values, int on_d
         PP_HIP(hipMemcpy(B->v64, B->h_v.data(), B->nnz * sizeof(double), hipMemcpyHostToDevice));
         std::vector<float> v32 = to_f32(B->h_v);
         PP_HIP(hipMemcpy(B->v32, v32.data(), B->nnz * sizeof(float), hipMemcpyHostToDevice));
+        if (B->ordering == PP_ORDER_MULTICOLOR) batch_upload_permuted_values(B);
+        batch_factor(B);
+    });
+}
+
+int pp_batch_set_ordering(pp_ctx *ctx, pp_batch *B, int ordering) {
+    if (!ctx) return PP_ERR_INVALID;
+    return guarded_batch(ctx, [&] {
+        PP_CHECK(B, PP_ERR_INVALID, "NULL batch");
+        PP_CHECK(ordering == PP_ORDER_NATURAL || ordering == PP_ORDER_MULTICOLOR, PP_ERR_INVALID, "unknown ordering");
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        B->ordering = ordering;
+        if (ordering == PP_ORDER_MULTICOLOR && !B->gperm) {
+            batch_build_order(B);
+            B->gperm = upload(B->h_gperm);
+            B->prp = upload(B->h_prp);
+            B->pci = upload(B->h_pci);
+            B->bp = dalloc<double>(B->ntot);
+            B->xp = dalloc<double>(B->ntot);
+            batch_upload_permuted_values(B);
+        }
         batch_factor(B);
     });
 }
@@ -561,9 +651,15 @@ static int batch_solve_impl(pp_ctx *ctx, pp_batch *B, const pp_solve_opts *opts,
         PP_CHECK(B && opts && b && x, PP_ERR_INVALID, "NULL argument");
         PP_CHECK(opts->tol > 0 && opts->maxit >= 1, PP_ERR_INVALID, "bad tolerance / maxit");
         const bool mixed = opts->precision == PP_MIXED;
-        BatchArgs a = make_args(B, *opts, b, x);
+        const bool perm = B->ordering == PP_ORDER_MULTICOLOR;
+        BatchArgs a = make_args(B, *opts, perm ? B->bp : b, perm ? B->xp : x);
         PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        if (perm) launch_permute(ctx, B->ntot, B->gperm, PP_F64, b, B->bp, true);
         run_batch(ctx, B, mixed, a);
+        if (perm) {
+            launch_permute(ctx, B->ntot, B->gperm, PP_F64, B->xp, x, false);
+            PP_HIP(hipStreamSynchronize(ctx->stream));
+        }
         PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         PP_HIP(hipEventSynchronize(ctx->ev1));
         float ms = 0;

@@ -154,3 +154,34 @@ def test_sparse_solve_gradient(pp, torch):
     lam = np.linalg.solve(D.T, 2 * (xs - xt))
     np.testing.assert_allclose(bt.grad.cpu().numpy(), lam, rtol=1e-6, atol=1e-9)
     np.testing.assert_allclose(values.grad.cpu().numpy(), -lam[rows] * xs[A[1]], rtol=1e-6, atol=1e-9)
+
+
+def test_batched_multicolor_against_oracle(pp, torch):
+    # every system in colour order: the oracle's FlexibleCG + IC(0) on P A P^T
+    from mc_order import greedy_order, permute
+    sides, systems = make_systems(10, 6, 48, 4)
+    B = pp.BatchedSystems(systems, ordering=1)
+    bs = [ppo.spmv(A, ppo.uniform(len(A[0]) - 1, 70 + s)) for s, A in enumerate(systems)]
+    b = torch.tensor(np.concatenate(bs), device="cuda")
+    x64 = torch.zeros_like(b)
+    its = B.solve(b, x64, precision="fp64", tol=1e-8)
+    assert B.report.converged and B.report.true_relres < 1e-8
+    xm = torch.zeros_like(b)
+    B.solve(b, xm, precision="mixed", tol=1e-8)
+    assert B.report.converged and B.report.true_relres < 1e-8
+    off = B.row_off
+    for s, A in enumerate(systems):
+        order, _ = greedy_order(A)
+        Ap = permute(A, order)
+        o = ppo.cg(Ap, bs[s][order], ppo.Prec(Ap, "ic0"), tol=1e-8, maxit=5000)
+        xo = np.empty_like(o["x"])
+        xo[order] = o["x"]
+        assert abs(int(its[s]) - o["iters"]) <= 2, (s, its[s], o["iters"])
+        for xx in (x64, xm):
+            xs = xx[off[s]:off[s + 1]].cpu().numpy()
+            assert np.linalg.norm(xs - xo) / np.linalg.norm(xo) <= 1e-6
+    # new values on the same pattern keep the ordering
+    B.set_values(B.v * 1.5)
+    x2 = torch.zeros_like(b)
+    B.solve(b, x2, precision="fp64", tol=1e-8)
+    np.testing.assert_allclose(x2.cpu().numpy() * 1.5, x64.cpu().numpy(), rtol=1e-6, atol=1e-9)

@@ -290,13 +290,17 @@ def main():
     y = torch.zeros(n, dtype=torch.float64, device="cuda")
     px, py = C.c_void_p(d_xt.data_ptr()), C.c_void_p(y.data_ptr())
     spmv_ms = kernel_time_ms(torch, stream, lambda: L.pp_spmv(ctx.h, M.h, 0, 1.0, px, 0.0, py), args.kernel_reps)
-    spmv_bytes = 12 * nnz + 20 * n + 4
+    def spmv_bytes_of(dtype):
+        nb = C.c_uint64()
+        ctx.check(L.pp_csr_spmv_bytes(M.h, dtype, C.byref(nb)))
+        return int(nb.value)
+    spmv_bytes = spmv_bytes_of(0)
     x32 = d_xt.float()
     y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
     p32x, p32y = C.c_void_p(x32.data_ptr()), C.c_void_p(y32.data_ptr())
     spmv32_ms = kernel_time_ms(torch, stream, lambda: L.pp_spmv(ctx.h, M.h, 1, 1.0, p32x, 0.0, p32y),
                                args.kernel_reps)
-    spmv32_bytes = 8 * nnz + 12 * n + 4
+    spmv32_bytes = spmv_bytes_of(1)
     # one preconditioner apply = forward sweep on L + backward sweep on L^T;
     # per sweep: 12 nnz_L (values + columns) + 4(n+1) row pointers + 4n level
     # permutation + 8n right-hand side + 8n solution (diagonal folded in the row)

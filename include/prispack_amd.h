@@ -76,6 +76,12 @@ int pp_csr_create(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *row
 /* Replace the values (same pattern).  on_device != 0: values is a device pointer. */
 int pp_csr_set_values(pp_ctx *ctx, pp_csr *A, const double *values, int on_device);
 int pp_csr_dims(const pp_csr *A, uint32_t *nrow, uint32_t *ncol, uint32_t *nnz);
+/* Bytes one y = A x (beta = 0) streams through HBM in this build's storage:
+ * values, column indices (16-bit block-relative offsets where a row block's
+ * column span fits, else 32-bit), row pointers, row-block table, x and y.
+ * The algorithmic traffic the SpMV roofline is priced on.  No reference
+ * counterpart. */
+int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes);
 /* Device pointers of the stored arrays (values64 FP64; values32 NULL until an
  * FP32 copy exists). */
 int pp_csr_device_ptrs(const pp_csr *A, const uint32_t **rowptr, const uint32_t **colidx,

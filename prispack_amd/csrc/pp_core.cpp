@@ -142,6 +142,32 @@ static pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t
     A->row_blocks = dalloc<uint32_t>(blocks.size());
     PP_HIP(hipMemcpyAsync(A->row_blocks, blocks.data(), blocks.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                           ctx->stream));
+    // 16-bit block-relative column offsets where a block's column span fits
+    std::vector<uint32_t> base(A->nblocks, kNoBase);
+    std::vector<uint16_t> off16;
+    A->nblocks16 = 0;
+    for (uint32_t bi = 0; bi < A->nblocks; ++bi) {
+        const uint32_t k0 = rp[blocks[bi]], k1 = rp[blocks[bi + 1]];
+        if (k1 == k0 || k1 - k0 > kSpmvNnzBlock) continue;
+        uint32_t lo = ci[k0], hi = ci[k0];
+        for (uint32_t k = k0; k < k1; ++k) {
+            lo = std::min(lo, ci[k]);
+            hi = std::max(hi, ci[k]);
+        }
+        if (hi - lo > 0xFFFFu) continue;
+        if (off16.empty()) off16.assign(nnz, 0);
+        base[bi] = lo;
+        for (uint32_t k = k0; k < k1; ++k) off16[k] = (uint16_t)(ci[k] - lo);
+        ++A->nblocks16;
+        A->nnz16 += k1 - k0;
+    }
+    if (A->nblocks16) {
+        A->ci16 = dalloc<uint16_t>(nnz);
+        A->blk_base = dalloc<uint32_t>(A->nblocks);
+        PP_HIP(hipMemcpyAsync(A->ci16, off16.data(), nnz * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(A->blk_base, base.data(), A->nblocks * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+    }
     PP_HIP(hipStreamSynchronize(ctx->stream));
     return A.release();
 }
@@ -158,6 +184,8 @@ void csr_free(pp_csr *A) {
     dfree(A->v64);
     dfree(A->v32);
     dfree(A->row_blocks);
+    dfree(A->ci16);
+    dfree(A->blk_base);
     delete A;
 }
 
@@ -327,6 +355,18 @@ int pp_csr_set_values(pp_ctx *ctx, pp_csr *A, const double *values, int on_devic
             A->trans = nullptr;
         }
     });
+}
+
+int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes) {
+    if (!A || !bytes) return PP_ERR_INVALID;
+    const uint64_t esz = dtype == PP_F64 ? 8 : 4;
+    uint64_t b = esz * A->nnz;                                    // values
+    b += 2 * A->nnz16 + 4 * ((uint64_t)A->nnz - A->nnz16);       // column indices
+    b += 4ull * (A->nrow + 1) + 4ull * (A->nblocks + 1);          // row pointers, row blocks
+    if (A->ci16) b += 4ull * A->nblocks;                          // block bases
+    b += esz * ((uint64_t)A->ncol + A->nrow);                     // x read, y written
+    *bytes = b;
+    return PP_OK;
 }
 
 int pp_csr_dims(const pp_csr *A, uint32_t *nrow, uint32_t *ncol, uint32_t *nnz) {

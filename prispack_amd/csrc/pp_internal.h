@@ -34,6 +34,7 @@ struct Error : std::runtime_error {
 constexpr int kBlock = 256;             // threads per workgroup of the streaming kernels
 constexpr uint32_t kSpmvNnzBlock = 2048; // nnz staged in LDS per SpMV workgroup
 constexpr uint32_t kStreamLevelRows = 4096; // sweep levels this wide use the LDS-staged kernel
+constexpr uint32_t kNoBase = 0xFFFFFFFFu;   // row block without 16-bit column offsets
 constexpr uint32_t kSpmvRowsBlock = 256; // rows per SpMV workgroup (one per thread)
 constexpr int kMaxPartials = 4096;       // reduction partials per slot
 
@@ -158,6 +159,13 @@ struct pp_csr {
     float *v32 = nullptr;
     uint32_t *row_blocks = nullptr;  // adaptive SpMV row blocks
     uint32_t nblocks = 0;
+    // 16-bit column indices relative to a per-block base: block b's columns
+    // are blk_base[b] + ci16[k]; blk_base[b] == kNoBase marks a block whose
+    // column span does not fit 16 bits (it reads ci).  NULL when no block fits.
+    uint16_t *ci16 = nullptr;
+    uint32_t *blk_base = nullptr;
+    uint32_t nblocks16 = 0;          // blocks that use ci16
+    uint64_t nnz16 = 0;              // entries in those blocks
     pp_csr *trans = nullptr;         // explicit transpose, built on demand
     std::vector<uint32_t> h_rp, h_ci;
     std::vector<double> h_v;

@@ -90,6 +90,8 @@ static inline unsigned grid_for(pp_ctx *ctx, uint64_t n, int per_thread = 1) {
 template <class T, bool DOT>
 __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
                                               const uint32_t *__restrict__ ci,
+                                              const uint16_t *__restrict__ ci16,
+                                              const uint32_t *__restrict__ blk_base,
                                               const T *__restrict__ val, const T *__restrict__ x,
                                               T *__restrict__ y, const uint32_t *__restrict__ rb,
                                               uint32_t nb, T alpha, T beta, int beta_zero,
@@ -110,12 +112,24 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
             constexpr int PER = kSpmvNnzBlock / kBlock;
             uint32_t c[PER];
             T v[PER];
+            const uint32_t base = ci16 ? blk_base[b] : kNoBase;  // uniform per block
+            if (base != kNoBase) {
 #pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const uint32_t k = threadIdx.x + j * kBlock;
-                if (k < cnt) {
-                    c[j] = __builtin_nontemporal_load(ci + k0 + k);
-                    v[j] = __builtin_nontemporal_load(val + k0 + k);
+                for (int j = 0; j < PER; ++j) {
+                    const uint32_t k = threadIdx.x + j * kBlock;
+                    if (k < cnt) {
+                        c[j] = base + (uint32_t)__builtin_nontemporal_load(ci16 + k0 + k);
+                        v[j] = __builtin_nontemporal_load(val + k0 + k);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < PER; ++j) {
+                    const uint32_t k = threadIdx.x + j * kBlock;
+                    if (k < cnt) {
+                        c[j] = __builtin_nontemporal_load(ci + k0 + k);
+                        v[j] = __builtin_nontemporal_load(val + k0 + k);
+                    }
                 }
             }
 #pragma unroll
@@ -183,22 +197,26 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
     int bz = beta == 0.0;
     if (dtype == PP_F64) {
         if (dot_slot >= 0)
-            k_spmv<double, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v64, (const double *)x,
+            k_spmv<double, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->ci16, A->blk_base, A->v64,
+                                                                 (const double *)x,
                                                                  (double *)y, A->row_blocks, A->nblocks, alpha, beta, bz,
                                                                  (const double *)dot_with, part, cnt, res, ctx->stop_word);
         else
-            k_spmv<double, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v64, (const double *)x,
+            k_spmv<double, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->ci16, A->blk_base, A->v64,
+                                                                 (const double *)x,
                                                                   (double *)y, A->row_blocks, A->nblocks, alpha, beta,
                                                                   bz, nullptr, nullptr, nullptr, nullptr,
                                                                   ctx->stop_word);
     } else {
         if (dot_slot >= 0)
-            k_spmv<float, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v32, (const float *)x,
+            k_spmv<float, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->ci16, A->blk_base, A->v32,
+                                                                (const float *)x,
                                                                 (float *)y, A->row_blocks, A->nblocks, (float)alpha,
                                                                 (float)beta, bz, (const float *)dot_with, part, cnt, res,
                                                                 ctx->stop_word);
         else
-            k_spmv<float, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->v32, (const float *)x,
+            k_spmv<float, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->ci16, A->blk_base, A->v32,
+                                                                (const float *)x,
                                                                  (float *)y, A->row_blocks, A->nblocks, (float)alpha,
                                                                  (float)beta, bz, nullptr, nullptr, nullptr, nullptr,
                                                                  ctx->stop_word);

@@ -60,8 +60,27 @@ def long_row_matrix():
     return rp, ci, v
 
 
+def wide_span_matrix():
+    # a banded matrix (16-bit column offsets fit) with long-range couplings in
+    # some row blocks (their column span exceeds 16 bits: 32-bit indices)
+    rng = np.random.default_rng(9)
+    n = 200_000
+    rows = []
+    for i in range(n):
+        cols = {max(0, i - 1), i, min(n - 1, i + 1)}
+        if (i // 256) % 5 == 2 and i % 7 == 0:
+            cols.add((i + 90_000) % n)
+        rows.append(np.array(sorted(cols), dtype=np.int64))
+    rp = np.zeros(n + 1, np.uint32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate(rows).astype(np.uint32)
+    v = rng.standard_normal(len(ci))
+    return rp, ci, v
+
+
 MATS = {
     "lap3d_20": lambda: ppo.laplacian(20, 20, 20),
+    "wide_span": wide_span_matrix,
     "lap2d_128": lambda: ppo.laplacian(128, 128, 1),
     "upwind_17": lambda: ppo.upwind(17, 17, 17, 20.0, 10.0, 5.0),
     "ragged": long_row_matrix,
@@ -92,7 +111,7 @@ def test_spmv_fp64_bit_exact(pp, torch, name):
     np.testing.assert_allclose(host(yb), b - want, rtol=0, atol=1e-13 * (1 + np.abs(want).max()))
 
 
-@pytest.mark.parametrize("name", ["lap3d_20", "upwind_17", "ragged"])
+@pytest.mark.parametrize("name", ["lap3d_20", "upwind_17", "ragged", "wide_span"])
 def test_spmv_t_and_fp32(pp, torch, name):
     A = MATS[name]()
     n = len(A[0]) - 1

@@ -301,14 +301,15 @@ def main():
     spmv32_ms = kernel_time_ms(torch, stream, lambda: L.pp_spmv(ctx.h, M.h, 1, 1.0, p32x, 0.0, p32y),
                                args.kernel_reps)
     spmv32_bytes = spmv_bytes_of(1)
-    # one preconditioner apply = forward sweep on L + backward sweep on L^T;
-    # per sweep: 12 nnz_L (values + columns) + 4(n+1) row pointers + 4n level
-    # permutation + 8n right-hand side + 8n solution (diagonal folded in the row)
-    nnz_l = (nnz - n) // 2
-    sweep_bytes = 12 * nnz_l + 4 * (n + 1) + 4 * n + 16 * n + 8 * n
-    trsv_bytes = 2 * sweep_bytes
-    if args.ordering == 1:  # gather into and scatter out of the colour order: 4n index + 8n in + 8n out each
-        trsv_bytes += 2 * (4 * n + 16 * n)
+    # one preconditioner apply = forward sweep on L + backward sweep on L^T,
+    # priced on the bytes the library's sweep storage streams (direct apply:
+    # with the colour-order gather and scatter)
+    def apply_bytes_of(dtype, in_solve=0):
+        nb = C.c_uint64()
+        ctx.check(L.pp_precond_apply_bytes(P.h, dtype, in_solve, C.byref(nb)))
+        return int(nb.value)
+    trsv_bytes = apply_bytes_of(0)
+    trsv32_bytes = apply_bytes_of(1)
     tv = torch.tensor(xt, device="cuda")
     ptv = C.c_void_p(tv.data_ptr())
     trsv_ms = kernel_time_ms(torch, stream, lambda: L.pp_precond_apply(ctx.h, P.h, 0, ptv, ptv),
@@ -319,7 +320,8 @@ def main():
                                max(3, args.kernel_reps // 10))
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
     log(f"[bench] SpMV fp64 {spmv_ms * 1e3:.1f} us {gbs(spmv_bytes, spmv_ms):.0f} GB/s; fp32 {spmv32_ms * 1e3:.1f} us; "
-        f"M^-1 apply fp64 {trsv_ms * 1e3:.1f} us {gbs(trsv_bytes, trsv_ms):.1f} GB/s; fp32 {trsv32_ms * 1e3:.1f} us")
+        f"M^-1 apply fp64 {trsv_ms * 1e3:.1f} us {gbs(trsv_bytes, trsv_ms):.1f} GB/s; fp32 {trsv32_ms * 1e3:.1f} us "
+        f"{gbs(trsv32_bytes, trsv32_ms):.1f} GB/s")
 
     # the other ordering, one solve of each precision, for the record
     other = None
@@ -350,7 +352,8 @@ def main():
                 "bound": "hbm", "achieved": round(gbs(trsv_bytes, trsv_ms), 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(gbs(trsv_bytes, trsv_ms) / HBM_PEAK_GBS, 5),
                 "traffic": None, "bytes_per_launch": trsv_bytes, "avg_ms": round(trsv_ms, 4),
-                "fp32_avg_ms": round(trsv32_ms, 4)}
+                "fp32_avg_ms": round(trsv32_ms, 4), "fp32_bytes_per_launch": trsv32_bytes,
+                "fp32_achieved": round(gbs(trsv32_bytes, trsv32_ms), 2)}
     out = {
         "metric": "time-to-rtol 1e-8 (s) mixed-prec vs FP64 on GPU; SpMV/SpTRSV HBM GB/s",
         "value": round(medmx, 5), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -178,6 +178,13 @@ int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void 
  * in place (TriangularPrecondition, BasePreconditon.h:200-303) */
 int pp_precond_apply_part(pp_ctx *ctx, pp_precond *M, int part, int dtype, void *x);
 int pp_precond_info_get(const pp_precond *M, pp_precond_info *info);
+/* Bytes one M^{-1} x streams through HBM in this build's sweep storage (both
+ * sweeps: values, 16/32-bit dependency indices, row pointers, level row map
+ * where read, diagonal, block tables, vectors).  in_solve != 0: as applied
+ * inside a solve on the factor's own matrix (no colour-order gather/scatter).
+ * The algorithmic traffic the sweep roofline is priced on.  Triangular
+ * factors only; no reference counterpart. */
+int pp_precond_apply_bytes(const pp_precond *M, int dtype, int in_solve, uint64_t *bytes);
 /* Copy the host factor out (for inspection / parity tests).
  * which: 0 L^T (IC, ILDLT strict upper), 1 L (ILU), 2 U (ILU).
  * Call with NULL arrays to get nnz; d receives the diagonal (ILU 1/u_ii, ILDLT d). */

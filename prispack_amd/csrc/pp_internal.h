@@ -135,6 +135,11 @@ struct DevSweep {
     // level_blk[2l+1]) (host), an empty range for a narrow level
     uint32_t *blocks = nullptr;
     std::vector<uint32_t> level_blk;
+    // 16-bit dependency offsets of the row blocks (as for the SpMV): block
+    // j's dependencies are blk_base[j] + ci16[e] unless blk_base[j] == kNoBase
+    uint16_t *ci16 = nullptr;
+    uint32_t *blk_base = nullptr;
+    uint64_t nnz16 = 0, nblk = 0, perm_rows = 0;  // entries on 16-bit offsets, row blocks, rows read through perm
     uint32_t max_level_rows = 0;
     void free_all();
 };

@@ -366,7 +366,9 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
                                                       const uint32_t *__restrict__ ci,
                                                       const T *__restrict__ val, const T *__restrict__ diag,
                                                       int mode, const T *b, T *x,
-                                                      const uint32_t *__restrict__ blocks, uint32_t blk0,
+                                                      const uint32_t *__restrict__ blocks,
+                                                      const uint16_t *__restrict__ ci16,
+                                                      const uint32_t *__restrict__ blk_base, uint32_t blk0,
                                                       uint32_t blk1, long long off, const int *stop) {
 #pragma clang fp contract(off)
     __shared__ T prod[kSpmvNnzBlock];
@@ -379,12 +381,24 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
         if (staged) {
             uint32_t c[PER];
             T v[PER];
+            const uint32_t base = ci16 ? blk_base[bi] : kNoBase;  // uniform per block
+            if (base != kNoBase) {
 #pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const uint32_t k = threadIdx.x + j * kBlock;
-                if (k < cnt) {
-                    c[j] = __builtin_nontemporal_load(ci + e0 + k);
-                    v[j] = __builtin_nontemporal_load(val + e0 + k);
+                for (int j = 0; j < PER; ++j) {
+                    const uint32_t k = threadIdx.x + j * kBlock;
+                    if (k < cnt) {
+                        c[j] = base + (uint32_t)__builtin_nontemporal_load(ci16 + e0 + k);
+                        v[j] = __builtin_nontemporal_load(val + e0 + k);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < PER; ++j) {
+                    const uint32_t k = threadIdx.x + j * kBlock;
+                    if (k < cnt) {
+                        c[j] = __builtin_nontemporal_load(ci + e0 + k);
+                        v[j] = __builtin_nontemporal_load(val + e0 + k);
+                    }
                 }
             }
 #pragma unroll
@@ -428,7 +442,8 @@ void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x)
             const uint32_t b0 = s.level_blk[2 * l], b1 = s.level_blk[2 * l + 1];
             const unsigned g = std::min<unsigned>(b1 - b0, 8192u);
             k_sweep_stream<T><<<g, kBlock, 0, ctx->stream>>>(contiguous ? nullptr : s.perm, s.rp, s.ci, s.val,
-                                                             s.diag, mode, b, x, s.blocks, b0, b1,
+                                                             s.diag, mode, b, x, s.blocks, s.ci16, s.blk_base, b0,
+                                                             b1,
                                                              contiguous ? s.level_off[l] : 0, stop);
             continue;
         }

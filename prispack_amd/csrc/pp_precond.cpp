@@ -24,12 +24,17 @@ void DevSweep<T>::free_all() {
     dfree(val);
     dfree(diag);
     dfree(blocks);
+    dfree(ci16);
+    dfree(blk_base);
     blocks = nullptr;
+    ci16 = nullptr;
+    blk_base = nullptr;
     level_blk.clear();
     perm = rp = ci = nullptr;
     val = diag = nullptr;
     level_ptr.clear();
     n = nnz = nlev = 0;
+    nnz16 = nblk = perm_rows = 0;
 }
 template struct DevSweep<double>;
 template struct DevSweep<float>;
@@ -111,7 +116,33 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
     if (blk.size() > 1) {
         d.blocks = dalloc<uint32_t>(blk.size());
         PP_HIP(hipMemcpy(d.blocks, blk.data(), blk.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        // 16-bit offsets where a block's dependency columns span < 2^16
+        std::vector<uint32_t> base(blk.size() - 1, kNoBase);
+        std::vector<uint16_t> off16;
+        for (size_t j = 0; j + 1 < blk.size(); ++j) {
+            const uint32_t e0 = rp[blk[j]], e1 = rp[blk[j + 1]];
+            if (e1 == e0 || e1 - e0 > kSpmvNnzBlock) continue;
+            uint32_t lo = ci[e0], hi = ci[e0];
+            for (uint32_t e = e0; e < e1; ++e) {
+                lo = std::min(lo, ci[e]);
+                hi = std::max(hi, ci[e]);
+            }
+            if (hi - lo > 0xFFFFu) continue;
+            if (off16.empty()) off16.assign(ci.size(), 0);
+            base[j] = lo;
+            for (uint32_t e = e0; e < e1; ++e) off16[e] = (uint16_t)(ci[e] - lo);
+            d.nnz16 += e1 - e0;
+        }
+        d.nblk = blk.size() - 1;
+        if (!off16.empty()) {
+            d.ci16 = dalloc<uint16_t>(off16.size());
+            d.blk_base = dalloc<uint32_t>(base.size());
+            PP_HIP(hipMemcpy(d.ci16, off16.data(), off16.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+            PP_HIP(hipMemcpy(d.blk_base, base.data(), base.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        }
     }
+    for (uint32_t l = 0; l < d.nlev; ++l)
+        if (d.level_off[l] == DevSweep<T>::kNoOffset) d.perm_rows += lp[l + 1] - lp[l];
     d.perm = dalloc<uint32_t>(n);
     d.rp = dalloc<uint32_t>(n + 1);
     d.ci = dalloc<uint32_t>(d.nnz);
@@ -485,6 +516,38 @@ int pp_precond_apply_part(pp_ctx *ctx, pp_precond *M, int part, int dtype, void 
             throw Error(PP_ERR_INVALID, "part must be 0, 1 or 2");
         }
     });
+}
+
+// Bytes one sweep streams: values, dependency indices, row pointers, the
+// level row map where read, the diagonal, the block tables, and the vectors
+// (right-hand side read, solution written, dependency values read once).
+}  // extern "C"
+template <class T>
+static uint64_t sweep_bytes(const DevSweep<T> &d) {
+    const uint64_t esz = sizeof(T);
+    uint64_t b = esz * d.nnz + 2 * d.nnz16 + 4 * (d.nnz - d.nnz16) + 4ull * (d.n + 1) + 4 * d.perm_rows;
+    if (d.diag) b += esz * d.n;
+    if (d.nblk) b += 4 * (d.nblk + 1) + (d.ci16 ? 4 * d.nblk : 0);
+    return b + 3 * esz * d.n;
+}
+extern "C" {
+
+int pp_precond_apply_bytes(const pp_precond *M, int dtype, int in_solve, uint64_t *bytes) {
+    if (!M || !bytes) return PP_ERR_INVALID;
+    if (!M->ready) return PP_ERR_NOT_READY;
+    if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR || M->opts.kind == PP_PC_NONE)
+        return PP_ERR_UNSUPPORTED;
+    uint64_t b;
+    if (dtype == PP_F64) {
+        b = sweep_bytes(M->low64) + sweep_bytes(M->up64);
+    } else {
+        if (!M->have32) return PP_ERR_NOT_READY;
+        b = sweep_bytes(M->low32) + sweep_bytes(M->up32);
+    }
+    const uint64_t esz = dtype == PP_F64 ? 8 : 4;
+    if (!M->order.empty() && !in_solve) b += 2 * (4ull * M->n + 2 * esz * M->n);  // gather in, scatter out
+    *bytes = b;
+    return PP_OK;
 }
 
 int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {

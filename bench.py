@@ -169,21 +169,24 @@ def extra_records(torch, pp, args):
         P.setup()
         setup = time.perf_counter() - t0
         sizes, ranks = P.info_levels()
-        def run():
-            if kind == "amsed":
-                s = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000)
-            else:
-                s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000)
-                s.setRestart(True, 30)
-            x = torch.zeros(n, dtype=torch.float64, device="cuda")
-            s.solve(M, b, x)
-            return s
-        timed(torch, run)
-        dt, s = timed(torch, run)
-        return {"workload": f"{kind.upper()} 3 levels, 16 subdomains, rank 32, 64^3 "
-                            f"{'Poisson, FlexibleCG' if kind == 'amsed' else 'upwind (20,10,5)h, FGMRES(30)'} FP64",
-                "s": round(dt, 5), "setup_s": round(setup, 3), "iters": s.getNumIter(),
-                "relres": s.report.true_relres, "level_sizes": sizes, "ranks": ranks}
+        rec = {"workload": f"{kind.upper()} 3 levels, 16 subdomains, rank 32, 64^3 "
+                           f"{'Poisson, FlexibleCG' if kind == 'amsed' else 'upwind (20,10,5)h, FGMRES(30)'}",
+               "setup_s": round(setup, 3), "level_sizes": sizes, "ranks": ranks}
+        for prec in ("fp64", "mixed"):
+            def run():
+                if kind == "amsed":
+                    s = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision=prec)
+                else:
+                    s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000, precision=prec)
+                    s.setRestart(True, 30)
+                x = torch.zeros(n, dtype=torch.float64, device="cuda")
+                s.solve(M, b, x)
+                return s
+            timed(torch, run)
+            dt, s = timed(torch, run)
+            rec[prec] = {"s": round(dt, 5), "iters": s.getNumIter(), "relres": s.report.true_relres,
+                         "converged": s.getConvergence()}
+        return rec
 
     def batch():
         nsys = 256

@@ -239,6 +239,9 @@ HostCsr transpose(const HostCsr &A);
 std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors);
 HostCsr permute_symmetric(const HostCsr &A, const std::vector<uint32_t> &order);
 pp_csr *precond_permuted_matrix(pp_ctx *ctx, pp_precond *M, pp_csr *A);
+void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
+void precond_need_f32(pp_precond *M);
+void csr_need_f32(pp_ctx *ctx, pp_csr *A);
 
 // ---- tall-skinny GEMVs and GMRES helpers (pp_kernels_krylov.hip) ----
 // device coefficient buffer: [0,320) h, [320,640) y or a copy of h, then partials

@@ -33,6 +33,7 @@ struct MlLevel {
     double *Md = nullptr;                 // nonsymmetric: dense r x r core (W orthonormal)
     uint32_t rank = 0;
     double *save = nullptr, *t = nullptr, *h = nullptr;  // device work vectors
+    float *W32 = nullptr, *save32 = nullptr, *t32 = nullptr;  // FP32 apply
     std::vector<double> Hhost, theta;     // host copy of H and the Ritz values
 };
 
@@ -44,6 +45,8 @@ struct MlState {
     pp_precond *last = nullptr;
     uint32_t *dperm = nullptr;            // device perm
     double *xp = nullptr;                 // permuted work vector
+    float *xp32 = nullptr;
+    bool have32 = false;                  // FP32 copies of every operand exist
     uint32_t n = 0;
     uint32_t lanczos_steps = 0;
 };

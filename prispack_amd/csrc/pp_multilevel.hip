@@ -30,22 +30,27 @@ namespace {
 
 constexpr int kLrBlocks = 256;
 
-__global__ void k_gather(uint32_t n, const uint32_t *perm, const double *x, double *xp) {
+template <class T>
+__global__ void k_gather(uint32_t n, const uint32_t *perm, const T *x, T *xp) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) xp[k] = x[perm[k]];
 }
-__global__ void k_scatter(uint32_t n, const uint32_t *perm, const double *xp, double *y) {
+template <class T>
+__global__ void k_scatter(uint32_t n, const uint32_t *perm, const T *xp, T *y) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) y[perm[k]] = xp[k];
 }
 
-// part[c][b] = sum over block b's rows of W[:, c] * s
-__global__ __launch_bounds__(256) void k_lr_partial(uint32_t n, uint32_t r, const double *__restrict__ W,
-                                                    const double *__restrict__ s, double *part) {
+// part[c][b] = sum over block b's rows of W[:, c] * s (accumulated in T
+// within a thread, in FP64 across threads)
+template <class T>
+__global__ __launch_bounds__(256) void k_lr_partial(uint32_t n, uint32_t r, const T *__restrict__ W,
+                                                    const T *__restrict__ s, double *part) {
     __shared__ double sh[4];
     const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const uint32_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
     for (uint32_t c = 0; c < r; ++c) {
-        double acc = 0.0;
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) acc += W[(size_t)c * n + i] * s[i];
+        T acct = 0;
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) acct += W[(size_t)c * n + i] * s[i];
+        double acc = (double)acct;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
@@ -76,6 +81,22 @@ __global__ __launch_bounds__(64) void k_lr_dense(uint32_t r, const double *__res
         for (uint32_t j = 0; j < r; ++j) acc += M[(size_t)j * r + i] * t[j];
         h[i] = acc;
     }
+}
+
+// x[i] += sum_c W[i, c] h[c] (W, x in T; h in FP64)
+template <class T>
+__global__ __launch_bounds__(256) void k_lr_add(uint32_t n, uint32_t r, const T *__restrict__ W,
+                                                const double *__restrict__ h, T *__restrict__ x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        T acc = 0;
+        for (uint32_t c = 0; c < r; ++c) acc += W[(size_t)c * n + i] * (T)h[c];
+        x[i] += acc;
+    }
+}
+
+__global__ void k_to_f32(size_t n, const double *s, float *d) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = (float)s[i];
 }
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -143,40 +164,61 @@ pp_precond *make_factor(pp_ctx *ctx, pp_csr *A, int kind, double tau, uint32_t l
     return P;
 }
 
+template <class T>
+struct MlBuf;
+template <>
+struct MlBuf<double> {
+    static double *save(MlLevel &L) { return L.save; }
+    static double *t(MlLevel &L) { return L.t; }
+    static const double *W(MlLevel &L) { return L.W; }
+};
+template <>
+struct MlBuf<float> {
+    static float *save(MlLevel &L) { return L.save32; }
+    static float *t(MlLevel &L) { return L.t32; }
+    static const float *W(MlLevel &L) { return L.W32; }
+};
+
 // x (device, length n - lev_ptr[level+1]) <- C_level^{-1} x; level -1 is A.
-void rec_solve(pp_ctx *ctx, MlState &S, int level, double *x) {
+// T = float: every operand in FP32 (factors, couplings, low-rank basis), the
+// low-rank coefficients in FP64.
+template <class T>
+void rec_solve(pp_ctx *ctx, MlState &S, int level, T *x) {
+    constexpr int DT = sizeof(T) == 8 ? PP_F64 : PP_F32;
     const int next = level + 1;
     if (next == (int)S.hid.nlev - 1) {
-        precond_apply(ctx, S.last, PP_F64, x, x);
+        precond_apply(ctx, S.last, DT, x, x);
         return;
     }
     MlLevel &L = S.lev[next];
-    double *x1 = x, *x2 = x + L.nb;
-    precond_apply(ctx, L.Bpc, PP_F64, x1, x1);
-    launch_spmv(ctx, L.F, PP_F64, -1.0, x1, 1.0, x2, -1, nullptr, false);  // x2 -= F x1
+    T *x1 = x, *x2 = x + L.nb;
+    T *save = MlBuf<T>::save(L), *t = MlBuf<T>::t(L);
+    precond_apply(ctx, L.Bpc, DT, x1, x1);
+    launch_spmv(ctx, L.F, DT, -1.0, x1, 1.0, x2, -1, nullptr, false);  // x2 -= F x1
     // symmetric: S^{-1} ~ C^{-1} + W H W^T (on the input of C^{-1});
     // nonsymmetric: S^{-1} ~ (I + W H W^T) C^{-1} (on its output)
     if (L.rank && S.symmetric)
-        PP_HIP(hipMemcpyAsync(L.save, x2, (size_t)L.nrest * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    rec_solve(ctx, S, next, x2);
+        PP_HIP(hipMemcpyAsync(save, x2, (size_t)L.nrest * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    rec_solve<T>(ctx, S, next, x2);
     if (L.rank && !S.symmetric)
-        PP_HIP(hipMemcpyAsync(L.save, x2, (size_t)L.nrest * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(save, x2, (size_t)L.nrest * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
     if (L.rank) {
         double *part = L.h + 64 * 1024;
-        k_lr_partial<<<kLrBlocks, 256, 0, ctx->stream>>>(L.nrest, L.rank, L.W, L.save, part);
+        k_lr_partial<T><<<kLrBlocks, 256, 0, ctx->stream>>>(L.nrest, L.rank, MlBuf<T>::W(L), save, part);
         if (L.Md) {  // nonsymmetric: h = ((I - R)^{-1} - I) U^T s
-            double *t = L.h + 32 * 1024;
-            k_lr_reduce<<<L.rank, 256, 0, ctx->stream>>>(kLrBlocks, part, L.Hd, t);
-            k_lr_dense<<<1, 64, 0, ctx->stream>>>(L.rank, L.Md, t, L.h);
+            double *tt = L.h + 32 * 1024;
+            k_lr_reduce<<<L.rank, 256, 0, ctx->stream>>>(kLrBlocks, part, L.Hd, tt);
+            k_lr_dense<<<1, 64, 0, ctx->stream>>>(L.rank, L.Md, tt, L.h);
         } else {
             k_lr_reduce<<<L.rank, 256, 0, ctx->stream>>>(kLrBlocks, part, L.Hd, L.h);
         }
+        const unsigned g = std::min<uint32_t>((L.nrest + 255) / 256, 8192);
+        k_lr_add<T><<<std::max(1u, g), 256, 0, ctx->stream>>>(L.nrest, L.rank, MlBuf<T>::W(L), L.h, x2);  // x2 += W h
         PP_HIP(hipGetLastError());
-        launch_gemv_add(ctx, L.nrest, L.rank, nullptr, L.W, L.nrest, L.h, x2);  // x2 += W h
     }
-    launch_spmv(ctx, L.E, PP_F64, 1.0, x2, 0.0, L.t, -1, nullptr, false);  // t = E x2
-    precond_apply(ctx, L.Bpc, PP_F64, L.t, L.t);
-    launch_axpby(ctx, PP_F64, L.nb, 1.0, x1, -1.0, L.t, x1);  // x1 -= t
+    launch_spmv(ctx, L.E, DT, 1.0, x2, 0.0, t, -1, nullptr, false);  // t = E x2
+    precond_apply(ctx, L.Bpc, DT, t, t);
+    launch_axpby(ctx, DT, L.nb, 1.0, x1, -1.0, t, x1);  // x1 -= t
 }
 
 double dot_dev(pp_ctx *ctx, uint32_t n, const double *a, const double *b) {
@@ -434,11 +476,13 @@ void ml_free(pp_precond *M) {
         pp_csr_destroy(L.E);
         pp_csr_destroy(L.F);
         dfree(L.W); dfree(L.Hd); dfree(L.Md); dfree(L.save); dfree(L.t); dfree(L.h);
+        dfree(L.W32); dfree(L.save32); dfree(L.t32);
     }
     pp_precond_destroy(S->last);
     pp_csr_destroy(S->Clast);
     dfree(S->dperm);
     dfree(S->xp);
+    dfree(S->xp32);
     delete S;
     M->multilevel = nullptr;
 }
@@ -512,14 +556,45 @@ void ml_setup(pp_ctx *ctx, pp_precond *M) {
     }
 }
 
-void ml_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
-    PP_CHECK(dtype == PP_F64, PP_ERR_UNSUPPORTED, "multilevel apply is FP64");
+// FP32 copies of every operand of the apply (before any graph capture:
+// allocations and conversions happen here, not inside an apply)
+void ml_need_f32(pp_ctx *ctx, pp_precond *M) {
     auto *S = static_cast<MlState *>(M->multilevel);
     PP_CHECK(S, PP_ERR_NOT_READY, "multilevel preconditioner not set up");
-    const unsigned g = std::min<uint32_t>((S->n + 255) / 256, 4096);
-    k_gather<<<std::max(1u, g), 256, 0, ctx->stream>>>(S->n, S->dperm, (const double *)x, S->xp);
-    rec_solve(ctx, *S, -1, S->xp);
-    k_scatter<<<std::max(1u, g), 256, 0, ctx->stream>>>(S->n, S->dperm, S->xp, (double *)y);
+    if (S->have32) return;
+    precond_need_f32(S->last);
+    for (MlLevel &L : S->lev) {
+        precond_need_f32(L.Bpc);
+        csr_need_f32(ctx, L.E);
+        csr_need_f32(ctx, L.F);
+        L.t32 = dalloc<float>(std::max(1u, L.nb));
+        if (L.rank) {
+            L.save32 = dalloc<float>(L.nrest);
+            L.W32 = dalloc<float>((size_t)L.nrest * L.rank);
+            const size_t cnt = (size_t)L.nrest * L.rank;
+            k_to_f32<<<(unsigned)std::min<size_t>((cnt + 255) / 256, 4096), 256, 0, ctx->stream>>>(cnt, L.W, L.W32);
+            PP_HIP(hipGetLastError());
+        }
+    }
+    S->xp32 = dalloc<float>(S->n);
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    S->have32 = true;
+}
+
+void ml_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
+    auto *S = static_cast<MlState *>(M->multilevel);
+    PP_CHECK(S, PP_ERR_NOT_READY, "multilevel preconditioner not set up");
+    const unsigned g = std::max(1u, std::min<uint32_t>((S->n + 255) / 256, 4096));
+    if (dtype == PP_F64) {
+        k_gather<double><<<g, 256, 0, ctx->stream>>>(S->n, S->dperm, (const double *)x, S->xp);
+        rec_solve<double>(ctx, *S, -1, S->xp);
+        k_scatter<double><<<g, 256, 0, ctx->stream>>>(S->n, S->dperm, S->xp, (double *)y);
+    } else {
+        PP_CHECK(S->have32, PP_ERR_NOT_READY, "FP32 multilevel apply before its FP32 copies exist");
+        k_gather<float><<<g, 256, 0, ctx->stream>>>(S->n, S->dperm, (const float *)x, S->xp32);
+        rec_solve<float>(ctx, *S, -1, S->xp32);
+        k_scatter<float><<<g, 256, 0, ctx->stream>>>(S->n, S->dperm, S->xp32, (float *)y);
+    }
     PP_HIP(hipGetLastError());
 }
 

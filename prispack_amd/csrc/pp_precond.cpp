@@ -15,6 +15,7 @@ void csr_need_f32(pp_ctx *ctx, pp_csr *A);
 void ml_setup(pp_ctx *ctx, pp_precond *M);
 void ml_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
 void ml_free(pp_precond *M);
+void ml_need_f32(pp_ctx *ctx, pp_precond *M);
 
 template <class T>
 void DevSweep<T>::free_all() {
@@ -285,7 +286,10 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
 }
 
 void precond_need_f32(pp_precond *M) {
-    if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) return;
+    if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) {
+        ml_need_f32(M->ctx, M);
+        return;
+    }
     // every buffer an FP32 apply touches exists before any graph capture
     if (!M->order.empty() && !M->xp32) M->xp32 = dalloc<float>(M->n);
     if (M->trsv == PP_TRSV_SYNCFREE && !M->tmp32) M->tmp32 = dalloc<float>(M->n);

@@ -260,3 +260,32 @@ def test_gmslr_gmres_upwind(pp, torch, rank):
     g.solve(M, torch.tensor(b, device="cuda"), x)
     assert g.getConvergence() and g.report.true_relres < 1e-7
     print(f"[gmslr] rank {rank}: {g.getNumIter()} iterations", flush=True)
+
+
+@pytest.mark.parametrize("kind", ["amsed", "gmslr"])
+def test_multilevel_fp32_apply_and_mixed_solve(pp, torch, kind):
+    # FP32 apply (factors, couplings, low-rank basis in FP32) against the FP64
+    # apply, then the mixed-precision solve to the FP64 tolerance
+    A = ppo.laplacian(24, 24, 24) if kind == "amsed" else ppo.upwind(24, 24, 24, 30.0, 10.0, 5.0)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    P = (pp.AMSED if kind == "amsed" else pp.GMSLR)(M, levels=3, ndom=8, rank=8)
+    P.setup()
+    x = ppo.uniform(n, 6)
+    y64 = torch.tensor(x, device="cuda")
+    P.MInvSolve(y64)
+    y32 = torch.tensor(x, dtype=torch.float32, device="cuda")
+    P.MInvSolve(y32)
+    err = np.linalg.norm(y32.double().cpu().numpy() - y64.cpu().numpy()) / np.linalg.norm(y64.cpu().numpy())
+    print(f"[{kind}] fp32 apply rel err {err:.2e}", flush=True)
+    assert err < 1e-4
+    b = ppo.spmv(A, ppo.normal(n, 2))
+    xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+    if kind == "amsed":
+        s = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision="mixed")
+    else:
+        s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000, precision="mixed")
+        s.setRestart(True, 30)
+    s.solve(M, torch.tensor(b, device="cuda"), xs)
+    print(f"[{kind}] mixed: {s.getNumIter()} iterations, relres {s.report.true_relres:.2e}", flush=True)
+    assert s.getConvergence() and s.report.true_relres < 1e-8

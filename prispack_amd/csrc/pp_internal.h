@@ -195,6 +195,8 @@ struct pp_precond {
     pp::DevSweep<float> low32, up32;
     bool have32 = false;
     int trsv = PP_TRSV_LEVELS;    // resolved SpTRSV schedule
+    bool fuse_mid = false;        // last forward level == first backward level (fused launch)
+    uint32_t fuse_rows = 0;
     std::vector<uint32_t> order;  // factor row k = original row order[k] (empty: natural)
     uint32_t ncolors = 0;
     uint32_t *dorder = nullptr;
@@ -288,6 +290,9 @@ void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &bl
 
 template <class T>
 void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x);
+template <class T>
+void launch_sweep_pair(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup, bool fuse,
+                       T *y);
 template <class T>
 void launch_sweep_syncfree(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *y);
 

@@ -337,7 +337,8 @@ __global__ __launch_bounds__(256) void k_sweep_level(const uint32_t *__restrict_
                                                      const uint32_t *__restrict__ ci,
                                                      const T *__restrict__ val, const T *__restrict__ diag,
                                                      int mode, const T *b, T *x, uint32_t k0, uint32_t k1,
-                                                     long long off, const int *stop) {
+                                                     long long off, const int *stop, int mode2,
+                                                     const T *__restrict__ diag2) {
 #pragma clang fp contract(off)
     const uint32_t k = k0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= k1) return;
@@ -352,6 +353,9 @@ __global__ __launch_bounds__(256) void k_sweep_level(const uint32_t *__restrict_
     }
     if (mode == 0) s = s / diag[i];
     else if (mode == 2) s = s * diag[i];
+    // fused first level of the next sweep (rows without dependencies)
+    if (mode2 == 0 || mode2 == 3) s = s / diag2[i];
+    else if (mode2 == 2) s = s * diag2[i];
     x[i] = s;
 }
 
@@ -369,7 +373,8 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
                                                       const uint32_t *__restrict__ blocks,
                                                       const uint16_t *__restrict__ ci16,
                                                       const uint32_t *__restrict__ blk_base, uint32_t blk0,
-                                                      uint32_t blk1, long long off, const int *stop) {
+                                                      uint32_t blk1, long long off, const int *stop, int mode2,
+                                                      const T *__restrict__ diag2) {
 #pragma clang fp contract(off)
     __shared__ T prod[kSpmvNnzBlock];
     if (stop && *stop) return;
@@ -426,16 +431,21 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
             }
             if (mode == 0) s = s / diag[i];
             else if (mode == 2) s = s * diag[i];
+            if (mode2 == 0 || mode2 == 3) s = s / diag2[i];
+            else if (mode2 == 2) s = s * diag2[i];
             x[i] = s;
         }
         __syncthreads();
     }
 }
 
+// Levels [l0, l1) of one sweep; mode2 >= 0 appends the next sweep's
+// dependency-free first level (diag2) to every row of these levels.
 template <class T>
-void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x) {
+void launch_levels(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x, uint32_t l0, uint32_t l1,
+                   int mode2, const T *diag2) {
     const int *stop = ctx->ctl + C_STOP;
-    for (uint32_t l = 0; l < s.nlev; ++l) {
+    for (uint32_t l = l0; l < l1; ++l) {
         uint32_t k0 = s.level_ptr[l], k1 = s.level_ptr[l + 1];
         const bool contiguous = l < s.level_off.size() && s.level_off[l] != DevSweep<T>::kNoOffset;
         if (s.blocks && 2 * l + 1 < s.level_blk.size() && s.level_blk[2 * l + 1] > s.level_blk[2 * l]) {
@@ -443,16 +453,41 @@ void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x)
             const unsigned g = std::min<unsigned>(b1 - b0, 8192u);
             k_sweep_stream<T><<<g, kBlock, 0, ctx->stream>>>(contiguous ? nullptr : s.perm, s.rp, s.ci, s.val,
                                                              s.diag, mode, b, x, s.blocks, s.ci16, s.blk_base, b0,
-                                                             b1,
-                                                             contiguous ? s.level_off[l] : 0, stop);
+                                                             b1, contiguous ? s.level_off[l] : 0, stop, mode2, diag2);
             continue;
         }
         unsigned g = (k1 - k0 + kBlock - 1) / kBlock;
         k_sweep_level<T><<<g, kBlock, 0, ctx->stream>>>(contiguous ? nullptr : s.perm, s.rp, s.ci, s.val, s.diag,
-                                                        mode, b, x, k0, k1, contiguous ? s.level_off[l] : 0, stop);
+                                                        mode, b, x, k0, k1, contiguous ? s.level_off[l] : 0, stop,
+                                                        mode2, diag2);
     }
     PP_HIP(hipGetLastError());
 }
+
+template <class T>
+void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x) {
+    launch_levels(ctx, s, mode, b, x, 0, s.nlev, -1, (const T *)nullptr);
+}
+
+// Both sweeps in place on y.  fuse: the last level of `lo` holds exactly the
+// rows of the first level of `up` (which have no dependencies), so one
+// kernel finishes the forward sweep and starts the backward one.
+template <class T>
+void launch_sweep_pair(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup, bool fuse,
+                       T *y) {
+    if (!fuse || lo.nlev == 0 || up.nlev == 0) {
+        launch_sweep(ctx, lo, mlo, y, y);
+        launch_sweep(ctx, up, mup, y, y);
+        return;
+    }
+    launch_levels(ctx, lo, mlo, y, y, 0, lo.nlev - 1, -1, (const T *)nullptr);
+    launch_levels(ctx, lo, mlo, y, y, lo.nlev - 1, lo.nlev, mup, up.diag);
+    launch_levels(ctx, up, mup, y, y, 1, up.nlev, -1, (const T *)nullptr);
+}
+template void launch_sweep_pair<double>(pp_ctx *, const DevSweep<double> &, int, const DevSweep<double> &, int,
+                                        bool, double *);
+template void launch_sweep_pair<float>(pp_ctx *, const DevSweep<float> &, int, const DevSweep<float> &, int, bool,
+                                       float *);
 template void launch_sweep<double>(pp_ctx *, const DevSweep<double> &, int, const double *, double *);
 template void launch_sweep<float>(pp_ctx *, const DevSweep<float> &, int, const float *, float *);
 

@@ -270,6 +270,22 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
     } else {
         throw Error(PP_ERR_UNSUPPORTED, "unknown preconditioner kind");
     }
+    // the forward sweep's last level and the backward sweep's first level
+    // (rows without dependencies) hold the same rows: one kernel does both
+    {
+        std::vector<uint32_t> pl, ll, pu, lu;
+        levels_of(M->hlow, pl, ll);
+        levels_of(M->hup, pu, lu);
+        M->fuse_mid = false;
+        M->fuse_rows = 0;
+        if (ll.size() >= 2 && lu.size() >= 2) {
+            const uint32_t a0 = ll[ll.size() - 2], a1 = ll.back(), b0 = lu[0], b1 = lu[1];
+            if (a1 - a0 == b1 - b0 && std::equal(pl.begin() + a0, pl.begin() + a1, pu.begin() + b0)) {
+                M->fuse_mid = true;
+                M->fuse_rows = a1 - a0;
+            }
+        }
+    }
     if (!upload) return;
     if (!M->order.empty()) {
         M->dorder = dalloc<uint32_t>(A.n);
@@ -358,12 +374,10 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
     }
     if (x != y) PP_HIP(hipMemcpyAsync(y, x, (size_t)M->n * esz, hipMemcpyDeviceToDevice, ctx->stream));
     if (dtype == PP_F64) {
-        launch_sweep<double>(ctx, M->low64, M->hlow.mode, (const double *)y, (double *)y);
-        launch_sweep<double>(ctx, M->up64, M->hup.mode, (const double *)y, (double *)y);
+        launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, (double *)y);
     } else {
         precond_need_f32(M);
-        launch_sweep<float>(ctx, M->low32, M->hlow.mode, (const float *)y, (float *)y);
-        launch_sweep<float>(ctx, M->up32, M->hup.mode, (const float *)y, (float *)y);
+        launch_sweep_pair<float>(ctx, M->low32, M->hlow.mode, M->up32, M->hup.mode, M->fuse_mid, (float *)y);
     }
 }
 
@@ -550,6 +564,8 @@ int pp_precond_apply_bytes(const pp_precond *M, int dtype, int in_solve, uint64_
     }
     const uint64_t esz = dtype == PP_F64 ? 8 : 4;
     if (!M->order.empty() && !in_solve) b += 2 * (4ull * M->n + 2 * esz * M->n);  // gather in, scatter out
+    if (M->trsv != PP_TRSV_SYNCFREE && M->fuse_mid)  // the fused level: no second row pointer, read and write
+        b -= (uint64_t)M->fuse_rows * (4 + 2 * esz);
     *bytes = b;
     return PP_OK;
 }

@@ -311,6 +311,8 @@ def main():
         nb = C.c_uint64()
         ctx.check(L.pp_precond_apply_bytes(P.h, dtype, in_solve, C.byref(nb)))
         return int(nb.value)
+    spmv_var = C.c_int()
+    ctx.check(L.pp_csr_spmv_variant(M.h, C.byref(spmv_var)))
     trsv_bytes = apply_bytes_of(0)
     trsv32_bytes = apply_bytes_of(1)
     tv = torch.tensor(xt, device="cuda")
@@ -356,6 +358,7 @@ def main():
                 "unit": "GB/s", "frac": round(gbs(trsv_bytes, trsv_ms) / HBM_PEAK_GBS, 5),
                 "traffic": None, "bytes_per_launch": trsv_bytes, "avg_ms": round(trsv_ms, 4),
                 "fp32_avg_ms": round(trsv32_ms, 4), "fp32_bytes_per_launch": trsv32_bytes,
+                "sweep_variant": info.sweep_variant, "fused_levels": info.fused_levels,
                 "fp32_achieved": round(gbs(trsv32_bytes, trsv32_ms), 2)}
     out = {
         "metric": "time-to-rtol 1e-8 (s) mixed-prec vs FP64 on GPU; SpMV/SpTRSV HBM GB/s",
@@ -374,7 +377,8 @@ def main():
         "roofline": dominant,
         "roofline_spmv": {"kernel": "k_spmv<double>", "bound": "hbm", "achieved": round(gbs(spmv_bytes, spmv_ms), 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs(spmv_bytes, spmv_ms) / HBM_PEAK_GBS, 4),
-                          "traffic": None, "bytes_per_launch": spmv_bytes, "avg_us": round(spmv_ms * 1e3, 2)},
+                          "traffic": None, "bytes_per_launch": spmv_bytes, "avg_us": round(spmv_ms * 1e3, 2),
+                          "variant": spmv_var.value},
         "roofline_spmv_f32": {"achieved": round(gbs(spmv32_bytes, spmv32_ms), 1), "unit": "GB/s",
                               "frac": round(gbs(spmv32_bytes, spmv32_ms) / HBM_PEAK_GBS, 4),
                               "bytes_per_launch": spmv32_bytes, "avg_us": round(spmv32_ms * 1e3, 2)},

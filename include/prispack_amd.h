@@ -82,6 +82,9 @@ int pp_csr_dims(const pp_csr *A, uint32_t *nrow, uint32_t *ncol, uint32_t *nnz);
  * The algorithmic traffic the SpMV roofline is priced on.  No reference
  * counterpart. */
 int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes);
+/* SpMV load variant chosen by timing when the matrix was created (large
+ * matrices): bit 0 non-temporal operand loads, bit 1 16-bit column offsets. */
+int pp_csr_spmv_variant(const pp_csr *A, int *variant);
 /* Device pointers of the stored arrays (values64 FP64; values32 NULL until an
  * FP32 copy exists). */
 int pp_csr_device_ptrs(const pp_csr *A, const uint32_t **rowptr, const uint32_t **colidx,
@@ -165,7 +168,10 @@ typedef struct {
     uint32_t levels_upper;
     double setup_ms;        /* host factorization + analysis + upload */
     uint32_t fail_row;      /* row of a failed pivot */
-    uint32_t reserved[7];
+    int32_t sweep_variant;  /* staged-sweep loads chosen at setup: bit 0 non-temporal,
+                               bit 1 16-bit dependency offsets */
+    int32_t fused_levels;   /* 1: last forward and first backward level share one launch */
+    uint32_t reserved[5];
 } pp_precond_info;
 
 void pp_precond_opts_default(pp_precond_opts *o);

@@ -42,7 +42,7 @@ class PrecondInfo(C.Structure):
     _fields_ = [("kind", C.c_int), ("ready", C.c_int), ("n", C.c_uint32), ("annz", C.c_uint32),
                 ("mnnz", C.c_uint32), ("fill_ratio", C.c_double), ("levels_lower", C.c_uint32),
                 ("levels_upper", C.c_uint32), ("setup_ms", C.c_double), ("fail_row", C.c_uint32),
-                ("reserved", C.c_uint32 * 7)]
+                ("sweep_variant", C.c_int32), ("fused_levels", C.c_int32), ("reserved", C.c_uint32 * 5)]
 
 
 class SolveOpts(C.Structure):
@@ -118,6 +118,7 @@ SIGS = {
     "pp_batch_create": (C.c_int, [vp, u32, u32p, u32p, u32p, f64p, C.POINTER(vp)]),
     "pp_batch_set_values": (C.c_int, [vp, vp, vp, C.c_int]),
     "pp_csr_spmv_bytes": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint64)]),
+    "pp_csr_spmv_variant": (C.c_int, [vp, C.POINTER(C.c_int)]),
     "pp_precond_apply_bytes": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_uint64)]),
     "pp_batch_set_ordering": (C.c_int, [vp, vp, C.c_int]),
     "pp_batch_solve": (C.c_int, [vp, vp, C.POINTER(SolveOpts), vp, vp, i32p, C.POINTER(SolveReport)]),

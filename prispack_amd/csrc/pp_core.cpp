@@ -112,6 +112,8 @@ void csr_need_f32(pp_ctx *ctx, pp_csr *A) {
     if (A->f32_dirty || !A->v32) csr_upload_values_f32(ctx, A);
 }
 
+static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A);
+
 static pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci,
                        const double *v) {
     PP_CHECK(rp != nullptr, PP_ERR_INVALID, "rowptr is NULL");
@@ -169,7 +171,43 @@ static pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t
                               ctx->stream));
     }
     PP_HIP(hipStreamSynchronize(ctx->stream));
+    csr_tune_spmv(ctx, A.get());
     return A.release();
+}
+
+// Times each SpMV load variant (ordinary / non-temporal operand loads, 32-bit
+// / 16-bit column offsets) on this matrix and keeps the fastest.  Only for
+// matrices large enough for the choice to matter.
+static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A) {
+    if (A->nnz < (1u << 20)) return;
+    double *x = dalloc<double>(A->ncol), *y = dalloc<double>(A->nrow);
+    try {
+        launch_fill_zero(ctx, x, (size_t)A->ncol * sizeof(double));
+        float best = 1e30f;
+        int best_var = A->spmv_variant;
+        for (int var = 0; var < 4; ++var) {
+            if ((var & kVarIdx16) && !A->ci16) continue;
+            A->spmv_variant = var;
+            launch_spmv(ctx, A, PP_F64, 1.0, x, 0.0, y, -1, nullptr, false);  // warm-up
+            PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+            for (int rep = 0; rep < 5; ++rep) launch_spmv(ctx, A, PP_F64, 1.0, x, 0.0, y, -1, nullptr, false);
+            PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+            PP_HIP(hipEventSynchronize(ctx->ev1));
+            float ms = 0;
+            PP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ms < best) {
+                best = ms;
+                best_var = var;
+            }
+        }
+        A->spmv_variant = best_var;
+    } catch (...) {
+        dfree(x);
+        dfree(y);
+        throw;
+    }
+    dfree(x);
+    dfree(y);
 }
 
 pp_csr *csr_create_host(pp_ctx *ctx, const HostCsr &H) {
@@ -246,14 +284,15 @@ int pp_ctx_create(int device, pp_ctx **out) {
             clock_khz = 100000;
         c->spin_deadline = (long long)clock_khz * 1000LL * 2;  // a dependency wait may last 2 s
         c->red.partials = dalloc<double>((size_t)S_COUNT * kMaxPartials);
-        c->red.counters = (unsigned *)dalloc<uint32_t>(S_COUNT);
         c->scal = dalloc<double>(S_COUNT);
         c->ctl = dalloc<int>(C_COUNT);
-        PP_HIP(hipMemset(c->red.counters, 0, S_COUNT * sizeof(unsigned)));
         PP_HIP(hipMemset(c->scal, 0, S_COUNT * sizeof(double)));
         PP_HIP(hipMemset(c->ctl, 0, C_COUNT * sizeof(int)));
         PP_HIP(hipHostMalloc((void **)&c->pinned, 640 * sizeof(double), hipHostMallocDefault));
         PP_HIP(hipHostMalloc((void **)&c->pinned_ctl, 64 * sizeof(int), hipHostMallocDefault));
+        PP_HIP(hipHostMalloc((void **)&c->ring_host, pp_ctx::kRing * C_COUNT * sizeof(int),
+                             hipHostMallocMapped | hipHostMallocCoherent));
+        PP_HIP(hipHostGetDevicePointer((void **)&c->ring_dev, c->ring_host, 0));
         PP_HIP(hipEventCreate(&c->ev0));
         PP_HIP(hipEventCreate(&c->ev1));
         for (hipEvent_t &e : c->ev_ring) PP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -280,13 +319,13 @@ void pp_ctx_destroy(pp_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     dfree(ctx->red.partials);
-    dfree(ctx->red.counters);
     dfree(ctx->scal);
     dfree(ctx->ctl);
     dfree(ctx->hist);
     for (int i = 0; i < 32; ++i) dfree(ctx->ws[i]);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->pinned_ctl) (void)hipHostFree(ctx->pinned_ctl);
+    if (ctx->ring_host) (void)hipHostFree(ctx->ring_host);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     for (hipEvent_t e : ctx->ev_ring)
@@ -357,13 +396,20 @@ int pp_csr_set_values(pp_ctx *ctx, pp_csr *A, const double *values, int on_devic
     });
 }
 
+int pp_csr_spmv_variant(const pp_csr *A, int *variant) {
+    if (!A || !variant) return PP_ERR_INVALID;
+    *variant = A->spmv_variant;
+    return PP_OK;
+}
+
 int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes) {
     if (!A || !bytes) return PP_ERR_INVALID;
     const uint64_t esz = dtype == PP_F64 ? 8 : 4;
     uint64_t b = esz * A->nnz;                                    // values
-    b += 2 * A->nnz16 + 4 * ((uint64_t)A->nnz - A->nnz16);       // column indices
+    const uint64_t n16 = (A->spmv_variant & kVarIdx16) ? A->nnz16 : 0;
+    b += 2 * n16 + 4 * ((uint64_t)A->nnz - n16);                 // column indices
     b += 4ull * (A->nrow + 1) + 4ull * (A->nblocks + 1);          // row pointers, row blocks
-    if (A->ci16) b += 4ull * A->nblocks;                          // block bases
+    if (n16) b += 4ull * A->nblocks;                              // block bases
     b += esz * ((uint64_t)A->ncol + A->nrow);                     // x read, y written
     *bytes = b;
     return PP_OK;

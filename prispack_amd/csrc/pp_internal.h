@@ -35,6 +35,9 @@ constexpr int kBlock = 256;             // threads per workgroup of the streamin
 constexpr uint32_t kSpmvNnzBlock = 2048; // nnz staged in LDS per SpMV workgroup
 constexpr uint32_t kStreamLevelRows = 4096; // sweep levels this wide use the LDS-staged kernel
 constexpr uint32_t kNoBase = 0xFFFFFFFFu;   // row block without 16-bit column offsets
+// streaming-kernel variants (bit set): non-temporal operand loads, 16-bit offsets
+constexpr int kVarNT = 1, kVarIdx16 = 2;
+constexpr int kVarDefault = kVarIdx16;
 constexpr uint32_t kSpmvRowsBlock = 256; // rows per SpMV workgroup (one per thread)
 constexpr int kMaxPartials = 4096;       // reduction partials per slot
 
@@ -66,7 +69,6 @@ enum Ctl : int {
 // Deterministic reduction workspace: partials + arrival counter per slot.
 struct RedWs {
     double *partials = nullptr;  // [S_COUNT][kMaxPartials]
-    unsigned *counters = nullptr; // [S_COUNT]
 };
 
 struct Graph {
@@ -91,6 +93,8 @@ struct pp_ctx {
     uint32_t hist_cap = 0;
     double *pinned = nullptr;     // pinned host mirror (scalars, 64 doubles)
     int *pinned_ctl = nullptr;    // pinned host mirror of ctl
+    int *ring_host = nullptr;     // mapped pinned slots the PCG bodies publish ctl into
+    int *ring_dev = nullptr;      // the same memory as the device sees it
     std::vector<double> last_hist;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     static constexpr int kRing = 4;  // iteration bodies kept in flight (pinned_ctl holds kRing slots)
@@ -140,6 +144,7 @@ struct DevSweep {
     uint16_t *ci16 = nullptr;
     uint32_t *blk_base = nullptr;
     uint64_t nnz16 = 0, nblk = 0, perm_rows = 0;  // entries on 16-bit offsets, row blocks, rows read through perm
+    int variant = kVarDefault;                     // kVarNT | kVarIdx16 for the staged kernel
     uint32_t max_level_rows = 0;
     void free_all();
 };
@@ -170,6 +175,7 @@ struct pp_csr {
     uint16_t *ci16 = nullptr;
     uint32_t *blk_base = nullptr;
     uint32_t nblocks16 = 0;          // blocks that use ci16
+    int spmv_variant = pp::kVarDefault;  // chosen by timing at creation for large matrices
     uint64_t nnz16 = 0;              // entries in those blocks
     pp_csr *trans = nullptr;         // explicit transpose, built on demand
     std::vector<uint32_t> h_rp, h_ci;
@@ -196,6 +202,7 @@ struct pp_precond {
     bool have32 = false;
     int trsv = PP_TRSV_LEVELS;    // resolved SpTRSV schedule
     bool fuse_mid = false;        // last forward level == first backward level (fused launch)
+    int sweep_variant = pp::kVarDefault;  // staged-sweep load variant (timed at setup)
     uint32_t fuse_rows = 0;
     std::vector<uint32_t> order;  // factor row k = original row order[k] (empty: natural)
     uint32_t ncolors = 0;
@@ -292,7 +299,7 @@ template <class T>
 void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x);
 template <class T>
 void launch_sweep_pair(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup, bool fuse,
-                       T *y);
+                       const T *x, T *y);
 template <class T>
 void launch_sweep_syncfree(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *y);
 
@@ -305,6 +312,7 @@ void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, 
 template <class T>
 void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v);
 void ctl_init(pp_ctx *ctx, int first_check, int last);
+void launch_ctl_publish(pp_ctx *ctx, int slot_index);
 void copy_slot(pp_ctx *ctx, int dst, int src);
 
 // Reads slot s to the host (synchronizes the stream).

@@ -36,36 +36,20 @@ __device__ inline double block_sum(double v, double *sh) {
     return t;
 }
 
-__device__ inline void store_agent(double *p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline double load_agent(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Streamed operand load: non-temporal (NT) or ordinary.
+template <bool NT, class V>
+__device__ inline V ld_stream(const V *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
 }
 
-// Publishes this workgroup's partial; the last workgroup returns true with the
-// total in *total (thread 0) and writes it to *result.
-__device__ inline bool finish_partial(double blockval, double *partials, unsigned *counter,
-                                      double *result, double *sh, double *total) {
-    __shared__ int am_last;
-    if (threadIdx.x == 0) {
-        store_agent(&partials[blockIdx.x], blockval);
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        am_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
-    double s = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) s += load_agent(&partials[i]);
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) {
-        *result = s;
-        *total = s;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return true;
+// Publishes this workgroup's partial sum (plain store).  A separate
+// one-workgroup k_finish launch adds the partials in index order: the kernel
+// boundary makes them visible, so no workgroup pays a device-wide fence or a
+// contended atomic (on a multi-XCD part an agent-scope fence writes back and
+// invalidates the L2, which costs more than the whole streaming kernel).
+__device__ inline void publish_partial(double blockval, double *partials) {
+    if (threadIdx.x == 0) partials[blockIdx.x] = blockval;
 }
 
 static inline unsigned grid_for(pp_ctx *ctx, uint64_t n, int per_thread = 1) {
@@ -87,7 +71,7 @@ static inline unsigned grid_for(pp_ctx *ctx, uint64_t n, int per_thread = 1) {
 // longer than the LDS stage gets a whole workgroup and a tree sum.
 // Optional fused dot: partial sum of w_row * y_row.
 // ---------------------------------------------------------------------------
-template <class T, bool DOT>
+template <class T, bool DOT, bool NT>
 __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
                                               const uint32_t *__restrict__ ci,
                                               const uint16_t *__restrict__ ci16,
@@ -95,8 +79,7 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
                                               const T *__restrict__ val, const T *__restrict__ x,
                                               T *__restrict__ y, const uint32_t *__restrict__ rb,
                                               uint32_t nb, T alpha, T beta, int beta_zero,
-                                              const T *__restrict__ dotw, double *partials,
-                                              unsigned *counter, double *result, const int *stop) {
+                                              const T *__restrict__ dotw, double *partials, const int *stop) {
     __shared__ T prod[kSpmvNnzBlock];
     __shared__ double sh[4];
     if (stop && *stop) return;
@@ -118,8 +101,8 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
                 for (int j = 0; j < PER; ++j) {
                     const uint32_t k = threadIdx.x + j * kBlock;
                     if (k < cnt) {
-                        c[j] = base + (uint32_t)__builtin_nontemporal_load(ci16 + k0 + k);
-                        v[j] = __builtin_nontemporal_load(val + k0 + k);
+                        c[j] = base + (uint32_t)ld_stream<NT>(ci16 + k0 + k);
+                        v[j] = ld_stream<NT>(val + k0 + k);
                     }
                 }
             } else {
@@ -127,8 +110,8 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
                 for (int j = 0; j < PER; ++j) {
                     const uint32_t k = threadIdx.x + j * kBlock;
                     if (k < cnt) {
-                        c[j] = __builtin_nontemporal_load(ci + k0 + k);
-                        v[j] = __builtin_nontemporal_load(val + k0 + k);
+                        c[j] = ld_stream<NT>(ci + k0 + k);
+                        v[j] = ld_stream<NT>(val + k0 + k);
                     }
                 }
             }
@@ -163,11 +146,7 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
             __syncthreads();
         }
     }
-    if (DOT) {
-        double tot;
-        double bs = block_sum(dacc, sh);
-        finish_partial(bs, partials, counter, result, sh, &tot);
-    }
+    if (DOT) publish_partial(block_sum(dacc, sh), partials);
 }
 
 void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks) {
@@ -187,40 +166,66 @@ void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &bl
     }
 }
 
+// result = sum of partials[0..np) in index order (one workgroup); op
+// FIN_PCG_CHECK also runs the PCG convergence test on it.
+enum FinishOp : int { FIN_SUM = 0, FIN_PCG_CHECK = 1 };
+__device__ void pcg_check(double rr, const double *scal, int *ctl, double tol, uint32_t maxit, double *hist,
+                          uint32_t hist_cap);
+__global__ __launch_bounds__(256) void k_finish(const double *__restrict__ partials, unsigned np, double *result,
+                                                const int *stop, int op, const double *scal, int *ctl, double tol,
+                                                uint32_t maxit, double *hist, uint32_t hist_cap) {
+    __shared__ double sh[4];
+    if (stop && *stop) return;
+    double s = 0.0;
+    for (unsigned i = threadIdx.x; i < np; i += blockDim.x) s += partials[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) {
+        *result = s;
+        if (op == FIN_PCG_CHECK) pcg_check(s, scal, ctl, tol, maxit, hist, hist_cap);
+    }
+}
+
+static void launch_finish(pp_ctx *ctx, int slot, unsigned np, const int *stop, int op = FIN_SUM, double tol = 0.0,
+                          uint32_t maxit = 0, double *hist = nullptr, uint32_t hist_cap = 0) {
+    k_finish<<<1, kBlock, 0, ctx->stream>>>(ctx->red.partials + (size_t)slot * kMaxPartials, np, ctx->scal + slot,
+                                            stop, op, ctx->scal, ctx->ctl, tol, maxit, hist, hist_cap);
+}
+
 void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x, double beta,
                  void *y, int dot_slot, const void *dot_with, bool) {
     unsigned grid = std::min<unsigned>(A->nblocks, kMaxPartials);
     if (grid == 0) return;
     double *part = dot_slot >= 0 ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
-    unsigned *cnt = dot_slot >= 0 ? ctx->red.counters + dot_slot : nullptr;
-    double *res = dot_slot >= 0 ? ctx->scal + dot_slot : nullptr;
-    int bz = beta == 0.0;
+    const int bz = beta == 0.0;
+    const bool dot = dot_slot >= 0;
+    const int var = A->spmv_variant;
+    const bool nt = var & kVarNT;
+    const uint16_t *c16 = (var & kVarIdx16) ? A->ci16 : nullptr;
+#define PP_SPMV(T, DOTV, NTV, VALS, DW, PART)                                                                     \
+    k_spmv<T, DOTV, NTV><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, c16, A->blk_base, VALS, (const T *)x,   \
+                                                          (T *)y, A->row_blocks, A->nblocks, (T)alpha, (T)beta,  \
+                                                          bz, DW, PART, ctx->stop_word)
     if (dtype == PP_F64) {
-        if (dot_slot >= 0)
-            k_spmv<double, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->ci16, A->blk_base, A->v64,
-                                                                 (const double *)x,
-                                                                 (double *)y, A->row_blocks, A->nblocks, alpha, beta, bz,
-                                                                 (const double *)dot_with, part, cnt, res, ctx->stop_word);
-        else
-            k_spmv<double, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->ci16, A->blk_base, A->v64,
-                                                                 (const double *)x,
-                                                                  (double *)y, A->row_blocks, A->nblocks, alpha, beta,
-                                                                  bz, nullptr, nullptr, nullptr, nullptr,
-                                                                  ctx->stop_word);
+        const double *dw = (const double *)dot_with;
+        if (dot) {
+            if (nt) PP_SPMV(double, true, true, A->v64, dw, part);
+            else PP_SPMV(double, true, false, A->v64, dw, part);
+        } else {
+            if (nt) PP_SPMV(double, false, true, A->v64, nullptr, nullptr);
+            else PP_SPMV(double, false, false, A->v64, nullptr, nullptr);
+        }
     } else {
-        if (dot_slot >= 0)
-            k_spmv<float, true><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->ci16, A->blk_base, A->v32,
-                                                                (const float *)x,
-                                                                (float *)y, A->row_blocks, A->nblocks, (float)alpha,
-                                                                (float)beta, bz, (const float *)dot_with, part, cnt, res,
-                                                                ctx->stop_word);
-        else
-            k_spmv<float, false><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, A->ci16, A->blk_base, A->v32,
-                                                                (const float *)x,
-                                                                 (float *)y, A->row_blocks, A->nblocks, (float)alpha,
-                                                                 (float)beta, bz, nullptr, nullptr, nullptr, nullptr,
-                                                                 ctx->stop_word);
+        const float *dw = (const float *)dot_with;
+        if (dot) {
+            if (nt) PP_SPMV(float, true, true, A->v32, dw, part);
+            else PP_SPMV(float, true, false, A->v32, dw, part);
+        } else {
+            if (nt) PP_SPMV(float, false, true, A->v32, nullptr, nullptr);
+            else PP_SPMV(float, false, false, A->v32, nullptr, nullptr);
+        }
     }
+#undef PP_SPMV
+    if (dot) launch_finish(ctx, dot_slot, grid, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
 
@@ -285,26 +290,22 @@ void launch_axpby(pp_ctx *ctx, int dtype, uint32_t n, double a, const void *x, d
 // across threads/workgroups.
 template <class T>
 __global__ __launch_bounds__(256) void k_dot(uint32_t n, const T *x, const T *y, double *partials,
-                                             unsigned *counter, double *result, const int *stop) {
+                                             const int *stop) {
     __shared__ double sh[4];
     if (stop && *stop) return;
     T acc = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += x[i] * y[i];
-    double bs = block_sum((double)acc, sh);
-    double tot;
-    finish_partial(bs, partials, counter, result, sh, &tot);
+    publish_partial(block_sum((double)acc, sh), partials);
 }
 
 void launch_dot(pp_ctx *ctx, int dtype, uint32_t n, const void *x, const void *y, int slot) {
     unsigned g = grid_for(ctx, n, 4);
     double *part = ctx->red.partials + (size_t)slot * kMaxPartials;
-    unsigned *cnt = ctx->red.counters + slot;
     if (dtype == PP_F64)
-        k_dot<double><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)x, (const double *)y, part, cnt,
-                                                     ctx->scal + slot, ctx->stop_word);
+        k_dot<double><<<g, kBlock, 0, ctx->stream>>>(n, (const double *)x, (const double *)y, part, ctx->stop_word);
     else
-        k_dot<float><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)x, (const float *)y, part, cnt,
-                                                    ctx->scal + slot, ctx->stop_word);
+        k_dot<float><<<g, kBlock, 0, ctx->stream>>>(n, (const float *)x, (const float *)y, part, ctx->stop_word);
+    launch_finish(ctx, slot, g, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
 
@@ -364,7 +365,7 @@ __global__ __launch_bounds__(256) void k_sweep_level(const uint32_t *__restrict_
 // products, then one thread per row subtracts them in storage order.  Same
 // operations in the same order as k_sweep_level (bit-identical results).
 // Rows of one level never depend on each other, so x may alias b.
-template <class T>
+template <class T, bool NT>
 __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict__ perm,
                                                       const uint32_t *__restrict__ rp,
                                                       const uint32_t *__restrict__ ci,
@@ -392,8 +393,8 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
                 for (int j = 0; j < PER; ++j) {
                     const uint32_t k = threadIdx.x + j * kBlock;
                     if (k < cnt) {
-                        c[j] = base + (uint32_t)__builtin_nontemporal_load(ci16 + e0 + k);
-                        v[j] = __builtin_nontemporal_load(val + e0 + k);
+                        c[j] = base + (uint32_t)ld_stream<NT>(ci16 + e0 + k);
+                        v[j] = ld_stream<NT>(val + e0 + k);
                     }
                 }
             } else {
@@ -401,8 +402,8 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
                 for (int j = 0; j < PER; ++j) {
                     const uint32_t k = threadIdx.x + j * kBlock;
                     if (k < cnt) {
-                        c[j] = __builtin_nontemporal_load(ci + e0 + k);
-                        v[j] = __builtin_nontemporal_load(val + e0 + k);
+                        c[j] = ld_stream<NT>(ci + e0 + k);
+                        v[j] = ld_stream<NT>(val + e0 + k);
                     }
                 }
             }
@@ -451,9 +452,15 @@ void launch_levels(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x
         if (s.blocks && 2 * l + 1 < s.level_blk.size() && s.level_blk[2 * l + 1] > s.level_blk[2 * l]) {
             const uint32_t b0 = s.level_blk[2 * l], b1 = s.level_blk[2 * l + 1];
             const unsigned g = std::min<unsigned>(b1 - b0, 8192u);
-            k_sweep_stream<T><<<g, kBlock, 0, ctx->stream>>>(contiguous ? nullptr : s.perm, s.rp, s.ci, s.val,
-                                                             s.diag, mode, b, x, s.blocks, s.ci16, s.blk_base, b0,
-                                                             b1, contiguous ? s.level_off[l] : 0, stop, mode2, diag2);
+            const uint16_t *c16 = (s.variant & kVarIdx16) ? s.ci16 : nullptr;
+            if (s.variant & kVarNT)
+                k_sweep_stream<T, true><<<g, kBlock, 0, ctx->stream>>>(
+                    contiguous ? nullptr : s.perm, s.rp, s.ci, s.val, s.diag, mode, b, x, s.blocks, c16, s.blk_base,
+                    b0, b1, contiguous ? s.level_off[l] : 0, stop, mode2, diag2);
+            else
+                k_sweep_stream<T, false><<<g, kBlock, 0, ctx->stream>>>(
+                    contiguous ? nullptr : s.perm, s.rp, s.ci, s.val, s.diag, mode, b, x, s.blocks, c16, s.blk_base,
+                    b0, b1, contiguous ? s.level_off[l] : 0, stop, mode2, diag2);
             continue;
         }
         unsigned g = (k1 - k0 + kBlock - 1) / kBlock;
@@ -469,25 +476,27 @@ void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x)
     launch_levels(ctx, s, mode, b, x, 0, s.nlev, -1, (const T *)nullptr);
 }
 
-// Both sweeps in place on y.  fuse: the last level of `lo` holds exactly the
-// rows of the first level of `up` (which have no dependencies), so one
-// kernel finishes the forward sweep and starts the backward one.
+// y = U^{-1} L^{-1} x: the forward sweep reads its right-hand side from x
+// and writes y (x may alias y), the backward sweep runs in place on y.
+// fuse: the last level of `lo` holds exactly the rows of the first level of
+// `up` (which have no dependencies), so one kernel finishes the forward
+// sweep and starts the backward one.
 template <class T>
 void launch_sweep_pair(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup, bool fuse,
-                       T *y) {
+                       const T *x, T *y) {
     if (!fuse || lo.nlev == 0 || up.nlev == 0) {
-        launch_sweep(ctx, lo, mlo, y, y);
+        launch_sweep(ctx, lo, mlo, x, y);
         launch_sweep(ctx, up, mup, y, y);
         return;
     }
-    launch_levels(ctx, lo, mlo, y, y, 0, lo.nlev - 1, -1, (const T *)nullptr);
-    launch_levels(ctx, lo, mlo, y, y, lo.nlev - 1, lo.nlev, mup, up.diag);
+    launch_levels(ctx, lo, mlo, x, y, 0, lo.nlev - 1, -1, (const T *)nullptr);
+    launch_levels(ctx, lo, mlo, x, y, lo.nlev - 1, lo.nlev, mup, up.diag);
     launch_levels(ctx, up, mup, y, y, 1, up.nlev, -1, (const T *)nullptr);
 }
 template void launch_sweep_pair<double>(pp_ctx *, const DevSweep<double> &, int, const DevSweep<double> &, int,
-                                        bool, double *);
+                                        bool, const double *, double *);
 template void launch_sweep_pair<float>(pp_ctx *, const DevSweep<float> &, int, const DevSweep<float> &, int, bool,
-                                       float *);
+                                       const float *, float *);
 template void launch_sweep<double>(pp_ctx *, const DevSweep<double> &, int, const double *, double *);
 template void launch_sweep<float>(pp_ctx *, const DevSweep<float> &, int, const float *, float *);
 
@@ -599,8 +608,7 @@ template void launch_sweep_syncfree<float>(pp_ctx *, const DevSweep<float> &, in
 template <class T>
 __global__ __launch_bounds__(256) void k_pcg_d(uint32_t n, int first, const T *__restrict__ w,
                                                T *__restrict__ d, const T *__restrict__ r,
-                                               const double *scal, const int *ctl, double *partials,
-                                               unsigned *counter, double *result) {
+                                               const double *scal, const int *ctl, double *partials) {
 #pragma clang fp contract(off)
     __shared__ double sh[4];
     if (ctl[C_STOP]) return;
@@ -617,25 +625,22 @@ __global__ __launch_bounds__(256) void k_pcg_d(uint32_t n, int first, const T *_
         d[i] = dn;
         acc += dn * r[i];
     }
-    double bs = block_sum((double)acc, sh), tot;
-    finish_partial(bs, partials, counter, result, sh, &tot);
+    publish_partial(block_sum((double)acc, sh), partials);
 }
 
 template <class T>
 __global__ __launch_bounds__(256) void k_pcg_wv(uint32_t n, const T *__restrict__ w, const T *__restrict__ v,
-                                                const int *ctl, double *partials, unsigned *counter,
-                                                double *result) {
+                                                const int *ctl, double *partials) {
     __shared__ double sh[4];
     if (ctl[C_STOP]) return;
     T acc = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += w[i] * v[i];
-    double bs = block_sum((double)acc, sh), tot;
-    finish_partial(bs, partials, counter, result, sh, &tot);
+    publish_partial(block_sum((double)acc, sh), partials);
 }
 
-// Convergence check of CG.cpp:57-63 done by the last workgroup: the check
+// Convergence check of CG.cpp:57-63 (run by k_finish on (r, r)): the check
 // index c counts the reference loop index i.
-__device__ inline void pcg_check(double rr, const double *scal, int *ctl, double tol, uint32_t maxit,
+__device__ void pcg_check(double rr, const double *scal, int *ctl, double tol, uint32_t maxit,
                                  double *hist, uint32_t hist_cap) {
     if (ctl[C_LAST]) {  // the body after the final check has run
         ctl[C_STOP] = 1;
@@ -666,9 +671,7 @@ __device__ inline void pcg_check(double rr, const double *scal, int *ctl, double
 template <class T>
 __global__ __launch_bounds__(256) void k_pcg_xr(uint32_t n, T *__restrict__ x, T *__restrict__ r,
                                                 const T *__restrict__ d, const T *__restrict__ v,
-                                                const double *scal, int *ctl, double *partials,
-                                                unsigned *counter, double *result, double tol, uint32_t maxit,
-                                                double *hist, uint32_t hist_cap) {
+                                                const double *scal, int *ctl, double *partials) {
 #pragma clang fp contract(off)
     __shared__ double sh[4];
     if (ctl[C_STOP]) return;
@@ -692,25 +695,23 @@ __global__ __launch_bounds__(256) void k_pcg_xr(uint32_t n, T *__restrict__ x, T
         r[i] = rn;
         acc += rn * rn;
     }
-    double bs = block_sum((double)acc, sh), tot;
-    if (finish_partial(bs, partials, counter, result, sh, &tot) && threadIdx.x == 0)
-        pcg_check(tot, scal, ctl, tol, maxit, hist, hist_cap);
+    publish_partial(block_sum((double)acc, sh), partials);
 }
 
 template <class T>
 void pcg_d_update(pp_ctx *ctx, uint32_t n, bool first, const T *w, T *d, const T *r) {
     unsigned g = grid_for(ctx, n, 4);
     k_pcg_d<T><<<g, kBlock, 0, ctx->stream>>>(n, first ? 1 : 0, w, d, r, ctx->scal, ctx->ctl,
-                                              ctx->red.partials + (size_t)S_DR * kMaxPartials,
-                                              ctx->red.counters + S_DR, ctx->scal + S_DR);
+                                              ctx->red.partials + (size_t)S_DR * kMaxPartials);
+    launch_finish(ctx, S_DR, g, ctx->ctl + C_STOP);
     PP_HIP(hipGetLastError());
 }
 
 template <class T>
 void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v) {
     unsigned g = grid_for(ctx, n, 4);
-    k_pcg_wv<T><<<g, kBlock, 0, ctx->stream>>>(n, w, v, ctx->ctl, ctx->red.partials + (size_t)S_WV * kMaxPartials,
-                                               ctx->red.counters + S_WV, ctx->scal + S_WV);
+    k_pcg_wv<T><<<g, kBlock, 0, ctx->stream>>>(n, w, v, ctx->ctl, ctx->red.partials + (size_t)S_WV * kMaxPartials);
+    launch_finish(ctx, S_WV, g, ctx->ctl + C_STOP);
     PP_HIP(hipGetLastError());
 }
 
@@ -719,9 +720,9 @@ void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, 
                    int hist_on) {
     unsigned g = grid_for(ctx, n, 4);
     k_pcg_xr<T><<<g, kBlock, 0, ctx->stream>>>(n, x, r, d, v, ctx->scal, ctx->ctl,
-                                               ctx->red.partials + (size_t)S_RR * kMaxPartials,
-                                               ctx->red.counters + S_RR, ctx->scal + S_RR, tol, maxit,
-                                               hist_on ? ctx->hist : nullptr, hist_on ? ctx->hist_cap : 0);
+                                               ctx->red.partials + (size_t)S_RR * kMaxPartials);
+    launch_finish(ctx, S_RR, g, ctx->ctl + C_STOP, FIN_PCG_CHECK, tol, maxit, hist_on ? ctx->hist : nullptr,
+                  hist_on ? ctx->hist_cap : 0);
     PP_HIP(hipGetLastError());
 }
 
@@ -744,6 +745,18 @@ __global__ void k_ctl_init(int *ctl, int first_check, int last) {
 }
 
 StopScope::StopScope(pp_ctx *ctx) : c(ctx), prev(ctx->stop_word) { ctx->stop_word = ctx->ctl + C_STOP; }
+
+// Copies the control words into a mapped pinned slot (plain vector stores
+// over the fabric); the event recorded behind this kernel tells the host the
+// slot is complete.  Replaces a device-to-host copy per iteration body.
+__global__ void k_ctl_publish(const int *ctl, int *slot) {
+    if (threadIdx.x < C_COUNT) slot[threadIdx.x] = ctl[threadIdx.x];
+}
+
+void launch_ctl_publish(pp_ctx *ctx, int slot_index) {
+    k_ctl_publish<<<1, 64, 0, ctx->stream>>>(ctx->ctl, ctx->ring_dev + slot_index * C_COUNT);
+    PP_HIP(hipGetLastError());
+}
 
 void ctl_init(pp_ctx *ctx, int first_check, int last) {
     k_ctl_init<<<1, 64, 0, ctx->stream>>>(ctx->ctl, first_check, last);

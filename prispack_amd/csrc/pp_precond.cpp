@@ -312,6 +312,7 @@ void precond_need_f32(pp_precond *M) {
     if (M->have32) return;
     upload_sweep(M->hlow, M->low32);
     upload_sweep(M->hup, M->up32);
+    M->low32.variant = M->up32.variant = M->sweep_variant;
     M->have32 = true;
 }
 
@@ -372,12 +373,13 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
         }
         return;
     }
-    if (x != y) PP_HIP(hipMemcpyAsync(y, x, (size_t)M->n * esz, hipMemcpyDeviceToDevice, ctx->stream));
     if (dtype == PP_F64) {
-        launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, (double *)y);
+        launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid,
+                                  (const double *)x, (double *)y);
     } else {
         precond_need_f32(M);
-        launch_sweep_pair<float>(ctx, M->low32, M->hlow.mode, M->up32, M->hup.mode, M->fuse_mid, (float *)y);
+        launch_sweep_pair<float>(ctx, M->low32, M->hlow.mode, M->up32, M->hup.mode, M->fuse_mid,
+                                 (const float *)x, (float *)y);
     }
 }
 
@@ -431,6 +433,47 @@ int pp_precond_create(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *opts, pp_pr
     });
 }
 
+}  // extern "C"
+
+// Times the staged-sweep load variants (ordinary / non-temporal loads, 32 /
+// 16-bit dependency offsets) on one FP64 apply and keeps the fastest; the
+// FP32 sweeps use the same choice.  Only when a level is wide enough to use
+// the staged kernel and the factor is large.
+static void tune_sweeps(pp_ctx *ctx, pp_precond *M) {
+    if (M->n < (1u << 20) || (!M->low64.blocks && !M->up64.blocks)) return;
+    double *y = dalloc<double>(M->n);
+    try {
+        launch_fill_zero(ctx, y, (size_t)M->n * sizeof(double));
+        float best = 1e30f;
+        int best_var = M->sweep_variant;
+        const bool have16 = M->low64.ci16 || M->up64.ci16;
+        for (int var = 0; var < 4; ++var) {
+            if ((var & kVarIdx16) && !have16) continue;
+            M->low64.variant = M->up64.variant = var;
+            launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, y, y);
+            PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+            for (int rep = 0; rep < 3; ++rep)
+                launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, y, y);
+            PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+            PP_HIP(hipEventSynchronize(ctx->ev1));
+            float ms = 0;
+            PP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            if (ms < best) {
+                best = ms;
+                best_var = var;
+            }
+        }
+        M->sweep_variant = best_var;
+        M->low64.variant = M->up64.variant = M->low32.variant = M->up32.variant = best_var;
+    } catch (...) {
+        dfree(y);
+        throw;
+    }
+    dfree(y);
+}
+
+extern "C" {
+
 int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
     if (!ctx) return PP_ERR_INVALID;
     return guarded_pc(ctx, [&] {
@@ -449,6 +492,7 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
             A.v = M->A->h_v;
             setup_triangular(M, A, true);
             M->trsv = M->opts.trsv == PP_TRSV_AUTO ? PP_TRSV_LEVELS : M->opts.trsv;
+            if (M->trsv == PP_TRSV_LEVELS) tune_sweeps(ctx, M);
         }
         M->ready = 1;
         M->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -459,6 +503,8 @@ int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void 
     if (!ctx) return PP_ERR_INVALID;
     return guarded_pc(ctx, [&] {
         PP_CHECK(M && x && y, PP_ERR_INVALID, "NULL argument");
+        PP_CHECK(M->ready, PP_ERR_NOT_READY, "preconditioner used before setup()");
+        if (dtype == PP_F32) precond_need_f32(M);  // FP32 copies, once, outside any capture
         launch_fill_zero(ctx, ctx->ctl, C_COUNT * sizeof(int));  // clear stop / error words (graph-safe)
         precond_apply(ctx, M, dtype, x, y);
         if (M->trsv == PP_TRSV_SYNCFREE) {
@@ -543,9 +589,10 @@ int pp_precond_apply_part(pp_ctx *ctx, pp_precond *M, int part, int dtype, void 
 template <class T>
 static uint64_t sweep_bytes(const DevSweep<T> &d) {
     const uint64_t esz = sizeof(T);
-    uint64_t b = esz * d.nnz + 2 * d.nnz16 + 4 * (d.nnz - d.nnz16) + 4ull * (d.n + 1) + 4 * d.perm_rows;
+    const uint64_t n16 = (d.variant & kVarIdx16) ? d.nnz16 : 0;
+    uint64_t b = esz * d.nnz + 2 * n16 + 4 * (d.nnz - n16) + 4ull * (d.n + 1) + 4 * d.perm_rows;
     if (d.diag) b += esz * d.n;
-    if (d.nblk) b += 4 * (d.nblk + 1) + (d.ci16 ? 4 * d.nblk : 0);
+    if (d.nblk) b += 4 * (d.nblk + 1) + (n16 ? 4 * d.nblk : 0);
     return b + 3 * esz * d.n;
 }
 extern "C" {
@@ -583,6 +630,8 @@ int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {
     info->levels_upper = M->up64.nlev;
     info->setup_ms = M->setup_ms;
     info->fail_row = M->fail_row;
+    info->sweep_variant = M->sweep_variant;
+    info->fused_levels = M->fuse_mid ? 1 : 0;
     return PP_OK;
 }
 

@@ -113,15 +113,13 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
     // kRing bodies in flight; the host polls their fences (a blocking wait
     // would add an interrupt wake-up per body)
     constexpr int R = pp_ctx::kRing;
-    static_assert(R * C_COUNT <= 64, "pinned control slots");
-    int *slots = ctx->pinned_ctl;
+    int *slots = ctx->ring_host;
     const uint64_t max_replays = (uint64_t)maxit / every + 2;
     uint64_t issued = 0, checked = 0;
     auto issue = [&] {
         if (use_graph) PP_HIP(hipGraphLaunch(G.exec, ctx->stream));
         else body();
-        int *slot = slots + (issued % R) * C_COUNT;
-        PP_HIP(hipMemcpyAsync(slot, ctx->ctl, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        launch_ctl_publish(ctx, (int)(issued % R));
         PP_HIP(hipEventRecord(ctx->ev_ring[issued % R], ctx->stream));
         ++issued;
     };

@@ -258,8 +258,9 @@ def main():
     info = P.info()
     log(f"[bench] IC(0) setup {setup_s:.2f} s, levels {info.levels_lower}/{info.levels_upper}")
 
-    def solve(precision, P=P):
+    def solve(precision, P=P, variant="auto"):
         cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision=precision)
+        cg.setCGVariant(variant)
         cg.opts.record_history = 0
         cg.opts.launch = args.launch
         cg.opts.check_every = args.check_every
@@ -282,11 +283,18 @@ def main():
     for _ in range(args.steps):
         dt, repmx, xmx = solve("mixed")
         tmx.append(dt)
+    # the same single-reduction PCG in FP64: the precision effect alone
+    tsr, repsr = [], None
+    solve("fp64", variant="single_reduction")
+    for _ in range(args.steps):
+        dt, repsr, _ = solve("fp64", variant="single_reduction")
+        tsr.append(dt)
     torch.cuda.synchronize()
     err_mixed = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
     err_true = float(torch.linalg.norm(xmx - d_xt) / torch.linalg.norm(d_xt))
     log(f"[bench] fp64 {np.median(t64):.4f} s ({rep64.iters} it), mixed {np.median(tmx):.4f} s "
-        f"({repmx.outer_iters} outer / {repmx.inner_iters} inner)")
+        f"({repmx.outer_iters} outer / {repmx.inner_iters} inner), fp64 single-reduction "
+        f"{np.median(tsr):.4f} s ({repsr.iters} it)")
 
     # ---- kernel throughput on the library's stream
     stream = L.pp_ctx_stream(ctx.h)
@@ -370,6 +378,8 @@ def main():
                    "ordering": "multicolour" if args.ordering == 1 else "natural"},
         "other_ordering": other,
         "fp64_s": round(med64, 5), "speedup_mixed_vs_fp64": round(med64 / medmx, 3),
+        "fp64_single_reduction_s": round(float(np.median(tsr)), 5), "iters_fp64_single_reduction": repsr.iters,
+        "speedup_mixed_vs_fp64_same_algorithm": round(float(np.median(tsr)) / medmx, 3),
         "iters_fp64": rep64.iters, "iters_mixed_inner": repmx.inner_iters, "outer_mixed": repmx.outer_iters,
         "relres_fp64": rep64.true_relres, "relres_mixed": repmx.true_relres,
         "err_mixed_vs_fp64": err_mixed, "err_vs_xtrue": err_true, "setup_s": round(setup_s, 3),

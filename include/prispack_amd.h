@@ -227,6 +227,12 @@ int pp_gemm_tall_f64(pp_ctx *ctx, uint32_t n, uint32_t k, uint32_t r, const doub
 enum { PP_SOLVER_CG = 0, PP_SOLVER_GMRES = 1 };
 enum { PP_FP64 = 0, PP_MIXED = 1 };
 enum { PP_LAUNCH_GRAPH = 0, PP_LAUNCH_STREAM = 1 };
+/* CG iteration: AUTO = the reference's FlexibleCG / NormalCG for FP64 solves
+ * and the single-reduction PCG for the FP32 inner solves of PP_MIXED;
+ * REFERENCE = FlexibleCG / NormalCG everywhere (CG.cpp:33-83);
+ * SINGLE_REDUCTION = Chronopoulos-Gear PCG everywhere (one SpMV carrying the
+ * three inner products, one fused update; no reference counterpart). */
+enum { PP_CG_AUTO = 0, PP_CG_REFERENCE = 1, PP_CG_SINGLE_REDUCTION = 2 };
 
 typedef struct {
     int solver;          /* PP_SOLVER_* */
@@ -244,7 +250,8 @@ typedef struct {
     int launch;          /* PP_LAUNCH_GRAPH: the iteration body is captured once in a
                             hipGraph and replayed; PP_LAUNCH_STREAM: its kernels are
                             launched on the stream each iteration */
-    uint32_t reserved[7];
+    int cg_variant;      /* PP_CG_* */
+    uint32_t reserved[6];
 } pp_solve_opts;
 
 typedef struct {

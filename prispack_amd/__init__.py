@@ -414,6 +414,13 @@ class _Solver:
         self.opts.precision = PP_MIXED if precision == "mixed" else PP_FP64
         self.report = _capi.SolveReport()
 
+    def setCGVariant(self, variant):
+        """"auto" (reference FlexibleCG in FP64, single-reduction PCG for the
+        FP32 inner solves of mixed precision), "reference" or "single_reduction"."""
+        self.opts.cg_variant = {"auto": _capi.PP_CG_AUTO, "reference": _capi.PP_CG_REFERENCE,
+                                "single_reduction": _capi.PP_CG_SINGLE_REDUCTION}[variant]
+        return self
+
     def solve(self, A, b, x):
         """BaseSolver::solve(A, b, x) (include/Solver/BaseSolver.h:16).  b, x:
         numpy (host in / host out, x0 in x) or CUDA float64 tensors (device-resident)."""

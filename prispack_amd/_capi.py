@@ -19,6 +19,7 @@ PP_TRSV_AUTO, PP_TRSV_LEVELS, PP_TRSV_SYNCFREE, PP_TRSV_BLOCK = range(4)
 PP_SOLVER_CG, PP_SOLVER_GMRES = 0, 1
 PP_FP64, PP_MIXED = 0, 1
 PP_LAUNCH_GRAPH, PP_LAUNCH_STREAM = 0, 1
+PP_CG_AUTO, PP_CG_REFERENCE, PP_CG_SINGLE_REDUCTION = 0, 1, 2
 PP_ORDER_NATURAL, PP_ORDER_MULTICOLOR = 0, 1
 
 STATUS_NAMES = {0: "PP_OK", 1: "PP_ERR_INVALID", 2: "PP_ERR_HIP", 3: "PP_ERR_NOT_SPD", 4: "PP_ERR_ZERO_PIVOT",
@@ -49,7 +50,8 @@ class SolveOpts(C.Structure):
     _fields_ = [("solver", C.c_int), ("flexible", C.c_int), ("precision", C.c_int), ("tol", C.c_double),
                 ("maxit", C.c_uint32), ("restart", C.c_uint32), ("reference_quirks", C.c_int),
                 ("check_every", C.c_uint32), ("inner_tol", C.c_double), ("max_outer", C.c_uint32),
-                ("record_history", C.c_int), ("launch", C.c_int), ("reserved", C.c_uint32 * 7)]
+                ("record_history", C.c_int), ("launch", C.c_int), ("cg_variant", C.c_int),
+                ("reserved", C.c_uint32 * 6)]
 
 
 class SolveReport(C.Structure):

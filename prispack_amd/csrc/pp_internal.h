@@ -52,6 +52,13 @@ enum Slot : int {
     S_TMP1,
     S_TMP2,
     S_TMP3,
+    S_CG_DELTA,  // single-reduction PCG: (w, u), (r, u), (r, r) -- consecutive
+    S_CG_GAMMA,
+    S_CG_RHO,
+    S_CG_ALPHA,  // alpha, beta, previous gamma, first-iteration flag
+    S_CG_BETA,
+    S_CG_GOLD,
+    S_CG_FIRST,
     S_COUNT = 16
 };
 // Integer control words.
@@ -63,10 +70,11 @@ enum Ctl : int {
     C_LAST,       // the next body is the last one
     C_STATUS,     // PP_OK / PP_ERR_BREAKDOWN / PP_ERR_NOT_FINITE
     C_SPIN,       // a sync-free sweep hit its wait deadline
+    C_PUB,        // iteration bodies that have published their control words
     C_COUNT = 16
 };
 
-// Deterministic reduction workspace: partials + arrival counter per slot.
+// Deterministic reduction workspace: one row of per-workgroup partials per slot.
 struct RedWs {
     double *partials = nullptr;  // [S_COUNT][kMaxPartials]
 };
@@ -284,8 +292,12 @@ template <class T>
 void upload_sweep(const HostSweep &s, DevSweep<T> &d);
 
 // ---- kernels (pp_kernels.hip) ----
+// y = alpha A x + beta y; dot_slot >= 0: also (dot_with, y) into that slot;
+// aux != NULL (with dot_slot): also (aux, dot_with) and (aux, aux) into the
+// next two slots, left as partials for the caller's finish kernel
 void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x,
-                 double beta, void *y, int dot_slot, const void *dot_with, bool dot_first);
+                 double beta, void *y, int dot_slot, const void *dot_with, bool dot_first,
+                 const void *aux = nullptr);
 void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes);
 void launch_convert(pp_ctx *ctx, uint32_t n, int sdt, const void *s, int ddt, void *d);
 void launch_axpby(pp_ctx *ctx, int dtype, uint32_t n, double a, const void *x, double b,
@@ -308,11 +320,16 @@ template <class T>
 void pcg_d_update(pp_ctx *ctx, uint32_t n, bool first, const T *w, T *d, const T *r);
 template <class T>
 void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, double tol,
-                   uint32_t maxit, int hist_on);
+                   uint32_t maxit, int hist_on, bool publish);
 template <class T>
 void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v);
+// single-reduction PCG: finish of the three SpMV reductions (+ check), update
+void cgsr_finish(pp_ctx *ctx, unsigned np, double tol, uint32_t maxit, int hist_on, bool publish);
+template <class T>
+void cgsr_update(pp_ctx *ctx, uint32_t n, const T *u, const T *w, T *p, T *s, T *x, T *r);
+unsigned spmv_grid(const pp_csr *A);
+void set_slot(pp_ctx *ctx, int slot, double v);
 void ctl_init(pp_ctx *ctx, int first_check, int last);
-void launch_ctl_publish(pp_ctx *ctx, int slot_index);
 void copy_slot(pp_ctx *ctx, int dst, int src);
 
 // Reads slot s to the host (synchronizes the stream).

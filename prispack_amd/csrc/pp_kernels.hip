@@ -11,9 +11,9 @@ namespace pp {
 
 // ---------------------------------------------------------------------------
 // Deterministic reductions.  A workgroup reduces with a fixed butterfly inside
-// each wavefront and a fixed left-to-right pass over the wavefronts; the last
-// workgroup to finish sums the per-workgroup partials in index order.  The
-// result depends only on the grid shape, never on scheduling.
+// each wavefront and a fixed left-to-right pass over the wavefronts; a
+// one-workgroup k_finish launch then sums the per-workgroup partials in index
+// order.  The result depends only on the grid shape, never on scheduling.
 // ---------------------------------------------------------------------------
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -79,46 +79,29 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
                                               const T *__restrict__ val, const T *__restrict__ x,
                                               T *__restrict__ y, const uint32_t *__restrict__ rb,
                                               uint32_t nb, T alpha, T beta, int beta_zero,
-                                              const T *__restrict__ dotw, double *partials, const int *stop) {
+                                              const T *__restrict__ dotw, const T *__restrict__ aux,
+                                              double *partials, const int *stop) {
     __shared__ T prod[kSpmvNnzBlock];
     __shared__ double sh[4];
     if (stop && *stop) return;
-    double dacc = 0.0;
+    // DOT: (dotw, y); with aux also (aux, dotw) and (aux, aux) into the next
+    // two partial rows (the three reductions of the single-reduction PCG)
+    double dacc = 0.0, dacc2 = 0.0, dacc3 = 0.0;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const uint32_t r0 = rb[b], r1 = rb[b + 1];
         const uint32_t k0 = rp[r0], k1 = rp[r1];
         const uint32_t cnt = k1 - k0;
         if (cnt <= kSpmvNnzBlock) {
-            // all of the block's column indices and values are loaded before
-            // the first gather (PER independent loads in flight per thread);
-            // they stream through once, so they bypass the caches that keep x
-            constexpr int PER = kSpmvNnzBlock / kBlock;
-            uint32_t c[PER];
-            T v[PER];
-            const uint32_t base = ci16 ? blk_base[b] : kNoBase;  // uniform per block
+            // products a_k * x[col_k] into LDS, unit stride over the block's
+            // entries; column offsets 16-bit relative to the block's base
+            // when the block has one (uniform branch)
+            const uint32_t base = ci16 ? blk_base[b] : kNoBase;
             if (base != kNoBase) {
-#pragma unroll
-                for (int j = 0; j < PER; ++j) {
-                    const uint32_t k = threadIdx.x + j * kBlock;
-                    if (k < cnt) {
-                        c[j] = base + (uint32_t)ld_stream<NT>(ci16 + k0 + k);
-                        v[j] = ld_stream<NT>(val + k0 + k);
-                    }
-                }
+                for (uint32_t k = threadIdx.x; k < cnt; k += kBlock)
+                    prod[k] = ld_stream<NT>(val + k0 + k) * x[base + (uint32_t)ld_stream<NT>(ci16 + k0 + k)];
             } else {
-#pragma unroll
-                for (int j = 0; j < PER; ++j) {
-                    const uint32_t k = threadIdx.x + j * kBlock;
-                    if (k < cnt) {
-                        c[j] = ld_stream<NT>(ci + k0 + k);
-                        v[j] = ld_stream<NT>(val + k0 + k);
-                    }
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const uint32_t k = threadIdx.x + j * kBlock;
-                if (k < cnt) prod[k] = v[j] * x[c[j]];
+                for (uint32_t k = threadIdx.x; k < cnt; k += kBlock)
+                    prod[k] = ld_stream<NT>(val + k0 + k) * x[ld_stream<NT>(ci + k0 + k)];
             }
             __syncthreads();
             if (threadIdx.x < r1 - r0) {
@@ -130,7 +113,15 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
                 T out = alpha * s;
                 if (!beta_zero) out += beta * y[row];
                 y[row] = out;
-                if (DOT) dacc += (double)dotw[row] * (double)out;
+                if (DOT) {
+                    const double u = (double)dotw[row];
+                    dacc += u * (double)out;
+                    if (aux) {
+                        const double rr = (double)aux[row];
+                        dacc2 += rr * u;
+                        dacc3 += rr * rr;
+                    }
+                }
             }
             __syncthreads();
         } else {
@@ -141,12 +132,26 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
                 T out = alpha * (T)s;
                 if (!beta_zero) out += beta * y[r0];
                 y[r0] = out;
-                if (DOT) dacc += (double)dotw[r0] * (double)out;
+                if (DOT) {
+                    const double u = (double)dotw[r0];
+                    dacc += u * (double)out;
+                    if (aux) {
+                        const double rr = (double)aux[r0];
+                        dacc2 += rr * u;
+                        dacc3 += rr * rr;
+                    }
+                }
             }
             __syncthreads();
         }
     }
-    if (DOT) publish_partial(block_sum(dacc, sh), partials);
+    if (DOT) {
+        publish_partial(block_sum(dacc, sh), partials);
+        if (aux) {
+            publish_partial(block_sum(dacc2, sh), partials + kMaxPartials);
+            publish_partial(block_sum(dacc3, sh), partials + 2 * kMaxPartials);
+        }
+    }
 }
 
 void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks) {
@@ -171,28 +176,41 @@ void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &bl
 enum FinishOp : int { FIN_SUM = 0, FIN_PCG_CHECK = 1 };
 __device__ void pcg_check(double rr, const double *scal, int *ctl, double tol, uint32_t maxit, double *hist,
                           uint32_t hist_cap);
+// ring != NULL (the last check of an iteration body): afterwards the control
+// words are copied into the next mapped pinned slot for the host (plain
+// vector stores over the fabric), also when the body was skipped.
+__device__ inline void publish_ctl(int *ctl, int *ring) {
+    const int slot = ctl[C_PUB] % pp_ctx::kRing;
+    ctl[C_PUB] = ctl[C_PUB] + 1;
+    for (int i = 0; i < C_COUNT; ++i) ring[slot * C_COUNT + i] = ctl[i];
+}
+
 __global__ __launch_bounds__(256) void k_finish(const double *__restrict__ partials, unsigned np, double *result,
                                                 const int *stop, int op, const double *scal, int *ctl, double tol,
-                                                uint32_t maxit, double *hist, uint32_t hist_cap) {
+                                                uint32_t maxit, double *hist, uint32_t hist_cap, int *ring) {
     __shared__ double sh[4];
-    if (stop && *stop) return;
+    if (stop && *stop) {
+        if (ring && threadIdx.x == 0) publish_ctl(ctl, ring);
+        return;
+    }
     double s = 0.0;
     for (unsigned i = threadIdx.x; i < np; i += blockDim.x) s += partials[i];
     s = block_sum(s, sh);
     if (threadIdx.x == 0) {
         *result = s;
         if (op == FIN_PCG_CHECK) pcg_check(s, scal, ctl, tol, maxit, hist, hist_cap);
+        if (ring) publish_ctl(ctl, ring);
     }
 }
 
 static void launch_finish(pp_ctx *ctx, int slot, unsigned np, const int *stop, int op = FIN_SUM, double tol = 0.0,
-                          uint32_t maxit = 0, double *hist = nullptr, uint32_t hist_cap = 0) {
+                          uint32_t maxit = 0, double *hist = nullptr, uint32_t hist_cap = 0, int *ring = nullptr) {
     k_finish<<<1, kBlock, 0, ctx->stream>>>(ctx->red.partials + (size_t)slot * kMaxPartials, np, ctx->scal + slot,
-                                            stop, op, ctx->scal, ctx->ctl, tol, maxit, hist, hist_cap);
+                                            stop, op, ctx->scal, ctx->ctl, tol, maxit, hist, hist_cap, ring);
 }
 
 void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x, double beta,
-                 void *y, int dot_slot, const void *dot_with, bool) {
+                 void *y, int dot_slot, const void *dot_with, bool, const void *aux) {
     unsigned grid = std::min<unsigned>(A->nblocks, kMaxPartials);
     if (grid == 0) return;
     double *part = dot_slot >= 0 ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
@@ -204,7 +222,7 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
 #define PP_SPMV(T, DOTV, NTV, VALS, DW, PART)                                                                     \
     k_spmv<T, DOTV, NTV><<<grid, kBlock, 0, ctx->stream>>>(A->rp, A->ci, c16, A->blk_base, VALS, (const T *)x,   \
                                                           (T *)y, A->row_blocks, A->nblocks, (T)alpha, (T)beta,  \
-                                                          bz, DW, PART, ctx->stop_word)
+                                                          bz, DW, (const T *)aux, PART, ctx->stop_word)
     if (dtype == PP_F64) {
         const double *dw = (const double *)dot_with;
         if (dot) {
@@ -225,7 +243,7 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
         }
     }
 #undef PP_SPMV
-    if (dot) launch_finish(ctx, dot_slot, grid, ctx->stop_word);
+    if (dot && !aux) launch_finish(ctx, dot_slot, grid, ctx->stop_word);  // with aux the caller finishes
     PP_HIP(hipGetLastError());
 }
 
@@ -379,39 +397,20 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
 #pragma clang fp contract(off)
     __shared__ T prod[kSpmvNnzBlock];
     if (stop && *stop) return;
-    constexpr int PER = kSpmvNnzBlock / kBlock;
     for (uint32_t bi = blk0 + blockIdx.x; bi < blk1; bi += gridDim.x) {
         const uint32_t r0 = blocks[bi], r1 = blocks[bi + 1];
         const uint32_t e0 = rp[r0], cnt = rp[r1] - e0;
         const bool staged = cnt <= kSpmvNnzBlock;
         if (staged) {
-            uint32_t c[PER];
-            T v[PER];
             const uint32_t base = ci16 ? blk_base[bi] : kNoBase;  // uniform per block
             if (base != kNoBase) {
-#pragma unroll
-                for (int j = 0; j < PER; ++j) {
-                    const uint32_t k = threadIdx.x + j * kBlock;
-                    if (k < cnt) {
-                        c[j] = base + (uint32_t)ld_stream<NT>(ci16 + e0 + k);
-                        v[j] = ld_stream<NT>(val + e0 + k);
-                    }
+                for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+                    T p = ld_stream<NT>(val + e0 + k) * x[base + (uint32_t)ld_stream<NT>(ci16 + e0 + k)];
+                    prod[k] = p;
                 }
             } else {
-#pragma unroll
-                for (int j = 0; j < PER; ++j) {
-                    const uint32_t k = threadIdx.x + j * kBlock;
-                    if (k < cnt) {
-                        c[j] = ld_stream<NT>(ci + e0 + k);
-                        v[j] = ld_stream<NT>(val + e0 + k);
-                    }
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < PER; ++j) {
-                const uint32_t k = threadIdx.x + j * kBlock;
-                if (k < cnt) {
-                    T p = v[j] * x[c[j]];
+                for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+                    T p = ld_stream<NT>(val + e0 + k) * x[ld_stream<NT>(ci + e0 + k)];
                     prod[k] = p;
                 }
             }
@@ -717,12 +716,12 @@ void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v) {
 
 template <class T>
 void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, double tol, uint32_t maxit,
-                   int hist_on) {
+                   int hist_on, bool publish) {
     unsigned g = grid_for(ctx, n, 4);
     k_pcg_xr<T><<<g, kBlock, 0, ctx->stream>>>(n, x, r, d, v, ctx->scal, ctx->ctl,
                                                ctx->red.partials + (size_t)S_RR * kMaxPartials);
     launch_finish(ctx, S_RR, g, ctx->ctl + C_STOP, FIN_PCG_CHECK, tol, maxit, hist_on ? ctx->hist : nullptr,
-                  hist_on ? ctx->hist_cap : 0);
+                  hist_on ? ctx->hist_cap : 0, publish ? ctx->ring_dev : nullptr);
     PP_HIP(hipGetLastError());
 }
 
@@ -731,9 +730,95 @@ template void pcg_d_update<float>(pp_ctx *, uint32_t, bool, const float *, float
 template void pcg_wv<double>(pp_ctx *, uint32_t, const double *, const double *);
 template void pcg_wv<float>(pp_ctx *, uint32_t, const float *, const float *);
 template void pcg_xr_update<double>(pp_ctx *, uint32_t, double *, double *, const double *, const double *, double,
-                                    uint32_t, int);
+                                    uint32_t, int, bool);
 template void pcg_xr_update<float>(pp_ctx *, uint32_t, float *, float *, const float *, const float *, double,
-                                   uint32_t, int);
+                                   uint32_t, int, bool);
+
+// ---------------------------------------------------------------------------
+// Single-reduction preconditioned CG (Chronopoulos & Gear, J. Comput. Appl.
+// Math. 25 (1989)): with u = M^{-1} r and w = A u, the three inner products
+// gamma = (r, u), delta = (w, u), rho = (r, r) come out of the SpMV in one
+// reduction; then
+//   beta = gamma / gamma_old,  alpha = gamma / (delta - beta gamma / alpha_old)
+//   p = u + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s
+// (first iteration: beta = 0, alpha = gamma / delta).  Same Krylov iterates
+// as CG in exact arithmetic; three kernels per iteration besides the sweeps.
+// The convergence test is on rho = (r_i, r_i) before the update.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cgsr_finish(const double *__restrict__ partials, unsigned np,
+                                                     double *scal, int *ctl, double tol, uint32_t maxit,
+                                                     double *hist, uint32_t hist_cap, int *ring) {
+    __shared__ double sh[4];
+    if (ctl[C_STOP]) {
+        if (ring && threadIdx.x == 0) publish_ctl(ctl, ring);
+        return;
+    }
+    double v[3];
+    for (int q = 0; q < 3; ++q) {
+        double t = 0.0;
+        for (unsigned i = threadIdx.x; i < np; i += blockDim.x) t += partials[(size_t)q * kMaxPartials + i];
+        v[q] = block_sum(t, sh);
+    }
+    if (threadIdx.x == 0) {
+        const double delta = v[0], gamma = v[1], rho = v[2];
+        scal[S_CG_DELTA] = delta;
+        scal[S_CG_GAMMA] = gamma;
+        scal[S_CG_RHO] = rho;
+        pcg_check(rho, scal, ctl, tol, maxit, hist, hist_cap);
+        if (!ctl[C_STOP]) {
+            const bool first = scal[S_CG_FIRST] != 0.0;
+            const double beta = first ? 0.0 : gamma / scal[S_CG_GOLD];
+            const double den = first ? delta : delta - beta * gamma / scal[S_CG_ALPHA];
+            if (den == 0.0 || !(den == den)) {
+                ctl[C_STATUS] = den == 0.0 ? PP_ERR_BREAKDOWN : PP_ERR_NOT_FINITE;
+                ctl[C_STOP] = 1;
+                ctl[C_M] = ctl[C_CHECK];
+            } else {
+                scal[S_CG_ALPHA] = gamma / den;
+                scal[S_CG_BETA] = beta;
+                scal[S_CG_GOLD] = gamma;
+                scal[S_CG_FIRST] = 0.0;
+            }
+        }
+        if (ring) publish_ctl(ctl, ring);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_cgsr_update(uint32_t n, const T *__restrict__ u, const T *__restrict__ w,
+                                                     T *__restrict__ p, T *__restrict__ s, T *__restrict__ x,
+                                                     T *__restrict__ r, const double *scal, const int *ctl) {
+#pragma clang fp contract(off)
+    if (ctl[C_STOP]) return;
+    const T alpha = (T)scal[S_CG_ALPHA], beta = (T)scal[S_CG_BETA];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const T pn = u[i] + beta * p[i];
+        const T sn = w[i] + beta * s[i];
+        p[i] = pn;
+        s[i] = sn;
+        x[i] = x[i] + alpha * pn;
+        r[i] = r[i] - alpha * sn;
+    }
+}
+
+unsigned spmv_grid(const pp_csr *A) { return std::min<unsigned>(A->nblocks, kMaxPartials); }
+
+void cgsr_finish(pp_ctx *ctx, unsigned np, double tol, uint32_t maxit, int hist_on, bool publish) {
+    k_cgsr_finish<<<1, kBlock, 0, ctx->stream>>>(ctx->red.partials + (size_t)S_CG_DELTA * kMaxPartials, np, ctx->scal,
+                                                 ctx->ctl, tol, maxit, hist_on ? ctx->hist : nullptr,
+                                                 hist_on ? ctx->hist_cap : 0, publish ? ctx->ring_dev : nullptr);
+    PP_HIP(hipGetLastError());
+}
+
+template <class T>
+void cgsr_update(pp_ctx *ctx, uint32_t n, const T *u, const T *w, T *p, T *s, T *x, T *r) {
+    k_cgsr_update<T><<<grid_for(ctx, n, 4), kBlock, 0, ctx->stream>>>(n, u, w, p, s, x, r, ctx->scal, ctx->ctl);
+    PP_HIP(hipGetLastError());
+}
+template void cgsr_update<double>(pp_ctx *, uint32_t, const double *, const double *, double *, double *, double *,
+                                  double *);
+template void cgsr_update<float>(pp_ctx *, uint32_t, const float *, const float *, float *, float *, float *,
+                                 float *);
 
 // Sets the control words for a fresh PCG solve (graph-safe: a kernel, no memset).
 __global__ void k_ctl_init(int *ctl, int first_check, int last) {
@@ -746,20 +831,14 @@ __global__ void k_ctl_init(int *ctl, int first_check, int last) {
 
 StopScope::StopScope(pp_ctx *ctx) : c(ctx), prev(ctx->stop_word) { ctx->stop_word = ctx->ctl + C_STOP; }
 
-// Copies the control words into a mapped pinned slot (plain vector stores
-// over the fabric); the event recorded behind this kernel tells the host the
-// slot is complete.  Replaces a device-to-host copy per iteration body.
-__global__ void k_ctl_publish(const int *ctl, int *slot) {
-    if (threadIdx.x < C_COUNT) slot[threadIdx.x] = ctl[threadIdx.x];
-}
-
-void launch_ctl_publish(pp_ctx *ctx, int slot_index) {
-    k_ctl_publish<<<1, 64, 0, ctx->stream>>>(ctx->ctl, ctx->ring_dev + slot_index * C_COUNT);
+void ctl_init(pp_ctx *ctx, int first_check, int last) {
+    k_ctl_init<<<1, 64, 0, ctx->stream>>>(ctx->ctl, first_check, last);
     PP_HIP(hipGetLastError());
 }
 
-void ctl_init(pp_ctx *ctx, int first_check, int last) {
-    k_ctl_init<<<1, 64, 0, ctx->stream>>>(ctx->ctl, first_check, last);
+__global__ void k_set_slot(double *scal, int slot, double v) { scal[slot] = v; }
+void set_slot(pp_ctx *ctx, int slot, double v) {
+    k_set_slot<<<1, 1, 0, ctx->stream>>>(ctx->scal, slot, v);
     PP_HIP(hipGetLastError());
 }
 

@@ -20,7 +20,7 @@ namespace {
 
 // Workspace indices in the context cache.
 enum { W_R = 0, W_W, W_D, W_V, W_R32, W_W32, W_D32, W_V32, W_X32, W_B32, W_E32, W_VB, W_ZB, W_ZB32, W_T, W_T32,
-       W_PB, W_PX };
+       W_PB, W_PX, W_SR64, W_SR32 = W_SR64 + 5 };  // W_SR*: five vectors each (up to slot 27)
 
 template <class T>
 constexpr int dt() {
@@ -50,52 +50,18 @@ void wait_event(hipEvent_t ev) {
     }
 }
 
-// FlexibleCG on the device (source/Solver/CG.cpp:33-83), working precision T.
-// A, M already hold T copies.  x holds x0.  One v = A d per iteration: the
-// reference's repeated product at CG.cpp:68 returns the same vector.
-template <class T>
-PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const T *b, T *x, double tol,
-                     uint32_t maxit, uint32_t check_every, bool hist_on, int wbase, bool use_graph) {
-    const uint32_t n = A->nrow;
-    const int DT = dt<T>();
-    T *r = (T *)ws_get(ctx, wbase + 0, (size_t)n * sizeof(T));
-    T *w = (T *)ws_get(ctx, wbase + 1, (size_t)n * sizeof(T));
-    T *d = (T *)ws_get(ctx, wbase + 2, (size_t)n * sizeof(T));
-    T *v = (T *)ws_get(ctx, wbase + 3, (size_t)n * sizeof(T));
-    if (hist_on) ensure_hist(ctx, maxit + 2);
-    PcgResult res;
-    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0);
-    // r = b - A x
-    PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-    launch_spmv(ctx, A, DT, -1.0, x, 1.0, r, -1, nullptr, false);
-    launch_dot(ctx, DT, n, b, b, S_BB);
-    // w = M^{-1} r ; d = w ; v = A d ; x1, r1
-    if (M) precond_apply(ctx, M, DT, r, w);
-    else PP_HIP(hipMemcpyAsync(w, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-    pcg_d_update<T>(ctx, n, true, w, d, r);
-    launch_spmv(ctx, A, DT, 1.0, d, 0.0, v, S_DV, d, false);
-    pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on);
-
-    int ctl[C_COUNT];
-    const uint32_t every = std::max<uint32_t>(1, check_every);
-    auto body = [&] {
-        StopScope scope(ctx);  // a body past convergence returns at once, SpMV included
-        for (uint32_t k = 0; k < every; ++k) {
-            if (flexible && M) precond_apply(ctx, M, DT, r, w);
-            else PP_HIP(hipMemcpyAsync(w, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-            pcg_wv<T>(ctx, n, w, v);
-            pcg_d_update<T>(ctx, n, false, w, d, r);
-            launch_spmv(ctx, A, DT, 1.0, d, 0.0, v, S_DV, d, false);
-            pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on);
-        }
-    };
-    // The iteration body (every launch of it: sweeps, updates, SpMV) is
-    // captured once into a hipGraph and replayed, or (use_graph false)
-    // launched on the stream each time.  Two bodies stay in flight: after
-    // each one the control words are copied to pinned memory behind an
-    // event, and the host reads the copy of the previous body, so the GPU
-    // never waits for the host; a body past convergence only runs kernels
-    // that return at once.
+// Runs iteration bodies until the stop word: the body (every launch of it:
+// sweeps, updates, SpMV) is captured once into a hipGraph and replayed, or
+// (use_graph false) launched on the stream each time.  kRing bodies stay in
+// flight: the last convergence check of each body writes the control words
+// into the next slot of a mapped pinned ring, an event follows each body,
+// and the host polls a body's event (a blocking wait would add an interrupt
+// wake-up per body) and then reads its slot, so the GPU never waits for the
+// host; a body past convergence only runs kernels that return at once.
+// check_first: the control words before the first body decide whether any
+// body runs (they are read synchronously).
+template <class Body>
+void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays, int *ctl, bool check_first = true) {
     Graph G;
     if (use_graph) {
         PP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -110,23 +76,22 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
         PP_HIP(hipStreamEndCapture(ctx->stream, &G.g));
         PP_HIP(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
     }
-    // kRing bodies in flight; the host polls their fences (a blocking wait
-    // would add an interrupt wake-up per body)
     constexpr int R = pp_ctx::kRing;
-    int *slots = ctx->ring_host;
-    const uint64_t max_replays = (uint64_t)maxit / every + 2;
+    const int *slots = ctx->ring_host;
     uint64_t issued = 0, checked = 0;
     auto issue = [&] {
         if (use_graph) PP_HIP(hipGraphLaunch(G.exec, ctx->stream));
         else body();
-        launch_ctl_publish(ctx, (int)(issued % R));
         PP_HIP(hipEventRecord(ctx->ev_ring[issued % R], ctx->stream));
         ++issued;
     };
-    // the control words after the first step decide whether any body runs
-    read_ctl(ctx, ctl);
-    PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
-    if (!ctl[C_STOP]) {
+    bool run = true;
+    if (check_first) {
+        read_ctl(ctx, ctl);
+        PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
+        run = !ctl[C_STOP];
+    }
+    if (run) {
         for (;;) {
             while (issued < checked + R && issued < max_replays) issue();
             wait_event(ctx->ev_ring[checked % R]);
@@ -136,15 +101,20 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
             if (ctl[C_STOP] || checked >= max_replays) break;
         }
         PP_HIP(hipStreamSynchronize(ctx->stream));
-        read_ctl(ctx, ctl);
     }
+    read_ctl(ctx, ctl);
+}
+
+// Result of a device PCG from its final control words.
+PcgResult collect_pcg(pp_ctx *ctx, const int *ctl, uint32_t maxit, bool hist_on) {
+    PcgResult res;
     res.conv = ctl[C_CONV];
     res.status = ctl[C_STATUS];
     const int m = ctl[C_M];
     res.iters = m + 1;  // CG.cpp:81
     res.final_error = read_slot(ctx, S_TMP0);
     if (ctl[C_CHECK] <= 1 && !res.conv && maxit <= 1) res.final_error = -1;
-    res.nhist = std::max(0, std::min<int>(m, (int)maxit) - (res.conv ? 0 : 0));
+    res.nhist = std::max(0, std::min<int>(m, (int)maxit));
     if (hist_on) {
         int cnt = res.conv ? m : std::max(0, ctl[C_CHECK] - 1);
         if (!res.conv && ctl[C_LAST]) cnt = (int)maxit - 1;
@@ -154,6 +124,91 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
         res.nhist = cnt;
     }
     return res;
+}
+
+// FlexibleCG on the device (source/Solver/CG.cpp:33-83), working precision T.
+// A, M already hold T copies.  x holds x0.  One v = A d per iteration: the
+// reference's repeated product at CG.cpp:68 returns the same vector.
+template <class T>
+PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const T *b, T *x, double tol,
+                     uint32_t maxit, uint32_t check_every, bool hist_on, int wbase, bool use_graph) {
+    const uint32_t n = A->nrow;
+    const int DT = dt<T>();
+    T *r = (T *)ws_get(ctx, wbase + 0, (size_t)n * sizeof(T));
+    T *w = (T *)ws_get(ctx, wbase + 1, (size_t)n * sizeof(T));
+    T *d = (T *)ws_get(ctx, wbase + 2, (size_t)n * sizeof(T));
+    T *v = (T *)ws_get(ctx, wbase + 3, (size_t)n * sizeof(T));
+    if (hist_on) ensure_hist(ctx, maxit + 2);
+    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0);
+    // r = b - A x
+    PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    launch_spmv(ctx, A, DT, -1.0, x, 1.0, r, -1, nullptr, false);
+    launch_dot(ctx, DT, n, b, b, S_BB);
+    // w = M^{-1} r ; d = w ; v = A d ; x1, r1
+    if (M) precond_apply(ctx, M, DT, r, w);
+    else PP_HIP(hipMemcpyAsync(w, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    pcg_d_update<T>(ctx, n, true, w, d, r);
+    launch_spmv(ctx, A, DT, 1.0, d, 0.0, v, S_DV, d, false);
+    pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on, false);
+
+    int ctl[C_COUNT];
+    const uint32_t every = std::max<uint32_t>(1, check_every);
+    auto body = [&] {
+        StopScope scope(ctx);  // a body past convergence returns at once, SpMV included
+        for (uint32_t k = 0; k < every; ++k) {
+            if (flexible && M) precond_apply(ctx, M, DT, r, w);
+            else PP_HIP(hipMemcpyAsync(w, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+            pcg_wv<T>(ctx, n, w, v);
+            pcg_d_update<T>(ctx, n, false, w, d, r);
+            launch_spmv(ctx, A, DT, 1.0, d, 0.0, v, S_DV, d, false);
+            // the body's last check publishes the control words to the host
+            pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on, k + 1 == every);
+        }
+    };
+    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl);
+    return collect_pcg(ctx, ctl, maxit, hist_on);
+}
+
+// Single-reduction PCG (k_cgsr_finish / k_cgsr_update): per iteration the
+// preconditioner sweeps, one SpMV carrying the three inner products, a
+// one-workgroup finish (test, alpha, beta) and one fused vector update.
+// Used as the FP32 inner solver of the mixed-precision refinement, and on
+// request in FP64.  x holds x0.
+template <class T>
+PcgResult pcg_sr_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const T *b, T *x, double tol, uint32_t maxit,
+                        uint32_t check_every, bool hist_on, bool use_graph) {
+    const uint32_t n = A->nrow;
+    const int DT = dt<T>();
+    const int base = sizeof(T) == 8 ? W_SR64 : W_SR32;
+    T *r = (T *)ws_get(ctx, base + 0, (size_t)n * sizeof(T));
+    T *u = (T *)ws_get(ctx, base + 1, (size_t)n * sizeof(T));
+    T *w = (T *)ws_get(ctx, base + 2, (size_t)n * sizeof(T));
+    T *p = (T *)ws_get(ctx, base + 3, (size_t)n * sizeof(T));
+    T *s = (T *)ws_get(ctx, base + 4, (size_t)n * sizeof(T));
+    if (hist_on) ensure_hist(ctx, maxit + 2);
+    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0);
+    // r = b - A x, p = s = 0, first-iteration flag
+    PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
+    launch_spmv(ctx, A, DT, -1.0, x, 1.0, r, -1, nullptr, false);
+    launch_dot(ctx, DT, n, b, b, S_BB);
+    launch_fill_zero(ctx, p, (size_t)n * sizeof(T));
+    launch_fill_zero(ctx, s, (size_t)n * sizeof(T));
+    set_slot(ctx, S_CG_FIRST, 1.0);
+    const unsigned np = spmv_grid(A);
+    const uint32_t every = std::max<uint32_t>(1, check_every);
+    auto body = [&] {
+        StopScope scope(ctx);
+        for (uint32_t k = 0; k < every; ++k) {
+            if (M) precond_apply(ctx, M, DT, r, u);
+            else launch_convert(ctx, n, DT, r, DT, u);
+            launch_spmv(ctx, A, DT, 1.0, u, 0.0, w, S_CG_DELTA, u, false, r);
+            cgsr_finish(ctx, np, tol, maxit, hist_on, k + 1 == every);
+            cgsr_update<T>(ctx, n, u, w, p, s, x, r);
+        }
+    };
+    int ctl[C_COUNT];
+    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false);
+    return collect_pcg(ctx, ctl, maxit, hist_on);
 }
 
 double true_relres(pp_ctx *ctx, pp_csr *A, const double *b, const double *x) {
@@ -205,9 +260,12 @@ PcgResult pcg_mixed(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts &
         launch_scale_convert(ctx, n, r, 1.0 / rnorm, r32);
         launch_fill_zero(ctx, e32, (size_t)n * sizeof(float));
         const double want = std::max(inner_tol, 0.5 * o.tol * bnorm / rnorm);
-        PcgResult in = pcg_device<float>(ctx, A, M, o.flexible != 0, r32, e32, want,
-                                         std::max<uint32_t>(2, o.maxit - inner_total), o.check_every, false, W_R32,
-                                         o.launch != PP_LAUNCH_STREAM);
+        const uint32_t inner_max = std::max<uint32_t>(2, o.maxit - inner_total);
+        const bool graph = o.launch != PP_LAUNCH_STREAM;
+        PcgResult in = o.cg_variant == PP_CG_REFERENCE
+                           ? pcg_device<float>(ctx, A, M, o.flexible != 0, r32, e32, want, inner_max, o.check_every,
+                                               false, W_R32, graph)
+                           : pcg_sr_device<float>(ctx, A, M, r32, e32, want, inner_max, o.check_every, false, graph);
         inner_total += in.iters;
         ++outer;
         if (in.status != PP_OK && in.status != PP_ERR_NOT_FINITE) {
@@ -481,9 +539,13 @@ static int solve_impl(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts
                 double rel;
                 res = pcg_mixed(ctx, As, M, o, bs, xs, R.outer_iters, R.inner_iters, rel);
             } else {
-                res = pcg_device<double>(ctx, As, M, o.flexible != 0, bs, xs, o.tol, o.maxit,
-                                         o.check_every ? o.check_every : 1, o.record_history != 0, W_R,
-                                         o.launch != PP_LAUNCH_STREAM);
+                if (o.cg_variant == PP_CG_SINGLE_REDUCTION)
+                    res = pcg_sr_device<double>(ctx, As, M, bs, xs, o.tol, o.maxit, o.check_every ? o.check_every : 1,
+                                                o.record_history != 0, o.launch != PP_LAUNCH_STREAM);
+                else
+                    res = pcg_device<double>(ctx, As, M, o.flexible != 0, bs, xs, o.tol, o.maxit,
+                                             o.check_every ? o.check_every : 1, o.record_history != 0, W_R,
+                                             o.launch != PP_LAUNCH_STREAM);
             }
         } else {
             res = gmres_device(ctx, As, M, o, bs, xs);

@@ -344,3 +344,31 @@ def test_gmres_mixed(pp, torch):
     o = ppo.gmres(A, b, ppo.Prec(A, "ilu0"), tol=1e-10, restart=30)
     err = np.linalg.norm(host(x) - o["x"]) / np.linalg.norm(o["x"])
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["ic0", "ict"])
+def test_single_reduction_pcg(pp, torch, kind):
+    # Chronopoulos-Gear PCG: same Krylov iterates as CG in exact arithmetic, so
+    # iteration counts within a few of FlexibleCG and the same solution
+    A = ppo.laplacian(24, 22, 20)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 4))
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M) if kind == "ic0" else pp.IncompleteCholesky(M, 1e-3)
+    out = {}
+    for variant in ("reference", "single_reduction"):
+        cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-10, maxIter=2000).setCGVariant(variant)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        cg.solve(M, dev(torch, b), x)
+        assert cg.getConvergence() and cg.report.true_relres < 1e-9, variant
+        out[variant] = (cg.getNumIter(), host(x))
+    print(f"[sr] {kind}: reference {out['reference'][0]}, single reduction {out['single_reduction'][0]}", flush=True)
+    assert abs(out["reference"][0] - out["single_reduction"][0]) <= 3
+    xr, xs = out["reference"][1], out["single_reduction"][1]
+    assert np.linalg.norm(xs - xr) / np.linalg.norm(xr) <= 1e-7
+    # mixed precision with each inner variant reaches the FP64 tolerance
+    for variant in ("reference", "single_reduction"):
+        cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision="mixed").setCGVariant(variant)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        cg.solve(M, dev(torch, b), x)
+        assert cg.getConvergence() and cg.report.true_relres < 1e-8, variant

@@ -295,7 +295,6 @@ int pp_ctx_create(int device, pp_ctx **out) {
         PP_HIP(hipHostGetDevicePointer((void **)&c->ring_dev, c->ring_host, 0));
         PP_HIP(hipEventCreate(&c->ev0));
         PP_HIP(hipEventCreate(&c->ev1));
-        for (hipEvent_t &e : c->ev_ring) PP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         ensure_hist(c.get(), 4096);
         PP_HIP(hipDeviceSynchronize());
         *out = c.release();
@@ -328,8 +327,6 @@ void pp_ctx_destroy(pp_ctx *ctx) {
     if (ctx->ring_host) (void)hipHostFree(ctx->ring_host);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
-    for (hipEvent_t e : ctx->ev_ring)
-        if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }

@@ -105,8 +105,7 @@ struct pp_ctx {
     int *ring_dev = nullptr;      // the same memory as the device sees it
     std::vector<double> last_hist;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    static constexpr int kRing = 4;  // iteration bodies kept in flight (pinned_ctl holds kRing slots)
-    hipEvent_t ev_ring[kRing] = {};   // their fences (no timing)
+    static constexpr int kRing = 4;  // iteration bodies kept in flight (ring_host holds kRing slots)
     void *ws[32] = {};             // cached solver workspaces
     size_t ws_bytes[32] = {};
     long long spin_deadline = 0;   // wall_clock64 ticks a sync-free wait may last

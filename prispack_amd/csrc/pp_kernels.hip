@@ -178,24 +178,45 @@ __device__ void pcg_check(double rr, const double *scal, int *ctl, double tol, u
                           uint32_t hist_cap);
 // ring != NULL (the last check of an iteration body): afterwards the control
 // words are copied into the next mapped pinned slot for the host (plain
-// vector stores over the fabric), also when the body was skipped.
+// vector stores over the fabric), also when the body was skipped.  The
+// slot's C_PUB word, written last behind a system-scope fence, is the
+// body's sequence number: the host polls it instead of waiting on an event.
 __device__ inline void publish_ctl(int *ctl, int *ring) {
-    const int slot = ctl[C_PUB] % pp_ctx::kRing;
-    ctl[C_PUB] = ctl[C_PUB] + 1;
-    for (int i = 0; i < C_COUNT; ++i) ring[slot * C_COUNT + i] = ctl[i];
+    const int seq = ctl[C_PUB] + 1;
+    ctl[C_PUB] = seq;
+    int *slot = ring + ((seq - 1) % pp_ctx::kRing) * C_COUNT;
+    for (int i = 0; i < C_COUNT; ++i)
+        if (i != C_PUB) slot[i] = ctl[i];
+    __threadfence_system();
+    slot[C_PUB] = seq;
 }
 
-__global__ __launch_bounds__(256) void k_finish(const double *__restrict__ partials, unsigned np, double *result,
-                                                const int *stop, int op, const double *scal, int *ctl, double tol,
-                                                uint32_t maxit, double *hist, uint32_t hist_cap, int *ring) {
-    __shared__ double sh[4];
+// One workgroup of kFinishThreads: each thread loads its (at most four)
+// partials before adding them, so the loads overlap instead of forming a
+// chain of memory latencies.
+constexpr int kFinishThreads = 1024;
+__device__ inline double sum_partials(const double *__restrict__ partials, unsigned np) {
+    double v[kMaxPartials / kFinishThreads];
+#pragma unroll
+    for (int j = 0; j < kMaxPartials / kFinishThreads; ++j) {
+        const unsigned i = threadIdx.x + j * kFinishThreads;
+        v[j] = i < np ? partials[i] : 0.0;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < kMaxPartials / kFinishThreads; ++j) s += v[j];
+    return s;
+}
+
+__global__ __launch_bounds__(1024) void k_finish(const double *__restrict__ partials, unsigned np, double *result,
+                                                 const int *stop, int op, const double *scal, int *ctl, double tol,
+                                                 uint32_t maxit, double *hist, uint32_t hist_cap, int *ring) {
+    __shared__ double sh[kFinishThreads / 64];
     if (stop && *stop) {
         if (ring && threadIdx.x == 0) publish_ctl(ctl, ring);
         return;
     }
-    double s = 0.0;
-    for (unsigned i = threadIdx.x; i < np; i += blockDim.x) s += partials[i];
-    s = block_sum(s, sh);
+    double s = block_sum(sum_partials(partials, np), sh);
     if (threadIdx.x == 0) {
         *result = s;
         if (op == FIN_PCG_CHECK) pcg_check(s, scal, ctl, tol, maxit, hist, hist_cap);
@@ -205,8 +226,9 @@ __global__ __launch_bounds__(256) void k_finish(const double *__restrict__ parti
 
 static void launch_finish(pp_ctx *ctx, int slot, unsigned np, const int *stop, int op = FIN_SUM, double tol = 0.0,
                           uint32_t maxit = 0, double *hist = nullptr, uint32_t hist_cap = 0, int *ring = nullptr) {
-    k_finish<<<1, kBlock, 0, ctx->stream>>>(ctx->red.partials + (size_t)slot * kMaxPartials, np, ctx->scal + slot,
-                                            stop, op, ctx->scal, ctx->ctl, tol, maxit, hist, hist_cap, ring);
+    k_finish<<<1, kFinishThreads, 0, ctx->stream>>>(ctx->red.partials + (size_t)slot * kMaxPartials, np,
+                                                    ctx->scal + slot, stop, op, ctx->scal, ctx->ctl, tol, maxit, hist,
+                                                    hist_cap, ring);
 }
 
 void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x, double beta,
@@ -745,20 +767,20 @@ template void pcg_xr_update<float>(pp_ctx *, uint32_t, float *, float *, const f
 // as CG in exact arithmetic; three kernels per iteration besides the sweeps.
 // The convergence test is on rho = (r_i, r_i) before the update.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_cgsr_finish(const double *__restrict__ partials, unsigned np,
-                                                     double *scal, int *ctl, double tol, uint32_t maxit,
-                                                     double *hist, uint32_t hist_cap, int *ring) {
-    __shared__ double sh[4];
+__global__ __launch_bounds__(1024) void k_cgsr_finish(const double *__restrict__ partials, unsigned np,
+                                                      double *scal, int *ctl, double tol, uint32_t maxit,
+                                                      double *hist, uint32_t hist_cap, int *ring) {
+    __shared__ double sh[kFinishThreads / 64];
     if (ctl[C_STOP]) {
         if (ring && threadIdx.x == 0) publish_ctl(ctl, ring);
         return;
     }
+    double t[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) t[q] = sum_partials(partials + (size_t)q * kMaxPartials, np);
     double v[3];
-    for (int q = 0; q < 3; ++q) {
-        double t = 0.0;
-        for (unsigned i = threadIdx.x; i < np; i += blockDim.x) t += partials[(size_t)q * kMaxPartials + i];
-        v[q] = block_sum(t, sh);
-    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[q] = block_sum(t[q], sh);
     if (threadIdx.x == 0) {
         const double delta = v[0], gamma = v[1], rho = v[2];
         scal[S_CG_DELTA] = delta;
@@ -804,7 +826,8 @@ __global__ __launch_bounds__(256) void k_cgsr_update(uint32_t n, const T *__rest
 unsigned spmv_grid(const pp_csr *A) { return std::min<unsigned>(A->nblocks, kMaxPartials); }
 
 void cgsr_finish(pp_ctx *ctx, unsigned np, double tol, uint32_t maxit, int hist_on, bool publish) {
-    k_cgsr_finish<<<1, kBlock, 0, ctx->stream>>>(ctx->red.partials + (size_t)S_CG_DELTA * kMaxPartials, np, ctx->scal,
+    k_cgsr_finish<<<1, kFinishThreads, 0, ctx->stream>>>(ctx->red.partials + (size_t)S_CG_DELTA * kMaxPartials, np,
+                                                         ctx->scal,
                                                  ctx->ctl, tol, maxit, hist_on ? ctx->hist : nullptr,
                                                  hist_on ? ctx->hist_cap : 0, publish ? ctx->ring_dev : nullptr);
     PP_HIP(hipGetLastError());

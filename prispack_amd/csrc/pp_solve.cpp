@@ -2,7 +2,6 @@
 // mixed precision = FP32 inner PCG inside FP64 iterative refinement) and
 // FGMRES(m) with CGS2 orthogonalisation.
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstring>
 
@@ -35,29 +34,14 @@ struct PcgResult {
     int nhist = 0;
 };
 
-// Wait for an event by polling it (no interrupt wake-up); after 20 ms of
-// polling, fall back to a blocking wait.
-void wait_event(hipEvent_t ev) {
-    const auto t0 = std::chrono::steady_clock::now();
-    for (;;) {
-        const hipError_t e = hipEventQuery(ev);
-        if (e == hipSuccess) return;
-        if (e != hipErrorNotReady) PP_HIP(e);
-        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
-            PP_HIP(hipEventSynchronize(ev));
-            return;
-        }
-    }
-}
-
 // Runs iteration bodies until the stop word: the body (every launch of it:
 // sweeps, updates, SpMV) is captured once into a hipGraph and replayed, or
 // (use_graph false) launched on the stream each time.  kRing bodies stay in
 // flight: the last convergence check of each body writes the control words
-// into the next slot of a mapped pinned ring, an event follows each body,
-// and the host polls a body's event (a blocking wait would add an interrupt
-// wake-up per body) and then reads its slot, so the GPU never waits for the
-// host; a body past convergence only runs kernels that return at once.
+// into the next slot of a mapped pinned ring, sequence number last, and the
+// host polls that number (no event or blocking wait between bodies, so
+// consecutive graph launches run back to back), so the GPU never waits for
+// the host; a body past convergence only runs kernels that return at once.
 // check_first: the control words before the first body decide whether any
 // body runs (they are read synchronously).
 template <class Body>
@@ -77,13 +61,27 @@ void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays,
         PP_HIP(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
     }
     constexpr int R = pp_ctx::kRing;
-    const int *slots = ctx->ring_host;
+    volatile int *slots = ctx->ring_host;
+    for (int i = 0; i < R * C_COUNT; ++i) slots[i] = 0;  // no sequence number of an earlier solve remains
     uint64_t issued = 0, checked = 0;
     auto issue = [&] {
         if (use_graph) PP_HIP(hipGraphLaunch(G.exec, ctx->stream));
         else body();
-        PP_HIP(hipEventRecord(ctx->ev_ring[issued % R], ctx->stream));
         ++issued;
+    };
+    // body k has published when its slot carries sequence number k + 1; the
+    // stream is queried now and then so that a failed launch cannot leave
+    // the host waiting (an idle stream with no publication ends the wait)
+    auto wait_published = [&](uint64_t k) -> bool {
+        volatile int *slot = slots + (k % R) * C_COUNT;
+        for (uint64_t spin = 1;; ++spin) {
+            if ((uint64_t)slot[C_PUB] == k + 1) return true;
+            if ((spin & 1023) == 0) {
+                const hipError_t e = hipStreamQuery(ctx->stream);
+                if (e == hipSuccess) return (uint64_t)slot[C_PUB] == k + 1;
+                if (e != hipErrorNotReady) PP_HIP(e);
+            }
+        }
     };
     bool run = true;
     if (check_first) {
@@ -94,8 +92,8 @@ void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays,
     if (run) {
         for (;;) {
             while (issued < checked + R && issued < max_replays) issue();
-            wait_event(ctx->ev_ring[checked % R]);
-            std::memcpy(ctl, slots + (checked % R) * C_COUNT, C_COUNT * sizeof(int));
+            if (!wait_published(checked)) break;  // read the words synchronously below
+            for (int i = 0; i < C_COUNT; ++i) ctl[i] = slots[(checked % R) * C_COUNT + i];
             ++checked;
             PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
             if (ctl[C_STOP] || checked >= max_replays) break;

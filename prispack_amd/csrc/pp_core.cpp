@@ -114,6 +114,31 @@ void csr_need_f32(pp_ctx *ctx, pp_csr *A) {
 
 static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A);
 
+bool two_base_offsets(const uint32_t *cols, uint32_t cnt, uint32_t &base_lo, uint32_t &base_hi, uint16_t *out) {
+    if (cnt == 0) return false;
+    std::vector<uint32_t> u(cols, cols + cnt);
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    // split at the widest gap between consecutive distinct columns
+    size_t split = u.size();  // first column of the high window
+    uint32_t widest = 0;
+    for (size_t i = 1; i < u.size(); ++i)
+        if (u[i] - u[i - 1] > widest) {
+            widest = u[i] - u[i - 1];
+            split = i;
+        }
+    if (u.back() - u.front() <= kOffMask) split = u.size();  // one window suffices
+    base_lo = u.front();
+    if (u[split - 1] - base_lo > kOffMask) return false;
+    base_hi = split < u.size() ? u[split] : base_lo;
+    if (split < u.size() && u.back() - base_hi > kOffMask) return false;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const bool hi = split < u.size() && cols[k] >= base_hi;
+        out[k] = hi ? (uint16_t)(kHiBase | (cols[k] - base_hi)) : (uint16_t)(cols[k] - base_lo);
+    }
+    return true;
+}
+
 static pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci,
                        const double *v) {
     PP_CHECK(rp != nullptr, PP_ERR_INVALID, "rowptr is NULL");
@@ -144,30 +169,25 @@ static pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t
     A->row_blocks = dalloc<uint32_t>(blocks.size());
     PP_HIP(hipMemcpyAsync(A->row_blocks, blocks.data(), blocks.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                           ctx->stream));
-    // 16-bit block-relative column offsets where a block's column span fits
-    std::vector<uint32_t> base(A->nblocks, kNoBase);
-    std::vector<uint16_t> off16;
+    // 16-bit two-base column offsets where a block's columns fit two windows
+    std::vector<uint32_t> base(2 * (size_t)A->nblocks, kNoBase);
+    std::vector<uint16_t> off16(nnz, 0);
     A->nblocks16 = 0;
     for (uint32_t bi = 0; bi < A->nblocks; ++bi) {
         const uint32_t k0 = rp[blocks[bi]], k1 = rp[blocks[bi + 1]];
         if (k1 == k0 || k1 - k0 > kSpmvNnzBlock) continue;
-        uint32_t lo = ci[k0], hi = ci[k0];
-        for (uint32_t k = k0; k < k1; ++k) {
-            lo = std::min(lo, ci[k]);
-            hi = std::max(hi, ci[k]);
-        }
-        if (hi - lo > 0xFFFFu) continue;
-        if (off16.empty()) off16.assign(nnz, 0);
-        base[bi] = lo;
-        for (uint32_t k = k0; k < k1; ++k) off16[k] = (uint16_t)(ci[k] - lo);
+        uint32_t lo = 0, hi = 0;
+        if (!two_base_offsets(ci + k0, k1 - k0, lo, hi, off16.data() + k0)) continue;
+        base[2 * bi] = lo;
+        base[2 * bi + 1] = hi;
         ++A->nblocks16;
         A->nnz16 += k1 - k0;
     }
     if (A->nblocks16) {
         A->ci16 = dalloc<uint16_t>(nnz);
-        A->blk_base = dalloc<uint32_t>(A->nblocks);
+        A->blk_base = dalloc<uint32_t>(base.size());
         PP_HIP(hipMemcpyAsync(A->ci16, off16.data(), nnz * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
-        PP_HIP(hipMemcpyAsync(A->blk_base, base.data(), A->nblocks * sizeof(uint32_t), hipMemcpyHostToDevice,
+        PP_HIP(hipMemcpyAsync(A->blk_base, base.data(), base.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                               ctx->stream));
     }
     PP_HIP(hipStreamSynchronize(ctx->stream));
@@ -406,7 +426,7 @@ int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes) {
     const uint64_t n16 = (A->spmv_variant & kVarIdx16) ? A->nnz16 : 0;
     b += 2 * n16 + 4 * ((uint64_t)A->nnz - n16);                 // column indices
     b += 4ull * (A->nrow + 1) + 4ull * (A->nblocks + 1);          // row pointers, row blocks
-    if (n16) b += 4ull * A->nblocks;                              // block bases
+    if (n16) b += 8ull * A->nblocks;                              // block bases
     b += esz * ((uint64_t)A->ncol + A->nrow);                     // x read, y written
     *bytes = b;
     return PP_OK;

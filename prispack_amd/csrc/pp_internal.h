@@ -35,6 +35,12 @@ constexpr int kBlock = 256;             // threads per workgroup of the streamin
 constexpr uint32_t kSpmvNnzBlock = 2048; // nnz staged in LDS per SpMV workgroup
 constexpr uint32_t kStreamLevelRows = 4096; // sweep levels this wide use the LDS-staged kernel
 constexpr uint32_t kNoBase = 0xFFFFFFFFu;   // row block without 16-bit column offsets
+// 16-bit column offsets of a row block: bit 15 selects one of the block's two
+// bases, bits 0-14 the distance from it (two windows of 32768 columns each,
+// e.g. the two colour halves of a colour-ordered matrix)
+constexpr uint16_t kHiBase = 0x8000u, kOffMask = 0x7FFFu;
+// Encodes cols[0..cnt) as two-base offsets; false if they do not fit.
+bool two_base_offsets(const uint32_t *cols, uint32_t cnt, uint32_t &base_lo, uint32_t &base_hi, uint16_t *out);
 // streaming-kernel variants (bit set): non-temporal operand loads, 16-bit offsets
 constexpr int kVarNT = 1, kVarIdx16 = 2;
 constexpr int kVarDefault = kVarIdx16;

@@ -92,32 +92,49 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
         const uint32_t k0 = rp[r0], k1 = rp[r1];
         const uint32_t cnt = k1 - k0;
         if (cnt <= kSpmvNnzBlock) {
+            // the row operands of phase 2 are loaded first, so their latency
+            // overlaps the product stage
+            const bool mine = threadIdx.x < r1 - r0;
+            const uint32_t row = r0 + threadIdx.x;
+            uint32_t pa = 0, pe = 0;
+            T yold = 0, dw = 0, ax = 0;
+            if (mine) {
+                pa = rp[row];
+                pe = rp[row + 1];
+                if (!beta_zero) yold = y[row];
+                if (DOT) {
+                    dw = dotw[row];
+                    if (aux) ax = aux[row];
+                }
+            }
             // products a_k * x[col_k] into LDS, unit stride over the block's
-            // entries; column offsets 16-bit relative to the block's base
-            // when the block has one (uniform branch)
-            const uint32_t base = ci16 ? blk_base[b] : kNoBase;
-            if (base != kNoBase) {
-                for (uint32_t k = threadIdx.x; k < cnt; k += kBlock)
-                    prod[k] = ld_stream<NT>(val + k0 + k) * x[base + (uint32_t)ld_stream<NT>(ci16 + k0 + k)];
+            // entries; 16-bit column offsets from one of the block's two
+            // bases when the block has them (uniform branch)
+            const uint32_t blo = ci16 ? blk_base[2 * b] : kNoBase;
+            if (blo != kNoBase) {
+                const uint32_t bhi = blk_base[2 * b + 1];
+                for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+                    const uint32_t o = ld_stream<NT>(ci16 + k0 + k);
+                    prod[k] = ld_stream<NT>(val + k0 + k) * x[((o & kHiBase) ? bhi : blo) + (o & kOffMask)];
+                }
             } else {
                 for (uint32_t k = threadIdx.x; k < cnt; k += kBlock)
                     prod[k] = ld_stream<NT>(val + k0 + k) * x[ld_stream<NT>(ci + k0 + k)];
             }
             __syncthreads();
-            if (threadIdx.x < r1 - r0) {
-                const uint32_t row = r0 + threadIdx.x;
-                uint32_t a = rp[row] - k0;
-                const uint32_t e = rp[row + 1] - k0;
+            if (mine) {
+                uint32_t a = pa - k0;
+                const uint32_t e = pe - k0;
                 T s = 0;
                 for (; a < e; ++a) s += prod[a];
                 T out = alpha * s;
-                if (!beta_zero) out += beta * y[row];
+                if (!beta_zero) out += beta * yold;
                 y[row] = out;
                 if (DOT) {
-                    const double u = (double)dotw[row];
+                    const double u = (double)dw;
                     dacc += u * (double)out;
                     if (aux) {
-                        const double rr = (double)aux[row];
+                        const double rr = (double)ax;
                         dacc2 += rr * u;
                         dacc3 += rr * rr;
                     }
@@ -423,11 +440,27 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
         const uint32_t r0 = blocks[bi], r1 = blocks[bi + 1];
         const uint32_t e0 = rp[r0], cnt = rp[r1] - e0;
         const bool staged = cnt <= kSpmvNnzBlock;
+        // row operands first (their latency overlaps the product stage)
+        static_assert(kSpmvRowsBlock <= (uint32_t)kBlock, "one row per thread in phase 2");
+        const bool mine = r0 + threadIdx.x < r1;
+        uint32_t pi = 0, pa = 0, pe = 0;
+        T pb = 0, pd = 0, pd2 = 0;
+        if (mine) {
+            const uint32_t kk = r0 + threadIdx.x;
+            pi = perm ? perm[kk] : (uint32_t)((long long)kk + off);
+            pa = rp[kk];
+            pe = rp[kk + 1];
+            pb = b[pi];
+            if (mode != 1) pd = diag[pi];
+            if (mode2 == 0 || mode2 == 2 || mode2 == 3) pd2 = diag2[pi];
+        }
         if (staged) {
-            const uint32_t base = ci16 ? blk_base[bi] : kNoBase;  // uniform per block
-            if (base != kNoBase) {
+            const uint32_t blo = ci16 ? blk_base[2 * bi] : kNoBase;  // uniform per block
+            if (blo != kNoBase) {
+                const uint32_t bhi = blk_base[2 * bi + 1];
                 for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
-                    T p = ld_stream<NT>(val + e0 + k) * x[base + (uint32_t)ld_stream<NT>(ci16 + e0 + k)];
+                    const uint32_t o = ld_stream<NT>(ci16 + e0 + k);
+                    T p = ld_stream<NT>(val + e0 + k) * x[((o & kHiBase) ? bhi : blo) + (o & kOffMask)];
                     prod[k] = p;
                 }
             } else {
@@ -438,24 +471,22 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
             }
         }
         __syncthreads();
-        for (uint32_t k = r0 + threadIdx.x; k < r1; k += kBlock) {
-            const uint32_t i = perm ? perm[k] : (uint32_t)((long long)k + off);
-            T s = b[i];
-            if (mode == 3) s = s / diag[i];
-            const uint32_t a0 = rp[k], a1 = rp[k + 1];
+        if (mine) {  // row blocks hold at most kBlock rows: one row per thread
+            T s = pb;
+            if (mode == 3) s = s / pd;
             if (staged) {
-                for (uint32_t e = a0; e < a1; ++e) s = s - prod[e - e0];
+                for (uint32_t e = pa; e < pe; ++e) s = s - prod[e - e0];
             } else {  // a single row longer than the staging buffer
-                for (uint32_t e = a0; e < a1; ++e) {
+                for (uint32_t e = pa; e < pe; ++e) {
                     T p = val[e] * x[ci[e]];
                     s = s - p;
                 }
             }
-            if (mode == 0) s = s / diag[i];
-            else if (mode == 2) s = s * diag[i];
-            if (mode2 == 0 || mode2 == 3) s = s / diag2[i];
-            else if (mode2 == 2) s = s * diag2[i];
-            x[i] = s;
+            if (mode == 0) s = s / pd;
+            else if (mode == 2) s = s * pd;
+            if (mode2 == 0 || mode2 == 3) s = s / pd2;
+            else if (mode2 == 2) s = s * pd2;
+            x[pi] = s;
         }
         __syncthreads();
     }

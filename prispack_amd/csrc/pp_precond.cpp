@@ -117,25 +117,20 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
     if (blk.size() > 1) {
         d.blocks = dalloc<uint32_t>(blk.size());
         PP_HIP(hipMemcpy(d.blocks, blk.data(), blk.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-        // 16-bit offsets where a block's dependency columns span < 2^16
-        std::vector<uint32_t> base(blk.size() - 1, kNoBase);
-        std::vector<uint16_t> off16;
+        // 16-bit two-base offsets where a block's dependency columns fit
+        std::vector<uint32_t> base(2 * (blk.size() - 1), kNoBase);
+        std::vector<uint16_t> off16(ci.size(), 0);
         for (size_t j = 0; j + 1 < blk.size(); ++j) {
             const uint32_t e0 = rp[blk[j]], e1 = rp[blk[j + 1]];
             if (e1 == e0 || e1 - e0 > kSpmvNnzBlock) continue;
-            uint32_t lo = ci[e0], hi = ci[e0];
-            for (uint32_t e = e0; e < e1; ++e) {
-                lo = std::min(lo, ci[e]);
-                hi = std::max(hi, ci[e]);
-            }
-            if (hi - lo > 0xFFFFu) continue;
-            if (off16.empty()) off16.assign(ci.size(), 0);
-            base[j] = lo;
-            for (uint32_t e = e0; e < e1; ++e) off16[e] = (uint16_t)(ci[e] - lo);
+            uint32_t lo = 0, hi = 0;
+            if (!two_base_offsets(ci.data() + e0, e1 - e0, lo, hi, off16.data() + e0)) continue;
+            base[2 * j] = lo;
+            base[2 * j + 1] = hi;
             d.nnz16 += e1 - e0;
         }
         d.nblk = blk.size() - 1;
-        if (!off16.empty()) {
+        if (d.nnz16) {
             d.ci16 = dalloc<uint16_t>(off16.size());
             d.blk_base = dalloc<uint32_t>(base.size());
             PP_HIP(hipMemcpy(d.ci16, off16.data(), off16.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
@@ -592,7 +587,7 @@ static uint64_t sweep_bytes(const DevSweep<T> &d) {
     const uint64_t n16 = (d.variant & kVarIdx16) ? d.nnz16 : 0;
     uint64_t b = esz * d.nnz + 2 * n16 + 4 * (d.nnz - n16) + 4ull * (d.n + 1) + 4 * d.perm_rows;
     if (d.diag) b += esz * d.n;
-    if (d.nblk) b += 4 * (d.nblk + 1) + (n16 ? 4 * d.nblk : 0);
+    if (d.nblk) b += 4 * (d.nblk + 1) + (n16 ? 8 * d.nblk : 0);
     return b + 3 * esz * d.n;
 }
 extern "C" {

@@ -61,8 +61,9 @@ def long_row_matrix():
 
 
 def wide_span_matrix():
-    # a banded matrix (16-bit column offsets fit) with long-range couplings in
-    # some row blocks (their column span exceeds 16 bits: 32-bit indices)
+    # a banded matrix (one 15-bit window per row block) with long-range
+    # couplings: some row blocks need two windows (two-base 16-bit offsets),
+    # some three (32-bit indices)
     rng = np.random.default_rng(9)
     n = 200_000
     rows = []
@@ -70,6 +71,9 @@ def wide_span_matrix():
         cols = {max(0, i - 1), i, min(n - 1, i + 1)}
         if (i // 256) % 5 == 2 and i % 7 == 0:
             cols.add((i + 90_000) % n)
+        if (i // 256) % 5 == 4 and i % 7 == 0:
+            cols.add((i + 60_000) % n)
+            cols.add((i + 130_000) % n)
         rows.append(np.array(sorted(cols), dtype=np.int64))
     rp = np.zeros(n + 1, np.uint32)
     rp[1:] = np.cumsum([len(c) for c in rows])

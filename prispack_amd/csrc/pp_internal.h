@@ -215,6 +215,11 @@ struct pp_precond {
     bool have32 = false;
     int trsv = PP_TRSV_LEVELS;    // resolved SpTRSV schedule
     bool fuse_mid = false;        // last forward level == first backward level (fused launch)
+    // the forward sweep's first level is the contiguous original rows
+    // [lvl0_lo, lvl0_hi) (no dependencies): a caller that produces the
+    // right-hand side can also produce those rows of the result
+    bool lvl0_contiguous = false;
+    uint32_t lvl0_lo = 0, lvl0_hi = 0;
     int sweep_variant = pp::kVarDefault;  // staged-sweep load variant (timed at setup)
     uint32_t fuse_rows = 0;
     std::vector<uint32_t> order;  // factor row k = original row order[k] (empty: natural)
@@ -262,6 +267,9 @@ std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors);
 HostCsr permute_symmetric(const HostCsr &A, const std::vector<uint32_t> &order);
 pp_csr *precond_permuted_matrix(pp_ctx *ctx, pp_precond *M, pp_csr *A);
 void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
+bool precond_level0_fusable(const pp_precond *M);
+void precond_level0(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
+void precond_apply_after_level0(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
 void precond_need_f32(pp_precond *M);
 void csr_need_f32(pp_ctx *ctx, pp_csr *A);
 
@@ -316,7 +324,10 @@ template <class T>
 void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x);
 template <class T>
 void launch_sweep_pair(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup, bool fuse,
-                       const T *x, T *y);
+                       const T *x, T *y, uint32_t first_level = 0);
+// y[i] = x[i] op diag[i] over [lo, hi) (the dependency-free first level)
+template <class T>
+void launch_level0(pp_ctx *ctx, uint32_t lo, uint32_t hi, int mode, const T *diag, const T *x, T *y);
 template <class T>
 void launch_sweep_syncfree(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *y);
 
@@ -330,8 +341,11 @@ template <class T>
 void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v);
 // single-reduction PCG: finish of the three SpMV reductions (+ check), update
 void cgsr_finish(pp_ctx *ctx, unsigned np, double tol, uint32_t maxit, int hist_on, bool publish);
+// lvl0_hi > lvl0_lo: also u[i] = r_new[i] op diag0[i] over [lvl0_lo, lvl0_hi)
+// (the next preconditioner apply's first level)
 template <class T>
-void cgsr_update(pp_ctx *ctx, uint32_t n, const T *u, const T *w, T *p, T *s, T *x, T *r);
+void cgsr_update(pp_ctx *ctx, uint32_t n, T *u, const T *w, T *p, T *s, T *x, T *r, uint32_t lvl0_lo = 0,
+                 uint32_t lvl0_hi = 0, int mode0 = 1, const T *diag0 = nullptr);
 unsigned spmv_grid(const pp_csr *A);
 void set_slot(pp_ctx *ctx, int slot, double v);
 void ctl_init(pp_ctx *ctx, int first_check, int last);

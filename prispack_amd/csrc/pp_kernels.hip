@@ -533,22 +533,45 @@ void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x)
 // fuse: the last level of `lo` holds exactly the rows of the first level of
 // `up` (which have no dependencies), so one kernel finishes the forward
 // sweep and starts the backward one.
+// first_level > 0: the forward sweep's levels below it are already in y.
 template <class T>
 void launch_sweep_pair(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup, bool fuse,
-                       const T *x, T *y) {
-    if (!fuse || lo.nlev == 0 || up.nlev == 0) {
-        launch_sweep(ctx, lo, mlo, x, y);
+                       const T *x, T *y, uint32_t first_level) {
+    if (!fuse || lo.nlev == 0 || up.nlev == 0 || first_level >= lo.nlev) {
+        launch_levels(ctx, lo, mlo, x, y, std::min(first_level, lo.nlev), lo.nlev, -1, (const T *)nullptr);
         launch_sweep(ctx, up, mup, y, y);
         return;
     }
-    launch_levels(ctx, lo, mlo, x, y, 0, lo.nlev - 1, -1, (const T *)nullptr);
+    launch_levels(ctx, lo, mlo, x, y, first_level, lo.nlev - 1, -1, (const T *)nullptr);
     launch_levels(ctx, lo, mlo, x, y, lo.nlev - 1, lo.nlev, mup, up.diag);
     launch_levels(ctx, up, mup, y, y, 1, up.nlev, -1, (const T *)nullptr);
 }
 template void launch_sweep_pair<double>(pp_ctx *, const DevSweep<double> &, int, const DevSweep<double> &, int,
-                                        bool, const double *, double *);
+                                        bool, const double *, double *, uint32_t);
 template void launch_sweep_pair<float>(pp_ctx *, const DevSweep<float> &, int, const DevSweep<float> &, int, bool,
-                                       const float *, float *);
+                                       const float *, float *, uint32_t);
+
+template <class T>
+__global__ void k_level0(uint32_t lo, uint32_t hi, int mode, const T *__restrict__ diag, const T *__restrict__ x,
+                         T *__restrict__ y, const int *stop) {
+#pragma clang fp contract(off)
+    if (stop && *stop) return;
+    for (uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x; i < hi; i += gridDim.x * blockDim.x) {
+        T s = x[i];
+        if (mode == 0 || mode == 3) s = s / diag[i];
+        else if (mode == 2) s = s * diag[i];
+        y[i] = s;
+    }
+}
+
+template <class T>
+void launch_level0(pp_ctx *ctx, uint32_t lo, uint32_t hi, int mode, const T *diag, const T *x, T *y) {
+    if (hi <= lo) return;
+    k_level0<T><<<grid_for(ctx, hi - lo, 4), kBlock, 0, ctx->stream>>>(lo, hi, mode, diag, x, y, ctx->stop_word);
+    PP_HIP(hipGetLastError());
+}
+template void launch_level0<double>(pp_ctx *, uint32_t, uint32_t, int, const double *, const double *, double *);
+template void launch_level0<float>(pp_ctx *, uint32_t, uint32_t, int, const float *, const float *, float *);
 template void launch_sweep<double>(pp_ctx *, const DevSweep<double> &, int, const double *, double *);
 template void launch_sweep<float>(pp_ctx *, const DevSweep<float> &, int, const float *, float *);
 
@@ -838,9 +861,11 @@ __global__ __launch_bounds__(1024) void k_cgsr_finish(const double *__restrict__
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_cgsr_update(uint32_t n, const T *__restrict__ u, const T *__restrict__ w,
+__global__ __launch_bounds__(256) void k_cgsr_update(uint32_t n, T *__restrict__ u, const T *__restrict__ w,
                                                      T *__restrict__ p, T *__restrict__ s, T *__restrict__ x,
-                                                     T *__restrict__ r, const double *scal, const int *ctl) {
+                                                     T *__restrict__ r, const double *scal, const int *ctl,
+                                                     uint32_t l0, uint32_t l1, int mode0,
+                                                     const T *__restrict__ diag0) {
 #pragma clang fp contract(off)
     if (ctl[C_STOP]) return;
     const T alpha = (T)scal[S_CG_ALPHA], beta = (T)scal[S_CG_BETA];
@@ -850,7 +875,14 @@ __global__ __launch_bounds__(256) void k_cgsr_update(uint32_t n, const T *__rest
         p[i] = pn;
         s[i] = sn;
         x[i] = x[i] + alpha * pn;
-        r[i] = r[i] - alpha * sn;
+        const T rn = r[i] - alpha * sn;
+        r[i] = rn;
+        if (i >= l0 && i < l1) {  // first forward level of the next apply (no dependencies)
+            T q = rn;
+            if (mode0 == 0 || mode0 == 3) q = q / diag0[i];
+            else if (mode0 == 2) q = q * diag0[i];
+            u[i] = q;
+        }
     }
 }
 
@@ -865,14 +897,16 @@ void cgsr_finish(pp_ctx *ctx, unsigned np, double tol, uint32_t maxit, int hist_
 }
 
 template <class T>
-void cgsr_update(pp_ctx *ctx, uint32_t n, const T *u, const T *w, T *p, T *s, T *x, T *r) {
-    k_cgsr_update<T><<<grid_for(ctx, n, 4), kBlock, 0, ctx->stream>>>(n, u, w, p, s, x, r, ctx->scal, ctx->ctl);
+void cgsr_update(pp_ctx *ctx, uint32_t n, T *u, const T *w, T *p, T *s, T *x, T *r, uint32_t lvl0_lo,
+                 uint32_t lvl0_hi, int mode0, const T *diag0) {
+    k_cgsr_update<T><<<grid_for(ctx, n, 4), kBlock, 0, ctx->stream>>>(n, u, w, p, s, x, r, ctx->scal, ctx->ctl,
+                                                                     lvl0_lo, lvl0_hi, mode0, diag0);
     PP_HIP(hipGetLastError());
 }
-template void cgsr_update<double>(pp_ctx *, uint32_t, const double *, const double *, double *, double *, double *,
-                                  double *);
-template void cgsr_update<float>(pp_ctx *, uint32_t, const float *, const float *, float *, float *, float *,
-                                 float *);
+template void cgsr_update<double>(pp_ctx *, uint32_t, double *, const double *, double *, double *, double *,
+                                  double *, uint32_t, uint32_t, int, const double *);
+template void cgsr_update<float>(pp_ctx *, uint32_t, float *, const float *, float *, float *, float *, float *,
+                                 uint32_t, uint32_t, int, const float *);
 
 // Sets the control words for a fresh PCG solve (graph-safe: a kernel, no memset).
 __global__ void k_ctl_init(int *ctl, int first_check, int last) {

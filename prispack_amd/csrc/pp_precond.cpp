@@ -289,6 +289,12 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
     }
     upload_sweep(M->hlow, M->low64);
     upload_sweep(M->hup, M->up64);
+    M->lvl0_contiguous = false;
+    if (M->low64.nlev > 0 && M->low64.level_off[0] != DevSweep<double>::kNoOffset) {
+        M->lvl0_contiguous = true;
+        M->lvl0_lo = (uint32_t)((int64_t)M->low64.level_ptr[0] + M->low64.level_off[0]);
+        M->lvl0_hi = (uint32_t)((int64_t)M->low64.level_ptr[1] + M->low64.level_off[0]);
+    }
     if (M->opts.make_f32) {
         upload_sweep(M->hlow, M->low32);
         upload_sweep(M->hup, M->up32);
@@ -326,6 +332,33 @@ pp_csr *precond_permuted_matrix(pp_ctx *ctx, pp_precond *M, pp_csr *A) {
     M->Aperm_src = A;
     M->Aperm_version = A->version;
     return M->Aperm;
+}
+
+// True when a solve may produce the forward sweep's first level itself
+// (launch_level0 on the right-hand side) and call precond_apply_after_level0.
+bool precond_level0_fusable(const pp_precond *M) {
+    return M && M->ready && M->trsv == PP_TRSV_LEVELS && M->lvl0_contiguous && !M->multilevel &&
+           M->opts.kind != PP_PC_NONE && (M->order.empty() || M->io_permuted);
+}
+
+// y[lvl0] = first forward level applied to x[lvl0]
+void precond_level0(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
+    if (dtype == PP_F64)
+        launch_level0<double>(ctx, M->lvl0_lo, M->lvl0_hi, M->hlow.mode, M->low64.diag, (const double *)x,
+                              (double *)y);
+    else
+        launch_level0<float>(ctx, M->lvl0_lo, M->lvl0_hi, M->hlow.mode, M->low32.diag, (const float *)x,
+                             (float *)y);
+}
+
+// y = M^{-1} x with the forward sweep's first level already in y
+void precond_apply_after_level0(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
+    if (dtype == PP_F64)
+        launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid,
+                                  (const double *)x, (double *)y, 1);
+    else
+        launch_sweep_pair<float>(ctx, M->low32, M->hlow.mode, M->up32, M->hup.mode, M->fuse_mid, (const float *)x,
+                                 (float *)y, 1);
 }
 
 // y = M^{-1} x on the context stream (y may alias x).

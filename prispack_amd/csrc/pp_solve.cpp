@@ -194,14 +194,27 @@ PcgResult pcg_sr_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const T *b, T *x,
     set_slot(ctx, S_CG_FIRST, 1.0);
     const unsigned np = spmv_grid(A);
     const uint32_t every = std::max<uint32_t>(1, check_every);
+    // the update kernel also produces the next apply's dependency-free first
+    // forward level (one launch less per iteration)
+    const bool l0 = precond_level0_fusable(M);
+    uint32_t l0_lo = 0, l0_hi = 0;
+    const T *diag0 = nullptr;
+    if (l0) {
+        l0_lo = M->lvl0_lo;
+        l0_hi = M->lvl0_hi;
+        diag0 = sizeof(T) == 8 ? (const T *)M->low64.diag : (const T *)M->low32.diag;
+        precond_level0(ctx, M, DT, r, u);
+    }
     auto body = [&] {
         StopScope scope(ctx);
         for (uint32_t k = 0; k < every; ++k) {
-            if (M) precond_apply(ctx, M, DT, r, u);
+            if (l0) precond_apply_after_level0(ctx, M, DT, r, u);
+            else if (M) precond_apply(ctx, M, DT, r, u);
             else launch_convert(ctx, n, DT, r, DT, u);
             launch_spmv(ctx, A, DT, 1.0, u, 0.0, w, S_CG_DELTA, u, false, r);
             cgsr_finish(ctx, np, tol, maxit, hist_on, k + 1 == every);
-            cgsr_update<T>(ctx, n, u, w, p, s, x, r);
+            if (l0) cgsr_update<T>(ctx, n, u, w, p, s, x, r, l0_lo, l0_hi, M->hlow.mode, diag0);
+            else cgsr_update<T>(ctx, n, u, w, p, s, x, r);
         }
     };
     int ctl[C_COUNT];
@@ -580,7 +593,7 @@ void pp_solve_opts_default(pp_solve_opts *o, int solver) {
     o->maxit = solver == PP_SOLVER_GMRES ? 500 : 1000;
     o->restart = 30;
     o->reference_quirks = 0;
-    o->check_every = 1;
+    o->check_every = 4;
     o->inner_tol = 1e-6;
     o->max_outer = 20;
     o->record_history = 1;

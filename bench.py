@@ -43,7 +43,7 @@ def parse():
                     help="only copy a MIB-sized FP64 buffer 5 times (device-to-device copy kernel) and exit: "
                          "the known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--launch", type=int, default=0, help="PCG body: 0 hipGraph replay, 1 stream launches")
-    ap.add_argument("--check-every", type=int, default=1, help="iterations per host convergence poll")
+    ap.add_argument("--check-every", type=int, default=4, help="iterations per graph body (per host convergence poll)")
     ap.add_argument("--ordering", type=int, default=1,
                     help="IC(0) row ordering: 1 multicolour (2 sweep levels, default), 0 natural (382 levels)")
     return ap.parse_args()

@@ -36,6 +36,28 @@ __device__ inline double block_sum(double v, double *sh) {
     return t;
 }
 
+// XCD-aware work split.  Workgroups are dispatched round-robin over the
+// eight XCDs (workgroup i runs on XCD i % 8), each with its own L2.  Giving
+// XCD q the contiguous range [q nb / 8, (q+1) nb / 8) of row blocks keeps the
+// x entries that neighbouring row blocks share in one L2, instead of every
+// XCD fetching all of x.  The grid is a multiple of eight when this is used.
+constexpr uint32_t kXcds = 8;
+struct XcdRange {
+    uint32_t first, step, end;
+};
+__device__ inline XcdRange xcd_range(uint32_t b0, uint32_t b1) {
+    const uint32_t nb = b1 - b0, G = gridDim.x;
+    if (G % kXcds != 0) return {b0 + blockIdx.x, G, b1};
+    const uint32_t q = blockIdx.x % kXcds, per = G / kXcds;
+    const uint32_t lo = b0 + (uint32_t)((uint64_t)nb * q / kXcds), hi = b0 + (uint32_t)((uint64_t)nb * (q + 1) / kXcds);
+    return {lo + blockIdx.x / kXcds, per, hi};
+}
+static inline unsigned xcd_grid(uint32_t nblocks, unsigned cap) {
+    unsigned g = std::min<unsigned>(nblocks, cap);
+    if (g >= kXcds) g -= g % kXcds;
+    return std::max(1u, g);
+}
+
 // Streamed operand load: non-temporal (NT) or ordinary.
 template <bool NT, class V>
 __device__ inline V ld_stream(const V *p) {
@@ -87,7 +109,8 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ rp,
     // DOT: (dotw, y); with aux also (aux, dotw) and (aux, aux) into the next
     // two partial rows (the three reductions of the single-reduction PCG)
     double dacc = 0.0, dacc2 = 0.0, dacc3 = 0.0;
-    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const XcdRange xr = xcd_range(0, nb);
+    for (uint32_t b = xr.first; b < xr.end; b += xr.step) {
         const uint32_t r0 = rb[b], r1 = rb[b + 1];
         const uint32_t k0 = rp[r0], k1 = rp[r1];
         const uint32_t cnt = k1 - k0;
@@ -250,7 +273,7 @@ static void launch_finish(pp_ctx *ctx, int slot, unsigned np, const int *stop, i
 
 void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x, double beta,
                  void *y, int dot_slot, const void *dot_with, bool, const void *aux) {
-    unsigned grid = std::min<unsigned>(A->nblocks, kMaxPartials);
+    unsigned grid = A->nblocks ? xcd_grid(A->nblocks, kMaxPartials) : 0;
     if (grid == 0) return;
     double *part = dot_slot >= 0 ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
     const int bz = beta == 0.0;
@@ -436,7 +459,8 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
 #pragma clang fp contract(off)
     __shared__ T prod[kSpmvNnzBlock];
     if (stop && *stop) return;
-    for (uint32_t bi = blk0 + blockIdx.x; bi < blk1; bi += gridDim.x) {
+    const XcdRange xr = xcd_range(blk0, blk1);
+    for (uint32_t bi = xr.first; bi < xr.end; bi += xr.step) {
         const uint32_t r0 = blocks[bi], r1 = blocks[bi + 1];
         const uint32_t e0 = rp[r0], cnt = rp[r1] - e0;
         const bool staged = cnt <= kSpmvNnzBlock;
@@ -503,7 +527,7 @@ void launch_levels(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x
         const bool contiguous = l < s.level_off.size() && s.level_off[l] != DevSweep<T>::kNoOffset;
         if (s.blocks && 2 * l + 1 < s.level_blk.size() && s.level_blk[2 * l + 1] > s.level_blk[2 * l]) {
             const uint32_t b0 = s.level_blk[2 * l], b1 = s.level_blk[2 * l + 1];
-            const unsigned g = std::min<unsigned>(b1 - b0, 8192u);
+            const unsigned g = xcd_grid(b1 - b0, 8192u);
             const uint16_t *c16 = (s.variant & kVarIdx16) ? s.ci16 : nullptr;
             if (s.variant & kVarNT)
                 k_sweep_stream<T, true><<<g, kBlock, 0, ctx->stream>>>(
@@ -886,7 +910,7 @@ __global__ __launch_bounds__(256) void k_cgsr_update(uint32_t n, T *__restrict__
     }
 }
 
-unsigned spmv_grid(const pp_csr *A) { return std::min<unsigned>(A->nblocks, kMaxPartials); }
+unsigned spmv_grid(const pp_csr *A) { return A->nblocks ? xcd_grid(A->nblocks, kMaxPartials) : 0; }
 
 void cgsr_finish(pp_ctx *ctx, unsigned np, double tol, uint32_t maxit, int hist_on, bool publish) {
     k_cgsr_finish<<<1, kFinishThreads, 0, ctx->stream>>>(ctx->red.partials + (size_t)S_CG_DELTA * kMaxPartials, np,

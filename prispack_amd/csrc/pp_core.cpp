@@ -2,6 +2,8 @@
 // model-problem generators and seeded inputs of the C ABI.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -114,6 +116,11 @@ void csr_need_f32(pp_ctx *ctx, pp_csr *A) {
 
 static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A);
 
+uint64_t next_uid() {
+    static std::atomic<uint64_t> counter{1};
+    return counter.fetch_add(1);
+}
+
 bool two_base_offsets(const uint32_t *cols, uint32_t cnt, uint32_t &base_lo, uint32_t &base_hi, uint16_t *out) {
     if (cnt == 0) return false;
     std::vector<uint32_t> u(cols, cols + cnt);
@@ -149,6 +156,7 @@ static pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t
     for (uint32_t k = 0; k < nnz; ++k) PP_CHECK(ci[k] < ncol, PP_ERR_INVALID, "column index out of range");
     std::unique_ptr<pp_csr> A(new pp_csr);
     A->ctx = ctx;
+    A->uid = next_uid();
     A->nrow = nrow;
     A->ncol = ncol;
     A->nnz = nnz;

@@ -65,7 +65,8 @@ enum Slot : int {
     S_CG_BETA,
     S_CG_GOLD,
     S_CG_FIRST,
-    S_COUNT = 16
+    S_TOL,       // convergence tolerance of the running PCG (read by its check)
+    S_COUNT = 24
 };
 // Integer control words.
 enum Ctl : int {
@@ -77,6 +78,7 @@ enum Ctl : int {
     C_STATUS,     // PP_OK / PP_ERR_BREAKDOWN / PP_ERR_NOT_FINITE
     C_SPIN,       // a sync-free sweep hit its wait deadline
     C_PUB,        // iteration bodies that have published their control words
+    C_MAXIT,      // iteration cap of the running PCG (read by its check)
     C_COUNT = 16
 };
 
@@ -118,6 +120,9 @@ struct pp_ctx {
     // while set (StopScope), SpMV / dot / GEMV / convert kernels return at
     // once when the word is non-zero, like the sweeps and the PCG kernels
     const int *stop_word = nullptr;
+    // instantiated iteration-body graphs, keyed by everything their kernels
+    // were launched with (object ids, buffers, options); reused across solves
+    std::vector<std::pair<std::vector<uint64_t>, std::unique_ptr<pp::Graph>>> graphs;
 };
 
 namespace pp {
@@ -195,10 +200,12 @@ struct pp_csr {
     std::vector<double> h_v;
     bool f32_dirty = true;
     uint64_t version = 0;            // bumped by every value update
+    uint64_t uid = 0;                // unique per object (never reused)
 };
 
 struct pp_precond {
     pp_ctx *ctx = nullptr;
+    uint64_t uid = 0;             // unique per object (never reused)
     pp_csr *A = nullptr;
     pp_precond_opts opts{};
     int ready = 0;
@@ -243,6 +250,7 @@ struct pp_precond {
 namespace pp {
 
 // ---- core helpers (pp_core.cpp) ----
+uint64_t next_uid();
 void set_device(pp_ctx *ctx);
 template <class T>
 T *dalloc(size_t n);
@@ -335,12 +343,11 @@ void launch_sweep_syncfree(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T 
 template <class T>
 void pcg_d_update(pp_ctx *ctx, uint32_t n, bool first, const T *w, T *d, const T *r);
 template <class T>
-void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, double tol,
-                   uint32_t maxit, int hist_on, bool publish);
+void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, int hist_on, bool publish);
 template <class T>
 void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v);
 // single-reduction PCG: finish of the three SpMV reductions (+ check), update
-void cgsr_finish(pp_ctx *ctx, unsigned np, double tol, uint32_t maxit, int hist_on, bool publish);
+void cgsr_finish(pp_ctx *ctx, unsigned np, int hist_on, bool publish);
 // lvl0_hi > lvl0_lo: also u[i] = r_new[i] op diag0[i] over [lvl0_lo, lvl0_hi)
 // (the next preconditioner apply's first level)
 template <class T>
@@ -348,7 +355,7 @@ void cgsr_update(pp_ctx *ctx, uint32_t n, T *u, const T *w, T *p, T *s, T *x, T 
                  uint32_t lvl0_hi = 0, int mode0 = 1, const T *diag0 = nullptr);
 unsigned spmv_grid(const pp_csr *A);
 void set_slot(pp_ctx *ctx, int slot, double v);
-void ctl_init(pp_ctx *ctx, int first_check, int last);
+void ctl_init(pp_ctx *ctx, int first_check, int last, uint32_t maxit = 0);
 void copy_slot(pp_ctx *ctx, int dst, int src);
 
 // Reads slot s to the host (synchronizes the stream).

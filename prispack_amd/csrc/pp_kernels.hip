@@ -214,8 +214,7 @@ void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &bl
 // result = sum of partials[0..np) in index order (one workgroup); op
 // FIN_PCG_CHECK also runs the PCG convergence test on it.
 enum FinishOp : int { FIN_SUM = 0, FIN_PCG_CHECK = 1 };
-__device__ void pcg_check(double rr, const double *scal, int *ctl, double tol, uint32_t maxit, double *hist,
-                          uint32_t hist_cap);
+__device__ void pcg_check(double rr, const double *scal, int *ctl, double *hist, uint32_t hist_cap);
 // ring != NULL (the last check of an iteration body): afterwards the control
 // words are copied into the next mapped pinned slot for the host (plain
 // vector stores over the fabric), also when the body was skipped.  The
@@ -249,8 +248,8 @@ __device__ inline double sum_partials(const double *__restrict__ partials, unsig
 }
 
 __global__ __launch_bounds__(1024) void k_finish(const double *__restrict__ partials, unsigned np, double *result,
-                                                 const int *stop, int op, const double *scal, int *ctl, double tol,
-                                                 uint32_t maxit, double *hist, uint32_t hist_cap, int *ring) {
+                                                 const int *stop, int op, const double *scal, int *ctl, double *hist,
+                                                 uint32_t hist_cap, int *ring) {
     __shared__ double sh[kFinishThreads / 64];
     if (stop && *stop) {
         if (ring && threadIdx.x == 0) publish_ctl(ctl, ring);
@@ -259,16 +258,16 @@ __global__ __launch_bounds__(1024) void k_finish(const double *__restrict__ part
     double s = block_sum(sum_partials(partials, np), sh);
     if (threadIdx.x == 0) {
         *result = s;
-        if (op == FIN_PCG_CHECK) pcg_check(s, scal, ctl, tol, maxit, hist, hist_cap);
+        if (op == FIN_PCG_CHECK) pcg_check(s, scal, ctl, hist, hist_cap);
         if (ring) publish_ctl(ctl, ring);
     }
 }
 
-static void launch_finish(pp_ctx *ctx, int slot, unsigned np, const int *stop, int op = FIN_SUM, double tol = 0.0,
-                          uint32_t maxit = 0, double *hist = nullptr, uint32_t hist_cap = 0, int *ring = nullptr) {
+static void launch_finish(pp_ctx *ctx, int slot, unsigned np, const int *stop, int op = FIN_SUM,
+                          double *hist = nullptr, uint32_t hist_cap = 0, int *ring = nullptr) {
     k_finish<<<1, kFinishThreads, 0, ctx->stream>>>(ctx->red.partials + (size_t)slot * kMaxPartials, np,
-                                                    ctx->scal + slot, stop, op, ctx->scal, ctx->ctl, tol, maxit, hist,
-                                                    hist_cap, ring);
+                                                    ctx->scal + slot, stop, op, ctx->scal, ctx->ctl, hist, hist_cap,
+                                                    ring);
 }
 
 void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x, double beta,
@@ -739,8 +738,9 @@ __global__ __launch_bounds__(256) void k_pcg_wv(uint32_t n, const T *__restrict_
 
 // Convergence check of CG.cpp:57-63 (run by k_finish on (r, r)): the check
 // index c counts the reference loop index i.
-__device__ void pcg_check(double rr, const double *scal, int *ctl, double tol, uint32_t maxit,
-                                 double *hist, uint32_t hist_cap) {
+__device__ void pcg_check(double rr, const double *scal, int *ctl, double *hist, uint32_t hist_cap) {
+    const double tol = scal[S_TOL];
+    const uint32_t maxit = (uint32_t)ctl[C_MAXIT];
     if (ctl[C_LAST]) {  // the body after the final check has run
         ctl[C_STOP] = 1;
         ctl[C_M] = (int)maxit;
@@ -815,12 +815,11 @@ void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v) {
 }
 
 template <class T>
-void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, double tol, uint32_t maxit,
-                   int hist_on, bool publish) {
+void pcg_xr_update(pp_ctx *ctx, uint32_t n, T *x, T *r, const T *d, const T *v, int hist_on, bool publish) {
     unsigned g = grid_for(ctx, n, 4);
     k_pcg_xr<T><<<g, kBlock, 0, ctx->stream>>>(n, x, r, d, v, ctx->scal, ctx->ctl,
                                                ctx->red.partials + (size_t)S_RR * kMaxPartials);
-    launch_finish(ctx, S_RR, g, ctx->ctl + C_STOP, FIN_PCG_CHECK, tol, maxit, hist_on ? ctx->hist : nullptr,
+    launch_finish(ctx, S_RR, g, ctx->ctl + C_STOP, FIN_PCG_CHECK, hist_on ? ctx->hist : nullptr,
                   hist_on ? ctx->hist_cap : 0, publish ? ctx->ring_dev : nullptr);
     PP_HIP(hipGetLastError());
 }
@@ -829,10 +828,9 @@ template void pcg_d_update<double>(pp_ctx *, uint32_t, bool, const double *, dou
 template void pcg_d_update<float>(pp_ctx *, uint32_t, bool, const float *, float *, const float *);
 template void pcg_wv<double>(pp_ctx *, uint32_t, const double *, const double *);
 template void pcg_wv<float>(pp_ctx *, uint32_t, const float *, const float *);
-template void pcg_xr_update<double>(pp_ctx *, uint32_t, double *, double *, const double *, const double *, double,
-                                    uint32_t, int, bool);
-template void pcg_xr_update<float>(pp_ctx *, uint32_t, float *, float *, const float *, const float *, double,
-                                   uint32_t, int, bool);
+template void pcg_xr_update<double>(pp_ctx *, uint32_t, double *, double *, const double *, const double *, int,
+                                    bool);
+template void pcg_xr_update<float>(pp_ctx *, uint32_t, float *, float *, const float *, const float *, int, bool);
 
 // ---------------------------------------------------------------------------
 // Single-reduction preconditioned CG (Chronopoulos & Gear, J. Comput. Appl.
@@ -846,8 +844,8 @@ template void pcg_xr_update<float>(pp_ctx *, uint32_t, float *, float *, const f
 // The convergence test is on rho = (r_i, r_i) before the update.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_cgsr_finish(const double *__restrict__ partials, unsigned np,
-                                                      double *scal, int *ctl, double tol, uint32_t maxit,
-                                                      double *hist, uint32_t hist_cap, int *ring) {
+                                                      double *scal, int *ctl, double *hist, uint32_t hist_cap,
+                                                      int *ring) {
     __shared__ double sh[kFinishThreads / 64];
     if (ctl[C_STOP]) {
         if (ring && threadIdx.x == 0) publish_ctl(ctl, ring);
@@ -864,7 +862,7 @@ __global__ __launch_bounds__(1024) void k_cgsr_finish(const double *__restrict__
         scal[S_CG_DELTA] = delta;
         scal[S_CG_GAMMA] = gamma;
         scal[S_CG_RHO] = rho;
-        pcg_check(rho, scal, ctl, tol, maxit, hist, hist_cap);
+        pcg_check(rho, scal, ctl, hist, hist_cap);
         if (!ctl[C_STOP]) {
             const bool first = scal[S_CG_FIRST] != 0.0;
             const double beta = first ? 0.0 : gamma / scal[S_CG_GOLD];
@@ -912,10 +910,10 @@ __global__ __launch_bounds__(256) void k_cgsr_update(uint32_t n, T *__restrict__
 
 unsigned spmv_grid(const pp_csr *A) { return A->nblocks ? xcd_grid(A->nblocks, kMaxPartials) : 0; }
 
-void cgsr_finish(pp_ctx *ctx, unsigned np, double tol, uint32_t maxit, int hist_on, bool publish) {
+void cgsr_finish(pp_ctx *ctx, unsigned np, int hist_on, bool publish) {
     k_cgsr_finish<<<1, kFinishThreads, 0, ctx->stream>>>(ctx->red.partials + (size_t)S_CG_DELTA * kMaxPartials, np,
                                                          ctx->scal,
-                                                 ctx->ctl, tol, maxit, hist_on ? ctx->hist : nullptr,
+                                                 ctx->ctl, hist_on ? ctx->hist : nullptr,
                                                  hist_on ? ctx->hist_cap : 0, publish ? ctx->ring_dev : nullptr);
     PP_HIP(hipGetLastError());
 }
@@ -933,18 +931,20 @@ template void cgsr_update<float>(pp_ctx *, uint32_t, float *, const float *, flo
                                  uint32_t, uint32_t, int, const float *);
 
 // Sets the control words for a fresh PCG solve (graph-safe: a kernel, no memset).
-__global__ void k_ctl_init(int *ctl, int first_check, int last) {
+__global__ void k_ctl_init(int *ctl, int first_check, int last, int maxit) {
     if (threadIdx.x < C_COUNT) ctl[threadIdx.x] = 0;
+    __syncthreads();
     if (threadIdx.x == 0) {
         ctl[C_CHECK] = first_check;
         ctl[C_LAST] = last;
+        ctl[C_MAXIT] = maxit;
     }
 }
 
 StopScope::StopScope(pp_ctx *ctx) : c(ctx), prev(ctx->stop_word) { ctx->stop_word = ctx->ctl + C_STOP; }
 
-void ctl_init(pp_ctx *ctx, int first_check, int last) {
-    k_ctl_init<<<1, 64, 0, ctx->stream>>>(ctx->ctl, first_check, last);
+void ctl_init(pp_ctx *ctx, int first_check, int last, uint32_t maxit) {
+    k_ctl_init<<<1, 64, 0, ctx->stream>>>(ctx->ctl, first_check, last, (int)maxit);
     PP_HIP(hipGetLastError());
 }
 

@@ -454,6 +454,7 @@ int pp_precond_create(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *opts, pp_pr
         PP_CHECK(A->nrow == A->ncol, PP_ERR_INVALID, "preconditioner needs a square matrix");
         auto *M = new pp_precond;
         M->ctx = ctx;
+        M->uid = next_uid();
         M->A = A;
         M->opts = *opts;
         M->n = A->nrow;

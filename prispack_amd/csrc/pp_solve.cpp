@@ -44,28 +44,41 @@ struct PcgResult {
 // the host; a body past convergence only runs kernels that return at once.
 // check_first: the control words before the first body decide whether any
 // body runs (they are read synchronously).
+// key: everything the body's kernels are launched with; an instantiated
+// graph with the same key is reused (capture and instantiation cost more
+// than a few iterations).
 template <class Body>
-void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays, int *ctl, bool check_first = true) {
-    Graph G;
+void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays, int *ctl, bool check_first,
+                  const std::vector<uint64_t> &key) {
+    Graph *G = nullptr;
+    std::unique_ptr<Graph> fresh;
     if (use_graph) {
-        PP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        try {
-            body();
-        } catch (...) {
-            hipGraph_t tmp = nullptr;
-            (void)hipStreamEndCapture(ctx->stream, &tmp);
-            if (tmp) (void)hipGraphDestroy(tmp);
-            throw;
+        for (auto &e : ctx->graphs)
+            if (e.first == key) G = e.second.get();
+        if (!G) {
+            fresh.reset(new Graph);
+            PP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                body();
+            } catch (...) {
+                hipGraph_t tmp = nullptr;
+                (void)hipStreamEndCapture(ctx->stream, &tmp);
+                if (tmp) (void)hipGraphDestroy(tmp);
+                throw;
+            }
+            PP_HIP(hipStreamEndCapture(ctx->stream, &fresh->g));
+            PP_HIP(hipGraphInstantiate(&fresh->exec, fresh->g, nullptr, nullptr, 0));
+            G = fresh.get();
+            if (ctx->graphs.size() >= 8) ctx->graphs.erase(ctx->graphs.begin());
+            ctx->graphs.emplace_back(key, std::move(fresh));
         }
-        PP_HIP(hipStreamEndCapture(ctx->stream, &G.g));
-        PP_HIP(hipGraphInstantiate(&G.exec, G.g, nullptr, nullptr, 0));
     }
     constexpr int R = pp_ctx::kRing;
     volatile int *slots = ctx->ring_host;
     for (int i = 0; i < R * C_COUNT; ++i) slots[i] = 0;  // no sequence number of an earlier solve remains
     uint64_t issued = 0, checked = 0;
     auto issue = [&] {
-        if (use_graph) PP_HIP(hipGraphLaunch(G.exec, ctx->stream));
+        if (use_graph) PP_HIP(hipGraphLaunch(G->exec, ctx->stream));
         else body();
         ++issued;
     };
@@ -137,7 +150,8 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
     T *d = (T *)ws_get(ctx, wbase + 2, (size_t)n * sizeof(T));
     T *v = (T *)ws_get(ctx, wbase + 3, (size_t)n * sizeof(T));
     if (hist_on) ensure_hist(ctx, maxit + 2);
-    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0);
+    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0, maxit);
+    set_slot(ctx, S_TOL, tol);
     // r = b - A x
     PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
     launch_spmv(ctx, A, DT, -1.0, x, 1.0, r, -1, nullptr, false);
@@ -147,7 +161,7 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
     else PP_HIP(hipMemcpyAsync(w, r, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
     pcg_d_update<T>(ctx, n, true, w, d, r);
     launch_spmv(ctx, A, DT, 1.0, d, 0.0, v, S_DV, d, false);
-    pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on, false);
+    pcg_xr_update<T>(ctx, n, x, r, d, v, hist_on, false);
 
     int ctl[C_COUNT];
     const uint32_t every = std::max<uint32_t>(1, check_every);
@@ -160,10 +174,15 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
             pcg_d_update<T>(ctx, n, false, w, d, r);
             launch_spmv(ctx, A, DT, 1.0, d, 0.0, v, S_DV, d, false);
             // the body's last check publishes the control words to the host
-            pcg_xr_update<T>(ctx, n, x, r, d, v, tol, maxit, hist_on, k + 1 == every);
+            pcg_xr_update<T>(ctx, n, x, r, d, v, hist_on, k + 1 == every);
         }
     };
-    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl);
+    const std::vector<uint64_t> key = {1,          sizeof(T),  A->uid,     (uint64_t)A->v32,
+                                       M ? M->uid : 0, M ? (uint64_t)M->io_permuted : 0, M ? (uint64_t)M->have32 : 0,
+                                       (uint64_t)flexible, every, (uint64_t)hist_on, (uint64_t)ctx->hist,
+                                       ctx->hist_cap, (uint64_t)r, (uint64_t)w, (uint64_t)d, (uint64_t)v,
+                                       (uint64_t)x, (uint64_t)b};
+    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, true, key);
     return collect_pcg(ctx, ctl, maxit, hist_on);
 }
 
@@ -184,7 +203,8 @@ PcgResult pcg_sr_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const T *b, T *x,
     T *p = (T *)ws_get(ctx, base + 3, (size_t)n * sizeof(T));
     T *s = (T *)ws_get(ctx, base + 4, (size_t)n * sizeof(T));
     if (hist_on) ensure_hist(ctx, maxit + 2);
-    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0);
+    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0, maxit);
+    set_slot(ctx, S_TOL, tol);
     // r = b - A x, p = s = 0, first-iteration flag
     PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
     launch_spmv(ctx, A, DT, -1.0, x, 1.0, r, -1, nullptr, false);
@@ -212,13 +232,18 @@ PcgResult pcg_sr_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const T *b, T *x,
             else if (M) precond_apply(ctx, M, DT, r, u);
             else launch_convert(ctx, n, DT, r, DT, u);
             launch_spmv(ctx, A, DT, 1.0, u, 0.0, w, S_CG_DELTA, u, false, r);
-            cgsr_finish(ctx, np, tol, maxit, hist_on, k + 1 == every);
+            cgsr_finish(ctx, np, hist_on, k + 1 == every);
             if (l0) cgsr_update<T>(ctx, n, u, w, p, s, x, r, l0_lo, l0_hi, M->hlow.mode, diag0);
             else cgsr_update<T>(ctx, n, u, w, p, s, x, r);
         }
     };
     int ctl[C_COUNT];
-    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false);
+    const std::vector<uint64_t> key = {2,          sizeof(T),  A->uid,     (uint64_t)A->v32,
+                                       M ? M->uid : 0, M ? (uint64_t)M->io_permuted : 0, M ? (uint64_t)M->have32 : 0,
+                                       (uint64_t)l0, every, (uint64_t)hist_on, (uint64_t)ctx->hist, ctx->hist_cap,
+                                       (uint64_t)r, (uint64_t)u, (uint64_t)w, (uint64_t)p, (uint64_t)s,
+                                       (uint64_t)x, (uint64_t)b};
+    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false, key);
     return collect_pcg(ctx, ctl, maxit, hist_on);
 }
 

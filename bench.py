@@ -40,13 +40,28 @@ def parse():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--trsv", type=int, default=0, help="SpTRSV schedule: 0 auto, 1 levels, 2 sync-free")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="only copy a MIB-sized FP64 buffer 5 times (device-to-device copy kernel) and exit: "
+                    help="only copy a MIB-sized FP64 buffer 5 times (elementwise copy kernel) and exit: "
                          "the known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only build the system and time the SpMV and preconditioner-apply launches (no solves)")
     ap.add_argument("--launch", type=int, default=0, help="PCG body: 0 hipGraph replay, 1 stream launches")
     ap.add_argument("--check-every", type=int, default=4, help="iterations per graph body (per host convergence poll)")
     ap.add_argument("--ordering", type=int, default=1,
                     help="IC(0) row ordering: 1 multicolour (2 sweep levels, default), 0 natural (382 levels)")
     return ap.parse_args()
+
+
+# HBM bytes per launch of k_spmv<double> read from two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE, KiB per dispatch) over this script, each scaled by
+# what the same counter reads for a device-to-device copy of a known size
+# (--copy-probe).  Keyed by (grid side, SpMV storage variant); the command and
+# the raw per-dispatch figures are in profiles/k_spmv_poisson128_pmc.txt.
+PMC_TRAFFIC = {}
+
+
+def measured_traffic(grid, variant):
+    rec = PMC_TRAFFIC.get((grid, variant))
+    return None if rec is None else rec["bytes"]
 
 
 def log(*a):
@@ -230,7 +245,7 @@ def main():
         src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(5):
-            dst.copy_(src)
+            torch.mul(src, 1.0, out=dst)   # an elementwise kernel: reads nb bytes, writes nb bytes
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": nb, "launches": 5}), flush=True)
         return
@@ -272,35 +287,54 @@ def main():
         dt = time.perf_counter() - t
         return dt, cg.report, x
 
-    for _ in range(args.warmup):
-        solve("fp64")
-        solve("mixed")
-    t64, tmx, rep64, repmx = [], [], None, None
-    torch.cuda.synchronize()
-    for _ in range(args.steps):
-        dt, rep64, x64 = solve("fp64")
-        t64.append(dt)
-    for _ in range(args.steps):
-        dt, repmx, xmx = solve("mixed")
-        tmx.append(dt)
-    # the same single-reduction PCG in FP64: the precision effect alone
-    tsr, repsr = [], None
-    solve("fp64", variant="single_reduction")
-    for _ in range(args.steps):
-        dt, repsr, _ = solve("fp64", variant="single_reduction")
-        tsr.append(dt)
-    torch.cuda.synchronize()
-    err_mixed = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
-    err_true = float(torch.linalg.norm(xmx - d_xt) / torch.linalg.norm(d_xt))
-    log(f"[bench] fp64 {np.median(t64):.4f} s ({rep64.iters} it), mixed {np.median(tmx):.4f} s "
-        f"({repmx.outer_iters} outer / {repmx.inner_iters} inner), fp64 single-reduction "
-        f"{np.median(tsr):.4f} s ({repsr.iters} it)")
+    if not args.kernels_only:
+        for _ in range(args.warmup):
+            solve("fp64")
+            solve("mixed")
+        t64, tmx, rep64, repmx = [], [], None, None
+        torch.cuda.synchronize()
+        for _ in range(args.steps):
+            dt, rep64, x64 = solve("fp64")
+            t64.append(dt)
+        for _ in range(args.steps):
+            dt, repmx, xmx = solve("mixed")
+            tmx.append(dt)
+        # the same single-reduction PCG in FP64: the precision effect alone
+        tsr, repsr = [], None
+        solve("fp64", variant="single_reduction")
+        for _ in range(args.steps):
+            dt, repsr, _ = solve("fp64", variant="single_reduction")
+            tsr.append(dt)
+        torch.cuda.synchronize()
+        err_mixed = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
+        err_true = float(torch.linalg.norm(xmx - d_xt) / torch.linalg.norm(d_xt))
+        log(f"[bench] fp64 {np.median(t64):.4f} s ({rep64.iters} it), mixed {np.median(tmx):.4f} s "
+            f"({repmx.outer_iters} outer / {repmx.inner_iters} inner), fp64 single-reduction "
+            f"{np.median(tsr):.4f} s ({repsr.iters} it)")
 
     # ---- kernel throughput on the library's stream
     stream = L.pp_ctx_stream(ctx.h)
+    spmv_var = C.c_int()
+    ctx.check(L.pp_csr_spmv_variant(M.h, C.byref(spmv_var)))
+    # the FP64 SpMV alternates between two identical copies of the matrix and
+    # of x, y (2 x 188 MB, more than the 256 MB last-level cache): each launch
+    # streams its operands from HBM instead of finding them cached by the
+    # launch before
+    M2 = pp.CSRMatrix(*A, ctx=ctx)
+    ctx.check(L.pp_csr_set_spmv_variant(M2.h, spmv_var.value))
     y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    x2, y2 = d_xt.clone(), torch.zeros_like(y)
     px, py = C.c_void_p(d_xt.data_ptr()), C.c_void_p(y.data_ptr())
-    spmv_ms = kernel_time_ms(torch, stream, lambda: L.pp_spmv(ctx.h, M.h, 0, 1.0, px, 0.0, py), args.kernel_reps)
+    px2, py2 = C.c_void_p(x2.data_ptr()), C.c_void_p(y2.data_ptr())
+    flip = [0]
+    def spmv_alt():
+        flip[0] ^= 1
+        if flip[0]:
+            L.pp_spmv(ctx.h, M.h, 0, 1.0, px, 0.0, py)
+        else:
+            L.pp_spmv(ctx.h, M2.h, 0, 1.0, px2, 0.0, py2)
+    spmv_ms = kernel_time_ms(torch, stream, spmv_alt, args.kernel_reps)
+    del M2, x2, y2
     def spmv_bytes_of(dtype):
         nb = C.c_uint64()
         ctx.check(L.pp_csr_spmv_bytes(M.h, dtype, C.byref(nb)))
@@ -319,8 +353,6 @@ def main():
         nb = C.c_uint64()
         ctx.check(L.pp_precond_apply_bytes(P.h, dtype, in_solve, C.byref(nb)))
         return int(nb.value)
-    spmv_var = C.c_int()
-    ctx.check(L.pp_csr_spmv_variant(M.h, C.byref(spmv_var)))
     trsv_bytes = apply_bytes_of(0)
     trsv32_bytes = apply_bytes_of(1)
     tv = torch.tensor(xt, device="cuda")
@@ -332,9 +364,40 @@ def main():
     trsv32_ms = kernel_time_ms(torch, stream, lambda: L.pp_precond_apply(ctx.h, P.h, 1, ptv32, ptv32),
                                max(3, args.kernel_reps // 10))
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
-    log(f"[bench] SpMV fp64 {spmv_ms * 1e3:.1f} us {gbs(spmv_bytes, spmv_ms):.0f} GB/s; fp32 {spmv32_ms * 1e3:.1f} us; "
-        f"M^-1 apply fp64 {trsv_ms * 1e3:.1f} us {gbs(trsv_bytes, trsv_ms):.1f} GB/s; fp32 {trsv32_ms * 1e3:.1f} us "
-        f"{gbs(trsv32_bytes, trsv32_ms):.1f} GB/s")
+    log(f"[bench] SpMV fp64 {spmv_ms * 1e3:.1f} us ({gbs(12 * nnz + 20 * n + 4, spmv_ms):.0f} GB/s algorithmic, "
+        f"{gbs(spmv_bytes, spmv_ms):.0f} GB/s streamed); fp32 {spmv32_ms * 1e3:.1f} us; "
+        f"M^-1 apply fp64 {trsv_ms * 1e3:.1f} us ({gbs(trsv_bytes, trsv_ms):.1f} GB/s streamed); "
+        f"fp32 {trsv32_ms * 1e3:.1f} us ({gbs(trsv32_bytes, trsv32_ms):.1f} GB/s streamed)")
+
+    sched = {0: "levels", 1: "levels", 2: "syncfree"}[args.trsv]
+    # algorithmic bytes (32-bit indices, x read once, y written once):
+    # SpMV 12 nnz + 20 n + 4 (FP64), 8 nnz + 12 n + 4 (FP32); one sweep over
+    # the IC(0) factor 12 nnz_L + 4 (n + 1) + 16 n (FP64), 8 nnz_L + 4 (n + 1)
+    # + 8 n (FP32), nnz_L = (nnz + n) / 2, and an apply is two sweeps
+    alg_spmv = 12 * nnz + 20 * n + 4
+    alg_spmv32 = 8 * nnz + 12 * n + 4
+    nnz_l = (nnz + n) // 2
+    alg_apply = 2 * (12 * nnz_l + 4 * (n + 1) + 16 * n)
+    alg_apply32 = 2 * (8 * nnz_l + 4 * (n + 1) + 8 * n)
+    traffic = measured_traffic(g, spmv_var.value)
+    dominant = {"kernel": "k_spmv<double> (y = A x, FP64 CSR, launches alternating between two copies of A, x, y "
+                          "on the library stream)",
+                "bound": "hbm", "achieved": round(gbs(alg_spmv, spmv_ms), 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(gbs(alg_spmv, spmv_ms) / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "algorithmic_bytes": alg_spmv, "avg_us": round(spmv_ms * 1e3, 2),
+                "launches_timed": args.kernel_reps, "bytes_streamed": spmv_bytes,
+                "streamed_GBs": round(gbs(spmv_bytes, spmv_ms), 1), "variant": spmv_var.value}
+    apply_rec = {"kernel": f"IC(0) apply = forward + backward sweep ({sched} schedule, {info.levels_lower} levels)",
+                 "bound": "hbm", "achieved": round(gbs(alg_apply, trsv_ms), 1), "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "frac": round(gbs(alg_apply, trsv_ms) / HBM_PEAK_GBS, 4),
+                 "algorithmic_bytes": alg_apply, "avg_us": round(trsv_ms * 1e3, 2),
+                 "bytes_streamed": trsv_bytes, "sweep_variant": info.sweep_variant,
+                 "fused_levels": info.fused_levels,
+                 "fp32": {"achieved": round(gbs(alg_apply32, trsv32_ms), 1), "algorithmic_bytes": alg_apply32,
+                          "avg_us": round(trsv32_ms * 1e3, 2), "bytes_streamed": trsv32_bytes}}
+    if args.kernels_only:
+        print(json.dumps({"roofline": dominant, "roofline_apply": apply_rec}), flush=True)
+        return
 
     # the other ordering, one solve of each precision, for the record
     other = None
@@ -360,14 +423,6 @@ def main():
             cpu = {"value": None, "unit": "s", "cores": 0, "kind": "reference", "sample": f"unavailable: {e}"}
 
     med64, medmx = float(np.median(t64)), float(np.median(tmx))
-    sched = {0: "levels", 1: "levels", 2: "syncfree"}[args.trsv]
-    dominant = {"kernel": f"IC(0) apply = forward + backward sweep ({sched} schedule, {info.levels_lower} levels)",
-                "bound": "hbm", "achieved": round(gbs(trsv_bytes, trsv_ms), 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(gbs(trsv_bytes, trsv_ms) / HBM_PEAK_GBS, 5),
-                "traffic": None, "bytes_per_launch": trsv_bytes, "avg_ms": round(trsv_ms, 4),
-                "fp32_avg_ms": round(trsv32_ms, 4), "fp32_bytes_per_launch": trsv32_bytes,
-                "sweep_variant": info.sweep_variant, "fused_levels": info.fused_levels,
-                "fp32_achieved": round(gbs(trsv32_bytes, trsv32_ms), 2)}
     out = {
         "metric": "time-to-rtol 1e-8 (s) mixed-prec vs FP64 on GPU; SpMV/SpTRSV HBM GB/s",
         "value": round(medmx, 5), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -385,13 +440,11 @@ def main():
         "err_mixed_vs_fp64": err_mixed, "err_vs_xtrue": err_true, "setup_s": round(setup_s, 3),
         "levels": [info.levels_lower, info.levels_upper],
         "roofline": dominant,
-        "roofline_spmv": {"kernel": "k_spmv<double>", "bound": "hbm", "achieved": round(gbs(spmv_bytes, spmv_ms), 1),
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs(spmv_bytes, spmv_ms) / HBM_PEAK_GBS, 4),
-                          "traffic": None, "bytes_per_launch": spmv_bytes, "avg_us": round(spmv_ms * 1e3, 2),
-                          "variant": spmv_var.value},
-        "roofline_spmv_f32": {"achieved": round(gbs(spmv32_bytes, spmv32_ms), 1), "unit": "GB/s",
-                              "frac": round(gbs(spmv32_bytes, spmv32_ms) / HBM_PEAK_GBS, 4),
-                              "bytes_per_launch": spmv32_bytes, "avg_us": round(spmv32_ms * 1e3, 2)},
+        "roofline_apply": apply_rec,
+        "roofline_spmv_f32": {"kernel": "k_spmv<float>", "achieved": round(gbs(alg_spmv32, spmv32_ms), 1),
+                              "unit": "GB/s", "frac": round(gbs(alg_spmv32, spmv32_ms) / HBM_PEAK_GBS, 4),
+                              "algorithmic_bytes": alg_spmv32, "avg_us": round(spmv32_ms * 1e3, 2),
+                              "bytes_streamed": spmv32_bytes},
         "cpu_baseline": cpu,
         "secondary": extra,
     }

@@ -85,6 +85,10 @@ int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes);
 /* SpMV load variant chosen by timing when the matrix was created (large
  * matrices): bit 0 non-temporal operand loads, bit 1 16-bit column offsets. */
 int pp_csr_spmv_variant(const pp_csr *A, int *variant);
+/* Pins the SpMV load variant instead of the timed choice (reproducible
+ * profiling, identical copies of one matrix).  PP_ERR_INVALID for a value
+ * outside 0..3 or for 16-bit offsets on a matrix that has none. */
+int pp_csr_set_spmv_variant(pp_csr *A, int variant);
 /* Device pointers of the stored arrays (values64 FP64; values32 NULL until an
  * FP32 copy exists). */
 int pp_csr_device_ptrs(const pp_csr *A, const uint32_t **rowptr, const uint32_t **colidx,

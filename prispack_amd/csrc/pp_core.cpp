@@ -62,6 +62,8 @@ template float *dalloc<float>(size_t);
 template uint32_t *dalloc<uint32_t>(size_t);
 template int *dalloc<int>(size_t);
 template unsigned char *dalloc<unsigned char>(size_t);
+template BlockDesc *dalloc<BlockDesc>(size_t);
+template uint4 *dalloc<uint4>(size_t);
 
 void dfree(void *p) {
     if (p) (void)hipFree(p);
@@ -119,6 +121,16 @@ static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A);
 uint64_t next_uid() {
     static std::atomic<uint64_t> counter{1};
     return counter.fetch_add(1);
+}
+
+std::vector<BlockDesc> block_descs(const std::vector<uint32_t> &blk, const std::vector<uint32_t> &rp,
+                                   const std::vector<uint32_t> &base) {
+    std::vector<BlockDesc> d(blk.size());
+    for (size_t j = 0; j < blk.size(); ++j) {
+        const bool has = j + 1 < blk.size() && 2 * j + 1 < base.size();
+        d[j] = {blk[j], rp[blk[j]], has ? base[2 * j] : kNoBase, has ? base[2 * j + 1] : kNoBase};
+    }
+    return d;
 }
 
 bool two_base_offsets(const uint32_t *cols, uint32_t cnt, uint32_t &base_lo, uint32_t &base_hi, uint16_t *out) {
@@ -198,7 +210,13 @@ static pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t
         PP_HIP(hipMemcpyAsync(A->blk_base, base.data(), base.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                               ctx->stream));
     }
-    PP_HIP(hipStreamSynchronize(ctx->stream));
+    {
+        const std::vector<BlockDesc> ds = block_descs(blocks, A->h_rp, base);
+        A->blk_desc = dalloc<BlockDesc>(ds.size());
+        PP_HIP(hipMemcpyAsync(A->blk_desc, ds.data(), ds.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
+                              ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));  // ds leaves scope
+    }
     csr_tune_spmv(ctx, A.get());
     return A.release();
 }
@@ -213,16 +231,10 @@ static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A) {
         launch_fill_zero(ctx, x, (size_t)A->ncol * sizeof(double));
         float best = 1e30f;
         int best_var = A->spmv_variant;
-        for (int var = 0; var < 4; ++var) {
+        for (int var = 0; var < kVarCount; ++var) {
             if ((var & kVarIdx16) && !A->ci16) continue;
             A->spmv_variant = var;
-            launch_spmv(ctx, A, PP_F64, 1.0, x, 0.0, y, -1, nullptr, false);  // warm-up
-            PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-            for (int rep = 0; rep < 5; ++rep) launch_spmv(ctx, A, PP_F64, 1.0, x, 0.0, y, -1, nullptr, false);
-            PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-            PP_HIP(hipEventSynchronize(ctx->ev1));
-            float ms = 0;
-            PP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            const float ms = time_cold_ms(ctx, [&] { launch_spmv(ctx, A, PP_F64, 1.0, x, 0.0, y, -1, nullptr, false); }, 3);
             if (ms < best) {
                 best = ms;
                 best_var = var;
@@ -252,6 +264,7 @@ void csr_free(pp_csr *A) {
     dfree(A->row_blocks);
     dfree(A->ci16);
     dfree(A->blk_base);
+    dfree(A->blk_desc);
     delete A;
 }
 
@@ -424,6 +437,13 @@ int pp_csr_set_values(pp_ctx *ctx, pp_csr *A, const double *values, int on_devic
 int pp_csr_spmv_variant(const pp_csr *A, int *variant) {
     if (!A || !variant) return PP_ERR_INVALID;
     *variant = A->spmv_variant;
+    return PP_OK;
+}
+
+int pp_csr_set_spmv_variant(pp_csr *A, int variant) {
+    if (!A || variant < 0 || variant >= kVarCount) return PP_ERR_INVALID;
+    if ((variant & kVarIdx16) && !A->ci16) return PP_ERR_INVALID;
+    A->spmv_variant = variant;
     return PP_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/prispack_amd.h"
@@ -41,9 +42,17 @@ constexpr uint32_t kNoBase = 0xFFFFFFFFu;   // row block without 16-bit column o
 constexpr uint16_t kHiBase = 0x8000u, kOffMask = 0x7FFFu;
 // Encodes cols[0..cnt) as two-base offsets; false if they do not fit.
 bool two_base_offsets(const uint32_t *cols, uint32_t cnt, uint32_t &base_lo, uint32_t &base_hi, uint16_t *out);
-// streaming-kernel variants (bit set): non-temporal operand loads, 16-bit offsets
-constexpr int kVarNT = 1, kVarIdx16 = 2;
-constexpr int kVarDefault = kVarIdx16;
+// streaming-kernel variants (bit set): non-temporal operand loads, 16-bit
+// offsets, batched loads (every load of a thread issued before the first use)
+constexpr int kVarNT = 1, kVarIdx16 = 2, kVarBatch = 4;
+constexpr int kVarCount = 8;
+constexpr int kVarDefault = kVarIdx16 | kVarBatch;
+// One row block of a staged kernel: first row (position) r0, first entry k0,
+// the two 16-bit offset bases (kNoBase: the block reads 32-bit columns).  An
+// array of nblocks + 1 descriptors; the next descriptor's r0 / k0 end a block.
+struct alignas(16) BlockDesc {
+    uint32_t r0, k0, blo, bhi;
+};
 constexpr uint32_t kSpmvRowsBlock = 256; // rows per SpMV workgroup (one per thread)
 constexpr int kMaxPartials = 4096;       // reduction partials per slot
 
@@ -156,6 +165,7 @@ struct DevSweep {
     // blocks[j]..blocks[j+1]); level l owns blocks [level_blk[2l],
     // level_blk[2l+1]) (host), an empty range for a narrow level
     uint32_t *blocks = nullptr;
+    BlockDesc *desc = nullptr;  // [nblk + 1] descriptors of those blocks
     std::vector<uint32_t> level_blk;
     // 16-bit dependency offsets of the row blocks (as for the SpMV): block
     // j's dependencies are blk_base[j] + ci16[e] unless blk_base[j] == kNoBase
@@ -187,6 +197,7 @@ struct pp_csr {
     float *v32 = nullptr;
     uint32_t *row_blocks = nullptr;  // adaptive SpMV row blocks
     uint32_t nblocks = 0;
+    pp::BlockDesc *blk_desc = nullptr;  // [nblocks + 1] descriptors of the row blocks
     // 16-bit column indices relative to a per-block base: block b's columns
     // are blk_base[b] + ci16[k]; blk_base[b] == kNoBase marks a block whose
     // column span does not fit 16 bits (it reads ci).  NULL when no block fits.
@@ -320,6 +331,14 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
                  double beta, void *y, int dot_slot, const void *dot_with, bool dot_first,
                  const void *aux = nullptr);
 void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes);
+// Average duration (ms) of fn()'s launches on ctx->stream with the last-level
+// cache cleared before each one (a read of `flush_bytes` of other memory), so
+// that operands are streamed from HBM as they are inside a solve.
+float time_cold_ms(pp_ctx *ctx, const std::function<void()> &fn, int reps);
+// Host-built descriptors of row blocks blk[j]..blk[j+1] (positions into rp);
+// base[2j], base[2j+1] the 16-bit offset bases or kNoBase.
+std::vector<BlockDesc> block_descs(const std::vector<uint32_t> &blk, const std::vector<uint32_t> &rp,
+                                   const std::vector<uint32_t> &base);
 void launch_convert(pp_ctx *ctx, uint32_t n, int sdt, const void *s, int ddt, void *d);
 void launch_axpby(pp_ctx *ctx, int dtype, uint32_t n, double a, const void *x, double b,
                   const void *y, void *z);

@@ -27,9 +27,11 @@ void DevSweep<T>::free_all() {
     dfree(blocks);
     dfree(ci16);
     dfree(blk_base);
+    dfree(desc);
     blocks = nullptr;
     ci16 = nullptr;
     blk_base = nullptr;
+    desc = nullptr;
     level_blk.clear();
     perm = rp = ci = nullptr;
     val = diag = nullptr;
@@ -130,6 +132,9 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
             d.nnz16 += e1 - e0;
         }
         d.nblk = blk.size() - 1;
+        const std::vector<BlockDesc> ds = block_descs(blk, rp, base);
+        d.desc = dalloc<BlockDesc>(ds.size());
+        PP_HIP(hipMemcpy(d.desc, ds.data(), ds.size() * sizeof(BlockDesc), hipMemcpyHostToDevice));
         if (d.nnz16) {
             d.ci16 = dalloc<uint16_t>(off16.size());
             d.blk_base = dalloc<uint32_t>(base.size());
@@ -476,17 +481,12 @@ static void tune_sweeps(pp_ctx *ctx, pp_precond *M) {
         float best = 1e30f;
         int best_var = M->sweep_variant;
         const bool have16 = M->low64.ci16 || M->up64.ci16;
-        for (int var = 0; var < 4; ++var) {
+        for (int var = 0; var < kVarCount; ++var) {
             if ((var & kVarIdx16) && !have16) continue;
             M->low64.variant = M->up64.variant = var;
-            launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, y, y);
-            PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-            for (int rep = 0; rep < 3; ++rep)
+            const float ms = time_cold_ms(ctx, [&] {
                 launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, y, y);
-            PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-            PP_HIP(hipEventSynchronize(ctx->ev1));
-            float ms = 0;
-            PP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            }, 3);
             if (ms < best) {
                 best = ms;
                 best_var = var;

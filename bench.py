@@ -299,18 +299,28 @@ def main():
         for _ in range(args.steps):
             dt, repmx, xmx = solve("mixed")
             tmx.append(dt)
-        # the same single-reduction PCG in FP64: the precision effect alone
-        tsr, repsr = [], None
-        solve("fp64", variant="single_reduction")
-        for _ in range(args.steps):
-            dt, repsr, _ = solve("fp64", variant="single_reduction")
-            tsr.append(dt)
+        # the inner iteration of the mixed solve run in FP64 (the precision
+        # effect alone), and the FP64 single-reduction PCG and mixed solve
+        # with that inner iteration, for the record
+        variant_names = {1: "reference", 2: "single_reduction", 3: "reduced"}
+        inner_variant = variant_names.get(repmx.cg_variant, "reference")
+        same = {}
+        for name, prec, var in (("fp64_same", "fp64", inner_variant), ("fp64_sr", "fp64", "single_reduction"),
+                                ("mixed_sr", "mixed", "single_reduction")):
+            solve(prec, variant=var)
+            ts, rp = [], None
+            for _ in range(args.steps):
+                dt, rp, _ = solve(prec, variant=var)
+                ts.append(dt)
+            same[name] = (float(np.median(ts)), rp)
+        tsr, repsr = [same["fp64_sr"][0]], same["fp64_sr"][1]
         torch.cuda.synchronize()
         err_mixed = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
         err_true = float(torch.linalg.norm(xmx - d_xt) / torch.linalg.norm(d_xt))
         log(f"[bench] fp64 {np.median(t64):.4f} s ({rep64.iters} it), mixed {np.median(tmx):.4f} s "
-            f"({repmx.outer_iters} outer / {repmx.inner_iters} inner), fp64 single-reduction "
-            f"{np.median(tsr):.4f} s ({repsr.iters} it)")
+            f"({repmx.outer_iters} outer / {repmx.inner_iters} inner, {inner_variant}); same iteration in fp64 "
+            f"{same['fp64_same'][0]:.4f} s ({same['fp64_same'][1].iters} it); fp64 single-reduction "
+            f"{np.median(tsr):.4f} s ({repsr.iters} it); mixed with single-reduction inner {same['mixed_sr'][0]:.4f} s")
 
     # ---- kernel throughput on the library's stream
     stream = L.pp_ctx_stream(ctx.h)
@@ -433,8 +443,12 @@ def main():
                    "ordering": "multicolour" if args.ordering == 1 else "natural"},
         "other_ordering": other,
         "fp64_s": round(med64, 5), "speedup_mixed_vs_fp64": round(med64 / medmx, 3),
+        "mixed_inner_iteration": inner_variant,
+        "fp64_same_algorithm_s": round(same["fp64_same"][0], 5), "iters_fp64_same_algorithm": same["fp64_same"][1].iters,
+        "speedup_mixed_vs_fp64_same_algorithm": round(same["fp64_same"][0] / medmx, 3),
         "fp64_single_reduction_s": round(float(np.median(tsr)), 5), "iters_fp64_single_reduction": repsr.iters,
-        "speedup_mixed_vs_fp64_same_algorithm": round(float(np.median(tsr)) / medmx, 3),
+        "mixed_single_reduction_inner_s": round(same["mixed_sr"][0], 5),
+        "iters_mixed_single_reduction_inner": same["mixed_sr"][1].inner_iters,
         "iters_fp64": rep64.iters, "iters_mixed_inner": repmx.inner_iters, "outer_mixed": repmx.outer_iters,
         "relres_fp64": rep64.true_relres, "relres_mixed": repmx.true_relres,
         "err_mixed_vs_fp64": err_mixed, "err_vs_xtrue": err_true, "setup_s": round(setup_s, 3),

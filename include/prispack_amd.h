@@ -231,12 +231,21 @@ int pp_gemm_tall_f64(pp_ctx *ctx, uint32_t n, uint32_t k, uint32_t r, const doub
 enum { PP_SOLVER_CG = 0, PP_SOLVER_GMRES = 1 };
 enum { PP_FP64 = 0, PP_MIXED = 1 };
 enum { PP_LAUNCH_GRAPH = 0, PP_LAUNCH_STREAM = 1 };
-/* CG iteration: AUTO = the reference's FlexibleCG / NormalCG for FP64 solves
- * and the single-reduction PCG for the FP32 inner solves of PP_MIXED;
+/* CG iteration: AUTO = the reference's FlexibleCG / NormalCG for FP64 solves;
+ * for the FP32 inner solves of PP_MIXED the reduced CG when it applies, else
+ * the single-reduction PCG;
  * REFERENCE = FlexibleCG / NormalCG everywhere (CG.cpp:33-83);
  * SINGLE_REDUCTION = Chronopoulos-Gear PCG everywhere (one SpMV carrying the
- * three inner products, one fused update; no reference counterpart). */
-enum { PP_CG_AUTO = 0, PP_CG_REFERENCE = 1, PP_CG_SINGLE_REDUCTION = 2 };
+ * three inner products, one fused update; no reference counterpart);
+ * REDUCED = IC(0) of a two-colour ordering (PP_ORDER_MULTICOLOR on a matrix
+ * whose colour blocks are diagonal, e.g. 5- and 7-point stencils) applied in
+ * Schur-complement form: the split-preconditioned operator L^-1 A L^-T is
+ * the identity on the first colour and the scaled Schur complement on the
+ * second (Eisenstat 1981), so CG runs on the second colour's unknowns with
+ * no triangular sweep -- the same Krylov iteration as IC(0)-PCG (an FP64
+ * solve then confirms the true residual in an outer step).  No reference
+ * counterpart; PP_ERR_UNSUPPORTED when the structure is absent. */
+enum { PP_CG_AUTO = 0, PP_CG_REFERENCE = 1, PP_CG_SINGLE_REDUCTION = 2, PP_CG_REDUCED = 3 };
 
 typedef struct {
     int solver;          /* PP_SOLVER_* */
@@ -268,7 +277,10 @@ typedef struct {
     double solve_ms;        /* stream time, setup excluded */
     int32_t status;         /* PP_OK or the failure */
     int32_t nhist;
-    uint32_t reserved[8];
+    int32_t cg_variant;     /* CG: the iteration that ran (PP_CG_REFERENCE,
+                               _SINGLE_REDUCTION or _REDUCED; for PP_MIXED the
+                               inner solves'); GMRES: PP_CG_AUTO */
+    uint32_t reserved[7];
 } pp_solve_report;
 
 void pp_solve_opts_default(pp_solve_opts *o, int solver);

@@ -415,10 +415,14 @@ class _Solver:
         self.report = _capi.SolveReport()
 
     def setCGVariant(self, variant):
-        """"auto" (reference FlexibleCG in FP64, single-reduction PCG for the
-        FP32 inner solves of mixed precision), "reference" or "single_reduction"."""
+        """"auto" (reference FlexibleCG in FP64; for the FP32 inner solves of
+        mixed precision the reduced CG where it applies, else the
+        single-reduction PCG), "reference", "single_reduction" or "reduced"
+        (IC(0) of a two-colour ordering applied in Schur-complement form: CG
+        on the second colour's unknowns, no triangular sweeps)."""
         self.opts.cg_variant = {"auto": _capi.PP_CG_AUTO, "reference": _capi.PP_CG_REFERENCE,
-                                "single_reduction": _capi.PP_CG_SINGLE_REDUCTION}[variant]
+                                "single_reduction": _capi.PP_CG_SINGLE_REDUCTION,
+                                "reduced": _capi.PP_CG_REDUCED}[variant]
         return self
 
     def solve(self, A, b, x):

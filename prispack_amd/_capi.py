@@ -19,7 +19,7 @@ PP_TRSV_AUTO, PP_TRSV_LEVELS, PP_TRSV_SYNCFREE, PP_TRSV_BLOCK = range(4)
 PP_SOLVER_CG, PP_SOLVER_GMRES = 0, 1
 PP_FP64, PP_MIXED = 0, 1
 PP_LAUNCH_GRAPH, PP_LAUNCH_STREAM = 0, 1
-PP_CG_AUTO, PP_CG_REFERENCE, PP_CG_SINGLE_REDUCTION = 0, 1, 2
+PP_CG_AUTO, PP_CG_REFERENCE, PP_CG_SINGLE_REDUCTION, PP_CG_REDUCED = 0, 1, 2, 3
 PP_ORDER_NATURAL, PP_ORDER_MULTICOLOR = 0, 1
 
 STATUS_NAMES = {0: "PP_OK", 1: "PP_ERR_INVALID", 2: "PP_ERR_HIP", 3: "PP_ERR_NOT_SPD", 4: "PP_ERR_ZERO_PIVOT",
@@ -58,7 +58,7 @@ class SolveReport(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("final_error", C.c_double),
                 ("true_relres", C.c_double), ("outer_iters", C.c_int32), ("inner_iters", C.c_int32),
                 ("solve_ms", C.c_double), ("status", C.c_int32), ("nhist", C.c_int32),
-                ("reserved", C.c_uint32 * 8)]
+                ("cg_variant", C.c_int32), ("reserved", C.c_uint32 * 7)]
 
 
 vp = C.c_void_p

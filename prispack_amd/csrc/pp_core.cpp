@@ -77,7 +77,7 @@ void ensure_hist(pp_ctx *ctx, uint32_t cap) {
 }
 
 void *ws_get(pp_ctx *ctx, int idx, size_t bytes) {
-    PP_CHECK(idx >= 0 && idx < 32, PP_ERR_INVALID, "workspace index");
+    PP_CHECK(idx >= 0 && idx < 48, PP_ERR_INVALID, "workspace index");
     if (ctx->ws_bytes[idx] < bytes) {
         PP_HIP(hipStreamSynchronize(ctx->stream));
         dfree(ctx->ws[idx]);
@@ -158,7 +158,7 @@ bool two_base_offsets(const uint32_t *cols, uint32_t cnt, uint32_t &base_lo, uin
     return true;
 }
 
-static pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci,
+pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci,
                        const double *v) {
     PP_CHECK(rp != nullptr, PP_ERR_INVALID, "rowptr is NULL");
     PP_CHECK(rp[0] == 0, PP_ERR_INVALID, "rowptr[0] must be 0");
@@ -362,7 +362,7 @@ void pp_ctx_destroy(pp_ctx *ctx) {
     dfree(ctx->scal);
     dfree(ctx->ctl);
     dfree(ctx->hist);
-    for (int i = 0; i < 32; ++i) dfree(ctx->ws[i]);
+    for (int i = 0; i < 48; ++i) dfree(ctx->ws[i]);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->pinned_ctl) (void)hipHostFree(ctx->pinned_ctl);
     if (ctx->ring_host) (void)hipHostFree(ctx->ring_host);

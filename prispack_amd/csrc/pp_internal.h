@@ -123,8 +123,8 @@ struct pp_ctx {
     std::vector<double> last_hist;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     static constexpr int kRing = 4;  // iteration bodies kept in flight (ring_host holds kRing slots)
-    void *ws[32] = {};             // cached solver workspaces
-    size_t ws_bytes[32] = {};
+    void *ws[48] = {};             // cached solver workspaces
+    size_t ws_bytes[48] = {};
     long long spin_deadline = 0;   // wall_clock64 ticks a sync-free wait may last
     // while set (StopScope), SpMV / dot / GEMV / convert kernels return at
     // once when the word is non-zero, like the sweeps and the PCG kernels
@@ -214,6 +214,45 @@ struct pp_csr {
     uint64_t uid = 0;                // unique per object (never reused)
 };
 
+namespace pp {
+// IC(0) of a two-colour matrix in Schur-complement form.  When the rows
+// split into colours [0, n0) and [n0, n) whose diagonal blocks are diagonal
+// (a 5- or 7-point stencil in red-black order), the IC(0) factor is
+// L = [D0^1/2, 0; A10 D0^-1/2, Dt^1/2] with Dt = diag(A11 - A10 D0^-1 A01),
+// and the split-preconditioned operator L^-1 A L^-T is block-diagonal:
+// the identity on the first colour and Sb = Dt^-1/2 (A11 - A10 D0^-1 A01)
+// Dt^-1/2 on the second (Eisenstat, SIAM J. Sci. Stat. Comput. 2 (1981) 1-4:
+// the preconditioner fused with the operator).  CG on Sb therefore runs the
+// same Krylov iteration as IC(0)-PCG on A, on half the unknowns and without
+// any triangular sweep: one operator application is y0 = B v (first-colour
+// rows) and w = c * v - Bt y0 (second-colour rows), two SpMVs over the
+// off-diagonal entries of A.  All four operators are stored as CSR over the
+// full column range [0, n) so that they read one vector z = [y0 or r0; v]:
+//   KA  rows [0, n0):  B_ij  = a_ij / sqrt(a_ii Dt_j)          (j >= n0)
+//   KB  rows [n0, n):  -a_ij / sqrt(a_jj Dt_i) (j < n0), c_i = a_ii / Dt_i at j = i
+//   KF  rows [n0, n):  -a_ij / (a_jj sqrt(Dt_i)) (j < n0), 1 / sqrt(Dt_i) at j = i
+//                      (f = Dt^-1/2 (r1 - A10 D0^-1 r0), the reduced right-hand side)
+//   KE  rows [0, n0):  1 / a_ii at j = i, -a_ij / a_ii (j >= n0)
+//                      (e0 = D0^-1 (r0 - A01 e1), the eliminated unknowns)
+// and s1 = Dt^-1/2 maps the reduced solution back: e1 = s1 * x.
+struct ReducedForm {
+    bool ok = false;  // false: the matrix / factor does not have this structure
+    uint32_t n0 = 0, n1 = 0;
+    pp_csr *KA = nullptr, *KB = nullptr, *KF = nullptr, *KE = nullptr;
+    double *s1_64 = nullptr;  // Dt^-1/2
+    float *s1_32 = nullptr;
+    double *dt_64 = nullptr;  // Dt: the true residual of the second colour is Dt^1/2 times the reduced one
+    float *dt_32 = nullptr;
+    const pp_csr *src = nullptr;  // built from this matrix at src_version
+    uint64_t src_version = 0;
+};
+// The reduced form of M for the colour-ordered matrix As (built once and
+// cached in M), or nullptr when it does not apply.
+ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As);
+void reduced_need_f32(pp_ctx *ctx, ReducedForm *R);
+void reduced_free(pp_precond *M);
+}  // namespace pp
+
 struct pp_precond {
     pp_ctx *ctx = nullptr;
     uint64_t uid = 0;             // unique per object (never reused)
@@ -256,6 +295,7 @@ struct pp_precond {
     const pp_csr *Aperm_src = nullptr;
     uint64_t Aperm_version = 0;
     bool io_permuted = false;
+    pp::ReducedForm *reduced = nullptr;  // two-colour Schur-complement form (IC(0) only)
 };
 
 namespace pp {
@@ -326,10 +366,11 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d);
 // ---- kernels (pp_kernels.hip) ----
 // y = alpha A x + beta y; dot_slot >= 0: also (dot_with, y) into that slot;
 // aux != NULL (with dot_slot): also (aux, dot_with) and (aux, aux) into the
-// next two slots, left as partials for the caller's finish kernel
+// next two slots, left as partials for the caller's finish kernel; wgt !=
+// NULL weights the last: sum wgt_i aux_i^2
 void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x,
                  double beta, void *y, int dot_slot, const void *dot_with, bool dot_first,
-                 const void *aux = nullptr);
+                 const void *aux = nullptr, const void *wgt = nullptr);
 void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes);
 // Average duration (ms) of fn()'s launches on ctx->stream with the last-level
 // cache cleared before each one (a read of `flush_bytes` of other memory), so
@@ -367,6 +408,10 @@ template <class T>
 void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v);
 // single-reduction PCG: finish of the three SpMV reductions (+ check), update
 void cgsr_finish(pp_ctx *ctx, unsigned np, int hist_on, bool publish);
+template <class T>
+void red_update(pp_ctx *ctx, uint32_t n, const T *w, T *p, T *s, T *x, T *r);
+template <class T>
+void red_post(pp_ctx *ctx, uint32_t n0, uint32_t n1, const T *x, const T *s1, const T *rin, T *z, T *e);
 // lvl0_hi > lvl0_lo: also u[i] = r_new[i] op diag0[i] over [lvl0_lo, lvl0_hi)
 // (the next preconditioner apply's first level)
 template <class T>

@@ -161,12 +161,13 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
                                               const T *__restrict__ val, const T *__restrict__ x,
                                               T *__restrict__ y, uint32_t nb, T alpha, T beta, int beta_zero,
                                               const T *__restrict__ dotw, const T *__restrict__ aux,
-                                              double *partials, const int *stop) {
+                                              const T *__restrict__ wgt, double *partials, const int *stop) {
     __shared__ T prod[kSpmvNnzBlock];
     __shared__ double sh[4];
     if (stop && *stop) return;
     // DOT: (dotw, y); with aux also (aux, dotw) and (aux, aux) into the next
-    // two partial rows (the three reductions of the single-reduction PCG)
+    // two partial rows (the three reductions of the single-reduction PCG);
+    // wgt weights the last one: sum wgt_i aux_i^2
     double dacc = 0.0, dacc2 = 0.0, dacc3 = 0.0;
     const XcdRange xr = xcd_range(0, nb);
     for (uint32_t b = xr.first; b < xr.end; b += xr.step) {
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
             const bool mine = threadIdx.x < r1 - r0;
             const uint32_t row = r0 + threadIdx.x;
             uint32_t pa = 0, pe = 0;
-            T yold = 0, dw = 0, ax = 0;
+            T yold = 0, dw = 0, ax = 0, wg = 1;
             if (mine) {
                 pa = rp[row];
                 pe = rp[row + 1];
@@ -188,6 +189,7 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
                 if (DOT) {
                     dw = dotw[row];
                     if (aux) ax = aux[row];
+                    if (wgt) wg = wgt[row];
                 }
             }
             // products a_k * x[col_k] into LDS (uniform choice of offsets)
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
                     if (aux) {
                         const double rr = (double)ax;
                         dacc2 += rr * u;
-                        dacc3 += rr * rr;
+                        dacc3 += rr * rr * (double)wg;
                     }
                 }
             }
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
                     if (aux) {
                         const double rr = (double)aux[r0];
                         dacc2 += rr * u;
-                        dacc3 += rr * rr;
+                        dacc3 += rr * rr * (wgt ? (double)wgt[r0] : 1.0);
                     }
                 }
             }
@@ -319,7 +321,7 @@ static void launch_finish(pp_ctx *ctx, int slot, unsigned np, const int *stop, i
 }
 
 void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x, double beta,
-                 void *y, int dot_slot, const void *dot_with, bool, const void *aux) {
+                 void *y, int dot_slot, const void *dot_with, bool, const void *aux, const void *wgt) {
     unsigned grid = A->nblocks ? xcd_grid(A->nblocks, kMaxPartials) : 0;
     if (grid == 0) return;
     double *part = dot_slot >= 0 ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
@@ -330,7 +332,8 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
 #define PP_SPMV(T, DOTV, NTV, BV, VALS, DW, PART)                                                                 \
     k_spmv<T, DOTV, NTV, BV><<<grid, kBlock, 0, ctx->stream>>>(A->blk_desc, A->rp, A->ci, c16, VALS, (const T *)x, \
                                                               (T *)y, A->nblocks, (T)alpha, (T)beta, bz, DW,     \
-                                                              (const T *)aux, PART, ctx->stop_word)
+                                                              (const T *)aux, (const T *)wgt, PART,          \
+                                                              ctx->stop_word)
 #define PP_SPMV_VAR(T, DOTV, VALS, DW, PART)                                           \
     switch (var & (kVarNT | kVarBatch)) {                                               \
         case 0: PP_SPMV(T, DOTV, false, false, VALS, DW, PART); break;                  \
@@ -1002,6 +1005,62 @@ template void cgsr_update<double>(pp_ctx *, uint32_t, double *, const double *, 
                                   double *, uint32_t, uint32_t, int, const double *);
 template void cgsr_update<float>(pp_ctx *, uint32_t, float *, const float *, float *, float *, float *, float *,
                                  uint32_t, uint32_t, int, const float *);
+
+// Update of the single-reduction CG on the reduced (Schur-complement) system,
+// where the preconditioned residual is the residual itself:
+// p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s.
+template <class T>
+__global__ __launch_bounds__(256) void k_red_update(uint32_t n, const T *__restrict__ w, T *__restrict__ p,
+                                                    T *__restrict__ s, T *__restrict__ x, T *__restrict__ r,
+                                                    const double *scal, const int *ctl) {
+#pragma clang fp contract(off)
+    if (ctl[C_STOP]) return;
+    const T alpha = (T)scal[S_CG_ALPHA], beta = (T)scal[S_CG_BETA];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const T ri = r[i];
+        const T pn = ri + beta * p[i];
+        const T sn = w[i] + beta * s[i];
+        p[i] = pn;
+        s[i] = sn;
+        x[i] = x[i] + alpha * pn;
+        r[i] = ri - alpha * sn;
+    }
+}
+
+template <class T>
+void red_update(pp_ctx *ctx, uint32_t n, const T *w, T *p, T *s, T *x, T *r) {
+    k_red_update<T><<<grid_for(ctx, n, 4), kBlock, 0, ctx->stream>>>(n, w, p, s, x, r, ctx->scal, ctx->ctl);
+    PP_HIP(hipGetLastError());
+}
+template void red_update<double>(pp_ctx *, uint32_t, const double *, double *, double *, double *, double *);
+template void red_update<float>(pp_ctx *, uint32_t, const float *, float *, float *, float *, float *);
+
+// Back-substitution staging of the reduced solve: e1 = s1 * x into both
+// z[n0..) and e[n0..); z[0..n0) = rin[0..n0) (the first colour's residual,
+// which the elimination step D0^{-1} (r0 - A01 e1) reads next).
+template <class T>
+__global__ void k_red_post(uint32_t n0, uint32_t n1, const T *__restrict__ x, const T *__restrict__ s1,
+                           const T *__restrict__ rin, T *__restrict__ z, T *__restrict__ e) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n0 + n1; i += gridDim.x * blockDim.x) {
+        if (i < n0) {
+            z[i] = rin[i];
+        } else {
+            const T v = x[i - n0] * s1[i - n0];
+            z[i] = v;
+            e[i] = v;
+        }
+    }
+}
+
+template <class T>
+void red_post(pp_ctx *ctx, uint32_t n0, uint32_t n1, const T *x, const T *s1, const T *rin, T *z, T *e) {
+    k_red_post<T><<<grid_for(ctx, (uint64_t)n0 + n1, 4), kBlock, 0, ctx->stream>>>(n0, n1, x, s1, rin, z, e);
+    PP_HIP(hipGetLastError());
+}
+template void red_post<double>(pp_ctx *, uint32_t, uint32_t, const double *, const double *, const double *,
+                               double *, double *);
+template void red_post<float>(pp_ctx *, uint32_t, uint32_t, const float *, const float *, const float *, float *,
+                              float *);
 
 // Sets the control words for a fresh PCG solve (graph-safe: a kernel, no memset).
 __global__ void k_ctl_init(int *ctl, int first_check, int last, int maxit) {

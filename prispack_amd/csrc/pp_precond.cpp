@@ -692,6 +692,7 @@ void pp_precond_destroy(pp_precond *M) {
     dfree(M->tmp64);
     dfree(M->tmp32);
     csr_free(M->Aperm);
+    pp::reduced_free(M);
     M->low64.free_all();
     M->up64.free_all();
     M->low32.free_all();

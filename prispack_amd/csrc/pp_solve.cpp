@@ -19,7 +19,9 @@ namespace {
 
 // Workspace indices in the context cache.
 enum { W_R = 0, W_W, W_D, W_V, W_R32, W_W32, W_D32, W_V32, W_X32, W_B32, W_E32, W_VB, W_ZB, W_ZB32, W_T, W_T32,
-       W_PB, W_PX, W_SR64, W_SR32 = W_SR64 + 5 };  // W_SR*: five vectors each (up to slot 27)
+       W_PB, W_PX, W_SR64, W_SR32 = W_SR64 + 5,  // W_SR*: five vectors each
+       W_RD64 = W_SR32 + 5, W_RD32 = W_RD64 + 5,   // W_RD*: five vectors each (reduced CG)
+       W_RIN64 = W_RD32 + 5, W_E64 };              // up to slot 39
 
 template <class T>
 constexpr int dt() {
@@ -247,6 +249,70 @@ PcgResult pcg_sr_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const T *b, T *x,
     return collect_pcg(ctx, ctl, maxit, hist_on);
 }
 
+// CG on the reduced system of a two-colour IC(0) (ReducedForm): solves
+// A e = rin approximately, to the relative reduction tol of the reduced
+// residual.  Per iteration y0 = KA z, w = KB z carrying the three inner
+// products of the single-reduction recurrence (the preconditioned residual
+// is the residual: the IC(0) preconditioner is folded into the operator), a
+// one-workgroup finish and one vector update, all on the second colour's n1
+// unknowns.  Before: the reduced right-hand side r = KF rin; after:
+// e = [KE [rin0; e1]; e1], e1 = s1 * x.  Vectors of the second colour live
+// in z[n0..n), where KA reads them.
+template <class T>
+PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, double tol, uint32_t maxit,
+                             uint32_t check_every, bool use_graph) {
+    const uint32_t n0 = R->n0, n1 = R->n1, n = n0 + n1;
+    const int DT = dt<T>();
+    const int base = sizeof(T) == 8 ? W_RD64 : W_RD32;
+    T *z = (T *)ws_get(ctx, base + 0, (size_t)n * sizeof(T));
+    T *w = (T *)ws_get(ctx, base + 1, (size_t)n1 * sizeof(T));
+    T *p = (T *)ws_get(ctx, base + 2, (size_t)n1 * sizeof(T));
+    T *s = (T *)ws_get(ctx, base + 3, (size_t)n1 * sizeof(T));
+    T *x = (T *)ws_get(ctx, base + 4, (size_t)n1 * sizeof(T));
+    T *r = z + n0;
+    const T *s1 = sizeof(T) == 8 ? (const T *)R->s1_64 : (const T *)R->s1_32;
+    const T *wt = sizeof(T) == 8 ? (const T *)R->dt_64 : (const T *)R->dt_32;
+    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0, maxit);
+    set_slot(ctx, S_TOL, tol);
+    launch_spmv(ctx, R->KF, DT, 1.0, rin, 0.0, r, -1, nullptr, false);
+    // the test is on the inner system's own residual: ||rin - A e|| is
+    // ||Dt^1/2 r|| (the first colour's part vanishes), relative to ||rin||
+    launch_dot(ctx, DT, n, rin, rin, S_BB);
+    launch_fill_zero(ctx, p, (size_t)n1 * sizeof(T));
+    launch_fill_zero(ctx, s, (size_t)n1 * sizeof(T));
+    launch_fill_zero(ctx, x, (size_t)n1 * sizeof(T));
+    set_slot(ctx, S_CG_FIRST, 1.0);
+    const unsigned np = spmv_grid(R->KB);
+    const uint32_t every = std::max<uint32_t>(1, check_every);
+    auto body = [&] {
+        StopScope scope(ctx);
+        for (uint32_t k = 0; k < every; ++k) {
+            launch_spmv(ctx, R->KA, DT, 1.0, z, 0.0, z, -1, nullptr, false);      // z[0, n0) = B r
+            launch_spmv(ctx, R->KB, DT, 1.0, z, 0.0, w, S_CG_DELTA, r, false, r, wt);  // w = Sb r, (r,w), (r,r)
+            cgsr_finish(ctx, np, false, k + 1 == every);
+            red_update<T>(ctx, n1, w, p, s, x, r);
+        }
+    };
+    int ctl[C_COUNT];
+    const std::vector<uint64_t> key = {3,
+                                       sizeof(T),
+                                       R->KA->uid,
+                                       R->KB->uid,
+                                       (uint64_t)(sizeof(T) == 8 ? (const void *)R->KA->v64 : (const void *)R->KA->v32),
+                                       (uint64_t)(sizeof(T) == 8 ? (const void *)R->KB->v64 : (const void *)R->KB->v32),
+                                       every,
+                                       (uint64_t)z,
+                                       (uint64_t)w,
+                                       (uint64_t)p,
+                                       (uint64_t)s,
+                                       (uint64_t)x};
+    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false, key);
+    PcgResult res = collect_pcg(ctx, ctl, maxit, false);
+    red_post<T>(ctx, n0, n1, x, s1, rin, z, e);
+    launch_spmv(ctx, R->KE, DT, 1.0, z, 0.0, e, -1, nullptr, false);
+    return res;
+}
+
 double true_relres(pp_ctx *ctx, pp_csr *A, const double *b, const double *x) {
     const uint32_t n = A->nrow;
     double *t = (double *)ws_get(ctx, W_T, (size_t)n * sizeof(double));
@@ -262,21 +328,36 @@ double true_relres(pp_ctx *ctx, pp_csr *A, const double *b, const double *x) {
 // Mixed precision: FP64 residual r = b - A x and update x += ||r|| e on the
 // device, inner FP32 FlexibleCG on A e = r / ||r|| (FP32 factor, SpMV, sweeps
 // and accumulation) to a relative reduction max(inner_tol, 0.5 tol ||b||/||r||).
-PcgResult pcg_mixed(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts &o, const double *b, double *x,
-                    int32_t &outer, int32_t &inner_total, double &relres) {
+// Iterative refinement: FP64 residual r = b - A x and update x += ||r|| e,
+// with e an inner solve of A e = r / ||r|| in precision Tin -- FP32 for the
+// mixed-precision solve (tolerance max(inner_tol, what the outer tolerance
+// still needs)), FP64 for the reduced CG run as a plain FP64 solve (the
+// outer loop then only confirms the true residual).  Inner solver: the
+// reduced two-colour CG when R is given, else the single-reduction PCG or
+// (PP_CG_REFERENCE) FlexibleCG.
+template <class Tin>
+PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, const pp_solve_opts &o, const double *b,
+                     double *x, int32_t &outer, int32_t &inner_total, double &relres) {
+    constexpr bool f32 = sizeof(Tin) == 4;
     const uint32_t n = A->nrow;
-    csr_need_f32(ctx, A);
-    if (M) precond_need_f32(M);
+    if (f32) {
+        if (R) reduced_need_f32(ctx, R);
+        else {
+            csr_need_f32(ctx, A);
+            if (M) precond_need_f32(M);
+        }
+    }
     double *r = (double *)ws_get(ctx, W_T, (size_t)n * sizeof(double));
-    float *r32 = (float *)ws_get(ctx, W_B32, (size_t)n * sizeof(float));
-    float *e32 = (float *)ws_get(ctx, W_E32, (size_t)n * sizeof(float));
+    Tin *rin = (Tin *)ws_get(ctx, f32 ? W_B32 : W_RIN64, (size_t)n * sizeof(Tin));
+    Tin *e = (Tin *)ws_get(ctx, f32 ? W_E32 : W_E64, (size_t)n * sizeof(Tin));
     PcgResult res;
     launch_dot(ctx, PP_F64, n, b, b, S_TMP3);
     const double bnorm = std::sqrt(read_slot(ctx, S_TMP3));
     outer = 0;
     inner_total = 0;
-    const double inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-6;
+    const double inner_tol = f32 ? (o.inner_tol > 0 ? o.inner_tol : 1e-6) : 0.0;
     const uint32_t max_outer = o.max_outer ? o.max_outer : 20;
+    const bool graph = o.launch != PP_LAUNCH_STREAM;
     relres = 1.0;
     for (;;) {
         PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
@@ -293,22 +374,27 @@ PcgResult pcg_mixed(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts &
             break;
         }
         if ((uint32_t)outer >= max_outer || (uint32_t)inner_total >= o.maxit) break;
-        launch_scale_convert(ctx, n, r, 1.0 / rnorm, r32);
-        launch_fill_zero(ctx, e32, (size_t)n * sizeof(float));
-        const double want = std::max(inner_tol, 0.5 * o.tol * bnorm / rnorm);
+        if (f32) launch_scale_convert(ctx, n, r, 1.0 / rnorm, (float *)rin);
+        else launch_axpby(ctx, PP_F64, n, 1.0 / rnorm, r, 0.0, r, rin);
+        launch_fill_zero(ctx, e, (size_t)n * sizeof(Tin));
+        // FP32: a margin for the drift between the recurrence and the true
+        // residual; FP64 inner solves stop where a plain FP64 solve would
+        const double want = f32 ? std::max(inner_tol, 0.5 * o.tol * bnorm / rnorm) : o.tol * bnorm / rnorm;
         const uint32_t inner_max = std::max<uint32_t>(2, o.maxit - inner_total);
-        const bool graph = o.launch != PP_LAUNCH_STREAM;
-        PcgResult in = o.cg_variant == PP_CG_REFERENCE
-                           ? pcg_device<float>(ctx, A, M, o.flexible != 0, r32, e32, want, inner_max, o.check_every,
-                                               false, W_R32, graph)
-                           : pcg_sr_device<float>(ctx, A, M, r32, e32, want, inner_max, o.check_every, false, graph);
+        PcgResult in;
+        if (R) in = pcg_reduced_device<Tin>(ctx, R, rin, e, want, inner_max, o.check_every, graph);
+        else if (o.cg_variant == PP_CG_REFERENCE)
+            in = pcg_device<Tin>(ctx, A, M, o.flexible != 0, rin, e, want, inner_max, o.check_every, false,
+                                 f32 ? W_R32 : W_R, graph);
+        else in = pcg_sr_device<Tin>(ctx, A, M, rin, e, want, inner_max, o.check_every, false, graph);
         inner_total += in.iters;
         ++outer;
         if (in.status != PP_OK && in.status != PP_ERR_NOT_FINITE) {
             res.status = in.status;
             break;
         }
-        launch_axpy_f32_to_f64(ctx, n, rnorm, e32, x);
+        if (f32) launch_axpy_f32_to_f64(ctx, n, rnorm, (const float *)e, x);
+        else launch_axpby(ctx, PP_F64, n, rnorm, e, 1.0, x, x);
     }
     res.iters = inner_total;
     res.final_error = relres;
@@ -571,10 +657,24 @@ static int solve_impl(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts
         PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         PcgResult res;
         if (o.solver == PP_SOLVER_CG) {
+            ReducedForm *red = nullptr;
+            if (permuted && (o.cg_variant == PP_CG_AUTO || o.cg_variant == PP_CG_REDUCED) &&
+                (o.precision == PP_MIXED || o.cg_variant == PP_CG_REDUCED))
+                red = reduced_form(ctx, M, As);
+            PP_CHECK(o.cg_variant != PP_CG_REDUCED || red, PP_ERR_UNSUPPORTED,
+                     "reduced CG needs IC(0) in a two-colour (multicolour) ordering of a matrix whose colour "
+                     "blocks are diagonal");
             if (o.precision == PP_MIXED) {
                 double rel;
-                res = pcg_mixed(ctx, As, M, o, bs, xs, R.outer_iters, R.inner_iters, rel);
+                res = pcg_refine<float>(ctx, As, M, red, o, bs, xs, R.outer_iters, R.inner_iters, rel);
+                R.cg_variant = red ? PP_CG_REDUCED
+                                   : (o.cg_variant == PP_CG_REFERENCE ? PP_CG_REFERENCE : PP_CG_SINGLE_REDUCTION);
+            } else if (red) {
+                double rel;
+                res = pcg_refine<double>(ctx, As, M, red, o, bs, xs, R.outer_iters, R.inner_iters, rel);
+                R.cg_variant = PP_CG_REDUCED;
             } else {
+                R.cg_variant = o.cg_variant == PP_CG_SINGLE_REDUCTION ? PP_CG_SINGLE_REDUCTION : PP_CG_REFERENCE;
                 if (o.cg_variant == PP_CG_SINGLE_REDUCTION)
                     res = pcg_sr_device<double>(ctx, As, M, bs, xs, o.tol, o.maxit, o.check_every ? o.check_every : 1,
                                                 o.record_history != 0, o.launch != PP_LAUNCH_STREAM);

@@ -1,0 +1,140 @@
+"""Reduced (Schur-complement) form of two-colour IC(0)-PCG on the GPU.
+
+For a matrix whose rows split into two colours with diagonal colour blocks,
+L^-1 A L^-T is the identity on the first colour and the scaled Schur
+complement on the second, so CG on the second colour runs the same Krylov
+iteration as IC(0)-PCG.  Checked against the oracle's FlexibleCG + IC(0) on
+the colour-permuted matrix (the same ordering the library builds), to the
+solver bar: iterations within +-2, true relative residual <= tol,
+||x - x_ref|| / ||x_ref|| <= 1e-6; and mixed precision (FP32 reduced inner
+solves, FP64 refinement) to the FP64 tolerance.
+"""
+import numpy as np
+import pytest
+
+from mc_order import greedy_order, permute
+from oracle import ppo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def pp(torch):
+    import prispack_amd
+    return prispack_amd
+
+
+def solve(pp, torch, M, P, b, precision, variant, tol=1e-8, check_every=4):
+    n = len(b)
+    cg = pp.CG(n, P, pp.FlexibleCG, tol=tol, maxIter=3000, precision=precision).setCGVariant(variant)
+    cg.opts.check_every = check_every
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cg.solve(M, torch.tensor(b, device="cuda"), x)
+    return cg, x.cpu().numpy()
+
+
+@pytest.mark.parametrize("shape", [(20, 19, 18), (64, 63, 1), (33, 34, 35)])
+def test_reduced_fp64_matches_ic0_pcg(pp, torch, shape):
+    A = ppo.laplacian(*shape)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 11))
+    print(f"[reduced] fp64 shape {shape}", flush=True)
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M, ordering=1)
+    cg, x = solve(pp, torch, M, P, b, "fp64", "reduced")
+    assert cg.report.cg_variant == 3  # PP_CG_REDUCED ran
+    assert cg.getConvergence() and cg.report.true_relres < 1e-8
+    order, ncol = greedy_order(A)
+    assert ncol == 2
+    Ap = permute(A, order)
+    o = ppo.cg(Ap, b[order], ppo.Prec(Ap, "ic0"), tol=1e-8, maxit=3000)
+    xo = np.empty(n)
+    xo[order] = o["x"]
+    print(f"[reduced] iters {cg.getNumIter()} (outer {cg.report.outer_iters}), oracle IC(0)-PCG {o['iters']}",
+          flush=True)
+    assert abs(cg.getNumIter() - o["iters"]) <= 2
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+@pytest.mark.parametrize("check_every", [1, 4])
+def test_reduced_mixed(pp, torch, check_every):
+    A = ppo.laplacian(40, 40, 40)
+    n = len(A[0]) - 1
+    xt = ppo.normal(n, 0)
+    b = ppo.spmv(A, xt)
+    print(f"[reduced] mixed 40^3 check_every {check_every}", flush=True)
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M, ordering=1, make_f32=True)
+    cg, x = solve(pp, torch, M, P, b, "mixed", "auto", check_every=check_every)
+    assert cg.report.cg_variant == 3  # auto picks the reduced inner solve here
+    assert cg.getConvergence() and cg.report.true_relres < 1e-8
+    ref, xr = solve(pp, torch, M, P, b, "fp64", "reference")
+    print(f"[reduced] mixed inner {cg.report.inner_iters} / outer {cg.report.outer_iters}, "
+          f"fp64 FlexibleCG {ref.getNumIter()}", flush=True)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-6
+    assert np.linalg.norm(x - xt) / np.linalg.norm(xt) <= 1e-5
+    # the single-reduction inner solve is still available and agrees
+    sr, xs = solve(pp, torch, M, P, b, "mixed", "single_reduction", check_every=check_every)
+    assert sr.report.cg_variant == 2
+    assert sr.getConvergence() and np.linalg.norm(xs - xr) / np.linalg.norm(xr) <= 1e-6
+
+
+def test_reduced_repeated_and_graph_reuse(pp, torch):
+    # the reduced form and its iteration graphs are built once and reused;
+    # a second right-hand side gives its own solution
+    A = ppo.laplacian(24, 24, 24)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M, ordering=1, make_f32=True)
+    for seed in (1, 2, 3):
+        xt = ppo.normal(n, seed)
+        b = ppo.spmv(A, xt)
+        for prec in ("mixed", "fp64"):
+            cg, x = solve(pp, torch, M, P, b, prec, "reduced")
+            assert cg.getConvergence() and cg.report.true_relres < 1e-8, (seed, prec)
+            assert np.linalg.norm(x - xt) / np.linalg.norm(xt) <= 1e-5, (seed, prec)
+
+
+def test_reduced_unsupported_structures(pp, torch):
+    # natural ordering (no two-colour split) and a factor that is not IC(0)
+    A = ppo.laplacian(12, 11, 10)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 3))
+    M = pp.CSRMatrix(*A)
+    for P in (pp.IC0(M, ordering=0), pp.IncompleteCholesky(M, 1e-3)):
+        with pytest.raises(pp.PrisPackError) as e:
+            solve(pp, torch, M, P, b, "fp64", "reduced")
+        assert "PP_ERR_UNSUPPORTED" in str(e.value)
+        # auto falls back to the single-reduction inner solve
+        cg, _ = solve(pp, torch, M, P, b, "mixed", "auto")
+        assert cg.report.cg_variant == 2 and cg.getConvergence()
+    # a 27-point-like coupling inside a colour: three colours, no reduced form
+    A9 = ppo.laplacian(16, 16, 1)
+    rp, ci, v = A9
+    rows = [list(zip(ci[rp[i]:rp[i + 1]], v[rp[i]:rp[i + 1]])) for i in range(len(rp) - 1)]
+    side = 16
+    for i in range(side * side):
+        x, y = i % side, i // side
+        for dx, dy in ((1, 1), (-1, -1), (1, -1), (-1, 1)):
+            if 0 <= x + dx < side and 0 <= y + dy < side:
+                rows[i].append((np.uint32((y + dy) * side + x + dx), -0.25))
+    for i in range(len(rows)):
+        rows[i].sort()
+        rows[i] = [(c, (w + 1.0 if c == i else w)) for c, w in rows[i]]
+    rp2 = np.zeros(len(rows) + 1, np.uint32)
+    rp2[1:] = np.cumsum([len(r) for r in rows])
+    ci2 = np.array([c for r in rows for c, _ in r], np.uint32)
+    v2 = np.array([w for r in rows for _, w in r])
+    M9 = pp.CSRMatrix(rp2, ci2, v2)
+    P9 = pp.IC0(M9, ordering=1)
+    b9 = ppo.spmv((rp2, ci2, v2), ppo.normal(len(rows), 4))
+    with pytest.raises(pp.PrisPackError):
+        solve(pp, torch, M9, P9, b9, "fp64", "reduced")

@@ -271,7 +271,8 @@ typedef struct {
     int32_t iters;          /* getNumIter(), reference counting convention */
     int32_t converged;      /* getConvergence() */
     double final_error;     /* getError(): the recursive residual estimate */
-    double true_relres;     /* ||b - A x|| / ||b|| recomputed in FP64 at exit */
+    double true_relres;     /* ||b - A x|| / ||b|| recomputed in FP64 at exit (a
+                               refinement solve's last FP64 residual check) */
     int32_t outer_iters;    /* mixed: refinement steps taken */
     int32_t inner_iters;    /* mixed: total inner iterations */
     double solve_ms;        /* stream time, setup excluded */

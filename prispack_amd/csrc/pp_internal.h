@@ -122,7 +122,11 @@ struct pp_ctx {
     int *ring_dev = nullptr;      // the same memory as the device sees it
     std::vector<double> last_hist;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    static constexpr int kRing = 4;  // iteration bodies kept in flight (ring_host holds kRing slots)
+    static constexpr int kRing = 4;      // publication slots of ring_host
+    // iteration bodies kept in flight: one running and one queued hides the
+    // host's poll-and-launch latency; every further one only adds bodies
+    // that run (as early-exit kernels) after convergence
+    static constexpr int kInFlight = 2;
     void *ws[48] = {};             // cached solver workspaces
     size_t ws_bytes[48] = {};
     long long spin_deadline = 0;   // wall_clock64 ticks a sync-free wait may last
@@ -419,6 +423,7 @@ void cgsr_update(pp_ctx *ctx, uint32_t n, T *u, const T *w, T *p, T *s, T *x, T 
                  uint32_t lvl0_hi = 0, int mode0 = 1, const T *diag0 = nullptr);
 unsigned spmv_grid(const pp_csr *A);
 void set_slot(pp_ctx *ctx, int slot, double v);
+void scale_slot(pp_ctx *ctx, int slot, double v);  // scal[slot] *= v
 void ctl_init(pp_ctx *ctx, int first_check, int last, uint32_t maxit = 0);
 void copy_slot(pp_ctx *ctx, int dst, int src);
 

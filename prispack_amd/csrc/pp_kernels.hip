@@ -1086,6 +1086,12 @@ void set_slot(pp_ctx *ctx, int slot, double v) {
     PP_HIP(hipGetLastError());
 }
 
+__global__ void k_scale_slot(double *scal, int slot, double v) { scal[slot] *= v; }
+void scale_slot(pp_ctx *ctx, int slot, double v) {
+    k_scale_slot<<<1, 1, 0, ctx->stream>>>(ctx->scal, slot, v);
+    PP_HIP(hipGetLastError());
+}
+
 // scal[dst] = scal[src] (used to keep (b,b) while reusing slots)
 __global__ void k_copy_slot(double *scal, int dst, int src) { scal[dst] = scal[src]; }
 void copy_slot(pp_ctx *ctx, int dst, int src) {

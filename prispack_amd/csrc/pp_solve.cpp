@@ -38,7 +38,7 @@ struct PcgResult {
 
 // Runs iteration bodies until the stop word: the body (every launch of it:
 // sweeps, updates, SpMV) is captured once into a hipGraph and replayed, or
-// (use_graph false) launched on the stream each time.  kRing bodies stay in
+// (use_graph false) launched on the stream each time.  kInFlight bodies stay in
 // flight: the last convergence check of each body writes the control words
 // into the next slot of a mapped pinned ring, sequence number last, and the
 // host polls that number (no event or blocking wait between bodies, so
@@ -106,7 +106,7 @@ void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays,
     }
     if (run) {
         for (;;) {
-            while (issued < checked + R && issued < max_replays) issue();
+            while (issued < checked + pp_ctx::kInFlight && issued < max_replays) issue();
             if (!wait_published(checked)) break;  // read the words synchronously below
             for (int i = 0; i < C_COUNT; ++i) ctl[i] = slots[(checked % R) * C_COUNT + i];
             ++checked;
@@ -258,9 +258,16 @@ PcgResult pcg_sr_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const T *b, T *x,
 // unknowns.  Before: the reduced right-hand side r = KF rin; after:
 // e = [KE [rin0; e1]; e1], e1 = s1 * x.  Vectors of the second colour live
 // in z[n0..n), where KA reads them.
+//
+// resume: continue the previous call's iteration instead of starting anew
+// (a refinement step that keeps the search directions: the caller replaced
+// the residual by the true one, rescaled by `scale` relative to the previous
+// right-hand side, so p and s are rescaled by it and gamma_old by its square;
+// alpha is scale-free).  Without it every refinement step would restart the
+// Krylov iteration and lose its convergence.
 template <class T>
 PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, double tol, uint32_t maxit,
-                             uint32_t check_every, bool use_graph) {
+                             uint32_t check_every, bool use_graph, bool resume = false, double scale = 1.0) {
     const uint32_t n0 = R->n0, n1 = R->n1, n = n0 + n1;
     const int DT = dt<T>();
     const int base = sizeof(T) == 8 ? W_RD64 : W_RD32;
@@ -278,10 +285,16 @@ PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, do
     // the test is on the inner system's own residual: ||rin - A e|| is
     // ||Dt^1/2 r|| (the first colour's part vanishes), relative to ||rin||
     launch_dot(ctx, DT, n, rin, rin, S_BB);
-    launch_fill_zero(ctx, p, (size_t)n1 * sizeof(T));
-    launch_fill_zero(ctx, s, (size_t)n1 * sizeof(T));
     launch_fill_zero(ctx, x, (size_t)n1 * sizeof(T));
-    set_slot(ctx, S_CG_FIRST, 1.0);
+    if (resume) {
+        launch_axpby(ctx, DT, n1, scale, p, 0.0, p, p);
+        launch_axpby(ctx, DT, n1, scale, s, 0.0, s, s);
+        scale_slot(ctx, S_CG_GOLD, scale * scale);
+    } else {
+        launch_fill_zero(ctx, p, (size_t)n1 * sizeof(T));
+        launch_fill_zero(ctx, s, (size_t)n1 * sizeof(T));
+        set_slot(ctx, S_CG_FIRST, 1.0);
+    }
     const unsigned np = spmv_grid(R->KB);
     const uint32_t every = std::max<uint32_t>(1, check_every);
     auto body = [&] {
@@ -352,7 +365,8 @@ PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, cons
     Tin *e = (Tin *)ws_get(ctx, f32 ? W_E32 : W_E64, (size_t)n * sizeof(Tin));
     PcgResult res;
     launch_dot(ctx, PP_F64, n, b, b, S_TMP3);
-    const double bnorm = std::sqrt(read_slot(ctx, S_TMP3));
+    double bnorm = -1.0, prev_rnorm = 0.0;  // ||b|| is read with the first residual norm
+    bool resume = false;
     outer = 0;
     inner_total = 0;
     const double inner_tol = f32 ? (o.inner_tol > 0 ? o.inner_tol : 1e-6) : 0.0;
@@ -363,7 +377,15 @@ PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, cons
         PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         launch_spmv(ctx, A, PP_F64, -1.0, x, 1.0, r, -1, nullptr, false);
         launch_dot(ctx, PP_F64, n, r, r, S_TMP2);
-        const double rnorm = std::sqrt(read_slot(ctx, S_TMP2));
+        double rnorm;
+        if (bnorm < 0) {  // S_TMP2, S_TMP3 in one read
+            double v[2];
+            read_slots(ctx, S_TMP2, 2, v);
+            rnorm = std::sqrt(v[0]);
+            bnorm = std::sqrt(v[1]);
+        } else {
+            rnorm = std::sqrt(read_slot(ctx, S_TMP2));
+        }
         relres = bnorm > 0 ? rnorm / bnorm : rnorm;
         if (!(relres == relres)) {
             res.status = PP_ERR_NOT_FINITE;
@@ -378,17 +400,27 @@ PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, cons
         else launch_axpby(ctx, PP_F64, n, 1.0 / rnorm, r, 0.0, r, rin);
         launch_fill_zero(ctx, e, (size_t)n * sizeof(Tin));
         // FP32: a margin for the drift between the recurrence and the true
-        // residual; FP64 inner solves stop where a plain FP64 solve would
-        const double want = f32 ? std::max(inner_tol, 0.5 * o.tol * bnorm / rnorm) : o.tol * bnorm / rnorm;
+        // residual; FP64 inner solves stop where a plain FP64 solve would.
+        // A reduced inner solve that the next step resumes asks at most
+        // kResumeStep of a reduction per step: the FP32 recurrence residual
+        // then still agrees with the true one to about a percent, so the
+        // kept search directions stay conjugate to the replaced residual.
+        constexpr double kResumeStep = 1e-4;
+        const double floor_tol = R ? std::max(inner_tol, kResumeStep) : inner_tol;
+        const double want = f32 ? std::max(floor_tol, 0.5 * o.tol * bnorm / rnorm) : o.tol * bnorm / rnorm;
         const uint32_t inner_max = std::max<uint32_t>(2, o.maxit - inner_total);
         PcgResult in;
-        if (R) in = pcg_reduced_device<Tin>(ctx, R, rin, e, want, inner_max, o.check_every, graph);
+        if (R)
+            in = pcg_reduced_device<Tin>(ctx, R, rin, e, want, inner_max, o.check_every, graph, resume,
+                                         resume ? prev_rnorm / rnorm : 1.0);
         else if (o.cg_variant == PP_CG_REFERENCE)
             in = pcg_device<Tin>(ctx, A, M, o.flexible != 0, rin, e, want, inner_max, o.check_every, false,
                                  f32 ? W_R32 : W_R, graph);
         else in = pcg_sr_device<Tin>(ctx, A, M, rin, e, want, inner_max, o.check_every, false, graph);
         inner_total += in.iters;
         ++outer;
+        prev_rnorm = rnorm;
+        resume = R && in.conv;  // a step that stopped on its test is continued, any other restarts
         if (in.status != PP_OK && in.status != PP_ERR_NOT_FINITE) {
             res.status = in.status;
             break;
@@ -656,6 +688,9 @@ static int solve_impl(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts
         if (permuted) M->io_permuted = true;
         PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         PcgResult res;
+        // the refinement loop ends on an FP64 true residual of the final x
+        // (on the colour-permuted system when permuted: the same vector)
+        double refined_relres = -1.0;
         if (o.solver == PP_SOLVER_CG) {
             ReducedForm *red = nullptr;
             if (permuted && (o.cg_variant == PP_CG_AUTO || o.cg_variant == PP_CG_REDUCED) &&
@@ -665,12 +700,12 @@ static int solve_impl(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts
                      "reduced CG needs IC(0) in a two-colour (multicolour) ordering of a matrix whose colour "
                      "blocks are diagonal");
             if (o.precision == PP_MIXED) {
-                double rel;
+                double &rel = refined_relres;
                 res = pcg_refine<float>(ctx, As, M, red, o, bs, xs, R.outer_iters, R.inner_iters, rel);
                 R.cg_variant = red ? PP_CG_REDUCED
                                    : (o.cg_variant == PP_CG_REFERENCE ? PP_CG_REFERENCE : PP_CG_SINGLE_REDUCTION);
             } else if (red) {
-                double rel;
+                double &rel = refined_relres;
                 res = pcg_refine<double>(ctx, As, M, red, o, bs, xs, R.outer_iters, R.inner_iters, rel);
                 R.cg_variant = PP_CG_REDUCED;
             } else {
@@ -697,7 +732,7 @@ static int solve_impl(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts
         R.status = res.status;
         R.nhist = res.nhist;
         R.solve_ms = ms;
-        R.true_relres = true_relres(ctx, A, b, x);
+        R.true_relres = refined_relres >= 0 ? refined_relres : true_relres(ctx, A, b, x);
         if (rep) *rep = R;
         if (res.status != PP_OK) {
             throw Error(res.status, res.status == PP_ERR_BREAKDOWN ? "Krylov breakdown ((d, A d) = 0)"

@@ -77,8 +77,12 @@ def test_reduced_mixed(pp, torch, check_every):
     assert cg.report.cg_variant == 3  # auto picks the reduced inner solve here
     assert cg.getConvergence() and cg.report.true_relres < 1e-8
     ref, xr = solve(pp, torch, M, P, b, "fp64", "reference")
+    red, _ = solve(pp, torch, M, P, b, "fp64", "reduced", check_every=check_every)
     print(f"[reduced] mixed inner {cg.report.inner_iters} / outer {cg.report.outer_iters}, "
-          f"fp64 FlexibleCG {ref.getNumIter()}", flush=True)
+          f"fp64 FlexibleCG {ref.getNumIter()}, fp64 reduced {red.getNumIter()}", flush=True)
+    # refinement steps continue one Krylov iteration: about as many
+    # iterations in all as the FP64 solve, not a restart per step
+    assert cg.report.inner_iters <= red.getNumIter() + 6
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) <= 1e-6
     assert np.linalg.norm(x - xt) / np.linalg.norm(xt) <= 1e-5
     # the single-reduction inner solve is still available and agrees

@@ -75,8 +75,16 @@ enum Slot : int {
     S_CG_GOLD,
     S_CG_FIRST,
     S_TOL,       // convergence tolerance of the running PCG (read by its check)
-    S_COUNT = 24
+    // fused finish-and-update of the reduced CG: the iteration state it reads
+    // and the one it writes alternate between two blocks (every workgroup
+    // reads the state while the first writes the next one), each holding
+    // RS_GOLD (previous gamma), RS_AOLD (previous alpha), RS_FIRST, RS_LAST
+    // (the next check is the last), RS_CHECK (index of this check)
+    S_RED0,
+    S_RED1 = S_RED0 + 5,
+    S_COUNT = S_RED1 + 5
 };
+enum RedState : int { RS_GOLD = 0, RS_AOLD, RS_FIRST, RS_LAST, RS_CHECK };
 // Integer control words.
 enum Ctl : int {
     C_STOP = 0,   // bodies skip when set
@@ -414,6 +422,17 @@ void pcg_wv(pp_ctx *ctx, uint32_t n, const T *w, const T *v);
 void cgsr_finish(pp_ctx *ctx, unsigned np, int hist_on, bool publish);
 template <class T>
 void red_update(pp_ctx *ctx, uint32_t n, const T *w, T *p, T *s, T *x, T *r);
+// One launch that finishes the three reductions of the reduced CG (every
+// workgroup sums the np partials in the same order, so all hold the same
+// alpha and beta), runs the convergence test (the first workgroup keeps the
+// control words, the S_CG_* slots and the ring publication when `publish`)
+// and applies the update; parity selects the state block read (the other
+// is written).  Iteration bodies hold an even number of these.
+template <class T>
+void red_step(pp_ctx *ctx, unsigned np, int parity, bool publish, uint32_t n, const T *w, T *p, T *s, T *x, T *r);
+// State block 0 for a fresh reduced solve, or (resume) for one that
+// continues from S_CG_GOLD / S_CG_ALPHA / S_CG_FIRST, gamma scaled by gscale.
+void red_init(pp_ctx *ctx, bool resume, double gscale, bool last);
 template <class T>
 void red_post(pp_ctx *ctx, uint32_t n0, uint32_t n1, const T *x, const T *s1, const T *rin, T *z, T *e);
 // lvl0_hi > lvl0_lo: also u[i] = r_new[i] op diag0[i] over [lvl0_lo, lvl0_hi)

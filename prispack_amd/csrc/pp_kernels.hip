@@ -1035,6 +1035,149 @@ void red_update(pp_ctx *ctx, uint32_t n, const T *w, T *p, T *s, T *x, T *r) {
 template void red_update<double>(pp_ctx *, uint32_t, const double *, double *, double *, double *, double *);
 template void red_update<float>(pp_ctx *, uint32_t, const float *, float *, float *, float *, float *);
 
+// Fused finish + update of the reduced CG (see red_step in pp_internal.h).
+// The test mirrors pcg_check and k_cgsr_finish: a pending last check stops
+// with m = maxit; a non-finite norm or norm < tol stops with m = the check
+// index; a zero or non-finite denominator is a breakdown.
+template <class T>
+__global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ partials, unsigned np, double *scal,
+                                                   int *ctl, int *ring, int parity, uint32_t n,
+                                                   const T *__restrict__ w, T *__restrict__ p, T *__restrict__ s,
+                                                   T *__restrict__ x, T *__restrict__ r) {
+#pragma clang fp contract(off)
+    __shared__ double sh[kFinishThreads / 64];
+    __shared__ double bc[2];
+    __shared__ int go;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (ctl[C_STOP]) {  // an earlier check stopped (or this launch's first workgroup already did)
+        if (ring && lead) publish_ctl(ctl, ring);
+        return;
+    }
+    double t[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) t[q] = sum_partials(partials + (size_t)q * kMaxPartials, np);
+    double v[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[q] = block_sum(t[q], sh);
+    if (threadIdx.x == 0) {
+        const double *st = scal + (parity ? S_RED1 : S_RED0);
+        double *nx = scal + (parity ? S_RED0 : S_RED1);
+        const double delta = v[0], gamma = v[1], rho = v[2];
+        const double gold = st[RS_GOLD], aold = st[RS_AOLD];
+        const bool first = st[RS_FIRST] != 0.0, last = st[RS_LAST] != 0.0;
+        const int c = (int)st[RS_CHECK];
+        const int maxit = ctl[C_MAXIT];
+        const double norm = sqrt(rho) / sqrt(scal[S_BB]);
+        int stop = 0, conv = 0, status = PP_OK, m = 0;
+        bool den_fail = false;
+        double alpha = 0.0, beta = 0.0;
+        if (last) {
+            stop = 1;
+            m = maxit;
+        } else if (!(norm == norm) || isinf(norm)) {
+            stop = 1;
+            status = PP_ERR_NOT_FINITE;
+            m = c;
+        } else if (norm < scal[S_TOL]) {
+            stop = 1;
+            conv = 1;
+            m = c;
+        } else {
+            beta = first ? 0.0 : gamma / gold;
+            const double den = first ? delta : delta - beta * gamma / aold;
+            if (den == 0.0 || !(den == den)) {
+                stop = 1;
+                den_fail = true;
+                status = den == 0.0 ? PP_ERR_BREAKDOWN : PP_ERR_NOT_FINITE;
+                m = c + 1;
+            } else {
+                alpha = gamma / den;
+            }
+        }
+        bc[0] = alpha;
+        bc[1] = beta;
+        go = !stop;
+        if (blockIdx.x == 0) {
+            scal[S_CG_DELTA] = delta;
+            scal[S_CG_GAMMA] = gamma;
+            scal[S_CG_RHO] = rho;
+            if (!last) scal[S_TMP0] = norm;
+            const bool next_last = !stop && c >= maxit - 1;
+            if (stop) {
+                // the state a resumed solve continues from
+                scal[S_CG_GOLD] = gold;
+                scal[S_CG_ALPHA] = aold;
+                scal[S_CG_FIRST] = first ? 1.0 : 0.0;
+                ctl[C_CONV] = conv;
+                if (status != PP_OK) ctl[C_STATUS] = status;
+                ctl[C_M] = m;
+                if (den_fail) ctl[C_CHECK] = c + 1;  // the check itself had passed
+                ctl[C_STOP] = 1;
+            } else {
+                nx[RS_GOLD] = gamma;
+                nx[RS_AOLD] = alpha;
+                nx[RS_FIRST] = 0.0;
+                nx[RS_LAST] = next_last ? 1.0 : 0.0;
+                nx[RS_CHECK] = (double)(c + 1);
+                scal[S_CG_ALPHA] = alpha;
+                scal[S_CG_BETA] = beta;
+                scal[S_CG_GOLD] = gamma;
+                scal[S_CG_FIRST] = 0.0;
+                ctl[C_CHECK] = c + 1;
+                if (next_last) ctl[C_LAST] = 1;
+            }
+            if (ring) publish_ctl(ctl, ring);
+        }
+    }
+    __syncthreads();
+    if (!go) return;
+    const T alpha = (T)bc[0], beta = (T)bc[1];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const T ri = r[i];
+        const T pn = ri + beta * p[i];
+        const T sn = w[i] + beta * s[i];
+        p[i] = pn;
+        s[i] = sn;
+        x[i] = x[i] + alpha * pn;
+        r[i] = ri - alpha * sn;
+    }
+}
+
+template <class T>
+void red_step(pp_ctx *ctx, unsigned np, int parity, bool publish, uint32_t n, const T *w, T *p, T *s, T *x, T *r) {
+    // one 1024-thread workgroup per CU: each sums the partials once
+    const unsigned g = std::max(1u, std::min<unsigned>((unsigned)ctx->cu_count,
+                                                       (unsigned)((n + kFinishThreads * 4 - 1) / (kFinishThreads * 4))));
+    k_red_step<T><<<g, kFinishThreads, 0, ctx->stream>>>(ctx->red.partials + (size_t)S_CG_DELTA * kMaxPartials, np,
+                                                        ctx->scal, ctx->ctl, publish ? ctx->ring_dev : nullptr,
+                                                        parity, n, w, p, s, x, r);
+    PP_HIP(hipGetLastError());
+}
+template void red_step<double>(pp_ctx *, unsigned, int, bool, uint32_t, const double *, double *, double *,
+                               double *, double *);
+template void red_step<float>(pp_ctx *, unsigned, int, bool, uint32_t, const float *, float *, float *, float *,
+                              float *);
+
+__global__ void k_red_init(double *scal, int resume, double gscale, int last) {
+    double *st = scal + S_RED0;
+    if (resume) {
+        st[RS_GOLD] = scal[S_CG_GOLD] * gscale;
+        st[RS_AOLD] = scal[S_CG_ALPHA];
+        st[RS_FIRST] = scal[S_CG_FIRST];
+    } else {
+        st[RS_GOLD] = 0.0;
+        st[RS_AOLD] = 1.0;
+        st[RS_FIRST] = 1.0;
+    }
+    st[RS_LAST] = last ? 1.0 : 0.0;
+    st[RS_CHECK] = 1.0;
+}
+
+void red_init(pp_ctx *ctx, bool resume, double gscale, bool last) {
+    k_red_init<<<1, 1, 0, ctx->stream>>>(ctx->scal, resume ? 1 : 0, gscale, last ? 1 : 0);
+    PP_HIP(hipGetLastError());
+}
+
 // Back-substitution staging of the reduced solve: e1 = s1 * x into both
 // z[n0..) and e[n0..); z[0..n0) = rin[0..n0) (the first colour's residual,
 // which the elimination step D0^{-1} (r0 - A01 e1) reads next).

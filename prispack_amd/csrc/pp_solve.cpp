@@ -289,21 +289,21 @@ PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, do
     if (resume) {
         launch_axpby(ctx, DT, n1, scale, p, 0.0, p, p);
         launch_axpby(ctx, DT, n1, scale, s, 0.0, s, s);
-        scale_slot(ctx, S_CG_GOLD, scale * scale);
     } else {
         launch_fill_zero(ctx, p, (size_t)n1 * sizeof(T));
         launch_fill_zero(ctx, s, (size_t)n1 * sizeof(T));
-        set_slot(ctx, S_CG_FIRST, 1.0);
     }
+    red_init(ctx, resume, scale * scale, maxit <= 1);
     const unsigned np = spmv_grid(R->KB);
-    const uint32_t every = std::max<uint32_t>(1, check_every);
+    // the fused finish-and-update alternates its state blocks: an even
+    // number of iterations per body (the device still tests every one)
+    const uint32_t every = std::max<uint32_t>(2, check_every + (check_every & 1));
     auto body = [&] {
         StopScope scope(ctx);
         for (uint32_t k = 0; k < every; ++k) {
             launch_spmv(ctx, R->KA, DT, 1.0, z, 0.0, z, -1, nullptr, false);      // z[0, n0) = B r
             launch_spmv(ctx, R->KB, DT, 1.0, z, 0.0, w, S_CG_DELTA, r, false, r, wt);  // w = Sb r, (r,w), (r,r)
-            cgsr_finish(ctx, np, false, k + 1 == every);
-            red_update<T>(ctx, n1, w, p, s, x, r);
+            red_step<T>(ctx, np, (int)(k & 1), k + 1 == every, n1, w, p, s, x, r);
         }
     };
     int ctl[C_COUNT];

@@ -70,7 +70,7 @@ def log(*a):
 def kernel_time_ms(torch, stream_handle, fn, reps):
     """Average duration of fn() (which launches on the library's stream),
     measured with HIP events recorded on that same stream."""
-    ext = torch.cuda.ExternalStream(stream_handle)
+    ext = torch.cuda.current_stream() if stream_handle is None else torch.cuda.ExternalStream(stream_handle)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     fn()
     torch.cuda.synchronize()
@@ -373,6 +373,13 @@ def main():
     trsv32_ms = kernel_time_ms(torch, stream, lambda: L.pp_precond_apply(ctx.h, P.h, 1, ptv32, ptv32),
                                max(3, args.kernel_reps // 10))
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
+    # what a plain streaming kernel reaches on this device: an elementwise
+    # copy of 1 GiB (read 1 GiB, write 1 GiB), the same HIP-event timing
+    src = torch.ones((1 << 30) // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    copy_ms = kernel_time_ms(torch, None, lambda: torch.mul(src, 1.0, out=dst), 10)
+    copy_gbs = gbs(2 << 30, copy_ms)
+    del src, dst
     log(f"[bench] SpMV fp64 {spmv_ms * 1e3:.1f} us ({gbs(12 * nnz + 20 * n + 4, spmv_ms):.0f} GB/s algorithmic, "
         f"{gbs(spmv_bytes, spmv_ms):.0f} GB/s streamed); fp32 {spmv32_ms * 1e3:.1f} us; "
         f"M^-1 apply fp64 {trsv_ms * 1e3:.1f} us ({gbs(trsv_bytes, trsv_ms):.1f} GB/s streamed); "
@@ -395,6 +402,7 @@ def main():
                 "unit": "GB/s", "frac": round(gbs(alg_spmv, spmv_ms) / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": alg_spmv, "avg_us": round(spmv_ms * 1e3, 2),
                 "launches_timed": args.kernel_reps, "bytes_streamed": spmv_bytes,
+                "copy_kernel_GBs": round(copy_gbs, 1), "frac_of_copy": round(gbs(alg_spmv, spmv_ms) / copy_gbs, 4),
                 "streamed_GBs": round(gbs(spmv_bytes, spmv_ms), 1), "variant": spmv_var.value}
     apply_rec = {"kernel": f"IC(0) apply = forward + backward sweep ({sched} schedule, {info.levels_lower} levels)",
                  "bound": "hbm", "achieved": round(gbs(alg_apply, trsv_ms), 1), "peak": HBM_PEAK_GBS,

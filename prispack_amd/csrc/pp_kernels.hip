@@ -322,7 +322,10 @@ static void launch_finish(pp_ctx *ctx, int slot, unsigned np, const int *stop, i
 
 void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const void *x, double beta,
                  void *y, int dot_slot, const void *dot_with, bool, const void *aux, const void *wgt) {
-    unsigned grid = A->nblocks ? xcd_grid(A->nblocks, kMaxPartials) : 0;
+    // one row block per workgroup; a launch with inner products keeps to
+    // kMaxPartials workgroups (one partial each), looping over blocks
+    const bool dotl = dot_slot >= 0;
+    unsigned grid = A->nblocks ? xcd_grid(A->nblocks, dotl ? (unsigned)kMaxPartials : (1u << 20)) : 0;
     if (grid == 0) return;
     double *part = dot_slot >= 0 ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
     const int bz = beta == 0.0;

@@ -36,6 +36,34 @@ __device__ inline double block_sum(double v, double *sh) {
     return t;
 }
 
+// Three block sums in one LDS round (sh holds 3 * waves doubles); the same
+// operation order per value as block_sum.  Valid in thread 0.
+__device__ inline void block_sum3(double &a, double &b, double &c, double *sh) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) {
+        sh[wid] = a;
+        sh[nw + wid] = b;
+        sh[2 * nw + wid] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0, z = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            x += sh[w];
+            y += sh[nw + w];
+            z += sh[2 * nw + w];
+        }
+        a = x;
+        b = y;
+        c = z;
+    }
+    __syncthreads();
+}
+
 // XCD-aware work split.  Workgroups are dispatched round-robin over the
 // eight XCDs (workgroup i runs on XCD i % 8), each with its own L2.  Giving
 // XCD q the contiguous range [q nb / 8, (q+1) nb / 8) of row blocks keeps the
@@ -163,7 +191,7 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
                                               const T *__restrict__ dotw, const T *__restrict__ aux,
                                               const T *__restrict__ wgt, double *partials, const int *stop) {
     __shared__ T prod[kSpmvNnzBlock];
-    __shared__ double sh[4];
+    __shared__ double sh[3 * (kBlock / 64)];
     if (stop && *stop) return;
     // DOT: (dotw, y); with aux also (aux, dotw) and (aux, aux) into the next
     // two partial rows (the three reductions of the single-reduction PCG);
@@ -236,10 +264,13 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
         }
     }
     if (DOT) {
-        publish_partial(block_sum(dacc, sh), partials);
         if (aux) {
-            publish_partial(block_sum(dacc2, sh), partials + kMaxPartials);
-            publish_partial(block_sum(dacc3, sh), partials + 2 * kMaxPartials);
+            block_sum3(dacc, dacc2, dacc3, sh);
+            publish_partial(dacc, partials);
+            publish_partial(dacc2, partials + kMaxPartials);
+            publish_partial(dacc3, partials + 2 * kMaxPartials);
+        } else {
+            publish_partial(block_sum(dacc, sh), partials);
         }
     }
 }
@@ -925,7 +956,7 @@ template void pcg_xr_update<float>(pp_ctx *, uint32_t, float *, float *, const f
 __global__ __launch_bounds__(1024) void k_cgsr_finish(const double *__restrict__ partials, unsigned np,
                                                       double *scal, int *ctl, double *hist, uint32_t hist_cap,
                                                       int *ring) {
-    __shared__ double sh[kFinishThreads / 64];
+    __shared__ double sh[3 * (kFinishThreads / 64)];
     if (ctl[C_STOP]) {
         if (ring && threadIdx.x == 0) publish_ctl(ctl, ring);
         return;
@@ -933,9 +964,8 @@ __global__ __launch_bounds__(1024) void k_cgsr_finish(const double *__restrict__
     double t[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) t[q] = sum_partials(partials + (size_t)q * kMaxPartials, np);
-    double v[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) v[q] = block_sum(t[q], sh);
+    double v[3] = {t[0], t[1], t[2]};
+    block_sum3(v[0], v[1], v[2], sh);
     if (threadIdx.x == 0) {
         const double delta = v[0], gamma = v[1], rho = v[2];
         scal[S_CG_DELTA] = delta;
@@ -1048,7 +1078,7 @@ __global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ pa
                                                    const T *__restrict__ w, T *__restrict__ p, T *__restrict__ s,
                                                    T *__restrict__ x, T *__restrict__ r) {
 #pragma clang fp contract(off)
-    __shared__ double sh[kFinishThreads / 64];
+    __shared__ double sh[3 * (kFinishThreads / 64)];
     __shared__ double bc[2];
     __shared__ int go;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -1056,12 +1086,28 @@ __global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ pa
         if (ring && lead) publish_ctl(ctl, ring);
         return;
     }
+    // the first kU elements of this thread are loaded before the reduction
+    // (they do not depend on alpha / beta), so their latency overlaps it
+    constexpr int kU = 4;
+    const uint32_t stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    T vr[kU], vp[kU], vw[kU], vs[kU], vx[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const uint32_t i = i0 + u * stride;
+        vr[u] = vp[u] = vw[u] = vs[u] = vx[u] = 0;
+        if (i < n) {
+            vr[u] = r[i];
+            vp[u] = p[i];
+            vw[u] = w[i];
+            vs[u] = s[i];
+            vx[u] = x[i];
+        }
+    }
     double t[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) t[q] = sum_partials(partials + (size_t)q * kMaxPartials, np);
-    double v[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) v[q] = block_sum(t[q], sh);
+    double v[3] = {t[0], t[1], t[2]};
+    block_sum3(v[0], v[1], v[2], sh);
     if (threadIdx.x == 0) {
         const double *st = scal + (parity ? S_RED1 : S_RED0);
         double *nx = scal + (parity ? S_RED0 : S_RED1);
@@ -1135,7 +1181,19 @@ __global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ pa
     __syncthreads();
     if (!go) return;
     const T alpha = (T)bc[0], beta = (T)bc[1];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const uint32_t i = i0 + u * stride;
+        if (i < n) {
+            const T pn = vr[u] + beta * vp[u];
+            const T sn = vw[u] + beta * vs[u];
+            p[i] = pn;
+            s[i] = sn;
+            x[i] = vx[u] + alpha * pn;
+            r[i] = vr[u] - alpha * sn;
+        }
+    }
+    for (uint32_t i = i0 + kU * stride; i < n; i += stride) {
         const T ri = r[i];
         const T pn = ri + beta * p[i];
         const T sn = w[i] + beta * s[i];

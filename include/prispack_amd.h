@@ -134,7 +134,10 @@ enum {
     PP_TRSV_AUTO = 0,
     PP_TRSV_LEVELS = 1,    /* one launch per level (graph-captured)               */
     PP_TRSV_SYNCFREE = 2,  /* one launch per sweep, per-row ready flags            */
-    PP_TRSV_BLOCK = 3      /* one workgroup, rows of a level held in LDS           */
+    PP_TRSV_BLOCK = 3      /* both sweeps in one workgroup, the vector in LDS, a
+                              barrier per level (n * 8 bytes must fit in LDS;
+                              larger systems use LEVELS).  AUTO times it against
+                              LEVELS at setup when it fits                         */
 };
 
 /* Row ordering of the factor (no reference counterpart; the reference factors
@@ -173,9 +176,10 @@ typedef struct {
     double setup_ms;        /* host factorization + analysis + upload */
     uint32_t fail_row;      /* row of a failed pivot */
     int32_t sweep_variant;  /* staged-sweep loads chosen at setup: bit 0 non-temporal,
-                               bit 1 16-bit dependency offsets */
+                               bit 1 16-bit dependency offsets, bit 2 batched loads */
     int32_t fused_levels;   /* 1: last forward and first backward level share one launch */
-    uint32_t reserved[5];
+    int32_t trsv;           /* the SpTRSV schedule in use (PP_TRSV_AUTO resolved at setup) */
+    uint32_t reserved[4];
 } pp_precond_info;
 
 void pp_precond_opts_default(pp_precond_opts *o);

@@ -43,7 +43,8 @@ class PrecondInfo(C.Structure):
     _fields_ = [("kind", C.c_int), ("ready", C.c_int), ("n", C.c_uint32), ("annz", C.c_uint32),
                 ("mnnz", C.c_uint32), ("fill_ratio", C.c_double), ("levels_lower", C.c_uint32),
                 ("levels_upper", C.c_uint32), ("setup_ms", C.c_double), ("fail_row", C.c_uint32),
-                ("sweep_variant", C.c_int32), ("fused_levels", C.c_int32), ("reserved", C.c_uint32 * 5)]
+                ("sweep_variant", C.c_int32), ("fused_levels", C.c_int32), ("trsv", C.c_int32),
+                ("reserved", C.c_uint32 * 4)]
 
 
 class SolveOpts(C.Structure):

@@ -319,6 +319,11 @@ int pp_ctx_create(int device, pp_ctx **out) {
         hipDeviceProp_t prop;
         PP_HIP(hipGetDeviceProperties(&prop, device));
         c->cu_count = prop.multiProcessorCount;
+        {
+            int lds = 0;
+            if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+                c->max_lds = (size_t)lds;
+        }
         int clock_khz = 0;
         if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
             clock_khz <= 0)

@@ -119,6 +119,7 @@ struct pp_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int cu_count = 256;
+    size_t max_lds = 65536;         // LDS bytes one workgroup may use (device attribute)
     pp::RedWs red;
     double *scal = nullptr;       // device scalars [S_COUNT]
     int *ctl = nullptr;           // device control words [C_COUNT]
@@ -167,6 +168,7 @@ struct DevSweep {
     T *val = nullptr;           // [nnz]
     T *diag = nullptr;          // [n] (original numbering) or nullptr
     std::vector<uint32_t> level_ptr;  // host [nlev+1]
+    uint32_t *dlevel_ptr = nullptr;   // the same on the device (one-workgroup sweeps)
     // host [nlev]: perm[k] = k + level_off[l] over level l when the level's rows
     // are contiguous in the original numbering (the sweep then skips perm),
     // else kNoOffset
@@ -402,6 +404,13 @@ void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &bl
 
 template <class T>
 void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x);
+// Both sweeps in ONE workgroup of one launch, the vector held in LDS and a
+// workgroup barrier between levels (PP_TRSV_BLOCK): for systems whose
+// dependency chains are long and whose rows fit in LDS (n * sizeof(T) <=
+// ctx->max_lds).  y = U^{-1} L^{-1} x, y may alias x.
+template <class T>
+void launch_sweep_pair_block(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup,
+                             const T *x, T *y);
 template <class T>
 void launch_sweep_pair(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup, bool fuse,
                        const T *x, T *y, uint32_t first_level = 0);

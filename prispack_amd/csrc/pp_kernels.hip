@@ -684,6 +684,69 @@ template void launch_sweep_pair<double>(pp_ctx *, const DevSweep<double> &, int,
 template void launch_sweep_pair<float>(pp_ctx *, const DevSweep<float> &, int, const DevSweep<float> &, int, bool,
                                        const float *, float *, uint32_t);
 
+// Both sweeps of an apply in one workgroup (PP_TRSV_BLOCK).  The vector
+// lives in LDS; level after level the threads take the level's rows and a
+// workgroup barrier separates levels, so a dependency chain of L levels costs
+// L barriers instead of L launches.  Per row: the operations of
+// k_sweep_level in the same order (bit-identical results).
+template <class T>
+__device__ inline void block_sweep(const uint32_t *__restrict__ lp, uint32_t nlev, const uint32_t *__restrict__ perm,
+                                   const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+                                   const T *__restrict__ val, const T *__restrict__ diag, int mode, const T *b, T *xs) {
+#pragma clang fp contract(off)
+    uint32_t k0 = nlev ? lp[0] : 0;
+    for (uint32_t l = 0; l < nlev; ++l) {
+        const uint32_t k1 = lp[l + 1];
+        for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+            const uint32_t i = perm[k];
+            T s = b ? b[i] : xs[i];
+            if (mode == 3) s = s / diag[i];
+            const uint32_t e1 = rp[k + 1];
+            for (uint32_t e = rp[k]; e < e1; ++e) {
+                T p = val[e] * xs[ci[e]];
+                s = s - p;
+            }
+            if (mode == 0) s = s / diag[i];
+            else if (mode == 2) s = s * diag[i];
+            xs[i] = s;
+        }
+        k0 = k1;
+        __syncthreads();
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_sweep_pair_block(
+    uint32_t n, const uint32_t *lo_lp, uint32_t lo_nlev, const uint32_t *lo_perm, const uint32_t *lo_rp,
+    const uint32_t *lo_ci, const T *lo_val, const T *lo_diag, int lo_mode, const uint32_t *up_lp, uint32_t up_nlev,
+    const uint32_t *up_perm, const uint32_t *up_rp, const uint32_t *up_ci, const T *up_val, const T *up_diag,
+    int up_mode, const T *x, T *y, const int *stop) {
+    extern __shared__ unsigned char lds_raw[];
+    T *xs = reinterpret_cast<T *>(lds_raw);
+    if (stop && *stop) return;
+    block_sweep<T>(lo_lp, lo_nlev, lo_perm, lo_rp, lo_ci, lo_val, lo_diag, lo_mode, x, xs);
+    block_sweep<T>(up_lp, up_nlev, up_perm, up_rp, up_ci, up_val, up_diag, up_mode, nullptr, xs);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) y[i] = xs[i];
+}
+
+template <class T>
+void launch_sweep_pair_block(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup,
+                             const T *x, T *y) {
+    const size_t lds = (size_t)lo.n * sizeof(T);
+    PP_CHECK(lo.n == up.n && lds <= ctx->max_lds, PP_ERR_INVALID, "one-workgroup sweep: vector exceeds LDS");
+    PP_CHECK(lo.dlevel_ptr && up.dlevel_ptr, PP_ERR_INVALID, "one-workgroup sweep: level pointers missing");
+    if (lds > 65536) PP_HIP(hipFuncSetAttribute((const void *)k_sweep_pair_block<T>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_sweep_pair_block<T><<<1, 1024, lds, ctx->stream>>>(lo.n, lo.dlevel_ptr, lo.nlev, lo.perm, lo.rp, lo.ci, lo.val,
+                                                         lo.diag, mlo, up.dlevel_ptr, up.nlev, up.perm, up.rp, up.ci,
+                                                         up.val, up.diag, mup, x, y, ctx->stop_word);
+    PP_HIP(hipGetLastError());
+}
+template void launch_sweep_pair_block<double>(pp_ctx *, const DevSweep<double> &, int, const DevSweep<double> &, int,
+                                              const double *, double *);
+template void launch_sweep_pair_block<float>(pp_ctx *, const DevSweep<float> &, int, const DevSweep<float> &, int,
+                                             const float *, float *);
+
 template <class T>
 __global__ void k_level0(uint32_t lo, uint32_t hi, int mode, const T *__restrict__ diag, const T *__restrict__ x,
                          T *__restrict__ y, const int *stop) {

@@ -28,6 +28,8 @@ void DevSweep<T>::free_all() {
     dfree(ci16);
     dfree(blk_base);
     dfree(desc);
+    dfree(dlevel_ptr);
+    dlevel_ptr = nullptr;
     blocks = nullptr;
     ci16 = nullptr;
     blk_base = nullptr;
@@ -88,6 +90,10 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
     d.nnz = (uint32_t)ci.size();
     d.nlev = lp.empty() ? 0 : (uint32_t)lp.size() - 1;
     d.level_ptr = lp;
+    if (!lp.empty()) {
+        d.dlevel_ptr = dalloc<uint32_t>(lp.size());
+        PP_HIP(hipMemcpy(d.dlevel_ptr, lp.data(), lp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
     d.max_level_rows = 0;
     for (uint32_t l = 0; l < d.nlev; ++l) d.max_level_rows = std::max(d.max_level_rows, lp[l + 1] - lp[l]);
     d.level_off.assign(d.nlev, DevSweep<T>::kNoOffset);
@@ -406,6 +412,17 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
         }
         return;
     }
+    if (M->trsv == PP_TRSV_BLOCK && (size_t)M->n * esz <= ctx->max_lds) {
+        if (dtype == PP_F64) {
+            launch_sweep_pair_block<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, (const double *)x,
+                                            (double *)y);
+        } else {
+            precond_need_f32(M);
+            launch_sweep_pair_block<float>(ctx, M->low32, M->hlow.mode, M->up32, M->hup.mode, (const float *)x,
+                                           (float *)y);
+        }
+        return;
+    }
     if (dtype == PP_F64) {
         launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid,
                                   (const double *)x, (double *)y);
@@ -501,6 +518,38 @@ static void tune_sweeps(pp_ctx *ctx, pp_precond *M) {
     dfree(y);
 }
 
+// PP_TRSV_AUTO: the one-workgroup schedule when the FP64 vector fits in LDS
+// and an apply with it is faster than one launch per level (timed here, a
+// few applies each), else one launch per level.
+static int choose_schedule(pp_ctx *ctx, pp_precond *M) {
+    const uint32_t nlev = M->low64.nlev + M->up64.nlev;
+    if ((size_t)M->n * sizeof(double) > ctx->max_lds || nlev < 8) return PP_TRSV_LEVELS;
+    double *v = dalloc<double>(M->n);
+    float best[2] = {0.f, 0.f};
+    try {
+        launch_fill_zero(ctx, v, (size_t)M->n * sizeof(double));
+        for (int k = 0; k < 2; ++k) {
+            auto apply = [&] {
+                if (k == 0)
+                    launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, v, v);
+                else
+                    launch_sweep_pair_block<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, v, v);
+            };
+            apply();
+            PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+            for (int r = 0; r < 3; ++r) apply();
+            PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+            PP_HIP(hipEventSynchronize(ctx->ev1));
+            PP_HIP(hipEventElapsedTime(&best[k], ctx->ev0, ctx->ev1));
+        }
+    } catch (...) {
+        dfree(v);
+        throw;
+    }
+    dfree(v);
+    return best[1] < best[0] ? PP_TRSV_BLOCK : PP_TRSV_LEVELS;
+}
+
 extern "C" {
 
 int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
@@ -520,7 +569,7 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
             A.ci = M->A->h_ci;
             A.v = M->A->h_v;
             setup_triangular(M, A, true);
-            M->trsv = M->opts.trsv == PP_TRSV_AUTO ? PP_TRSV_LEVELS : M->opts.trsv;
+            M->trsv = M->opts.trsv == PP_TRSV_AUTO ? choose_schedule(ctx, M) : M->opts.trsv;
             if (M->trsv == PP_TRSV_LEVELS) tune_sweeps(ctx, M);
         }
         M->ready = 1;
@@ -661,6 +710,7 @@ int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {
     info->fail_row = M->fail_row;
     info->sweep_variant = M->sweep_variant;
     info->fused_levels = M->fuse_mid ? 1 : 0;
+    info->trsv = M->trsv;
     return PP_OK;
 }
 

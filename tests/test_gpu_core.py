@@ -172,10 +172,10 @@ def make_prec(pp, A, kind, args, trsv=0):
     return P
 
 
-TRSV = {"levels": 1, "syncfree": 2}
+TRSV = {"levels": 1, "syncfree": 2, "block": 3}
 
 
-@pytest.mark.parametrize("trsv", ["levels", "syncfree"])
+@pytest.mark.parametrize("trsv", ["levels", "syncfree", "block"])
 @pytest.mark.parametrize("kind,args,mat", PRECS)
 def test_precond_apply_bit_exact(pp, torch, kind, args, mat, trsv):
     A = prec_matrix(mat)
@@ -376,3 +376,42 @@ def test_single_reduction_pcg(pp, torch, kind):
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         cg.solve(M, dev(torch, b), x)
         assert cg.getConvergence() and cg.report.true_relres < 1e-8, variant
+
+
+@pytest.mark.parametrize("kind,args", [("ict", (1e-3,)), ("ilut", (1e-3, 10)), ("ildlt", (1e-3,))])
+def test_block_schedule_solves(pp, torch, kind, args):
+    # one-workgroup sweeps (PP_TRSV_BLOCK): the applies are bit-identical to
+    # the level schedule, so the FP64 solves are too; AUTO reports what it chose
+    A = ppo.laplacian(64, 64, 1) if kind != "ilut" else ppo.upwind(64, 64, 1, 20.0, 10.0, 5.0)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 6))
+    print(f"[block] {kind} n={n}", flush=True)
+    M = pp.CSRMatrix(*A)
+    out = {}
+    for trsv in (1, 3, 0):
+        P = make_prec(pp, A, kind, args, trsv)
+        info = P.info()
+        assert info.trsv == (trsv if trsv else info.trsv) and info.trsv in (1, 3)
+        if kind == "ilut":
+            s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000)
+            s.setRestart(True, 30)
+        else:
+            s = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        s.solve(M, dev(torch, b), x)
+        assert s.getConvergence() and s.report.true_relres < 1e-8
+        out[trsv] = (s.getNumIter(), host(x))
+    print(f"[block] {kind}: levels {out[1][0]} it, block {out[3][0]} it", flush=True)
+    assert out[1][0] == out[3][0] == out[0][0]
+    assert np.array_equal(out[1][1], out[3][1]) and np.array_equal(out[1][1], out[0][1])
+    # mixed precision runs the FP32 one-workgroup sweeps
+    P = make_prec(pp, A, kind, args, 3)
+    if kind == "ilut":
+        s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000, precision="mixed")
+        s.setRestart(True, 30)
+    else:
+        s = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision="mixed")
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    s.solve(M, dev(torch, b), x)
+    assert s.getConvergence() and s.report.true_relres < 1e-8
+

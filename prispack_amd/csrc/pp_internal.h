@@ -168,7 +168,6 @@ struct DevSweep {
     T *val = nullptr;           // [nnz]
     T *diag = nullptr;          // [n] (original numbering) or nullptr
     std::vector<uint32_t> level_ptr;  // host [nlev+1]
-    uint32_t *dlevel_ptr = nullptr;   // the same on the device (one-workgroup sweeps)
     // host [nlev]: perm[k] = k + level_off[l] over level l when the level's rows
     // are contiguous in the original numbering (the sweep then skips perm),
     // else kNoOffset
@@ -188,6 +187,31 @@ struct DevSweep {
     uint64_t nnz16 = 0, nblk = 0, perm_rows = 0;  // entries on 16-bit offsets, row blocks, rows read through perm
     int variant = kVarDefault;                     // kVarNT | kVarIdx16 for the staged kernel
     uint32_t max_level_rows = 0;
+    void free_all();
+};
+
+// PP_TRSV_BLOCK: the rows split into contiguous chunks that no dependency of
+// either sweep crosses, each small enough for its part of the vector to sit
+// in LDS.  One workgroup per chunk runs both sweeps over its rows, level by
+// level with a workgroup barrier between levels, so a dependency chain of L
+// levels costs L barriers instead of L launches, and independent diagonal
+// blocks (the subdomains of a multilevel preconditioner) run side by side.
+struct ChunkSet {
+    std::vector<uint32_t> start;   // host [nchunk+1] first original row of each chunk
+    uint32_t *dstart = nullptr;    // the same on the device
+    uint32_t nchunk = 0, max_rows = 0;
+    void free_all();
+};
+template <class T>
+struct DevChunkSweep {
+    uint32_t n = 0, nnz = 0, max_levels = 0;
+    uint32_t *perm = nullptr;      // [n] position -> original row (chunk-major, level order inside)
+    uint32_t *lp = nullptr;        // level boundaries (positions); chunk c: lp[lev_off[c] .. lev_off[c+1]-1]
+    uint32_t *lev_off = nullptr;   // [nchunk+1]
+    uint32_t *rp = nullptr;        // [n+1] by position
+    uint16_t *ci = nullptr;        // [nnz] dependency row minus the chunk's first row
+    T *val = nullptr;              // [nnz]
+    T *diag = nullptr;             // [n] original numbering, or nullptr
     void free_all();
 };
 
@@ -310,6 +334,10 @@ struct pp_precond {
     uint64_t Aperm_version = 0;
     bool io_permuted = false;
     pp::ReducedForm *reduced = nullptr;  // two-colour Schur-complement form (IC(0) only)
+    // PP_TRSV_BLOCK storage (built when the rows chunk, see ChunkSet)
+    pp::ChunkSet chunks;
+    pp::DevChunkSweep<double> clow64, cup64;
+    pp::DevChunkSweep<float> clow32, cup32;
 };
 
 namespace pp {
@@ -322,6 +350,9 @@ T *dalloc(size_t n);
 void dfree(void *p);
 void ensure_hist(pp_ctx *ctx, uint32_t cap);
 void *ws_get(pp_ctx *ctx, int idx, size_t bytes);
+// Workspace slots owned by the GMRES / Gram-Schmidt helpers; the solvers'
+// own vectors use the slots below kWsGmresState and from kWsFree on.
+constexpr int kWsGmresState = 30, kWsGmresCoeff = 31, kWsFree = 32;
 
 // ---- host factorizations (pp_factor.cpp) ----
 struct HostCsr {
@@ -404,13 +435,11 @@ void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &bl
 
 template <class T>
 void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x);
-// Both sweeps in ONE workgroup of one launch, the vector held in LDS and a
-// workgroup barrier between levels (PP_TRSV_BLOCK): for systems whose
-// dependency chains are long and whose rows fit in LDS (n * sizeof(T) <=
-// ctx->max_lds).  y = U^{-1} L^{-1} x, y may alias x.
+// Both sweeps of an apply in one launch, one workgroup per chunk
+// (PP_TRSV_BLOCK; DevChunkSweep).  y = U^{-1} L^{-1} x, y may alias x.
 template <class T>
-void launch_sweep_pair_block(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup,
-                             const T *x, T *y);
+void launch_sweep_pair_chunks(pp_ctx *ctx, const ChunkSet &C, const DevChunkSweep<T> &lo, int mlo,
+                              const DevChunkSweep<T> &up, int mup, const T *x, T *y);
 template <class T>
 void launch_sweep_pair(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup, bool fuse,
                        const T *x, T *y, uint32_t first_level = 0);

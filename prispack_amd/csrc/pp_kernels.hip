@@ -626,7 +626,10 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
 template <class T>
 void launch_levels(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x, uint32_t l0, uint32_t l1,
                    int mode2, const T *diag2) {
-    const int *stop = ctx->ctl + C_STOP;
+    // early exit only inside a solve's iteration body (StopScope): a sweep
+    // launched elsewhere (a preconditioner setup, a plain apply) always runs,
+    // whatever stop word an earlier solve left behind
+    const int *stop = ctx->stop_word;
     for (uint32_t l = l0; l < l1; ++l) {
         uint32_t k0 = s.level_ptr[l], k1 = s.level_ptr[l + 1];
         const bool contiguous = l < s.level_off.size() && s.level_off[l] != DevSweep<T>::kNoOffset;
@@ -684,22 +687,21 @@ template void launch_sweep_pair<double>(pp_ctx *, const DevSweep<double> &, int,
 template void launch_sweep_pair<float>(pp_ctx *, const DevSweep<float> &, int, const DevSweep<float> &, int, bool,
                                        const float *, float *, uint32_t);
 
-// Both sweeps of an apply in one workgroup (PP_TRSV_BLOCK).  The vector
-// lives in LDS; level after level the threads take the level's rows and a
-// workgroup barrier separates levels, so a dependency chain of L levels costs
-// L barriers instead of L launches.  Per row: the operations of
-// k_sweep_level in the same order (bit-identical results).
+// Both sweeps of an apply, one workgroup per chunk (PP_TRSV_BLOCK).  The
+// chunk's part of the vector lives in LDS; level after level the threads take
+// the level's rows and a workgroup barrier separates levels.  Per row: the
+// operations of k_sweep_level in the same order (bit-identical results).
 template <class T>
-__device__ inline void block_sweep(const uint32_t *__restrict__ lp, uint32_t nlev, const uint32_t *__restrict__ perm,
-                                   const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
-                                   const T *__restrict__ val, const T *__restrict__ diag, int mode, const T *b, T *xs) {
+__device__ inline void chunk_sweep(const uint32_t *__restrict__ lp, uint32_t nlev, const uint32_t *__restrict__ perm,
+                                   const uint32_t *__restrict__ rp, const uint16_t *__restrict__ ci,
+                                   const T *__restrict__ val, const T *__restrict__ diag, int mode, uint32_t r0,
+                                   const T *b, T *xs) {
 #pragma clang fp contract(off)
-    uint32_t k0 = nlev ? lp[0] : 0;
     for (uint32_t l = 0; l < nlev; ++l) {
         const uint32_t k1 = lp[l + 1];
-        for (uint32_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) {
+        for (uint32_t k = lp[l] + threadIdx.x; k < k1; k += blockDim.x) {
             const uint32_t i = perm[k];
-            T s = b ? b[i] : xs[i];
+            T s = b ? b[i] : xs[i - r0];
             if (mode == 3) s = s / diag[i];
             const uint32_t e1 = rp[k + 1];
             for (uint32_t e = rp[k]; e < e1; ++e) {
@@ -708,44 +710,47 @@ __device__ inline void block_sweep(const uint32_t *__restrict__ lp, uint32_t nle
             }
             if (mode == 0) s = s / diag[i];
             else if (mode == 2) s = s * diag[i];
-            xs[i] = s;
+            xs[i - r0] = s;
         }
-        k0 = k1;
         __syncthreads();
     }
 }
 
 template <class T>
-__global__ __launch_bounds__(1024) void k_sweep_pair_block(
-    uint32_t n, const uint32_t *lo_lp, uint32_t lo_nlev, const uint32_t *lo_perm, const uint32_t *lo_rp,
-    const uint32_t *lo_ci, const T *lo_val, const T *lo_diag, int lo_mode, const uint32_t *up_lp, uint32_t up_nlev,
-    const uint32_t *up_perm, const uint32_t *up_rp, const uint32_t *up_ci, const T *up_val, const T *up_diag,
-    int up_mode, const T *x, T *y, const int *stop) {
+__global__ __launch_bounds__(1024) void k_sweep_chunks(
+    const uint32_t *__restrict__ start, const uint32_t *lo_lp, const uint32_t *lo_off, const uint32_t *lo_perm,
+    const uint32_t *lo_rp, const uint16_t *lo_ci, const T *lo_val, const T *lo_diag, int lo_mode,
+    const uint32_t *up_lp, const uint32_t *up_off, const uint32_t *up_perm, const uint32_t *up_rp,
+    const uint16_t *up_ci, const T *up_val, const T *up_diag, int up_mode, const T *x, T *y, const int *stop) {
     extern __shared__ unsigned char lds_raw[];
     T *xs = reinterpret_cast<T *>(lds_raw);
     if (stop && *stop) return;
-    block_sweep<T>(lo_lp, lo_nlev, lo_perm, lo_rp, lo_ci, lo_val, lo_diag, lo_mode, x, xs);
-    block_sweep<T>(up_lp, up_nlev, up_perm, up_rp, up_ci, up_val, up_diag, up_mode, nullptr, xs);
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) y[i] = xs[i];
+    const uint32_t c = blockIdx.x, r0 = start[c], r1 = start[c + 1];
+    const uint32_t lo0 = lo_off[c], lo1 = lo_off[c + 1], up0 = up_off[c], up1 = up_off[c + 1];
+    chunk_sweep<T>(lo_lp + lo0, lo1 - lo0 - 1, lo_perm, lo_rp, lo_ci, lo_val, lo_diag, lo_mode, r0, x, xs);
+    chunk_sweep<T>(up_lp + up0, up1 - up0 - 1, up_perm, up_rp, up_ci, up_val, up_diag, up_mode, r0, nullptr, xs);
+    for (uint32_t j = threadIdx.x; j < r1 - r0; j += blockDim.x) y[r0 + j] = xs[j];
 }
 
 template <class T>
-void launch_sweep_pair_block(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup,
-                             const T *x, T *y) {
-    const size_t lds = (size_t)lo.n * sizeof(T);
-    PP_CHECK(lo.n == up.n && lds <= ctx->max_lds, PP_ERR_INVALID, "one-workgroup sweep: vector exceeds LDS");
-    PP_CHECK(lo.dlevel_ptr && up.dlevel_ptr, PP_ERR_INVALID, "one-workgroup sweep: level pointers missing");
-    if (lds > 65536) PP_HIP(hipFuncSetAttribute((const void *)k_sweep_pair_block<T>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_sweep_pair_block<T><<<1, 1024, lds, ctx->stream>>>(lo.n, lo.dlevel_ptr, lo.nlev, lo.perm, lo.rp, lo.ci, lo.val,
-                                                         lo.diag, mlo, up.dlevel_ptr, up.nlev, up.perm, up.rp, up.ci,
-                                                         up.val, up.diag, mup, x, y, ctx->stop_word);
+void launch_sweep_pair_chunks(pp_ctx *ctx, const ChunkSet &C, const DevChunkSweep<T> &lo, int mlo,
+                              const DevChunkSweep<T> &up, int mup, const T *x, T *y) {
+    const size_t lds = (size_t)C.max_rows * sizeof(T);
+    PP_CHECK(C.nchunk && lds <= ctx->max_lds && lo.perm && up.perm, PP_ERR_INVALID,
+             "chunked sweep: storage missing or chunk exceeds LDS");
+    if (lds > 65536)
+        PP_HIP(hipFuncSetAttribute((const void *)k_sweep_chunks<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    const unsigned threads = C.max_rows >= 1024 ? 1024u : std::max(64u, (C.max_rows + 63) / 64 * 64);
+    k_sweep_chunks<T><<<C.nchunk, threads, lds, ctx->stream>>>(
+        C.dstart, lo.lp, lo.lev_off, lo.perm, lo.rp, lo.ci, lo.val, lo.diag, mlo, up.lp, up.lev_off, up.perm, up.rp,
+        up.ci, up.val, up.diag, mup, x, y, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
-template void launch_sweep_pair_block<double>(pp_ctx *, const DevSweep<double> &, int, const DevSweep<double> &, int,
-                                              const double *, double *);
-template void launch_sweep_pair_block<float>(pp_ctx *, const DevSweep<float> &, int, const DevSweep<float> &, int,
-                                             const float *, float *);
+template void launch_sweep_pair_chunks<double>(pp_ctx *, const ChunkSet &, const DevChunkSweep<double> &, int,
+                                               const DevChunkSweep<double> &, int, const double *, double *);
+template void launch_sweep_pair_chunks<float>(pp_ctx *, const ChunkSet &, const DevChunkSweep<float> &, int,
+                                              const DevChunkSweep<float> &, int, const float *, float *);
 
 template <class T>
 __global__ void k_level0(uint32_t lo, uint32_t hi, int mode, const T *__restrict__ diag, const T *__restrict__ x,

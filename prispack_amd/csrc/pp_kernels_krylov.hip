@@ -22,7 +22,7 @@ __device__ inline double wsum(double v) {
 // Buffer of GMRES coefficients on the device: [0,320) h, [320, 640) y or a copy of h, and
 // the per-column partials of the GEMV^T behind it.
 double *gmres_coeff_buffer(pp_ctx *ctx) {
-    return (double *)ws_get(ctx, 31, (size_t)(640 + (size_t)kMaxCols * kGemvBlocks) * sizeof(double));
+    return (double *)ws_get(ctx, kWsGmresCoeff, (size_t)(640 + (size_t)kMaxCols * kGemvBlocks) * sizeof(double));
 }
 
 // partial[c][block] = sum over the block's rows of V[r, c] * w[r]
@@ -152,7 +152,7 @@ namespace pp {
 GmresDev gmres_state(pp_ctx *ctx, uint32_t mm) {
     const size_t ld = mm + 1;
     const size_t total = ld * mm + 5 * ld + kGmMisc;
-    double *base = (double *)ws_get(ctx, 30, total * sizeof(double));
+    double *base = (double *)ws_get(ctx, kWsGmresState, total * sizeof(double));
     GmresDev st;
     st.mm = mm;
     st.H = base;

@@ -505,7 +505,9 @@ void ml_setup(pp_ctx *ctx, pp_precond *M) {
         const uint32_t ndom = M->opts.ndom ? M->opts.ndom : 16;
         S->hid = build_hid(A, ndom, nlev);
         const Hid &H = S->hid;
-        const int trsv = M->opts.trsv == PP_TRSV_AUTO ? PP_TRSV_LEVELS : M->opts.trsv;
+        // the block factors choose their own schedule (AUTO: the subdomain
+        // blocks of B run as independent one-workgroup chunks when they fit)
+        const int trsv = M->opts.trsv;
         // factor drop tolerance: the reference's are 1e-3 (AMSED) and 1e-2
         // (GMSLR) (AMSEDPrecondition.h:39-51, GMSLRPrecondition.h:25-37); 0 gives
         // exact block factors

@@ -28,8 +28,6 @@ void DevSweep<T>::free_all() {
     dfree(ci16);
     dfree(blk_base);
     dfree(desc);
-    dfree(dlevel_ptr);
-    dlevel_ptr = nullptr;
     blocks = nullptr;
     ci16 = nullptr;
     blk_base = nullptr;
@@ -90,10 +88,6 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
     d.nnz = (uint32_t)ci.size();
     d.nlev = lp.empty() ? 0 : (uint32_t)lp.size() - 1;
     d.level_ptr = lp;
-    if (!lp.empty()) {
-        d.dlevel_ptr = dalloc<uint32_t>(lp.size());
-        PP_HIP(hipMemcpy(d.dlevel_ptr, lp.data(), lp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    }
     d.max_level_rows = 0;
     for (uint32_t l = 0; l < d.nlev; ++l) d.max_level_rows = std::max(d.max_level_rows, lp[l + 1] - lp[l]);
     d.level_off.assign(d.nlev, DevSweep<T>::kNoOffset);
@@ -167,6 +161,149 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
         PP_HIP(hipMemcpy(d.diag, dg.data(), n * sizeof(T), hipMemcpyHostToDevice));
     }
 }
+void ChunkSet::free_all() {
+    dfree(dstart);
+    dstart = nullptr;
+    start.clear();
+    nchunk = max_rows = 0;
+}
+
+template <class T>
+void DevChunkSweep<T>::free_all() {
+    dfree(perm);
+    dfree(lp);
+    dfree(lev_off);
+    dfree(rp);
+    dfree(ci);
+    dfree(val);
+    dfree(diag);
+    perm = lp = lev_off = rp = nullptr;
+    ci = nullptr;
+    val = diag = nullptr;
+    n = nnz = max_levels = 0;
+}
+template struct DevChunkSweep<double>;
+template struct DevChunkSweep<float>;
+
+// Contiguous chunks of at most cap rows that no dependency of the given
+// sweeps crosses; false when a dependency-closed range is larger than cap.
+static bool build_chunks(const std::vector<const HostSweep *> &sw, uint32_t n, uint32_t cap, ChunkSet &C) {
+    // blocked[r] > 0: some dependency spans the cut between rows r-1 and r
+    std::vector<int32_t> blocked(n + 1, 0);
+    for (const HostSweep *s : sw)
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t a = i, b = i;
+            for (uint32_t k = s->rp[i]; k < s->rp[i + 1]; ++k) {
+                a = std::min(a, s->ci[k]);
+                b = std::max(b, s->ci[k]);
+            }
+            if (a < b) {
+                blocked[a + 1] += 1;
+                blocked[b + 1] -= 1;
+            }
+        }
+    C.start.assign(1, 0);
+    C.max_rows = 0;
+    int32_t run = 0;
+    uint32_t comp0 = 0;  // first row of the current dependency-closed component
+    for (uint32_t r = 1; r <= n; ++r) {
+        run += blocked[r];
+        if (r < n && run > 0) continue;  // no cut before row r
+        // [comp0, r) is a component
+        if (r - comp0 > cap) return false;
+        if (r - C.start.back() > cap) C.start.push_back(comp0);  // close the chunk before this component
+        comp0 = r;
+    }
+    C.start.push_back(n);
+    C.nchunk = (uint32_t)C.start.size() - 1;
+    for (uint32_t c = 0; c < C.nchunk; ++c) C.max_rows = std::max(C.max_rows, C.start[c + 1] - C.start[c]);
+    return true;
+}
+
+// The sweep in chunk-major, level-ordered positions.
+template <class T>
+static void upload_chunk_sweep(const HostSweep &s, const ChunkSet &C, DevChunkSweep<T> &d) {
+    d.free_all();
+    const uint32_t n = s.n;
+    std::vector<uint32_t> gperm, glp;
+    levels_of(s, gperm, glp);
+    std::vector<uint32_t> lev(n), chunk_of(n);
+    for (uint32_t l = 0; l + 1 < glp.size(); ++l)
+        for (uint32_t k = glp[l]; k < glp[l + 1]; ++k) lev[gperm[k]] = l;
+    for (uint32_t c = 0; c < C.nchunk; ++c)
+        for (uint32_t i = C.start[c]; i < C.start[c + 1]; ++i) chunk_of[i] = c;
+    // positions: chunk-major, then global level order (stable)
+    std::vector<std::vector<uint32_t>> rows(C.nchunk);
+    for (uint32_t k = 0; k < n; ++k) rows[chunk_of[gperm[k]]].push_back(gperm[k]);
+    std::vector<uint32_t> perm, lp, off, rp(1, 0);
+    std::vector<uint16_t> ci;
+    std::vector<T> val;
+    perm.reserve(n);
+    off.push_back(0);
+    for (uint32_t c = 0; c < C.nchunk; ++c) {
+        const uint32_t r0 = C.start[c];
+        uint32_t nl = 0;
+        for (size_t q = 0; q < rows[c].size(); ++q) {
+            const uint32_t i = rows[c][q];
+            if (q == 0 || lev[i] != lev[rows[c][q - 1]]) {
+                lp.push_back((uint32_t)perm.size());
+                ++nl;
+            }
+            perm.push_back(i);
+            for (uint32_t k = s.rp[i]; k < s.rp[i + 1]; ++k) {
+                ci.push_back((uint16_t)(s.ci[k] - r0));
+                val.push_back((T)s.val[k]);
+            }
+            rp.push_back((uint32_t)ci.size());
+        }
+        lp.push_back((uint32_t)perm.size());
+        off.push_back((uint32_t)lp.size());
+        d.max_levels = std::max(d.max_levels, nl);
+    }
+    d.n = n;
+    d.nnz = (uint32_t)ci.size();
+    auto up = [](const auto &v, auto *&dst) {
+        using E = std::remove_reference_t<decltype(v[0])>;
+        dst = dalloc<std::remove_const_t<E>>(v.size());
+        if (!v.empty()) PP_HIP(hipMemcpy(dst, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice));
+    };
+    up(perm, d.perm);
+    up(lp, d.lp);
+    up(off, d.lev_off);
+    up(rp, d.rp);
+    up(ci, d.ci);
+    up(val, d.val);
+    if (s.mode != HostSweep::UNIT) {
+        std::vector<T> dg(s.diag.begin(), s.diag.end());
+        up(dg, d.diag);
+    }
+}
+
+// PP_TRSV_BLOCK storage for M (FP64; FP32 too when the FP32 sweeps exist).
+// false (nothing kept) when the rows do not chunk within the LDS size.
+static bool build_block_sweeps(pp_ctx *ctx, pp_precond *M) {
+    M->chunks.free_all();
+    M->clow64.free_all();
+    M->cup64.free_all();
+    M->clow32.free_all();
+    M->cup32.free_all();
+    const uint32_t cap = (uint32_t)std::min<size_t>(ctx->max_lds / sizeof(double), 65535);
+    if (!build_chunks({&M->hlow, &M->hup}, M->n, cap, M->chunks)) {
+        M->chunks.free_all();
+        return false;
+    }
+    M->chunks.dstart = dalloc<uint32_t>(M->chunks.start.size());
+    PP_HIP(hipMemcpy(M->chunks.dstart, M->chunks.start.data(), M->chunks.start.size() * sizeof(uint32_t),
+                     hipMemcpyHostToDevice));
+    upload_chunk_sweep(M->hlow, M->chunks, M->clow64);
+    upload_chunk_sweep(M->hup, M->chunks, M->cup64);
+    if (M->have32) {
+        upload_chunk_sweep(M->hlow, M->chunks, M->clow32);
+        upload_chunk_sweep(M->hup, M->chunks, M->cup32);
+    }
+    return true;
+}
+
 template void upload_sweep<double>(const HostSweep &, DevSweep<double> &);
 template void upload_sweep<float>(const HostSweep &, DevSweep<float> &);
 
@@ -325,6 +462,10 @@ void precond_need_f32(pp_precond *M) {
     upload_sweep(M->hlow, M->low32);
     upload_sweep(M->hup, M->up32);
     M->low32.variant = M->up32.variant = M->sweep_variant;
+    if (M->chunks.nchunk) {
+        upload_chunk_sweep(M->hlow, M->chunks, M->clow32);
+        upload_chunk_sweep(M->hup, M->chunks, M->cup32);
+    }
     M->have32 = true;
 }
 
@@ -412,14 +553,14 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
         }
         return;
     }
-    if (M->trsv == PP_TRSV_BLOCK && (size_t)M->n * esz <= ctx->max_lds) {
+    if (M->trsv == PP_TRSV_BLOCK) {
         if (dtype == PP_F64) {
-            launch_sweep_pair_block<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, (const double *)x,
-                                            (double *)y);
+            launch_sweep_pair_chunks<double>(ctx, M->chunks, M->clow64, M->hlow.mode, M->cup64, M->hup.mode,
+                                             (const double *)x, (double *)y);
         } else {
             precond_need_f32(M);
-            launch_sweep_pair_block<float>(ctx, M->low32, M->hlow.mode, M->up32, M->hup.mode, (const float *)x,
-                                           (float *)y);
+            launch_sweep_pair_chunks<float>(ctx, M->chunks, M->clow32, M->hlow.mode, M->cup32, M->hup.mode,
+                                            (const float *)x, (float *)y);
         }
         return;
     }
@@ -518,12 +659,14 @@ static void tune_sweeps(pp_ctx *ctx, pp_precond *M) {
     dfree(y);
 }
 
-// PP_TRSV_AUTO: the one-workgroup schedule when the FP64 vector fits in LDS
-// and an apply with it is faster than one launch per level (timed here, a
-// few applies each), else one launch per level.
-static int choose_schedule(pp_ctx *ctx, pp_precond *M) {
+// PP_TRSV_AUTO: the chunked one-workgroup schedule when the rows chunk
+// within LDS and an apply with it is faster than one launch per level (timed
+// here, a few applies each), else one launch per level.  PP_TRSV_BLOCK asked
+// for explicitly falls back to levels when the rows do not chunk.
+static int choose_schedule(pp_ctx *ctx, pp_precond *M, bool force_block) {
     const uint32_t nlev = M->low64.nlev + M->up64.nlev;
-    if ((size_t)M->n * sizeof(double) > ctx->max_lds || nlev < 8) return PP_TRSV_LEVELS;
+    if ((!force_block && nlev < 8) || !build_block_sweeps(ctx, M)) return PP_TRSV_LEVELS;
+    if (force_block) return PP_TRSV_BLOCK;
     double *v = dalloc<double>(M->n);
     float best[2] = {0.f, 0.f};
     try {
@@ -533,7 +676,8 @@ static int choose_schedule(pp_ctx *ctx, pp_precond *M) {
                 if (k == 0)
                     launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, v, v);
                 else
-                    launch_sweep_pair_block<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, v, v);
+                    launch_sweep_pair_chunks<double>(ctx, M->chunks, M->clow64, M->hlow.mode, M->cup64, M->hup.mode,
+                                                     v, v);
             };
             apply();
             PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
@@ -547,7 +691,11 @@ static int choose_schedule(pp_ctx *ctx, pp_precond *M) {
         throw;
     }
     dfree(v);
-    return best[1] < best[0] ? PP_TRSV_BLOCK : PP_TRSV_LEVELS;
+    if (best[1] < best[0]) return PP_TRSV_BLOCK;
+    M->chunks.free_all();  // not used: give the memory back
+    M->clow64.free_all();
+    M->cup64.free_all();
+    return PP_TRSV_LEVELS;
 }
 
 extern "C" {
@@ -569,7 +717,9 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
             A.ci = M->A->h_ci;
             A.v = M->A->h_v;
             setup_triangular(M, A, true);
-            M->trsv = M->opts.trsv == PP_TRSV_AUTO ? choose_schedule(ctx, M) : M->opts.trsv;
+            M->trsv = M->opts.trsv == PP_TRSV_AUTO    ? choose_schedule(ctx, M, false)
+                      : M->opts.trsv == PP_TRSV_BLOCK ? choose_schedule(ctx, M, true)
+                                                      : M->opts.trsv;
             if (M->trsv == PP_TRSV_LEVELS) tune_sweeps(ctx, M);
         }
         M->ready = 1;
@@ -743,6 +893,11 @@ void pp_precond_destroy(pp_precond *M) {
     dfree(M->tmp32);
     csr_free(M->Aperm);
     pp::reduced_free(M);
+    M->chunks.free_all();
+    M->clow64.free_all();
+    M->cup64.free_all();
+    M->clow32.free_all();
+    M->cup32.free_all();
     M->low64.free_all();
     M->up64.free_all();
     M->low32.free_all();

@@ -19,9 +19,10 @@ namespace {
 
 // Workspace indices in the context cache.
 enum { W_R = 0, W_W, W_D, W_V, W_R32, W_W32, W_D32, W_V32, W_X32, W_B32, W_E32, W_VB, W_ZB, W_ZB32, W_T, W_T32,
-       W_PB, W_PX, W_SR64, W_SR32 = W_SR64 + 5,  // W_SR*: five vectors each
-       W_RD64 = W_SR32 + 5, W_RD32 = W_RD64 + 5,   // W_RD*: five vectors each (reduced CG)
-       W_RIN64 = W_RD32 + 5, W_E64 };              // up to slot 39
+       W_PB, W_PX, W_SR64, W_SR32 = W_SR64 + 5,  // W_SR*: five vectors each (up to slot 27)
+       W_RD64 = kWsFree, W_RD32 = W_RD64 + 5,      // W_RD*: five vectors each (reduced CG)
+       W_RIN64 = W_RD32 + 5, W_E64 };              // up to slot 43
+static_assert(W_SR32 + 5 <= kWsGmresState && W_E64 < 48, "workspace slots overlap");
 
 template <class T>
 constexpr int dt() {

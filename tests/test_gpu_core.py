@@ -415,3 +415,41 @@ def test_block_schedule_solves(pp, torch, kind, args):
     s.solve(M, dev(torch, b), x)
     assert s.getConvergence() and s.report.true_relres < 1e-8
 
+
+
+def block_diag(*mats):
+    rps, cis, vs, off, nnz = [np.zeros(1, np.uint32)], [], [], 0, 0
+    for rp, ci, v in mats:
+        rps.append(rp[1:].astype(np.uint32) + nnz)
+        cis.append(ci.astype(np.uint32) + off)
+        vs.append(v)
+        off += len(rp) - 1
+        nnz += int(rp[-1])
+    return np.concatenate(rps), np.concatenate(cis), np.concatenate(vs)
+
+
+@pytest.mark.parametrize("kind,args", [("ict", (1e-3,)), ("ildlt", (1e-3,)), ("ilut", (1e-3, 10))])
+def test_block_schedule_chunks(pp, torch, kind, args):
+    # independent diagonal blocks of 9000, 9000 and 5000 rows: the chunked
+    # schedule runs them in separate workgroups (chunks of at most the LDS
+    # size); applies bit-identical to the level schedule
+    mk = ppo.laplacian if kind != "ilut" else (lambda a, b, c: ppo.upwind(a, b, c, 20.0, 10.0, 5.0))
+    A = block_diag(mk(90, 100, 1), mk(100, 90, 1), mk(50, 100, 1))
+    n = len(A[0]) - 1
+    print(f"[chunks] {kind} n={n}", flush=True)
+    P1 = make_prec(pp, A, kind, args, 1)
+    P3 = make_prec(pp, A, kind, args, 3)
+    assert P3.info().trsv == 3
+    x = ppo.uniform(n, 8)
+    t1, t3 = dev(torch, x), dev(torch, x)
+    P1.MInvSolve(t1)
+    P3.MInvSolve(t3)
+    assert np.array_equal(host(t1), host(t3))
+    f1, f3 = dev(torch, x, torch.float32), dev(torch, x, torch.float32)
+    P1.MInvSolve(f1)
+    P3.MInvSolve(f3)
+    assert np.array_equal(host(f1), host(f3))
+    # one connected block larger than the LDS holds: PP_TRSV_BLOCK falls back
+    B = ppo.laplacian(160, 160, 1)
+    PB = make_prec(pp, B, kind if kind != "ilut" else "ict", (1e-3,), 3)
+    assert PB.info().trsv == 1

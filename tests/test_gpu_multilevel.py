@@ -289,3 +289,29 @@ def test_multilevel_fp32_apply_and_mixed_solve(pp, torch, kind):
     s.solve(M, torch.tensor(b, device="cuda"), xs)
     print(f"[{kind}] mixed: {s.getNumIter()} iterations, relres {s.report.true_relres:.2e}", flush=True)
     assert s.getConvergence() and s.report.true_relres < 1e-8
+
+
+def test_setup_after_a_converged_solve(pp, torch):
+    # a converged solve leaves its stop word set on the context; a multilevel
+    # setup that runs afterwards (Lanczos through block sweeps) must not see it
+    A = ppo.laplacian(20, 20, 20)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    x = ppo.uniform(n, 4)
+
+    def build_apply():
+        P = pp.AMSED(M, levels=3, ndom=8, rank=16)
+        P.setup()
+        t = torch.tensor(x, device="cuda")
+        P.MInvSolve(t)
+        return P.info_levels()[1], t.cpu().numpy()
+
+    r1, v1 = build_apply()
+    P0 = pp.IC0(M)
+    cg = pp.CG(n, P0, pp.FlexibleCG, tol=1e-8, maxIter=1000)
+    xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cg.solve(M, torch.tensor(ppo.spmv(A, x), device="cuda"), xs)
+    assert cg.getConvergence()
+    r2, v2 = build_apply()
+    assert list(r1) == list(r2)
+    assert np.array_equal(v1, v2)

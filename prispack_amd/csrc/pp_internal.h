@@ -202,16 +202,22 @@ struct ChunkSet {
     uint32_t nchunk = 0, max_rows = 0;
     void free_all();
 };
+// Rows are stored by position (chunk-major, level order inside a chunk), the
+// entries of consecutive positions contiguous, so the entries of a level are
+// one contiguous range: the workgroup loads them coalesced, forms the
+// products in LDS, and each row's thread subtracts its products in storage
+// order.  Row and column numbers are local to the chunk (16 bits).
 template <class T>
 struct DevChunkSweep {
-    uint32_t n = 0, nnz = 0, max_levels = 0;
-    uint32_t *perm = nullptr;      // [n] position -> original row (chunk-major, level order inside)
-    uint32_t *lp = nullptr;        // level boundaries (positions); chunk c: lp[lev_off[c] .. lev_off[c+1]-1]
-    uint32_t *lev_off = nullptr;   // [nchunk+1]
-    uint32_t *rp = nullptr;        // [n+1] by position
+    uint32_t n = 0, nnz = 0, max_levels = 0, max_level_nnz = 0;
+    uint16_t *row = nullptr;       // [n] position -> row minus the chunk's first row
+    uint32_t *rp = nullptr;        // [n+1] entries by position
     uint16_t *ci = nullptr;        // [nnz] dependency row minus the chunk's first row
     T *val = nullptr;              // [nnz]
-    T *diag = nullptr;             // [n] original numbering, or nullptr
+    T *pdiag = nullptr;            // [n] diagonal by position, or nullptr (unit sweep)
+    uint32_t *lp = nullptr;        // level boundaries (positions); chunk c: lp[lev_off[c] .. lev_off[c+1]-1]
+    uint32_t *le = nullptr;        // rp at those boundaries (first entry of each level)
+    uint32_t *lev_off = nullptr;   // [nchunk+1]
     void free_all();
 };
 

@@ -691,60 +691,172 @@ template void launch_sweep_pair<float>(pp_ctx *, const DevSweep<float> &, int, c
 // chunk's part of the vector lives in LDS; level after level the threads take
 // the level's rows and a workgroup barrier separates levels.  Per row: the
 // operations of k_sweep_level in the same order (bit-identical results).
+// A thread's share of the next level, loaded while the current level
+// computes: its first entry of the level (for the products) and its first
+// row (for the sums).  Loaded into registers that the caller alternates, so
+// the barrier between levels never waits for these loads.
 template <class T>
-__device__ inline void chunk_sweep(const uint32_t *__restrict__ lp, uint32_t nlev, const uint32_t *__restrict__ perm,
-                                   const uint32_t *__restrict__ rp, const uint16_t *__restrict__ ci,
-                                   const T *__restrict__ val, const T *__restrict__ diag, int mode, uint32_t r0,
-                                   const T *b, T *xs) {
-#pragma clang fp contract(off)
-    for (uint32_t l = 0; l < nlev; ++l) {
-        const uint32_t k1 = lp[l + 1];
-        for (uint32_t k = lp[l] + threadIdx.x; k < k1; k += blockDim.x) {
-            const uint32_t i = perm[k];
-            T s = b ? b[i] : xs[i - r0];
-            if (mode == 3) s = s / diag[i];
-            const uint32_t e1 = rp[k + 1];
-            for (uint32_t e = rp[k]; e < e1; ++e) {
-                T p = val[e] * xs[ci[e]];
-                s = s - p;
-            }
-            if (mode == 0) s = s / diag[i];
-            else if (mode == 2) s = s * diag[i];
-            xs[i - r0] = s;
-        }
-        __syncthreads();
-    }
+struct ChunkAhead {
+    uint32_t c, i, e0, e1;
+    T v, d;
+    bool eok, rok;
+};
+
+template <class T>
+__device__ inline void chunk_ahead(ChunkAhead<T> &q, uint32_t L, uint32_t Lend, uint32_t E, uint32_t Eend,
+                                   const DevChunkSweep<T> &S) {
+    const uint32_t e = E + threadIdx.x, k = L + threadIdx.x;
+    q.eok = e < Eend;
+    q.rok = k < Lend;
+    // unconditional loads (index 0 stands in when out of range; every array
+    // holds at least one element): without branches around them the
+    // compiler can wait for exactly these loads where they are used
+    const uint32_t es = q.eok ? e : 0, ks = q.rok ? k : 0;
+    q.c = S.ci[es];
+    q.v = S.val[es];
+    q.i = S.row[ks];
+    q.e0 = S.rp[ks];
+    q.e1 = S.rp[ks + 1];
+    q.d = S.pdiag ? S.pdiag[ks] : T(1);
 }
 
 template <class T>
-__global__ __launch_bounds__(1024) void k_sweep_chunks(
-    const uint32_t *__restrict__ start, const uint32_t *lo_lp, const uint32_t *lo_off, const uint32_t *lo_perm,
-    const uint32_t *lo_rp, const uint16_t *lo_ci, const T *lo_val, const T *lo_diag, int lo_mode,
-    const uint32_t *up_lp, const uint32_t *up_off, const uint32_t *up_perm, const uint32_t *up_rp,
-    const uint16_t *up_ci, const T *up_val, const T *up_diag, int up_mode, const T *x, T *y, const int *stop) {
+__device__ inline void chunk_row_finish(T s, uint32_t i, T d, int mode, T *xs) {
+#pragma clang fp contract(off)
+    if (mode == 0) s = s / d;
+    else if (mode == 2) s = s * d;
+    xs[i] = s;
+}
+
+// One level of a chunked sweep (entries [E0, E1) of rows [L0, L1)), then the
+// level boundaries advance.  Staged (the level's entries fit the product
+// buffer): every thread forms products of the level's entries into LDS,
+// barrier, then each row's thread subtracts its products in storage order.
+// Otherwise each row's thread forms and subtracts its own products.  Per
+// row the operations of k_sweep_level in the same order (bit-identical).
+template <class T>
+__device__ inline void chunk_level(ChunkAhead<T> &cur, ChunkAhead<T> &nxt, const uint32_t *__restrict__ lp,
+                                   const uint32_t *__restrict__ le, uint32_t nlev, uint32_t l, uint32_t &L0,
+                                   uint32_t &L1, uint32_t &L2, uint32_t &E0, uint32_t &E1, uint32_t &E2,
+                                   const DevChunkSweep<T> &S, int mode, T *xs, T *prod, uint32_t cap) {
+#pragma clang fp contract(off)
+    const uint32_t L3 = l + 3 <= nlev ? lp[l + 3] : L2, E3 = l + 3 <= nlev ? le[l + 3] : E2;
+    nxt.eok = nxt.rok = false;
+    if (l + 1 < nlev) chunk_ahead<T>(nxt, L1, L2, E1, E2, S);
+    if (E1 - E0 <= cap) {
+        if (cur.eok) {
+            T p = cur.v * xs[cur.c];
+            prod[threadIdx.x] = p;
+        }
+        for (uint32_t e = E0 + threadIdx.x + blockDim.x; e < E1; e += blockDim.x) {
+            T p = S.val[e] * xs[S.ci[e]];
+            prod[e - E0] = p;
+        }
+        __syncthreads();
+        if (cur.rok) {
+            T s = xs[cur.i];
+            if (mode == 3) s = s / cur.d;
+            for (uint32_t e = cur.e0; e < cur.e1; ++e) s = s - prod[e - E0];
+            chunk_row_finish(s, cur.i, cur.d, mode, xs);
+        }
+        for (uint32_t k = L0 + threadIdx.x + blockDim.x; k < L1; k += blockDim.x) {
+            const uint32_t i = S.row[k];
+            const T d = S.pdiag ? S.pdiag[k] : T(1);
+            T s = xs[i];
+            if (mode == 3) s = s / d;
+            for (uint32_t e = S.rp[k], e1 = S.rp[k + 1]; e < e1; ++e) s = s - prod[e - E0];
+            chunk_row_finish(s, i, d, mode, xs);
+        }
+    } else {
+        for (uint32_t k = L0 + threadIdx.x; k < L1; k += blockDim.x) {
+            const uint32_t i = S.row[k];
+            const T d = S.pdiag ? S.pdiag[k] : T(1);
+            T s = xs[i];
+            if (mode == 3) s = s / d;
+            for (uint32_t e = S.rp[k], e1 = S.rp[k + 1]; e < e1; ++e) {
+                T p = S.val[e] * xs[S.ci[e]];
+                s = s - p;
+            }
+            chunk_row_finish(s, i, d, mode, xs);
+        }
+    }
+    __syncthreads();
+    L0 = L1;
+    L1 = L2;
+    L2 = L3;
+    E0 = E1;
+    E1 = E2;
+    E2 = E3;
+}
+
+// Levels of one sweep over a chunk, in place on xs (the chunk's right-hand
+// side on entry).
+template <class T>
+__device__ inline void chunk_sweep(const uint32_t *__restrict__ lp, const uint32_t *__restrict__ le, uint32_t nlev,
+                                   const DevChunkSweep<T> &S, int mode, T *xs, T *prod, uint32_t cap) {
+    if (nlev == 0) return;
+    uint32_t L0 = lp[0], L1 = lp[1], L2 = nlev > 1 ? lp[2] : L1;
+    uint32_t E0 = le[0], E1 = le[1], E2 = nlev > 1 ? le[2] : E1;
+    ChunkAhead<T> ra, rb;
+    chunk_ahead<T>(ra, L0, L1, E0, E1, S);
+    for (uint32_t l = 0; l < nlev; l += 2) {
+        chunk_level<T>(ra, rb, lp, le, nlev, l, L0, L1, L2, E0, E1, E2, S, mode, xs, prod, cap);
+        if (l + 1 < nlev)
+            chunk_level<T>(rb, ra, lp, le, nlev, l + 1, L0, L1, L2, E0, E1, E2, S, mode, xs, prod, cap);
+    }
+}
+
+// Both sweeps of an apply, one workgroup per chunk (PP_TRSV_BLOCK); LDS holds
+// the chunk's part of the vector (max_rows entries) and the product buffer
+// (cap entries).
+template <class T>
+__global__ __launch_bounds__(1024) void k_sweep_chunks(const uint32_t *__restrict__ start, DevChunkSweep<T> lo,
+                                                       int lo_mode, DevChunkSweep<T> up, int up_mode, const T *x,
+                                                       T *y, uint32_t max_rows, uint32_t lvl_words, uint32_t cap,
+                                                       const int *stop) {
     extern __shared__ unsigned char lds_raw[];
     T *xs = reinterpret_cast<T *>(lds_raw);
+    // the chunk's level boundaries of both sweeps (lp, le) staged in LDS, so
+    // a level reads them without waiting on global loads; then the products
+    uint32_t *lvl = reinterpret_cast<uint32_t *>(xs + max_rows);
+    T *prod = reinterpret_cast<T *>(lvl + lvl_words);
     if (stop && *stop) return;
     const uint32_t c = blockIdx.x, r0 = start[c], r1 = start[c + 1];
-    const uint32_t lo0 = lo_off[c], lo1 = lo_off[c + 1], up0 = up_off[c], up1 = up_off[c + 1];
-    chunk_sweep<T>(lo_lp + lo0, lo1 - lo0 - 1, lo_perm, lo_rp, lo_ci, lo_val, lo_diag, lo_mode, r0, x, xs);
-    chunk_sweep<T>(up_lp + up0, up1 - up0 - 1, up_perm, up_rp, up_ci, up_val, up_diag, up_mode, r0, nullptr, xs);
+    const uint32_t lo0 = lo.lev_off[c], lo1 = lo.lev_off[c + 1], up0 = up.lev_off[c], up1 = up.lev_off[c + 1];
+    const uint32_t nlo = lo1 - lo0, nup = up1 - up0;  // boundaries of each sweep (levels + 1)
+    uint32_t *llp = lvl, *lle = lvl + nlo, *ulp = lvl + 2 * nlo, *ule = lvl + 2 * nlo + nup;
+    for (uint32_t j = threadIdx.x; j < r1 - r0; j += blockDim.x) xs[j] = x[r0 + j];
+    for (uint32_t j = threadIdx.x; j < nlo; j += blockDim.x) {
+        llp[j] = lo.lp[lo0 + j];
+        lle[j] = lo.le[lo0 + j];
+    }
+    for (uint32_t j = threadIdx.x; j < nup; j += blockDim.x) {
+        ulp[j] = up.lp[up0 + j];
+        ule[j] = up.le[up0 + j];
+    }
+    __syncthreads();
+    chunk_sweep<T>(llp, lle, nlo - 1, lo, lo_mode, xs, prod, cap);
+    chunk_sweep<T>(ulp, ule, nup - 1, up, up_mode, xs, prod, cap);
     for (uint32_t j = threadIdx.x; j < r1 - r0; j += blockDim.x) y[r0 + j] = xs[j];
 }
 
 template <class T>
 void launch_sweep_pair_chunks(pp_ctx *ctx, const ChunkSet &C, const DevChunkSweep<T> &lo, int mlo,
                               const DevChunkSweep<T> &up, int mup, const T *x, T *y) {
-    const size_t lds = (size_t)C.max_rows * sizeof(T);
-    PP_CHECK(C.nchunk && lds <= ctx->max_lds && lo.perm && up.perm, PP_ERR_INVALID,
+    const uint32_t lvl_words = 2 * (lo.max_levels + 1) + 2 * (up.max_levels + 1);
+    const size_t xbytes = (size_t)C.max_rows * sizeof(T) + (size_t)lvl_words * 4;
+    PP_CHECK(C.nchunk && xbytes <= ctx->max_lds && lo.row && up.row, PP_ERR_INVALID,
              "chunked sweep: storage missing or chunk exceeds LDS");
+    // the product buffer takes what LDS has left, up to the widest level
+    const uint32_t want = std::max(lo.max_level_nnz, up.max_level_nnz);
+    const uint32_t cap = (uint32_t)std::min<size_t>(want, (ctx->max_lds - xbytes) / sizeof(T));
+    const size_t lds = xbytes + (size_t)cap * sizeof(T);
     if (lds > 65536)
         PP_HIP(hipFuncSetAttribute((const void *)k_sweep_chunks<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
     const unsigned threads = C.max_rows >= 1024 ? 1024u : std::max(64u, (C.max_rows + 63) / 64 * 64);
-    k_sweep_chunks<T><<<C.nchunk, threads, lds, ctx->stream>>>(
-        C.dstart, lo.lp, lo.lev_off, lo.perm, lo.rp, lo.ci, lo.val, lo.diag, mlo, up.lp, up.lev_off, up.perm, up.rp,
-        up.ci, up.val, up.diag, mup, x, y, ctx->stop_word);
+    k_sweep_chunks<T><<<C.nchunk, threads, lds, ctx->stream>>>(C.dstart, lo, mlo, up, mup, x, y, C.max_rows,
+                                                              lvl_words, cap, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
 template void launch_sweep_pair_chunks<double>(pp_ctx *, const ChunkSet &, const DevChunkSweep<double> &, int,

@@ -170,17 +170,18 @@ void ChunkSet::free_all() {
 
 template <class T>
 void DevChunkSweep<T>::free_all() {
-    dfree(perm);
-    dfree(lp);
-    dfree(lev_off);
+    dfree(row);
     dfree(rp);
     dfree(ci);
     dfree(val);
-    dfree(diag);
-    perm = lp = lev_off = rp = nullptr;
-    ci = nullptr;
-    val = diag = nullptr;
-    n = nnz = max_levels = 0;
+    dfree(pdiag);
+    dfree(lp);
+    dfree(le);
+    dfree(lev_off);
+    row = ci = nullptr;
+    rp = lp = le = lev_off = nullptr;
+    val = pdiag = nullptr;
+    n = nnz = max_levels = max_level_nnz = 0;
 }
 template struct DevChunkSweep<double>;
 template struct DevChunkSweep<float>;
@@ -220,7 +221,7 @@ static bool build_chunks(const std::vector<const HostSweep *> &sw, uint32_t n, u
     return true;
 }
 
-// The sweep in chunk-major, level-ordered positions.
+// The sweep in chunk-major, level-ordered positions (DevChunkSweep layout).
 template <class T>
 static void upload_chunk_sweep(const HostSweep &s, const ChunkSet &C, DevChunkSweep<T> &d) {
     d.free_all();
@@ -232,51 +233,52 @@ static void upload_chunk_sweep(const HostSweep &s, const ChunkSet &C, DevChunkSw
         for (uint32_t k = glp[l]; k < glp[l + 1]; ++k) lev[gperm[k]] = l;
     for (uint32_t c = 0; c < C.nchunk; ++c)
         for (uint32_t i = C.start[c]; i < C.start[c + 1]; ++i) chunk_of[i] = c;
-    // positions: chunk-major, then global level order (stable)
     std::vector<std::vector<uint32_t>> rows(C.nchunk);
     for (uint32_t k = 0; k < n; ++k) rows[chunk_of[gperm[k]]].push_back(gperm[k]);
-    std::vector<uint32_t> perm, lp, off, rp(1, 0);
-    std::vector<uint16_t> ci;
-    std::vector<T> val;
-    perm.reserve(n);
-    off.push_back(0);
+    std::vector<uint16_t> row, ci;
+    std::vector<T> val, pdiag;
+    std::vector<uint32_t> lp, off(1, 0), rp(1, 0);
+    row.reserve(n);
     for (uint32_t c = 0; c < C.nchunk; ++c) {
         const uint32_t r0 = C.start[c];
         uint32_t nl = 0;
         for (size_t q = 0; q < rows[c].size(); ++q) {
             const uint32_t i = rows[c][q];
             if (q == 0 || lev[i] != lev[rows[c][q - 1]]) {
-                lp.push_back((uint32_t)perm.size());
+                if (!lp.empty() && nl) d.max_level_nnz = std::max(d.max_level_nnz, rp.back() - rp[lp.back()]);
+                lp.push_back((uint32_t)row.size());
                 ++nl;
             }
-            perm.push_back(i);
-            for (uint32_t k = s.rp[i]; k < s.rp[i + 1]; ++k) {
-                ci.push_back((uint16_t)(s.ci[k] - r0));
-                val.push_back((T)s.val[k]);
+            row.push_back((uint16_t)(i - r0));
+            if (s.mode != HostSweep::UNIT) pdiag.push_back((T)s.diag[i]);
+            for (uint32_t e = s.rp[i]; e < s.rp[i + 1]; ++e) {
+                ci.push_back((uint16_t)(s.ci[e] - r0));
+                val.push_back((T)s.val[e]);
             }
             rp.push_back((uint32_t)ci.size());
         }
-        lp.push_back((uint32_t)perm.size());
+        if (nl) d.max_level_nnz = std::max(d.max_level_nnz, rp.back() - rp[lp.back()]);
+        lp.push_back((uint32_t)row.size());
         off.push_back((uint32_t)lp.size());
         d.max_levels = std::max(d.max_levels, nl);
     }
     d.n = n;
     d.nnz = (uint32_t)ci.size();
     auto up = [](const auto &v, auto *&dst) {
-        using E = std::remove_reference_t<decltype(v[0])>;
-        dst = dalloc<std::remove_const_t<E>>(v.size());
+        using E = std::remove_const_t<std::remove_reference_t<decltype(v[0])>>;
+        dst = dalloc<E>(std::max<size_t>(1, v.size()));
         if (!v.empty()) PP_HIP(hipMemcpy(dst, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice));
     };
-    up(perm, d.perm);
-    up(lp, d.lp);
-    up(off, d.lev_off);
+    up(row, d.row);
     up(rp, d.rp);
     up(ci, d.ci);
     up(val, d.val);
-    if (s.mode != HostSweep::UNIT) {
-        std::vector<T> dg(s.diag.begin(), s.diag.end());
-        up(dg, d.diag);
-    }
+    std::vector<uint32_t> le(lp.size());
+    for (size_t j = 0; j < lp.size(); ++j) le[j] = rp[lp[j]];
+    up(lp, d.lp);
+    up(le, d.le);
+    up(off, d.lev_off);
+    if (s.mode != HostSweep::UNIT) up(pdiag, d.pdiag);
 }
 
 // PP_TRSV_BLOCK storage for M (FP64; FP32 too when the FP32 sweeps exist).
@@ -297,6 +299,15 @@ static bool build_block_sweeps(pp_ctx *ctx, pp_precond *M) {
                      hipMemcpyHostToDevice));
     upload_chunk_sweep(M->hlow, M->chunks, M->clow64);
     upload_chunk_sweep(M->hup, M->chunks, M->cup64);
+    // the chunk's vector and both sweeps' level boundaries must fit in LDS
+    const size_t need = (size_t)M->chunks.max_rows * sizeof(double) +
+                        4 * (size_t)(2 * (M->clow64.max_levels + 1) + 2 * (M->cup64.max_levels + 1));
+    if (need > ctx->max_lds) {
+        M->chunks.free_all();
+        M->clow64.free_all();
+        M->cup64.free_all();
+        return false;
+    }
     if (M->have32) {
         upload_chunk_sweep(M->hlow, M->chunks, M->clow32);
         upload_chunk_sweep(M->hup, M->chunks, M->cup32);

@@ -324,13 +324,13 @@ def main():
     # ---- kernel throughput on the library's stream
     stream = L.pp_ctx_stream(ctx.h)
     spmv_var = C.c_int()
-    ctx.check(L.pp_csr_spmv_variant(M.h, C.byref(spmv_var)))
+    ctx.check(L.pp_csr_spmv_variant(M.h, 0, C.byref(spmv_var)))
     # the FP64 SpMV alternates between two identical copies of the matrix and
     # of x, y (2 x 188 MB, more than the 256 MB last-level cache): each launch
     # streams its operands from HBM instead of finding them cached by the
     # launch before
     M2 = pp.CSRMatrix(*A, ctx=ctx)
-    ctx.check(L.pp_csr_set_spmv_variant(M2.h, spmv_var.value))
+    ctx.check(L.pp_csr_set_spmv_variant(M2.h, 0, spmv_var.value))
     y = torch.zeros(n, dtype=torch.float64, device="cuda")
     x2, y2 = d_xt.clone(), torch.zeros_like(y)
     px, py = C.c_void_p(d_xt.data_ptr()), C.c_void_p(y.data_ptr())

@@ -82,13 +82,15 @@ int pp_csr_dims(const pp_csr *A, uint32_t *nrow, uint32_t *ncol, uint32_t *nnz);
  * The algorithmic traffic the SpMV roofline is priced on.  No reference
  * counterpart. */
 int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes);
-/* SpMV load variant chosen by timing when the matrix was created (large
- * matrices): bit 0 non-temporal operand loads, bit 1 16-bit column offsets. */
-int pp_csr_spmv_variant(const pp_csr *A, int *variant);
-/* Pins the SpMV load variant instead of the timed choice (reproducible
- * profiling, identical copies of one matrix).  PP_ERR_INVALID for a value
- * outside 0..3 or for 16-bit offsets on a matrix that has none. */
-int pp_csr_set_spmv_variant(pp_csr *A, int variant);
+/* SpMV load variant of one precision (dtype PP_F64 / PP_F32), chosen by
+ * timing on large matrices (FP64 when the matrix is created, FP32 when its
+ * FP32 copy is made): bit 0 non-temporal operand loads and output stores,
+ * bit 1 16-bit column offsets, bit 2 batched loads. */
+int pp_csr_spmv_variant(const pp_csr *A, int dtype, int *variant);
+/* Pins the SpMV load variant of one precision instead of the timed choice
+ * (reproducible profiling, identical copies of one matrix).  PP_ERR_INVALID
+ * for a value outside 0..7 or for 16-bit offsets on a matrix that has none. */
+int pp_csr_set_spmv_variant(pp_csr *A, int dtype, int variant);
 /* Device pointers of the stored arrays (values64 FP64; values32 NULL until an
  * FP32 copy exists). */
 int pp_csr_device_ptrs(const pp_csr *A, const uint32_t **rowptr, const uint32_t **colidx,

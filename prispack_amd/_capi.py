@@ -121,8 +121,8 @@ SIGS = {
     "pp_batch_create": (C.c_int, [vp, u32, u32p, u32p, u32p, f64p, C.POINTER(vp)]),
     "pp_batch_set_values": (C.c_int, [vp, vp, vp, C.c_int]),
     "pp_csr_spmv_bytes": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint64)]),
-    "pp_csr_spmv_variant": (C.c_int, [vp, C.POINTER(C.c_int)]),
-    "pp_csr_set_spmv_variant": (C.c_int, [vp, C.c_int]),
+    "pp_csr_spmv_variant": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int)]),
+    "pp_csr_set_spmv_variant": (C.c_int, [vp, C.c_int, C.c_int]),
     "pp_precond_apply_bytes": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_uint64)]),
     "pp_batch_set_ordering": (C.c_int, [vp, vp, C.c_int]),
     "pp_batch_solve": (C.c_int, [vp, vp, C.POINTER(SolveOpts), vp, vp, i32p, C.POINTER(SolveReport)]),

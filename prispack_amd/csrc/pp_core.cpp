@@ -112,11 +112,16 @@ static void csr_upload_values_f32(pp_ctx *ctx, pp_csr *A) {
     A->f32_dirty = false;
 }
 
+static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A, int dtype);
+
 void csr_need_f32(pp_ctx *ctx, pp_csr *A) {
     if (A->f32_dirty || !A->v32) csr_upload_values_f32(ctx, A);
+    if (!A->tuned32) {  // the FP32 loads are timed on their own (their best variant differs)
+        A->tuned32 = true;
+        csr_tune_spmv(ctx, A, PP_F32);
+    }
 }
 
-static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A);
 
 uint64_t next_uid() {
     static std::atomic<uint64_t> counter{1};
@@ -217,30 +222,31 @@ pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, c
                               ctx->stream));
         PP_HIP(hipStreamSynchronize(ctx->stream));  // ds leaves scope
     }
-    csr_tune_spmv(ctx, A.get());
+    csr_tune_spmv(ctx, A.get(), PP_F64);
     return A.release();
 }
 
 // Times each SpMV load variant (ordinary / non-temporal operand loads, 32-bit
 // / 16-bit column offsets) on this matrix and keeps the fastest.  Only for
 // matrices large enough for the choice to matter.
-static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A) {
+static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A, int dtype) {
     if (A->nnz < (1u << 20)) return;
-    double *x = dalloc<double>(A->ncol), *y = dalloc<double>(A->nrow);
+    int &variant = dtype == PP_F64 ? A->spmv_variant : A->spmv_variant32;
+    double *x = dalloc<double>(A->ncol), *y = dalloc<double>(A->nrow);  // large enough for either type
     try {
         launch_fill_zero(ctx, x, (size_t)A->ncol * sizeof(double));
         float best = 1e30f;
-        int best_var = A->spmv_variant;
+        int best_var = variant;
         for (int var = 0; var < kVarCount; ++var) {
             if ((var & kVarIdx16) && !A->ci16) continue;
-            A->spmv_variant = var;
-            const float ms = time_cold_ms(ctx, [&] { launch_spmv(ctx, A, PP_F64, 1.0, x, 0.0, y, -1, nullptr, false); }, 3);
+            variant = var;
+            const float ms = time_cold_ms(ctx, [&] { launch_spmv(ctx, A, dtype, 1.0, x, 0.0, y, -1, nullptr, false); }, 3);
             if (ms < best) {
                 best = ms;
                 best_var = var;
             }
         }
-        A->spmv_variant = best_var;
+        variant = best_var;
     } catch (...) {
         dfree(x);
         dfree(y);
@@ -439,16 +445,21 @@ int pp_csr_set_values(pp_ctx *ctx, pp_csr *A, const double *values, int on_devic
     });
 }
 
-int pp_csr_spmv_variant(const pp_csr *A, int *variant) {
-    if (!A || !variant) return PP_ERR_INVALID;
-    *variant = A->spmv_variant;
+int pp_csr_spmv_variant(const pp_csr *A, int dtype, int *variant) {
+    if (!A || !variant || (dtype != PP_F64 && dtype != PP_F32)) return PP_ERR_INVALID;
+    *variant = dtype == PP_F64 ? A->spmv_variant : A->spmv_variant32;
     return PP_OK;
 }
 
-int pp_csr_set_spmv_variant(pp_csr *A, int variant) {
-    if (!A || variant < 0 || variant >= kVarCount) return PP_ERR_INVALID;
+int pp_csr_set_spmv_variant(pp_csr *A, int dtype, int variant) {
+    if (!A || variant < 0 || variant >= kVarCount || (dtype != PP_F64 && dtype != PP_F32)) return PP_ERR_INVALID;
     if ((variant & kVarIdx16) && !A->ci16) return PP_ERR_INVALID;
-    A->spmv_variant = variant;
+    if (dtype == PP_F64) {
+        A->spmv_variant = variant;
+    } else {
+        A->spmv_variant32 = variant;
+        A->tuned32 = true;  // pinned: the FP32 copy's creation keeps it
+    }
     return PP_OK;
 }
 
@@ -456,7 +467,7 @@ int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes) {
     if (!A || !bytes) return PP_ERR_INVALID;
     const uint64_t esz = dtype == PP_F64 ? 8 : 4;
     uint64_t b = esz * A->nnz;                                    // values
-    const uint64_t n16 = (A->spmv_variant & kVarIdx16) ? A->nnz16 : 0;
+    const uint64_t n16 = ((dtype == PP_F64 ? A->spmv_variant : A->spmv_variant32) & kVarIdx16) ? A->nnz16 : 0;
     b += 2 * n16 + 4 * ((uint64_t)A->nnz - n16);                 // column indices
     b += 4ull * (A->nrow + 1) + 4ull * (A->nblocks + 1);          // row pointers, row blocks
     if (n16) b += 8ull * A->nblocks;                              // block bases

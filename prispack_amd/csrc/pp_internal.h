@@ -248,7 +248,9 @@ struct pp_csr {
     uint16_t *ci16 = nullptr;
     uint32_t *blk_base = nullptr;
     uint32_t nblocks16 = 0;          // blocks that use ci16
-    int spmv_variant = pp::kVarDefault;  // chosen by timing at creation for large matrices
+    int spmv_variant = pp::kVarDefault;    // FP64 loads, chosen by timing at creation for large matrices
+    int spmv_variant32 = pp::kVarDefault;  // FP32 loads, chosen by timing when the FP32 copy is made
+    bool tuned32 = false;
     uint64_t nnz16 = 0;              // entries in those blocks
     pp_csr *trans = nullptr;         // explicit transpose, built on demand
     std::vector<uint32_t> h_rp, h_ci;

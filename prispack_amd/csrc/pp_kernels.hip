@@ -230,7 +230,8 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
                 for (; a < e; ++a) s += prod[a];
                 T out = alpha * s;
                 if (!beta_zero) out += beta * yold;
-                y[row] = out;
+                if constexpr (NT) __builtin_nontemporal_store(out, y + row);  // streamed out, not re-read here
+                else y[row] = out;
                 if (DOT) {
                     const double u = (double)dw;
                     dacc += u * (double)out;
@@ -361,7 +362,7 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
     double *part = dot_slot >= 0 ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
     const int bz = beta == 0.0;
     const bool dot = dot_slot >= 0;
-    const int var = A->spmv_variant;
+    const int var = dtype == PP_F64 ? A->spmv_variant : A->spmv_variant32;
     const uint16_t *c16 = (var & kVarIdx16) ? A->ci16 : nullptr;
 #define PP_SPMV(T, DOTV, NTV, BV, VALS, DW, PART)                                                                 \
     k_spmv<T, DOTV, NTV, BV><<<grid, kBlock, 0, ctx->stream>>>(A->blk_desc, A->rp, A->ci, c16, VALS, (const T *)x, \

@@ -461,6 +461,36 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
     }
 }
 
+// Times each staged-sweep load variant on one precision's sweeps (cold
+// cache) and keeps the fastest for that precision.
+template <class T>
+int tune_sweep_pair(pp_ctx *ctx, pp_precond *M, DevSweep<T> &lo, DevSweep<T> &up, int start) {
+    T *y = dalloc<T>(M->n);
+    int best_var = start;
+    try {
+        launch_fill_zero(ctx, y, (size_t)M->n * sizeof(T));
+        float best = 1e30f;
+        const bool have16 = lo.ci16 || up.ci16;
+        for (int var = 0; var < kVarCount; ++var) {
+            if ((var & kVarIdx16) && !have16) continue;
+            lo.variant = up.variant = var;
+            const float ms = time_cold_ms(ctx, [&] {
+                launch_sweep_pair<T>(ctx, lo, M->hlow.mode, up, M->hup.mode, M->fuse_mid, y, y);
+            }, 3);
+            if (ms < best) {
+                best = ms;
+                best_var = var;
+            }
+        }
+        lo.variant = up.variant = best_var;
+    } catch (...) {
+        dfree(y);
+        throw;
+    }
+    dfree(y);
+    return best_var;
+}
+
 void precond_need_f32(pp_precond *M) {
     if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) {
         ml_need_f32(M->ctx, M);
@@ -473,6 +503,8 @@ void precond_need_f32(pp_precond *M) {
     upload_sweep(M->hlow, M->low32);
     upload_sweep(M->hup, M->up32);
     M->low32.variant = M->up32.variant = M->sweep_variant;
+    if (M->trsv == PP_TRSV_LEVELS && M->n >= (1u << 20) && (M->low32.blocks || M->up32.blocks))
+        tune_sweep_pair<float>(M->ctx, M, M->low32, M->up32, M->sweep_variant);
     if (M->chunks.nchunk) {
         upload_chunk_sweep(M->hlow, M->chunks, M->clow32);
         upload_chunk_sweep(M->hup, M->chunks, M->cup32);
@@ -644,30 +676,7 @@ int pp_precond_create(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *opts, pp_pr
 // the staged kernel and the factor is large.
 static void tune_sweeps(pp_ctx *ctx, pp_precond *M) {
     if (M->n < (1u << 20) || (!M->low64.blocks && !M->up64.blocks)) return;
-    double *y = dalloc<double>(M->n);
-    try {
-        launch_fill_zero(ctx, y, (size_t)M->n * sizeof(double));
-        float best = 1e30f;
-        int best_var = M->sweep_variant;
-        const bool have16 = M->low64.ci16 || M->up64.ci16;
-        for (int var = 0; var < kVarCount; ++var) {
-            if ((var & kVarIdx16) && !have16) continue;
-            M->low64.variant = M->up64.variant = var;
-            const float ms = time_cold_ms(ctx, [&] {
-                launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, y, y);
-            }, 3);
-            if (ms < best) {
-                best = ms;
-                best_var = var;
-            }
-        }
-        M->sweep_variant = best_var;
-        M->low64.variant = M->up64.variant = M->low32.variant = M->up32.variant = best_var;
-    } catch (...) {
-        dfree(y);
-        throw;
-    }
-    dfree(y);
+    M->sweep_variant = tune_sweep_pair<double>(ctx, M, M->low64, M->up64, M->sweep_variant);
 }
 
 // PP_TRSV_AUTO: the chunked one-workgroup schedule when the rows chunk

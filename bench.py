@@ -270,7 +270,9 @@ def main():
     P.setup()
     setup_s = time.perf_counter() - t0
     info = P.info()
-    log(f"[bench] IC(0) setup {setup_s:.2f} s, levels {info.levels_lower}/{info.levels_upper}")
+    log(f"[bench] IC(0) setup {setup_s:.2f} s (ordering {info.phase_ms[0]:.0f} ms, factorization "
+        f"{info.phase_ms[1]:.0f} ms, analysis + upload {info.phase_ms[2]:.0f} ms, schedule {info.phase_ms[3]:.0f} ms), "
+        f"levels {info.levels_lower}/{info.levels_upper}")
 
     def solve(precision, P=P, variant="auto"):
         cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision=precision)
@@ -286,10 +288,13 @@ def main():
         dt = time.perf_counter() - t
         return dt, cg.report, x
 
+    first_mixed_s = None
     if not args.kernels_only:
-        for _ in range(args.warmup):
+        for w in range(args.warmup):
             solve("fp64")
-            solve("mixed")
+            dt_w, _, _ = solve("mixed")
+            if w == 0:
+                first_mixed_s = dt_w  # includes the colour-ordered matrix and reduced form built on first use
         t64, tmx, rep64, repmx = [], [], None, None
         torch.cuda.synchronize()
         for _ in range(args.steps):
@@ -460,6 +465,8 @@ def main():
         "relres_fp64": rep64.true_relres, "relres_mixed": repmx.true_relres,
         "err_mixed_vs_fp64": err_mixed, "err_vs_xtrue": err_true, "setup_s": round(setup_s, 3),
         "levels": [info.levels_lower, info.levels_upper],
+        "setup_phases_ms": [round(float(v), 1) for v in info.phase_ms],
+        "first_mixed_solve_s": None if first_mixed_s is None else round(first_mixed_s, 4),
         "roofline": dominant,
         "roofline_apply": apply_rec,
         "roofline_spmv_f32": {"kernel": "k_spmv<float>", "achieved": round(gbs(alg_spmv32, spmv32_ms), 1),

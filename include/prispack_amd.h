@@ -181,7 +181,8 @@ typedef struct {
                                bit 1 16-bit dependency offsets, bit 2 batched loads */
     int32_t fused_levels;   /* 1: last forward and first backward level share one launch */
     int32_t trsv;           /* the SpTRSV schedule in use (PP_TRSV_AUTO resolved at setup) */
-    uint32_t reserved[4];
+    float phase_ms[4];      /* setup phases: ordering, factorization, sweep analysis + upload,
+                               schedule choice + load-variant timing */
 } pp_precond_info;
 
 void pp_precond_opts_default(pp_precond_opts *o);
@@ -287,7 +288,11 @@ typedef struct {
     int32_t cg_variant;     /* CG: the iteration that ran (PP_CG_REFERENCE,
                                _SINGLE_REDUCTION or _REDUCED; for PP_MIXED the
                                inner solves'); GMRES: PP_CG_AUTO */
-    uint32_t reserved[7];
+    int32_t inner_fp64_steps; /* refinement steps whose inner solve ran in FP64
+                               (PP_MIXED reduced CG: the steps after an FP32
+                               step fell short of its own residual test on the
+                               true residual) */
+    uint32_t reserved[6];
 } pp_solve_report;
 
 void pp_solve_opts_default(pp_solve_opts *o, int solver);

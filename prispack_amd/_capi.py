@@ -44,7 +44,7 @@ class PrecondInfo(C.Structure):
                 ("mnnz", C.c_uint32), ("fill_ratio", C.c_double), ("levels_lower", C.c_uint32),
                 ("levels_upper", C.c_uint32), ("setup_ms", C.c_double), ("fail_row", C.c_uint32),
                 ("sweep_variant", C.c_int32), ("fused_levels", C.c_int32), ("trsv", C.c_int32),
-                ("reserved", C.c_uint32 * 4)]
+                ("phase_ms", C.c_float * 4)]
 
 
 class SolveOpts(C.Structure):
@@ -59,7 +59,8 @@ class SolveReport(C.Structure):
     _fields_ = [("iters", C.c_int32), ("converged", C.c_int32), ("final_error", C.c_double),
                 ("true_relres", C.c_double), ("outer_iters", C.c_int32), ("inner_iters", C.c_int32),
                 ("solve_ms", C.c_double), ("status", C.c_int32), ("nhist", C.c_int32),
-                ("cg_variant", C.c_int32), ("reserved", C.c_uint32 * 7)]
+                ("cg_variant", C.c_int32), ("inner_fp64_steps", C.c_int32),
+                ("reserved", C.c_uint32 * 6)]
 
 
 vp = C.c_void_p

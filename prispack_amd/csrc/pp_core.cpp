@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <thread>
 
 #include "pp_internal.h"
 
@@ -128,6 +129,52 @@ uint64_t next_uid() {
     return counter.fetch_add(1);
 }
 
+void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_t)> &fn) {
+    const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t parts = std::min(hw, grain ? std::max<size_t>(1, n / grain) : hw);
+    if (parts <= 1) {
+        if (n) fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(parts - 1);
+    std::exception_ptr err;
+    std::mutex m;
+    auto run = [&](size_t b, size_t e) {
+        try {
+            fn(b, e);
+        } catch (...) {
+            std::lock_guard<std::mutex> g(m);
+            if (!err) err = std::current_exception();
+        }
+    };
+    for (size_t p = 1; p < parts; ++p) th.emplace_back(run, n * p / parts, n * (p + 1) / parts);
+    run(0, n / parts);
+    for (auto &t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
+
+uint64_t two_base_blocks(const std::vector<uint32_t> &blk, const std::vector<uint32_t> &rp, const uint32_t *ci,
+                         std::vector<uint16_t> &off16, std::vector<uint32_t> &base) {
+    const size_t nb = blk.size() > 1 ? blk.size() - 1 : 0;
+    base.assign(2 * nb, kNoBase);
+    std::atomic<uint64_t> n16{0};
+    parallel_for(nb, 256, [&](size_t b0, size_t b1) {
+        uint64_t local = 0;
+        for (size_t j = b0; j < b1; ++j) {
+            const uint32_t e0 = rp[blk[j]], e1 = rp[blk[j + 1]];
+            if (e1 == e0 || e1 - e0 > kSpmvNnzBlock) continue;
+            uint32_t lo = 0, hi = 0;
+            if (!two_base_offsets(ci + e0, e1 - e0, lo, hi, off16.data() + e0)) continue;
+            base[2 * j] = lo;
+            base[2 * j + 1] = hi;
+            local += e1 - e0;
+        }
+        n16 += local;
+    });
+    return n16.load();
+}
+
 std::vector<BlockDesc> block_descs(const std::vector<uint32_t> &blk, const std::vector<uint32_t> &rp,
                                    const std::vector<uint32_t> &base) {
     std::vector<BlockDesc> d(blk.size());
@@ -170,7 +217,9 @@ pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, c
     uint32_t nnz = rp[nrow];
     PP_CHECK(nnz == 0 || (ci && v), PP_ERR_INVALID, "colidx/values are NULL");
     for (uint32_t i = 0; i < nrow; ++i) PP_CHECK(rp[i] <= rp[i + 1], PP_ERR_INVALID, "rowptr not monotone");
-    for (uint32_t k = 0; k < nnz; ++k) PP_CHECK(ci[k] < ncol, PP_ERR_INVALID, "column index out of range");
+    parallel_for(nnz, 1u << 16, [&](size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; ++k) PP_CHECK(ci[k] < ncol, PP_ERR_INVALID, "column index out of range");
+    });
     std::unique_ptr<pp_csr> A(new pp_csr);
     A->ctx = ctx;
     A->uid = next_uid();
@@ -195,19 +244,11 @@ pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, c
     PP_HIP(hipMemcpyAsync(A->row_blocks, blocks.data(), blocks.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                           ctx->stream));
     // 16-bit two-base column offsets where a block's columns fit two windows
-    std::vector<uint32_t> base(2 * (size_t)A->nblocks, kNoBase);
+    std::vector<uint32_t> base;
     std::vector<uint16_t> off16(nnz, 0);
+    A->nnz16 = two_base_blocks(blocks, A->h_rp, ci, off16, base);
     A->nblocks16 = 0;
-    for (uint32_t bi = 0; bi < A->nblocks; ++bi) {
-        const uint32_t k0 = rp[blocks[bi]], k1 = rp[blocks[bi + 1]];
-        if (k1 == k0 || k1 - k0 > kSpmvNnzBlock) continue;
-        uint32_t lo = 0, hi = 0;
-        if (!two_base_offsets(ci + k0, k1 - k0, lo, hi, off16.data() + k0)) continue;
-        base[2 * bi] = lo;
-        base[2 * bi + 1] = hi;
-        ++A->nblocks16;
-        A->nnz16 += k1 - k0;
-    }
+    for (uint32_t bi = 0; bi < A->nblocks; ++bi) A->nblocks16 += base[2 * bi] != kNoBase;
     if (A->nblocks16) {
         A->ci16 = dalloc<uint16_t>(nnz);
         A->blk_base = dalloc<uint32_t>(base.size());

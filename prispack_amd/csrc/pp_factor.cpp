@@ -7,6 +7,7 @@
 // on the order of the floating-point updates.  The zero-fill IC(0)/ILU(0)
 // have no reference counterpart.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -281,34 +282,102 @@ int factor_ic0(const HostCsr &A, HostCsr &lt, uint32_t &fail_row) {
         rp[i + 1] = rp[i] + c;
     }
     std::vector<uint32_t> ci(rp[n]);
-    std::vector<double> lv(rp[n]), diag(n);
-    std::vector<int64_t> slot(n, -1);
-    int status = PP_OK;
+    std::vector<double> lv(rp[n]), diag(n), aii(n, 0.0);
+    // the strict lower pattern (in the order of A's rows) and the
+    // dependency level of every row: rows of one level factor independently
+    std::vector<uint32_t> lev(n, 0);
+    uint32_t nlev = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        uint32_t q = rp[i];
-        double aii = 0.0;
+        uint32_t q = rp[i], l = 0;
         for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
             const uint32_t c = A.ci[k];
-            if (c < i) { ci[q] = c; lv[q] = A.v[k]; slot[c] = q; ++q; }
-            else if (c == i) aii = A.v[k];
-        }
-        for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) {
-            const uint32_t k = ci[t];
-            double s = lv[t];
-            for (uint32_t u = rp[k]; u < rp[k + 1]; ++u) {
-                const int64_t p = slot[ci[u]];
-                if (p >= 0) s -= lv[p] * lv[u];
+            if (c < i) {
+                ci[q] = c;
+                lv[q] = A.v[k];
+                l = std::max(l, lev[c] + 1);
+                ++q;
+            } else if (c == i) {
+                aii[i] = A.v[k];
             }
-            lv[t] = s / diag[k];
         }
-        double s = aii;
+        lev[i] = l;
+        nlev = std::max(nlev, l + 1);
+    }
+    std::vector<uint32_t> lptr(nlev + 1, 0), rows(n);
+    for (uint32_t i = 0; i < n; ++i) lptr[lev[i] + 1]++;
+    for (uint32_t l = 0; l < nlev; ++l) lptr[l + 1] += lptr[l];
+    {
+        std::vector<uint32_t> f(lptr.begin(), lptr.end() - 1);
+        for (uint32_t i = 0; i < n; ++i) rows[f[lev[i]]++] = i;
+    }
+    // row i: l_ij = (a_ij - sum_{k<j} l_ik l_jk) / l_jj with the sum taken
+    // in the order of row j's stored entries, l_ii = sqrt(a_ii - sum l_ij^2)
+    std::atomic<int64_t> first_fail{-1};
+    auto pivot = [&](uint32_t i) {
+        double s = aii[i];
         for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) s -= lv[t] * lv[t];
-        for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) slot[ci[t]] = -1;
         if (!(s > 0.0)) {
-            if (status == PP_OK) { status = PP_ERR_NOT_SPD; fail_row = i; }
+            int64_t expect = first_fail.load();
+            while ((expect < 0 || (int64_t)i < expect) && !first_fail.compare_exchange_weak(expect, i)) {
+            }
             s = 1.0;
         }
         diag[i] = std::sqrt(s);
+    };
+    bool ascending = true;
+    for (uint32_t i = 0; i < n && ascending; ++i)
+        for (uint32_t t = rp[i]; t + 1 < rp[i + 1]; ++t)
+            if (!(ci[t] < ci[t + 1])) {
+                ascending = false;
+                break;
+            }
+    if (ascending) {
+        // rows of one level in parallel; row i's entries before t meet row
+        // k's entries in a merge of the two ascending column lists
+        for (uint32_t l = 0; l < nlev; ++l) {
+            const uint32_t r0 = lptr[l], r1 = lptr[l + 1];
+            parallel_for(r1 - r0, 4096, [&](size_t a, size_t b) {
+                for (size_t q = a; q < b; ++q) {
+                    const uint32_t i = rows[r0 + q];
+                    const uint32_t i0 = rp[i], i1 = rp[i + 1];
+                    for (uint32_t t = i0; t < i1; ++t) {
+                        const uint32_t k = ci[t];
+                        double s = lv[t];
+                        uint32_t p = i0;
+                        for (uint32_t u = rp[k]; u < rp[k + 1]; ++u) {
+                            const uint32_t c = ci[u];
+                            while (p < t && ci[p] < c) ++p;
+                            if (p < t && ci[p] == c) s -= lv[p] * lv[u];
+                        }
+                        lv[t] = s / diag[k];
+                    }
+                    pivot(i);
+                }
+            });
+        }
+    } else {
+        // unsorted or repeated columns: serial, row i's entries found by
+        // column through a slot table (the last of repeated columns)
+        std::vector<int64_t> slot(n, -1);
+        for (uint32_t i = 0; i < n; ++i) {
+            for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) slot[ci[t]] = t;
+            for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) {
+                const uint32_t k = ci[t];
+                double s = lv[t];
+                for (uint32_t u = rp[k]; u < rp[k + 1]; ++u) {
+                    const int64_t p = slot[ci[u]];
+                    if (p >= 0) s -= lv[p] * lv[u];
+                }
+                lv[t] = s / diag[k];
+            }
+            pivot(i);
+            for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) slot[ci[t]] = -1;
+        }
+    }
+    int status = PP_OK;
+    if (first_fail.load() >= 0) {
+        status = PP_ERR_NOT_SPD;
+        fail_row = (uint32_t)first_fail.load();
     }
     lt.n = n;
     lt.rp.assign(n + 1, 0);
@@ -384,19 +453,32 @@ namespace pp {
 // then lies in an earlier colour, so the sweeps have <= ncolors levels.
 std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors) {
     const uint32_t n = A.n;
-    std::vector<std::vector<uint32_t>> nb(n);
+    // symmetrised adjacency without the diagonal, as CSR (duplicates harmless)
+    std::vector<uint32_t> deg(n + 1, 0);
     for (uint32_t i = 0; i < n; ++i)
         for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
             if (A.ci[k] != i) {
-                nb[i].push_back(A.ci[k]);
-                nb[A.ci[k]].push_back(i);
+                deg[i + 1]++;
+                deg[A.ci[k] + 1]++;
             }
+    for (uint32_t i = 0; i < n; ++i) deg[i + 1] += deg[i];
+    std::vector<uint32_t> adj(deg[n]), fill(deg.begin(), deg.end() - 1);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.ci[k] != i) {
+                adj[fill[i]++] = A.ci[k];
+                adj[fill[A.ci[k]]++] = i;
+            }
+    // greedy: each row takes the smallest colour none of its coloured
+    // neighbours has
     std::vector<uint32_t> color(n, kNone), mark;
     ncolors = 0;
     for (uint32_t i = 0; i < n; ++i) {
         if (mark.size() < ncolors + 1) mark.resize(ncolors + 1, kNone);
-        for (uint32_t j : nb[i])
+        for (uint32_t q = deg[i]; q < deg[i + 1]; ++q) {
+            const uint32_t j = adj[q];
             if (color[j] != kNone) mark[color[j]] = i;
+        }
         uint32_t c = 0;
         while (c < ncolors && mark[c] == i) ++c;
         color[i] = c;
@@ -418,20 +500,24 @@ HostCsr permute_symmetric(const HostCsr &A, const std::vector<uint32_t> &order) 
     HostCsr B;
     B.n = n;
     B.rp.assign(n + 1, 0);
-    B.ci.reserve(A.ci.size());
-    B.v.reserve(A.v.size());
-    std::vector<std::pair<uint32_t, double>> row;
-    for (uint32_t k = 0; k < n; ++k) {
-        const uint32_t i = order[k];
-        row.clear();
-        for (uint32_t q = A.rp[i]; q < A.rp[i + 1]; ++q) row.emplace_back(inv[A.ci[q]], A.v[q]);
-        std::sort(row.begin(), row.end());
-        for (auto &e : row) {
-            B.ci.push_back(e.first);
-            B.v.push_back(e.second);
+    for (uint32_t k = 0; k < n; ++k) B.rp[k + 1] = B.rp[k] + (A.rp[order[k] + 1] - A.rp[order[k]]);
+    B.ci.resize(B.rp[n]);
+    B.v.resize(B.rp[n]);
+    parallel_for(n, 1u << 14, [&](size_t k0, size_t k1) {
+        std::vector<std::pair<uint32_t, double>> row;
+        for (size_t k = k0; k < k1; ++k) {
+            const uint32_t i = order[k];
+            row.clear();
+            for (uint32_t q = A.rp[i]; q < A.rp[i + 1]; ++q) row.emplace_back(inv[A.ci[q]], A.v[q]);
+            std::sort(row.begin(), row.end());
+            uint32_t p = B.rp[k];
+            for (auto &e : row) {
+                B.ci[p] = e.first;
+                B.v[p] = e.second;
+                ++p;
+            }
         }
-        B.rp[k + 1] = (uint32_t)B.ci.size();
-    }
+    });
     return B;
 }
 

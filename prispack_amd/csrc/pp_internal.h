@@ -289,6 +289,7 @@ struct ReducedForm {
     float *s1_32 = nullptr;
     double *dt_64 = nullptr;  // Dt: the true residual of the second colour is Dt^1/2 times the reduced one
     float *dt_32 = nullptr;
+    double anorm = 0.0;           // max absolute row sum of the colour-ordered matrix
     const pp_csr *src = nullptr;  // built from this matrix at src_version
     uint64_t src_version = 0;
 };
@@ -342,6 +343,7 @@ struct pp_precond {
     uint64_t Aperm_version = 0;
     bool io_permuted = false;
     pp::ReducedForm *reduced = nullptr;  // two-colour Schur-complement form (IC(0) only)
+    float phase_ms[4] = {0, 0, 0, 0};     // setup: ordering, factorization, analysis + upload, schedule
     // PP_TRSV_BLOCK storage (built when the rows chunk, see ChunkSet)
     pp::ChunkSet chunks;
     pp::DevChunkSweep<double> clow64, cup64;
@@ -429,6 +431,16 @@ void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes);
 // cache cleared before each one (a read of `flush_bytes` of other memory), so
 // that operands are streamed from HBM as they are inside a solve.
 float time_cold_ms(pp_ctx *ctx, const std::function<void()> &fn, int reps);
+// Host setup work split over threads: fn(begin, end) on contiguous ranges of
+// [0, n), at most 16 threads and at least `grain` items each (small n runs
+// on the calling thread).  fn must only write what its range owns.
+void parallel_for(size_t n, size_t grain, const std::function<void(size_t, size_t)> &fn);
+// 16-bit two-base offsets for every row block of a CSR in one pass (blocks
+// blk[j]..blk[j+1] of positions into rp, columns ci): off16 per entry, base
+// 2 per block (kNoBase where a block does not fit); returns the entries on
+// 16-bit offsets.
+uint64_t two_base_blocks(const std::vector<uint32_t> &blk, const std::vector<uint32_t> &rp, const uint32_t *ci,
+                         std::vector<uint16_t> &off16, std::vector<uint32_t> &base);
 // Host-built descriptors of row blocks blk[j]..blk[j+1] (positions into rp);
 // base[2j], base[2j+1] the 16-bit offset bases or kNoBase.
 std::vector<BlockDesc> block_descs(const std::vector<uint32_t> &blk, const std::vector<uint32_t> &rp,

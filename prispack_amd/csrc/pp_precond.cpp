@@ -72,18 +72,19 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
     std::vector<uint32_t> perm, lp;
     levels_of(s, perm, lp);
     const uint32_t n = s.n;
-    std::vector<uint32_t> rp(n + 1, 0), ci;
-    std::vector<T> val;
-    ci.reserve(s.ci.size());
-    val.reserve(s.val.size());
-    for (uint32_t k = 0; k < n; ++k) {
-        const uint32_t i = perm[k];
-        for (uint32_t e = s.rp[i]; e < s.rp[i + 1]; ++e) {
-            ci.push_back(s.ci[e]);
-            val.push_back((T)s.val[e]);
+    std::vector<uint32_t> rp(n + 1, 0);
+    for (uint32_t k = 0; k < n; ++k) rp[k + 1] = rp[k] + (s.rp[perm[k] + 1] - s.rp[perm[k]]);
+    std::vector<uint32_t> ci(rp[n]);
+    std::vector<T> val(rp[n]);
+    parallel_for(n, 1u << 14, [&](size_t k0, size_t k1) {
+        for (size_t k = k0; k < k1; ++k) {
+            const uint32_t i = perm[k];
+            for (uint32_t e = s.rp[i], q = rp[k]; e < s.rp[i + 1]; ++e, ++q) {
+                ci[q] = s.ci[e];
+                val[q] = (T)s.val[e];
+            }
         }
-        rp[k + 1] = (uint32_t)ci.size();
-    }
+    });
     d.n = n;
     d.nnz = (uint32_t)ci.size();
     d.nlev = lp.empty() ? 0 : (uint32_t)lp.size() - 1;
@@ -120,17 +121,9 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
         d.blocks = dalloc<uint32_t>(blk.size());
         PP_HIP(hipMemcpy(d.blocks, blk.data(), blk.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         // 16-bit two-base offsets where a block's dependency columns fit
-        std::vector<uint32_t> base(2 * (blk.size() - 1), kNoBase);
+        std::vector<uint32_t> base;
         std::vector<uint16_t> off16(ci.size(), 0);
-        for (size_t j = 0; j + 1 < blk.size(); ++j) {
-            const uint32_t e0 = rp[blk[j]], e1 = rp[blk[j + 1]];
-            if (e1 == e0 || e1 - e0 > kSpmvNnzBlock) continue;
-            uint32_t lo = 0, hi = 0;
-            if (!two_base_offsets(ci.data() + e0, e1 - e0, lo, hi, off16.data() + e0)) continue;
-            base[2 * j] = lo;
-            base[2 * j + 1] = hi;
-            d.nnz16 += e1 - e0;
-        }
+        d.nnz16 = two_base_blocks(blk, rp, ci.data(), off16, base);
         d.nblk = blk.size() - 1;
         const std::vector<BlockDesc> ds = block_descs(blk, rp, base);
         d.desc = dalloc<BlockDesc>(ds.size());
@@ -368,7 +361,15 @@ static HostSweep lower_from_upper_rows(const HostCsr &Ut, int mode, bool diag_fi
     return s;
 }
 
+static double ms_since(std::chrono::steady_clock::time_point &t) {
+    const auto now = std::chrono::steady_clock::now();
+    const double ms = std::chrono::duration<double, std::milli>(now - t).count();
+    t = now;
+    return ms;
+}
+
 static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
+    auto tp = std::chrono::steady_clock::now();
     HostCsr Aperm;
     const HostCsr *Ap = &A0;
     M->order.clear();
@@ -377,6 +378,7 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
         Aperm = permute_symmetric(A0, M->order);
         Ap = &Aperm;
     }
+    M->phase_ms[0] = (float)ms_since(tp);
     const HostCsr &A = *Ap;
     const int kind = M->opts.kind;
     int st = PP_OK;
@@ -424,6 +426,7 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
     } else {
         throw Error(PP_ERR_UNSUPPORTED, "unknown preconditioner kind");
     }
+    M->phase_ms[1] = (float)ms_since(tp);
     // the forward sweep's last level and the backward sweep's first level
     // (rows without dependencies) hold the same rows: one kernel does both
     {
@@ -737,10 +740,15 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
             A.ci = M->A->h_ci;
             A.v = M->A->h_v;
             setup_triangular(M, A, true);
+            auto tp = std::chrono::steady_clock::now();
+            M->phase_ms[2] = (float)(std::chrono::duration<double, std::milli>(tp - t0).count() - M->phase_ms[0] -
+                                     M->phase_ms[1]);
             M->trsv = M->opts.trsv == PP_TRSV_AUTO    ? choose_schedule(ctx, M, false)
                       : M->opts.trsv == PP_TRSV_BLOCK ? choose_schedule(ctx, M, true)
                                                       : M->opts.trsv;
             if (M->trsv == PP_TRSV_LEVELS) tune_sweeps(ctx, M);
+            PP_HIP(hipStreamSynchronize(ctx->stream));
+            M->phase_ms[3] = (float)ms_since(tp);
         }
         M->ready = 1;
         M->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -881,6 +889,7 @@ int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {
     info->sweep_variant = M->sweep_variant;
     info->fused_levels = M->fuse_mid ? 1 : 0;
     info->trsv = M->trsv;
+    for (int k = 0; k < 4; ++k) info->phase_ms[k] = M->phase_ms[k];
     return PP_OK;
 }
 

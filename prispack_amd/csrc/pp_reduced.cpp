@@ -2,6 +2,7 @@
 // (the operators of the reduced CG; see ReducedForm in pp_internal.h).
 // Host-side construction from the colour-ordered matrix, once per matrix
 // version; the device work is SpMV (k_spmv) on the four operators.
+#include <algorithm>
 #include <cmath>
 #include <memory>
 
@@ -101,6 +102,11 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
         return nullptr;
     }
     const uint32_t n1 = n - n0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double t = 0.0;
+        for (uint32_t k = As->h_rp[i]; k < As->h_rp[i + 1]; ++k) t += std::fabs(As->h_v[k]);
+        R->anorm = std::max(R->anorm, t);
+    }
     Rows KA, KB, KF, KE;
     for (uint32_t i = 0; i < n0; ++i) {
         KE.put(i, 1.0 / d[i]);

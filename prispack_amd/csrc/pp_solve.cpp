@@ -351,7 +351,7 @@ double true_relres(pp_ctx *ctx, pp_csr *A, const double *b, const double *x) {
 // (PP_CG_REFERENCE) FlexibleCG.
 template <class Tin>
 PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, const pp_solve_opts &o, const double *b,
-                     double *x, int32_t &outer, int32_t &inner_total, double &relres) {
+                     double *x, int32_t &outer, int32_t &inner_total, int32_t &steps64, double &relres) {
     constexpr bool f32 = sizeof(Tin) == 4;
     const uint32_t n = A->nrow;
     if (f32) {
@@ -367,26 +367,38 @@ PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, cons
     PcgResult res;
     launch_dot(ctx, PP_F64, n, b, b, S_TMP3);
     double bnorm = -1.0, prev_rnorm = 0.0;  // ||b|| is read with the first residual norm
+    // Reduced FP32 steps continue one Krylov iteration across refinement
+    // steps (resume), which needs the replaced residual to agree with the
+    // FP32 recurrence's in direction.  FP32 rounding separates the two by
+    // about eps32 ||A|| ||e|| per unit residual, e the step's correction --
+    // large when the residual is rich in the slowly converging modes.  Each
+    // step asks a reduction `step` at least kMargin times that gap (from
+    // kResumeStep up to kMaxStep), and the next step resumes only where the
+    // finished step met that margin; where the margin would need steps
+    // larger than kMaxStep, or a resumed step fell well short of its own
+    // test on the true residual, the remaining steps run in FP64.
+    constexpr double kResumeStep = 1e-4, kMaxStep = 1e-2, kMargin = 100.0, kEps32 = 5.96e-8;
     bool resume = false;
+    bool inner64 = !f32;
+    double step = kResumeStep;
+    double step_claim = 0.0;  // the reduction the last inner solve was asked for
     outer = 0;
     inner_total = 0;
+    steps64 = 0;
     const double inner_tol = f32 ? (o.inner_tol > 0 ? o.inner_tol : 1e-6) : 0.0;
     const uint32_t max_outer = o.max_outer ? o.max_outer : 20;
     const bool graph = o.launch != PP_LAUNCH_STREAM;
+    bool have_enorm = false;
     relres = 1.0;
     for (;;) {
         PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         launch_spmv(ctx, A, PP_F64, -1.0, x, 1.0, r, -1, nullptr, false);
         launch_dot(ctx, PP_F64, n, r, r, S_TMP2);
-        double rnorm;
-        if (bnorm < 0) {  // S_TMP2, S_TMP3 in one read
-            double v[2];
-            read_slots(ctx, S_TMP2, 2, v);
-            rnorm = std::sqrt(v[0]);
-            bnorm = std::sqrt(v[1]);
-        } else {
-            rnorm = std::sqrt(read_slot(ctx, S_TMP2));
-        }
+        // S_TMP3 holds ||b||^2 on the first read, then ||e||^2 of the last FP32 reduced step
+        double v[2];
+        read_slots(ctx, S_TMP2, 2, v);
+        const double rnorm = std::sqrt(v[0]);
+        if (bnorm < 0) bnorm = std::sqrt(v[1]);
         relres = bnorm > 0 ? rnorm / bnorm : rnorm;
         if (!(relres == relres)) {
             res.status = PP_ERR_NOT_FINITE;
@@ -396,21 +408,51 @@ PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, cons
             res.conv = 1;
             break;
         }
+        if (f32 && R && have_enorm && !inner64) {
+            const double gap = kEps32 * R->anorm * std::sqrt(v[1]);
+            if (resume && rnorm > 10.0 * step_claim * prev_rnorm) {
+                inner64 = true;  // the resumed step fell short on the true residual
+            } else if (kMargin * gap > step) {
+                // the finished step's residual is not the recurrence's: restart,
+                // with steps the gap allows, or in FP64 past kMaxStep
+                resume = false;
+                step = kMargin * gap;
+                if (step > kMaxStep) inner64 = true;
+            }
+            if (inner64) resume = false;
+        }
+        have_enorm = false;
         if ((uint32_t)outer >= max_outer || (uint32_t)inner_total >= o.maxit) break;
+        uint32_t inner_max = std::max<uint32_t>(2, o.maxit - inner_total);
+        PcgResult in;
+        if (f32 && R && inner64) {
+            double *rin64 = (double *)ws_get(ctx, W_RIN64, (size_t)n * sizeof(double));
+            double *e64 = (double *)ws_get(ctx, W_E64, (size_t)n * sizeof(double));
+            launch_axpby(ctx, PP_F64, n, 1.0 / rnorm, r, 0.0, r, rin64);
+            launch_fill_zero(ctx, e64, (size_t)n * sizeof(double));
+            in = pcg_reduced_device<double>(ctx, R, rin64, e64, o.tol * bnorm / rnorm, inner_max, o.check_every,
+                                            graph);
+            inner_total += in.iters;
+            ++outer;
+            ++steps64;
+            prev_rnorm = rnorm;
+            if (in.status != PP_OK && in.status != PP_ERR_NOT_FINITE) {
+                res.status = in.status;
+                break;
+            }
+            launch_axpby(ctx, PP_F64, n, rnorm, e64, 1.0, x, x);
+            continue;
+        }
         if (f32) launch_scale_convert(ctx, n, r, 1.0 / rnorm, (float *)rin);
         else launch_axpby(ctx, PP_F64, n, 1.0 / rnorm, r, 0.0, r, rin);
         launch_fill_zero(ctx, e, (size_t)n * sizeof(Tin));
         // FP32: a margin for the drift between the recurrence and the true
-        // residual; FP64 inner solves stop where a plain FP64 solve would.
-        // A reduced inner solve that the next step resumes asks at most
-        // kResumeStep of a reduction per step: the FP32 recurrence residual
-        // then still agrees with the true one to about a percent, so the
-        // kept search directions stay conjugate to the replaced residual.
-        constexpr double kResumeStep = 1e-4;
-        const double floor_tol = R ? std::max(inner_tol, kResumeStep) : inner_tol;
+        // residual; FP64 inner solves stop where a plain FP64 solve would
+        const double floor_tol = R ? std::max(inner_tol, step) : inner_tol;
         const double want = f32 ? std::max(floor_tol, 0.5 * o.tol * bnorm / rnorm) : o.tol * bnorm / rnorm;
-        const uint32_t inner_max = std::max<uint32_t>(2, o.maxit - inner_total);
-        PcgResult in;
+        // a resumed step that has not converged within this many iterations
+        // is ended, and the next step restarts
+        if (resume) inner_max = std::min<uint32_t>(inner_max, std::max<uint32_t>(512, 4 * inner_total));
         if (R)
             in = pcg_reduced_device<Tin>(ctx, R, rin, e, want, inner_max, o.check_every, graph, resume,
                                          resume ? prev_rnorm / rnorm : 1.0);
@@ -420,11 +462,17 @@ PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, cons
         else in = pcg_sr_device<Tin>(ctx, A, M, rin, e, want, inner_max, o.check_every, false, graph);
         inner_total += in.iters;
         ++outer;
+        if (!f32) ++steps64;
         prev_rnorm = rnorm;
         resume = R && in.conv;  // a step that stopped on its test is continued, any other restarts
+        step_claim = want;
         if (in.status != PP_OK && in.status != PP_ERR_NOT_FINITE) {
             res.status = in.status;
             break;
+        }
+        if (f32 && R) {
+            launch_dot(ctx, PP_F32, n, e, e, S_TMP3);
+            have_enorm = true;
         }
         if (f32) launch_axpy_f32_to_f64(ctx, n, rnorm, (const float *)e, x);
         else launch_axpby(ctx, PP_F64, n, rnorm, e, 1.0, x, x);
@@ -702,12 +750,14 @@ static int solve_impl(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts
                      "blocks are diagonal");
             if (o.precision == PP_MIXED) {
                 double &rel = refined_relres;
-                res = pcg_refine<float>(ctx, As, M, red, o, bs, xs, R.outer_iters, R.inner_iters, rel);
+                res = pcg_refine<float>(ctx, As, M, red, o, bs, xs, R.outer_iters, R.inner_iters, R.inner_fp64_steps,
+                                        rel);
                 R.cg_variant = red ? PP_CG_REDUCED
                                    : (o.cg_variant == PP_CG_REFERENCE ? PP_CG_REFERENCE : PP_CG_SINGLE_REDUCTION);
             } else if (red) {
                 double &rel = refined_relres;
-                res = pcg_refine<double>(ctx, As, M, red, o, bs, xs, R.outer_iters, R.inner_iters, rel);
+                res = pcg_refine<double>(ctx, As, M, red, o, bs, xs, R.outer_iters, R.inner_iters, R.inner_fp64_steps,
+                                         rel);
                 R.cg_variant = PP_CG_REDUCED;
             } else {
                 R.cg_variant = o.cg_variant == PP_CG_SINGLE_REDUCTION ? PP_CG_SINGLE_REDUCTION : PP_CG_REFERENCE;

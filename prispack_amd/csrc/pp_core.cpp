@@ -210,12 +210,14 @@ bool two_base_offsets(const uint32_t *cols, uint32_t cnt, uint32_t &base_lo, uin
     return true;
 }
 
-pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci,
-                       const double *v) {
-    PP_CHECK(rp != nullptr, PP_ERR_INVALID, "rowptr is NULL");
+// The device CSR of a host matrix whose arrays it takes over; tune: time
+// the SpMV load variants on it (for matrices in an iteration's hot loop).
+pp_csr *csr_from_vectors(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, std::vector<uint32_t> &&rp,
+                         std::vector<uint32_t> &&ci, std::vector<double> &&v, bool tune) {
+    PP_CHECK(rp.size() == (size_t)nrow + 1, PP_ERR_INVALID, "rowptr size");
     PP_CHECK(rp[0] == 0, PP_ERR_INVALID, "rowptr[0] must be 0");
-    uint32_t nnz = rp[nrow];
-    PP_CHECK(nnz == 0 || (ci && v), PP_ERR_INVALID, "colidx/values are NULL");
+    const uint32_t nnz = rp[nrow];
+    PP_CHECK(ci.size() == nnz && v.size() == nnz, PP_ERR_INVALID, "colidx/values size");
     for (uint32_t i = 0; i < nrow; ++i) PP_CHECK(rp[i] <= rp[i + 1], PP_ERR_INVALID, "rowptr not monotone");
     parallel_for(nnz, 1u << 16, [&](size_t k0, size_t k1) {
         for (size_t k = k0; k < k1; ++k) PP_CHECK(ci[k] < ncol, PP_ERR_INVALID, "column index out of range");
@@ -226,16 +228,17 @@ pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, c
     A->nrow = nrow;
     A->ncol = ncol;
     A->nnz = nnz;
-    A->h_rp.assign(rp, rp + nrow + 1);
-    A->h_ci.assign(ci, ci + nnz);
-    A->h_v.assign(v, v + nnz);
+    A->h_rp = std::move(rp);
+    A->h_ci = std::move(ci);
+    A->h_v = std::move(v);
     A->rp = dalloc<uint32_t>(nrow + 1);
     A->ci = dalloc<uint32_t>(nnz);
     A->v64 = dalloc<double>(nnz);
-    PP_HIP(hipMemcpyAsync(A->rp, rp, (nrow + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    PP_HIP(hipMemcpyAsync(A->rp, A->h_rp.data(), (nrow + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
     if (nnz) {
-        PP_HIP(hipMemcpyAsync(A->ci, ci, nnz * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-        PP_HIP(hipMemcpyAsync(A->v64, v, nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(A->ci, A->h_ci.data(), nnz * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(A->v64, A->h_v.data(), nnz * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     }
     std::vector<uint32_t> blocks;
     build_row_blocks(A->h_rp, blocks);
@@ -245,8 +248,8 @@ pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, c
                           ctx->stream));
     // 16-bit two-base column offsets where a block's columns fit two windows
     std::vector<uint32_t> base;
-    std::vector<uint16_t> off16(nnz, 0);
-    A->nnz16 = two_base_blocks(blocks, A->h_rp, ci, off16, base);
+    std::vector<uint16_t> off16(nnz);
+    A->nnz16 = two_base_blocks(blocks, A->h_rp, A->h_ci.data(), off16, base);
     A->nblocks16 = 0;
     for (uint32_t bi = 0; bi < A->nblocks; ++bi) A->nblocks16 += base[2 * bi] != kNoBase;
     if (A->nblocks16) {
@@ -261,10 +264,19 @@ pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, c
         A->blk_desc = dalloc<BlockDesc>(ds.size());
         PP_HIP(hipMemcpyAsync(A->blk_desc, ds.data(), ds.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
                               ctx->stream));
-        PP_HIP(hipStreamSynchronize(ctx->stream));  // ds leaves scope
+        PP_HIP(hipStreamSynchronize(ctx->stream));  // ds, blocks, off16 leave scope
     }
-    csr_tune_spmv(ctx, A.get(), PP_F64);
+    if (tune) csr_tune_spmv(ctx, A.get(), PP_F64);
     return A.release();
+}
+
+pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci,
+                const double *v) {
+    PP_CHECK(rp != nullptr, PP_ERR_INVALID, "rowptr is NULL");
+    const uint32_t nnz = rp[nrow];
+    PP_CHECK(nnz == 0 || (ci && v), PP_ERR_INVALID, "colidx/values are NULL");
+    return csr_from_vectors(ctx, nrow, ncol, std::vector<uint32_t>(rp, rp + nrow + 1),
+                            std::vector<uint32_t>(ci, ci + nnz), std::vector<double>(v, v + nnz), true);
 }
 
 // Times each SpMV load variant (ordinary / non-temporal operand loads, 32-bit
@@ -297,8 +309,8 @@ static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A, int dtype) {
     dfree(y);
 }
 
-pp_csr *csr_create_host(pp_ctx *ctx, const HostCsr &H) {
-    return csr_new(ctx, H.n, H.n, H.rp.data(), H.ci.data(), H.v.data());
+pp_csr *csr_create_host(pp_ctx *ctx, HostCsr &&H) {
+    return csr_from_vectors(ctx, H.n, H.n, std::move(H.rp), std::move(H.ci), std::move(H.v), true);
 }
 
 void csr_free(pp_csr *A) {
@@ -324,8 +336,7 @@ pp_csr *csr_transpose(pp_ctx *ctx, pp_csr *A) {
     H.ci = A->h_ci;
     H.v = A->h_v;
     PP_CHECK(A->nrow == A->ncol, PP_ERR_UNSUPPORTED, "transpose of a rectangular matrix");
-    HostCsr T = transpose(H);
-    A->trans = csr_create_host(ctx, T);
+    A->trans = csr_create_host(ctx, transpose(H));
     return A->trans;
 }
 

@@ -18,22 +18,61 @@ namespace pp {
 
 static constexpr uint32_t kNone = 0xFFFFFFFFu;
 
+// T = transpose of the n x ncol CSR (rp, ci, v), each row of T starting
+// with `front` reserved slots.  Row j of T lists the entries of column j in
+// the order of their positions in A (rows ascending).  Large matrices:
+// column counts and placement by atomics over row ranges, then each row of
+// T sorted by source position, which restores that order.
+static void transpose_into(uint32_t n, uint32_t ncol, const uint32_t *rp, const uint32_t *ci, const double *v,
+                           uint32_t front, HostCsr &T) {
+    const uint32_t nnz = rp[n];
+    T.n = ncol;
+    T.rp.assign((size_t)ncol + 1, 0);
+    T.ci.resize((size_t)nnz + (size_t)front * ncol);
+    T.v.resize(T.ci.size());
+    if (nnz < (1u << 20)) {
+        for (uint32_t k = 0; k < nnz; ++k) T.rp[ci[k] + 1]++;
+        for (uint32_t j = 0; j < ncol; ++j) T.rp[j + 1] += T.rp[j] + front;
+        std::vector<uint32_t> next(ncol);
+        for (uint32_t j = 0; j < ncol; ++j) next[j] = T.rp[j] + front;
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const uint32_t p = next[ci[k]]++;
+                T.ci[p] = i;
+                T.v[p] = v[k];
+            }
+        return;
+    }
+    uint32_t *cnt = T.rp.data() + 1;
+    parallel_for(n, 1u << 14, [&](size_t i0, size_t i1) {
+        for (uint32_t k = rp[i0]; k < rp[i1]; ++k) __atomic_fetch_add(&cnt[ci[k]], 1u, __ATOMIC_RELAXED);
+    });
+    for (uint32_t j = 0; j < ncol; ++j) T.rp[j + 1] += T.rp[j] + front;
+    std::vector<uint32_t> next(ncol);
+    for (uint32_t j = 0; j < ncol; ++j) next[j] = T.rp[j] + front;
+    std::vector<uint64_t> key(T.ci.size());  // source position << 32 | source row
+    parallel_for(n, 1u << 14, [&](size_t i0, size_t i1) {
+        for (uint32_t i = (uint32_t)i0; i < (uint32_t)i1; ++i)
+            for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const uint32_t p = __atomic_fetch_add(&next[ci[k]], 1u, __ATOMIC_RELAXED);
+                key[p] = (uint64_t)k << 32 | i;
+            }
+    });
+    parallel_for(ncol, 1u << 14, [&](size_t j0, size_t j1) {
+        for (size_t j = j0; j < j1; ++j) {
+            const uint32_t b = T.rp[j] + front, e = T.rp[j + 1];
+            std::sort(key.begin() + b, key.begin() + e);
+            for (uint32_t p = b; p < e; ++p) {
+                T.ci[p] = (uint32_t)key[p];
+                T.v[p] = v[key[p] >> 32];
+            }
+        }
+    });
+}
+
 HostCsr transpose(const HostCsr &A) {
     HostCsr T;
-    T.n = A.n;
-    T.rp.assign(A.n + 1, 0);
-    const uint32_t nnz = A.rp[A.n];
-    T.ci.resize(nnz);
-    T.v.resize(nnz);
-    for (uint32_t k = 0; k < nnz; ++k) T.rp[A.ci[k] + 1]++;
-    for (uint32_t i = 0; i < A.n; ++i) T.rp[i + 1] += T.rp[i];
-    std::vector<uint32_t> next(T.rp.begin(), T.rp.end() - 1);
-    for (uint32_t i = 0; i < A.n; ++i)
-        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-            uint32_t p = next[A.ci[k]]++;
-            T.ci[p] = i;
-            T.v[p] = A.v[k];
-        }
+    transpose_into(A.n, A.n, A.rp.data(), A.ci.data(), A.v.data(), 0, T);
     return T;
 }
 
@@ -273,33 +312,41 @@ int factor_ilut(const HostCsr &A, uint32_t lfil, double tau, HostCsr &L, HostCsr
 // IC(0): zero-fill incomplete Cholesky on the pattern of tril(A), row by row:
 //   L_ik = (a_ik - sum_{j<k in rows i and k} L_ij L_kj) / L_kk,  L_ii = sqrt(a_ii - sum_k L_ik^2).
 // Returned as L^T with the diagonal first (the threshold-IC layout).
-int factor_ic0(const HostCsr &A, HostCsr &lt, uint32_t &fail_row) {
+int factor_ic0(const HostCsr &A, HostCsr &lt, uint32_t &fail_row, HostSweep *lower) {
     const uint32_t n = A.n;
     std::vector<uint32_t> rp(n + 1, 0);
-    for (uint32_t i = 0; i < n; ++i) {
-        uint32_t c = 0;
-        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) c += A.ci[k] < i;
-        rp[i + 1] = rp[i] + c;
-    }
+    parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) {
+            uint32_t c = 0;
+            for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) c += A.ci[k] < i;
+            rp[i + 1] = c;
+        }
+    });
+    for (uint32_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
     std::vector<uint32_t> ci(rp[n]);
     std::vector<double> lv(rp[n]), diag(n), aii(n, 0.0);
-    // the strict lower pattern (in the order of A's rows) and the
-    // dependency level of every row: rows of one level factor independently
+    // the strict lower pattern (in the order of A's rows)
+    parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) {
+            uint32_t q = rp[i];
+            for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
+                const uint32_t c = A.ci[k];
+                if (c < i) {
+                    ci[q] = c;
+                    lv[q] = A.v[k];
+                    ++q;
+                } else if (c == i) {
+                    aii[i] = A.v[k];
+                }
+            }
+        }
+    });
+    // the dependency level of every row: rows of one level factor independently
     std::vector<uint32_t> lev(n, 0);
     uint32_t nlev = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        uint32_t q = rp[i], l = 0;
-        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k) {
-            const uint32_t c = A.ci[k];
-            if (c < i) {
-                ci[q] = c;
-                lv[q] = A.v[k];
-                l = std::max(l, lev[c] + 1);
-                ++q;
-            } else if (c == i) {
-                aii[i] = A.v[k];
-            }
-        }
+        uint32_t l = 0;
+        for (uint32_t q = rp[i]; q < rp[i + 1]; ++q) l = std::max(l, lev[ci[q]] + 1);
         lev[i] = l;
         nlev = std::max(nlev, l + 1);
     }
@@ -379,24 +426,77 @@ int factor_ic0(const HostCsr &A, HostCsr &lt, uint32_t &fail_row) {
         status = PP_ERR_NOT_SPD;
         fail_row = (uint32_t)first_fail.load();
     }
-    lt.n = n;
-    lt.rp.assign(n + 1, 0);
-    for (uint32_t t = 0; t < rp[n]; ++t) lt.rp[ci[t] + 1]++;
-    for (uint32_t j = 0; j < n; ++j) lt.rp[j + 1] += lt.rp[j] + 1;
-    lt.ci.resize(rp[n] + n);
-    lt.v.resize(rp[n] + n);
-    std::vector<uint32_t> fill(n);
-    for (uint32_t j = 0; j < n; ++j) {
-        lt.ci[lt.rp[j]] = j;
-        lt.v[lt.rp[j]] = diag[j];
-        fill[j] = lt.rp[j] + 1;
-    }
-    for (uint32_t i = 0; i < n; ++i)
-        for (uint32_t t = rp[i]; t < rp[i + 1]; ++t) {
-            const uint32_t p = fill[ci[t]]++;
-            lt.ci[p] = i;
-            lt.v[p] = lv[t];
+    // L^T by rows: sqrt(pivot) first, then column j of the strict lower part
+    // (rows ascending).  With sorted rows and a symmetric pattern, column j
+    // of L is the upper part of A's row j and each value sits in row i of L
+    // at column j: found by binary search, row-parallel, no scatter.
+    bool sym = ascending;
+    if (sym) {
+        lt.n = n;
+        lt.rp.assign(n + 1, 0);
+        std::atomic<bool> ok{true};
+        parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
+            for (size_t i = i0; i < i1; ++i) {
+                uint32_t c = 1, prev = (uint32_t)i;
+                for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+                    if (A.ci[k] > i) {
+                        if (A.ci[k] <= prev) ok = false;  // upper part not strictly ascending
+                        prev = A.ci[k];
+                        ++c;
+                    }
+                lt.rp[i + 1] = c;
+            }
+        });
+        for (uint32_t i = 0; i < n; ++i) lt.rp[i + 1] += lt.rp[i];
+        sym = ok && lt.rp[n] == rp[n] + n;
+        if (sym) {
+            lt.ci.resize(lt.rp[n]);
+            lt.v.resize(lt.rp[n]);
+            parallel_for(n, 1u << 15, [&](size_t j0, size_t j1) {
+                for (uint32_t j = (uint32_t)j0; j < (uint32_t)j1; ++j) {
+                    uint32_t q = lt.rp[j];
+                    lt.ci[q] = j;
+                    lt.v[q++] = diag[j];
+                    for (uint32_t k = A.rp[j]; k < A.rp[j + 1]; ++k) {
+                        const uint32_t i = A.ci[k];
+                        if (i <= j) continue;
+                        const auto b = ci.begin() + rp[i], e = ci.begin() + rp[i + 1];
+                        const auto it = std::lower_bound(b, e, j);
+                        if (it == e || *it != j) {
+                            ok = false;
+                            return;
+                        }
+                        lt.ci[q] = i;
+                        lt.v[q++] = lv[it - ci.begin()];
+                    }
+                }
+            });
+            sym = ok;
         }
+    }
+    if (!sym) {
+        transpose_into(n, n, rp.data(), ci.data(), lv.data(), 1, lt);
+        parallel_for(n, 1u << 16, [&](size_t j0, size_t j1) {
+            for (size_t j = j0; j < j1; ++j) {
+                lt.ci[lt.rp[j]] = (uint32_t)j;
+                lt.v[lt.rp[j]] = diag[j];
+            }
+        });
+    }
+    // the forward sweep directly from the rows of L (sorted rows: the same
+    // rows the transpose of L^T gives)
+    if (lower) {
+        *lower = HostSweep();
+        if (ascending) {
+            lower->n = n;
+            lower->lower = true;
+            lower->mode = HostSweep::DIVIDE;
+            lower->rp = std::move(rp);
+            lower->ci = std::move(ci);
+            lower->val = std::move(lv);
+            lower->diag = std::move(diag);
+        }
+    }
     return status;
 }
 
@@ -451,8 +551,40 @@ namespace pp {
 // Greedy distance-1 colouring of the symmetrised pattern in natural order;
 // rows sorted by (colour, index).  Every dependency of a zero-fill factor row
 // then lies in an earlier colour, so the sweeps have <= ncolors levels.
+// Colours in one pass while they fit a 64-bit mask: row i takes the
+// smallest colour not in forbid[i] (the colours of earlier rows with an
+// entry in column i) nor among its own earlier columns, then forbids it to
+// its later columns.  Exactly the greedy colouring below; false when a
+// 65th colour would be needed.
+static bool greedy_colours_masked(const HostCsr &A, std::vector<uint32_t> &color, uint32_t &ncolors) {
+    const uint32_t n = A.n;
+    std::vector<uint64_t> forbid(n, 0);
+    color.assign(n, 0);
+    ncolors = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint64_t m = forbid[i];
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.ci[k] < i) m |= 1ull << color[A.ci[k]];
+        if (m == ~0ull) return false;
+        const uint32_t c = (uint32_t)__builtin_ctzll(~m);
+        color[i] = c;
+        ncolors = std::max(ncolors, c + 1);
+        for (uint32_t k = A.rp[i]; k < A.rp[i + 1]; ++k)
+            if (A.ci[k] > i) forbid[A.ci[k]] |= 1ull << c;
+    }
+    return true;
+}
+
 std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors) {
     const uint32_t n = A.n;
+    std::vector<uint32_t> color;
+    if (greedy_colours_masked(A, color, ncolors)) {
+        std::vector<uint32_t> order(n), start(ncolors + 1, 0);
+        for (uint32_t i = 0; i < n; ++i) start[color[i] + 1]++;
+        for (uint32_t c = 0; c < ncolors; ++c) start[c + 1] += start[c];
+        for (uint32_t i = 0; i < n; ++i) order[start[color[i]]++] = i;
+        return order;
+    }
     // symmetrised adjacency without the diagonal, as CSR (duplicates harmless)
     std::vector<uint32_t> deg(n + 1, 0);
     for (uint32_t i = 0; i < n; ++i)
@@ -471,7 +603,8 @@ std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors) {
             }
     // greedy: each row takes the smallest colour none of its coloured
     // neighbours has
-    std::vector<uint32_t> color(n, kNone), mark;
+    color.assign(n, kNone);
+    std::vector<uint32_t> mark;
     ncolors = 0;
     for (uint32_t i = 0; i < n; ++i) {
         if (mark.size() < ncolors + 1) mark.resize(ncolors + 1, kNone);

@@ -229,6 +229,8 @@ struct HostSweep {
     uint32_t n = 0;
     std::vector<uint32_t> rp, ci;
     std::vector<double> val, diag;
+    // dependency levels, computed once: rows in level order, level bounds
+    mutable std::vector<uint32_t> lev_perm, lev_ptr;
 };
 
 }  // namespace pp
@@ -374,7 +376,8 @@ int factor_ict(const HostCsr &A, double tau, bool ldlt, HostCsr &lt, std::vector
                uint32_t &fail_row);
 int factor_ilut(const HostCsr &A, uint32_t lfil, double tau, HostCsr &L, HostCsr &U,
                 std::vector<double> &D, uint32_t &fail_row);
-int factor_ic0(const HostCsr &A, HostCsr &lt, uint32_t &fail_row);
+// lower (optional): the forward sweep of L, filled when A's rows are sorted
+int factor_ic0(const HostCsr &A, HostCsr &lt, uint32_t &fail_row, HostSweep *lower = nullptr);
 int factor_ilu0(const HostCsr &A, HostCsr &L, HostCsr &U, std::vector<double> &D, uint32_t &fail_row);
 HostCsr transpose(const HostCsr &A);
 std::vector<uint32_t> multicolor_order(const HostCsr &A, uint32_t &ncolors);
@@ -386,6 +389,13 @@ void precond_level0(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *
 void precond_apply_after_level0(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
 void precond_need_f32(pp_precond *M);
 void csr_need_f32(pp_ctx *ctx, pp_csr *A);
+// device CSR of host arrays (copied / taken over); tune: time the SpMV load
+// variants on it, for matrices in an iteration's hot loop
+pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci, const double *v);
+pp_csr *csr_from_vectors(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, std::vector<uint32_t> &&rp,
+                         std::vector<uint32_t> &&ci, std::vector<double> &&v, bool tune);
+pp_csr *csr_create_host(pp_ctx *ctx, HostCsr &&H);
+void csr_free(pp_csr *A);
 
 // ---- tall-skinny GEMVs and GMRES helpers (pp_kernels_krylov.hip) ----
 // device coefficient buffer: [0,320) h, [320,640) y or a copy of h, then partials

@@ -8,9 +8,6 @@
 #include "pp_internal.h"
 
 namespace pp {
-pp_csr *csr_create_host(pp_ctx *ctx, const HostCsr &H);
-void csr_free(pp_csr *A);
-void csr_need_f32(pp_ctx *ctx, pp_csr *A);
 // multilevel preconditioners (pp_multilevel.cpp)
 void ml_setup(pp_ctx *ctx, pp_precond *M);
 void ml_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
@@ -43,7 +40,7 @@ template struct DevSweep<double>;
 template struct DevSweep<float>;
 
 // Levels of the dependency DAG: level(i) = 1 + max level of its dependencies.
-static void levels_of(const HostSweep &s, std::vector<uint32_t> &perm, std::vector<uint32_t> &level_ptr) {
+static void compute_levels(const HostSweep &s, std::vector<uint32_t> &perm, std::vector<uint32_t> &level_ptr) {
     const uint32_t n = s.n;
     std::vector<uint32_t> lev(n, 0);
     uint32_t maxlev = 0;
@@ -64,6 +61,13 @@ static void levels_of(const HostSweep &s, std::vector<uint32_t> &perm, std::vect
     std::vector<uint32_t> fill(level_ptr.begin(), level_ptr.end() - (nlev ? 1 : 0));
     perm.assign(n, 0);
     for (uint32_t i = 0; i < n; ++i) perm[fill[lev[i]]++] = i;
+}
+
+// The sweep's levels, computed on first use and kept with it.
+static void levels_of(const HostSweep &s, std::vector<uint32_t> &perm, std::vector<uint32_t> &level_ptr) {
+    if (s.lev_ptr.empty() || s.lev_perm.size() != s.n) compute_levels(s, s.lev_perm, s.lev_ptr);
+    perm = s.lev_perm;
+    level_ptr = s.lev_ptr;
 }
 
 template <class T>
@@ -311,8 +315,8 @@ static bool build_block_sweeps(pp_ctx *ctx, pp_precond *M) {
 template void upload_sweep<double>(const HostSweep &, DevSweep<double> &);
 template void upload_sweep<float>(const HostSweep &, DevSweep<float> &);
 
-// Gather form of a sweep given by the rows of `rows_src`:
-//   skip_first: the first entry of each row is its diagonal (IC L^T layout).
+// Gather form of a sweep given by the rows of R:
+//   diag_first: the first entry of each row is its diagonal (IC L^T layout).
 static HostSweep sweep_from_rows(const HostCsr &R, bool lower, int mode, bool diag_first,
                                  const std::vector<double> *diag_vec) {
     HostSweep s;
@@ -321,18 +325,18 @@ static HostSweep sweep_from_rows(const HostCsr &R, bool lower, int mode, bool di
     s.mode = mode;
     s.rp.assign(R.n + 1, 0);
     s.diag.assign(R.n, 1.0);
-    for (uint32_t i = 0; i < R.n; ++i) {
-        uint32_t b = R.rp[i], e = R.rp[i + 1];
-        if (diag_first) {
-            s.diag[i] = R.v[b];
-            ++b;
+    const uint32_t skip = diag_first ? 1 : 0;
+    for (uint32_t i = 0; i < R.n; ++i) s.rp[i + 1] = s.rp[i] + (R.rp[i + 1] - R.rp[i] - skip);
+    s.ci.resize(s.rp[R.n]);
+    s.val.resize(s.rp[R.n]);
+    parallel_for(R.n, 1u << 15, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) {
+            const uint32_t b = R.rp[i] + skip;
+            if (diag_first) s.diag[i] = R.v[R.rp[i]];
+            std::copy(R.ci.begin() + b, R.ci.begin() + R.rp[i + 1], s.ci.begin() + s.rp[i]);
+            std::copy(R.v.begin() + b, R.v.begin() + R.rp[i + 1], s.val.begin() + s.rp[i]);
         }
-        for (uint32_t k = b; k < e; ++k) {
-            s.ci.push_back(R.ci[k]);
-            s.val.push_back(R.v[k]);
-        }
-        s.rp[i + 1] = (uint32_t)s.ci.size();
-    }
+    });
     if (diag_vec) s.diag = *diag_vec;
     return s;
 }
@@ -340,24 +344,38 @@ static HostSweep sweep_from_rows(const HostCsr &R, bool lower, int mode, bool di
 // Column j of a CSR with the diagonal first becomes row j of the transpose
 // with the diagonal last; returns the gather-form lower sweep.
 static HostSweep lower_from_upper_rows(const HostCsr &Ut, int mode, bool diag_first) {
-    HostCsr L = transpose(Ut);  // rows ascending in column
+    const HostCsr L = transpose(Ut);  // rows ascending in column
     HostSweep s;
     s.n = L.n;
     s.lower = true;
     s.mode = mode;
     s.rp.assign(L.n + 1, 0);
     s.diag.assign(L.n, 1.0);
-    for (uint32_t i = 0; i < L.n; ++i) {
-        for (uint32_t k = L.rp[i]; k < L.rp[i + 1]; ++k) {
-            if (L.ci[k] == i) {
-                if (diag_first) s.diag[i] = L.v[k];
-                continue;
-            }
-            s.ci.push_back(L.ci[k]);
-            s.val.push_back(L.v[k]);
+    std::vector<uint32_t> cnt(L.n);
+    parallel_for(L.n, 1u << 15, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) {
+            uint32_t c = 0;
+            for (uint32_t k = L.rp[i]; k < L.rp[i + 1]; ++k) c += L.ci[k] != i;
+            cnt[i] = c;
         }
-        s.rp[i + 1] = (uint32_t)s.ci.size();
-    }
+    });
+    for (uint32_t i = 0; i < L.n; ++i) s.rp[i + 1] = s.rp[i] + cnt[i];
+    s.ci.resize(s.rp[L.n]);
+    s.val.resize(s.rp[L.n]);
+    parallel_for(L.n, 1u << 15, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) {
+            uint32_t q = s.rp[i];
+            for (uint32_t k = L.rp[i]; k < L.rp[i + 1]; ++k) {
+                if (L.ci[k] == i) {
+                    if (diag_first) s.diag[i] = L.v[k];
+                    continue;
+                }
+                s.ci[q] = L.ci[k];
+                s.val[q] = L.v[k];
+                ++q;
+            }
+        }
+    });
     return s;
 }
 
@@ -368,7 +386,9 @@ static double ms_since(std::chrono::steady_clock::time_point &t) {
     return ms;
 }
 
-static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
+// perm_out (multicolour order): receives P A P^T, which the factor was
+// computed from.
+static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload, HostCsr *perm_out = nullptr) {
     auto tp = std::chrono::steady_clock::now();
     HostCsr Aperm;
     const HostCsr *Ap = &A0;
@@ -388,24 +408,27 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
         std::vector<double> d;
         if (kind == PP_PC_ICT) st = factor_ict(A, M->opts.tau, false, lt, d, fail);
         else if (kind == PP_PC_ILDLT) st = factor_ict(A, M->opts.tau, true, lt, d, fail);
-        else st = factor_ic0(A, lt, fail);
-        M->lt_rp = lt.rp;
-        M->lt_ci = lt.ci;
-        M->lt_v = lt.v;
-        M->d = d;
+        else st = factor_ic0(A, lt, fail, &M->hlow);
+        M->d = std::move(d);
         M->mnnz = (uint32_t)(lt.ci.size() * 2 + A.n);  // IncompleteCholesky.cpp:245 convention
+        if (st == PP_OK) {
+            if (kind == PP_PC_ILDLT) {
+                M->hlow = lower_from_upper_rows(lt, HostSweep::UNIT, false);
+                M->hup = sweep_from_rows(lt, false, HostSweep::PREDIVIDE, false, &M->d);
+            } else {
+                if (kind != PP_PC_IC0 || M->hlow.rp.empty())  // IC(0) of sorted rows: filled by the factorization
+                    M->hlow = lower_from_upper_rows(lt, HostSweep::DIVIDE, true);
+                M->hup = sweep_from_rows(lt, false, HostSweep::DIVIDE, true, nullptr);
+            }
+        }
+        M->lt_rp = std::move(lt.rp);
+        M->lt_ci = std::move(lt.ci);
+        M->lt_v = std::move(lt.v);
         if (st != PP_OK) {
             M->fail_row = fail;
             throw Error(st, kind == PP_PC_ILDLT ? "ILDLT: zero pivot at row " + std::to_string(fail)
                                                 : "IC: matrix is not positive definite (pivot row " +
                                                       std::to_string(fail) + ")");
-        }
-        if (kind == PP_PC_ILDLT) {
-            M->hlow = lower_from_upper_rows(lt, HostSweep::UNIT, false);
-            M->hup = sweep_from_rows(lt, false, HostSweep::PREDIVIDE, false, &M->d);
-        } else {
-            M->hlow = lower_from_upper_rows(lt, HostSweep::DIVIDE, true);
-            M->hup = sweep_from_rows(lt, false, HostSweep::DIVIDE, true, nullptr);
         }
     } else if (kind == PP_PC_ILUT || kind == PP_PC_ILU0) {
         HostCsr L, U;
@@ -462,6 +485,7 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload) {
         upload_sweep(M->hup, M->up32);
         M->have32 = true;
     }
+    if (perm_out && Ap == &Aperm) *perm_out = std::move(Aperm);
 }
 
 // Times each staged-sweep load variant on one precision's sweeps (cold
@@ -526,7 +550,7 @@ pp_csr *precond_permuted_matrix(pp_ctx *ctx, pp_precond *M, pp_csr *A) {
     H.rp = A->h_rp;
     H.ci = A->h_ci;
     H.v = A->h_v;
-    M->Aperm = csr_create_host(ctx, permute_symmetric(H, M->order));
+    M->Aperm = csr_create_host(ctx, permute_symmetric(H, M->order));  // the matrix changed since setup
     M->Aperm_src = A;
     M->Aperm_version = A->version;
     return M->Aperm;
@@ -734,12 +758,32 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
         } else if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) {
             ml_setup(ctx, M);
         } else {
-            HostCsr A;
+            // the matrix's host arrays are lent to the factorization, not copied
+            HostCsr A, Ap;
             A.n = M->A->nrow;
-            A.rp = M->A->h_rp;
-            A.ci = M->A->h_ci;
-            A.v = M->A->h_v;
-            setup_triangular(M, A, true);
+            struct Lend {
+                HostCsr &A;
+                pp_csr *src;
+                Lend(HostCsr &a, pp_csr *s) : A(a), src(s) { swap(); }
+                ~Lend() { swap(); }
+                void swap() {
+                    A.rp.swap(src->h_rp);
+                    A.ci.swap(src->h_ci);
+                    A.v.swap(src->h_v);
+                }
+            };
+            {
+                Lend lend(A, M->A);
+                setup_triangular(M, A, true, &Ap);
+            }
+            // P A P^T on the device for the solves (they run on the colour-ordered system)
+            if (!Ap.rp.empty()) {
+                csr_free(M->Aperm);
+                M->Aperm = nullptr;
+                M->Aperm = csr_create_host(ctx, std::move(Ap));
+                M->Aperm_src = M->A;
+                M->Aperm_version = M->A->version;
+            }
             auto tp = std::chrono::steady_clock::now();
             M->phase_ms[2] = (float)(std::chrono::duration<double, std::milli>(tp - t0).count() - M->phase_ms[0] -
                                      M->phase_ms[1]);

@@ -3,31 +3,20 @@
 // Host-side construction from the colour-ordered matrix, once per matrix
 // version; the device work is SpMV (k_spmv) on the four operators.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <memory>
+#include <mutex>
 
 #include "pp_internal.h"
 
 namespace pp {
-pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci, const double *v);
-void csr_free(pp_csr *A);
-void csr_need_f32(pp_ctx *ctx, pp_csr *A);
-
 namespace {
-
-struct Rows {
-    std::vector<uint32_t> rp{0}, ci;
-    std::vector<double> v;
-    void put(uint32_t c, double x) {
-        ci.push_back(c);
-        v.push_back(x);
-    }
-    void end_row() { rp.push_back((uint32_t)ci.size()); }
-};
 
 // Colour split of a colour-ordered matrix: n0 = the first row with an
 // entry left of its diagonal.  Valid when every first-colour row couples
-// only to the second colour and vice versa, and every row has a diagonal.
+// only to the second colour and vice versa, and every row has one diagonal
+// entry.
 bool split_two_colours(const pp_csr *A, uint32_t &n0) {
     const uint32_t n = A->nrow;
     const auto &rp = A->h_rp;
@@ -45,6 +34,7 @@ bool split_two_colours(const pp_csr *A, uint32_t &n0) {
         for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
             const uint32_t j = ci[k];
             if (j == i) {
+                if (diag) return false;  // a repeated diagonal entry
                 diag = true;
                 continue;
             }
@@ -77,72 +67,110 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
     bool ok = As->nrow == As->ncol && split_two_colours(As, n0) && M->low64.nlev == 2 && M->up64.nlev == 2 &&
               M->lvl0_contiguous && M->lvl0_lo == 0 && M->lvl0_hi == n0;
     std::vector<double> d(n), dt;
+    const auto &rp = As->h_rp;
+    const auto &ci = As->h_ci;
+    const auto &av = As->h_v;
+    std::atomic<bool> good{ok};
     if (ok) {
-        for (uint32_t i = 0; i < n && ok; ++i) {
-            d[i] = diag_of(As, i);
-            ok = d[i] > 0.0;
-        }
+        parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
+            for (size_t i = i0; i < i1; ++i) {
+                d[i] = diag_of(As, (uint32_t)i);
+                if (!(d[i] > 0.0)) good = false;
+            }
+        });
+        ok = good;
     }
     if (ok) {
         // Dt_i = a_ii - sum_j a_ij^2 / a_jj over the first-colour neighbours
         // (the IC(0) pivot of a second-colour row)
         dt.assign(n - n0, 0.0);
-        for (uint32_t i = n0; i < n && ok; ++i) {
-            double t = d[i];
-            for (uint32_t k = As->h_rp[i]; k < As->h_rp[i + 1]; ++k) {
-                const uint32_t j = As->h_ci[k];
-                if (j < n0) t -= As->h_v[k] * As->h_v[k] / d[j];
+        parallel_for(n - n0, 1u << 15, [&](size_t q0, size_t q1) {
+            for (size_t q = q0; q < q1; ++q) {
+                const uint32_t i = n0 + (uint32_t)q;
+                double t = d[i];
+                for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
+                    if (ci[k] < n0) t -= av[k] * av[k] / d[ci[k]];
+                dt[q] = t;
+                if (!(t > 0.0 && std::isfinite(t))) good = false;
             }
-            dt[i - n0] = t;
-            ok = t > 0.0 && std::isfinite(t);
-        }
+        });
+        ok = good;
     }
     if (!ok) {
         M->reduced = R.release();
         return nullptr;
     }
     const uint32_t n1 = n - n0;
-    for (uint32_t i = 0; i < n; ++i) {
-        double t = 0.0;
-        for (uint32_t k = As->h_rp[i]; k < As->h_rp[i + 1]; ++k) t += std::fabs(As->h_v[k]);
-        R->anorm = std::max(R->anorm, t);
+    {
+        std::mutex mu;
+        parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
+            double m = 0.0;
+            for (size_t i = i0; i < i1; ++i) {
+                double t = 0.0;
+                for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) t += std::fabs(av[k]);
+                m = std::max(m, t);
+            }
+            std::lock_guard<std::mutex> g(mu);
+            R->anorm = std::max(R->anorm, m);
+        });
     }
-    Rows KA, KB, KF, KE;
-    for (uint32_t i = 0; i < n0; ++i) {
-        KE.put(i, 1.0 / d[i]);
-        for (uint32_t k = As->h_rp[i]; k < As->h_rp[i + 1]; ++k) {
-            const uint32_t j = As->h_ci[k];
-            if (j == i) continue;
-            const double a = As->h_v[k];
-            KA.put(j, a / std::sqrt(d[i] * dt[j - n0]));
-            KE.put(j, -a / d[i]);
-        }
-        KA.end_row();
-        KE.end_row();
-    }
+    // every row holds one diagonal: row lengths of the four operators follow
+    // from A's (KA: the off-diagonals; KE, KB, KF: the off-diagonals and one more)
+    auto rows_of = [&](uint32_t r0, uint32_t r1, uint32_t extra) {
+        std::vector<uint32_t> p(r1 - r0 + 1, 0);
+        for (uint32_t i = r0; i < r1; ++i) p[i - r0 + 1] = p[i - r0] + (rp[i + 1] - rp[i] - 1) + extra;
+        return p;
+    };
+    std::vector<uint32_t> ka_rp = rows_of(0, n0, 0), ke_rp = rows_of(0, n0, 1), kb_rp = rows_of(n0, n, 1),
+                          kf_rp = rows_of(n0, n, 1);
+    std::vector<uint32_t> ka_ci(ka_rp.back()), ke_ci(ke_rp.back()), kb_ci(kb_rp.back()), kf_ci(kf_rp.back());
+    std::vector<double> ka_v(ka_rp.back()), ke_v(ke_rp.back()), kb_v(kb_rp.back()), kf_v(kf_rp.back());
     std::vector<double> s1(n1);
-    for (uint32_t i = n0; i < n; ++i) {
-        const double t = dt[i - n0], rs = 1.0 / std::sqrt(t);
-        for (uint32_t k = As->h_rp[i]; k < As->h_rp[i + 1]; ++k) {
-            const uint32_t j = As->h_ci[k];
-            if (j == i) continue;
-            const double a = As->h_v[k];
-            KB.put(j, -a / std::sqrt(d[j] * t));
-            KF.put(j, -a / d[j] * rs);
+    parallel_for(n0, 1u << 15, [&](size_t i0, size_t i1) {
+        for (uint32_t i = (uint32_t)i0; i < (uint32_t)i1; ++i) {
+            uint32_t pa = ka_rp[i], pe = ke_rp[i];
+            ke_ci[pe] = i;
+            ke_v[pe++] = 1.0 / d[i];
+            for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const uint32_t j = ci[k];
+                if (j == i) continue;
+                const double a = av[k];
+                ka_ci[pa] = j;
+                ka_v[pa++] = a / std::sqrt(d[i] * dt[j - n0]);
+                ke_ci[pe] = j;
+                ke_v[pe++] = -a / d[i];
+            }
         }
-        KB.put(i, d[i] / t);  // the diagonal column i > every first-colour column
-        KF.put(i, rs);
-        KB.end_row();
-        KF.end_row();
-        s1[i - n0] = rs;
-    }
+    });
+    parallel_for(n1, 1u << 15, [&](size_t q0, size_t q1) {
+        for (uint32_t q = (uint32_t)q0; q < (uint32_t)q1; ++q) {
+            const uint32_t i = n0 + q;
+            const double t = dt[q], rs = 1.0 / std::sqrt(t);
+            uint32_t pb = kb_rp[q], pf = kf_rp[q];
+            for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const uint32_t j = ci[k];
+                if (j == i) continue;
+                const double a = av[k];
+                kb_ci[pb] = j;
+                kb_v[pb++] = -a / std::sqrt(d[j] * t);
+                kf_ci[pf] = j;
+                kf_v[pf++] = -a / d[j] * rs;
+            }
+            kb_ci[pb] = i;  // the diagonal column i > every first-colour column
+            kb_v[pb] = d[i] / t;
+            kf_ci[pf] = i;
+            kf_v[pf] = rs;
+            s1[q] = rs;
+        }
+    });
     try {
         R->n0 = n0;
         R->n1 = n1;
-        R->KA = csr_new(ctx, n0, n, KA.rp.data(), KA.ci.data(), KA.v.data());
-        R->KB = csr_new(ctx, n1, n, KB.rp.data(), KB.ci.data(), KB.v.data());
-        R->KF = csr_new(ctx, n1, n, KF.rp.data(), KF.ci.data(), KF.v.data());
-        R->KE = csr_new(ctx, n0, n, KE.rp.data(), KE.ci.data(), KE.v.data());
+        // KA and KB run every iteration; KF and KE once per refinement step
+        R->KA = csr_from_vectors(ctx, n0, n, std::move(ka_rp), std::move(ka_ci), std::move(ka_v), true);
+        R->KB = csr_from_vectors(ctx, n1, n, std::move(kb_rp), std::move(kb_ci), std::move(kb_v), true);
+        R->KF = csr_from_vectors(ctx, n1, n, std::move(kf_rp), std::move(kf_ci), std::move(kf_v), false);
+        R->KE = csr_from_vectors(ctx, n0, n, std::move(ke_rp), std::move(ke_ci), std::move(ke_v), false);
         R->s1_64 = dalloc<double>(n1);
         PP_HIP(hipMemcpy(R->s1_64, s1.data(), n1 * sizeof(double), hipMemcpyHostToDevice));
         R->dt_64 = dalloc<double>(n1);

@@ -84,6 +84,79 @@ def test_factorization_failures(pp):
     assert pp.host_factor(Z, "ilut", 0.0, 5)["status"] == PP_ERR_ZERO_ROW
 
 
+def shuffled_rows(A, seed):
+    # the same matrix with the entries of every row in a random order
+    rp, ci, v = A
+    rng = np.random.default_rng(seed)
+    ci2, v2 = ci.copy(), v.copy()
+    for i in range(len(rp) - 1):
+        s = rng.permutation(int(rp[i + 1] - rp[i])) + rp[i]
+        ci2[rp[i]:rp[i + 1]] = ci[s]
+        v2[rp[i]:rp[i + 1]] = v[s]
+    return rp, ci2, v2
+
+
+def upper_extra(A, shift):
+    # an entry (i, i + shift) that has no mirror (i + shift, i): a pattern
+    # that is not symmetric; IC(0) reads only the lower part
+    rp, ci, v = A
+    n = len(rp) - 1
+    rows = []
+    for i in range(n):
+        r = list(zip(ci[rp[i]:rp[i + 1]].tolist(), v[rp[i]:rp[i + 1]].tolist()))
+        if i + shift < n and all(c != i + shift for c, _ in r):
+            r.append((i + shift, -0.01))
+        rows.append(sorted(r))
+    rp2 = np.zeros(n + 1, np.uint32)
+    rp2[1:] = np.cumsum([len(r) for r in rows])
+    return (rp2, np.array([c for r in rows for c, _ in r], np.uint32), np.array([x for r in rows for _, x in r]))
+
+
+@pytest.mark.parametrize("case", ["sorted", "shuffled", "nonsymmetric"])
+@pytest.mark.parametrize("dims", [(9, 8, 7), (82, 81, 80)])
+def test_ic0_row_layouts(pp, case, dims):
+    # sorted rows with a symmetric pattern take the level-parallel merge and
+    # the scatter-free L^T; shuffled rows the serial slot table; a pattern
+    # that is not symmetric the transpose (parallel beyond 2^20 entries):
+    # all bit for bit the oracle's
+    A = ppo.laplacian(*dims)
+    if case == "shuffled":
+        A = shuffled_rows(A, 3)
+    elif case == "nonsymmetric":
+        A = upper_extra(A, 2)
+    f = pp.host_factor(A, "ic0")
+    g = ppo.Prec(A, "ic0").factors()
+    assert f["status"] == 0 and eq(f["lt"], g["lt"])
+
+
+@pytest.mark.parametrize("clique", [0, 70])
+def test_multicolour_order_matches_greedy(pp, clique):
+    # a 70-row clique needs more colours than one 64-bit mask holds: the
+    # adjacency-list colouring; otherwise the one-pass masked colouring
+    from mc_order import greedy_order, permute
+    A = ppo.laplacian(11, 10, 9)
+    if clique:
+        rp, ci, v = A
+        n = len(rp) - 1
+        rows = []
+        for i in range(n):
+            r = dict(zip(ci[rp[i]:rp[i + 1]].tolist(), v[rp[i]:rp[i + 1]].tolist()))
+            if i < clique:
+                for j in range(clique):
+                    if j != i:
+                        r.setdefault(j, -1e-3)
+                r[i] += clique * 1e-3
+            rows.append(sorted(r.items()))
+        A = (np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint32),
+             np.array([c for r in rows for c, _ in r], np.uint32), np.array([x for r in rows for _, x in r]))
+    order, ncol = greedy_order(A)
+    assert (ncol > 64) == (clique > 64)
+    f = pp.host_factor(A, "ic0", ordering=1)
+    g = ppo.Prec(permute(A, order), "ic0").factors()
+    assert f["status"] == 0 and eq(f["lt"], g["lt"])
+    assert f["levels_lower"] == ncol
+
+
 def test_level_counts_of_natural_order_ic0(pp):
     # hyperplanes i+j+k = const of a natural-order 7-point IC(0): 3(n-1)+1 levels
     f = pp.host_factor(ppo.laplacian(9, 8, 7), "ic0")

@@ -211,9 +211,10 @@ bool two_base_offsets(const uint32_t *cols, uint32_t cnt, uint32_t &base_lo, uin
 }
 
 // The device CSR of a host matrix whose arrays it takes over; tune: time
-// the SpMV load variants on it (for matrices in an iteration's hot loop).
+// the SpMV load variants on it (for matrices in an iteration's hot loop):
+// 0 no, 1 with cold caches, 2 with warm caches.
 pp_csr *csr_from_vectors(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, std::vector<uint32_t> &&rp,
-                         std::vector<uint32_t> &&ci, std::vector<double> &&v, bool tune) {
+                         std::vector<uint32_t> &&ci, std::vector<double> &&v, int tune) {
     PP_CHECK(rp.size() == (size_t)nrow + 1, PP_ERR_INVALID, "rowptr size");
     PP_CHECK(rp[0] == 0, PP_ERR_INVALID, "rowptr[0] must be 0");
     const uint32_t nnz = rp[nrow];
@@ -266,7 +267,9 @@ pp_csr *csr_from_vectors(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, std::vector<
                               ctx->stream));
         PP_HIP(hipStreamSynchronize(ctx->stream));  // ds, blocks, off16 leave scope
     }
+    A->tune_warm = tune == 2;
     if (tune) csr_tune_spmv(ctx, A.get(), PP_F64);
+    else A->tuned32 = true;  // neither precision is timed
     return A.release();
 }
 
@@ -276,7 +279,7 @@ pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, c
     const uint32_t nnz = rp[nrow];
     PP_CHECK(nnz == 0 || (ci && v), PP_ERR_INVALID, "colidx/values are NULL");
     return csr_from_vectors(ctx, nrow, ncol, std::vector<uint32_t>(rp, rp + nrow + 1),
-                            std::vector<uint32_t>(ci, ci + nnz), std::vector<double>(v, v + nnz), true);
+                            std::vector<uint32_t>(ci, ci + nnz), std::vector<double>(v, v + nnz), 1);
 }
 
 // Times each SpMV load variant (ordinary / non-temporal operand loads, 32-bit
@@ -292,8 +295,11 @@ static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A, int dtype) {
         int best_var = variant;
         for (int var = 0; var < kVarCount; ++var) {
             if ((var & kVarIdx16) && !A->ci16) continue;
+            // non-temporal loads evict what the iteration's next launch reads
+            if (A->tune_warm && (var & kVarNT)) continue;
             variant = var;
-            const float ms = time_cold_ms(ctx, [&] { launch_spmv(ctx, A, dtype, 1.0, x, 0.0, y, -1, nullptr, false); }, 3);
+            const float ms = time_cold_ms(
+                ctx, [&] { launch_spmv(ctx, A, dtype, 1.0, x, 0.0, y, -1, nullptr, false); }, 3, !A->tune_warm);
             if (ms < best) {
                 best = ms;
                 best_var = var;
@@ -310,7 +316,7 @@ static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A, int dtype) {
 }
 
 pp_csr *csr_create_host(pp_ctx *ctx, HostCsr &&H) {
-    return csr_from_vectors(ctx, H.n, H.n, std::move(H.rp), std::move(H.ci), std::move(H.v), true);
+    return csr_from_vectors(ctx, H.n, H.n, std::move(H.rp), std::move(H.ci), std::move(H.v), 1);
 }
 
 void csr_free(pp_csr *A) {
@@ -425,6 +431,7 @@ void pp_ctx_destroy(pp_ctx *ctx) {
     dfree(ctx->scal);
     dfree(ctx->ctl);
     dfree(ctx->hist);
+    dfree(ctx->flush_buf);
     for (int i = 0; i < 48; ++i) dfree(ctx->ws[i]);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->pinned_ctl) (void)hipHostFree(ctx->pinned_ctl);

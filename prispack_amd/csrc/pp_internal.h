@@ -12,6 +12,7 @@
 
 #include "../../include/prispack_amd.h"
 
+
 namespace pp {
 
 // A failure carried to the C ABI boundary as a status + message.
@@ -126,6 +127,7 @@ struct pp_ctx {
     double *hist = nullptr;       // device residual history
     uint32_t hist_cap = 0;
     double *pinned = nullptr;     // pinned host mirror (scalars, 64 doubles)
+    void *flush_buf = nullptr;    // cache-flush buffer of the variant timers (allocated on first use)
     int *pinned_ctl = nullptr;    // pinned host mirror of ctl
     int *ring_host = nullptr;     // mapped pinned slots the PCG bodies publish ctl into
     int *ring_dev = nullptr;      // the same memory as the device sees it
@@ -187,6 +189,9 @@ struct DevSweep {
     uint64_t nnz16 = 0, nblk = 0, perm_rows = 0;  // entries on 16-bit offsets, row blocks, rows read through perm
     int variant = kVarDefault;                     // kVarNT | kVarIdx16 for the staged kernel
     uint32_t max_level_rows = 0;
+    // the index arrays (perm, rp, ci, blocks, desc, ci16, blk_base) belong to
+    // another sweep of the same rows (the FP32 sweep shares the FP64 one's)
+    bool borrowed = false;
     void free_all();
 };
 
@@ -253,6 +258,9 @@ struct pp_csr {
     int spmv_variant = pp::kVarDefault;    // FP64 loads, chosen by timing at creation for large matrices
     int spmv_variant32 = pp::kVarDefault;  // FP32 loads, chosen by timing when the FP32 copy is made
     bool tuned32 = false;
+    // time the load variants with warm caches: the matrix runs in an
+    // iteration whose operands stay cache-resident between launches
+    bool tune_warm = false;
     uint64_t nnz16 = 0;              // entries in those blocks
     pp_csr *trans = nullptr;         // explicit transpose, built on demand
     std::vector<uint32_t> h_rp, h_ci;
@@ -390,10 +398,11 @@ void precond_apply_after_level0(pp_ctx *ctx, pp_precond *M, int dtype, const voi
 void precond_need_f32(pp_precond *M);
 void csr_need_f32(pp_ctx *ctx, pp_csr *A);
 // device CSR of host arrays (copied / taken over); tune: time the SpMV load
-// variants on it, for matrices in an iteration's hot loop
+// variants on it, for matrices in an iteration's hot loop (0 no, 1 with
+// cold caches, 2 with warm caches: an iteration whose operands stay resident)
 pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci, const double *v);
 pp_csr *csr_from_vectors(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, std::vector<uint32_t> &&rp,
-                         std::vector<uint32_t> &&ci, std::vector<double> &&v, bool tune);
+                         std::vector<uint32_t> &&ci, std::vector<double> &&v, int tune);
 pp_csr *csr_create_host(pp_ctx *ctx, HostCsr &&H);
 void csr_free(pp_csr *A);
 
@@ -440,7 +449,9 @@ void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes);
 // Average duration (ms) of fn()'s launches on ctx->stream with the last-level
 // cache cleared before each one (a read of `flush_bytes` of other memory), so
 // that operands are streamed from HBM as they are inside a solve.
-float time_cold_ms(pp_ctx *ctx, const std::function<void()> &fn, int reps);
+// mean time of fn over reps launches after one warm-up; cold: the caches
+// (L2 and the last-level MALL) flushed before each
+float time_cold_ms(pp_ctx *ctx, const std::function<void()> &fn, int reps, bool cold = true);
 // Host setup work split over threads: fn(begin, end) on contiguous ranges of
 // [0, n), at most 16 threads and at least `grain` items each (small n runs
 // on the calling thread).  fn must only write what its range owns.

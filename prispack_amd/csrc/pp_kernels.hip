@@ -404,29 +404,27 @@ __global__ __launch_bounds__(256) void k_flush(const uint4 *__restrict__ p, size
     if (acc == 0xFFFFFFFFu) *sink = acc;
 }
 
-float time_cold_ms(pp_ctx *ctx, const std::function<void()> &fn, int reps) {
+float time_cold_ms(pp_ctx *ctx, const std::function<void()> &fn, int reps, bool cold) {
     constexpr size_t kFlushBytes = 384ull << 20;
-    uint4 *buf = dalloc<uint4>(kFlushBytes / sizeof(uint4) + 1);
+    if (cold && !ctx->flush_buf) {
+        ctx->flush_buf = dalloc<uint4>(kFlushBytes / sizeof(uint4) + 1);
+        launch_fill_zero(ctx, ctx->flush_buf, kFlushBytes + sizeof(uint4));
+    }
+    uint4 *buf = (uint4 *)ctx->flush_buf;
     float total = 0.f;
-    try {
-        launch_fill_zero(ctx, buf, kFlushBytes + sizeof(uint4));
-        fn();  // warm-up (code, TLB)
-        for (int r = 0; r < reps; ++r) {
+    fn();  // warm-up (code, TLB)
+    for (int r = 0; r < reps; ++r) {
+        if (cold)
             k_flush<<<ctx->cu_count * 8, 256, 0, ctx->stream>>>(buf, kFlushBytes / sizeof(uint4),
                                                                  (uint32_t *)(buf + kFlushBytes / sizeof(uint4)));
-            PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-            fn();
-            PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-            PP_HIP(hipEventSynchronize(ctx->ev1));
-            float ms = 0.f;
-            PP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            total += ms;
-        }
-    } catch (...) {
-        dfree(buf);
-        throw;
+        PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        fn();
+        PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        PP_HIP(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        PP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        total += ms;
     }
-    dfree(buf);
     return total / reps;
 }
 

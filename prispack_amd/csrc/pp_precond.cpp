@@ -16,15 +16,18 @@ void ml_need_f32(pp_ctx *ctx, pp_precond *M);
 
 template <class T>
 void DevSweep<T>::free_all() {
-    dfree(perm);
-    dfree(rp);
-    dfree(ci);
+    if (!borrowed) {
+        dfree(perm);
+        dfree(rp);
+        dfree(ci);
+        dfree(blocks);
+        dfree(ci16);
+        dfree(blk_base);
+        dfree(desc);
+    }
+    borrowed = false;
     dfree(val);
     dfree(diag);
-    dfree(blocks);
-    dfree(ci16);
-    dfree(blk_base);
-    dfree(desc);
     blocks = nullptr;
     ci16 = nullptr;
     blk_base = nullptr;
@@ -158,6 +161,38 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
         PP_HIP(hipMemcpy(d.diag, dg.data(), n * sizeof(T), hipMemcpyHostToDevice));
     }
 }
+// The FP32 sweep of the same rows as an uploaded FP64 sweep: the index
+// arrays are shared, the values and pivots rounded to FP32 on the device
+// (round to nearest, as a host cast).
+static void derive_sweep_f32(pp_ctx *ctx, const DevSweep<double> &s, DevSweep<float> &d) {
+    d.free_all();
+    d.n = s.n;
+    d.nnz = s.nnz;
+    d.nlev = s.nlev;
+    d.perm = s.perm;
+    d.rp = s.rp;
+    d.ci = s.ci;
+    d.blocks = s.blocks;
+    d.desc = s.desc;
+    d.ci16 = s.ci16;
+    d.blk_base = s.blk_base;
+    d.borrowed = true;
+    d.level_ptr = s.level_ptr;
+    d.level_off = s.level_off;
+    d.level_blk = s.level_blk;
+    d.nnz16 = s.nnz16;
+    d.nblk = s.nblk;
+    d.perm_rows = s.perm_rows;
+    d.max_level_rows = s.max_level_rows;
+    d.val = dalloc<float>(s.nnz);
+    if (s.nnz) launch_convert(ctx, s.nnz, PP_F64, s.val, PP_F32, d.val);
+    if (s.diag) {
+        d.diag = dalloc<float>(s.n);
+        launch_convert(ctx, s.n, PP_F64, s.diag, PP_F32, d.diag);
+    }
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 void ChunkSet::free_all() {
     dfree(dstart);
     dstart = nullptr;
@@ -481,8 +516,8 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload, Host
         M->lvl0_hi = (uint32_t)((int64_t)M->low64.level_ptr[1] + M->low64.level_off[0]);
     }
     if (M->opts.make_f32) {
-        upload_sweep(M->hlow, M->low32);
-        upload_sweep(M->hup, M->up32);
+        derive_sweep_f32(M->ctx, M->low64, M->low32);
+        derive_sweep_f32(M->ctx, M->up64, M->up32);
         M->have32 = true;
     }
     if (perm_out && Ap == &Aperm) *perm_out = std::move(Aperm);
@@ -527,8 +562,8 @@ void precond_need_f32(pp_precond *M) {
     if (!M->order.empty() && !M->xp32) M->xp32 = dalloc<float>(M->n);
     if (M->trsv == PP_TRSV_SYNCFREE && !M->tmp32) M->tmp32 = dalloc<float>(M->n);
     if (M->have32) return;
-    upload_sweep(M->hlow, M->low32);
-    upload_sweep(M->hup, M->up32);
+    derive_sweep_f32(M->ctx, M->low64, M->low32);
+    derive_sweep_f32(M->ctx, M->up64, M->up32);
     M->low32.variant = M->up32.variant = M->sweep_variant;
     if (M->trsv == PP_TRSV_LEVELS && M->n >= (1u << 20) && (M->low32.blocks || M->up32.blocks))
         tune_sweep_pair<float>(M->ctx, M, M->low32, M->up32, M->sweep_variant);

@@ -166,11 +166,13 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
     try {
         R->n0 = n0;
         R->n1 = n1;
-        // KA and KB run every iteration; KF and KE once per refinement step
-        R->KA = csr_from_vectors(ctx, n0, n, std::move(ka_rp), std::move(ka_ci), std::move(ka_v), true);
-        R->KB = csr_from_vectors(ctx, n1, n, std::move(kb_rp), std::move(kb_ci), std::move(kb_v), true);
-        R->KF = csr_from_vectors(ctx, n1, n, std::move(kf_rp), std::move(kf_ci), std::move(kf_v), false);
-        R->KE = csr_from_vectors(ctx, n0, n, std::move(ke_rp), std::move(ke_ci), std::move(ke_v), false);
+        // KA and KB run every iteration, whose operands (the two matrices and
+        // five vectors of the second colour) stay cache-resident between
+        // launches; KF and KE run once per refinement step
+        R->KA = csr_from_vectors(ctx, n0, n, std::move(ka_rp), std::move(ka_ci), std::move(ka_v), 2);
+        R->KB = csr_from_vectors(ctx, n1, n, std::move(kb_rp), std::move(kb_ci), std::move(kb_v), 2);
+        R->KF = csr_from_vectors(ctx, n1, n, std::move(kf_rp), std::move(kf_ci), std::move(kf_v), 0);
+        R->KE = csr_from_vectors(ctx, n0, n, std::move(ke_rp), std::move(ke_ci), std::move(ke_v), 0);
         R->s1_64 = dalloc<double>(n1);
         PP_HIP(hipMemcpy(R->s1_64, s1.data(), n1 * sizeof(double), hipMemcpyHostToDevice));
         R->dt_64 = dalloc<double>(n1);

@@ -55,8 +55,8 @@ def parse():
 # (FETCH_SIZE, WRITE_SIZE, KiB per dispatch) over this script, each scaled by
 # what the same counter reads for a device-to-device copy of a known size
 # (--copy-probe).  Keyed by (grid side, SpMV storage variant); the command and
-# the raw per-dispatch figures are in profiles/k_spmv_poisson128_pmc.txt.
-PMC_TRAFFIC = {(128, v): b for v, b in enumerate((227268535, 227369993, 197342847, 197785973, 226217978, 226336674, 194936974, 194955704))}
+# the raw per-dispatch figures are in profiles/k_spmv_poisson128_pmc.md.
+PMC_TRAFFIC = {(128, v): b for v, b in enumerate((224567189, 226682332, 194624398, 197270641, 219109458, 229230402, 189935329, 192882023))}
 
 
 def measured_traffic(grid, variant):

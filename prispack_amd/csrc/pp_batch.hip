@@ -73,30 +73,81 @@ __device__ inline double block_reduce(double v, double *sh) {
     return t;  // every thread holds the total (fixed order)
 }
 
-// y = A x on the rows [r0, r1) with values V (FP64 or FP32)
+// Rows handled per thread and pass: their index, entry and vector loads are
+// issued together (a system's workgroup is alone on its CU, so its
+// latency is hidden by loads in flight, not by other waves).
+constexpr int kRowsInFlight = 4;
+
+// y = A x on the rows [r0, r1) with values V (FP64 or FP32); each row's sum
+// in storage order
 template <class T, class V>
 __device__ inline void sys_spmv(const uint32_t *rp, const uint32_t *ci, const V *val, const T *x, T *y,
                                 uint32_t r0, uint32_t r1) {
-    for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
-        T s = 0;
-        for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) s += (T)val[k] * x[ci[k]];
-        y[i] = s;
+    constexpr int U = kRowsInFlight;
+    const uint32_t step = blockDim.x;
+    for (uint32_t i0 = r0 + threadIdx.x; i0 < r1; i0 += U * step) {
+        uint32_t k[U], e[U];
+        T sum[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * step;
+            k[u] = i < r1 ? rp[i] : 0;
+            e[u] = i < r1 ? rp[i + 1] : 0;
+            sum[u] = 0;
+        }
+        for (bool more = true; more;) {
+            more = false;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k[u] < e[u]) {
+                    sum[u] += (T)val[k[u]] * x[ci[k[u]]];
+                    ++k[u];
+                    more = true;
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * step;
+            if (i < r1) y[i] = sum[u];
+        }
     }
 }
 
 // Level-by-level sweep of one system; y may alias b (rows of a level only read
-// rows of earlier levels, finished before the barrier).  mode 0: divide by
-// diag (IC(0) both sweeps).
+// rows of earlier levels, finished before the barrier, and each row reads its
+// own b before writing its y).  mode 0: divide by diag (IC(0) both sweeps).
 template <class T>
 __device__ inline void sys_sweep(const uint32_t *perm, const uint32_t *rp, const uint32_t *ci, const T *val,
                                  const T *diag, const uint32_t *lev, uint32_t l0, uint32_t l1, const T *b, T *y) {
+    constexpr int U = kRowsInFlight;
+    const uint32_t step = blockDim.x;
     for (uint32_t l = l0; l < l1; ++l) {
         const uint32_t k1 = lev[l + 1];
-        for (uint32_t k = lev[l] + threadIdx.x; k < k1; k += blockDim.x) {
-            const uint32_t i = perm[k];
-            T s = b[i];
-            for (uint32_t e = rp[k]; e < rp[k + 1]; ++e) s -= val[e] * y[ci[e]];
-            y[i] = s / diag[i];
+        for (uint32_t k0 = lev[l] + threadIdx.x; k0 < k1; k0 += U * step) {
+            uint32_t row[U], e[U], f[U];
+            T sum[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t k = k0 + u * step;
+                const bool ok = k < k1;
+                row[u] = ok ? perm[k] : 0;
+                e[u] = ok ? rp[k] : 0;
+                f[u] = ok ? rp[k + 1] : 0;
+                sum[u] = ok ? b[row[u]] : (T)0;
+            }
+            for (bool more = true; more;) {
+                more = false;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (e[u] < f[u]) {
+                        sum[u] -= val[e[u]] * y[ci[e[u]]];
+                        ++e[u];
+                        more = true;
+                    }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k0 + u * step < k1) y[row[u]] = sum[u] / diag[row[u]];
         }
         __syncthreads();
     }

@@ -415,6 +415,19 @@ void append_sweep(HostBatchSweep &H, const HostCsr &R, bool lower, bool diag_fir
     nlev_out = nlev;
 }
 
+// Concatenate a part built by append_sweep into an empty HostBatchSweep
+// onto H (positions, entry offsets and level pointers shifted).
+void append_parts(HostBatchSweep &H, const HostBatchSweep &P) {
+    const uint32_t bp = (uint32_t)H.perm.size(), be = (uint32_t)H.ci.size(), bl = (uint32_t)H.lev.size();
+    H.perm.insert(H.perm.end(), P.perm.begin(), P.perm.end());
+    H.ci.insert(H.ci.end(), P.ci.begin(), P.ci.end());
+    H.val.insert(H.val.end(), P.val.begin(), P.val.end());
+    H.diag.insert(H.diag.end(), P.diag.begin(), P.diag.end());
+    for (size_t q = 1; q < P.rp.size(); ++q) H.rp.push_back(be + P.rp[q]);
+    for (uint32_t l : P.lev) H.lev.push_back(bp + l);
+    for (size_t q = 1; q < P.lev_off.size(); ++q) H.lev_off.push_back(bl + P.lev_off[q]);
+}
+
 template <class T>
 T *upload(const std::vector<T> &h) {
     T *d = dalloc<T>(h.size());
@@ -442,47 +455,48 @@ void upload_batch_sweep(const HostBatchSweep &H, DevBatchSweep &D) {
 void batch_build_order(pp_batch *B) {
     B->h_gperm.assign(B->ntot, 0);
     B->h_prp.assign(B->ntot + 1, 0);
-    B->h_pci.clear();
-    B->vmap.clear();
-    B->h_pci.reserve(B->nnz);
-    B->vmap.reserve(B->nnz);
-    for (uint32_t s = 0; s < B->nsys; ++s) {
-        const uint32_t r0 = B->h_roff[s], r1 = B->h_roff[s + 1], n = r1 - r0;
-        HostCsr A;
-        A.n = n;
-        A.rp.assign(n + 1, 0);
-        for (uint32_t i = 0; i < n; ++i) {
-            for (uint32_t k = B->h_rp[r0 + i]; k < B->h_rp[r0 + i + 1]; ++k) {
-                A.ci.push_back(B->h_ci[k] - r0);
-                A.v.push_back(0.0);
-            }
-            A.rp[i + 1] = (uint32_t)A.ci.size();
-        }
-        uint32_t ncol = 0;
-        const std::vector<uint32_t> order = multicolor_order(A, ncol);
-        std::vector<uint32_t> inv(n);
-        for (uint32_t k = 0; k < n; ++k) inv[order[k]] = k;
+    B->h_pci.assign(B->nnz, 0);
+    B->vmap.assign(B->nnz, 0);
+    // a system's permuted entries occupy the same range as its original ones
+    parallel_for(B->nsys, 1, [&](size_t s0, size_t s1) {
         std::vector<std::pair<uint32_t, uint32_t>> row;  // (new column, original entry)
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t i = order[k];
-            B->h_gperm[r0 + k] = r0 + i;
-            row.clear();
-            for (uint32_t e = B->h_rp[r0 + i]; e < B->h_rp[r0 + i + 1]; ++e)
-                row.emplace_back(inv[B->h_ci[e] - r0], e);
-            std::sort(row.begin(), row.end());
-            for (const auto &pe : row) {
-                B->h_pci.push_back(r0 + pe.first);
-                B->vmap.push_back(pe.second);
+        for (size_t s = s0; s < s1; ++s) {
+            const uint32_t r0 = B->h_roff[s], r1 = B->h_roff[s + 1], n = r1 - r0;
+            HostCsr A;
+            A.n = n;
+            A.rp.assign(n + 1, 0);
+            for (uint32_t i = 0; i < n; ++i) A.rp[i + 1] = B->h_rp[r0 + i + 1] - B->h_rp[r0];
+            A.ci.resize(A.rp[n]);
+            A.v.assign(A.rp[n], 0.0);
+            for (uint32_t k = 0; k < A.rp[n]; ++k) A.ci[k] = B->h_ci[B->h_rp[r0] + k] - r0;
+            uint32_t ncol = 0;
+            const std::vector<uint32_t> order = multicolor_order(A, ncol);
+            std::vector<uint32_t> inv(n);
+            for (uint32_t k = 0; k < n; ++k) inv[order[k]] = k;
+            uint32_t q = B->h_rp[r0];
+            for (uint32_t k = 0; k < n; ++k) {
+                const uint32_t i = order[k];
+                B->h_gperm[r0 + k] = r0 + i;
+                row.clear();
+                for (uint32_t e = B->h_rp[r0 + i]; e < B->h_rp[r0 + i + 1]; ++e)
+                    row.emplace_back(inv[B->h_ci[e] - r0], e);
+                std::sort(row.begin(), row.end());
+                for (const auto &pe : row) {
+                    B->h_pci[q] = r0 + pe.first;
+                    B->vmap[q++] = pe.second;
+                }
+                B->h_prp[r0 + k + 1] = q;
             }
-            B->h_prp[r0 + k + 1] = (uint32_t)B->h_pci.size();
         }
-    }
+    });
 }
 
 // values of the permuted copy from the original values
 void batch_upload_permuted_values(pp_batch *B) {
     std::vector<double> pv(B->nnz);
-    for (uint32_t e = 0; e < B->nnz; ++e) pv[e] = B->h_v[B->vmap[e]];
+    parallel_for(B->nnz, 1u << 16, [&](size_t e0, size_t e1) {
+        for (size_t e = e0; e < e1; ++e) pv[e] = B->h_v[B->vmap[e]];
+    });
     if (!B->pv64) B->pv64 = dalloc<double>(B->nnz);
     if (!B->pv32) B->pv32 = dalloc<float>(B->nnz);
     PP_HIP(hipMemcpy(B->pv64, pv.data(), B->nnz * sizeof(double), hipMemcpyHostToDevice));
@@ -493,34 +507,47 @@ void batch_upload_permuted_values(pp_batch *B) {
 // IC(0) of every diagonal block (of the permuted copy under multicolour
 // ordering), as two gather-form sweeps.
 void batch_factor(pp_batch *B) {
-    HostBatchSweep lo, up;
-    B->max_levels = 0;
     const bool perm = B->ordering == PP_ORDER_MULTICOLOR;
     const std::vector<uint32_t> &Rp = perm ? B->h_prp : B->h_rp;
     const std::vector<uint32_t> &Ci = perm ? B->h_pci : B->h_ci;
-    for (uint32_t s = 0; s < B->nsys; ++s) {
-        const uint32_t r0 = B->h_roff[s], r1 = B->h_roff[s + 1], n = r1 - r0;
-        HostCsr A;
-        A.n = n;
-        A.rp.assign(n + 1, 0);
-        for (uint32_t i = 0; i < n; ++i) {
-            for (uint32_t k = Rp[r0 + i]; k < Rp[r0 + i + 1]; ++k) {
-                A.ci.push_back(Ci[k] - r0);
-                A.v.push_back(B->h_v[perm ? B->vmap[k] : k]);
+    std::vector<HostBatchSweep> los(B->nsys), ups(B->nsys);
+    std::vector<uint32_t> nlev(B->nsys, 0), fail_row(B->nsys, 0);
+    std::vector<int> status(B->nsys, PP_OK);
+    // systems factor independently; their sweeps are concatenated in order
+    parallel_for(B->nsys, 1, [&](size_t s0, size_t s1) {
+        for (size_t s = s0; s < s1; ++s) {
+            const uint32_t r0 = B->h_roff[s], r1 = B->h_roff[s + 1], n = r1 - r0;
+            HostCsr A;
+            A.n = n;
+            A.rp.assign(n + 1, 0);
+            for (uint32_t i = 0; i < n; ++i) A.rp[i + 1] = Rp[r0 + i + 1] - Rp[r0];
+            A.ci.resize(A.rp[n]);
+            A.v.resize(A.rp[n]);
+            for (uint32_t k = 0; k < A.rp[n]; ++k) {
+                const uint32_t e = Rp[r0] + k;
+                A.ci[k] = Ci[e] - r0;
+                A.v[k] = B->h_v[perm ? B->vmap[e] : e];
             }
-            A.rp[i + 1] = (uint32_t)A.ci.size();
+            HostCsr lt;
+            status[s] = factor_ic0(A, lt, fail_row[s]);
+            if (status[s] != PP_OK) continue;
+            const HostCsr L = transpose(lt);
+            uint32_t nl = 0, nu = 0;
+            append_sweep(los[s], L, true, false, r0, nl);
+            append_sweep(ups[s], lt, false, true, r0, nu);
+            nlev[s] = std::max(nl, nu);
         }
-        HostCsr lt;
-        uint32_t fail = 0;
-        int st = factor_ic0(A, lt, fail);
-        if (st != PP_OK)
-            throw Error(st, "batch system " + std::to_string(s) + ": IC(0) pivot not positive at row " +
-                                std::to_string(fail));
-        HostCsr L = transpose(lt);
-        uint32_t nl = 0, nu = 0;
-        append_sweep(lo, L, true, false, r0, nl);
-        append_sweep(up, lt, false, true, r0, nu);
-        B->max_levels = std::max(B->max_levels, std::max(nl, nu));
+    });
+    for (uint32_t s = 0; s < B->nsys; ++s)  // the first failing system, as a serial loop reports it
+        if (status[s] != PP_OK)
+            throw Error(status[s], "batch system " + std::to_string(s) + ": IC(0) pivot not positive at row " +
+                                       std::to_string(fail_row[s]));
+    HostBatchSweep lo, up;
+    B->max_levels = 0;
+    for (uint32_t s = 0; s < B->nsys; ++s) {
+        append_parts(lo, los[s]);
+        append_parts(up, ups[s]);
+        B->max_levels = std::max(B->max_levels, nlev[s]);
     }
     upload_batch_sweep(lo, B->lo);
     upload_batch_sweep(up, B->up);
@@ -636,13 +663,15 @@ int pp_batch_create(pp_ctx *ctx, uint32_t nsys, const uint32_t *row_off, const u
         B->h_ci.assign(colidx, colidx + B->nnz);
         B->h_v.assign(values, values + B->nnz);
         // block-diagonal and symmetric (IC(0)-CG needs SPD blocks)
-        for (uint32_t s = 0; s < nsys; ++s) {
+        for (uint32_t s = 0; s < nsys; ++s)
             PP_CHECK(row_off[s] <= row_off[s + 1], PP_ERR_INVALID, "row_off not monotone");
-            for (uint32_t i = row_off[s]; i < row_off[s + 1]; ++i)
-                for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
-                    PP_CHECK(colidx[k] >= row_off[s] && colidx[k] < row_off[s + 1], PP_ERR_INVALID,
-                             "entry outside its system's diagonal block");
-        }
+        parallel_for(nsys, 1, [&](size_t s0, size_t s1) {
+            for (size_t s = s0; s < s1; ++s)
+                for (uint32_t i = row_off[s]; i < row_off[s + 1]; ++i)
+                    for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+                        PP_CHECK(colidx[k] >= row_off[s] && colidx[k] < row_off[s + 1], PP_ERR_INVALID,
+                                 "entry outside its system's diagonal block");
+        });
         B->roff = upload(B->h_roff);
         B->rp = upload(B->h_rp);
         B->ci = upload(B->h_ci);

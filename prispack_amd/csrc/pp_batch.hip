@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -51,6 +52,14 @@ struct BatchArgs {
     double *rr64, *w64, *d64, *p64;             // FP64 mode: residual, M^-1 r, direction, A d
     double tol, inner_tol;
     uint32_t maxit, max_outer;
+    // reduced form of every system's two-colour IC(0) (mixed inner solves),
+    // or red_n0 == nullptr: per system its n0 first-colour rows come first;
+    // kab: KA rows (first colour) then KB rows (second colour), kfe: KE rows
+    // then KF rows, all in batch row and column numbering; s1, dt over the
+    // second colour's positions
+    const uint32_t *red_n0;
+    const uint32_t *kab_rp, *kab_ci, *kfe_rp, *kfe_ci;
+    const float *kab_v, *kfe_v, *red_s1, *red_dt;
     int32_t *iters;       // [nsys] total inner iterations
     int32_t *outer;       // [nsys]
     int32_t *conv;        // [nsys]
@@ -255,11 +264,119 @@ __device__ int sys_pcg(const BatchArgs &a, uint32_t s, uint32_t r0, uint32_t r1,
     return (int)m + 1;
 }
 
+// three sums at once (fixed order); sh holds 3 * BS / 64 doubles
+template <int BS>
+__device__ inline void block_reduce3(double &a, double &b, double &c, double *sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+        c += __shfl_down(c, off, 64);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) {
+        sh[wid] = a;
+        sh[BS / 64 + wid] = b;
+        sh[2 * BS / 64 + wid] = c;
+    }
+    __syncthreads();
+    a = b = c = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) {
+        a += sh[w];
+        b += sh[BS / 64 + w];
+        c += sh[2 * BS / 64 + w];
+    }
+}
+
+// CG on the reduced system of system s's two-colour IC(0) (the same
+// iteration as the single-system reduced CG: single-reduction recurrence,
+// u = r): solves A e = rin approximately to the relative reduction tol of
+// the inner system's true residual sqrt(sum Dt r^2) / ||rin||.  z holds
+// y0 = KA r on the first colour and r on the second; x accumulates in e's
+// second-colour part.  Returns the number of residual checks.
+template <int BS>
+__device__ int sys_red_cg(const BatchArgs &a, uint32_t s, uint32_t r0, uint32_t r1, const float *rin, float *e,
+                          float *z, float *w, float *p, float *sv, double tol, uint32_t maxit, double *sh,
+                          double *sh3, int *converged) {
+    const uint32_t m0 = r0 + a.red_n0[s];
+    // r = KF rin (second colour); x = p = s = 0; ||rin||
+    sys_spmv<float, float>(a.kfe_rp, a.kfe_ci, a.kfe_v, rin, z, m0, r1);
+    double bb = 0.0;
+    for (uint32_t i = r0 + threadIdx.x; i < r1; i += BS) bb += (double)rin[i] * (double)rin[i];
+    for (uint32_t i = m0 + threadIdx.x; i < r1; i += BS) {
+        e[i] = 0.f;
+        p[i] = 0.f;
+        sv[i] = 0.f;
+    }
+    const double normb = sqrt(block_reduce<BS>(bb, sh));
+    __syncthreads();
+    *converged = 0;
+    uint32_t m = maxit;
+    double gold = 0.0, aold = 1.0;
+    bool first = true;
+    for (uint32_t c = 1;; ++c) {
+        sys_spmv<float, float>(a.kab_rp, a.kab_ci, a.kab_v, z, z, r0, m0);  // y0 = KA r
+        __syncthreads();
+        // w = KB [y0; r] with (r, w), (r, r) and sum Dt r^2
+        double delta = 0.0, gamma = 0.0, rho = 0.0;
+        sys_spmv<float, float>(a.kab_rp, a.kab_ci, a.kab_v, z, w, m0, r1);
+        for (uint32_t i = m0 + threadIdx.x; i < r1; i += BS) {
+            const float ri = z[i];
+            delta += (double)(ri * w[i]);
+            gamma += (double)(ri * ri);
+            rho += (double)a.red_dt[i] * (double)(ri * ri);
+        }
+        block_reduce3<BS>(delta, gamma, rho, sh3);
+        if (c >= maxit) break;
+        const double norm = normb > 0.0 ? sqrt(rho) / normb : 0.0;
+        if (!(norm == norm) || isinf(norm)) break;
+        if (norm < tol) {
+            *converged = 1;
+            m = c;
+            break;
+        }
+        const double beta = first ? 0.0 : gamma / gold;
+        const double den = first ? delta : delta - beta * gamma / aold;
+        if (den == 0.0 || !(den == den)) break;
+        const double alpha = gamma / den;
+        gold = gamma;
+        aold = alpha;
+        first = false;
+        const float fa = (float)alpha, fb = (float)beta;
+        for (uint32_t i = m0 + threadIdx.x; i < r1; i += BS) {
+            const float ri = z[i];
+            const float pn = ri + fb * p[i];
+            const float sn = w[i] + fb * sv[i];
+            p[i] = pn;
+            sv[i] = sn;
+            e[i] += fa * pn;
+            z[i] = ri - fa * sn;
+        }
+        __syncthreads();
+    }
+    // e = [KE [rin_0; e_1]; e_1], e_1 = s1 x
+    __syncthreads();
+    for (uint32_t i = m0 + threadIdx.x; i < r1; i += BS) {
+        const float ei = a.red_s1[i] * e[i];
+        e[i] = ei;
+        z[i] = ei;
+    }
+    for (uint32_t i = r0 + threadIdx.x; i < m0; i += BS) z[i] = rin[i];
+    __syncthreads();
+    sys_spmv<float, float>(a.kfe_rp, a.kfe_ci, a.kfe_v, z, e, r0, m0);
+    __syncthreads();
+    return (int)m;
+}
+
 // One workgroup per system.  MIXED: FP64 residual r = b - A x, FP32 inner
-// FlexibleCG on A e = r / ||r||, x += ||r|| e, until ||b - A x|| / ||b|| < tol.
+// FlexibleCG on A e = r / ||r|| (or the reduced CG where every system has
+// the two-colour structure), x += ||r|| e, until ||b - A x|| / ||b|| < tol.
 template <int BS, bool MIXED>
 __global__ __launch_bounds__(BS) void k_batch_pcg(BatchArgs a) {
     __shared__ double sh[BS / 64];
+    __shared__ double sh3[3 * (BS / 64)];
     const uint32_t s = blockIdx.x;
     if (s >= a.nsys) return;
     const uint32_t r0 = a.roff[s], r1 = a.roff[s + 1];
@@ -309,8 +426,12 @@ __global__ __launch_bounds__(BS) void k_batch_pcg(BatchArgs a) {
             __syncthreads();
             const double want = fmax(a.inner_tol, 0.5 * a.tol * normb / rn);
             int ic = 0;
-            total += sys_pcg<float, BS>(a, s, r0, r1, a.r32, a.e32, a.q32, a.w32, a.d32, a.p32, want,
-                                        a.maxit - (uint32_t)total, sh, &ic);
+            if (a.red_n0)
+                total += sys_red_cg<BS>(a, s, r0, r1, a.r32, a.e32, a.q32, a.w32, a.d32, a.p32, want,
+                                        a.maxit - (uint32_t)total, sh, sh3, &ic);
+            else
+                total += sys_pcg<float, BS>(a, s, r0, r1, a.r32, a.e32, a.q32, a.w32, a.d32, a.p32, want,
+                                            a.maxit - (uint32_t)total, sh, &ic);
             ++outer;
             for (uint32_t i = r0 + threadIdx.x; i < r1; i += BS) a.x[i] += rn * (double)a.e32[i];
         }
@@ -353,6 +474,10 @@ struct pp_batch {
     uint32_t *gperm = nullptr, *prp = nullptr, *pci = nullptr;
     double *pv64 = nullptr, *bp = nullptr, *xp = nullptr;
     float *pv32 = nullptr;
+    // the reduced form of every system's two-colour IC(0) (multicolour order
+    // only, and only when every system has the structure; see BatchArgs)
+    uint32_t *red_n0 = nullptr, *kab_rp = nullptr, *kab_ci = nullptr, *kfe_rp = nullptr, *kfe_ci = nullptr;
+    float *kab_v = nullptr, *kfe_v = nullptr, *red_s1 = nullptr, *red_dt = nullptr;
 };
 
 namespace pp {
@@ -504,6 +629,68 @@ void batch_upload_permuted_values(pp_batch *B) {
     PP_HIP(hipMemcpy(B->pv32, p32.data(), B->nnz * sizeof(float), hipMemcpyHostToDevice));
 }
 
+void batch_free_reduced(pp_batch *B) {
+    dfree(B->red_n0); dfree(B->kab_rp); dfree(B->kab_ci); dfree(B->kfe_rp); dfree(B->kfe_ci);
+    dfree(B->kab_v); dfree(B->kfe_v); dfree(B->red_s1); dfree(B->red_dt);
+    B->red_n0 = B->kab_rp = B->kab_ci = B->kfe_rp = B->kfe_ci = nullptr;
+    B->kab_v = B->kfe_v = B->red_s1 = B->red_dt = nullptr;
+}
+
+// The systems' reduced operators in batch numbering (FP32 values: they run
+// only in the mixed solves' inner iterations).
+void batch_upload_reduced(pp_batch *B, const std::vector<ReducedHost> &reds) {
+    const uint32_t ns = B->nsys, nt = B->ntot;
+    std::vector<uint32_t> n0(ns), ab_rp(nt + 1, 0), fe_rp(nt + 1, 0);
+    for (uint32_t s = 0; s < ns; ++s) {
+        const ReducedHost &H = reds[s];
+        const uint32_t r0 = B->h_roff[s];
+        n0[s] = H.n0;
+        for (uint32_t i = 0; i < H.n0; ++i) {
+            ab_rp[r0 + i + 1] = ab_rp[r0 + i] + (H.ka_rp[i + 1] - H.ka_rp[i]);
+            fe_rp[r0 + i + 1] = fe_rp[r0 + i] + (H.ke_rp[i + 1] - H.ke_rp[i]);
+        }
+        for (uint32_t q = 0; q < H.n1; ++q) {
+            const uint32_t i = r0 + H.n0 + q;
+            ab_rp[i + 1] = ab_rp[i] + (H.kb_rp[q + 1] - H.kb_rp[q]);
+            fe_rp[i + 1] = fe_rp[i] + (H.kf_rp[q + 1] - H.kf_rp[q]);
+        }
+    }
+    std::vector<uint32_t> ab_ci(ab_rp[nt]), fe_ci(fe_rp[nt]);
+    std::vector<float> ab_v(ab_rp[nt]), fe_v(fe_rp[nt]), s1(nt, 0.f), dt(nt, 0.f);
+    parallel_for(ns, 1, [&](size_t s0, size_t s1e) {
+        for (size_t s = s0; s < s1e; ++s) {
+            const ReducedHost &H = reds[s];
+            const uint32_t r0 = B->h_roff[s];
+            auto put = [&](const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
+                           const std::vector<double> &v, uint32_t rows, uint32_t row0, const std::vector<uint32_t> &grp,
+                           std::vector<uint32_t> &gci, std::vector<float> &gv) {
+                for (uint32_t i = 0; i < rows; ++i)
+                    for (uint32_t k = rp[i], q = grp[row0 + i]; k < rp[i + 1]; ++k, ++q) {
+                        gci[q] = r0 + ci[k];
+                        gv[q] = (float)v[k];
+                    }
+            };
+            put(H.ka_rp, H.ka_ci, H.ka_v, H.n0, r0, ab_rp, ab_ci, ab_v);
+            put(H.kb_rp, H.kb_ci, H.kb_v, H.n1, r0 + H.n0, ab_rp, ab_ci, ab_v);
+            put(H.ke_rp, H.ke_ci, H.ke_v, H.n0, r0, fe_rp, fe_ci, fe_v);
+            put(H.kf_rp, H.kf_ci, H.kf_v, H.n1, r0 + H.n0, fe_rp, fe_ci, fe_v);
+            for (uint32_t q = 0; q < H.n1; ++q) {
+                s1[r0 + H.n0 + q] = (float)H.s1[q];
+                dt[r0 + H.n0 + q] = (float)H.dt[q];
+            }
+        }
+    });
+    B->red_n0 = upload(n0);
+    B->kab_rp = upload(ab_rp);
+    B->kab_ci = upload(ab_ci);
+    B->kab_v = upload(ab_v);
+    B->kfe_rp = upload(fe_rp);
+    B->kfe_ci = upload(fe_ci);
+    B->kfe_v = upload(fe_v);
+    B->red_s1 = upload(s1);
+    B->red_dt = upload(dt);
+}
+
 // IC(0) of every diagonal block (of the permuted copy under multicolour
 // ordering), as two gather-form sweeps.
 void batch_factor(pp_batch *B) {
@@ -513,6 +700,8 @@ void batch_factor(pp_batch *B) {
     std::vector<HostBatchSweep> los(B->nsys), ups(B->nsys);
     std::vector<uint32_t> nlev(B->nsys, 0), fail_row(B->nsys, 0);
     std::vector<int> status(B->nsys, PP_OK);
+    std::vector<ReducedHost> reds(perm ? B->nsys : 0);
+    std::atomic<bool> red_ok{perm};
     // systems factor independently; their sweeps are concatenated in order
     parallel_for(B->nsys, 1, [&](size_t s0, size_t s1) {
         for (size_t s = s0; s < s1; ++s) {
@@ -531,6 +720,7 @@ void batch_factor(pp_batch *B) {
             HostCsr lt;
             status[s] = factor_ic0(A, lt, fail_row[s]);
             if (status[s] != PP_OK) continue;
+            if (red_ok && !reduced_host(n, A.rp, A.ci, A.v, reds[s])) red_ok = false;
             const HostCsr L = transpose(lt);
             uint32_t nl = 0, nu = 0;
             append_sweep(los[s], L, true, false, r0, nl);
@@ -551,6 +741,8 @@ void batch_factor(pp_batch *B) {
     }
     upload_batch_sweep(lo, B->lo);
     upload_batch_sweep(up, B->up);
+    batch_free_reduced(B);
+    if (red_ok) batch_upload_reduced(B, reds);
 }
 
 BatchArgs make_args(pp_batch *B, const pp_solve_opts &o, const double *b, double *x) {
@@ -574,6 +766,15 @@ BatchArgs make_args(pp_batch *B, const pp_solve_opts &o, const double *b, double
     a.inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-6;
     a.maxit = o.maxit;
     a.max_outer = o.max_outer ? o.max_outer : 20;
+    a.red_n0 = perm ? B->red_n0 : nullptr;
+    a.kab_rp = B->kab_rp;
+    a.kab_ci = B->kab_ci;
+    a.kab_v = B->kab_v;
+    a.kfe_rp = B->kfe_rp;
+    a.kfe_ci = B->kfe_ci;
+    a.kfe_v = B->kfe_v;
+    a.red_s1 = B->red_s1;
+    a.red_dt = B->red_dt;
     a.iters = B->iters;
     a.outer = B->outer;
     a.conv = B->conv;
@@ -623,6 +824,7 @@ void batch_free(pp_batch *B) {
     dfree(B->r32); dfree(B->e32); dfree(B->q32); dfree(B->w32); dfree(B->d32); dfree(B->p32);
     dfree(B->iters); dfree(B->outer); dfree(B->conv); dfree(B->relres);
     dfree(B->gperm); dfree(B->prp); dfree(B->pci); dfree(B->pv64); dfree(B->pv32); dfree(B->bp); dfree(B->xp);
+    batch_free_reduced(B);
 }
 
 }  // namespace
@@ -762,6 +964,7 @@ static int batch_solve_impl(pp_ctx *ctx, pp_batch *B, const pp_solve_opts *opts,
             rep->final_error = rep->true_relres;
             rep->solve_ms = ms;
             rep->status = PP_OK;
+            rep->cg_variant = mixed && a.red_n0 ? PP_CG_REDUCED : PP_CG_REFERENCE;
         }
     });
 }

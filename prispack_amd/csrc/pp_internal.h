@@ -303,6 +303,19 @@ struct ReducedForm {
     const pp_csr *src = nullptr;  // built from this matrix at src_version
     uint64_t src_version = 0;
 };
+// Host arrays of the reduced form of one colour-ordered matrix: KA (n0
+// rows), KB and KF (n1 rows), KE (n0 rows), columns in the matrix's
+// numbering; s1 = Dt^-1/2 and Dt over the second colour.
+struct ReducedHost {
+    uint32_t n0 = 0, n1 = 0;
+    std::vector<uint32_t> ka_rp, ka_ci, kb_rp, kb_ci, kf_rp, kf_ci, ke_rp, ke_ci;
+    std::vector<double> ka_v, kb_v, kf_v, ke_v, s1, dt;
+    double anorm = 0.0;
+};
+// false when the matrix has no two-colour split with diagonal colour
+// blocks, or a pivot of its IC(0) is not positive
+bool reduced_host(uint32_t n, const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
+                  const std::vector<double> &av, ReducedHost &H);
 // The reduced form of M for the colour-ordered matrix As (built once and
 // cached in M), or nullptr when it does not apply.
 ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As);

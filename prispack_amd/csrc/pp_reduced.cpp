@@ -17,10 +17,7 @@ namespace {
 // entry left of its diagonal.  Valid when every first-colour row couples
 // only to the second colour and vice versa, and every row has one diagonal
 // entry.
-bool split_two_colours(const pp_csr *A, uint32_t &n0) {
-    const uint32_t n = A->nrow;
-    const auto &rp = A->h_rp;
-    const auto &ci = A->h_ci;
+bool split_two_colours(uint32_t n, const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci, uint32_t &n0) {
     n0 = n;
     for (uint32_t i = 0; i < n && n0 == n; ++i)
         for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
@@ -45,62 +42,45 @@ bool split_two_colours(const pp_csr *A, uint32_t &n0) {
     return true;
 }
 
-double diag_of(const pp_csr *A, uint32_t i) {
-    for (uint32_t k = A->h_rp[i]; k < A->h_rp[i + 1]; ++k)
-        if (A->h_ci[k] == i) return A->h_v[k];
-    return 0.0;
-}
-
 }  // namespace
 
-ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
-    if (!M || M->opts.kind != PP_PC_IC0 || M->order.empty() || !M->io_permuted || M->multilevel || !As) return nullptr;
-    if (M->reduced && M->reduced->src == As && M->reduced->src_version == As->version)
-        return M->reduced->ok ? M->reduced : nullptr;
-    reduced_free(M);
-    std::unique_ptr<ReducedForm> R(new ReducedForm);
-    R->src = As;
-    R->src_version = As->version;
+bool reduced_host(uint32_t n, const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
+                  const std::vector<double> &av, ReducedHost &H) {
     uint32_t n0 = 0;
-    const uint32_t n = As->nrow;
-    // the factor's first level must be exactly the first colour
-    bool ok = As->nrow == As->ncol && split_two_colours(As, n0) && M->low64.nlev == 2 && M->up64.nlev == 2 &&
-              M->lvl0_contiguous && M->lvl0_lo == 0 && M->lvl0_hi == n0;
-    std::vector<double> d(n), dt;
-    const auto &rp = As->h_rp;
-    const auto &ci = As->h_ci;
-    const auto &av = As->h_v;
-    std::atomic<bool> good{ok};
-    if (ok) {
-        parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
-            for (size_t i = i0; i < i1; ++i) {
-                d[i] = diag_of(As, (uint32_t)i);
-                if (!(d[i] > 0.0)) good = false;
-            }
-        });
-        ok = good;
-    }
-    if (ok) {
-        // Dt_i = a_ii - sum_j a_ij^2 / a_jj over the first-colour neighbours
-        // (the IC(0) pivot of a second-colour row)
-        dt.assign(n - n0, 0.0);
-        parallel_for(n - n0, 1u << 15, [&](size_t q0, size_t q1) {
-            for (size_t q = q0; q < q1; ++q) {
-                const uint32_t i = n0 + (uint32_t)q;
-                double t = d[i];
-                for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
-                    if (ci[k] < n0) t -= av[k] * av[k] / d[ci[k]];
-                dt[q] = t;
-                if (!(t > 0.0 && std::isfinite(t))) good = false;
-            }
-        });
-        ok = good;
-    }
-    if (!ok) {
-        M->reduced = R.release();
-        return nullptr;
-    }
+    if (!split_two_colours(n, rp, ci, n0)) return false;
+    std::vector<double> d(n);
+    std::atomic<bool> good{true};
+    parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) {
+            d[i] = 0.0;
+            for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
+                if (ci[k] == i) {
+                    d[i] = av[k];
+                    break;
+                }
+            if (!(d[i] > 0.0)) good = false;
+        }
+    });
+    if (!good) return false;
+    // Dt_i = a_ii - sum_j a_ij^2 / a_jj over the first-colour neighbours
+    // (the IC(0) pivot of a second-colour row)
+    std::vector<double> &dt = H.dt;
+    dt.assign(n - n0, 0.0);
+    parallel_for(n - n0, 1u << 15, [&](size_t q0, size_t q1) {
+        for (size_t q = q0; q < q1; ++q) {
+            const uint32_t i = n0 + (uint32_t)q;
+            double t = d[i];
+            for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
+                if (ci[k] < n0) t -= av[k] * av[k] / d[ci[k]];
+            dt[q] = t;
+            if (!(t > 0.0 && std::isfinite(t))) good = false;
+        }
+    });
+    if (!good) return false;
     const uint32_t n1 = n - n0;
+    H.n0 = n0;
+    H.n1 = n1;
+    H.anorm = 0.0;
     {
         std::mutex mu;
         parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
@@ -111,7 +91,7 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
                 m = std::max(m, t);
             }
             std::lock_guard<std::mutex> g(mu);
-            R->anorm = std::max(R->anorm, m);
+            H.anorm = std::max(H.anorm, m);
         });
     }
     // every row holds one diagonal: row lengths of the four operators follow
@@ -121,24 +101,32 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
         for (uint32_t i = r0; i < r1; ++i) p[i - r0 + 1] = p[i - r0] + (rp[i + 1] - rp[i] - 1) + extra;
         return p;
     };
-    std::vector<uint32_t> ka_rp = rows_of(0, n0, 0), ke_rp = rows_of(0, n0, 1), kb_rp = rows_of(n0, n, 1),
-                          kf_rp = rows_of(n0, n, 1);
-    std::vector<uint32_t> ka_ci(ka_rp.back()), ke_ci(ke_rp.back()), kb_ci(kb_rp.back()), kf_ci(kf_rp.back());
-    std::vector<double> ka_v(ka_rp.back()), ke_v(ke_rp.back()), kb_v(kb_rp.back()), kf_v(kf_rp.back());
-    std::vector<double> s1(n1);
+    H.ka_rp = rows_of(0, n0, 0);
+    H.ke_rp = rows_of(0, n0, 1);
+    H.kb_rp = rows_of(n0, n, 1);
+    H.kf_rp = rows_of(n0, n, 1);
+    H.ka_ci.resize(H.ka_rp.back());
+    H.ke_ci.resize(H.ke_rp.back());
+    H.kb_ci.resize(H.kb_rp.back());
+    H.kf_ci.resize(H.kf_rp.back());
+    H.ka_v.resize(H.ka_rp.back());
+    H.ke_v.resize(H.ke_rp.back());
+    H.kb_v.resize(H.kb_rp.back());
+    H.kf_v.resize(H.kf_rp.back());
+    H.s1.resize(n1);
     parallel_for(n0, 1u << 15, [&](size_t i0, size_t i1) {
         for (uint32_t i = (uint32_t)i0; i < (uint32_t)i1; ++i) {
-            uint32_t pa = ka_rp[i], pe = ke_rp[i];
-            ke_ci[pe] = i;
-            ke_v[pe++] = 1.0 / d[i];
+            uint32_t pa = H.ka_rp[i], pe = H.ke_rp[i];
+            H.ke_ci[pe] = i;
+            H.ke_v[pe++] = 1.0 / d[i];
             for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
                 const uint32_t j = ci[k];
                 if (j == i) continue;
                 const double a = av[k];
-                ka_ci[pa] = j;
-                ka_v[pa++] = a / std::sqrt(d[i] * dt[j - n0]);
-                ke_ci[pe] = j;
-                ke_v[pe++] = -a / d[i];
+                H.ka_ci[pa] = j;
+                H.ka_v[pa++] = a / std::sqrt(d[i] * dt[j - n0]);
+                H.ke_ci[pe] = j;
+                H.ke_v[pe++] = -a / d[i];
             }
         }
     });
@@ -146,37 +134,59 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
         for (uint32_t q = (uint32_t)q0; q < (uint32_t)q1; ++q) {
             const uint32_t i = n0 + q;
             const double t = dt[q], rs = 1.0 / std::sqrt(t);
-            uint32_t pb = kb_rp[q], pf = kf_rp[q];
+            uint32_t pb = H.kb_rp[q], pf = H.kf_rp[q];
             for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
                 const uint32_t j = ci[k];
                 if (j == i) continue;
                 const double a = av[k];
-                kb_ci[pb] = j;
-                kb_v[pb++] = -a / std::sqrt(d[j] * t);
-                kf_ci[pf] = j;
-                kf_v[pf++] = -a / d[j] * rs;
+                H.kb_ci[pb] = j;
+                H.kb_v[pb++] = -a / std::sqrt(d[j] * t);
+                H.kf_ci[pf] = j;
+                H.kf_v[pf++] = -a / d[j] * rs;
             }
-            kb_ci[pb] = i;  // the diagonal column i > every first-colour column
-            kb_v[pb] = d[i] / t;
-            kf_ci[pf] = i;
-            kf_v[pf] = rs;
-            s1[q] = rs;
+            H.kb_ci[pb] = i;  // the diagonal column i > every first-colour column
+            H.kb_v[pb] = d[i] / t;
+            H.kf_ci[pf] = i;
+            H.kf_v[pf] = rs;
+            H.s1[q] = rs;
         }
     });
+    return true;
+}
+
+ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
+    if (!M || M->opts.kind != PP_PC_IC0 || M->order.empty() || !M->io_permuted || M->multilevel || !As) return nullptr;
+    if (M->reduced && M->reduced->src == As && M->reduced->src_version == As->version)
+        return M->reduced->ok ? M->reduced : nullptr;
+    reduced_free(M);
+    std::unique_ptr<ReducedForm> R(new ReducedForm);
+    R->src = As;
+    R->src_version = As->version;
+    ReducedHost H;
+    // the factor's first level must be exactly the first colour
+    const bool ok = As->nrow == As->ncol && reduced_host(As->nrow, As->h_rp, As->h_ci, As->h_v, H) &&
+                    M->low64.nlev == 2 && M->up64.nlev == 2 && M->lvl0_contiguous && M->lvl0_lo == 0 &&
+                    M->lvl0_hi == H.n0;
+    if (!ok) {
+        M->reduced = R.release();
+        return nullptr;
+    }
+    const uint32_t n = As->nrow, n0 = H.n0, n1 = H.n1;
+    R->anorm = H.anorm;
     try {
         R->n0 = n0;
         R->n1 = n1;
         // KA and KB run every iteration, whose operands (the two matrices and
         // five vectors of the second colour) stay cache-resident between
         // launches; KF and KE run once per refinement step
-        R->KA = csr_from_vectors(ctx, n0, n, std::move(ka_rp), std::move(ka_ci), std::move(ka_v), 2);
-        R->KB = csr_from_vectors(ctx, n1, n, std::move(kb_rp), std::move(kb_ci), std::move(kb_v), 2);
-        R->KF = csr_from_vectors(ctx, n1, n, std::move(kf_rp), std::move(kf_ci), std::move(kf_v), 0);
-        R->KE = csr_from_vectors(ctx, n0, n, std::move(ke_rp), std::move(ke_ci), std::move(ke_v), 0);
+        R->KA = csr_from_vectors(ctx, n0, n, std::move(H.ka_rp), std::move(H.ka_ci), std::move(H.ka_v), 2);
+        R->KB = csr_from_vectors(ctx, n1, n, std::move(H.kb_rp), std::move(H.kb_ci), std::move(H.kb_v), 2);
+        R->KF = csr_from_vectors(ctx, n1, n, std::move(H.kf_rp), std::move(H.kf_ci), std::move(H.kf_v), 0);
+        R->KE = csr_from_vectors(ctx, n0, n, std::move(H.ke_rp), std::move(H.ke_ci), std::move(H.ke_v), 0);
         R->s1_64 = dalloc<double>(n1);
-        PP_HIP(hipMemcpy(R->s1_64, s1.data(), n1 * sizeof(double), hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(R->s1_64, H.s1.data(), n1 * sizeof(double), hipMemcpyHostToDevice));
         R->dt_64 = dalloc<double>(n1);
-        PP_HIP(hipMemcpy(R->dt_64, dt.data(), n1 * sizeof(double), hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(R->dt_64, H.dt.data(), n1 * sizeof(double), hipMemcpyHostToDevice));
         R->ok = true;
     } catch (...) {
         M->reduced = R.release();

@@ -59,6 +59,39 @@ def test_batched_fp64_and_mixed_against_oracle(pp, torch):
         assert np.linalg.norm(xms - o["x"]) / np.linalg.norm(o["x"]) <= 1e-6
 
 
+def test_batched_mixed_reduced_and_flexible_inner_agree(pp, torch):
+    # the same colour-ordered batch: reduced inner solves (every system two
+    # colours) against FlexibleCG inner solves (one 9-point system among
+    # them has three colours, so the batch keeps the sweep-based inner solve)
+    from mc_order import greedy_order
+    _, systems = make_systems(16, 8, 72, 6)
+    rp, ci, v = ppo.laplacian(9, 9, 1)
+    rows = [dict(zip(ci[rp[i]:rp[i + 1]].tolist(), v[rp[i]:rp[i + 1]].tolist())) for i in range(81)]
+    for i in range(81):
+        x, y = i % 9, i // 9
+        for dx, dy in ((1, 1), (-1, -1), (1, -1), (-1, 1)):
+            if 0 <= x + dx < 9 and 0 <= y + dy < 9:
+                rows[i][(y + dy) * 9 + x + dx] = -0.25
+        rows[i][i] += 1.0
+    nine = (np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.uint32),
+            np.array([c for r in rows for c in sorted(r)], np.uint32),
+            np.array([r[c] for r in rows for c in sorted(r)]))
+    assert greedy_order(nine)[1] > 2
+    bs = [ppo.uniform(len(A[0]) - 1, 90 + s) for s, A in enumerate(systems)]
+    xs = {}
+    for name, batch in (("reduced", systems), ("flexible", systems + [nine])):
+        B = pp.BatchedSystems(batch, ordering=1)
+        bb = bs + ([ppo.uniform(81, 99)] if name == "flexible" else [])
+        b = torch.tensor(np.concatenate(bb), device="cuda")
+        x = torch.zeros_like(b)
+        B.solve(b, x, precision="mixed", tol=1e-8)
+        print(f"[batch] {name}: variant {B.report.cg_variant}, inner {B.report.inner_iters}", flush=True)
+        assert B.report.converged and B.report.true_relres < 1e-8
+        assert B.report.cg_variant == (3 if name == "reduced" else 1)
+        xs[name] = x[:B.row_off[len(systems)]].cpu().numpy()
+    assert np.linalg.norm(xs["reduced"] - xs["flexible"]) / np.linalg.norm(xs["flexible"]) <= 1e-6
+
+
 def test_batched_autotune_and_graph_replays(pp, torch):
     _, systems = make_systems(40, 8, 64, 4)
     B = pp.BatchedSystems(systems)
@@ -166,9 +199,12 @@ def test_batched_multicolor_against_oracle(pp, torch):
     x64 = torch.zeros_like(b)
     its = B.solve(b, x64, precision="fp64", tol=1e-8)
     assert B.report.converged and B.report.true_relres < 1e-8
+    assert B.report.cg_variant == 1  # FP64: the reference's FlexibleCG per system
     xm = torch.zeros_like(b)
     B.solve(b, xm, precision="mixed", tol=1e-8)
     assert B.report.converged and B.report.true_relres < 1e-8
+    # two-colour systems: the FP32 inner solves run the reduced CG
+    assert B.report.cg_variant == 3
     off = B.row_off
     for s, A in enumerate(systems):
         order, _ = greedy_order(A)

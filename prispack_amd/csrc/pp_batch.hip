@@ -300,6 +300,7 @@ template <int BS>
 __device__ int sys_red_cg(const BatchArgs &a, uint32_t s, uint32_t r0, uint32_t r1, const float *rin, float *e,
                           float *z, float *w, float *p, float *sv, double tol, uint32_t maxit, double *sh,
                           double *sh3, int *converged) {
+    constexpr int U = kRowsInFlight;
     const uint32_t m0 = r0 + a.red_n0[s];
     // r = KF rin (second colour); x = p = s = 0; ||rin||
     sys_spmv<float, float>(a.kfe_rp, a.kfe_ci, a.kfe_v, rin, z, m0, r1);
@@ -321,12 +322,39 @@ __device__ int sys_red_cg(const BatchArgs &a, uint32_t s, uint32_t r0, uint32_t 
         __syncthreads();
         // w = KB [y0; r] with (r, w), (r, r) and sum Dt r^2
         double delta = 0.0, gamma = 0.0, rho = 0.0;
-        sys_spmv<float, float>(a.kab_rp, a.kab_ci, a.kab_v, z, w, m0, r1);
-        for (uint32_t i = m0 + threadIdx.x; i < r1; i += BS) {
-            const float ri = z[i];
-            delta += (double)(ri * w[i]);
-            gamma += (double)(ri * ri);
-            rho += (double)a.red_dt[i] * (double)(ri * ri);
+        for (uint32_t i0 = m0 + threadIdx.x; i0 < r1; i0 += U * BS) {
+            uint32_t k[U], f[U];
+            float sum[U], ri[U], di[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t i = i0 + u * BS;
+                const bool ok = i < r1;
+                k[u] = ok ? a.kab_rp[i] : 0;
+                f[u] = ok ? a.kab_rp[i + 1] : 0;
+                ri[u] = ok ? z[i] : 0.f;
+                di[u] = ok ? a.red_dt[i] : 0.f;
+                sum[u] = 0.f;
+            }
+            for (bool more = true; more;) {
+                more = false;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (k[u] < f[u]) {
+                        sum[u] += a.kab_v[k[u]] * z[a.kab_ci[k[u]]];
+                        ++k[u];
+                        more = true;
+                    }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t i = i0 + u * BS;
+                if (i < r1) {
+                    w[i] = sum[u];
+                    delta += (double)(ri[u] * sum[u]);
+                    gamma += (double)(ri[u] * ri[u]);
+                    rho += (double)di[u] * (double)(ri[u] * ri[u]);
+                }
+            }
         }
         block_reduce3<BS>(delta, gamma, rho, sh3);
         if (c >= maxit) break;
@@ -345,14 +373,31 @@ __device__ int sys_red_cg(const BatchArgs &a, uint32_t s, uint32_t r0, uint32_t 
         aold = alpha;
         first = false;
         const float fa = (float)alpha, fb = (float)beta;
-        for (uint32_t i = m0 + threadIdx.x; i < r1; i += BS) {
-            const float ri = z[i];
-            const float pn = ri + fb * p[i];
-            const float sn = w[i] + fb * sv[i];
-            p[i] = pn;
-            sv[i] = sn;
-            e[i] += fa * pn;
-            z[i] = ri - fa * sn;
+        for (uint32_t i0 = m0 + threadIdx.x; i0 < r1; i0 += U * BS) {
+            float zr[U], wr[U], pr[U], sr[U], er[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t i = i0 + u * BS;
+                if (i < r1) {
+                    zr[u] = z[i];
+                    wr[u] = w[i];
+                    pr[u] = p[i];
+                    sr[u] = sv[i];
+                    er[u] = e[i];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t i = i0 + u * BS;
+                if (i < r1) {
+                    const float pn = zr[u] + fb * pr[u];
+                    const float sn = wr[u] + fb * sr[u];
+                    p[i] = pn;
+                    sv[i] = sn;
+                    e[i] = er[u] + fa * pn;
+                    z[i] = zr[u] - fa * sn;
+                }
+            }
         }
         __syncthreads();
     }

@@ -170,3 +170,45 @@ def test_reduced_unsupported_structures(pp, torch):
     b9 = ppo.spmv((rp2, ci2, v2), ppo.normal(len(rows), 4))
     with pytest.raises(pp.PrisPackError):
         solve(pp, torch, M9, P9, b9, "fp64", "reduced")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_reduced_edge_cases(pp, torch, precision):
+    from prispack_amd import PrisPackError
+    from prispack_amd._capi import PP_ERR_NOT_FINITE
+    # smallest two-colour systems: one row per colour, and 2 + 1
+    for dims in ((2, 1, 1), (3, 1, 1), (5, 4, 1)):
+        A = ppo.laplacian(*dims)
+        n = len(A[0]) - 1
+        xt = ppo.normal(n, 8)
+        M = pp.CSRMatrix(*A)
+        P = pp.IC0(M, ordering=1, make_f32=True)
+        cg, x = solve(pp, torch, M, P, ppo.spmv(A, xt), precision, "reduced")
+        assert cg.report.cg_variant == 3 and cg.getConvergence(), dims
+        assert np.linalg.norm(x - xt) / np.linalg.norm(xt) <= 1e-6, dims
+    A = ppo.laplacian(20, 18, 16)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M, ordering=1, make_f32=True)
+    # b = 0: converged at once with x = 0
+    cg, x = solve(pp, torch, M, P, np.zeros(n), precision, "reduced")
+    assert cg.getConvergence() and not np.any(x)
+    # a starting guess: the same solution
+    xt = ppo.normal(n, 9)
+    b = ppo.spmv(A, xt)
+    cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=3000, precision=precision).setCGVariant("reduced")
+    x0 = torch.tensor(ppo.normal(n, 10), device="cuda")
+    cg.solve(M, torch.tensor(b, device="cuda"), x0)
+    assert cg.getConvergence() and cg.report.true_relres < 1e-8
+    assert np.linalg.norm(x0.cpu().numpy() - xt) / np.linalg.norm(xt) <= 1e-6
+    # an iteration limit far below what the tolerance needs: no convergence, no error
+    cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-12, maxIter=6, precision=precision).setCGVariant("reduced")
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cg.solve(M, torch.tensor(b, device="cuda"), x)
+    assert not cg.getConvergence() and cg.report.inner_iters <= 8
+    # a non-finite right-hand side is an error
+    bad = b.copy()
+    bad[3] = np.nan
+    with pytest.raises(PrisPackError) as e:
+        cg.solve(M, torch.tensor(bad, device="cuda"), torch.zeros(n, dtype=torch.float64, device="cuda"))
+    assert e.value.status == PP_ERR_NOT_FINITE

@@ -65,6 +65,7 @@ template int *dalloc<int>(size_t);
 template unsigned char *dalloc<unsigned char>(size_t);
 template BlockDesc *dalloc<BlockDesc>(size_t);
 template uint4 *dalloc<uint4>(size_t);
+template int16_t *dalloc<int16_t>(size_t);
 
 void dfree(void *p) {
     if (p) (void)hipFree(p);

@@ -291,6 +291,23 @@ namespace pp {
 //   KE  rows [0, n0):  1 / a_ii at j = i, -a_ij / a_ii (j >= n0)
 //                      (e0 = D0^-1 (r0 - A01 e1), the eliminated unknowns)
 // and s1 = Dt^-1/2 maps the reduced solution back: e1 = s1 * x.
+// A row-per-thread sliced-column operator for the reduced CG's KA and KB:
+// entry j of row i at [j * n + i] (every load coalesced), its column
+// i + cbase + off (16-bit offset), rows padded to `width` with zero
+// values; an optional diagonal term dg[i] * x[dbase + i] is added last.
+// Built only when every row has an entry and every offset fits 16 bits.
+struct EllOp {
+    bool ok = false;
+    uint32_t n = 0, width = 0;
+    int64_t cbase = 0, dbase = 0;
+    int16_t *off = nullptr;      // [width * n]
+    double *v64 = nullptr;       // [width * n]
+    float *v32 = nullptr;
+    double *dg64 = nullptr;      // [n] or nullptr
+    float *dg32 = nullptr;
+    void free_all();
+};
+
 struct ReducedForm {
     bool ok = false;  // false: the matrix / factor does not have this structure
     uint32_t n0 = 0, n1 = 0;
@@ -299,6 +316,7 @@ struct ReducedForm {
     float *s1_32 = nullptr;
     double *dt_64 = nullptr;  // Dt: the true residual of the second colour is Dt^1/2 times the reduced one
     float *dt_32 = nullptr;
+    EllOp ka, kb;                 // KA and KB in sliced-column form (when they fit)
     double anorm = 0.0;           // max absolute row sum of the colour-ordered matrix
     const pp_csr *src = nullptr;  // built from this matrix at src_version
     uint64_t src_version = 0;
@@ -483,6 +501,16 @@ void launch_convert(pp_ctx *ctx, uint32_t n, int sdt, const void *s, int ddt, vo
 void launch_axpby(pp_ctx *ctx, int dtype, uint32_t n, double a, const void *x, double b,
                   const void *y, void *z);
 void launch_dot(pp_ctx *ctx, int dtype, uint32_t n, const void *x, const void *y, int slot);
+// y = E x (EllOp); dot_slot >= 0: also the three sums of the single-reduction
+// recurrence into that slot and the next two, as launch_spmv does with dotw,
+// aux and wgt.  ell_grid: the workgroups (partials) such a launch uses.
+void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, const void *x, void *y, int dot_slot, const void *dotw,
+                const void *aux, const void *wgt);
+unsigned ell_grid(const EllOp &E, bool dot);
+// host form -> EllOp (FP64 values on the device); false (nothing kept) when
+// a row is empty or an offset does not fit 16 bits
+bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci, const std::vector<double> &v,
+               int64_t cbase, bool diag_last, int64_t dbase, EllOp &E);
 void launch_outer_on_pattern(pp_ctx *ctx, const pp_csr *A, double scale, const double *lam,
                              const double *x, double *g);
 void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks);

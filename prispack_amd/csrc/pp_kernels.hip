@@ -276,6 +276,106 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
     }
 }
 
+// ---------------------------------------------------------------------------
+// Sliced-column SpMV for the reduced CG's operators (EllOp): one row per
+// thread, entry j of all rows contiguous, so every value and offset load is
+// coalesced and no LDS stage or barrier is needed.  Each row sums its
+// entries in storage order starting from zero, the diagonal term last, with
+// the products rounded before the sums (no contraction): the same
+// operation sequence as k_spmv on the CSR form.  x and y may be one vector
+// when the rows write where no row reads (KA: first colour from second).
+// ---------------------------------------------------------------------------
+template <class T, bool DOT, int MAXW>
+__global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const int16_t *__restrict__ off,
+                                             const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
+                                             int64_t dbase, const T *x, T *y, const T *__restrict__ dotw,
+                                             const T *__restrict__ aux, const T *__restrict__ wgt, double *partials,
+                                             const int *stop) {
+#pragma clang fp contract(off)
+    __shared__ double sh[3 * (kBlock / 64)];
+    if (stop && *stop) return;
+    double dacc = 0.0, dacc2 = 0.0, dacc3 = 0.0;
+    const uint32_t nb = (n + kBlock - 1) / kBlock;
+    const XcdRange xr = xcd_range(0, nb);
+    for (uint32_t b = xr.first; b < xr.end; b += xr.step) {
+        const uint32_t i = b * kBlock + threadIdx.x;
+        if (i < n) {
+            int16_t o[MAXW];
+            T v[MAXW], xv[MAXW];
+#pragma unroll
+            for (int j = 0; j < MAXW; ++j)
+                if (j < (int)width) {
+                    const size_t e = (size_t)j * n + i;
+                    o[j] = off[e];
+                    v[j] = val[e];
+                }
+            T dgi = 0, xd = 0, u = 0, ax = 0, wg = 0;
+            if (dg) {
+                dgi = dg[i];
+                xd = x[dbase + i];
+            }
+            if (DOT) {
+                u = dotw[i];
+                ax = aux[i];
+                wg = wgt[i];
+            }
+#pragma unroll
+            for (int j = 0; j < MAXW; ++j)
+                if (j < (int)width) xv[j] = x[(int64_t)i + cbase + o[j]];
+            T sum = 0;
+#pragma unroll
+            for (int j = 0; j < MAXW; ++j)
+                if (j < (int)width) sum += v[j] * xv[j];
+            if (dg) sum += dgi * xd;
+            y[i] = sum;
+            if (DOT) {
+                dacc += (double)u * (double)sum;
+                dacc2 += (double)ax * (double)u;
+                dacc3 += (double)ax * (double)ax * (double)wg;
+            }
+        }
+    }
+    if (DOT) {
+        block_sum3(dacc, dacc2, dacc3, sh);
+        publish_partial(dacc, partials);
+        publish_partial(dacc2, partials + kMaxPartials);
+        publish_partial(dacc3, partials + 2 * kMaxPartials);
+    }
+}
+
+unsigned ell_grid(const EllOp &E, bool dot) {
+    const uint32_t nb = (E.n + kBlock - 1) / kBlock;
+    return nb ? xcd_grid(nb, dot ? (unsigned)kMaxPartials : (1u << 20)) : 0;
+}
+
+void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, const void *x, void *y, int dot_slot, const void *dotw,
+                const void *aux, const void *wgt) {
+    const bool dot = dot_slot >= 0;
+    const unsigned grid = ell_grid(E, dot);
+    if (grid == 0) return;
+    double *part = dot ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
+#define PP_ELL(T, DOTV, W, VALS, DG)                                                                              \
+    k_ell<T, DOTV, W><<<grid, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, VALS, E.cbase, DG, E.dbase,         \
+                                                       (const T *)x, (T *)y, (const T *)dotw, (const T *)aux, \
+                                                       (const T *)wgt, part, ctx->stop_word)
+#define PP_ELL_W(T, VALS, DG)                                    \
+    if (E.width <= 8) {                                          \
+        if (dot) PP_ELL(T, true, 8, VALS, DG);                   \
+        else PP_ELL(T, false, 8, VALS, DG);                      \
+    } else {                                                     \
+        if (dot) PP_ELL(T, true, 16, VALS, DG);                  \
+        else PP_ELL(T, false, 16, VALS, DG);                     \
+    }
+    if (dtype == PP_F64) {
+        PP_ELL_W(double, E.v64, E.dg64)
+    } else {
+        PP_ELL_W(float, E.v32, E.dg32)
+    }
+#undef PP_ELL_W
+#undef PP_ELL
+    PP_HIP(hipGetLastError());
+}
+
 void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks) {
     blocks.clear();
     const uint32_t n = (uint32_t)rp.size() - 1;

@@ -4,6 +4,7 @@
 // version; the device work is SpMV (k_spmv) on the four operators.
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cmath>
 #include <memory>
 #include <mutex>
@@ -154,6 +155,79 @@ bool reduced_host(uint32_t n, const std::vector<uint32_t> &rp, const std::vector
     return true;
 }
 
+void EllOp::free_all() {
+    dfree(off);
+    dfree(v64);
+    dfree(v32);
+    dfree(dg64);
+    dfree(dg32);
+    off = nullptr;
+    v64 = dg64 = nullptr;
+    v32 = dg32 = nullptr;
+    ok = false;
+    n = width = 0;
+}
+
+// diag_last: the last entry of every row is its diagonal term, at column
+// dbase + row; it goes to dg instead of the sliced columns.
+bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci, const std::vector<double> &v,
+               int64_t cbase, bool diag_last, int64_t dbase, EllOp &E) {
+    E.free_all();
+    const uint32_t n = (uint32_t)rp.size() - 1;
+    const uint32_t tail = diag_last ? 1 : 0;
+    uint32_t width = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t len = rp[i + 1] - rp[i];
+        if (len <= tail) return false;  // a row with no sliced entry
+        width = std::max(width, len - tail);
+    }
+    if (n == 0 || width > 16) return false;
+    std::vector<int16_t> off((size_t)width * n);
+    std::vector<double> val((size_t)width * n), dg(diag_last ? n : 0);
+    std::atomic<bool> fits{true};
+    parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) {
+            const uint32_t k0 = rp[i], k1 = rp[i + 1] - tail;
+            int16_t last = 0;
+            for (uint32_t j = 0; j < width; ++j) {
+                const size_t e = (size_t)j * n + i;
+                if (k0 + j < k1) {
+                    const int64_t o = (int64_t)ci[k0 + j] - (int64_t)i - cbase;
+                    if (o < INT16_MIN || o > INT16_MAX) {
+                        fits = false;
+                        return;
+                    }
+                    last = (int16_t)o;
+                    off[e] = last;
+                    val[e] = v[k0 + j];
+                } else {  // padding: a zero on the row's last column
+                    off[e] = last;
+                    val[e] = 0.0;
+                }
+            }
+            if (diag_last) {
+                if ((int64_t)ci[k1] != dbase + (int64_t)i) fits = false;
+                dg[i] = v[k1];
+            }
+        }
+    });
+    if (!fits) return false;
+    E.n = n;
+    E.width = width;
+    E.cbase = cbase;
+    E.dbase = dbase;
+    E.off = dalloc<int16_t>(off.size());
+    E.v64 = dalloc<double>(val.size());
+    PP_HIP(hipMemcpy(E.off, off.data(), off.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+    PP_HIP(hipMemcpy(E.v64, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (diag_last) {
+        E.dg64 = dalloc<double>(n);
+        PP_HIP(hipMemcpy(E.dg64, dg.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    E.ok = true;
+    return true;
+}
+
 ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
     if (!M || M->opts.kind != PP_PC_IC0 || M->order.empty() || !M->io_permuted || M->multilevel || !As) return nullptr;
     if (M->reduced && M->reduced->src == As && M->reduced->src_version == As->version)
@@ -179,6 +253,11 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
         // KA and KB run every iteration, whose operands (the two matrices and
         // five vectors of the second colour) stay cache-resident between
         // launches; KF and KE run once per refinement step
+        // KA rows (first colour j) reach second-colour columns near n0 + j, KB
+        // rows (second colour q) first-colour columns near q and their own
+        // diagonal at n0 + q
+        if (ell_build(H.ka_rp, H.ka_ci, H.ka_v, n0, false, 0, R->ka))
+            if (!ell_build(H.kb_rp, H.kb_ci, H.kb_v, 0, true, n0, R->kb)) R->ka.free_all();
         R->KA = csr_from_vectors(ctx, n0, n, std::move(H.ka_rp), std::move(H.ka_ci), std::move(H.ka_v), 2);
         R->KB = csr_from_vectors(ctx, n1, n, std::move(H.kb_rp), std::move(H.kb_ci), std::move(H.kb_v), 2);
         R->KF = csr_from_vectors(ctx, n1, n, std::move(H.kf_rp), std::move(H.kf_ci), std::move(H.kf_v), 0);
@@ -213,6 +292,16 @@ void reduced_need_f32(pp_ctx *ctx, ReducedForm *R) {
     };
     to32(R->s1_64, R->s1_32);
     to32(R->dt_64, R->dt_32);
+    for (EllOp *E : {&R->ka, &R->kb}) {
+        if (!E->ok || E->v32) continue;
+        E->v32 = dalloc<float>((size_t)E->width * E->n);
+        launch_convert(ctx, E->width * E->n, PP_F64, E->v64, PP_F32, E->v32);
+        if (E->dg64) {
+            E->dg32 = dalloc<float>(E->n);
+            launch_convert(ctx, E->n, PP_F64, E->dg64, PP_F32, E->dg32);
+        }
+    }
+    PP_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 void reduced_free(pp_precond *M) {
@@ -226,6 +315,8 @@ void reduced_free(pp_precond *M) {
     dfree(R->s1_32);
     dfree(R->dt_64);
     dfree(R->dt_32);
+    R->ka.free_all();
+    R->kb.free_all();
     delete R;
     M->reduced = nullptr;
 }

@@ -295,15 +295,22 @@ PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, do
         launch_fill_zero(ctx, s, (size_t)n1 * sizeof(T));
     }
     red_init(ctx, resume, scale * scale, maxit <= 1);
-    const unsigned np = spmv_grid(R->KB);
+    // KA and KB in sliced-column form where they were built so, else CSR
+    const bool ell = R->ka.ok && R->kb.ok;
+    const unsigned np = ell ? ell_grid(R->kb, true) : spmv_grid(R->KB);
     // the fused finish-and-update alternates its state blocks: an even
     // number of iterations per body (the device still tests every one)
     const uint32_t every = std::max<uint32_t>(2, check_every + (check_every & 1));
     auto body = [&] {
         StopScope scope(ctx);
         for (uint32_t k = 0; k < every; ++k) {
-            launch_spmv(ctx, R->KA, DT, 1.0, z, 0.0, z, -1, nullptr, false);      // z[0, n0) = B r
-            launch_spmv(ctx, R->KB, DT, 1.0, z, 0.0, w, S_CG_DELTA, r, false, r, wt);  // w = Sb r, (r,w), (r,r)
+            if (ell) {
+                launch_ell(ctx, R->ka, DT, z, z, -1, nullptr, nullptr, nullptr);  // z[0, n0) = B r
+                launch_ell(ctx, R->kb, DT, z, w, S_CG_DELTA, r, r, wt);           // w = Sb r, (r,w), (r,r)
+            } else {
+                launch_spmv(ctx, R->KA, DT, 1.0, z, 0.0, z, -1, nullptr, false);
+                launch_spmv(ctx, R->KB, DT, 1.0, z, 0.0, w, S_CG_DELTA, r, false, r, wt);
+            }
             red_step<T>(ctx, np, (int)(k & 1), k + 1 == every, n1, w, p, s, x, r);
         }
     };

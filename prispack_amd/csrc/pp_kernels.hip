@@ -343,9 +343,13 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const i
     }
 }
 
+// With the sums, at most 1024 workgroups, each over several 256-row
+// slices: fewer partials for the consumer to add, measured 2% faster per
+// reduced iteration than one slice per workgroup at 1M rows (512: slower).
 unsigned ell_grid(const EllOp &E, bool dot) {
+    constexpr unsigned kEllDotGroups = 1024;
     const uint32_t nb = (E.n + kBlock - 1) / kBlock;
-    return nb ? xcd_grid(nb, dot ? (unsigned)kMaxPartials : (1u << 20)) : 0;
+    return nb ? xcd_grid(nb, dot ? kEllDotGroups : (1u << 20)) : 0;
 }
 
 void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, const void *x, void *y, int dot_slot, const void *dotw,

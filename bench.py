@@ -59,6 +59,9 @@ def parse():
 PMC_TRAFFIC = {(128, v): b for v, b in enumerate((224567189, 226682332, 194624398, 197270641, 219109458, 229230402, 189935329, 192882023))}
 
 
+ELL_VARIANT = 8  # the SpMV variant that runs the sliced-column form
+
+
 def measured_traffic(grid, variant):
     return PMC_TRAFFIC.get((grid, variant))
 
@@ -360,6 +363,8 @@ def main():
     spmv32_ms = kernel_time_ms(torch, stream, lambda: L.pp_spmv(ctx.h, M.h, 1, 1.0, p32x, 0.0, p32y),
                                args.kernel_reps)
     spmv32_bytes = spmv_bytes_of(1)
+    spmv32_var = C.c_int()
+    ctx.check(L.pp_csr_spmv_variant(M.h, 1, C.byref(spmv32_var)))
     # one preconditioner apply = forward sweep on L + backward sweep on L^T,
     # priced on the bytes the library's sweep storage streams (direct apply:
     # with the colour-order gather and scatter)
@@ -401,8 +406,9 @@ def main():
     alg_apply = 2 * (12 * nnz_l + 4 * (n + 1) + 16 * n)
     alg_apply32 = 2 * (8 * nnz_l + 4 * (n + 1) + 8 * n)
     traffic = measured_traffic(g, spmv_var.value)
-    dominant = {"kernel": "k_spmv<double> (y = A x, FP64 CSR, launches alternating between two copies of A, x, y "
-                          "on the library stream)",
+    spmv_kernel = ("k_ell<double> (y = A x, FP64 sliced-column form with 16-bit column offsets"
+                   if spmv_var.value == ELL_VARIANT else "k_spmv<double> (y = A x, FP64 CSR")
+    dominant = {"kernel": spmv_kernel + ", launches alternating between two copies of A, x, y on the library stream)",
                 "bound": "hbm", "achieved": round(gbs(alg_spmv, spmv_ms), 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(gbs(alg_spmv, spmv_ms) / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": alg_spmv, "avg_us": round(spmv_ms * 1e3, 2),
@@ -469,7 +475,8 @@ def main():
         "first_mixed_solve_s": None if first_mixed_s is None else round(first_mixed_s, 4),
         "roofline": dominant,
         "roofline_apply": apply_rec,
-        "roofline_spmv_f32": {"kernel": "k_spmv<float>", "achieved": round(gbs(alg_spmv32, spmv32_ms), 1),
+        "roofline_spmv_f32": {"kernel": "k_ell<float>" if spmv32_var.value == ELL_VARIANT else "k_spmv<float>",
+                              "variant": spmv32_var.value, "achieved": round(gbs(alg_spmv32, spmv32_ms), 1),
                               "unit": "GB/s", "frac": round(gbs(alg_spmv32, spmv32_ms) / HBM_PEAK_GBS, 4),
                               "algorithmic_bytes": alg_spmv32, "avg_us": round(spmv32_ms * 1e3, 2),
                               "bytes_streamed": spmv32_bytes},

@@ -84,12 +84,17 @@ int pp_csr_dims(const pp_csr *A, uint32_t *nrow, uint32_t *ncol, uint32_t *nnz);
 int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes);
 /* SpMV load variant of one precision (dtype PP_F64 / PP_F32), chosen by
  * timing on large matrices (FP64 when the matrix is created, FP32 when its
- * FP32 copy is made): bit 0 non-temporal operand loads and output stores,
- * bit 1 16-bit column offsets, bit 2 batched loads. */
+ * FP32 copy is made): 0-7 the CSR-stream kernel with bit 0 non-temporal
+ * operand loads and output stores, bit 1 16-bit column offsets, bit 2
+ * batched loads; 8 the sliced-column copy (one row per thread, entry j of
+ * all rows contiguous, 16-bit column offsets from the row), which large
+ * matrices with at most 16 entries per row, little padding and near-diagonal
+ * columns also hold.  Every variant gives the same result bit for bit. */
 int pp_csr_spmv_variant(const pp_csr *A, int dtype, int *variant);
 /* Pins the SpMV load variant of one precision instead of the timed choice
  * (reproducible profiling, identical copies of one matrix).  PP_ERR_INVALID
- * for a value outside 0..7 or for 16-bit offsets on a matrix that has none. */
+ * for a value outside 0..8, for 16-bit offsets on a matrix that has none, or
+ * for 8 on a matrix without the sliced-column copy. */
 int pp_csr_set_spmv_variant(pp_csr *A, int dtype, int variant);
 /* Device pointers of the stored arrays (values64 FP64; values32 NULL until an
  * FP32 copy exists). */

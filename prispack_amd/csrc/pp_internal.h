@@ -47,6 +47,9 @@ bool two_base_offsets(const uint32_t *cols, uint32_t cnt, uint32_t &base_lo, uin
 // offsets, batched loads (every load of a thread issued before the first use)
 constexpr int kVarNT = 1, kVarIdx16 = 2, kVarBatch = 4;
 constexpr int kVarCount = 8;
+// the sliced-column form (EllOp) instead of CSR-stream: a ninth variant,
+// timed only where the matrix has that form
+constexpr int kVarEll = 8;
 constexpr int kVarDefault = kVarIdx16 | kVarBatch;
 // One row block of a staged kernel: first row (position) r0, first entry k0,
 // the two 16-bit offset bases (kNoBase: the block reads 32-bit columns).  An
@@ -238,6 +241,25 @@ struct HostSweep {
     mutable std::vector<uint32_t> lev_perm, lev_ptr;
 };
 
+// A row-per-thread sliced-column form of a sparse matrix (the reduced CG's
+// KA and KB; the SpMV variant kVarEll of a CSR matrix): entry j of row i at
+// [j * n + i] (every load coalesced), its column i + cbase + off (16-bit
+// offset), rows padded to `width` with zero values on the row's last
+// column; an optional diagonal term dg[i] * x[dbase + i] is added last.
+// Built only when every row has an entry, every offset fits 16 bits and the
+// padding stays small.
+struct EllOp {
+    bool ok = false;
+    uint32_t n = 0, width = 0;
+    int64_t cbase = 0, dbase = 0;
+    int16_t *off = nullptr;      // [width * n]
+    double *v64 = nullptr;       // [width * n]
+    float *v32 = nullptr;
+    double *dg64 = nullptr;      // [n] or nullptr
+    float *dg32 = nullptr;
+    void free_all();
+};
+
 }  // namespace pp
 
 struct pp_csr {
@@ -258,6 +280,7 @@ struct pp_csr {
     int spmv_variant = pp::kVarDefault;    // FP64 loads, chosen by timing at creation for large matrices
     int spmv_variant32 = pp::kVarDefault;  // FP32 loads, chosen by timing when the FP32 copy is made
     bool tuned32 = false;
+    pp::EllOp ell;                   // sliced-column copy (SpMV variant kVarEll) when the rows fit it
     // time the load variants with warm caches: the matrix runs in an
     // iteration whose operands stay cache-resident between launches
     bool tune_warm = false;
@@ -291,23 +314,6 @@ namespace pp {
 //   KE  rows [0, n0):  1 / a_ii at j = i, -a_ij / a_ii (j >= n0)
 //                      (e0 = D0^-1 (r0 - A01 e1), the eliminated unknowns)
 // and s1 = Dt^-1/2 maps the reduced solution back: e1 = s1 * x.
-// A row-per-thread sliced-column operator for the reduced CG's KA and KB:
-// entry j of row i at [j * n + i] (every load coalesced), its column
-// i + cbase + off (16-bit offset), rows padded to `width` with zero
-// values; an optional diagonal term dg[i] * x[dbase + i] is added last.
-// Built only when every row has an entry and every offset fits 16 bits.
-struct EllOp {
-    bool ok = false;
-    uint32_t n = 0, width = 0;
-    int64_t cbase = 0, dbase = 0;
-    int16_t *off = nullptr;      // [width * n]
-    double *v64 = nullptr;       // [width * n]
-    float *v32 = nullptr;
-    double *dg64 = nullptr;      // [n] or nullptr
-    float *dg32 = nullptr;
-    void free_all();
-};
-
 struct ReducedForm {
     bool ok = false;  // false: the matrix / factor does not have this structure
     uint32_t n0 = 0, n1 = 0;
@@ -504,13 +510,15 @@ void launch_dot(pp_ctx *ctx, int dtype, uint32_t n, const void *x, const void *y
 // y = E x (EllOp); dot_slot >= 0: also the three sums of the single-reduction
 // recurrence into that slot and the next two, as launch_spmv does with dotw,
 // aux and wgt.  ell_grid: the workgroups (partials) such a launch uses.
-void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, const void *x, void *y, int dot_slot, const void *dotw,
-                const void *aux, const void *wgt);
+void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void *x, double beta, void *y,
+                int dot_slot, const void *dotw, const void *aux, const void *wgt, unsigned grid = 0);
 unsigned ell_grid(const EllOp &E, bool dot);
+// E's values (dtype) from the CSR values v of the matrix it was built from
+void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp, const void *v);
 // host form -> EllOp (FP64 values on the device); false (nothing kept) when
 // a row is empty or an offset does not fit 16 bits
 bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci, const std::vector<double> &v,
-               int64_t cbase, bool diag_last, int64_t dbase, EllOp &E);
+               int64_t cbase, bool diag_last, int64_t dbase, EllOp &E, double max_fill = 1.25);
 void launch_outer_on_pattern(pp_ctx *ctx, const pp_csr *A, double scale, const double *lam,
                              const double *x, double *g);
 void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks);

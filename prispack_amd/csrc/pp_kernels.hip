@@ -277,21 +277,21 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
 }
 
 // ---------------------------------------------------------------------------
-// Sliced-column SpMV for the reduced CG's operators (EllOp): one row per
-// thread, entry j of all rows contiguous, so every value and offset load is
-// coalesced and no LDS stage or barrier is needed.  Each row sums its
-// entries in storage order starting from zero, the diagonal term last, with
-// the products rounded before the sums (no contraction): the same
-// operation sequence as k_spmv on the CSR form.  x and y may be one vector
-// when the rows write where no row reads (KA: first colour from second).
+// Sliced-column SpMV (EllOp): one row per thread, entry j of all rows
+// contiguous, so every value and offset load is coalesced and no LDS stage
+// or barrier is needed.  Each row sums its products in storage order
+// starting from zero, the diagonal term last, each product rounded before
+// its sum, then y = alpha * sum (+ beta * y): the operation sequence of
+// k_spmv on the CSR form, so the results are the same bit for bit.  x and y
+// may be one vector when the rows write where no row reads (the reduced
+// CG's KA: first colour from second).  DOT: the same sums as k_spmv.
 // ---------------------------------------------------------------------------
 template <class T, bool DOT, int MAXW>
 __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const int16_t *__restrict__ off,
                                              const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
-                                             int64_t dbase, const T *x, T *y, const T *__restrict__ dotw,
-                                             const T *__restrict__ aux, const T *__restrict__ wgt, double *partials,
-                                             const int *stop) {
-#pragma clang fp contract(off)
+                                             int64_t dbase, const T *x, T *y, T alpha, T beta, int beta_zero,
+                                             const T *__restrict__ dotw, const T *__restrict__ aux,
+                                             const T *__restrict__ wgt, double *partials, const int *stop) {
     __shared__ double sh[3 * (kBlock / 64)];
     if (stop && *stop) return;
     double dacc = 0.0, dacc2 = 0.0, dacc3 = 0.0;
@@ -309,37 +309,49 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const i
                     o[j] = off[e];
                     v[j] = val[e];
                 }
-            T dgi = 0, xd = 0, u = 0, ax = 0, wg = 0;
+            T dgi = 0, xd = 0, yold = 0, u = 0, ax = 0, wg = 1;
             if (dg) {
                 dgi = dg[i];
                 xd = x[dbase + i];
             }
+            if (!beta_zero) yold = y[i];
             if (DOT) {
                 u = dotw[i];
-                ax = aux[i];
-                wg = wgt[i];
+                if (aux) ax = aux[i];
+                if (wgt) wg = wgt[i];
             }
 #pragma unroll
             for (int j = 0; j < MAXW; ++j)
                 if (j < (int)width) xv[j] = x[(int64_t)i + cbase + o[j]];
             T sum = 0;
+            {
+#pragma clang fp contract(off)
 #pragma unroll
-            for (int j = 0; j < MAXW; ++j)
-                if (j < (int)width) sum += v[j] * xv[j];
-            if (dg) sum += dgi * xd;
-            y[i] = sum;
+                for (int j = 0; j < MAXW; ++j)
+                    if (j < (int)width) sum += v[j] * xv[j];
+                if (dg) sum += dgi * xd;
+            }
+            T out = alpha * sum;
+            if (!beta_zero) out += beta * yold;
+            y[i] = out;
             if (DOT) {
-                dacc += (double)u * (double)sum;
-                dacc2 += (double)ax * (double)u;
-                dacc3 += (double)ax * (double)ax * (double)wg;
+                dacc += (double)u * (double)out;
+                if (aux) {
+                    dacc2 += (double)ax * (double)u;
+                    dacc3 += (double)ax * (double)ax * (double)wg;
+                }
             }
         }
     }
     if (DOT) {
-        block_sum3(dacc, dacc2, dacc3, sh);
-        publish_partial(dacc, partials);
-        publish_partial(dacc2, partials + kMaxPartials);
-        publish_partial(dacc3, partials + 2 * kMaxPartials);
+        if (aux) {
+            block_sum3(dacc, dacc2, dacc3, sh);
+            publish_partial(dacc, partials);
+            publish_partial(dacc2, partials + kMaxPartials);
+            publish_partial(dacc3, partials + 2 * kMaxPartials);
+        } else {
+            publish_partial(block_sum(dacc, sh), partials);
+        }
     }
 }
 
@@ -352,16 +364,20 @@ unsigned ell_grid(const EllOp &E, bool dot) {
     return nb ? xcd_grid(nb, dot ? kEllDotGroups : (1u << 20)) : 0;
 }
 
-void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, const void *x, void *y, int dot_slot, const void *dotw,
-                const void *aux, const void *wgt) {
+// grid: 0 for ell_grid's, else that many workgroups (a caller whose
+// consumers expect a given number of partials)
+void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void *x, double beta, void *y,
+                int dot_slot, const void *dotw, const void *aux, const void *wgt, unsigned grid) {
     const bool dot = dot_slot >= 0;
-    const unsigned grid = ell_grid(E, dot);
+    if (grid == 0) grid = ell_grid(E, dot);
     if (grid == 0) return;
     double *part = dot ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
+    const int bz = beta == 0.0;
 #define PP_ELL(T, DOTV, W, VALS, DG)                                                                              \
     k_ell<T, DOTV, W><<<grid, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, VALS, E.cbase, DG, E.dbase,         \
-                                                       (const T *)x, (T *)y, (const T *)dotw, (const T *)aux, \
-                                                       (const T *)wgt, part, ctx->stop_word)
+                                                       (const T *)x, (T *)y, (T)alpha, (T)beta, bz,           \
+                                                       (const T *)dotw, (const T *)aux, (const T *)wgt, part, \
+                                                       ctx->stop_word)
 #define PP_ELL_W(T, VALS, DG)                                    \
     if (E.width <= 8) {                                          \
         if (dot) PP_ELL(T, true, 8, VALS, DG);                   \
@@ -377,6 +393,26 @@ void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, const void *x, void *y, 
     }
 #undef PP_ELL_W
 #undef PP_ELL
+    PP_HIP(hipGetLastError());
+}
+
+template <class T>
+__global__ void k_ell_fill(uint32_t n, uint32_t width, const uint32_t *__restrict__ rp, const T *__restrict__ v,
+                           T *__restrict__ out) {
+    const size_t total = (size_t)width * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t j = (uint32_t)(e / n), i = (uint32_t)(e - (size_t)j * n);
+        const uint32_t k = rp[i] + j;
+        out[e] = k < rp[i + 1] ? v[k] : (T)0;
+    }
+}
+
+void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp, const void *v) {
+    const unsigned g = grid_for(ctx, (uint64_t)E.width * E.n, 4);
+    if (dtype == PP_F64)
+        k_ell_fill<double><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const double *)v, E.v64);
+    else
+        k_ell_fill<float><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const float *)v, E.v32);
     PP_HIP(hipGetLastError());
 }
 
@@ -467,6 +503,12 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
     const int bz = beta == 0.0;
     const bool dot = dot_slot >= 0;
     const int var = dtype == PP_F64 ? A->spmv_variant : A->spmv_variant32;
+    if (var == kVarEll) {  // same number of workgroups (partials) as the CSR-stream launch
+        launch_ell(ctx, A->ell, dtype, alpha, x, beta, y, dot_slot, dot_with, aux, wgt, grid);
+        if (dot && !aux) launch_finish(ctx, dot_slot, grid, ctx->stop_word);  // with aux the caller finishes
+        PP_HIP(hipGetLastError());
+        return;
+    }
     const uint16_t *c16 = (var & kVarIdx16) ? A->ci16 : nullptr;
 #define PP_SPMV(T, DOTV, NTV, BV, VALS, DW, PART)                                                                 \
     k_spmv<T, DOTV, NTV, BV><<<grid, kBlock, 0, ctx->stream>>>(A->blk_desc, A->rp, A->ci, c16, VALS, (const T *)x, \

@@ -170,8 +170,10 @@ void EllOp::free_all() {
 
 // diag_last: the last entry of every row is its diagonal term, at column
 // dbase + row; it goes to dg instead of the sliced columns.
+// max_fill: the largest padded size width * n allowed, relative to the
+// entries kept.
 bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci, const std::vector<double> &v,
-               int64_t cbase, bool diag_last, int64_t dbase, EllOp &E) {
+               int64_t cbase, bool diag_last, int64_t dbase, EllOp &E, double max_fill) {
     E.free_all();
     const uint32_t n = (uint32_t)rp.size() - 1;
     const uint32_t tail = diag_last ? 1 : 0;
@@ -182,6 +184,7 @@ bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
         width = std::max(width, len - tail);
     }
     if (n == 0 || width > 16) return false;
+    if ((double)width * n > max_fill * (double)(rp[n] - (uint64_t)tail * n)) return false;
     std::vector<int16_t> off((size_t)width * n);
     std::vector<double> val((size_t)width * n), dg(diag_last ? n : 0);
     std::atomic<bool> fits{true};

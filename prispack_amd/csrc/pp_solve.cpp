@@ -305,8 +305,8 @@ PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, do
         StopScope scope(ctx);
         for (uint32_t k = 0; k < every; ++k) {
             if (ell) {
-                launch_ell(ctx, R->ka, DT, z, z, -1, nullptr, nullptr, nullptr);  // z[0, n0) = B r
-                launch_ell(ctx, R->kb, DT, z, w, S_CG_DELTA, r, r, wt);           // w = Sb r, (r,w), (r,r)
+                launch_ell(ctx, R->ka, DT, 1.0, z, 0.0, z, -1, nullptr, nullptr, nullptr);  // z[0, n0) = B r
+                launch_ell(ctx, R->kb, DT, 1.0, z, 0.0, w, S_CG_DELTA, r, r, wt);  // w = Sb r, (r,w), (r,r)
             } else {
                 launch_spmv(ctx, R->KA, DT, 1.0, z, 0.0, z, -1, nullptr, false);
                 launch_spmv(ctx, R->KB, DT, 1.0, z, 0.0, w, S_CG_DELTA, r, false, r, wt);

@@ -131,6 +131,57 @@ def test_spmv_t_and_fp32(pp, torch, name):
     np.testing.assert_allclose(host(y32), ref, rtol=0, atol=1e-6 * scale)
 
 
+@pytest.mark.parametrize("name", ["lap3d_40", "upwind_33"])
+def test_spmv_variants_and_solves_agree(pp, torch, name):
+    # every SpMV variant (CSR-stream 0-7, sliced-column 8, which a matrix
+    # this size holds) gives the oracle's y bit for bit in FP64 and one FP32
+    # result; a solve differs only through the inner products' partial sums
+    import ctypes as C
+    from prispack_amd import _capi
+    L = _capi.lib()
+    A = ppo.laplacian(40, 40, 40) if name == "lap3d_40" else ppo.upwind(33, 33, 33, 20.0, 10.0, 5.0)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    x = ppo.uniform(n, 21)
+    want = ppo.spmv(A, x)
+    y32s, runs = [], []
+    for var in range(9):
+        st = L.pp_csr_set_spmv_variant(M.h, 0, var)
+        if st != 0:
+            assert (var & 2) == 2 or var == 8
+            continue
+        assert L.pp_csr_set_spmv_variant(M.h, 1, var) == 0
+        y = torch.zeros(n, dtype=torch.float64, device="cuda")
+        M.MatPVec(dev(torch, x), y)
+        assert np.array_equal(host(y), want), var
+        b = ppo.uniform(n, 22)
+        yb = dev(torch, b)
+        M.MatPVec(dev(torch, x), yb, alpha=-1.0, beta=1.0)
+        np.testing.assert_allclose(host(yb), b - want, rtol=0, atol=1e-13 * (1 + np.abs(want).max()))
+        y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+        M.MatPVec(dev(torch, x, torch.float32), y32)
+        y32s.append(host(y32))
+        rhs = dev(torch, want)
+        if name == "lap3d_40":
+            P = pp.IC0(M, make_f32=True)
+            s = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=1000)
+        else:
+            P = pp.ILU0(M, make_f32=True)
+            s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=1000)
+            s.setRestart(True, 30)
+        xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+        s.solve(M, rhs, xs)
+        assert s.getConvergence(), var
+        runs.append((var, s.getNumIter(), host(xs)))
+        print(f"[spmv variant] {name} {var}: {s.getNumIter()} iterations", flush=True)
+    assert any(v == 8 for v, _, _ in runs)  # the sliced-column copy exists at this size
+    for y32 in y32s[1:]:
+        assert np.array_equal(y32, y32s[0])
+    for var, its, xs in runs[1:]:
+        assert abs(its - runs[0][1]) <= 1, var
+        assert np.linalg.norm(xs - runs[0][2]) <= 1e-7 * np.linalg.norm(runs[0][2]), var
+
+
 def test_blas1(pp, torch):
     from prispack_amd import _capi
     import ctypes as C

@@ -56,7 +56,8 @@ def parse():
 # what the same counter reads for a device-to-device copy of a known size
 # (--copy-probe).  Keyed by (grid side, SpMV storage variant); the command and
 # the raw per-dispatch figures are in profiles/k_spmv_poisson128_pmc.md.
-PMC_TRAFFIC = {(128, v): b for v, b in enumerate((224567189, 226682332, 194624398, 197270641, 219109458, 229230402, 189935329, 192882023))}
+PMC_TRAFFIC = {(128, v): b for v, b in enumerate((224609335, 226678912, 194713431, 197261587, 219104885, 229002306,
+                                                        189947777, 192828681, 183323655))}
 
 
 ELL_VARIANT = 8  # the SpMV variant that runs the sliced-column form

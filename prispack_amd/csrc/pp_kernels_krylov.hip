@@ -9,7 +9,7 @@
 namespace pp {
 
 namespace {
-constexpr int kGemvBlocks = 512;
+constexpr int kGemvBlocks = 2048;
 constexpr int kMaxCols = 257;
 
 __device__ inline double wsum(double v) {
@@ -25,22 +25,42 @@ double *gmres_coeff_buffer(pp_ctx *ctx) {
     return (double *)ws_get(ctx, kWsGmresCoeff, (size_t)(640 + (size_t)kMaxCols * kGemvBlocks) * sizeof(double));
 }
 
-// partial[c][block] = sum over the block's rows of V[r, c] * w[r]
+// partial[c][block] = sum over the block's rows of V[r, c] * w[r].  The
+// columns go in groups of kGemvGroup: one pass over the block's rows
+// accumulates the whole group in registers (every column's loads in flight
+// together, no barrier per column), then one reduction per group.  Each
+// column is summed in the same order as a column-at-a-time pass (the
+// thread's rows in stride order, then the wave and the four waves), so the
+// result does not depend on the grouping.
+constexpr int kGemvGroup = 16;
 __global__ __launch_bounds__(256) void k_gemv_t_partial(uint32_t n, uint32_t ncol, const double *__restrict__ V,
                                                         size_t ld, const double *__restrict__ w, double *partial,
                                                         const int *stop) {
-    __shared__ double sh[4];
+    __shared__ double sh[kGemvGroup][4];
     if (stop && *stop) return;
     const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const uint32_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
-    for (uint32_t c = 0; c < ncol; ++c) {
-        const double *col = V + ld * c;
-        double acc = 0.0;
-        for (uint32_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) acc += col[r] * w[r];
-        acc = wsum(acc);
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    for (uint32_t c0 = 0; c0 < ncol; c0 += kGemvGroup) {
+        const uint32_t nc = min((uint32_t)kGemvGroup, ncol - c0);
+        double acc[kGemvGroup];
+#pragma unroll
+        for (int c = 0; c < kGemvGroup; ++c) acc[c] = 0.0;
+        for (uint32_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+            const double wr = w[r];
+#pragma unroll
+            for (int c = 0; c < kGemvGroup; ++c)
+                if ((uint32_t)c < nc) acc[c] += V[ld * (c0 + c) + r] * wr;
+        }
+#pragma unroll
+        for (int c = 0; c < kGemvGroup; ++c)
+            if ((uint32_t)c < nc) {
+                const double v = wsum(acc[c]);
+                if ((threadIdx.x & 63) == 0) sh[c][threadIdx.x >> 6] = v;
+            }
         __syncthreads();
-        if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+        if (threadIdx.x < nc)
+            partial[(size_t)(c0 + threadIdx.x) * gridDim.x + blockIdx.x] =
+                ((sh[threadIdx.x][0] + sh[threadIdx.x][1]) + sh[threadIdx.x][2]) + sh[threadIdx.x][3];
         __syncthreads();
     }
 }

@@ -460,9 +460,13 @@ struct GmresDev {
     double *H = nullptr, *c = nullptr, *s = nullptr, *g = nullptr, *h1 = nullptr, *h2 = nullptr, *misc = nullptr;
     uint32_t mm = 0;
 };
-enum GmMisc { GM_INVH = 0, GM_RELERR, GM_RNORM, GM_M, GM_STATUS, GM_CONVCYC, GM_BNORM, GM_TOL, GM_NREC, kGmMisc = 16 };
+// GM_HBASE: the residual-history index of the cycle's first step
+enum GmMisc {
+    GM_INVH = 0, GM_RELERR, GM_RNORM, GM_M, GM_STATUS, GM_CONVCYC, GM_BNORM, GM_TOL, GM_NREC, GM_HBASE, kGmMisc = 16
+};
 GmresDev gmres_state(pp_ctx *ctx, uint32_t mm);
-void launch_gmres_cycle_init(pp_ctx *ctx, const GmresDev &st, double beta, double bnorm, double tol);
+void launch_gmres_cycle_init(pp_ctx *ctx, const GmresDev &st, double beta, double bnorm, double tol,
+                             uint32_t hist_base);
 void launch_gmres_step(pp_ctx *ctx, const GmresDev &st, uint32_t j, int quirks, double *hist, uint32_t hist_idx,
                        uint32_t hist_cap);
 void launch_gmres_next_basis(pp_ctx *ctx, const GmresDev &st, uint32_t n, const double *w, double *vnext);

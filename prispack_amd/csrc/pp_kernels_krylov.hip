@@ -212,7 +212,7 @@ __device__ void dev_givens(const GmresDev &st, uint32_t j) {
     st.g[j] = st.c[j] * st.g[j];
 }
 
-__global__ void k_gmres_cycle_init(GmresDev st, double beta, double bnorm, double tol) {
+__global__ void k_gmres_cycle_init(GmresDev st, double beta, double bnorm, double tol, uint32_t hist_base) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (uint32_t i = 0; i < st.mm; ++i) st.g[i] = 0.0;  // g[mm] keeps its value (GMRES.cpp:160)
     st.g[0] = beta;
@@ -220,6 +220,7 @@ __global__ void k_gmres_cycle_init(GmresDev st, double beta, double bnorm, doubl
     st.misc[GM_STATUS] = 0.0;
     st.misc[GM_CONVCYC] = 0.0;
     st.misc[GM_NREC] = 0.0;
+    st.misc[GM_HBASE] = (double)hist_base;
     st.misc[GM_BNORM] = bnorm;
     st.misc[GM_TOL] = tol;
 }
@@ -258,7 +259,9 @@ __global__ void k_gmres_step(GmresDev st, uint32_t j, int quirks, const double *
     const double relerr = rnorm / bnorm;
     st.misc[GM_RNORM] = rnorm;
     st.misc[GM_RELERR] = relerr;
-    if (hist && hist_idx < hist_cap) hist[hist_idx] = relerr;
+    // hist_idx counts from the cycle's first step (a captured cycle is replayed unchanged)
+    const uint32_t hidx = (uint32_t)st.misc[GM_HBASE] + hist_idx;
+    if (hist && hidx < hist_cap) hist[hidx] = relerr;
     st.misc[GM_NREC] += 1.0;
     if (relerr < st.misc[GM_TOL]) {
         st.misc[GM_M] = quirks ? j : j + 1;  // the reference drops column j
@@ -296,8 +299,9 @@ __global__ void k_gmres_backsolve(GmresDev st, int quirks, double *y) {
 
 }  // namespace
 
-void launch_gmres_cycle_init(pp_ctx *ctx, const GmresDev &st, double beta, double bnorm, double tol) {
-    k_gmres_cycle_init<<<1, 64, 0, ctx->stream>>>(st, beta, bnorm, tol);
+void launch_gmres_cycle_init(pp_ctx *ctx, const GmresDev &st, double beta, double bnorm, double tol,
+                             uint32_t hist_base) {
+    k_gmres_cycle_init<<<1, 64, 0, ctx->stream>>>(st, beta, bnorm, tol, hist_base);
     PP_HIP(hipGetLastError());
 }
 

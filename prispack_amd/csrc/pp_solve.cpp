@@ -50,6 +50,33 @@ struct PcgResult {
 // key: everything the body's kernels are launched with; an instantiated
 // graph with the same key is reused (capture and instantiation cost more
 // than a few iterations).
+// The cached graph of `body` under `key` (captured on first use, as in
+// drive_bodies), launched once on the context's stream.
+template <class Body>
+void run_graph(pp_ctx *ctx, const std::vector<uint64_t> &key, Body &body) {
+    Graph *G = nullptr;
+    for (auto &e : ctx->graphs)
+        if (e.first == key) G = e.second.get();
+    if (!G) {
+        std::unique_ptr<Graph> fresh(new Graph);
+        PP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        PP_HIP(hipStreamEndCapture(ctx->stream, &fresh->g));
+        PP_HIP(hipGraphInstantiate(&fresh->exec, fresh->g, nullptr, nullptr, 0));
+        G = fresh.get();
+        if (ctx->graphs.size() >= 8) ctx->graphs.erase(ctx->graphs.begin());
+        ctx->graphs.emplace_back(key, std::move(fresh));
+    }
+    PP_HIP(hipGraphLaunch(G->exec, ctx->stream));
+}
+
 template <class Body>
 void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays, int *ctl, bool check_first,
                   const std::vector<uint64_t> &key) {
@@ -593,15 +620,16 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
         return res;
     }
     const int quirks = o.reference_quirks ? 1 : 0;
+    const bool graph = o.launch != PP_LAUNCH_STREAM;
     while (!conv && k < maxcyc) {
         ++k;
         // One Arnoldi cycle, issued without host round trips: the Hessenberg
         // column, Givens rotations and convergence test run on the device
         // (k_gmres_step) and the stop word ends the cycle's remaining kernels.
         ctl_init(ctx, 1, 0);
-        launch_gmres_cycle_init(ctx, st, beta, bnorm, o.tol);
+        launch_gmres_cycle_init(ctx, st, beta, bnorm, o.tol, nrec);
         launch_axpby(ctx, PP_F64, n, 1.0 / beta, r, 0.0, r, V);  // v0 = r / beta
-        {
+        auto cycle = [&] {
             StopScope scope(ctx);
             for (uint32_t j = 0; j < m; ++j) {
                 double *vj = V + ld * j;
@@ -626,9 +654,22 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
                 launch_gemv_t(ctx, n, j + 1, V, ld, w, st.h2);
                 launch_gemv_sub(ctx, n, j + 1, V, ld, st.h2, w);
                 launch_dot(ctx, PP_F64, n, w, w, S_TMP2);
-                launch_gmres_step(ctx, st, j, quirks, hist_on ? ctx->hist : nullptr, nrec + j, hist_cap);
+                launch_gmres_step(ctx, st, j, quirks, hist_on ? ctx->hist : nullptr, j, hist_cap);
                 launch_gmres_next_basis(ctx, st, n, w, V + ld * (j + 1));
             }
+        };
+        // the cycle's launches are the same every cycle of the same length:
+        // captured once into a graph and replayed (no host launch gaps)
+        if (graph) {
+            const std::vector<uint64_t> key = {4, m, (uint64_t)mixed, (uint64_t)flexible, (uint64_t)quirks, A->uid,
+                                               (uint64_t)A->v32, (uint64_t)A->spmv_variant << 8 | A->spmv_variant32,
+                                               M ? M->uid : 0, M ? (uint64_t)M->io_permuted : 0,
+                                               M ? (uint64_t)M->have32 : 0, (uint64_t)V, (uint64_t)Z, (uint64_t)Z32,
+                                               (uint64_t)w, (uint64_t)w32, (uint64_t)st.misc,
+                                               (uint64_t)(hist_on ? ctx->hist : nullptr), hist_cap, ld};
+            run_graph(ctx, key, cycle);
+        } else {
+            cycle();
         }
         double misc[kGmMisc];
         PP_HIP(hipMemcpyAsync(ctx->pinned, st.misc, kGmMisc * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -320,10 +320,16 @@ static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A, int dtype) {
                 A->ell.v32 = dalloc<float>((size_t)A->ell.width * A->ell.n);
                 ell_fill_values(ctx, A->ell, PP_F32, A->rp, A->v32);
             }
+            uint64_t csr_bytes = 0, ell_bytes = 0;
+            variant = best_var;
+            pp_csr_spmv_bytes(A, dtype, &csr_bytes);
             variant = kVarEll;
+            pp_csr_spmv_bytes(A, dtype, &ell_bytes);
             const float ms = time_cold_ms(
                 ctx, [&] { launch_spmv(ctx, A, dtype, 1.0, x, 0.0, y, -1, nullptr, false); }, 3, !A->tune_warm);
-            if (ms < best) {
+            // within the timing noise (3%) the form that streams fewer bytes,
+            // so the choice does not flip between runs
+            if (ms < best || (ms < 1.03f * best && ell_bytes < csr_bytes)) {
                 best = ms;
                 best_var = kVarEll;
             }

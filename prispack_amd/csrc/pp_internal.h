@@ -565,6 +565,10 @@ void red_step(pp_ctx *ctx, unsigned np, int parity, bool publish, uint32_t n, co
 // State block 0 for a fresh reduced solve, or (resume) for one that
 // continues from S_CG_GOLD / S_CG_ALPHA / S_CG_FIRST, gamma scaled by gscale.
 void red_init(pp_ctx *ctx, bool resume, double gscale, bool last);
+// ctl words, tolerance, ||rin||^2 (bb >= 0) and iteration state of a
+// reduced solve, then x = 0 and p, s rescaled (resume) or zeroed: two launches
+void red_begin(pp_ctx *ctx, uint32_t maxit, double tol, double bb, bool resume, double scale, int dtype,
+               uint32_t n1, void *x, void *p, void *s);
 template <class T>
 void red_post(pp_ctx *ctx, uint32_t n0, uint32_t n1, const T *x, const T *s1, const T *rin, T *z, T *e);
 // lvl0_hi > lvl0_lo: also u[i] = r_new[i] op diag0[i] over [lvl0_lo, lvl0_hi)

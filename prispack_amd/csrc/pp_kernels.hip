@@ -1559,6 +1559,56 @@ __global__ void k_red_init(double *scal, int resume, double gscale, int last) {
     st[RS_CHECK] = 1.0;
 }
 
+// Everything a reduced solve sets before its first iteration in one launch:
+// the control words (check 1, last when maxit <= 1), the tolerance, ||rin||^2
+// (when given, bb >= 0) and the iteration state (as k_red_init).
+__global__ void k_red_begin(int *ctl, double *scal, int maxit, double tol, double bb, int resume, double gscale) {
+    if (threadIdx.x < C_COUNT) ctl[threadIdx.x] = 0;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    ctl[C_CHECK] = 1;
+    ctl[C_LAST] = maxit <= 1 ? 1 : 0;
+    ctl[C_MAXIT] = maxit;
+    scal[S_TOL] = tol;
+    if (bb >= 0.0) scal[S_BB] = bb;
+    double *st = scal + S_RED0;
+    if (resume) {
+        st[RS_GOLD] = scal[S_CG_GOLD] * gscale;
+        st[RS_AOLD] = scal[S_CG_ALPHA];
+        st[RS_FIRST] = scal[S_CG_FIRST];
+    } else {
+        st[RS_GOLD] = 0.0;
+        st[RS_AOLD] = 1.0;
+        st[RS_FIRST] = 1.0;
+    }
+    st[RS_LAST] = maxit <= 1 ? 1.0 : 0.0;
+    st[RS_CHECK] = 1.0;
+}
+
+// x = 0; p, s = resume ? scale * (p, s) : 0
+template <class T>
+__global__ void k_red_vectors(uint32_t n, T *__restrict__ x, T *__restrict__ p, T *__restrict__ s, int resume,
+                              T scale) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] = 0;
+        p[i] = resume ? scale * p[i] : (T)0;
+        s[i] = resume ? scale * s[i] : (T)0;
+    }
+}
+
+void red_begin(pp_ctx *ctx, uint32_t maxit, double tol, double bb, bool resume, double scale, int dtype,
+               uint32_t n1, void *x, void *p, void *s) {
+    k_red_begin<<<1, 64, 0, ctx->stream>>>(ctx->ctl, ctx->scal, (int)maxit, tol, bb, resume ? 1 : 0, scale * scale);
+    const unsigned g = grid_for(ctx, n1, 4);
+    if (dtype == PP_F64)
+        k_red_vectors<double><<<g, kBlock, 0, ctx->stream>>>(n1, (double *)x, (double *)p, (double *)s,
+                                                              resume ? 1 : 0, scale);
+    else
+        k_red_vectors<float><<<g, kBlock, 0, ctx->stream>>>(n1, (float *)x, (float *)p, (float *)s, resume ? 1 : 0,
+                                                            (float)scale);
+    PP_HIP(hipGetLastError());
+}
+
 void red_init(pp_ctx *ctx, bool resume, double gscale, bool last) {
     k_red_init<<<1, 1, 0, ctx->stream>>>(ctx->scal, resume ? 1 : 0, gscale, last ? 1 : 0);
     PP_HIP(hipGetLastError());

@@ -293,9 +293,13 @@ PcgResult pcg_sr_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const T *b, T *x,
 // right-hand side, so p and s are rescaled by it and gamma_old by its square;
 // alpha is scale-free).  Without it every refinement step would restart the
 // Krylov iteration and lose its convergence.
+//
+// rin_norm2: ||rin||^2 when the caller knows it (a normalised residual), else
+// negative and it is computed.
 template <class T>
 PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, double tol, uint32_t maxit,
-                             uint32_t check_every, bool use_graph, bool resume = false, double scale = 1.0) {
+                             uint32_t check_every, bool use_graph, bool resume = false, double scale = 1.0,
+                             double rin_norm2 = -1.0) {
     const uint32_t n0 = R->n0, n1 = R->n1, n = n0 + n1;
     const int DT = dt<T>();
     const int base = sizeof(T) == 8 ? W_RD64 : W_RD32;
@@ -307,21 +311,11 @@ PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, do
     T *r = z + n0;
     const T *s1 = sizeof(T) == 8 ? (const T *)R->s1_64 : (const T *)R->s1_32;
     const T *wt = sizeof(T) == 8 ? (const T *)R->dt_64 : (const T *)R->dt_32;
-    ctl_init(ctx, 1, maxit <= 1 ? 1 : 0, maxit);
-    set_slot(ctx, S_TOL, tol);
     launch_spmv(ctx, R->KF, DT, 1.0, rin, 0.0, r, -1, nullptr, false);
     // the test is on the inner system's own residual: ||rin - A e|| is
     // ||Dt^1/2 r|| (the first colour's part vanishes), relative to ||rin||
-    launch_dot(ctx, DT, n, rin, rin, S_BB);
-    launch_fill_zero(ctx, x, (size_t)n1 * sizeof(T));
-    if (resume) {
-        launch_axpby(ctx, DT, n1, scale, p, 0.0, p, p);
-        launch_axpby(ctx, DT, n1, scale, s, 0.0, s, s);
-    } else {
-        launch_fill_zero(ctx, p, (size_t)n1 * sizeof(T));
-        launch_fill_zero(ctx, s, (size_t)n1 * sizeof(T));
-    }
-    red_init(ctx, resume, scale * scale, maxit <= 1);
+    if (rin_norm2 < 0.0) launch_dot(ctx, DT, n, rin, rin, S_BB);
+    red_begin(ctx, maxit, tol, rin_norm2, resume, scale, DT, n1, x, p, s);
     // KA and KB in sliced-column form where they were built so, else CSR
     const bool ell = R->ka.ok && R->kb.ok;
     const unsigned np = ell ? ell_grid(R->kb, true) : spmv_grid(R->KB);
@@ -463,9 +457,8 @@ PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, cons
             double *rin64 = (double *)ws_get(ctx, W_RIN64, (size_t)n * sizeof(double));
             double *e64 = (double *)ws_get(ctx, W_E64, (size_t)n * sizeof(double));
             launch_axpby(ctx, PP_F64, n, 1.0 / rnorm, r, 0.0, r, rin64);
-            launch_fill_zero(ctx, e64, (size_t)n * sizeof(double));
             in = pcg_reduced_device<double>(ctx, R, rin64, e64, o.tol * bnorm / rnorm, inner_max, o.check_every,
-                                            graph);
+                                            graph, false, 1.0, 1.0);
             inner_total += in.iters;
             ++outer;
             ++steps64;
@@ -479,7 +472,7 @@ PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, cons
         }
         if (f32) launch_scale_convert(ctx, n, r, 1.0 / rnorm, (float *)rin);
         else launch_axpby(ctx, PP_F64, n, 1.0 / rnorm, r, 0.0, r, rin);
-        launch_fill_zero(ctx, e, (size_t)n * sizeof(Tin));
+        if (!R) launch_fill_zero(ctx, e, (size_t)n * sizeof(Tin));  // x0 = 0 (the reduced solve writes all of e)
         // FP32: a margin for the drift between the recurrence and the true
         // residual; FP64 inner solves stop where a plain FP64 solve would
         const double floor_tol = R ? std::max(inner_tol, step) : inner_tol;
@@ -489,7 +482,7 @@ PcgResult pcg_refine(pp_ctx *ctx, pp_csr *A, pp_precond *M, ReducedForm *R, cons
         if (resume) inner_max = std::min<uint32_t>(inner_max, std::max<uint32_t>(512, 4 * inner_total));
         if (R)
             in = pcg_reduced_device<Tin>(ctx, R, rin, e, want, inner_max, o.check_every, graph, resume,
-                                         resume ? prev_rnorm / rnorm : 1.0);
+                                         resume ? prev_rnorm / rnorm : 1.0, 1.0);  // rin = r / ||r||
         else if (o.cg_variant == PP_CG_REFERENCE)
             in = pcg_device<Tin>(ctx, A, M, o.flexible != 0, rin, e, want, inner_max, o.check_every, false,
                                  f32 ? W_R32 : W_R, graph);

@@ -183,7 +183,7 @@ bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
         if (len <= tail) return false;  // a row with no sliced entry
         width = std::max(width, len - tail);
     }
-    if (n == 0 || width > 16) return false;
+    if (n == 0 || width > 16 || (uint64_t)width * n >= (1ull << 32)) return false;
     if ((double)width * n > max_fill * (double)(rp[n] - (uint64_t)tail * n)) return false;
     std::vector<int16_t> off((size_t)width * n);
     std::vector<double> val((size_t)width * n), dg(diag_last ? n : 0);

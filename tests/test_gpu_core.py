@@ -182,6 +182,33 @@ def test_spmv_variants_and_solves_agree(pp, torch, name):
         assert np.linalg.norm(xs - runs[0][2]) <= 1e-7 * np.linalg.norm(runs[0][2]), var
 
 
+def test_sliced_column_copy_follows_value_updates(pp, torch):
+    # new values on the same pattern (on the host, then on the device) reach
+    # the sliced-column copy in both precisions
+    import ctypes as C
+    from prispack_amd import _capi
+    L = _capi.lib()
+    A = ppo.laplacian(30, 30, 30)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    for dt in (0, 1):
+        assert L.pp_csr_set_spmv_variant(M.h, dt, 8) == 0
+    x = ppo.uniform(n, 31)
+    y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+    M.MatPVec(dev(torch, x, torch.float32), y32)  # the FP32 copies exist before the updates
+    for k, on_device in ((2.0, False), (0.5, True)):
+        v = A[2] * k
+        vals = dev(torch, v) if on_device else v
+        M.set_values(vals)
+        y = torch.zeros(n, dtype=torch.float64, device="cuda")
+        M.MatPVec(dev(torch, x), y)
+        want = ppo.spmv((A[0], A[1], v), x)
+        assert np.array_equal(host(y), want), k
+        y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+        M.MatPVec(dev(torch, x, torch.float32), y32)
+        np.testing.assert_allclose(host(y32), want, rtol=0, atol=1e-5 * (1 + np.abs(want).max()))
+
+
 def test_blas1(pp, torch):
     from prispack_amd import _capi
     import ctypes as C

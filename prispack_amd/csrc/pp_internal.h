@@ -490,8 +490,9 @@ void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes);
 // Average duration (ms) of fn()'s launches on ctx->stream with the last-level
 // cache cleared before each one (a read of `flush_bytes` of other memory), so
 // that operands are streamed from HBM as they are inside a solve.
-// mean time of fn over reps launches after one warm-up; cold: the caches
-// (L2 and the last-level MALL) flushed before each
+// shortest time of fn over reps launches after one warm-up (timing noise
+// only ever adds); cold: the caches (L2 and the last-level MALL) flushed
+// before each
 float time_cold_ms(pp_ctx *ctx, const std::function<void()> &fn, int reps, bool cold = true);
 // Host setup work split over threads: fn(begin, end) on contiguous ranges of
 // [0, n), at most 16 threads and at least `grain` items each (small n runs

@@ -557,7 +557,7 @@ float time_cold_ms(pp_ctx *ctx, const std::function<void()> &fn, int reps, bool 
         launch_fill_zero(ctx, ctx->flush_buf, kFlushBytes + sizeof(uint4));
     }
     uint4 *buf = (uint4 *)ctx->flush_buf;
-    float total = 0.f;
+    float best = 0.f;
     fn();  // warm-up (code, TLB)
     for (int r = 0; r < reps; ++r) {
         if (cold)
@@ -569,9 +569,9 @@ float time_cold_ms(pp_ctx *ctx, const std::function<void()> &fn, int reps, bool 
         PP_HIP(hipEventSynchronize(ctx->ev1));
         float ms = 0.f;
         PP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        total += ms;
+        if (r == 0 || ms < best) best = ms;
     }
-    return total / reps;
+    return best;
 }
 
 // ---------------------------------------------------------------------------

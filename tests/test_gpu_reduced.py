@@ -212,3 +212,26 @@ def test_reduced_edge_cases(pp, torch, precision):
     with pytest.raises(PrisPackError) as e:
         cg.solve(M, torch.tensor(bad, device="cuda"), torch.zeros(n, dtype=torch.float64, device="cuda"))
     assert e.value.status == PP_ERR_NOT_FINITE
+
+
+def test_reduced_stream_launches_match_graphs(pp, torch):
+    # the iteration bodies launched on the stream (no graph capture) give
+    # the same iterates as the captured, replayed bodies, in both precisions
+    from prispack_amd._capi import PP_LAUNCH_STREAM
+    A = ppo.laplacian(30, 29, 28)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 12))
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M, ordering=1, make_f32=True)
+    for prec in ("fp64", "mixed"):
+        out = []
+        for stream in (False, True):
+            cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=3000, precision=prec).setCGVariant("reduced")
+            if stream:
+                cg.opts.launch = PP_LAUNCH_STREAM
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            cg.solve(M, torch.tensor(b, device="cuda"), x)
+            assert cg.getConvergence() and cg.report.true_relres < 1e-8
+            out.append((cg.report.inner_iters, x.cpu().numpy()))
+        assert out[0][0] == out[1][0], prec
+        assert np.array_equal(out[0][1], out[1][1]), prec

@@ -238,8 +238,8 @@ def extra_records(torch, pp, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is described "
-              "in DESIGN.md and is not part of this build.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the solver path is single-GPU; "
+              "sharding across GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -76,7 +76,7 @@ int pp_csr_create(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *row
 /* Replace the values (same pattern).  on_device != 0: values is a device pointer. */
 int pp_csr_set_values(pp_ctx *ctx, pp_csr *A, const double *values, int on_device);
 int pp_csr_dims(const pp_csr *A, uint32_t *nrow, uint32_t *ncol, uint32_t *nnz);
-/* Bytes one y = A x (beta = 0) streams through HBM in this build's storage:
+/* Bytes one y = A x (beta = 0) streams through HBM in the library's storage:
  * values, column indices (16-bit block-relative offsets where a row block's
  * column span fits, else 32-bit), row pointers, row-block table, x and y.
  * The algorithmic traffic the SpMV roofline is priced on.  No reference
@@ -200,7 +200,7 @@ int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void 
  * in place (TriangularPrecondition, BasePreconditon.h:200-303) */
 int pp_precond_apply_part(pp_ctx *ctx, pp_precond *M, int part, int dtype, void *x);
 int pp_precond_info_get(const pp_precond *M, pp_precond_info *info);
-/* Bytes one M^{-1} x streams through HBM in this build's sweep storage (both
+/* Bytes one M^{-1} x streams through HBM in the library's sweep storage (both
  * sweeps: values, 16/32-bit dependency indices, row pointers, level row map
  * where read, diagonal, block tables, vectors).  in_solve != 0: as applied
  * inside a solve on the factor's own matrix (no colour-order gather/scatter).

@@ -5,8 +5,8 @@ IEEE flags and no OpenMP, so every result is deterministic).
 Run in the container that holds /root/reference:
     make -C oracle ref && python tests/golden/make_golden.py
 
-Inputs come from the build's seeded generators (SplitMix64; see oracle/ppo.c),
-so the GPU box can regenerate them without the reference.  Matrices come from
+Inputs come from the library's seeded generators (SplitMix64; see oracle/ppo.c),
+so a machine without the reference can regenerate them.  Matrices come from
 the reference's generateLaplacianCSR where it has one; the upwind operator has
 no reference generator and is produced by the oracle, then handed to the
 reference's ILUT/GMRES as a CSR.

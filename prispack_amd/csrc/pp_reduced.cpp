@@ -261,8 +261,10 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
         // diagonal at n0 + q
         if (ell_build(H.ka_rp, H.ka_ci, H.ka_v, n0, false, 0, R->ka))
             if (!ell_build(H.kb_rp, H.kb_ci, H.kb_v, 0, true, n0, R->kb)) R->ka.free_all();
-        R->KA = csr_from_vectors(ctx, n0, n, std::move(H.ka_rp), std::move(H.ka_ci), std::move(H.ka_v), 2);
-        R->KB = csr_from_vectors(ctx, n1, n, std::move(H.kb_rp), std::move(H.kb_ci), std::move(H.kb_v), 2);
+        // the CSR forms are timed only when the iteration runs them
+        const int tune = R->ka.ok ? 0 : 2;
+        R->KA = csr_from_vectors(ctx, n0, n, std::move(H.ka_rp), std::move(H.ka_ci), std::move(H.ka_v), tune);
+        R->KB = csr_from_vectors(ctx, n1, n, std::move(H.kb_rp), std::move(H.kb_ci), std::move(H.kb_v), tune);
         R->KF = csr_from_vectors(ctx, n1, n, std::move(H.kf_rp), std::move(H.kf_ci), std::move(H.kf_v), 0);
         R->KE = csr_from_vectors(ctx, n0, n, std::move(H.ke_rp), std::move(H.ke_ci), std::move(H.ke_v), 0);
         R->s1_64 = dalloc<double>(n1);

@@ -450,6 +450,9 @@ double *gmres_coeff_buffer(pp_ctx *ctx);
 void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w,
                    double *dest);
 void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h, double *w);
+// w -= V h1 followed by dest = V^T w, in one pass over V where it can
+void launch_gemv_sub_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h1,
+                       double *w, double *dest);
 void launch_gemv_add(pp_ctx *ctx, uint32_t n, uint32_t ncol, const float *Z32, const double *Z64, size_t ld,
                      const double *y, double *x);
 void read_slots(pp_ctx *ctx, int first, int count, double *out);

@@ -107,6 +107,67 @@ void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, si
     PP_HIP(hipGetLastError());
 }
 
+// The first pass's update and the second pass's products in one sweep over
+// V (ncol <= kFuseCols): w[r] -= sum_c V[r, c] h[c] exactly as k_gemv_sub
+// forms it, then partial[c][block] of V[r, c] * w[r] with the new w exactly
+// as k_gemv_t_partial forms it (same rows per workgroup, same order), so
+// the results are those of the two separate launches, with one read of V
+// fewer.  Each thread keeps its row's ncol basis entries in registers
+// between the two uses.
+constexpr int kFuseCols = 32;
+__global__ __launch_bounds__(256) void k_gemv_sub_t(uint32_t n, uint32_t ncol, const double *__restrict__ V,
+                                                    size_t ld, const double *__restrict__ h,
+                                                    double *__restrict__ w, double *partial, const int *stop) {
+    __shared__ double sh[kFuseCols][4];
+    if (stop && *stop) return;
+    const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+    double acc[kFuseCols];
+#pragma unroll
+    for (int c = 0; c < kFuseCols; ++c) acc[c] = 0.0;
+    for (uint32_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        double v[kFuseCols];
+        double a = 0.0;
+#pragma unroll
+        for (int c = 0; c < kFuseCols; ++c)
+            if ((uint32_t)c < ncol) {
+                v[c] = V[ld * c + r];
+                a += v[c] * h[c];
+            }
+        const double wr = w[r] - a;
+        w[r] = wr;
+#pragma unroll
+        for (int c = 0; c < kFuseCols; ++c)
+            if ((uint32_t)c < ncol) acc[c] += v[c] * wr;
+    }
+#pragma unroll
+    for (int c = 0; c < kFuseCols; ++c)
+        if ((uint32_t)c < ncol) {
+            const double t = wsum(acc[c]);
+            if ((threadIdx.x & 63) == 0) sh[c][threadIdx.x >> 6] = t;
+        }
+    __syncthreads();
+    if (threadIdx.x < ncol)
+        partial[(size_t)threadIdx.x * gridDim.x + blockIdx.x] =
+            ((sh[threadIdx.x][0] + sh[threadIdx.x][1]) + sh[threadIdx.x][2]) + sh[threadIdx.x][3];
+}
+
+// w -= V h1, then dest = V^T w: fused where the columns fit the registers
+void launch_gemv_sub_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h1,
+                       double *w, double *dest) {
+    if (ncol > (uint32_t)kFuseCols) {
+        launch_gemv_sub(ctx, n, ncol, V, ld, h1, w);
+        launch_gemv_t(ctx, n, ncol, V, ld, w, dest);
+        return;
+    }
+    double *buf = gmres_coeff_buffer(ctx);
+    double *partial = buf + 640;
+    k_gemv_sub_t<<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
+    k_gemv_t_reduce<<<ncol, 256, 0, ctx->stream>>>(kGemvBlocks, partial, dest ? dest : buf, ctx->stop_word);
+    PP_HIP(hipGetLastError());
+}
+
+
 // x[r] += sum_c Z[r, c] y[c] (Z in FP32 or FP64), columns in ascending order
 // like DenseMatrix::MatVec (DenseMatrix.cpp:208-224).
 template <class T>

@@ -643,8 +643,7 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
                 }
                 // CGS2: h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2
                 launch_gemv_t(ctx, n, j + 1, V, ld, w, st.h1);
-                launch_gemv_sub(ctx, n, j + 1, V, ld, st.h1, w);
-                launch_gemv_t(ctx, n, j + 1, V, ld, w, st.h2);
+                launch_gemv_sub_t(ctx, n, j + 1, V, ld, st.h1, w, st.h2);  // w -= V h1; h2 = V^T w
                 launch_gemv_sub(ctx, n, j + 1, V, ld, st.h2, w);
                 launch_dot(ctx, PP_F64, n, w, w, S_TMP2);
                 launch_gmres_step(ctx, st, j, quirks, hist_on ? ctx->hist : nullptr, j, hist_cap);

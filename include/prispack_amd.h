@@ -183,7 +183,8 @@ typedef struct {
     double setup_ms;        /* host factorization + analysis + upload */
     uint32_t fail_row;      /* row of a failed pivot */
     int32_t sweep_variant;  /* staged-sweep loads chosen at setup: bit 0 non-temporal,
-                               bit 1 16-bit dependency offsets, bit 2 batched loads */
+                               bit 1 16-bit dependency offsets, bit 2 batched loads;
+                               8: wide contiguous levels in sliced-column form */
     int32_t fused_levels;   /* 1: last forward and first backward level share one launch */
     int32_t trsv;           /* the SpTRSV schedule in use (PP_TRSV_AUTO resolved at setup) */
     float phase_ms[4];      /* setup phases: ordering, factorization, sweep analysis + upload,
@@ -194,6 +195,13 @@ void pp_precond_opts_default(pp_precond_opts *o);
 int pp_precond_create(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *opts, pp_precond **out);
 /* setup(): factorization on the host, level analysis, upload (BasePreconditon.h:102) */
 int pp_precond_setup(pp_ctx *ctx, pp_precond *M);
+/* Pins the level-sweep variant of both precisions instead of the timed choice
+ * (0..7 the staged kernel's loads, 8 the sliced-column form of the wide
+ * contiguous levels, the other levels staged with the default loads); every
+ * variant gives the same result bit for bit.  PP_ERR_INVALID outside 0..8 or
+ * for 8 when no level has that form; PP_ERR_NOT_READY before setup.  No
+ * reference counterpart. */
+int pp_precond_set_sweep_variant(pp_precond *M, int variant);
 /* y = M^{-1} x (MInvSolve in->out, BasePreconditon.h:113-128); y may equal x */
 int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
 /* part: 0 lower (MSolveLower), 1 diagonal (MSolveDiagonal), 2 upper (MSolveUpper),

@@ -103,6 +103,7 @@ SIGS = {
     "pp_precond_apply": (C.c_int, [vp, vp, C.c_int, vp, vp]),
     "pp_precond_apply_part": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
     "pp_precond_info_get": (C.c_int, [vp, C.POINTER(PrecondInfo)]),
+    "pp_precond_set_sweep_variant": (C.c_int, [vp, C.c_int]),
     "pp_precond_factor": (C.c_int, [vp, C.c_int, u32p, u32p, u32p, f64p, f64p]),
     "pp_precond_destroy": (None, [vp]),
     "pp_factor_host": (C.c_int, [C.POINTER(PrecondOpts), u32, u32p, u32p, f64p, C.POINTER(vp)]),

@@ -195,6 +195,20 @@ struct DevSweep {
     // the index arrays (perm, rp, ci, blocks, desc, ci16, blk_base) belong to
     // another sweep of the same rows (the FP32 sweep shares the FP64 one's)
     bool borrowed = false;
+    // sliced-column form of the wide, contiguous levels (variant kVarEll):
+    // row t of level l is original row row0 + t; its entry j at [j * nrows
+    // + t] with dependency column row0 + t + cbase + off (16-bit offset),
+    // padded with zeros on the row's last dependency
+    struct EllLevel {
+        bool ok = false;
+        uint32_t nrows = 0, width = 0;
+        uint64_t entries = 0;  // the level's dependencies (unpadded)
+        int64_t row0 = 0, cbase = 0;
+        int16_t *off = nullptr;  // shared with the FP32 sweep (borrowed)
+        T *val = nullptr;
+    };
+    std::vector<EllLevel> ell;
+    int ell_fallback = kVarDefault;  // the staged variant of the other levels when variant == kVarEll
     void free_all();
 };
 
@@ -390,6 +404,7 @@ struct pp_precond {
     uint64_t Aperm_version = 0;
     bool io_permuted = false;
     pp::ReducedForm *reduced = nullptr;  // two-colour Schur-complement form (IC(0) only)
+    bool sweep_pinned = false;           // pp_precond_set_sweep_variant: the FP32 sweeps are not timed
     float phase_ms[4] = {0, 0, 0, 0};     // setup: ordering, factorization, analysis + upload, schedule
     // PP_TRSV_BLOCK storage (built when the rows chunk, see ChunkSet)
     pp::ChunkSet chunks;
@@ -521,6 +536,8 @@ void launch_dot(pp_ctx *ctx, int dtype, uint32_t n, const void *x, const void *y
 void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void *x, double beta, void *y,
                 int dot_slot, const void *dotw, const void *aux, const void *wgt, unsigned grid = 0);
 unsigned ell_grid(const EllOp &E, bool dot);
+template <class T>
+void ell_fill_rows(uint32_t n, uint32_t width, const uint32_t *rp, const T *v, T *out);
 // E's values (dtype) from the CSR values v of the matrix it was built from
 void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp, const void *v);
 // host form -> EllOp (FP64 values on the device); false (nothing kept) when

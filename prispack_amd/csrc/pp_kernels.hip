@@ -407,6 +407,19 @@ __global__ void k_ell_fill(uint32_t n, uint32_t width, const uint32_t *__restric
     }
 }
 
+// out[j * n + i] from the CSR values v of rows i (row pointers rp), on the
+// null stream, waited for (setup only)
+template <class T>
+void ell_fill_rows(uint32_t n, uint32_t width, const uint32_t *rp, const T *v, T *out) {
+    const uint64_t total = (uint64_t)width * n;
+    const unsigned g = (unsigned)std::min<uint64_t>((total + kBlock * 4 - 1) / (kBlock * 4), 8192);
+    k_ell_fill<T><<<std::max(1u, g), kBlock, 0, 0>>>(n, width, rp, v, out);
+    PP_HIP(hipGetLastError());
+    PP_HIP(hipStreamSynchronize(0));
+}
+template void ell_fill_rows<double>(uint32_t, uint32_t, const uint32_t *, const double *, double *);
+template void ell_fill_rows<float>(uint32_t, uint32_t, const uint32_t *, const float *, float *);
+
 void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp, const void *v) {
     const unsigned g = grid_for(ctx, (uint64_t)E.width * E.n, 4);
     if (dtype == PP_F64)
@@ -766,6 +779,52 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
     }
 }
 
+// One contiguous level in sliced-column form (DevSweep::EllLevel): row
+// row0 + t per thread, entry j of all rows contiguous (coalesced loads, no
+// LDS stage).  The same operation sequence as k_sweep_stream (b, the
+// pre-division, the products subtracted in storage order each rounded, the
+// row's diagonal mode, the next sweep's first-level mode), so the results
+// are the same bit for bit; padding entries subtract a zero.
+template <class T, int MAXW>
+__global__ __launch_bounds__(256) void k_sweep_ell(uint32_t nrows, uint32_t width, const int16_t *__restrict__ off,
+                                                   const T *__restrict__ val, long long row0, long long cbase,
+                                                   const T *__restrict__ diag, int mode, const T *b, T *x,
+                                                   const int *stop, int mode2, const T *__restrict__ diag2) {
+#pragma clang fp contract(off)
+    if (stop && *stop) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows) return;
+    const long long i = row0 + t;
+    int16_t o[MAXW];
+    T v[MAXW], xv[MAXW];
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j)
+        if (j < (int)width) {
+            const size_t e = (size_t)j * nrows + t;
+            o[j] = off[e];
+            v[j] = val[e];
+        }
+    const T pb = b[i];
+    const T pd = mode != 1 ? diag[i] : (T)0;
+    const T pd2 = (mode2 == 0 || mode2 == 2 || mode2 == 3) ? diag2[i] : (T)0;
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j)
+        if (j < (int)width) xv[j] = x[i + cbase + o[j]];
+    T s = pb;
+    if (mode == 3) s = s / pd;
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j)
+        if (j < (int)width) {
+            const T p = v[j] * xv[j];
+            s = s - p;
+        }
+    if (mode == 0) s = s / pd;
+    else if (mode == 2) s = s * pd;
+    if (mode2 == 0 || mode2 == 3) s = s / pd2;
+    else if (mode2 == 2) s = s * pd2;
+    x[i] = s;
+}
+
 // Levels [l0, l1) of one sweep; mode2 >= 0 appends the next sweep's
 // dependency-free first level (diag2) to every row of these levels.
 template <class T>
@@ -775,19 +834,32 @@ void launch_levels(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x
     // launched elsewhere (a preconditioner setup, a plain apply) always runs,
     // whatever stop word an earlier solve left behind
     const int *stop = ctx->stop_word;
+    const bool ell = s.variant == kVarEll;
+    const int var = ell ? s.ell_fallback : s.variant;  // the staged kernel's loads on the other levels
     for (uint32_t l = l0; l < l1; ++l) {
         uint32_t k0 = s.level_ptr[l], k1 = s.level_ptr[l + 1];
         const bool contiguous = l < s.level_off.size() && s.level_off[l] != DevSweep<T>::kNoOffset;
+        if (ell && l < s.ell.size() && s.ell[l].ok) {
+            const auto &E = s.ell[l];
+            const unsigned g = (E.nrows + kBlock - 1) / kBlock;
+            if (E.width <= 8)
+                k_sweep_ell<T, 8><<<g, kBlock, 0, ctx->stream>>>(E.nrows, E.width, E.off, E.val, E.row0, E.cbase,
+                                                                 s.diag, mode, b, x, stop, mode2, diag2);
+            else
+                k_sweep_ell<T, 16><<<g, kBlock, 0, ctx->stream>>>(E.nrows, E.width, E.off, E.val, E.row0, E.cbase,
+                                                                  s.diag, mode, b, x, stop, mode2, diag2);
+            continue;
+        }
         if (s.blocks && 2 * l + 1 < s.level_blk.size() && s.level_blk[2 * l + 1] > s.level_blk[2 * l]) {
             const uint32_t b0 = s.level_blk[2 * l], b1 = s.level_blk[2 * l + 1];
             const unsigned g = xcd_grid(b1 - b0, 8192u);
-            const uint16_t *c16 = (s.variant & kVarIdx16) ? s.ci16 : nullptr;
+            const uint16_t *c16 = (var & kVarIdx16) ? s.ci16 : nullptr;
             const long long off = contiguous ? s.level_off[l] : 0;
             const uint32_t *perm = contiguous ? nullptr : s.perm;
 #define PP_SWEEP(NTV, BV)                                                                                         \
     k_sweep_stream<T, NTV, BV><<<g, kBlock, 0, ctx->stream>>>(perm, s.rp, s.ci, s.val, s.diag, mode, b, x, s.desc, \
                                                              c16, b0, b1, off, stop, mode2, diag2)
-            switch (s.variant & (kVarNT | kVarBatch)) {
+            switch (var & (kVarNT | kVarBatch)) {
                 case 0: PP_SWEEP(false, false); break;
                 case kVarNT: PP_SWEEP(true, false); break;
                 case kVarBatch: PP_SWEEP(false, true); break;

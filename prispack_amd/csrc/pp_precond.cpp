@@ -2,7 +2,9 @@
 // conversion to gather-form sweeps, dependency-level analysis and upload
 // (rows stored in level order so each level is a contiguous, coalesced range).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <climits>
 #include <cstring>
 
 #include "pp_internal.h"
@@ -25,6 +27,11 @@ void DevSweep<T>::free_all() {
         dfree(blk_base);
         dfree(desc);
     }
+    for (auto &e : ell) {
+        if (!borrowed) dfree(e.off);
+        dfree(e.val);
+    }
+    ell.clear();
     borrowed = false;
     dfree(val);
     dfree(diag);
@@ -160,6 +167,54 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
         d.diag = dalloc<T>(n);
         PP_HIP(hipMemcpy(d.diag, dg.data(), n * sizeof(T), hipMemcpyHostToDevice));
     }
+    // sliced-column form of the wide levels whose rows are contiguous
+    d.ell.assign(d.nlev, typename DevSweep<T>::EllLevel());
+    for (uint32_t l = 0; l < d.nlev; ++l) {
+        const uint32_t k0 = lp[l], k1 = lp[l + 1], nr = k1 - k0;
+        if (nr < kStreamLevelRows || d.level_off[l] == DevSweep<T>::kNoOffset) continue;
+        const int64_t row0 = (int64_t)k0 + d.level_off[l];
+        uint32_t w = 0;
+        bool ok = true;
+        for (uint32_t k = k0; k < k1 && ok; ++k) {
+            w = std::max(w, rp[k + 1] - rp[k]);
+            ok = rp[k + 1] > rp[k];
+        }
+        if (!ok || w > 16 || (double)w * nr > 1.25 * (double)(rp[k1] - rp[k0])) continue;
+        const int64_t cbase = (int64_t)ci[rp[k0]] - row0;
+        std::vector<int16_t> off((size_t)w * nr);
+        std::atomic<bool> fits{true};
+        parallel_for(nr, 1u << 15, [&](size_t t0, size_t t1) {
+            for (size_t t = t0; t < t1; ++t) {
+                const uint32_t k = k0 + (uint32_t)t;
+                int16_t last = 0;
+                for (uint32_t j = 0; j < w; ++j) {
+                    const size_t e = (size_t)j * nr + t;
+                    if (rp[k] + j < rp[k + 1]) {
+                        const int64_t o = (int64_t)ci[rp[k] + j] - (row0 + (int64_t)t) - cbase;
+                        if (o < INT16_MIN || o > INT16_MAX) {
+                            fits = false;
+                            return;
+                        }
+                        last = (int16_t)o;
+                    }
+                    off[e] = last;
+                }
+            }
+        });
+        if (!fits) continue;
+        auto &E = d.ell[l];
+        E.nrows = nr;
+        E.entries = rp[k1] - rp[k0];
+        E.width = w;
+        E.row0 = row0;
+        E.cbase = cbase;
+        E.off = dalloc<int16_t>(off.size());
+        E.val = dalloc<T>(off.size());
+        PP_HIP(hipMemcpy(E.off, off.data(), off.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+        // the values from the level's rows already on the device (zeros in the padding)
+        ell_fill_rows<T>(nr, w, d.rp + k0, d.val, E.val);
+        E.ok = true;
+    }
 }
 // The FP32 sweep of the same rows as an uploaded FP64 sweep: the index
 // arrays are shared, the values and pivots rounded to FP32 on the device
@@ -186,6 +241,21 @@ static void derive_sweep_f32(pp_ctx *ctx, const DevSweep<double> &s, DevSweep<fl
     d.max_level_rows = s.max_level_rows;
     d.val = dalloc<float>(s.nnz);
     if (s.nnz) launch_convert(ctx, s.nnz, PP_F64, s.val, PP_F32, d.val);
+    d.ell.assign(s.ell.size(), DevSweep<float>::EllLevel());
+    for (size_t l = 0; l < s.ell.size(); ++l) {
+        const auto &S = s.ell[l];
+        if (!S.ok) continue;
+        auto &E = d.ell[l];
+        E.nrows = S.nrows;
+        E.entries = S.entries;
+        E.width = S.width;
+        E.row0 = S.row0;
+        E.cbase = S.cbase;
+        E.off = S.off;
+        E.val = dalloc<float>((size_t)S.width * S.nrows);
+        launch_convert(ctx, S.width * S.nrows, PP_F64, S.val, PP_F32, E.val);
+        E.ok = true;
+    }
     if (s.diag) {
         d.diag = dalloc<float>(s.n);
         launch_convert(ctx, s.n, PP_F64, s.diag, PP_F32, d.diag);
@@ -544,6 +614,20 @@ int tune_sweep_pair(pp_ctx *ctx, pp_precond *M, DevSweep<T> &lo, DevSweep<T> &up
                 best_var = var;
             }
         }
+        bool have_ell = false;
+        for (const auto &e : lo.ell) have_ell |= e.ok;
+        for (const auto &e : up.ell) have_ell |= e.ok;
+        if (have_ell) {  // the sliced-column levels (the others keep the best staged variant)
+            lo.variant = up.variant = kVarEll;
+            lo.ell_fallback = up.ell_fallback = best_var;
+            const float ms = time_cold_ms(ctx, [&] {
+                launch_sweep_pair<T>(ctx, lo, M->hlow.mode, up, M->hup.mode, M->fuse_mid, y, y);
+            }, 3);
+            if (ms < best) {
+                best = ms;
+                best_var = kVarEll;
+            }
+        }
         lo.variant = up.variant = best_var;
     } catch (...) {
         dfree(y);
@@ -565,7 +649,7 @@ void precond_need_f32(pp_precond *M) {
     derive_sweep_f32(M->ctx, M->low64, M->low32);
     derive_sweep_f32(M->ctx, M->up64, M->up32);
     M->low32.variant = M->up32.variant = M->sweep_variant;
-    if (M->trsv == PP_TRSV_LEVELS && M->n >= (1u << 20) && (M->low32.blocks || M->up32.blocks))
+    if (M->trsv == PP_TRSV_LEVELS && M->n >= (1u << 20) && (M->low32.blocks || M->up32.blocks) && !M->sweep_pinned)
         tune_sweep_pair<float>(M->ctx, M, M->low32, M->up32, M->sweep_variant);
     if (M->chunks.nchunk) {
         upload_chunk_sweep(M->hlow, M->chunks, M->clow32);
@@ -924,6 +1008,21 @@ int pp_precond_apply_part(pp_ctx *ctx, pp_precond *M, int part, int dtype, void 
 template <class T>
 static uint64_t sweep_bytes(const DevSweep<T> &d) {
     const uint64_t esz = sizeof(T);
+    if (d.variant == kVarEll) {
+        // sliced-column levels: padded values and 16-bit offsets; the other
+        // levels as the staged kernel with the default loads streams them
+        uint64_t b = 0, nnz_e = 0, rows_e = 0;
+        for (const auto &E : d.ell)
+            if (E.ok) {
+                b += (esz + 2) * (uint64_t)E.width * E.nrows;
+                nnz_e += E.entries;
+                rows_e += E.nrows;
+            }
+        const uint64_t nnz_s = d.nnz - nnz_e;
+        b += (esz + 2) * nnz_s + 4ull * (d.n - rows_e + 1) + 4 * d.perm_rows;
+        if (d.diag) b += esz * d.n;
+        return b + 3 * esz * d.n;
+    }
     const uint64_t n16 = (d.variant & kVarIdx16) ? d.nnz16 : 0;
     uint64_t b = esz * d.nnz + 2 * n16 + 4 * (d.nnz - n16) + 4ull * (d.n + 1) + 4 * d.perm_rows;
     if (d.diag) b += esz * d.n;
@@ -969,6 +1068,20 @@ int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {
     info->fused_levels = M->fuse_mid ? 1 : 0;
     info->trsv = M->trsv;
     for (int k = 0; k < 4; ++k) info->phase_ms[k] = M->phase_ms[k];
+    return PP_OK;
+}
+
+int pp_precond_set_sweep_variant(pp_precond *M, int variant) {
+    if (!M || variant < 0 || variant > kVarEll) return PP_ERR_INVALID;
+    if (!M->ready || M->multilevel) return PP_ERR_NOT_READY;
+    bool have16 = M->low64.ci16 || M->up64.ci16, have_ell = false;
+    for (const auto &e : M->low64.ell) have_ell |= e.ok;
+    for (const auto &e : M->up64.ell) have_ell |= e.ok;
+    if (variant == kVarEll ? !have_ell : ((variant & kVarIdx16) && !have16)) return PP_ERR_INVALID;
+    M->sweep_variant = variant;
+    M->sweep_pinned = true;
+    for (DevSweep<double> *d : {&M->low64, &M->up64}) d->variant = variant;
+    for (DevSweep<float> *d : {&M->low32, &M->up32}) d->variant = variant;
     return PP_OK;
 }
 

@@ -209,6 +209,39 @@ def test_sliced_column_copy_follows_value_updates(pp, torch):
         np.testing.assert_allclose(host(y32), want, rtol=0, atol=1e-5 * (1 + np.abs(want).max()))
 
 
+def test_sweep_variants_agree_on_wide_levels(pp, torch):
+    # colour levels of 280k-560k rows: the staged kernel's variants and the
+    # sliced-column form (8) give one apply bit for bit, in FP64 and FP32,
+    # and one FlexibleCG solve
+    import ctypes as C
+    from prispack_amd import _capi
+    L = _capi.lib()
+    A = ppo.laplacian(104, 104, 104)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    x = ppo.uniform(n, 41)
+    b = ppo.spmv(A, ppo.normal(n, 42))
+    ref = None
+    for var in (0, 6, 7, 8):
+        P = pp.IC0(M, ordering=1, make_f32=True)
+        P.setup()
+        assert L.pp_precond_set_sweep_variant(P.h, var) == 0, var
+        assert P.info().sweep_variant == var
+        y = P.MInvSolve(dev(torch, x), torch.zeros(n, dtype=torch.float64, device="cuda"))
+        y32 = P.MInvSolve(dev(torch, x, torch.float32), torch.zeros(n, dtype=torch.float32, device="cuda"))
+        cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=1000)
+        xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+        cg.solve(M, dev(torch, b), xs)
+        got = (host(y), host(y32), cg.getNumIter(), host(xs))
+        print(f"[sweep variant] {var}: {cg.getNumIter()} iterations", flush=True)
+        if ref is None:
+            ref = got
+            continue
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), var
+        assert got[2] == ref[2] and np.array_equal(got[3], ref[3]), var
+    assert L.pp_precond_set_sweep_variant(P.h, 9) != 0
+
+
 def test_blas1(pp, torch):
     from prispack_amd import _capi
     import ctypes as C

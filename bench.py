@@ -56,8 +56,8 @@ def parse():
 # what the same counter reads for a device-to-device copy of a known size
 # (--copy-probe).  Keyed by (grid side, SpMV storage variant); the command and
 # the raw per-dispatch figures are in profiles/k_spmv_poisson128_pmc.md.
-PMC_TRAFFIC = {(128, v): b for v, b in enumerate((224637192, 226673959, 194739156, 197258000, 219103645, 228891756,
-                                                        189957417, 192822450, 183328688))}
+PMC_TRAFFIC = {(128, v): b for v, b in enumerate((224627870, 226711825, 194751552, 197276754, 219112328, 229056082,
+                                                        189937730, 192875293, 183261472))}
 
 
 ELL_VARIANT = 8  # the SpMV variant that runs the sliced-column form

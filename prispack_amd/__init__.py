@@ -248,6 +248,7 @@ class CSRMatrix:
             p, _ = _dev_ptr(values, PP_F64)
             self.ctx.check(L.pp_csr_set_values(self.ctx.h, self.h, p, 1))
             self.v = values.detach().cpu().numpy().copy()
+        self.values_version = getattr(self, "values_version", 0) + 1
 
     def MatPVec(self, x, y, alpha=1.0, beta=0.0):
         """y = alpha A x + beta y (CSRMatrix::MatPVec, CSRMatrix.h:379-447)."""
@@ -551,6 +552,7 @@ class BatchedSystems:
             _before(self.ctx)
             p, _ = _dev_ptr(values.contiguous(), PP_F64)
             self.ctx.check(L.pp_batch_set_values(self.ctx.h, self.h, p, 1))
+        self.values_version = getattr(self, "values_version", 0) + 1
 
     def _solve(self, fn, b, x, precision, tol, maxit, inner_tol):
         o = _capi.SolveOpts()

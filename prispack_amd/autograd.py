@@ -6,6 +6,11 @@ For x = A^{-1} b and a loss L(x), the adjoint solve A^T lambda = dL/dx gives
     dL/dA_ij = -lambda_i x_j      (on the sparsity pattern of A only).
 Both the forward and the adjoint solve run in the HIP library; the pattern
 outer product is a HIP kernel too.
+
+The backward pass solves with the values of its own forward pass: when the
+matrix received other values in between (a second forward on the same
+pattern), it sets the saved forward values again before the adjoint solve,
+and the matrix keeps them afterwards.
 """
 import ctypes as C
 
@@ -17,16 +22,21 @@ from . import _capi
 class _BatchedSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, values, b, batch, precision, tol):
-        batch.set_values(values.detach())
+        vals = values.detach().clone()
+        batch.set_values(vals)
         x = torch.zeros_like(b)
         batch.solve(b.detach().contiguous(), x, precision=precision, tol=tol)
         ctx.batch, ctx.precision, ctx.tol = batch, precision, tol
-        ctx.save_for_backward(x)
+        ctx.version = batch.values_version
+        ctx.save_for_backward(x, vals)
         return x
 
     @staticmethod
     def backward(ctx, gx):
-        (x,) = ctx.saved_tensors
+        x, vals = ctx.saved_tensors
+        if ctx.batch.values_version != ctx.version:  # another forward set other values since
+            ctx.batch.set_values(vals)
+            ctx.version = ctx.batch.values_version
         lam = torch.zeros_like(x)
         ctx.batch.solve_t(gx.contiguous(), lam, precision=ctx.precision, tol=ctx.tol)
         gvals = ctx.batch.outer_on_pattern(lam, x, scale=-1.0)
@@ -43,7 +53,8 @@ class _SparseSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, values, b, A, make_precond, solver_kw):
         from . import CG, GMRES
-        A.set_values(values.detach())
+        vals = values.detach().clone()
+        A.set_values(vals)
         M = make_precond(A)
         M.setup()
         kw = dict(solver_kw)
@@ -58,13 +69,17 @@ class _SparseSolve(torch.autograd.Function):
         x = torch.zeros_like(b)
         S.solve(A, b.detach().contiguous(), x)
         ctx.A, ctx.make_precond, ctx.kw = A, make_precond, solver_kw
-        ctx.save_for_backward(x)
+        ctx.version = A.values_version
+        ctx.save_for_backward(x, vals)
         return x
 
     @staticmethod
     def backward(ctx, gx):
-        (x,) = ctx.saved_tensors
+        x, vals = ctx.saved_tensors
         A = ctx.A
+        if A.values_version != ctx.version:  # another forward set other values since
+            A.set_values(vals)
+            ctx.version = A.values_version
         L = _capi.lib()
         lam = torch.zeros_like(x)
         kw = dict(ctx.kw)

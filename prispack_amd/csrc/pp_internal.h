@@ -338,7 +338,7 @@ struct ReducedForm {
     float *dt_32 = nullptr;
     EllOp ka, kb;                 // KA and KB in sliced-column form (when they fit)
     double anorm = 0.0;           // max absolute row sum of the colour-ordered matrix
-    const pp_csr *src = nullptr;  // built from this matrix at src_version
+    uint64_t src_uid = 0;  // built from the matrix of this uid at src_version
     uint64_t src_version = 0;
 };
 // Host arrays of the reduced form of one colour-ordered matrix: KA (n0
@@ -396,11 +396,11 @@ struct pp_precond {
     float *tmp32 = nullptr;
     void *multilevel = nullptr;   // AMSED / GMSLR state
     // P A P^T in the factor's row order, for solves that run in that order
-    // (no gather/scatter around each apply); built from Aperm_src at
+    // (no gather/scatter around each apply); built from the matrix of uid Aperm_src_uid at
     // Aperm_version.  io_permuted: the current solve's vectors are already in
     // the factor's order.
     pp_csr *Aperm = nullptr;
-    const pp_csr *Aperm_src = nullptr;
+    uint64_t Aperm_src_uid = 0;
     uint64_t Aperm_version = 0;
     bool io_permuted = false;
     pp::ReducedForm *reduced = nullptr;  // two-colour Schur-complement form (IC(0) only)

@@ -661,7 +661,7 @@ void precond_need_f32(pp_precond *M) {
 // P A P^T for a solve in the factor's row order (rebuilt when A or its values
 // changed since the last build)
 pp_csr *precond_permuted_matrix(pp_ctx *ctx, pp_precond *M, pp_csr *A) {
-    if (M->Aperm && M->Aperm_src == A && M->Aperm_version == A->version) return M->Aperm;
+    if (M->Aperm && M->Aperm_src_uid == A->uid && M->Aperm_version == A->version) return M->Aperm;
     csr_free(M->Aperm);
     M->Aperm = nullptr;
     HostCsr H;
@@ -670,7 +670,7 @@ pp_csr *precond_permuted_matrix(pp_ctx *ctx, pp_precond *M, pp_csr *A) {
     H.ci = A->h_ci;
     H.v = A->h_v;
     M->Aperm = csr_create_host(ctx, permute_symmetric(H, M->order));  // the matrix changed since setup
-    M->Aperm_src = A;
+    M->Aperm_src_uid = A->uid;
     M->Aperm_version = A->version;
     return M->Aperm;
 }
@@ -720,7 +720,7 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
         if (dtype == PP_F64) {
             xp = M->xp64;
         } else {
-            if (!M->xp32) M->xp32 = dalloc<float>(M->n);
+            PP_CHECK(M->xp32, PP_ERR_HIP, "FP32 permutation buffer missing");  // from precond_need_f32
             xp = M->xp32;
         }
         launch_permute(ctx, M->n, M->dorder, dtype, x, xp, true);
@@ -731,12 +731,12 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
     if (M->trsv == PP_TRSV_SYNCFREE) {
         // out of place: x -> tmp (lower) -> y (upper); y may alias x
         if (dtype == PP_F64) {
-            if (!M->tmp64) M->tmp64 = dalloc<double>(M->n);
+            PP_CHECK(M->tmp64, PP_ERR_HIP, "sync-free sweep buffer missing");  // allocated at setup
             launch_sweep_syncfree<double>(ctx, M->low64, M->hlow.mode, (const double *)x, M->tmp64);
             launch_sweep_syncfree<double>(ctx, M->up64, M->hup.mode, M->tmp64, (double *)y);
         } else {
             precond_need_f32(M);
-            if (!M->tmp32) M->tmp32 = dalloc<float>(M->n);
+            PP_CHECK(M->tmp32, PP_ERR_HIP, "sync-free sweep buffer missing");  // allocated by precond_need_f32
             launch_sweep_syncfree<float>(ctx, M->low32, M->hlow.mode, (const float *)x, M->tmp32);
             launch_sweep_syncfree<float>(ctx, M->up32, M->hup.mode, M->tmp32, (float *)y);
         }
@@ -900,7 +900,7 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
                 csr_free(M->Aperm);
                 M->Aperm = nullptr;
                 M->Aperm = csr_create_host(ctx, std::move(Ap));
-                M->Aperm_src = M->A;
+                M->Aperm_src_uid = M->A->uid;
                 M->Aperm_version = M->A->version;
             }
             auto tp = std::chrono::steady_clock::now();
@@ -910,6 +910,9 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
                       : M->opts.trsv == PP_TRSV_BLOCK ? choose_schedule(ctx, M, true)
                                                       : M->opts.trsv;
             if (M->trsv == PP_TRSV_LEVELS) tune_sweeps(ctx, M);
+            // the sync-free apply's buffer between its two sweeps: allocated here, so that no
+            // apply allocates (a solve may capture its first apply into a graph)
+            if (M->trsv == PP_TRSV_SYNCFREE && !M->tmp64) M->tmp64 = dalloc<double>(M->n);
             PP_HIP(hipStreamSynchronize(ctx->stream));
             M->phase_ms[3] = (float)ms_since(tp);
         }
@@ -1080,8 +1083,15 @@ int pp_precond_set_sweep_variant(pp_precond *M, int variant) {
     if (variant == kVarEll ? !have_ell : ((variant & kVarIdx16) && !have16)) return PP_ERR_INVALID;
     M->sweep_variant = variant;
     M->sweep_pinned = true;
-    for (DevSweep<double> *d : {&M->low64, &M->up64}) d->variant = variant;
-    for (DevSweep<float> *d : {&M->low32, &M->up32}) d->variant = variant;
+    // variant 8: the levels without a sliced-column form run the staged kernel with the default loads
+    for (DevSweep<double> *d : {&M->low64, &M->up64}) {
+        d->variant = variant;
+        if (variant == kVarEll) d->ell_fallback = kVarDefault;
+    }
+    for (DevSweep<float> *d : {&M->low32, &M->up32}) {
+        d->variant = variant;
+        if (variant == kVarEll) d->ell_fallback = kVarDefault;
+    }
     return PP_OK;
 }
 

@@ -233,11 +233,12 @@ bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
 
 ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
     if (!M || M->opts.kind != PP_PC_IC0 || M->order.empty() || !M->io_permuted || M->multilevel || !As) return nullptr;
-    if (M->reduced && M->reduced->src == As && M->reduced->src_version == As->version)
+    // keyed on the matrix's uid (never reused, unlike an address) and its value version
+    if (M->reduced && M->reduced->src_uid == As->uid && M->reduced->src_version == As->version)
         return M->reduced->ok ? M->reduced : nullptr;
     reduced_free(M);
     std::unique_ptr<ReducedForm> R(new ReducedForm);
-    R->src = As;
+    R->src_uid = As->uid;
     R->src_version = As->version;
     ReducedHost H;
     // the factor's first level must be exactly the first colour

@@ -29,6 +29,21 @@ constexpr int dt() {
     return sizeof(T) == 8 ? PP_F64 : PP_F32;
 }
 
+// The load variants a body's kernels were captured with: the SpMV variants of
+// A (both precisions) and the sweep variants of M's factor.  Part of every
+// graph key, so that pinning a variant after a solve takes effect in the next
+// solve instead of replaying a graph of the old launches.
+uint64_t variant_key(const pp_csr *A, const pp_precond *M) {
+    uint64_t k = A ? ((uint64_t)(A->spmv_variant & 0xff) | (uint64_t)(A->spmv_variant32 & 0xff) << 8) : 0;
+    if (M)
+        k |= (uint64_t)(M->low64.variant & 0xf) << 16 | (uint64_t)(M->up64.variant & 0xf) << 20 |
+             (uint64_t)(M->low32.variant & 0xf) << 24 | (uint64_t)(M->up32.variant & 0xf) << 28 |
+             (uint64_t)(M->low64.ell_fallback & 0xf) << 32 | (uint64_t)(M->up64.ell_fallback & 0xf) << 36 |
+             (uint64_t)(M->low32.ell_fallback & 0xf) << 40 | (uint64_t)(M->up32.ell_fallback & 0xf) << 44 |
+             (uint64_t)(M->trsv & 0xf) << 48;
+    return k;
+}
+
 struct PcgResult {
     int32_t iters = 0;
     int conv = 0;
@@ -211,7 +226,7 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
                                        M ? M->uid : 0, M ? (uint64_t)M->io_permuted : 0, M ? (uint64_t)M->have32 : 0,
                                        (uint64_t)flexible, every, (uint64_t)hist_on, (uint64_t)ctx->hist,
                                        ctx->hist_cap, (uint64_t)r, (uint64_t)w, (uint64_t)d, (uint64_t)v,
-                                       (uint64_t)x, (uint64_t)b};
+                                       (uint64_t)x, (uint64_t)b, variant_key(A, M)};
     drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, true, key);
     return collect_pcg(ctx, ctl, maxit, hist_on);
 }
@@ -272,7 +287,7 @@ PcgResult pcg_sr_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const T *b, T *x,
                                        M ? M->uid : 0, M ? (uint64_t)M->io_permuted : 0, M ? (uint64_t)M->have32 : 0,
                                        (uint64_t)l0, every, (uint64_t)hist_on, (uint64_t)ctx->hist, ctx->hist_cap,
                                        (uint64_t)r, (uint64_t)u, (uint64_t)w, (uint64_t)p, (uint64_t)s,
-                                       (uint64_t)x, (uint64_t)b};
+                                       (uint64_t)x, (uint64_t)b, variant_key(A, M)};
     drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false, key);
     return collect_pcg(ctx, ctl, maxit, hist_on);
 }
@@ -347,7 +362,8 @@ PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, do
                                        (uint64_t)w,
                                        (uint64_t)p,
                                        (uint64_t)s,
-                                       (uint64_t)x};
+                                       (uint64_t)x,
+                                       variant_key(R->KA, nullptr) << 16 | variant_key(R->KB, nullptr)};
     drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false, key);
     PcgResult res = collect_pcg(ctx, ctl, maxit, false);
     red_post<T>(ctx, n0, n1, x, s1, rin, z, e);
@@ -658,7 +674,8 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
                                                M ? M->uid : 0, M ? (uint64_t)M->io_permuted : 0,
                                                M ? (uint64_t)M->have32 : 0, (uint64_t)V, (uint64_t)Z, (uint64_t)Z32,
                                                (uint64_t)w, (uint64_t)w32, (uint64_t)st.misc,
-                                               (uint64_t)(hist_on ? ctx->hist : nullptr), hist_cap, ld};
+                                               (uint64_t)(hist_on ? ctx->hist : nullptr), hist_cap, ld,
+                                               variant_key(A, M)};
             run_graph(ctx, key, cycle);
         } else {
             cycle();

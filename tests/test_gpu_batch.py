@@ -221,3 +221,53 @@ def test_batched_multicolor_against_oracle(pp, torch):
     x2 = torch.zeros_like(b)
     B.solve(b, x2, precision="fp64", tol=1e-8)
     np.testing.assert_allclose(x2.cpu().numpy() * 1.5, x64.cpu().numpy(), rtol=1e-6, atol=1e-9)
+
+
+def test_gradients_of_two_forwards_before_backward(pp, torch):
+    # two forwards on one matrix with different values, then one backward:
+    # each gradient is the one of its own forward's values (dense FP64 check)
+    from prispack_amd.autograd import sparse_solve
+    A = ppo.upwind(6, 5, 4, 20.0, 10.0, 5.0)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    rows = np.repeat(np.arange(n), np.diff(A[0]))
+    b0, xt = ppo.uniform(n, 1), ppo.uniform(n, 2)
+    v1 = np.asarray(A[2], dtype=np.float64)
+    v2 = v1 * 1.3
+    v2[np.asarray(A[1]) == rows] += 0.5
+    t1 = torch.tensor(v1, device="cuda", requires_grad=True)
+    t2 = torch.tensor(v2, device="cuda", requires_grad=True)
+    bt = torch.tensor(b0, device="cuda")
+    x1 = sparse_solve(t1, bt, M, lambda m: pp.ILU0(m), gmres=True, tol=1e-12)
+    x2 = sparse_solve(t2, bt, M, lambda m: pp.ILU0(m), gmres=True, tol=1e-12)
+    xtt = torch.tensor(xt, device="cuda")
+    (torch.sum((x1 - xtt) ** 2) + torch.sum((x2 - xtt) ** 2)).backward()
+    for vals, t in ((v1, t1), (v2, t2)):
+        D = np.zeros((n, n))
+        D[rows, A[1]] = vals
+        xs = np.linalg.solve(D, b0)
+        lam = np.linalg.solve(D.T, 2 * (xs - xt))
+        np.testing.assert_allclose(t.grad.cpu().numpy(), -lam[rows] * xs[A[1]], rtol=1e-6, atol=1e-9)
+
+    # the batched form: two forwards with different coefficient values
+    from prispack_amd.autograd import batched_solve
+    sides, systems = make_systems(3, 5, 8, 21)
+    B = pp.BatchedSystems(systems)
+    nb = B.n
+    w1 = B.v.copy()
+    w2 = w1 * 2.0
+    bb = torch.tensor(ppo.uniform(nb, 5), device="cuda")
+    xtb = torch.tensor(ppo.uniform(nb, 6), device="cuda")
+    s1 = torch.tensor(w1, device="cuda", requires_grad=True)
+    s2 = torch.tensor(w2, device="cuda", requires_grad=True)
+    y1 = batched_solve(s1, bb, B, precision="fp64", tol=1e-13)
+    y2 = batched_solve(s2, bb, B, precision="fp64", tol=1e-13)
+    (torch.sum((y1 - xtb) ** 2) + torch.sum((y2 - xtb) ** 2)).backward()
+    # A scaled by 2: x2 = x1 / 2, lambda2 = A^-T 2 (x1/2 - xt) / 2
+    rb = np.repeat(np.arange(nb), np.diff(B.rp))
+    for vals, t in ((w1, s1), (w2, s2)):
+        D = np.zeros((nb, nb))
+        D[rb, B.ci] = vals
+        xs = np.linalg.solve(D, bb.cpu().numpy())
+        lam = np.linalg.solve(D.T, 2 * (xs - xtb.cpu().numpy()))
+        np.testing.assert_allclose(t.grad.cpu().numpy(), -lam[rb] * xs[B.ci], rtol=1e-6, atol=1e-9)

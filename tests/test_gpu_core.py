@@ -421,16 +421,19 @@ def test_mixed_precision_cg(pp, torch, dims, trsv):
     assert err <= 1e-6
 
 
+@pytest.mark.parametrize("trsv", ["levels", "syncfree"])
 @pytest.mark.parametrize("name", ["gmres_lapconv_12x12x12", "gmres_upwind_16x16x16", "gmres_upwind_10x10x10_l3"])
-def test_gmres_parity_with_reference(pp, torch, name):
+def test_gmres_parity_with_reference(pp, torch, name, trsv):
+    # the FGMRES cycle is captured into a graph before its first apply: the
+    # sync-free schedule's buffers must exist from setup on
     g = _golden(name)
     A = (g["rp"], g["ci"], g["v"])
     n = len(A[0]) - 1
-    print(f"[gmres] {name} n={n}", flush=True)
+    print(f"[gmres] {name} n={n} {trsv}", flush=True)
     xt = ppo.uniform(n, int(g["seed"]))
     b = ppo.spmv(A, xt)
     M = pp.CSRMatrix(*A)
-    P = pp.IncompleteLU(M, lfill=int(g["lfil"]), tolerance=float(g["tau"]))
+    P = pp.IncompleteLU(M, lfill=int(g["lfil"]), tolerance=float(g["tau"]), trsv=TRSV[trsv])
     for quirks in (True, False):
         gm = pp.GMRES(n, P, float(g["tol"]), pp.FlexibleGMRES, reference_quirks=quirks)
         gm.setRestart(True, int(g["restart"]))
@@ -459,6 +462,26 @@ def test_gmres_mixed(pp, torch):
     o = ppo.gmres(A, b, ppo.Prec(A, "ilu0"), tol=1e-10, restart=30)
     err = np.linalg.norm(host(x) - o["x"]) / np.linalg.norm(o["x"])
     assert err <= 1e-6
+
+
+def test_single_reduction_fp64_syncfree(pp, torch):
+    # the single-reduction body's first apply is inside the captured graph
+    A = ppo.laplacian(16, 16, 12)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 2))
+    M = pp.CSRMatrix(*A)
+    out = {}
+    for trsv in ("levels", "syncfree"):
+        P = pp.IncompleteCholesky(M, 1e-3, trsv=TRSV[trsv])
+        cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-10, maxIter=2000).setCGVariant("single_reduction")
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        cg.solve(M, dev(torch, b), x)
+        assert cg.getConvergence() and cg.report.true_relres < 1e-10, trsv
+        out[trsv] = (cg.getNumIter(), host(x))
+    # the schedules apply the same factor
+    assert abs(out["levels"][0] - out["syncfree"][0]) <= 1
+    xl, xs = out["levels"][1], out["syncfree"][1]
+    assert np.linalg.norm(xs - xl) / np.linalg.norm(xl) <= 1e-9
 
 
 @pytest.mark.parametrize("kind", ["ic0", "ict"])

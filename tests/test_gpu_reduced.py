@@ -235,3 +235,30 @@ def test_reduced_stream_launches_match_graphs(pp, torch):
             out.append((cg.report.inner_iters, x.cpu().numpy()))
         assert out[0][0] == out[1][0], prec
         assert np.array_equal(out[0][1], out[1][1]), prec
+
+
+def test_reduced_form_follows_value_updates(pp, torch):
+    # new values on the same matrix, the preconditioner set up again: the
+    # mixed solve must use reduced operators of the new values, i.e. behave
+    # as a solve on a freshly created matrix and preconditioner does
+    A = ppo.laplacian(24, 23, 22)
+    n = len(A[0]) - 1
+    b = ppo.spmv(A, ppo.normal(n, 3))
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M, ordering=1, make_f32=True)
+    cg0, _ = solve(pp, torch, M, P, b, "mixed", "reduced")
+    assert cg0.getConvergence()
+    v2 = np.asarray(A[2]) * 1.5
+    v2[np.asarray(A[1]) == np.repeat(np.arange(n), np.diff(A[0]))] += 0.25  # a diagonal shift: other iterates
+    M.set_values(v2)
+    P.setup()
+    cg1, x1 = solve(pp, torch, M, P, b, "mixed", "reduced")
+    A2 = (A[0], A[1], v2)
+    M2 = pp.CSRMatrix(*A2)
+    P2 = pp.IC0(M2, ordering=1, make_f32=True)
+    cg2, x2 = solve(pp, torch, M2, P2, b, "mixed", "reduced")
+    print(f"[reduced] updated values: outer {cg1.report.outer_iters} inner {cg1.report.inner_iters}; "
+          f"fresh outer {cg2.report.outer_iters} inner {cg2.report.inner_iters}", flush=True)
+    assert cg1.getConvergence() and cg1.report.true_relres < 1e-8
+    assert (cg1.report.outer_iters, cg1.report.inner_iters) == (cg2.report.outer_iters, cg2.report.inner_iters)
+    np.testing.assert_array_equal(x1, x2)

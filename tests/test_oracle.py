@@ -97,6 +97,27 @@ def test_upwind_and_vardiff_structure():
     assert np.all(np.linalg.eigvalsh(V) > 0)
 
 
+def test_vardiff_face_coefficients_by_hand():
+    # 3x3 grid, node p = 3*j + i holds kappa = p + 1.  Face coefficients worked
+    # out by hand: an interior face between kappa a and b is the harmonic mean
+    # 2ab/(a+b); a boundary face takes the node's own kappa.  The arithmetic
+    # (a+b)/2 or geometric sqrt(ab) means give other numbers at every entry below.
+    V = dense(ppo.vardiff2d(3, np.arange(1.0, 10.0)))
+    # node 0 (kappa 1): east to node 1 (kappa 2) 4/3, north to node 3 (kappa 4) 8/5,
+    # south and west are boundary faces of weight 1 each
+    assert V[0, 1] == -4 / 3 and V[0, 3] == -8 / 5
+    assert abs(V[0, 0] - 74 / 15) < 1e-15
+    # centre node 4 (kappa 5): neighbours kappa 2, 4, 6, 8
+    assert V[4, 1] == -20 / 7 and V[4, 3] == -40 / 9 and V[4, 5] == -60 / 11 and V[4, 7] == -80 / 13
+    assert abs(V[4, 4] - (20 / 7 + 40 / 9 + 60 / 11 + 80 / 13)) < 1e-14
+    # corner node 8 (kappa 9): west to node 7 (kappa 8) 144/17, south to node 5 (kappa 6) 108/15,
+    # east and north boundary faces of weight 9 each
+    assert V[8, 7] == -144 / 17 and V[8, 5] == -108 / 15
+    assert abs(V[8, 8] - (144 / 17 + 108 / 15 + 18)) < 1e-14
+    # no coupling beyond the 5-point stencil
+    assert V[0, 4] == 0 and V[0, 2] == 0 and np.count_nonzero(V) == 9 + 2 * 12
+
+
 # --------------------------------------------------------------------------- factorizations
 @pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "ict_*.npz"))))
 def test_ict_golden(path):

@@ -316,6 +316,25 @@ int pp_solve(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *opts,
  * solves, downloads x. */
 int pp_solve_host(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *opts,
                   const double *b_host, double *x_host, pp_solve_report *rep);
+/* pp_solve without any allocation: no hipMalloc, hipFree or other allocating
+ * call, no graph instantiation.  No counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  FP64 CG only (FlexibleCG /
+ * NormalCG, or PP_CG_SINGLE_REDUCTION) with no preconditioner or a triangular
+ * one (ICT, ILUT, ILDLT, IC(0), ILU(0), either ordering); the iteration
+ * kernels are launched on the stream (opts->launch is ignored) and no residual
+ * history is kept.  The work vectors live in one fixed device array of
+ * 128 MiB per GPU (held from the first context on that GPU until the process
+ * ends): four n-vectors (five for the single-reduction CG), the final
+ * residual, and b and x in colour order for a multicolour factor, each
+ * rounded up to 256 bytes -- n <= 2,396,745 for a colour-ordered FlexibleCG
+ * solve, n <= 3,355,443 in natural order.  Larger systems fail with
+ * PP_ERR_INVALID before any work; other solver kinds with PP_ERR_UNSUPPORTED;
+ * a multicolour factor whose matrix values changed since setup with
+ * PP_ERR_NOT_READY.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two contexts or threads) race. */
+int pp_solve_noalloc(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *opts,
+                     const double *b, double *x, pp_solve_report *rep);
 /* residual history of the last solve on this context (nhist entries) */
 int pp_solve_history(pp_ctx *ctx, double *out, uint32_t cap, uint32_t *count);
 

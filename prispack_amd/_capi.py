@@ -116,6 +116,7 @@ SIGS = {
     "pp_solve_opts_default": (None, [C.POINTER(SolveOpts), C.c_int]),
     "pp_solve": (C.c_int, [vp, vp, vp, C.POINTER(SolveOpts), vp, vp, C.POINTER(SolveReport)]),
     "pp_solve_host": (C.c_int, [vp, vp, vp, C.POINTER(SolveOpts), f64p, f64p, C.POINTER(SolveReport)]),
+    "pp_solve_noalloc": (C.c_int, [vp, vp, vp, C.POINTER(SolveOpts), vp, vp, C.POINTER(SolveReport)]),
     "pp_solve_history": (C.c_int, [vp, f64p, u32, u32p]),
     "pp_adjoint_solve": (C.c_int, [vp, vp, C.POINTER(PrecondOpts), C.POINTER(SolveOpts), vp, vp,
                                    C.POINTER(SolveReport)]),

@@ -80,6 +80,17 @@ void ensure_hist(pp_ctx *ctx, uint32_t cap) {
 
 void *ws_get(pp_ctx *ctx, int idx, size_t bytes) {
     PP_CHECK(idx >= 0 && idx < 48, PP_ERR_INVALID, "workspace index");
+    if (ctx->arena) {  // a zero-allocation call: slots are carved from the fixed scratch array
+        if (ctx->arena_len[idx] < bytes) {
+            const size_t off = (ctx->arena_used + 255) & ~(size_t)255;
+            PP_CHECK(off <= ctx->arena_cap && bytes <= ctx->arena_cap - off, PP_ERR_INVALID,
+                     "the solve's work vectors exceed the zero-allocation scratch array");
+            ctx->arena_off[idx] = off;
+            ctx->arena_len[idx] = bytes;
+            ctx->arena_used = off + bytes;
+        }
+        return ctx->arena + ctx->arena_off[idx];
+    }
     if (ctx->ws_bytes[idx] < bytes) {
         PP_HIP(hipStreamSynchronize(ctx->stream));
         dfree(ctx->ws[idx]);
@@ -408,6 +419,7 @@ int pp_ctx_create(int device, pp_ctx **out) {
         std::unique_ptr<pp_ctx> c(new pp_ctx);
         c->device = device;
         PP_HIP(hipSetDevice(device));
+        noalloc_setup(device);  // the zero-allocation solve's array address on this device
         PP_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
         c->stream = c->own_stream;
         hipDeviceProp_t prop;

@@ -143,6 +143,11 @@ struct pp_ctx {
     static constexpr int kInFlight = 2;
     void *ws[48] = {};             // cached solver workspaces
     size_t ws_bytes[48] = {};
+    // while arena is set (pp_solve_noalloc), ws_get carves the workspace
+    // slots out of [arena, arena + arena_cap) instead of allocating
+    unsigned char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    size_t arena_off[48] = {}, arena_len[48] = {};
     long long spin_deadline = 0;   // wall_clock64 ticks a sync-free wait may last
     // while set (StopScope), SpMV / dot / GEMV / convert kernels return at
     // once when the word is non-zero, like the sweeps and the PCG kernels
@@ -422,6 +427,13 @@ T *dalloc(size_t n);
 void dfree(void *p);
 void ensure_hist(pp_ctx *ctx, uint32_t cap);
 void *ws_get(pp_ctx *ctx, int idx, size_t bytes);
+// ---- zero-allocation solves (pp_noalloc.hip) ----
+// size of the fixed device array the work vectors of pp_solve_noalloc live in
+constexpr size_t kNoallocScratchBytes = (size_t)128 << 20;
+// looks up the array's address on `device` (which must be current); once per device
+void noalloc_setup(int device);
+// that address, or nullptr before noalloc_setup
+void *noalloc_scratch(int device);
 // Workspace slots owned by the GMRES / Gram-Schmidt helpers; the solvers'
 // own vectors use the slots below kWsGmresState and from kWsFree on.
 constexpr int kWsGmresState = 30, kWsGmresCoeff = 31, kWsFree = 32;

@@ -1,6 +1,7 @@
 // pp_solve.cpp -- Krylov drivers of the C ABI: device-resident PCG (FP64, and
 // mixed precision = FP32 inner PCG inside FP64 iterative refinement) and
 // FGMRES(m) with CGS2 orthogonalisation.
+#include <cstdio>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -896,6 +897,67 @@ int pp_solve_host(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *op
     dfree(db);
     dfree(dx);
     return st;
+}
+
+// The FP64 CG solve of pp_solve with every work vector carved from the
+// fixed scratch array of pp_noalloc.hip: no device or host allocation, no
+// graph instantiation (the iteration kernels are launched on the stream).
+int pp_solve_noalloc(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opts *opts, const double *b, double *x,
+                     pp_solve_report *rep) {
+    if (!ctx) return PP_ERR_INVALID;
+    pp_solve_opts o;
+    unsigned char *base = nullptr;
+    int st = guarded_solve(ctx, [&] {
+        PP_CHECK(A && opts && b && x, PP_ERR_INVALID, "NULL argument");
+        PP_CHECK(A->nrow == A->ncol, PP_ERR_INVALID, "square matrix required");
+        o = *opts;
+        PP_CHECK(o.solver == PP_SOLVER_CG && o.precision == PP_FP64, PP_ERR_UNSUPPORTED,
+                 "the zero-allocation solve runs FP64 CG only");
+        PP_CHECK(o.cg_variant == PP_CG_AUTO || o.cg_variant == PP_CG_REFERENCE ||
+                     o.cg_variant == PP_CG_SINGLE_REDUCTION,
+                 PP_ERR_UNSUPPORTED, "the zero-allocation solve runs FlexibleCG / NormalCG or the single-reduction CG");
+        bool permuted = false;
+        if (M) {
+            PP_CHECK(M->ready, PP_ERR_NOT_READY, "preconditioner used before setup()");
+            PP_CHECK(!M->multilevel && M->opts.kind != PP_PC_AMSED && M->opts.kind != PP_PC_GMSLR, PP_ERR_UNSUPPORTED,
+                     "the zero-allocation solve takes triangular preconditioners only");
+            permuted = !M->order.empty() && M->A == A;
+            // the colour-ordered copy of A exists from setup and is current
+            PP_CHECK(!permuted || (M->Aperm && M->Aperm_src_uid == A->uid && M->Aperm_version == A->version),
+                     PP_ERR_NOT_READY,
+                     "the matrix values changed since the preconditioner's setup: set it up again first");
+        }
+        base = (unsigned char *)noalloc_scratch(ctx->device);
+        PP_CHECK(base, PP_ERR_HIP, "the scratch array has no address on this device");
+        // carved slots: the iteration's vectors (four, five for the single
+        // reduction), the final residual, and b and x in colour order
+        const size_t vec = ((size_t)A->nrow * sizeof(double) + 255) & ~(size_t)255;
+        const size_t nvec = (o.cg_variant == PP_CG_SINGLE_REDUCTION ? 5 : 4) + 1 + (permuted ? 2 : 0);
+        if (vec * nvec > kNoallocScratchBytes) {
+            char msg[200];
+            std::snprintf(msg, sizeof msg,
+                          "n = %u needs %zu bytes of scratch; the zero-allocation solve holds at most %zu "
+                          "(n <= %zu here)",
+                          A->nrow, vec * nvec, kNoallocScratchBytes, kNoallocScratchBytes / nvec / sizeof(double));
+            throw Error(PP_ERR_INVALID, msg);
+        }
+        o.launch = PP_LAUNCH_STREAM;
+        o.record_history = 0;
+    });
+    if (st != PP_OK) return st;
+    struct Arena {
+        pp_ctx *c;
+        ~Arena() {
+            c->arena = nullptr;
+            c->arena_cap = c->arena_used = 0;
+            for (int i = 0; i < 48; ++i) c->arena_off[i] = c->arena_len[i] = 0;
+        }
+    } arena{ctx};
+    ctx->arena = base;
+    ctx->arena_cap = kNoallocScratchBytes;
+    ctx->arena_used = 0;
+    for (int i = 0; i < 48; ++i) ctx->arena_off[i] = ctx->arena_len[i] = 0;
+    return solve_impl(ctx, A, M, &o, b, x, rep);
 }
 
 int pp_solve_history(pp_ctx *ctx, double *out, uint32_t cap, uint32_t *count) {

@@ -154,7 +154,12 @@ enum {
  * matrix the factor was built from runs entirely in the colour order (P A P^T
  * is formed once); a direct apply permutes in and out.  It changes the
  * preconditioner (and the iteration count). */
-enum { PP_ORDER_NATURAL = 0, PP_ORDER_MULTICOLOR = 1 };
+enum { PP_ORDER_NATURAL = 0, PP_ORDER_MULTICOLOR = 1, PP_ORDER_RCM = 2 };
+/* PP_ORDER_RCM factors P A P^T with P the reverse Cuthill-McKee order of A's
+ * pattern (pp_reorder_rcm; the ordering HOST::rcmReorderCSR computes,
+ * include/utils/ExternalTools/MatrixReorderTools.h:59-80): a narrower band and
+ * less fill for the threshold factors; solves run in that order like
+ * MULTICOLOR's. */
 
 typedef struct {
     int kind;            /* PP_PC_* */
@@ -382,6 +387,35 @@ int pp_generate_upwind(int nx, int ny, int nz, double bx, double by, double bz, 
                        uint32_t *nnz, uint32_t *rowptr, uint32_t *colidx, double *values);
 int pp_generate_vardiff2d(int side, const double *kappa, uint32_t *n, uint32_t *nnz,
                           uint32_t *rowptr, uint32_t *colidx, double *values);
+/* ---- MatrixMarket input (HOST::ReadMtxTools, source/utils/TestTools/ReadMtxTools.cpp;
+ * no device needed).  pp_mtx_read takes a "%%MatrixMarket matrix coordinate real"
+ * file with general, symmetric or skew-symmetric storage (hermitian real data
+ * is taken as stored, as the reference does) and applies the reference
+ * reader's rules: indices minus `base` (1 for files written by the format's
+ * convention), entries of magnitude <= 10 DBL_EPSILON dropped, symmetric and
+ * skew-symmetric storage expanded with the mirrored (negated) entry of every
+ * off-diagonal entry, rows in file order then sorted by column with
+ * duplicates kept (transCOO2CSR, include/MatrixClass/MatrixTools.h:16-62).
+ * Other fields (complex, integer, pattern) and dense files are
+ * PP_ERR_UNSUPPORTED; pp_global_error() gives the message. ---- */
+typedef struct pp_mtx pp_mtx;
+int pp_mtx_read(const char *path, int base, pp_mtx **out);
+int pp_mtx_dims(const pp_mtx *m, uint32_t *nrow, uint32_t *ncol, uint32_t *nnz);
+/* copies the CSR arrays out (nrow + 1, nnz, nnz entries; any may be NULL) */
+int pp_mtx_csr(const pp_mtx *m, uint32_t *rowptr, uint32_t *colidx, double *values);
+void pp_mtx_destroy(pp_mtx *m);
+/* a dense "%%MatrixMarket matrix array real" file (right-hand sides,
+ * ReadMtxTools::loadRightHand): nrow x ncol values, column-major; values NULL
+ * to query the sizes */
+int pp_mtx_read_dense(const char *path, uint32_t *nrow, uint32_t *ncol, double *values);
+
+/* ---- reverse Cuthill-McKee (HOST::rcmReorderCSR, MatrixReorderTools.h:59-80,
+ * over the vendored genrcm of external_packages/RCM): perm[k] = the row placed
+ * at position k, for a structurally symmetric pattern (its stored entries, the
+ * diagonal included, are the adjacency lists).  The same permutation as the
+ * reference's. ---- */
+int pp_reorder_rcm(uint32_t n, const uint32_t *rowptr, const uint32_t *colidx, uint32_t *perm);
+
 /* seeded SplitMix64 inputs: kind 0 uniform(-1,1), 1 normal(0,1) */
 int pp_fill_random(int kind, uint64_t seed, uint64_t n, double *out);
 

@@ -290,6 +290,9 @@ class Ref:
         L.refh_cg.restype = C.c_int
         L.refh_cg.argtypes = [vp, vp, C.c_int, _f64p, _f64p, C.c_double, C.c_uint32,
                               C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.refh_read_mtx.restype = vp
+        L.refh_read_mtx.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
+        L.refh_rcm.argtypes = [vp, _u32p]
         L.refh_gmres.restype = C.c_int
         L.refh_gmres.argtypes = [vp, vp, C.c_int, _f64p, _f64p, C.c_double, C.c_uint32,
                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]

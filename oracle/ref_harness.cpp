@@ -17,6 +17,8 @@
 #include "include/Solver/CG.h"
 #include "include/Solver/GMRES.h"
 #include "include/VectorClass/AutoAllocateVector.h"
+#include "include/utils/TestTools/ReadMtxTools.h"
+#include "rcm.hpp"
 
 using Csr = HOST::CSRMatrix<double>;
 using Prec = BasePrecondition<double>;
@@ -181,6 +183,37 @@ int refh_gmres(void *ha, void *hp, int flexible, const double *b, double *x, dou
     *conv = g.getConvergence();
     *err = g.getError();
     return g.getNumIter();
+}
+
+// HOST::ReadMtxTools<double>(binaryPath, Apath, base).loadMatrix(csr)
+// (source/utils/TestTools/ReadMtxTools.cpp:11-46, 339-384).  The reader
+// changes the working directory to dirname(binaryPath); callers pass
+// absolute paths and restore their directory.
+void *refh_read_mtx(const char *binary_path, const char *path, int base) {
+    auto *h = new RefCsr;
+    h->a = std::make_shared<Csr>();
+    HOST::ReadMtxTools<double> rd(const_cast<char *>(binary_path), path, (UINT8)base);
+    rd.loadMatrix(*h->a);
+    return h;
+}
+
+// The reference's genrcm (external_packages/RCM/lib/rcm.cpp:999), called
+// the way HOST::rcmReorderCSR calls it (MatrixReorderTools.h:59-80, which
+// cannot be included here: its header also pulls in SuiteSparse's amd.h):
+// the CSR shifted to 1-based indices, genrcm, the result shifted back.
+// perm[k] = the row placed at position k.
+void refh_rcm(void *h, uint32_t *perm) {
+    auto &m = *static_cast<RefCsr *>(h)->a;
+    const uint32_t dim = m.getRowNum();
+    INT32 *rp = (INT32 *)m.getRowOffsetPtr(0);
+    INT32 *ci = (INT32 *)m.getColIndicesPtr(0);
+    const uint32_t nnz = m.getRowOffsetPtr(0)[dim];
+    for (uint32_t i = 0; i <= dim; ++i) rp[i] += 1;
+    for (uint32_t i = 0; i < nnz; ++i) ci[i] += 1;
+    genrcm((int)dim, (int)nnz, rp, ci, (int *)perm);
+    for (uint32_t i = 0; i <= dim; ++i) rp[i] -= 1;
+    for (uint32_t i = 0; i < nnz; ++i) ci[i] -= 1;
+    for (uint32_t i = 0; i < dim; ++i) perm[i] -= 1;
 }
 
 }  // extern "C"

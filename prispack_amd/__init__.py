@@ -29,7 +29,8 @@ _capi.lib()  # fail loudly at import when the HIP library is missing
 
 __all__ = ["Context", "CSRMatrix", "IncompleteCholesky", "IncompleteLU", "IncompleteLDLT", "IC0", "ILU0",
            "CG", "GMRES", "NormalCG", "FlexibleCG", "NormalGMRES", "FlexibleGMRES", "generateLaplacianCSR",
-           "generate_upwind", "generate_vardiff2d", "uniform", "normal", "PrisPackError", "host_factor"]
+           "generate_upwind", "generate_vardiff2d", "uniform", "normal", "PrisPackError", "host_factor",
+           "read_mtx", "read_mtx_dense", "rcm"]
 
 # CGType / GMRESType (include/Solver/CG.h:14-17, include/Solver/GMRES.h:15-18)
 NormalCG, FlexibleCG = 0, 1
@@ -94,6 +95,49 @@ def uniform(n, seed):
 def normal(n, seed):
     """x_i ~ N(0, 1) from SplitMix64(seed) + Box-Muller."""
     return _fill(1, n, seed)
+
+
+# ----------------------------------------------------------------------------- input and ordering
+def read_mtx(path, base=1):
+    """HOST::ReadMtxTools<double>(.., path, base).loadMatrix(csr)
+    (source/utils/TestTools/ReadMtxTools.cpp): a coordinate real MatrixMarket
+    file as CSR (rp, ci, v) with the reference reader's rules."""
+    L = _capi.lib()
+    h = C.c_void_p()
+    check(L.pp_mtx_read(str(path).encode(), int(base), C.byref(h)))
+    try:
+        nr, nc, nz = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(L.pp_mtx_dims(h, C.byref(nr), C.byref(nc), C.byref(nz)))
+        rp = np.zeros(nr.value + 1, np.uint32)
+        ci = np.zeros(max(nz.value, 1), np.uint32)
+        v = np.zeros(max(nz.value, 1), np.float64)
+        check(L.pp_mtx_csr(h, _p(rp, C.c_uint32), _p(ci, C.c_uint32), _p(v, C.c_double)))
+    finally:
+        L.pp_mtx_destroy(h)
+    return rp, ci[:nz.value], v[:nz.value]
+
+
+def read_mtx_dense(path):
+    """A dense (array) MatrixMarket file, column-major values and (nrow, ncol)
+    (ReadMtxTools::loadRightHand)."""
+    L = _capi.lib()
+    nr, nc = C.c_uint32(), C.c_uint32()
+    check(L.pp_mtx_read_dense(str(path).encode(), C.byref(nr), C.byref(nc), None))
+    vals = np.zeros(max(nr.value * nc.value, 1), np.float64)
+    check(L.pp_mtx_read_dense(str(path).encode(), C.byref(nr), C.byref(nc), _p(vals, C.c_double)))
+    return vals[:nr.value * nc.value], (nr.value, nc.value)
+
+
+def rcm(A):
+    """HOST::rcmReorderCSR (include/utils/ExternalTools/MatrixReorderTools.h:59-80):
+    reverse Cuthill-McKee order of a structurally symmetric CSR pattern;
+    perm[k] = the row placed at position k."""
+    rp, ci = _u32(A[0]), _u32(A[1])
+    n = len(rp) - 1
+    perm = np.zeros(max(n, 1), np.uint32)
+    cip = ci if len(ci) else np.zeros(1, np.uint32)
+    check(_capi.lib().pp_reorder_rcm(n, _p(rp, C.c_uint32), _p(cip, C.c_uint32), _p(perm, C.c_uint32)))
+    return perm[:n]
 
 
 # ----------------------------------------------------------------------------- host factor

@@ -20,7 +20,7 @@ PP_SOLVER_CG, PP_SOLVER_GMRES = 0, 1
 PP_FP64, PP_MIXED = 0, 1
 PP_LAUNCH_GRAPH, PP_LAUNCH_STREAM = 0, 1
 PP_CG_AUTO, PP_CG_REFERENCE, PP_CG_SINGLE_REDUCTION, PP_CG_REDUCED = 0, 1, 2, 3
-PP_ORDER_NATURAL, PP_ORDER_MULTICOLOR = 0, 1
+PP_ORDER_NATURAL, PP_ORDER_MULTICOLOR, PP_ORDER_RCM = 0, 1, 2
 
 STATUS_NAMES = {0: "PP_OK", 1: "PP_ERR_INVALID", 2: "PP_ERR_HIP", 3: "PP_ERR_NOT_SPD", 4: "PP_ERR_ZERO_PIVOT",
                 5: "PP_ERR_ZERO_ROW", 6: "PP_ERR_NOT_READY", 7: "PP_ERR_BREAKDOWN", 8: "PP_ERR_NOT_FINITE",
@@ -117,6 +117,12 @@ SIGS = {
     "pp_solve": (C.c_int, [vp, vp, vp, C.POINTER(SolveOpts), vp, vp, C.POINTER(SolveReport)]),
     "pp_solve_host": (C.c_int, [vp, vp, vp, C.POINTER(SolveOpts), f64p, f64p, C.POINTER(SolveReport)]),
     "pp_solve_noalloc": (C.c_int, [vp, vp, vp, C.POINTER(SolveOpts), vp, vp, C.POINTER(SolveReport)]),
+    "pp_mtx_read": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp)]),
+    "pp_mtx_dims": (C.c_int, [vp, u32p, u32p, u32p]),
+    "pp_mtx_csr": (C.c_int, [vp, u32p, u32p, f64p]),
+    "pp_mtx_destroy": (None, [vp]),
+    "pp_mtx_read_dense": (C.c_int, [C.c_char_p, u32p, u32p, f64p]),
+    "pp_reorder_rcm": (C.c_int, [C.c_uint32, u32p, u32p, u32p]),
     "pp_solve_history": (C.c_int, [vp, f64p, u32, u32p]),
     "pp_adjoint_solve": (C.c_int, [vp, vp, C.POINTER(PrecondOpts), C.POINTER(SolveOpts), vp, vp,
                                    C.POINTER(SolveReport)]),

@@ -1131,9 +1131,14 @@ struct SentinelBits<float> {
     static constexpr U value = 0x7FA5C3E1u;  // signalling NaN
 };
 
+// Skipped, like the sweep that follows it, once the stop word is set: an
+// iteration issued past convergence leaves its output as it was instead of
+// a vector of sentinels (which a later product with a zero coefficient would
+// turn into NaN).
 template <class T>
-__global__ void k_fill_sentinel(uint32_t n, T *y) {
+__global__ void k_fill_sentinel(uint32_t n, T *y, const int *ctl) {
     using U = typename SentinelBits<T>::U;
+    if (ctl[C_STOP]) return;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         reinterpret_cast<U *>(y)[i] = SentinelBits<T>::value;
 }
@@ -1194,7 +1199,7 @@ template <class T>
 void launch_sweep_syncfree(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *y) {
     if (s.n == 0) return;
     const unsigned g = (s.n + kBlock - 1) / kBlock;
-    k_fill_sentinel<T><<<grid_for(ctx, s.n), kBlock, 0, ctx->stream>>>(s.n, y);
+    k_fill_sentinel<T><<<grid_for(ctx, s.n), kBlock, 0, ctx->stream>>>(s.n, y, ctx->ctl);
     k_sweep_syncfree<T><<<g, kBlock, 0, ctx->stream>>>(s.perm, s.rp, s.ci, s.val, s.diag, mode, b, y, s.n,
                                                         ctx->spin_deadline, ctx->ctl);
     PP_HIP(hipGetLastError());

@@ -169,13 +169,16 @@ void launch_gemv_sub_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, 
 
 
 // x[r] += sum_c Z[r, c] y[c] (Z in FP32 or FP64), columns in ascending order
-// like DenseMatrix::MatVec (DenseMatrix.cpp:208-224).
+// like DenseMatrix::MatVec (DenseMatrix.cpp:208-224).  A column whose
+// coefficient is zero adds nothing and is not read: the columns past a
+// converged cycle's last step hold whatever the early-exit kernels left.
 template <class T>
 __global__ __launch_bounds__(256) void k_gemv_add(uint32_t n, uint32_t ncol, const T *__restrict__ Z, size_t ld,
                                                   const double *__restrict__ y, double *__restrict__ x) {
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
         double acc = 0.0;
-        for (uint32_t c = 0; c < ncol; ++c) acc += (double)Z[ld * c + r] * y[c];
+        for (uint32_t c = 0; c < ncol; ++c)
+            if (y[c] != 0.0) acc += (double)Z[ld * c + r] * y[c];
         x[r] += acc;
     }
 }

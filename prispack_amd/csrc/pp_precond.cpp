@@ -10,6 +10,8 @@
 #include "pp_internal.h"
 
 namespace pp {
+// reverse Cuthill-McKee order of a CSR pattern (pp_io.cpp)
+std::vector<uint32_t> rcm_order(uint32_t n, const uint32_t *rp, const uint32_t *ci);
 // multilevel preconditioners (pp_multilevel.cpp)
 void ml_setup(pp_ctx *ctx, pp_precond *M);
 void ml_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
@@ -502,6 +504,12 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload, Host
         M->order = multicolor_order(A0, M->ncolors);
         Aperm = permute_symmetric(A0, M->order);
         Ap = &Aperm;
+    } else if (M->opts.ordering == PP_ORDER_RCM) {
+        M->order = rcm_order(A0.n, A0.rp.data(), A0.ci.data());
+        Aperm = permute_symmetric(A0, M->order);
+        Ap = &Aperm;
+    } else {
+        PP_CHECK(M->opts.ordering == PP_ORDER_NATURAL, PP_ERR_INVALID, "unknown ordering");
     }
     M->phase_ms[0] = (float)ms_since(tp);
     const HostCsr &A = *Ap;

@@ -46,6 +46,9 @@ def parse():
                     help="only build the system and time the SpMV and preconditioner-apply launches (no solves)")
     ap.add_argument("--launch", type=int, default=0, help="PCG body: 0 hipGraph replay, 1 stream launches")
     ap.add_argument("--check-every", type=int, default=4, help="iterations per graph body (per host convergence poll)")
+    ap.add_argument("--spmv-variant", type=int, default=-1,
+                    help="pin the SpMV load variant of the timed launches (0-7 CSR-stream, 8 sliced-column; "
+                         "default: the library's timed choice)")
     ap.add_argument("--ordering", type=int, default=1,
                     help="IC(0) row ordering: 1 multicolour (2 sweep levels, default), 0 natural (382 levels)")
     return ap.parse_args()
@@ -103,10 +106,33 @@ def cpu_baseline(A, b, n):
     R.free_matrix(hA)
     return {"value": round(t2 - t1, 4), "unit": "s", "cores": 4, "kind": "reference",
             "sample": (f"one full solve of the same {n}-row system: reference FlexibleCG + IncompleteCholesky "
-                       f"(tau={tau}, the reference has no IC(0)), OMP_NUM_THREADS=4 as compiled in "
-                       f"(THREAD_NUM), release flags; {out['iters']} iterations, setup {t1 - t0:.3f} s excluded"),
+                       f"(tau={tau}, the reference has no IC(0)), 4 OpenMP threads: every parallel loop of the "
+                       f"reference carries num_threads(THREAD_NUM) with THREAD_NUM 4 fixed in config/config.h:28, "
+                       f"so the host's other cores cannot be used without editing its sources (its triangular "
+                       f"sweeps are serial either way); release flags; {out['iters']} iterations, setup "
+                       f"{t1 - t0:.3f} s excluded"),
             "iters": out["iters"], "converged": out["converged"], "setup_s": round(t1 - t0, 4),
             "host_cpus": os.cpu_count()}
+
+
+def oracle_errors(A, b, g, xs):
+    """The FP64 oracle's solve of the same system (FlexibleCG + IC(0) on the
+    red-black ordered matrix, the order the library's colouring gives a grid,
+    to rtol 1e-12) and each GPU solution's relative error against it."""
+    from oracle import ppo
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from mc_order import parity_order, permute_fast
+    order = parity_order(g, g, g)
+    Ap = permute_fast(A, order)
+    t0 = time.perf_counter()
+    o = ppo.cg(Ap, b[order], ppo.Prec(Ap, "ic0"), tol=1e-12, maxit=4000)
+    xo = np.empty(len(b))
+    xo[order] = o["x"]
+    out = {"oracle": f"FlexibleCG + IC(0) on the red-black matrix to rtol 1e-12, {o['iters']} iterations, "
+                     f"{time.perf_counter() - t0:.1f} s"}
+    for name, x in xs.items():
+        out[name] = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
+    return out
 
 
 def timed(torch, fn):
@@ -244,13 +270,18 @@ def main():
     import torch
 
     if args.copy_probe:
+        import prispack_amd as pp
+        from prispack_amd import _capi
+        L = _capi.lib()
+        ctx = pp.Context.default()
         nb = args.copy_probe << 20
         src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
-        for _ in range(5):
-            torch.mul(src, 1.0, out=dst)   # an elementwise kernel: reads nb bytes, writes nb bytes
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": nb, "launches": 5}), flush=True)
+        for _ in range(5):  # the streaming copy kernel: reads nb bytes, writes nb bytes
+            ctx.check(L.pp_copy_stream(ctx.h, C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), nb, 0))
+        ctx.sync()
+        print(json.dumps({"copy_probe_bytes": nb, "launches": 5, "kernel": "k_copy_stream"}), flush=True)
         return
 
     import prispack_amd as pp
@@ -333,6 +364,8 @@ def main():
     # ---- kernel throughput on the library's stream
     stream = L.pp_ctx_stream(ctx.h)
     spmv_var = C.c_int()
+    if args.spmv_variant >= 0:
+        ctx.check(L.pp_csr_set_spmv_variant(M.h, 0, args.spmv_variant))
     ctx.check(L.pp_csr_spmv_variant(M.h, 0, C.byref(spmv_var)))
     # the FP64 SpMV alternates between two identical copies of the matrix and
     # of x, y (2 x 188 MB, more than the 256 MB last-level cache): each launch
@@ -384,12 +417,21 @@ def main():
     trsv32_ms = kernel_time_ms(torch, stream, lambda: L.pp_precond_apply(ctx.h, P.h, 1, ptv32, ptv32),
                                max(3, args.kernel_reps // 10))
     gbs = lambda by, ms: by / (ms * 1e-3) / 1e9
-    # what a plain streaming kernel reaches on this device: an elementwise
-    # copy of 1 GiB (read 1 GiB, write 1 GiB), the same HIP-event timing
+    # what a kernel that only moves bytes reaches on this device: the
+    # library's streaming copy (16-byte non-temporal loads and stores) of
+    # 1 GiB (read 1 GiB, write 1 GiB) at four launch widths, the best kept,
+    # timed with HIP events on the library stream; torch's elementwise copy
+    # beside it for reference
     src = torch.ones((1 << 30) // 8, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    copy_ms = kernel_time_ms(torch, None, lambda: torch.mul(src, 1.0, out=dst), 10)
-    copy_gbs = gbs(2 << 30, copy_ms)
+    ps, pd = C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr())
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    copy_by_groups = {}
+    for per_cu in (8, 16, 32, 64):
+        ms = kernel_time_ms(torch, stream, lambda: L.pp_copy_stream(ctx.h, ps, pd, 1 << 30, cus * per_cu), 10)
+        copy_by_groups[cus * per_cu] = round(gbs(2 << 30, ms), 1)
+    copy_gbs = max(copy_by_groups.values())
+    torch_copy_gbs = gbs(2 << 30, kernel_time_ms(torch, None, lambda: torch.mul(src, 1.0, out=dst), 10))
     del src, dst
     log(f"[bench] SpMV fp64 {spmv_ms * 1e3:.1f} us ({gbs(12 * nnz + 20 * n + 4, spmv_ms):.0f} GB/s algorithmic, "
         f"{gbs(spmv_bytes, spmv_ms):.0f} GB/s streamed); fp32 {spmv32_ms * 1e3:.1f} us; "
@@ -414,7 +456,10 @@ def main():
                 "unit": "GB/s", "frac": round(gbs(alg_spmv, spmv_ms) / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": alg_spmv, "avg_us": round(spmv_ms * 1e3, 2),
                 "launches_timed": args.kernel_reps, "bytes_streamed": spmv_bytes,
-                "copy_kernel_GBs": round(copy_gbs, 1), "frac_of_copy": round(gbs(alg_spmv, spmv_ms) / copy_gbs, 4),
+                "copy_kernel_GBs": round(copy_gbs, 1), "copy_kernel": "k_copy_stream (pp_copy_stream), best of "
+                f"{sorted(copy_by_groups)} workgroups: {copy_by_groups}",
+                "torch_copy_GBs": round(torch_copy_gbs, 1),
+                "frac_of_copy": round(gbs(alg_spmv, spmv_ms) / copy_gbs, 4),
                 "streamed_GBs": round(gbs(spmv_bytes, spmv_ms), 1), "variant": spmv_var.value}
     apply_rec = {"kernel": f"IC(0) apply = forward + backward sweep ({sched} schedule, {info.levels_lower} levels)",
                  "bound": "hbm", "achieved": round(gbs(alg_apply, trsv_ms), 1), "peak": HBM_PEAK_GBS,
@@ -445,11 +490,15 @@ def main():
     extra = None if args.no_extra else extra_records(torch, pp, args)
 
     cpu = None
+    errs = None
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(A, b, n)
         except FileNotFoundError as e:
             cpu = {"value": None, "unit": "s", "cores": 0, "kind": "reference", "sample": f"unavailable: {e}"}
+        if args.ordering == 1:
+            errs = oracle_errors(A, b, g, {"mixed": xmx.cpu().numpy(), "fp64": x64.cpu().numpy()})
+            log(f"[bench] error against the FP64 oracle: {errs}")
 
     med64, medmx = float(np.median(t64)), float(np.median(tmx))
     out = {
@@ -470,7 +519,8 @@ def main():
         "iters_mixed_single_reduction_inner": same["mixed_sr"][1].inner_iters,
         "iters_fp64": rep64.iters, "iters_mixed_inner": repmx.inner_iters, "outer_mixed": repmx.outer_iters,
         "relres_fp64": rep64.true_relres, "relres_mixed": repmx.true_relres,
-        "err_mixed_vs_fp64": err_mixed, "err_vs_xtrue": err_true, "setup_s": round(setup_s, 3),
+        "err_mixed_vs_fp64": err_mixed, "err_vs_xtrue": err_true,
+        "err_vs_fp64_oracle": errs, "setup_s": round(setup_s, 3),
         "levels": [info.levels_lower, info.levels_upper],
         "setup_phases_ms": [round(float(v), 1) for v in info.phase_ms],
         "first_mixed_solve_s": None if first_mixed_s is None else round(first_mixed_s, 4),

@@ -67,6 +67,11 @@ int pp_memcpy_h2d(pp_ctx *ctx, void *dst, const void *src, size_t bytes);
 int pp_memcpy_d2h(pp_ctx *ctx, void *dst, const void *src, size_t bytes);
 int pp_memcpy_d2d(pp_ctx *ctx, void *dst, const void *src, size_t bytes);
 int pp_memset_zero(pp_ctx *ctx, void *dst, size_t bytes);   /* fill kernel, graph-safe */
+/* dst = src by a streaming kernel (16-byte non-temporal loads and stores;
+ * 16-byte aligned device pointers; groups 0 = 64 workgroups per CU).  No
+ * reference counterpart: the rate a kernel that only moves bytes reaches,
+ * which the SpMV roofline is compared with. */
+int pp_copy_stream(pp_ctx *ctx, const void *src, void *dst, size_t bytes, uint32_t groups);
 
 /* ---- device CSR (DEVICE::DeviceCSRMatrix, include/MatrixClass/DeviceCSRMatrix.cuh:19-93;
  *      host storage HOST::CSRMatrix, include/MatrixClass/CSRMatrix.h:19-31) ---- */
@@ -172,9 +177,28 @@ typedef struct {
     uint32_t levels;     /* multilevel depth */
     uint32_t ndom;       /* subdomains at the first level */
     uint32_t rank;       /* low-rank correction rank */
-    double eps;          /* AMSED deflation bound */
-    uint32_t reserved[8];
+    double eps;          /* AMSED deflation bound (m_eigValCorrectBound) */
+    uint32_t lanczos_steps; /* AMSED/GMSLR Krylov steps (0: the form's default; the
+                               deflated pairs at most for PP_LOWRANK_DEFLATION,
+                               AMSED_MSTEPS = 30 by default) */
+    int32_t lowrank;     /* PP_LOWRANK_* */
+    uint32_t reserved[6];
 } pp_precond_opts;
+
+/* Low-rank correction form of AMSED / GMSLR:
+ * REFERENCE (AUTO) -- AMSED: thick-restart (Krylov-Schur) generalised Lanczos
+ *   with m = 2 rank, k = 4 rank steps and at most three restarts, H = mu/(1-mu)
+ *   on the rank largest Ritz values (AMSEDBasic, AMSEDPrecondition.cpp:
+ *   1010-1231); GMSLR: Arnoldi, the first rank entries of the real Schur form
+ *   in its own order with a diagonal core lambda/(1-lambda)
+ *   (GMSLRDPrecondition.cpp:1008-1032);
+ * DEFLATION -- AMSED only: the same restarted Lanczos, H = (eps-lam)/lam with
+ *   lam = 1-mu for every mu > 1-eps (AMSEDEigenvalueDeflation, :1233-1265);
+ * SUBSPACE -- no reference counterpart: AMSED one Lanczos run of
+ *   max(2 rank, rank+16) steps, H = theta/(1-theta) on the largest Ritz values in
+ *   [0,1); GMSLR the largest real eigenvalues below 1, an orthonormal basis of
+ *   their eigenvectors and the full core (I-R)^-1 - I, exact on that subspace. */
+enum { PP_LOWRANK_AUTO = 0, PP_LOWRANK_REFERENCE = 1, PP_LOWRANK_DEFLATION = 2, PP_LOWRANK_SUBSPACE = 3 };
 
 typedef struct {
     int kind;
@@ -194,12 +218,25 @@ typedef struct {
     int32_t trsv;           /* the SpTRSV schedule in use (PP_TRSV_AUTO resolved at setup) */
     float phase_ms[4];      /* setup phases: ordering, factorization, sweep analysis + upload,
                                schedule choice + load-variant timing */
+    uint32_t device_refactors; /* pp_precond_refactor calls that factored on the device */
 } pp_precond_info;
 
 void pp_precond_opts_default(pp_precond_opts *o);
 int pp_precond_create(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *opts, pp_precond **out);
 /* setup(): factorization on the host, level analysis, upload (BasePreconditon.h:102) */
 int pp_precond_setup(pp_ctx *ctx, pp_precond *M);
+/* Numeric re-factorization from the matrix's current values (pp_csr_set_values
+ * keeps the pattern).  No reference counterpart: the reference's setup() does
+ * nothing once ready (IncompleteCholesky.cpp:105) and a new object has to be
+ * made.  IC(0) and ILU(0) (any ordering, any schedule but PP_TRSV_BLOCK) are
+ * factored on the device: one launch per dependency level, one thread per
+ * row, the host factorization's operation order (the factor is the same bit
+ * for bit), written into the existing sweeps (graphs captured with them stay
+ * valid); the symbolic maps are built by the first call.  Other kinds run the
+ * full setup() again.  A failing pivot returns PP_ERR_NOT_SPD /
+ * PP_ERR_ZERO_PIVOT with pp_precond_info.fail_row, and the preconditioner is
+ * then not ready until a successful call. */
+int pp_precond_refactor(pp_ctx *ctx, pp_precond *M);
 /* Pins the level-sweep variant of both precisions instead of the timed choice
  * (0..7 the staged kernel's loads, 8 the sliced-column form of the wide
  * contiguous levels, the other levels staged with the default loads); every

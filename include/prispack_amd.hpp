@@ -308,6 +308,8 @@ public:
     DevicePrecondition &operator=(const DevicePrecondition &) = delete;
 
     void setup() override { m_ctx->check(pp_precond_setup(m_ctx->handle(), m_h)); }
+    // numeric re-factorization from the matrix's current values (pp_precond_refactor)
+    void refactor() { m_ctx->check(pp_precond_refactor(m_ctx->handle(), m_h)); }
     uint8_t isReady() const override { return (uint8_t)info().ready; }
     void MInvSolve(BaseVector<double> &vec) override { MInvSolve(vec, 0, vec, 0); }
     void MInvSolve(BaseVector<double> &vec, uint32_t resStartIdx) override {
@@ -469,7 +471,9 @@ struct MultilevelOptions {
     uint32_t levels = 3, ndom = 16, rank = 32;
     double tolerance = -1.0;       // block factor drop tolerance (< 0: 1e-3 AMSED, 1e-2 GMSLR)
     uint32_t lfill = 20;           // GMSLR ILUT fill per row
-    uint32_t krylovSteps = 0;      // Lanczos / Arnoldi steps (0: max(2 rank, rank + 16))
+    uint32_t krylovSteps = 0;      // Lanczos / Arnoldi steps (0: the low-rank form's default)
+    int lowrank = PP_LOWRANK_AUTO; // PP_LOWRANK_*: the reference's restarted Lanczos / Schur selection by default
+    double eps = 0.8;              // AMSED deflation bound (PP_LOWRANK_DEFLATION)
 };
 
 class DeviceAMSED : public DevicePrecondition {
@@ -487,7 +491,9 @@ protected:
         o.rank = m.rank;
         o.tau = m.tolerance >= 0.0 ? m.tolerance : tau;
         o.lfil = m.lfill;
-        o.reserved[0] = m.krylovSteps;
+        o.lanczos_steps = m.krylovSteps;
+        o.lowrank = m.lowrank;
+        o.eps = m.eps;
         return o;
     }
 };

@@ -83,21 +83,17 @@ public:
     GPUTriangularPrecondition(const GPUTriangularPrecondition &) = delete;
     GPUTriangularPrecondition &operator=(const GPUTriangularPrecondition &) = delete;
 
-    // BasePreconditon.h:102: factor on the host, upload, sweep analysis.  A
-    // second call re-factors from the matrix's current values.
+    // BasePreconditon.h:102: factor on the host, upload, sweep analysis; a
+    // second call does nothing, as the reference's (IncompleteCholesky.cpp:105)
     void setup() override {
+        if (this->m_isReady == PRECONDITION_READY) return;
         const uint32_t n = this->m_ArowNum, nnz = this->m_Annz;
         const uint32_t *rp = m_matA->getRowOffsetPtr(0);
         if (!m_ctx) check_global(pp_ctx_create(m_device, &m_ctx));
         std::vector<double> v(m_matA->getCSRValuesPtr(0), m_matA->getCSRValuesPtr(0) + nnz);
-        if (!m_A) {
-            check(pp_csr_create(m_ctx, n, this->m_AcolNum, rp, m_matA->getColIndicesPtr(0), v.data(), &m_A));
-            check(pp_precond_create(m_ctx, m_A, &m_opts, &m_M));
-            check(pp_malloc(m_ctx, (size_t)n * sizeof(T), &m_stage));
-        } else {
-            check(pp_csr_set_values(m_ctx, m_A, v.data(), 0));
-        }
-        this->m_isReady = PRECONDITION_NOT_READY;
+        check(pp_csr_create(m_ctx, n, this->m_AcolNum, rp, m_matA->getColIndicesPtr(0), v.data(), &m_A));
+        check(pp_precond_create(m_ctx, m_A, &m_opts, &m_M));
+        check(pp_malloc(m_ctx, (size_t)n * sizeof(T), &m_stage));
         check(pp_precond_setup(m_ctx, m_M));
         pp_precond_info info{};
         check(pp_precond_info_get(m_M, &info));

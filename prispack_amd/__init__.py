@@ -334,13 +334,19 @@ class BasePrecondition:
         self._opts = _popts(self._kind, make_f32=make_f32, **opts)
         steps = getattr(self, "_lanczos_steps", 0)
         if steps:
-            self._opts.reserved[0] = steps
+            self._opts.lanczos_steps = steps
         h = C.c_void_p()
         self.ctx.check(_capi.lib().pp_precond_create(self.ctx.h, self.A.h, C.byref(self._opts), C.byref(h)))
         self.h = h
 
     def setup(self):
         self.ctx.check(_capi.lib().pp_precond_setup(self.ctx.h, self.h))
+
+    def refactor(self):
+        """Numeric re-factorization from the matrix's current values (same
+        pattern): on the device for IC(0) / ILU(0), the full setup otherwise
+        (no reference counterpart; the reference's setup() is a no-op once ready)."""
+        self.ctx.check(_capi.lib().pp_precond_refactor(self.ctx.h, self.h))
 
     def isReady(self):
         return self.info().ready
@@ -647,9 +653,18 @@ class AMSED(BasePrecondition):
     drop tolerance of the block factors (0 = exact blocks)."""
     _kind, _type = _capi.PP_PC_AMSED, PreconditionAMSED
 
-    def __init__(self, A, levels=3, ndom=16, rank=32, tolerance=1e-3, lanczos_steps=0, exact=False, **kw):
+    _LOWRANK = {"auto": 0, "reference": 1, "deflation": 2, "subspace": 3}
+
+    def __init__(self, A, levels=3, ndom=16, rank=32, tolerance=1e-3, lanczos_steps=0, exact=False,
+                 lowrank="auto", eps=0.8, **kw):
+        """lowrank: "reference" (default "auto"; AMSED: the reference's thick-restart
+        Lanczos with H = mu / (1 - mu); GMSLR: its first Schur pairs with a
+        diagonal core), "deflation" (AMSED: H = (eps - lam) / lam for mu > 1 - eps)
+        or "subspace" (this build's one-run Lanczos / exact-subspace core)."""
         self._lanczos_steps = lanczos_steps
-        super().__init__(A, tau=0.0 if exact else tolerance, levels=levels, ndom=ndom, rank=rank, **kw)
+        super().__init__(A, tau=0.0 if exact else tolerance, levels=levels, ndom=ndom, rank=rank,
+                         lowrank=self._LOWRANK[lowrank] if isinstance(lowrank, str) else int(lowrank), eps=eps,
+                         **kw)
 
     def info_levels(self):
         nlev = C.c_uint32()
@@ -670,6 +685,6 @@ class GMSLR(AMSED):
     _kind, _type = _capi.PP_PC_GMSLR, PreconditionGMSLR
 
     def __init__(self, A, levels=3, ndom=16, rank=32, tolerance=1e-2, lfill=20, lanczos_steps=0, exact=False,
-                 **kw):
+                 lowrank="auto", **kw):
         super().__init__(A, levels=levels, ndom=ndom, rank=rank, tolerance=tolerance, lanczos_steps=lanczos_steps,
-                         exact=exact, lfil=lfill if not exact else 1 << 30, **kw)
+                         exact=exact, lfil=lfill if not exact else 1 << 30, lowrank=lowrank, **kw)

@@ -21,6 +21,7 @@ PP_FP64, PP_MIXED = 0, 1
 PP_LAUNCH_GRAPH, PP_LAUNCH_STREAM = 0, 1
 PP_CG_AUTO, PP_CG_REFERENCE, PP_CG_SINGLE_REDUCTION, PP_CG_REDUCED = 0, 1, 2, 3
 PP_ORDER_NATURAL, PP_ORDER_MULTICOLOR, PP_ORDER_RCM = 0, 1, 2
+PP_LOWRANK_AUTO, PP_LOWRANK_REFERENCE, PP_LOWRANK_DEFLATION, PP_LOWRANK_SUBSPACE = 0, 1, 2, 3
 
 STATUS_NAMES = {0: "PP_OK", 1: "PP_ERR_INVALID", 2: "PP_ERR_HIP", 3: "PP_ERR_NOT_SPD", 4: "PP_ERR_ZERO_PIVOT",
                 5: "PP_ERR_ZERO_ROW", 6: "PP_ERR_NOT_READY", 7: "PP_ERR_BREAKDOWN", 8: "PP_ERR_NOT_FINITE",
@@ -36,7 +37,8 @@ class PrisPackError(RuntimeError):
 class PrecondOpts(C.Structure):
     _fields_ = [("kind", C.c_int), ("tau", C.c_double), ("lfil", C.c_uint32), ("trsv", C.c_int),
                 ("make_f32", C.c_int), ("ordering", C.c_int), ("levels", C.c_uint32), ("ndom", C.c_uint32),
-                ("rank", C.c_uint32), ("eps", C.c_double), ("reserved", C.c_uint32 * 8)]
+                ("rank", C.c_uint32), ("eps", C.c_double), ("lanczos_steps", C.c_uint32), ("lowrank", C.c_int32),
+                ("reserved", C.c_uint32 * 6)]
 
 
 class PrecondInfo(C.Structure):
@@ -44,7 +46,7 @@ class PrecondInfo(C.Structure):
                 ("mnnz", C.c_uint32), ("fill_ratio", C.c_double), ("levels_lower", C.c_uint32),
                 ("levels_upper", C.c_uint32), ("setup_ms", C.c_double), ("fail_row", C.c_uint32),
                 ("sweep_variant", C.c_int32), ("fused_levels", C.c_int32), ("trsv", C.c_int32),
-                ("phase_ms", C.c_float * 4)]
+                ("phase_ms", C.c_float * 4), ("device_refactors", C.c_uint32)]
 
 
 class SolveOpts(C.Structure):
@@ -85,6 +87,7 @@ SIGS = {
     "pp_memcpy_d2h": (C.c_int, [vp, vp, vp, C.c_size_t]),
     "pp_memcpy_d2d": (C.c_int, [vp, vp, vp, C.c_size_t]),
     "pp_memset_zero": (C.c_int, [vp, vp, C.c_size_t]),
+    "pp_copy_stream": (C.c_int, [vp, vp, vp, C.c_size_t, C.c_uint32]),
     "pp_csr_create": (C.c_int, [vp, u32, u32, u32p, u32p, f64p, C.POINTER(vp)]),
     "pp_csr_set_values": (C.c_int, [vp, vp, vp, C.c_int]),
     "pp_csr_dims": (C.c_int, [vp, u32p, u32p, u32p]),
@@ -100,6 +103,7 @@ SIGS = {
     "pp_precond_opts_default": (None, [C.POINTER(PrecondOpts)]),
     "pp_precond_create": (C.c_int, [vp, vp, C.POINTER(PrecondOpts), C.POINTER(vp)]),
     "pp_precond_setup": (C.c_int, [vp, vp]),
+    "pp_precond_refactor": (C.c_int, [vp, vp]),
     "pp_precond_apply": (C.c_int, [vp, vp, C.c_int, vp, vp]),
     "pp_precond_apply_part": (C.c_int, [vp, vp, C.c_int, C.c_int, vp]),
     "pp_precond_info_get": (C.c_int, [vp, C.POINTER(PrecondInfo)]),

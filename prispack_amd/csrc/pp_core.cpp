@@ -513,6 +513,14 @@ int pp_memcpy_d2d(pp_ctx *ctx, void *dst, const void *src, size_t bytes) {
     return guarded(ctx, [&] { PP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream)); });
 }
 
+int pp_copy_stream(pp_ctx *ctx, const void *src, void *dst, size_t bytes, uint32_t groups) {
+    return guarded(ctx, [&] {
+        PP_CHECK(src && dst, PP_ERR_INVALID, "NULL pointer");
+        PP_CHECK(((uintptr_t)src | (uintptr_t)dst) % 16 == 0, PP_ERR_INVALID, "pointers must be 16-byte aligned");
+        launch_copy_stream(ctx, src, dst, bytes, groups);
+    });
+}
+
 int pp_memset_zero(pp_ctx *ctx, void *dst, size_t bytes) {
     return guarded(ctx, [&] { launch_fill_zero(ctx, dst, bytes); });
 }

@@ -249,6 +249,30 @@ struct DevChunkSweep {
 };
 
 // Host gather-form sweep: row i = s_i -> (b_i - sum val*x[ci]) op diag_i.
+// Symbolic state of a numeric re-factorization on the device
+// (pp_precond_refactor, IC(0) / ILU(0)): the factor's pattern, its
+// dependency levels, and for every value slot of the sweeps (level-ordered
+// storage, pivots) the factor entry it holds.
+struct DevRefactor {
+    uint32_t n = 0, nnz = 0;       // factor entries: IC(0) L's strict lower part; ILU(0) the matrix pattern
+    uint32_t *rp = nullptr, *ci = nullptr;
+    uint32_t *src = nullptr;       // IC(0): the matrix entry each L entry starts from
+    uint32_t *dsrc = nullptr;      // IC(0): the diagonal entry of each row; ILU(0): its position in the row
+    double *val = nullptr;         // [nnz + n]: the factor entries, then the pivots (IC(0) l_ii; ILU(0) 1/u_ii)
+    uint32_t *rows = nullptr;      // rows in level order
+    std::vector<uint32_t> lptr;    // level bounds in rows
+    uint32_t *lo_map = nullptr, *lo_dmap = nullptr, *up_map = nullptr, *up_dmap = nullptr;
+    uint32_t lo_nnz = 0, up_nnz = 0;
+    bool lo_diag = false, up_diag = false;
+    uint32_t *aperm_map = nullptr; // entry k of P A P^T <- entry aperm_map[k] of A (ordered factors)
+    uint32_t aperm_nnz = 0;
+    int *fail = nullptr;
+    // host: which factor entry (index into val) each entry of the host copy holds
+    std::vector<uint32_t> h_lt_map, h_l_map, h_u_map;
+    void free_all();
+    ~DevRefactor() { free_all(); }
+};
+
 struct HostSweep {
     enum Mode { DIVIDE = 0, UNIT = 1, MULTIPLY = 2, PREDIVIDE = 3 };
     int mode = DIVIDE;
@@ -405,6 +429,10 @@ struct pp_precond {
     // Aperm_version.  io_permuted: the current solve's vectors are already in
     // the factor's order.
     pp_csr *Aperm = nullptr;
+    // numeric re-factorization on the device (built by the first pp_precond_refactor)
+    pp::DevRefactor *dref = nullptr;
+    uint32_t device_refactors = 0;
+    bool host_factor_stale = false;  // the host copy of the factor predates a device re-factorization
     uint64_t Aperm_src_uid = 0;
     uint64_t Aperm_version = 0;
     bool io_permuted = false;
@@ -517,6 +545,18 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
                  double beta, void *y, int dot_slot, const void *dot_with, bool dot_first,
                  const void *aux = nullptr, const void *wgt = nullptr);
 void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes);
+// dst = src, `bytes` bytes, 16-byte aligned pointers; groups 0: 64 per CU
+void launch_copy_stream(pp_ctx *ctx, const void *src, void *dst, size_t bytes, unsigned groups);
+// ---- numeric IC(0) / ILU(0) on the device (pp_devfactor.hip) ----
+void launch_ic0_level(pp_ctx *ctx, const uint32_t *rows, uint32_t nrows, const uint32_t *rp, const uint32_t *ci,
+                      const uint32_t *src, const uint32_t *dsrc, const double *av, double *lv, double *diag,
+                      int *fail);
+void launch_ilu0_level(pp_ctx *ctx, const uint32_t *rows, uint32_t nrows, const uint32_t *rp, const uint32_t *ci,
+                       const uint32_t *dpos, const double *av, double *w, double *dinv, int *fail);
+// out[q] = in[map[q]] (dtype of out)
+void launch_gather_map(pp_ctx *ctx, uint32_t n, const uint32_t *map, const double *in, int dtype, void *out);
+// out[q] = (uint32_t)in[q]
+void launch_to_index(pp_ctx *ctx, uint32_t n, const double *in, uint32_t *out);
 // Average duration (ms) of fn()'s launches on ctx->stream with the last-level
 // cache cleared before each one (a read of `flush_bytes` of other memory), so
 // that operands are streamed from HBM as they are inside a solve.

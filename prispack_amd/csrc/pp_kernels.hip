@@ -2,6 +2,7 @@
 // BLAS-1 with deterministic reductions, level-scheduled triangular sweeps and
 // the PCG update steps.  64-wide wavefronts, 256-thread workgroups.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <algorithm>
 
@@ -355,6 +356,84 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const i
     }
 }
 
+// The same product with two consecutive rows per thread (no sums): each
+// thread loads the pair's offsets as one 32-bit word and the pair's values
+// as one 2-element vector per entry slot, and stores the two results as one
+// vector, so a wavefront moves twice the bytes per load instruction.  Each
+// row's operation sequence is the one above, so y is the same bit for bit.
+// Needs an even row count and y aligned to two elements (checked by the
+// launcher).
+template <class T, int MAXW>
+__global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const int16_t *__restrict__ off,
+                                              const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
+                                              int64_t dbase, const T *x, T *y, T alpha, T beta, int beta_zero,
+                                              const int *stop) {
+    using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+    if (stop && *stop) return;
+    const uint32_t npair = n / 2;
+    const uint32_t nb = (npair + kBlock - 1) / kBlock;
+    const XcdRange xr = xcd_range(0, nb);
+    for (uint32_t b = xr.first; b < xr.end; b += xr.step) {
+        const uint32_t t = b * kBlock + threadIdx.x;
+        if (t >= npair) continue;
+        const uint32_t i = 2 * t;
+        int16_t o0[MAXW], o1[MAXW];
+        T v0[MAXW], v1[MAXW], x0[MAXW], x1[MAXW];
+#pragma unroll
+        for (int j = 0; j < MAXW; ++j)
+            if (j < (int)width) {
+                const size_t e = (size_t)j * n + i;
+                const uint32_t ow = *reinterpret_cast<const uint32_t *>(off + e);
+                o0[j] = (int16_t)(ow & 0xffffu);
+                o1[j] = (int16_t)(ow >> 16);
+                const T2 vv = *reinterpret_cast<const T2 *>(val + e);
+                v0[j] = vv.x;
+                v1[j] = vv.y;
+            }
+        T dg0 = 0, dg1 = 0, xd0 = 0, xd1 = 0, yo0 = 0, yo1 = 0;
+        if (dg) {
+            dg0 = dg[i];
+            dg1 = dg[i + 1];
+            xd0 = x[dbase + i];
+            xd1 = x[dbase + i + 1];
+        }
+        if (!beta_zero) {
+            const T2 yy = *reinterpret_cast<const T2 *>(y + i);
+            yo0 = yy.x;
+            yo1 = yy.y;
+        }
+#pragma unroll
+        for (int j = 0; j < MAXW; ++j)
+            if (j < (int)width) {
+                x0[j] = x[(int64_t)i + cbase + o0[j]];
+                x1[j] = x[(int64_t)i + 1 + cbase + o1[j]];
+            }
+        T s0 = 0, s1 = 0;
+        {
+#pragma clang fp contract(off)
+#pragma unroll
+            for (int j = 0; j < MAXW; ++j)
+                if (j < (int)width) {
+                    s0 += v0[j] * x0[j];
+                    s1 += v1[j] * x1[j];
+                }
+            if (dg) {
+                s0 += dg0 * xd0;
+                s1 += dg1 * xd1;
+            }
+        }
+        T r0 = alpha * s0, r1 = alpha * s1;
+        if (!beta_zero) {
+            r0 += beta * yo0;
+            r1 += beta * yo1;
+        }
+        T2 out;
+        out.x = r0;
+        out.y = r1;
+        *reinterpret_cast<T2 *>(y + i) = out;
+    }
+}
+
 // With the sums, at most 1024 workgroups, each over several 256-row
 // slices: fewer partials for the consumer to add, measured 2% faster per
 // reduced iteration than one slice per workgroup at 1M rows (512: slower).
@@ -369,6 +448,26 @@ unsigned ell_grid(const EllOp &E, bool dot) {
 void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void *x, double beta, void *y,
                 int dot_slot, const void *dotw, const void *aux, const void *wgt, unsigned grid) {
     const bool dot = dot_slot >= 0;
+    // two rows per thread where the pair loads and the pair store are aligned
+    const size_t esz = dtype == PP_F64 ? 8 : 4;
+    if (!dot && grid == 0 && E.n >= 2 && E.n % 2 == 0 && ((uintptr_t)y % (2 * esz)) == 0) {
+        const uint32_t nb2 = (E.n / 2 + kBlock - 1) / kBlock;
+        const unsigned g2 = xcd_grid(nb2, 1u << 20);
+        const int bz2 = beta == 0.0;
+#define PP_ELL2(T, W, VALS, DG)                                                                                  \
+    k_ell2<T, W><<<g2, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, VALS, E.cbase, DG, E.dbase, (const T *)x, \
+                                                  (T *)y, (T)alpha, (T)beta, bz2, ctx->stop_word)
+        if (dtype == PP_F64) {
+            if (E.width <= 8) PP_ELL2(double, 8, E.v64, E.dg64);
+            else PP_ELL2(double, 16, E.v64, E.dg64);
+        } else {
+            if (E.width <= 8) PP_ELL2(float, 8, E.v32, E.dg32);
+            else PP_ELL2(float, 16, E.v32, E.dg32);
+        }
+#undef PP_ELL2
+        PP_HIP(hipGetLastError());
+        return;
+    }
     if (grid == 0) grid = ell_grid(E, dot);
     if (grid == 0) return;
     double *part = dot ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
@@ -516,8 +615,8 @@ void launch_spmv(pp_ctx *ctx, const pp_csr *A, int dtype, double alpha, const vo
     const int bz = beta == 0.0;
     const bool dot = dot_slot >= 0;
     const int var = dtype == PP_F64 ? A->spmv_variant : A->spmv_variant32;
-    if (var == kVarEll) {  // same number of workgroups (partials) as the CSR-stream launch
-        launch_ell(ctx, A->ell, dtype, alpha, x, beta, y, dot_slot, dot_with, aux, wgt, grid);
+    if (var == kVarEll) {  // with sums: as many workgroups (partials) as the CSR-stream launch
+        launch_ell(ctx, A->ell, dtype, alpha, x, beta, y, dot_slot, dot_with, aux, wgt, dot ? grid : 0);
         if (dot && !aux) launch_finish(ctx, dot_slot, grid, ctx->stop_word);  // with aux the caller finishes
         PP_HIP(hipGetLastError());
         return;
@@ -599,6 +698,33 @@ void launch_fill_zero(pp_ctx *ctx, void *p, size_t bytes) {
     size_t words = bytes / 4;
     if (words) k_fill_zero<<<grid_for(ctx, words), kBlock, 0, ctx->stream>>>((uint32_t *)p, words);
     if (bytes % 4) PP_HIP(hipMemsetAsync((char *)p + words * 4, 0, bytes % 4, ctx->stream));
+    PP_HIP(hipGetLastError());
+}
+
+// Streaming copy: 16-byte non-temporal loads and stores, four in flight per
+// thread.  The reference point of the HBM roofline (what a kernel that only
+// moves bytes reaches on this device).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy_stream(size_t n16, const u32x4 *__restrict__ s, u32x4 *__restrict__ d) {
+    const size_t step = (size_t)gridDim.x * kBlock * 4;
+    for (size_t i = (size_t)blockIdx.x * kBlock * 4 + threadIdx.x; i < n16; i += step) {
+        u32x4 a[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + (size_t)k * kBlock < n16) a[k] = __builtin_nontemporal_load(s + i + (size_t)k * kBlock);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + (size_t)k * kBlock < n16) __builtin_nontemporal_store(a[k], d + i + (size_t)k * kBlock);
+    }
+}
+
+void launch_copy_stream(pp_ctx *ctx, const void *src, void *dst, size_t bytes, unsigned groups) {
+    const size_t n16 = bytes / 16;
+    if (groups == 0) groups = (unsigned)ctx->cu_count * 64;
+    if (n16) k_copy_stream<<<groups, kBlock, 0, ctx->stream>>>(n16, (const u32x4 *)src, (u32x4 *)dst);
+    if (bytes % 16)
+        PP_HIP(hipMemcpyAsync((char *)dst + n16 * 16, (const char *)src + n16 * 16, bytes % 16,
+                              hipMemcpyDeviceToDevice, ctx->stream));
     PP_HIP(hipGetLastError());
 }
 

@@ -19,6 +19,8 @@ HostCsr extract_block(const HostCsr &A, const Hid &H, uint32_t r0, uint32_t r1, 
 HostCsr transpose_rect(const HostCsr &A, uint32_t ncol);
 void tridiag_eig(std::vector<double> &d, std::vector<double> e, std::vector<double> &Z);
 bool hessenberg_eigenvalues(std::vector<double> H, int n, std::vector<double> &re, std::vector<double> &im);
+bool hessenberg_schur(std::vector<double> &H, int n, std::vector<double> &Z, std::vector<double> &wr,
+                      std::vector<double> &wi);
 std::vector<double> inverse_iteration(const std::vector<double> &H, int n, double lambda);
 // W (n x r, col-major) = V (n x k, ld n) * Y (k x r, ld k), FP64 on the matrix cores
 void launch_gemm_tall_f64(pp_ctx *ctx, uint32_t n, uint32_t k, uint32_t r, const double *V, const double *Y,
@@ -49,6 +51,7 @@ struct MlState {
     bool have32 = false;                  // FP32 copies of every operand exist
     uint32_t n = 0;
     uint32_t lanczos_steps = 0;
+    int lowrank = 0;                      // PP_LOWRANK_* form the low-rank terms were built with
 };
 
 }  // namespace pp

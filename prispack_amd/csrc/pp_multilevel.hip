@@ -313,6 +313,246 @@ void lowrank_lanczos(pp_ctx *ctx, MlState &S, int l, uint32_t rank, uint32_t ste
     dfree(V); dfree(Z); dfree(u); dfree(w); dfree(tb);
 }
 
+// ---------------------------------------------------------------------------
+// The reference's low-rank setup of AMSED: thick-restart (Krylov-Schur)
+// generalised Lanczos, AMSEDPrecondition.cpp:1010-1273, 1276-1345, 1382-1470.
+//   m = 2 rank steps (AMSED_LANCZOS_EXPAND_COEFFICIENT), extended to
+//   k = 4 rank (AMSED_LANCZOS_MAX_COEFFICIENT) before each of at most three
+//   restarts; a restart keeps the m largest Ritz pairs (V <- V Y_m), turns
+//   diag(theta_m) bordered by beta_k Y(k, 1:m) back into a tridiagonal matrix
+//   by the reference's chain of Givens rotations (applied to the kept basis
+//   too), and continues the three-term recurrence from the old residual
+//   vector; it stops early when |beta_k Y(k, k)| <= 1e-30.
+// The low-rank term from the final Ritz pairs mu (largest first):
+//   PP_LOWRANK_REFERENCE  rank pairs, H = mu / (1 - mu)        (AMSEDBasic)
+//   PP_LOWRANK_DEFLATION  the pairs with mu > 1 - eps (at most lanczos_cap),
+//                         H = (eps - lam) / lam, lam = 1 - mu  (AMSEDEigenvalueDeflation)
+// The basis V (C-orthonormal) and Z = C V live on the device; the products
+// V Y and V Q are MFMA GEMMs, the small tridiagonal work is on the host.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr double kAmsedExitZero = 1e-15;          // AMSED_EXIT_ZERO
+constexpr uint32_t kAmsedMsteps = 30;             // AMSED_MSTEPS: the deflated pairs at most
+constexpr double kAmsedHighExitZero = 1e-30;      // AMSED_HIGH_PRECISION_EXIT_ZERO
+
+// Lanczos steps j = from .. to-1 of the generalised problem
+// E^T B^{-1} E v = theta C v (expandReorthogonalLanczos): alpha_j = D[j],
+// beta_j = E[j]; returns the number of steps whose vectors exist.
+uint32_t lanczos_expand(pp_ctx *ctx, MlState &S, int l, double *V, double *Z, std::vector<double> &D,
+                        std::vector<double> &E, uint32_t from, uint32_t to, double *u, double *w, double *tb) {
+    MlLevel &L = S.lev[l];
+    const uint32_t n = L.nrest;
+    uint32_t actual = from;
+    for (uint32_t j = from; j < to; ++j) {
+        if (j > 0 && E[j - 1] <= kAmsedExitZero) return actual;
+        ++actual;
+        double *vj = V + (size_t)j * n, *zj = Z + (size_t)j * n;
+        launch_spmv(ctx, L.E, PP_F64, 1.0, vj, 0.0, tb, -1, nullptr, false);
+        precond_apply(ctx, L.Bpc, PP_F64, tb, tb);
+        launch_spmv(ctx, L.F, PP_F64, 1.0, tb, 0.0, u, -1, nullptr, false);
+        if (j > 0) launch_axpby(ctx, PP_F64, n, 1.0, u, -E[j - 1], Z + (size_t)(j - 1) * n, u);
+        D[j] = dot_dev(ctx, n, u, vj);
+        launch_axpby(ctx, PP_F64, n, 1.0, u, -D[j], zj, u);
+        // full reorthogonalisation u -= Z (V^T u), twice
+        for (int pass = 0; pass < 2; ++pass) {
+            launch_gemv_t(ctx, n, j + 1, V, n, u, nullptr);
+            launch_gemv_sub(ctx, n, j + 1, Z, n, gmres_coeff_buffer(ctx), u);
+        }
+        PP_HIP(hipMemcpyAsync(w, u, (size_t)n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        rec_solve(ctx, S, l, w);  // w = C^{-1} u
+        double bt = dot_dev(ctx, n, w, u);
+        if (!(bt >= 0.0)) {  // approximate B^{-1} and C^{-1} can make it negative: stop
+            E[j] = kAmsedExitZero;
+            break;
+        }
+        bt = std::sqrt(bt);
+        E[j] = bt;
+        if (bt <= kAmsedExitZero) break;
+        launch_axpby(ctx, PP_F64, n, 1.0 / bt, w, 0.0, w, V + (size_t)(j + 1) * n);
+        launch_axpby(ctx, PP_F64, n, 1.0 / bt, u, 0.0, u, Z + (size_t)(j + 1) * n);
+    }
+    return actual;
+}
+
+// Columns c1, c2 of the column-major rows x ? matrix Q: [c1 c2] <- [c1 c + c2 s, -c1 s + c2 c]
+void rotate_columns(std::vector<double> &Q, uint32_t rows, uint32_t c1, uint32_t c2, double c, double s) {
+    double *a = Q.data() + (size_t)c1 * rows, *b = Q.data() + (size_t)c2 * rows;
+    for (uint32_t r = 0; r < rows; ++r) {
+        const double x = a[r], y = b[r];
+        a[r] = x * c + y * s;
+        b[r] = -1.0 * x * s + y * c;
+    }
+}
+
+// diag(mu[0..m)) bordered by the row z back to a tridiagonal (D, E) by the
+// reference's rotation chain (convertDiagonal2TriDiagonal), each rotation
+// also applied to the columns of Q (rows x m); z ends with its weight in its
+// last entry.
+void diag_to_tridiag(const std::vector<double> &mu, std::vector<double> &z, uint32_t m, std::vector<double> &D,
+                     std::vector<double> &E, std::vector<double> &Q, uint32_t rows) {
+    for (uint32_t i = 0; i < m; ++i) D[i] = mu[i];
+    for (uint32_t i = 0; i + 1 < m; ++i) E[i] = 0.0;
+    for (uint32_t i = 0; i + 1 < m; ++i) {
+        const double zi = z[i], zi1 = z[i + 1];
+        const double delta = std::sqrt(zi * zi + zi1 * zi1);
+        double c = zi1 * 1.0 / delta, s = -1.0 * zi / delta;
+        z[i + 1] = delta;
+        z[i] = 0;
+        double cc = c * c, ss = s * s, cs = c * s;
+        {
+            const double ai = D[i], bi = E[i], ai1 = D[i + 1], t = 2 * cs * bi;
+            D[i] = cc * ai + t + ss * ai1;
+            E[i] = -1.0 * cs * ai + (cc - ss) * bi + cs * ai1;
+            D[i + 1] = ss * ai - t + cc * ai1;
+        }
+        rotate_columns(Q, rows, i, i + 1, c, s);
+        if (i > 0) {
+            double gamma = -1.0 * s * E[i - 1];
+            E[i - 1] *= c;
+            for (uint32_t r = i; r >= 1; --r) {
+                const double ar1 = D[r - 1], br1 = E[r - 1], br = E[r], ar = D[r];
+                const double sigma = std::sqrt(gamma * gamma + br * br);
+                c = br * 1.0 / sigma;
+                s = -1.0 * gamma / sigma;
+                cc = c * c;
+                ss = s * s;
+                cs = c * s;
+                const double t = 2 * cs * br1;
+                D[r - 1] = cc * ar1 + t + ss * ar;
+                E[r - 1] = -1.0 * cs * ar1 + (cc - ss) * br1 + cs * ar;
+                D[r] = ss * ar1 - t + cc * ar;
+                E[r] = sigma;
+                if (r > 1) {
+                    gamma = -1.0 * s * E[r - 2];
+                    E[r - 2] *= c;
+                }
+                rotate_columns(Q, rows, r - 1, r, c, s);
+            }
+        }
+    }
+}
+
+// eigenpairs of the tridiagonal (D[0..k), E[0..k-1)): mu ascending, Y k x k
+void tridiag_pairs(const std::vector<double> &D, const std::vector<double> &E, uint32_t k, std::vector<double> &mu,
+                   std::vector<double> &Y) {
+    mu.assign(D.begin(), D.begin() + k);
+    std::vector<double> e(E.begin(), E.begin() + (k ? k - 1 : 0));
+    tridiag_eig(mu, e, Y);
+}
+
+}  // namespace
+
+void lowrank_lanczos_restarted(pp_ctx *ctx, MlState &S, int l, uint32_t rank, int mode, double eps,
+                               uint32_t corr_cap) {
+    MlLevel &L = S.lev[l];
+    const uint32_t n = L.nrest;
+    if (rank == 0 || n < 2) return;
+    // m and k of the reference, kept within the level's size
+    const uint32_t kk = std::min<uint32_t>(4 * rank, n), mm = std::min<uint32_t>(2 * rank, kk);
+    const uint32_t restarts = 3;
+    double *V = dalloc<double>((size_t)n * (kk + 1));
+    double *Z = dalloc<double>((size_t)n * (kk + 1));
+    double *T = dalloc<double>((size_t)n * mm);
+    double *u = dalloc<double>(n), *w = dalloc<double>(n), *tb = dalloc<double>(std::max(1u, L.nb));
+    double *Qd = dalloc<double>((size_t)(kk + 1) * (kk + 1));
+    std::vector<double> D(kk + 1, 0.0), E(kk + 1, 0.0), mu, Y;
+    try {
+        // start: a seeded random z1, v1 = C^{-1} z1 scaled to v1^T C v1 = 1
+        std::vector<double> u0(n);
+        PP_CHECK(pp_fill_random(0, 12345 + l, n, u0.data()) == PP_OK, PP_ERR_INVALID, "rng");
+        PP_HIP(hipMemcpy(u, u0.data(), (size_t)n * 8, hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(w, u, (size_t)n * 8, hipMemcpyDeviceToDevice));
+        rec_solve(ctx, S, l, w);
+        double b0 = dot_dev(ctx, n, w, u);
+        PP_CHECK(b0 > 0 && std::isfinite(b0), PP_ERR_BREAKDOWN, "Lanczos start vector has non-positive C-norm");
+        b0 = std::sqrt(b0);
+        launch_axpby(ctx, PP_F64, n, 1.0 / b0, w, 0.0, w, V);
+        launch_axpby(ctx, PP_F64, n, 1.0 / b0, u, 0.0, u, Z);
+        uint32_t actual = lanczos_expand(ctx, S, l, V, Z, D, E, 0, mm, u, w, tb);
+        bool have_pairs = false;
+        for (uint32_t rs = 0; rs < restarts; ++rs) {
+            if (actual < mm) {  // ended early: the space is invariant
+                tridiag_pairs(D, E, actual, mu, Y);
+                have_pairs = true;
+                break;
+            }
+            actual = lanczos_expand(ctx, S, l, V, Z, D, E, mm, kk, u, w, tb);
+            const double betak = E[actual - 1];
+            tridiag_pairs(D, E, actual, mu, Y);
+            have_pairs = true;
+            const uint32_t last = actual - 1;
+            if (std::fabs(betak) * std::fabs(Y[(size_t)last * actual + last]) <= kAmsedHighExitZero ||
+                rs == restarts - 1 || actual <= mm)
+                break;
+            // keep the mm largest pairs (largest first), V <- V Y_m Q, Z likewise
+            std::vector<double> mud(mm), Qm((size_t)actual * mm), zrow(mm);
+            for (uint32_t c = 0; c < mm; ++c) {
+                const uint32_t src = actual - 1 - c;
+                mud[c] = mu[src];
+                std::copy(Y.begin() + (size_t)src * actual, Y.begin() + (size_t)(src + 1) * actual,
+                          Qm.begin() + (size_t)c * actual);
+                zrow[c] = Y[(size_t)src * actual + last];
+            }
+            diag_to_tridiag(mud, zrow, mm, D, E, Qm, actual);
+            // the rotations gather the border into its last entry: beta_m = z_m beta_k
+            E[mm - 1] = zrow[mm - 1] * betak;
+            PP_HIP(hipMemcpy(Qd, Qm.data(), Qm.size() * 8, hipMemcpyHostToDevice));
+            launch_gemm_tall_f64(ctx, n, actual, mm, V, Qd, T);
+            PP_HIP(hipMemcpyAsync(V, T, (size_t)n * mm * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            launch_gemm_tall_f64(ctx, n, actual, mm, Z, Qd, T);
+            PP_HIP(hipMemcpyAsync(Z, T, (size_t)n * mm * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            PP_HIP(hipMemcpyAsync(V + (size_t)mm * n, V + (size_t)actual * n, (size_t)n * 8, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+            PP_HIP(hipMemcpyAsync(Z + (size_t)mm * n, Z + (size_t)actual * n, (size_t)n * 8, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+            actual = mm;
+        }
+        if (!have_pairs) tridiag_pairs(D, E, actual, mu, Y);
+        // the low-rank term from the largest Ritz pairs
+        std::vector<uint32_t> pick;
+        std::vector<double> hval;
+        if (mode == PP_LOWRANK_DEFLATION) {
+            const double para = 1.0 - eps;
+            for (uint32_t i = 0; i < actual; ++i) {
+                const double m_i = mu[actual - 1 - i];
+                if (m_i <= para || pick.size() >= corr_cap) break;
+                const double lam = 1.0 - m_i;
+                pick.push_back(actual - 1 - i);
+                hval.push_back((eps - lam) / lam);
+            }
+        } else {
+            for (uint32_t i = 0; i < std::min(actual, rank); ++i) {
+                const double m_i = mu[actual - 1 - i];
+                if (!(m_i < 1.0 - 1e-12)) continue;  // mu >= 1 would make mu / (1 - mu) infinite or negative
+                pick.push_back(actual - 1 - i);
+                hval.push_back(m_i / (1.0 - m_i));
+            }
+        }
+        L.rank = (uint32_t)pick.size();
+        L.theta.assign(mu.begin(), mu.end());
+        if (L.rank) {
+            std::vector<double> Ysel((size_t)actual * L.rank);
+            for (uint32_t c = 0; c < L.rank; ++c)
+                std::copy(Y.begin() + (size_t)pick[c] * actual, Y.begin() + (size_t)(pick[c] + 1) * actual,
+                          Ysel.begin() + (size_t)c * actual);
+            PP_HIP(hipMemcpy(Qd, Ysel.data(), Ysel.size() * 8, hipMemcpyHostToDevice));
+            L.W = dalloc<double>((size_t)n * L.rank);
+            launch_gemm_tall_f64(ctx, n, actual, L.rank, V, Qd, L.W);
+            L.Hhost = hval;
+            L.Hd = dalloc<double>(L.rank);
+            PP_HIP(hipMemcpy(L.Hd, L.Hhost.data(), L.rank * 8, hipMemcpyHostToDevice));
+            L.save = dalloc<double>(n);
+            L.h = dalloc<double>(64 * 1024 + (size_t)L.rank * kLrBlocks);
+        }
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        dfree(V); dfree(Z); dfree(T); dfree(u); dfree(w); dfree(tb); dfree(Qd);
+        throw;
+    }
+    dfree(V); dfree(Z); dfree(T); dfree(u); dfree(w); dfree(tb); dfree(Qd);
+}
+
 // I - R for the r x r column-major R
 std::vector<double> identity_minus(const std::vector<double> &R, uint32_t r) {
     std::vector<double> A(R.size());
@@ -355,7 +595,12 @@ std::vector<double> solve_dense(std::vector<double> A, uint32_t r, std::vector<d
 // (GMSLRDPrecondition.cpp:980-1036 takes leading Schur vectors of H with a
 // diagonal core; here the core R = Q^T H Q of the selected invariant subspace
 // is kept whole, so the correction is exact on that subspace).
-void lowrank_arnoldi(pp_ctx *ctx, MlState &S, int l, uint32_t rank, uint32_t steps) {
+// schur_reference: the reference's selection instead (GMSLRDPrecondition.cpp:
+// 1008-1032): the real Schur form H_m = Z T Z^T, the first min(m, rank)
+// diagonal entries of T in the order the Schur form holds them (no sorting)
+// and their Schur vectors, W = V Z(:, 1:r) and a diagonal core
+// H = lambda / (1 - lambda), largest index first as the reference stores it.
+void lowrank_arnoldi(pp_ctx *ctx, MlState &S, int l, uint32_t rank, uint32_t steps, bool schur_reference) {
     MlLevel &L = S.lev[l];
     const uint32_t n = L.nrest;
     const uint32_t k = std::min<uint32_t>(std::min<uint32_t>(steps, n), 64);
@@ -390,6 +635,36 @@ void lowrank_arnoldi(pp_ctx *ctx, MlState &S, int l, uint32_t rank, uint32_t ste
         std::vector<double> Hm((size_t)m * m);
         for (uint32_t c = 0; c < m; ++c)
             for (uint32_t r = 0; r < m; ++r) Hm[(size_t)c * m + r] = Hh[(size_t)c * ld + r];
+        if (schur_reference) {
+            std::vector<double> T = Hm, Zs, wr, wi;
+            PP_CHECK(hessenberg_schur(T, (int)m, Zs, wr, wi), PP_ERR_BREAKDOWN,
+                     "Hessenberg Schur iteration did not converge");
+            const uint32_t r = std::min<uint32_t>(m, rank);
+            L.theta.assign(wr.begin(), wr.end());
+            std::vector<double> Zsel((size_t)m * r);
+            std::vector<double> hv(r);
+            for (uint32_t c = 0; c < r; ++c) {
+                const uint32_t i = r - 1 - c;  // i = r-1 .. 0 into columns 0 .. r-1
+                hv[c] = wr[i] / (1.0 - wr[i]);
+                std::copy(Zs.begin() + (size_t)i * m, Zs.begin() + (size_t)(i + 1) * m, Zsel.begin() + (size_t)c * m);
+            }
+            L.rank = r;
+            if (r) {
+                double *Qd = dalloc<double>(Zsel.size());
+                PP_HIP(hipMemcpy(Qd, Zsel.data(), Zsel.size() * 8, hipMemcpyHostToDevice));
+                L.W = dalloc<double>((size_t)n * r);
+                launch_gemm_tall_f64(ctx, n, m, r, V, Qd, L.W);
+                PP_HIP(hipStreamSynchronize(ctx->stream));
+                dfree(Qd);
+                L.Hhost = hv;
+                L.Hd = dalloc<double>(r);
+                PP_HIP(hipMemcpy(L.Hd, hv.data(), r * 8, hipMemcpyHostToDevice));
+                L.save = dalloc<double>(n);
+                L.h = dalloc<double>(64 * 1024 + (size_t)r * kLrBlocks);
+            }
+            dfree(V); dfree(w); dfree(tb);
+            return;
+        }
         std::vector<double> re, im;
         PP_CHECK(hessenberg_eigenvalues(Hm, (int)m, re, im), PP_ERR_BREAKDOWN,
                  "Hessenberg eigenvalue iteration did not converge");
@@ -542,11 +817,24 @@ void ml_setup(pp_ctx *ctx, pp_precond *M) {
         PP_HIP(hipMemcpy(S->dperm, H.perm.data(), A.n * sizeof(uint32_t), hipMemcpyHostToDevice));
         S->xp = dalloc<double>(A.n);
         const uint32_t rank = M->opts.rank;
-        const uint32_t steps = M->opts.reserved[0] ? M->opts.reserved[0] : std::max<uint32_t>(2 * rank, rank + 16);
+        const uint32_t steps =
+            M->opts.lanczos_steps ? M->opts.lanczos_steps : std::max<uint32_t>(2 * rank, rank + 16);
         S->lanczos_steps = steps;
+        const int form = M->opts.lowrank == PP_LOWRANK_AUTO ? PP_LOWRANK_REFERENCE : M->opts.lowrank;
+        PP_CHECK(form == PP_LOWRANK_REFERENCE || form == PP_LOWRANK_DEFLATION || form == PP_LOWRANK_SUBSPACE,
+                 PP_ERR_INVALID, "unknown low-rank form");
+        PP_CHECK(spd || form != PP_LOWRANK_DEFLATION, PP_ERR_UNSUPPORTED,
+                 "the eigenvalue-deflation low-rank form is AMSED's (symmetric) only");
+        S->lowrank = form;
         for (int l = (int)nlev - 2; l >= 0; --l) {
-            if (spd) lowrank_lanczos(ctx, *S, l, rank, steps);
-            else lowrank_arnoldi(ctx, *S, l, rank, steps);
+            if (spd) {
+                if (form == PP_LOWRANK_SUBSPACE) lowrank_lanczos(ctx, *S, l, rank, steps);
+                else
+                    lowrank_lanczos_restarted(ctx, *S, l, rank, form, M->opts.eps,
+                                              M->opts.lanczos_steps ? M->opts.lanczos_steps : kAmsedMsteps);
+            } else {
+                lowrank_arnoldi(ctx, *S, l, rank, steps, form == PP_LOWRANK_REFERENCE);
+            }
             MlLevel &L = S->lev[l];
             M->mnnz += 2 * L.rank * L.nrest + L.rank;  // AMSEDPrecondition.cpp:1269 convention
         }

@@ -601,6 +601,276 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload, Host
     if (perm_out && Ap == &Aperm) *perm_out = std::move(Aperm);
 }
 
+// ---------------------------------------------------------------------------
+// Numeric re-factorization on the device (pp_precond_refactor)
+// ---------------------------------------------------------------------------
+void DevRefactor::free_all() {
+    for (uint32_t **p : {&rp, &ci, &src, &dsrc, &rows, &lo_map, &lo_dmap, &up_map, &up_dmap, &aperm_map}) {
+        dfree(*p);
+        *p = nullptr;
+    }
+    dfree(val);
+    val = nullptr;
+    dfree(fail);
+    fail = nullptr;
+}
+
+template <class T>
+static T *upload_vec(const std::vector<T> &v) {
+    T *d = dalloc<T>(std::max<size_t>(v.size(), 1));
+    if (!v.empty()) PP_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+// The value slots of a sweep built from `s` (val = the factor entry index,
+// diag = the pivot index, both as exact doubles) as index maps: the same
+// upload the real sweep went through, so every slot lines up with the real
+// sweep's storage.
+static void sweep_maps(pp_ctx *ctx, const HostSweep &s, uint32_t *&map, uint32_t &nmap, uint32_t *&dmap,
+                       bool &has_diag) {
+    DevSweep<double> tmp;
+    try {
+        upload_sweep(s, tmp);
+        nmap = tmp.nnz;
+        map = dalloc<uint32_t>(std::max<uint32_t>(nmap, 1));
+        launch_to_index(ctx, nmap, tmp.val, map);
+        has_diag = tmp.diag != nullptr;
+        if (has_diag) {
+            dmap = dalloc<uint32_t>(s.n);
+            launch_to_index(ctx, s.n, tmp.diag, dmap);
+        }
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        tmp.free_all();
+        throw;
+    }
+    tmp.free_all();
+}
+
+// Dependency levels of the strict lower pattern (rp, ci): rows in level order.
+static void lower_levels(uint32_t n, const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
+                         std::vector<uint32_t> &rows, std::vector<uint32_t> &lptr) {
+    std::vector<uint32_t> lev(n, 0);
+    uint32_t nlev = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t l = 0;
+        for (uint32_t q = rp[i]; q < rp[i + 1]; ++q)
+            if (ci[q] < i) l = std::max(l, lev[ci[q]] + 1);
+        lev[i] = l;
+        nlev = std::max(nlev, l + 1);
+    }
+    lptr.assign(nlev + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) lptr[lev[i] + 1]++;
+    for (uint32_t l = 0; l < nlev; ++l) lptr[l + 1] += lptr[l];
+    rows.resize(n);
+    std::vector<uint32_t> f(lptr.begin(), lptr.end() - 1);
+    for (uint32_t i = 0; i < n; ++i) rows[f[lev[i]]++] = i;
+}
+
+// Builds the symbolic state from the preconditioner's host structures (the
+// pattern the last host setup factored).  False when this factor cannot be
+// re-factored on the device (unsorted or repeated columns, a missing
+// diagonal, the one-workgroup schedule).
+static bool devref_build(pp_ctx *ctx, pp_precond *M) {
+    const int kind = M->opts.kind;
+    if ((kind != PP_PC_IC0 && kind != PP_PC_ILU0) || M->trsv == PP_TRSV_BLOCK || M->multilevel) return false;
+    const bool ordered = !M->order.empty();
+    if (ordered && !M->Aperm) return false;
+    const pp_csr *F = ordered ? M->Aperm : M->A;  // the matrix the factor was computed from
+    const uint32_t n = F->nrow;
+    const std::vector<uint32_t> &frp = F->h_rp, &fci = F->h_ci;
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = frp[i] + 1; k < frp[i + 1]; ++k)
+            if (!(fci[k - 1] < fci[k])) return false;  // the device rows merge sorted column lists
+    std::vector<uint32_t> diagpos(n, UINT32_MAX);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t k = frp[i]; k < frp[i + 1]; ++k)
+            if (fci[k] == i) diagpos[i] = k;
+    for (uint32_t i = 0; i < n; ++i)
+        if (diagpos[i] == UINT32_MAX) return false;
+    std::unique_ptr<DevRefactor> R(new DevRefactor);
+    R->n = n;
+    std::vector<uint32_t> rows;
+    if (kind == PP_PC_IC0) {
+        // L's strict lower part in the order of the matrix's rows (factor_ic0)
+        std::vector<uint32_t> lrp(n + 1, 0), lci, lsrc;
+        for (uint32_t i = 0; i < n; ++i) {
+            for (uint32_t k = frp[i]; k < frp[i + 1]; ++k)
+                if (fci[k] < i) {
+                    lci.push_back(fci[k]);
+                    lsrc.push_back(k);
+                }
+            lrp[i + 1] = (uint32_t)lci.size();
+        }
+        const uint32_t nnz = (uint32_t)lci.size();
+        R->nnz = nnz;
+        lower_levels(n, lrp, lci, rows, R->lptr);
+        // the sweeps' value slots: forward sweep = L's rows (DIVIDE by l_ii),
+        // backward sweep = L^T's rows with the pivot first
+        if (M->hlow.rp != lrp || M->hlow.ci != lci) return false;  // not the sorted-rows factor layout
+        HostSweep lo = M->hlow;
+        for (uint32_t t = 0; t < nnz; ++t) lo.val[t] = (double)t;
+        for (uint32_t i = 0; i < n; ++i) lo.diag[i] = (double)(nnz + i);
+        HostCsr lt;
+        lt.n = n;
+        lt.rp = M->lt_rp;
+        lt.ci = M->lt_ci;
+        lt.v.resize(lt.ci.size());
+        for (uint32_t j = 0; j < n; ++j) {
+            PP_CHECK(lt.rp[j] < lt.rp[j + 1] && lt.ci[lt.rp[j]] == j, PP_ERR_INVALID, "L^T row without its pivot");
+            lt.v[lt.rp[j]] = (double)(nnz + j);
+            for (uint32_t q = lt.rp[j] + 1; q < lt.rp[j + 1]; ++q) {
+                const uint32_t i = lt.ci[q];
+                const auto b = lci.begin() + lrp[i], e = lci.begin() + lrp[i + 1];
+                const auto it = std::lower_bound(b, e, j);
+                PP_CHECK(it != e && *it == j, PP_ERR_INVALID, "L^T entry outside L's pattern");
+                lt.v[q] = (double)(it - lci.begin());
+            }
+        }
+        R->h_lt_map.resize(lt.v.size());
+        for (size_t q = 0; q < lt.v.size(); ++q) R->h_lt_map[q] = (uint32_t)lt.v[q];
+        HostSweep up = sweep_from_rows(lt, false, HostSweep::DIVIDE, true, nullptr);
+        sweep_maps(ctx, lo, R->lo_map, R->lo_nnz, R->lo_dmap, R->lo_diag);
+        sweep_maps(ctx, up, R->up_map, R->up_nnz, R->up_dmap, R->up_diag);
+        R->rp = upload_vec(lrp);
+        R->ci = upload_vec(lci);
+        R->src = upload_vec(lsrc);
+        R->dsrc = upload_vec(diagpos);
+    } else {
+        // ILU(0) on the matrix's own pattern: L strict lower (unit), U strict upper, D = 1/u_ii
+        const uint32_t nnz = (uint32_t)fci.size();
+        R->nnz = nnz;
+        lower_levels(n, frp, fci, rows, R->lptr);
+        HostCsr L, U;
+        L.n = U.n = n;
+        L.rp.assign(n + 1, 0);
+        U.rp.assign(n + 1, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            for (uint32_t k = frp[i]; k < frp[i + 1]; ++k) {
+                if (fci[k] < i) {
+                    L.ci.push_back(fci[k]);
+                    L.v.push_back((double)k);
+                } else if (fci[k] > i) {
+                    U.ci.push_back(fci[k]);
+                    U.v.push_back((double)k);
+                }
+            }
+            L.rp[i + 1] = (uint32_t)L.ci.size();
+            U.rp[i + 1] = (uint32_t)U.ci.size();
+        }
+        std::vector<double> dix(n);
+        for (uint32_t i = 0; i < n; ++i) dix[i] = (double)(nnz + i);
+        HostSweep lo = sweep_from_rows(L, true, HostSweep::UNIT, false, nullptr);
+        HostSweep up = sweep_from_rows(U, false, HostSweep::MULTIPLY, false, &dix);
+        R->h_l_map.assign(L.v.begin(), L.v.end());
+        R->h_u_map.assign(U.v.begin(), U.v.end());
+        if (lo.rp != M->hlow.rp || lo.ci != M->hlow.ci || up.rp != M->hup.rp || up.ci != M->hup.ci) return false;
+        sweep_maps(ctx, lo, R->lo_map, R->lo_nnz, R->lo_dmap, R->lo_diag);
+        sweep_maps(ctx, up, R->up_map, R->up_nnz, R->up_dmap, R->up_diag);
+        R->rp = upload_vec(frp);
+        R->ci = upload_vec(fci);
+        R->dsrc = upload_vec(diagpos);
+    }
+    PP_CHECK(R->lo_nnz == M->low64.nnz && R->up_nnz == M->up64.nnz, PP_ERR_INVALID, "sweep maps out of step");
+    R->rows = upload_vec(rows);
+    R->val = dalloc<double>((size_t)R->nnz + n);
+    R->fail = dalloc<int>(1);
+    if (ordered) {
+        // entry k of P A P^T <- entry aperm_map[k] of A: the permutation of the entry indices
+        HostCsr H;
+        H.n = M->A->nrow;
+        H.rp = M->A->h_rp;
+        H.ci = M->A->h_ci;
+        H.v.resize(H.ci.size());
+        for (size_t k = 0; k < H.v.size(); ++k) H.v[k] = (double)k;
+        const HostCsr P = permute_symmetric(H, M->order);
+        PP_CHECK(P.rp == frp && P.ci == fci, PP_ERR_INVALID, "colour-ordered pattern out of step");
+        std::vector<uint32_t> amap(P.v.size());
+        for (size_t k = 0; k < amap.size(); ++k) amap[k] = (uint32_t)P.v[k];
+        R->aperm_map = upload_vec(amap);
+        R->aperm_nnz = (uint32_t)amap.size();
+    }
+    delete M->dref;
+    M->dref = R.release();
+    return true;
+}
+
+// The FP32 sweep's values from the FP64 sweep's, into the arrays it has
+// (the same storage the first derivation allocated: cached graphs stay valid).
+static void refresh_sweep_f32(pp_ctx *ctx, const DevSweep<double> &s, DevSweep<float> &d) {
+    if (s.nnz) launch_convert(ctx, s.nnz, PP_F64, s.val, PP_F32, d.val);
+    for (size_t l = 0; l < s.ell.size() && l < d.ell.size(); ++l)
+        if (s.ell[l].ok && d.ell[l].ok)
+            launch_convert(ctx, s.ell[l].width * s.ell[l].nrows, PP_F64, s.ell[l].val, PP_F32, d.ell[l].val);
+    if (s.diag && d.diag) launch_convert(ctx, s.n, PP_F64, s.diag, PP_F32, d.diag);
+}
+
+static void refill_ell_levels(DevSweep<double> &d) {
+    for (uint32_t l = 0; l < d.nlev && l < d.ell.size(); ++l) {
+        auto &E = d.ell[l];
+        if (!E.ok) continue;
+        const uint32_t k0 = d.level_ptr[l];
+        ell_fill_rows<double>(E.nrows, E.width, d.rp + k0, d.val, E.val);
+    }
+}
+
+// Numeric IC(0) / ILU(0) of the matrix's current values on the device, into
+// the existing sweeps.  Throws the factorization's failure like the host.
+static void devref_run(pp_ctx *ctx, pp_precond *M) {
+    DevRefactor &R = *M->dref;
+    const bool ordered = !M->order.empty();
+    const double *av = M->A->v64;
+    if (ordered) {
+        pp_csr *P = M->Aperm;
+        launch_gather_map(ctx, R.aperm_nnz, R.aperm_map, M->A->v64, PP_F64, P->v64);
+        if (P->ell.ok) ell_fill_values(ctx, P->ell, PP_F64, P->rp, P->v64);
+        PP_HIP(hipMemcpyAsync(P->h_v.data(), P->v64, (size_t)R.aperm_nnz * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        P->f32_dirty = true;
+        ++P->version;
+        if (P->trans) {
+            csr_free(P->trans);
+            P->trans = nullptr;
+        }
+        M->Aperm_version = M->A->version;
+        av = P->v64;
+    }
+    const int big = INT32_MAX;
+    PP_HIP(hipMemcpyAsync(R.fail, &big, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    for (size_t l = 0; l + 1 < R.lptr.size(); ++l) {
+        const uint32_t r0 = R.lptr[l], nr = R.lptr[l + 1] - r0;
+        if (M->opts.kind == PP_PC_IC0)
+            launch_ic0_level(ctx, R.rows + r0, nr, R.rp, R.ci, R.src, R.dsrc, av, R.val, R.val + R.nnz, R.fail);
+        else
+            launch_ilu0_level(ctx, R.rows + r0, nr, R.rp, R.ci, R.dsrc, av, R.val, R.val + R.nnz, R.fail);
+    }
+    int fail = big;
+    PP_HIP(hipMemcpyAsync(&fail, R.fail, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    if (fail != big) {
+        M->fail_row = (uint32_t)fail;
+        M->ready = 0;
+        if (M->opts.kind == PP_PC_IC0)
+            throw Error(PP_ERR_NOT_SPD, "IC: matrix is not positive definite (pivot row " + std::to_string(fail) + ")");
+        throw Error(PP_ERR_ZERO_PIVOT, "ILU: zero pivot at row " + std::to_string(fail));
+    }
+    // the factor into the sweeps' level-ordered storage
+    launch_gather_map(ctx, R.lo_nnz, R.lo_map, R.val, PP_F64, M->low64.val);
+    if (R.lo_diag) launch_gather_map(ctx, R.n, R.lo_dmap, R.val, PP_F64, M->low64.diag);
+    launch_gather_map(ctx, R.up_nnz, R.up_map, R.val, PP_F64, M->up64.val);
+    if (R.up_diag) launch_gather_map(ctx, R.n, R.up_dmap, R.val, PP_F64, M->up64.diag);
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    refill_ell_levels(M->low64);
+    refill_ell_levels(M->up64);
+    if (M->have32) {
+        refresh_sweep_f32(ctx, M->low64, M->low32);
+        refresh_sweep_f32(ctx, M->up64, M->up32);
+    }
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    M->host_factor_stale = true;
+    ++M->device_refactors;
+}
+
 // Times each staged-sweep load variant on one precision's sweeps (cold
 // cache) and keeps the fastest for that precision.
 template <class T>
@@ -929,6 +1199,44 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
     });
 }
 
+int pp_precond_refactor(pp_ctx *ctx, pp_precond *M) {
+    if (!ctx) return PP_ERR_INVALID;
+    return guarded_pc(ctx, [&] {
+        PP_CHECK(M, PP_ERR_INVALID, "NULL preconditioner");
+        PP_CHECK(M->ready || M->dref, PP_ERR_NOT_READY, "refactor() before setup()");
+        auto t0 = std::chrono::steady_clock::now();
+        if (M->dref || devref_build(ctx, M)) {
+            devref_run(ctx, M);
+        } else {
+            // the host path: everything setup() builds, from the current values
+            M->ready = 0;
+            M->low64.free_all();
+            M->up64.free_all();
+            M->low32.free_all();
+            M->up32.free_all();
+            M->have32 = false;
+            dfree(M->dorder);
+            M->dorder = nullptr;
+            dfree(M->xp64);
+            M->xp64 = nullptr;
+            M->chunks.free_all();
+            M->clow64.free_all();
+            M->cup64.free_all();
+            M->clow32.free_all();
+            M->cup32.free_all();
+            if (M->multilevel) ml_free(M);
+            reduced_free(M);
+            M->uid = next_uid();  // graphs captured with the freed arrays are never replayed
+            const int st = pp_precond_setup(ctx, M);
+            if (st != PP_OK) throw Error(st, ctx->err);
+            if (M->opts.make_f32) precond_need_f32(M);
+            M->host_factor_stale = false;
+        }
+        M->ready = 1;
+        M->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
 int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
     if (!ctx) return PP_ERR_INVALID;
     return guarded_pc(ctx, [&] {
@@ -1079,6 +1387,7 @@ int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {
     info->fused_levels = M->fuse_mid ? 1 : 0;
     info->trsv = M->trsv;
     for (int k = 0; k < 4; ++k) info->phase_ms[k] = M->phase_ms[k];
+    info->device_refactors = M->device_refactors;
     return PP_OK;
 }
 
@@ -1103,9 +1412,34 @@ int pp_precond_set_sweep_variant(pp_precond *M, int variant) {
     return PP_OK;
 }
 
+// The host copy of a factor last computed on the device: its entries
+// downloaded and placed through the maps the device state keeps.
+static void refresh_host_factor(pp_precond *M) {
+    DevRefactor &R = *M->dref;
+    std::vector<double> v((size_t)R.nnz + R.n);
+    PP_HIP(hipMemcpy(v.data(), R.val, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (M->opts.kind == PP_PC_IC0) {
+        for (size_t q = 0; q < M->lt_v.size(); ++q) M->lt_v[q] = v[R.h_lt_map[q]];
+    } else {
+        for (size_t q = 0; q < M->l_v.size(); ++q) M->l_v[q] = v[R.h_l_map[q]];
+        for (size_t q = 0; q < M->u_v.size(); ++q) M->u_v[q] = v[R.h_u_map[q]];
+        M->d.assign(v.begin() + R.nnz, v.end());
+    }
+    M->host_factor_stale = false;
+}
+
 int pp_precond_factor(const pp_precond *M, int which, uint32_t *nnz, uint32_t *rowptr, uint32_t *colidx,
                       double *values, double *d) {
     if (!M || !nnz) return PP_ERR_INVALID;
+    if (M->host_factor_stale && M->dref) {
+        try {
+            PP_HIP(hipSetDevice(M->ctx->device));
+            refresh_host_factor(const_cast<pp_precond *>(M));
+        } catch (const Error &e) {
+            set_host_error(e.what());
+            return e.status;
+        }
+    }
     const std::vector<uint32_t> *rp, *ci;
     const std::vector<double> *v;
     if (which == 0) { rp = &M->lt_rp; ci = &M->lt_ci; v = &M->lt_v; }
@@ -1141,6 +1475,7 @@ void pp_precond_destroy(pp_precond *M) {
     M->up64.free_all();
     M->low32.free_all();
     M->up32.free_all();
+    delete M->dref;
     if (M->multilevel) ml_free(M);
     delete M;
 }

@@ -170,17 +170,19 @@ def test_amsed_full_rank_is_exact(pp, torch):
     np.testing.assert_allclose(y.cpu().numpy(), x, rtol=0, atol=1e-6)
 
 
-@pytest.mark.parametrize("rank", [0, 16])
-def test_amsed_pcg_config4_type(pp, torch, rank):
+@pytest.mark.parametrize("rank,lowrank", [(0, "reference"), (16, "reference"), (16, "deflation"), (16, "subspace")])
+def test_amsed_pcg_config4_type(pp, torch, rank, lowrank):
+    # the reference's restarted Lanczos with its basic (mu / (1 - mu)) and
+    # eigenvalue-deflation ((eps - lam) / lam) cores, and this build's one-run form
     A = ppo.laplacian(32, 32, 32)
     n = len(A[0]) - 1
     b = ppo.spmv(A, ppo.normal(n, 0))
     M = pp.CSRMatrix(*A)
-    P = pp.AMSED(M, levels=3, ndom=16, rank=rank)
+    P = pp.AMSED(M, levels=3, ndom=16, rank=rank, lowrank=lowrank)
     P.setup()
     nlev, sizes, ranks, _ = _ml_info(pp, P)
     print(f"[amsed] sizes {sizes} ranks {ranks}", flush=True)
-    if rank:
+    if rank and lowrank != "deflation":  # the deflated core keeps only the Ritz values above 1 - eps
         assert all(0 < r <= rank for r in ranks[:-1])
     cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=1000)
     x = torch.zeros(n, dtype=torch.float64, device="cuda")
@@ -188,7 +190,7 @@ def test_amsed_pcg_config4_type(pp, torch, rank):
     assert cg.getConvergence() and cg.report.true_relres < 1e-8
     o = ppo.cg(A, b, None, tol=1e-12, maxit=5000)
     assert np.linalg.norm(x.cpu().numpy() - o["x"]) / np.linalg.norm(o["x"]) <= 1e-6
-    print(f"[amsed] rank {rank}: {cg.getNumIter()} iterations", flush=True)
+    print(f"[amsed] rank {rank} {lowrank}: {cg.getNumIter()} iterations", flush=True)
 
 
 def scaled(A, seed):
@@ -233,7 +235,9 @@ def test_gmslr_full_rank_is_exact(pp, torch, levels, ndom):
     A = scaled(ppo.vardiff2d(side, kap), 6)
     n = len(A[0]) - 1
     M = pp.CSRMatrix(*A)
-    P = pp.GMSLR(M, levels=levels, ndom=ndom, rank=64, lanczos_steps=64, exact=True)
+    # (the exact-subspace core; the reference's diagonal core on Schur vectors
+    # is exact only when the Schur form is diagonal)
+    P = pp.GMSLR(M, levels=levels, ndom=ndom, rank=64, lanczos_steps=64, exact=True, lowrank="subspace")
     P.setup()
     nlev, sizes, ranks, _ = _ml_info(pp, P)
     print(f"[gmslr] sizes {sizes} ranks {ranks}", flush=True)
@@ -244,13 +248,13 @@ def test_gmslr_full_rank_is_exact(pp, torch, levels, ndom):
     np.testing.assert_allclose(y.cpu().numpy(), x, rtol=0, atol=1e-7)
 
 
-@pytest.mark.parametrize("rank", [0, 16])
-def test_gmslr_gmres_upwind(pp, torch, rank):
+@pytest.mark.parametrize("rank,lowrank", [(0, "reference"), (16, "reference"), (16, "subspace")])
+def test_gmslr_gmres_upwind(pp, torch, rank, lowrank):
     A = ppo.upwind(24, 24, 24, 40.0, 20.0, 10.0)
     n = len(A[0]) - 1
     b = ppo.spmv(A, ppo.normal(n, 1))
     M = pp.CSRMatrix(*A)
-    P = pp.GMSLR(M, levels=3, ndom=16, rank=rank)
+    P = pp.GMSLR(M, levels=3, ndom=16, rank=rank, lowrank=lowrank)
     P.setup()
     nlev, sizes, ranks, _ = _ml_info(pp, P)
     print(f"[gmslr] sizes {sizes} ranks {ranks}", flush=True)
@@ -259,7 +263,7 @@ def test_gmslr_gmres_upwind(pp, torch, rank):
     x = torch.zeros(n, dtype=torch.float64, device="cuda")
     g.solve(M, torch.tensor(b, device="cuda"), x)
     assert g.getConvergence() and g.report.true_relres < 1e-7
-    print(f"[gmslr] rank {rank}: {g.getNumIter()} iterations", flush=True)
+    print(f"[gmslr] rank {rank} {lowrank}: {g.getNumIter()} iterations", flush=True)
 
 
 @pytest.mark.parametrize("kind", ["amsed", "gmslr"])

@@ -1,0 +1,149 @@
+"""Numeric re-factorization on the device (pp_precond_refactor): IC(0) and
+ILU(0), natural and multicolour order, every sweep schedule it covers.
+
+Bars: after new values on the same pattern, the re-factored preconditioner's
+apply equals a preconditioner set up from scratch on those values bit for bit
+(the device factorization runs the host factorization's operation order), its
+host factor copy equals the fresh one, solves through it match; a failing
+pivot is reported with the fresh setup's row; the original values restore
+the original apply."""
+import numpy as np
+import pytest
+
+from oracle import ppo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def pp(torch):
+    import prispack_amd
+    return prispack_amd
+
+
+def make(pp, M, kind, ordering, trsv):
+    cls = pp.IC0 if kind == "ic0" else pp.ILU0
+    P = cls(M, ordering=ordering, trsv=trsv, make_f32=True)
+    P.setup()
+    return P
+
+
+def apply(torch, P, x, dtype):
+    t = torch.tensor(x, dtype=dtype, device="cuda")
+    P.MInvSolve(t)
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("trsv", [1, 2, 3])
+@pytest.mark.parametrize("ordering", [0, 1])
+@pytest.mark.parametrize("kind", ["ic0", "ilu0"])
+def test_refactor_matches_fresh_setup(pp, torch, kind, ordering, trsv):
+    A = ppo.laplacian(14, 13, 12) if kind == "ic0" else ppo.upwind(14, 13, 12, 20.0, 10.0, 5.0)
+    n = len(A[0]) - 1
+    rows = np.repeat(np.arange(n), np.diff(A[0]))
+    v1 = np.asarray(A[2], np.float64)
+    rng = np.random.default_rng(7)
+    v2 = v1 * (1.0 + 0.2 * rng.uniform(-1, 1, len(v1)))
+    if kind == "ic0":  # keep it symmetric positive definite
+        D = np.zeros((n, n))
+        D[rows, A[1]] = v2
+        D = 0.5 * (D + D.T)
+        v2 = D[rows, A[1]]
+    v2[A[1] == rows] += 0.5
+    M = pp.CSRMatrix(*A)
+    P = make(pp, M, kind, ordering, trsv)
+    x = ppo.uniform(n, 3)
+    y1 = apply(torch, P, x, torch.float64)
+    M.set_values(v2)
+    P.refactor()
+    # the one-workgroup schedule (3) re-runs the whole setup on the host
+    assert P.info().device_refactors == (0 if trsv == 3 else 1)
+    M2 = pp.CSRMatrix(A[0], A[1], v2)
+    F = make(pp, M2, kind, ordering, trsv)
+    print(f"[refactor] {kind} ordering {ordering} trsv {trsv}: levels {P.info().levels_lower}", flush=True)
+    for dt in (torch.float64, torch.float32):
+        np.testing.assert_array_equal(apply(torch, P, x, dt), apply(torch, F, x, dt))
+    for which in ((0,) if kind == "ic0" else (1, 2)):
+        a, b = P.factor(which), F.factor(which)
+        for u, w in zip(a, b):
+            np.testing.assert_array_equal(u, w)
+    # solves through the re-factored preconditioner
+    b = ppo.spmv((A[0], A[1], v2), ppo.normal(n, 1))
+    xs = []
+    for Pc, Mc in ((P, M), (F, M2)):
+        if kind == "ic0":
+            s = pp.CG(n, Pc, pp.FlexibleCG, tol=1e-10, maxIter=2000)
+        else:
+            s = pp.GMRES(n, Pc, 1e-10, pp.FlexibleGMRES)
+            s.setRestart(True, 30)
+        xx = torch.zeros(n, dtype=torch.float64, device="cuda")
+        s.solve(Mc, torch.tensor(b, device="cuda"), xx)
+        assert s.getConvergence()
+        xs.append((s.getNumIter(), xx.cpu().numpy()))
+    assert xs[0][0] == xs[1][0]
+    np.testing.assert_array_equal(xs[0][1], xs[1][1])
+    # back to the first values: the first apply again
+    M.set_values(v1)
+    P.refactor()
+    np.testing.assert_array_equal(apply(torch, P, x, torch.float64), y1)
+
+
+def test_refactor_reports_a_failing_pivot(pp, torch):
+    A = ppo.laplacian(10, 9, 8)
+    n = len(A[0]) - 1
+    rows = np.repeat(np.arange(n), np.diff(A[0]))
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M, ordering=1)
+    P.setup()
+    bad = np.asarray(A[2], np.float64).copy()
+    bad[(A[1] == rows) & (rows == 123)] = -1.0  # an indefinite diagonal
+    M.set_values(bad)
+    with pytest.raises(pp.PrisPackError) as e:
+        P.refactor()
+    assert e.value.status == 3  # PP_ERR_NOT_SPD
+    F = pp.IC0(pp.CSRMatrix(A[0], A[1], bad), ordering=1)
+    with pytest.raises(pp.PrisPackError):
+        F.setup()
+    assert P.info().fail_row == F.info().fail_row and not P.isReady()
+    M.set_values(np.asarray(A[2], np.float64))
+    P.refactor()
+    assert P.isReady()
+
+
+def test_refactor_at_128_cubed(pp, torch):
+    # the 128^3 workload: new values re-factored on the device, then the mixed
+    # and FP64 solves meet the tolerance; the device path is what ran
+    import time
+    g = 128
+    A = ppo.laplacian(g, g, g)
+    n = len(A[0]) - 1
+    M = pp.CSRMatrix(*A)
+    P = pp.IC0(M, ordering=1, make_f32=True)
+    P.setup()
+    v2 = np.asarray(A[2], np.float64) * 1.5
+    M.set_values(v2)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    P.refactor()
+    first = time.perf_counter() - t
+    M.set_values(np.asarray(A[2], np.float64) * 2.0)
+    t = time.perf_counter()
+    P.refactor()
+    second = time.perf_counter() - t
+    print(f"[refactor] 128^3 IC(0): first refactor {first * 1e3:.1f} ms (maps built), "
+          f"second {second * 1e3:.1f} ms; setup {P.info().setup_ms:.1f} ms", flush=True)
+    assert P.info().device_refactors == 2
+    b = ppo.spmv((A[0], A[1], np.asarray(A[2]) * 2.0), ppo.normal(n, 0))
+    for prec in ("fp64", "mixed"):
+        cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision=prec)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        cg.solve(M, torch.tensor(b, device="cuda"), x)
+        assert cg.getConvergence() and cg.report.true_relres <= 1e-8, prec

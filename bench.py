@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary records (128^2 plain FP64, 100^3 upwind GMRES(30), 64^3 AMSED/GMSLR, "
                          "256-system batch)")
+    ap.add_argument("--record", default="",
+                    help="only run these secondary records (comma-separated: plain_fp64_2d_128, upwind_gmres_100, "
+                         "amsed_64, gmslr_64, batch_256), print them as one JSON line and exit")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--trsv", type=int, default=0, help="SpTRSV schedule: 0 auto, 1 levels, 2 sync-free")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -253,11 +256,14 @@ def extra_records(torch, pp, args):
                 "iters_mean": float(its.mean()), "relres": B.report.true_relres,
                 "converged": bool(B.report.converged)}
 
-    guard("plain_fp64_2d_128", plain_2d)
-    guard("upwind_gmres_100", upwind_gmres)
-    guard("amsed_64", lambda: multilevel("amsed"))
-    guard("gmslr_64", lambda: multilevel("gmslr"))
-    guard("batch_256", batch)
+    records = {"plain_fp64_2d_128": plain_2d, "upwind_gmres_100": upwind_gmres,
+               "amsed_64": lambda: multilevel("amsed"), "gmslr_64": lambda: multilevel("gmslr"),
+               "batch_256": batch}
+    only = args.record.split(",") if args.record else list(records)
+    for name in only:
+        if name not in records:
+            raise SystemExit(f"unknown record {name!r}: one of {', '.join(records)}")
+        guard(name, records[name])
     return out
 
 
@@ -286,6 +292,10 @@ def main():
 
     import prispack_amd as pp
     from prispack_amd import _capi
+
+    if args.record:
+        print(json.dumps(extra_records(torch, pp, args)), flush=True)
+        return
 
     g = args.grid
     A = pp.generateLaplacianCSR(g, g, g)
@@ -449,7 +459,7 @@ def main():
     alg_apply = 2 * (12 * nnz_l + 4 * (n + 1) + 16 * n)
     alg_apply32 = 2 * (8 * nnz_l + 4 * (n + 1) + 8 * n)
     traffic = measured_traffic(g, spmv_var.value)
-    spmv_kernel = ("k_ell<double> (y = A x, FP64 sliced-column form with 16-bit column offsets"
+    spmv_kernel = ("k_ell2<double> (y = A x, FP64 sliced-column form with 16-bit column offsets, two rows per thread"
                    if spmv_var.value == ELL_VARIANT else "k_spmv<double> (y = A x, FP64 CSR")
     dominant = {"kernel": spmv_kernel + ", launches alternating between two copies of A, x, y on the library stream)",
                 "bound": "hbm", "achieved": round(gbs(alg_spmv, spmv_ms), 1), "peak": HBM_PEAK_GBS,

@@ -277,6 +277,28 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
     }
 }
 
+// Entry-slot count of the sliced-column kernels.  W > 0: exactly W slots,
+// known at compile time, so every slot's loads are issued unconditionally
+// and back to back; W < 0: the runtime width, at most -W slots, each behind
+// a test (measured 22% slower on the 128^3 7-point SpMV, 43.1 against
+// 35.3 us).  ell_width_dispatch picks the exact form for the widths of the
+// 7- and 5-point stencils (natural and two-colour orders).
+template <int W>
+struct EllWidth {
+    static constexpr int kMax = W > 0 ? W : -W;
+    __device__ static inline bool has(int j, uint32_t width) { return W > 0 || j < (int)width; }
+};
+#define PP_ELL_WIDTHS(w, CALL)                 \
+    switch (w) {                               \
+        case 4: CALL(4); break;                \
+        case 5: CALL(5); break;                \
+        case 6: CALL(6); break;                \
+        case 7: CALL(7); break;                \
+        default:                               \
+            if ((w) <= 8) CALL(-8);            \
+            else CALL(-16);                    \
+    }
+
 // ---------------------------------------------------------------------------
 // Sliced-column SpMV (EllOp): one row per thread, entry j of all rows
 // contiguous, so every value and offset load is coalesced and no LDS stage
@@ -287,7 +309,7 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
 // may be one vector when the rows write where no row reads (the reduced
 // CG's KA: first colour from second).  DOT: the same sums as k_spmv.
 // ---------------------------------------------------------------------------
-template <class T, bool DOT, int MAXW>
+template <class T, bool DOT, int W>
 __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const int16_t *__restrict__ off,
                                              const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
                                              int64_t dbase, const T *x, T *y, T alpha, T beta, int beta_zero,
@@ -301,11 +323,12 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const i
     for (uint32_t b = xr.first; b < xr.end; b += xr.step) {
         const uint32_t i = b * kBlock + threadIdx.x;
         if (i < n) {
+            constexpr int MAXW = EllWidth<W>::kMax;
             int16_t o[MAXW];
             T v[MAXW], xv[MAXW];
 #pragma unroll
             for (int j = 0; j < MAXW; ++j)
-                if (j < (int)width) {
+                if (EllWidth<W>::has(j, width)) {
                     const size_t e = (size_t)j * n + i;
                     o[j] = off[e];
                     v[j] = val[e];
@@ -323,13 +346,13 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const i
             }
 #pragma unroll
             for (int j = 0; j < MAXW; ++j)
-                if (j < (int)width) xv[j] = x[(int64_t)i + cbase + o[j]];
+                if (EllWidth<W>::has(j, width)) xv[j] = x[(int64_t)i + cbase + o[j]];
             T sum = 0;
             {
 #pragma clang fp contract(off)
 #pragma unroll
                 for (int j = 0; j < MAXW; ++j)
-                    if (j < (int)width) sum += v[j] * xv[j];
+                    if (EllWidth<W>::has(j, width)) sum += v[j] * xv[j];
                 if (dg) sum += dgi * xd;
             }
             T out = alpha * sum;
@@ -363,7 +386,7 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const i
 // row's operation sequence is the one above, so y is the same bit for bit.
 // Needs an even row count and y aligned to two elements (checked by the
 // launcher).
-template <class T, int MAXW>
+template <class T, int W>
 __global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const int16_t *__restrict__ off,
                                               const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
                                               int64_t dbase, const T *x, T *y, T alpha, T beta, int beta_zero,
@@ -377,11 +400,12 @@ __global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const 
         const uint32_t t = b * kBlock + threadIdx.x;
         if (t >= npair) continue;
         const uint32_t i = 2 * t;
+        constexpr int MAXW = EllWidth<W>::kMax;
         int16_t o0[MAXW], o1[MAXW];
         T v0[MAXW], v1[MAXW], x0[MAXW], x1[MAXW];
 #pragma unroll
         for (int j = 0; j < MAXW; ++j)
-            if (j < (int)width) {
+            if (EllWidth<W>::has(j, width)) {
                 const size_t e = (size_t)j * n + i;
                 const uint32_t ow = *reinterpret_cast<const uint32_t *>(off + e);
                 o0[j] = (int16_t)(ow & 0xffffu);
@@ -404,7 +428,7 @@ __global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const 
         }
 #pragma unroll
         for (int j = 0; j < MAXW; ++j)
-            if (j < (int)width) {
+            if (EllWidth<W>::has(j, width)) {
                 x0[j] = x[(int64_t)i + cbase + o0[j]];
                 x1[j] = x[(int64_t)i + 1 + cbase + o1[j]];
             }
@@ -413,7 +437,7 @@ __global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const 
 #pragma clang fp contract(off)
 #pragma unroll
             for (int j = 0; j < MAXW; ++j)
-                if (j < (int)width) {
+                if (EllWidth<W>::has(j, width)) {
                     s0 += v0[j] * x0[j];
                     s1 += v1[j] * x1[j];
                 }
@@ -454,17 +478,21 @@ void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void
         const uint32_t nb2 = (E.n / 2 + kBlock - 1) / kBlock;
         const unsigned g2 = xcd_grid(nb2, 1u << 20);
         const int bz2 = beta == 0.0;
-#define PP_ELL2(T, W, VALS, DG)                                                                                  \
-    k_ell2<T, W><<<g2, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, VALS, E.cbase, DG, E.dbase, (const T *)x, \
-                                                  (T *)y, (T)alpha, (T)beta, bz2, ctx->stop_word)
+#define PP_ELL2_64(W)                                                                                     \
+    k_ell2<double, W><<<g2, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, E.v64, E.cbase, E.dg64, E.dbase, \
+                                                       (const double *)x, (double *)y, alpha, beta, bz2,     \
+                                                       ctx->stop_word)
+#define PP_ELL2_32(W)                                                                                   \
+    k_ell2<float, W><<<g2, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, E.v32, E.cbase, E.dg32, E.dbase, \
+                                                      (const float *)x, (float *)y, (float)alpha, (float)beta, \
+                                                      bz2, ctx->stop_word)
         if (dtype == PP_F64) {
-            if (E.width <= 8) PP_ELL2(double, 8, E.v64, E.dg64);
-            else PP_ELL2(double, 16, E.v64, E.dg64);
+            PP_ELL_WIDTHS(E.width, PP_ELL2_64)
         } else {
-            if (E.width <= 8) PP_ELL2(float, 8, E.v32, E.dg32);
-            else PP_ELL2(float, 16, E.v32, E.dg32);
+            PP_ELL_WIDTHS(E.width, PP_ELL2_32)
         }
-#undef PP_ELL2
+#undef PP_ELL2_64
+#undef PP_ELL2_32
         PP_HIP(hipGetLastError());
         return;
     }
@@ -477,20 +505,27 @@ void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void
                                                        (const T *)x, (T *)y, (T)alpha, (T)beta, bz,           \
                                                        (const T *)dotw, (const T *)aux, (const T *)wgt, part, \
                                                        ctx->stop_word)
-#define PP_ELL_W(T, VALS, DG)                                    \
-    if (E.width <= 8) {                                          \
-        if (dot) PP_ELL(T, true, 8, VALS, DG);                   \
-        else PP_ELL(T, false, 8, VALS, DG);                      \
-    } else {                                                     \
-        if (dot) PP_ELL(T, true, 16, VALS, DG);                  \
-        else PP_ELL(T, false, 16, VALS, DG);                     \
-    }
+#define PP_ELL_64D(W) PP_ELL(double, true, W, E.v64, E.dg64)
+#define PP_ELL_64N(W) PP_ELL(double, false, W, E.v64, E.dg64)
+#define PP_ELL_32D(W) PP_ELL(float, true, W, E.v32, E.dg32)
+#define PP_ELL_32N(W) PP_ELL(float, false, W, E.v32, E.dg32)
     if (dtype == PP_F64) {
-        PP_ELL_W(double, E.v64, E.dg64)
+        if (dot) {
+            PP_ELL_WIDTHS(E.width, PP_ELL_64D)
+        } else {
+            PP_ELL_WIDTHS(E.width, PP_ELL_64N)
+        }
     } else {
-        PP_ELL_W(float, E.v32, E.dg32)
+        if (dot) {
+            PP_ELL_WIDTHS(E.width, PP_ELL_32D)
+        } else {
+            PP_ELL_WIDTHS(E.width, PP_ELL_32N)
+        }
     }
-#undef PP_ELL_W
+#undef PP_ELL_64D
+#undef PP_ELL_64N
+#undef PP_ELL_32D
+#undef PP_ELL_32N
 #undef PP_ELL
     PP_HIP(hipGetLastError());
 }
@@ -911,7 +946,7 @@ __global__ __launch_bounds__(256) void k_sweep_stream(const uint32_t *__restrict
 // pre-division, the products subtracted in storage order each rounded, the
 // row's diagonal mode, the next sweep's first-level mode), so the results
 // are the same bit for bit; padding entries subtract a zero.
-template <class T, int MAXW>
+template <class T, int W>
 __global__ __launch_bounds__(256) void k_sweep_ell(uint32_t nrows, uint32_t width, const int16_t *__restrict__ off,
                                                    const T *__restrict__ val, long long row0, long long cbase,
                                                    const T *__restrict__ diag, int mode, const T *b, T *x,
@@ -921,11 +956,12 @@ __global__ __launch_bounds__(256) void k_sweep_ell(uint32_t nrows, uint32_t widt
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nrows) return;
     const long long i = row0 + t;
+    constexpr int MAXW = EllWidth<W>::kMax;
     int16_t o[MAXW];
     T v[MAXW], xv[MAXW];
 #pragma unroll
     for (int j = 0; j < MAXW; ++j)
-        if (j < (int)width) {
+        if (EllWidth<W>::has(j, width)) {
             const size_t e = (size_t)j * nrows + t;
             o[j] = off[e];
             v[j] = val[e];
@@ -935,12 +971,12 @@ __global__ __launch_bounds__(256) void k_sweep_ell(uint32_t nrows, uint32_t widt
     const T pd2 = (mode2 == 0 || mode2 == 2 || mode2 == 3) ? diag2[i] : (T)0;
 #pragma unroll
     for (int j = 0; j < MAXW; ++j)
-        if (j < (int)width) xv[j] = x[i + cbase + o[j]];
+        if (EllWidth<W>::has(j, width)) xv[j] = x[i + cbase + o[j]];
     T s = pb;
     if (mode == 3) s = s / pd;
 #pragma unroll
     for (int j = 0; j < MAXW; ++j)
-        if (j < (int)width) {
+        if (EllWidth<W>::has(j, width)) {
             const T p = v[j] * xv[j];
             s = s - p;
         }
@@ -968,12 +1004,11 @@ void launch_levels(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x
         if (ell && l < s.ell.size() && s.ell[l].ok) {
             const auto &E = s.ell[l];
             const unsigned g = (E.nrows + kBlock - 1) / kBlock;
-            if (E.width <= 8)
-                k_sweep_ell<T, 8><<<g, kBlock, 0, ctx->stream>>>(E.nrows, E.width, E.off, E.val, E.row0, E.cbase,
-                                                                 s.diag, mode, b, x, stop, mode2, diag2);
-            else
-                k_sweep_ell<T, 16><<<g, kBlock, 0, ctx->stream>>>(E.nrows, E.width, E.off, E.val, E.row0, E.cbase,
-                                                                  s.diag, mode, b, x, stop, mode2, diag2);
+#define PP_SWEEP_ELL(W)                                                                                     \
+    k_sweep_ell<T, W><<<g, kBlock, 0, ctx->stream>>>(E.nrows, E.width, E.off, E.val, E.row0, E.cbase, s.diag, \
+                                                     mode, b, x, stop, mode2, diag2)
+            PP_ELL_WIDTHS(E.width, PP_SWEEP_ELL)
+#undef PP_SWEEP_ELL
             continue;
         }
         if (s.blocks && 2 * l + 1 < s.level_blk.size() && s.level_blk[2 * l + 1] > s.level_blk[2 * l]) {

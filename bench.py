@@ -63,7 +63,7 @@ def parse():
 # (--copy-probe).  Keyed by (grid side, SpMV storage variant); the command and
 # the raw per-dispatch figures are in profiles/k_spmv_poisson128_pmc.md.
 PMC_TRAFFIC = {(128, v): b for v, b in enumerate((224627870, 226711825, 194751552, 197276754, 219112328, 229056082,
-                                                        189937730, 192875293, 183261472))}
+                                                        189937730, 192875293, 154991311))}
 
 
 ELL_VARIANT = 8  # the SpMV variant that runs the sliced-column form
@@ -216,9 +216,11 @@ def extra_records(torch, pp, args):
         P.setup()
         setup = time.perf_counter() - t0
         sizes, ranks = P.info_levels()
+        info = P.info()
         rec = {"workload": f"{kind.upper()} 3 levels, 16 subdomains, rank 32, 64^3 "
                            f"{'Poisson, FlexibleCG' if kind == 'amsed' else 'upwind (20,10,5)h, FGMRES(30)'}",
-               "setup_s": round(setup, 3), "level_sizes": sizes, "ranks": ranks}
+               "setup_s": round(setup, 3), "level_sizes": sizes, "ranks": ranks,
+               "block_factor": {"levels": [info.levels_lower, info.levels_upper], "trsv": info.trsv}}
         for prec in ("fp64", "mixed"):
             def run():
                 if kind == "amsed":
@@ -459,7 +461,8 @@ def main():
     alg_apply = 2 * (12 * nnz_l + 4 * (n + 1) + 16 * n)
     alg_apply32 = 2 * (8 * nnz_l + 4 * (n + 1) + 8 * n)
     traffic = measured_traffic(g, spmv_var.value)
-    spmv_kernel = ("k_ell2<double> (y = A x, FP64 sliced-column form with 16-bit column offsets, two rows per thread"
+    spmv_kernel = ("k_ell2<double, 7, true> (y = A x, FP64 sliced-column form, offsets from one pattern byte per "
+                   "row, two rows per thread"
                    if spmv_var.value == ELL_VARIANT else "k_spmv<double> (y = A x, FP64 CSR")
     dominant = {"kernel": spmv_kernel + ", launches alternating between two copies of A, x, y on the library stream)",
                 "bound": "hbm", "achieved": round(gbs(alg_spmv, spmv_ms), 1), "peak": HBM_PEAK_GBS,

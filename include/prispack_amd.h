@@ -146,10 +146,14 @@ enum {
     PP_TRSV_AUTO = 0,
     PP_TRSV_LEVELS = 1,    /* one launch per level (graph-captured)               */
     PP_TRSV_SYNCFREE = 2,  /* one launch per sweep, per-row ready flags            */
-    PP_TRSV_BLOCK = 3      /* both sweeps in one workgroup, the vector in LDS, a
+    PP_TRSV_BLOCK = 3,     /* both sweeps in one workgroup, the vector in LDS, a
                               barrier per level (n * 8 bytes must fit in LDS;
                               larger systems use LEVELS).  AUTO times it against
                               LEVELS at setup when it fits                         */
+    PP_TRSV_PIPE = 4       /* BLOCK's storage; one wavefront computes the levels
+                              with no barrier while the workgroup's other waves
+                              stage the level data in LDS ahead of it (for many
+                              narrow levels).  AUTO times it with the others      */
 };
 
 /* Row ordering of the factor (no reference counterpart; the reference factors
@@ -207,7 +211,8 @@ typedef struct {
     uint32_t annz;          /* getOriginalNonZeroCount() */
     uint32_t mnnz;          /* getPreconditionNonZeroCount() */
     double fill_ratio;      /* getPreconditionFillinRatio() */
-    uint32_t levels_lower;  /* SpTRSV dependency depth */
+    uint32_t levels_lower;  /* SpTRSV dependency depth (AMSED / GMSLR: of the first
+                               level's subdomain-block factor, its schedule in trsv) */
     uint32_t levels_upper;
     double setup_ms;        /* host factorization + analysis + upload */
     uint32_t fail_row;      /* row of a failed pivot */
@@ -228,7 +233,7 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M);
 /* Numeric re-factorization from the matrix's current values (pp_csr_set_values
  * keeps the pattern).  No reference counterpart: the reference's setup() does
  * nothing once ready (IncompleteCholesky.cpp:105) and a new object has to be
- * made.  IC(0) and ILU(0) (any ordering, any schedule but PP_TRSV_BLOCK) are
+ * made.  IC(0) and ILU(0) (any ordering, any schedule but PP_TRSV_BLOCK/PIPE) are
  * factored on the device: one launch per dependency level, one thread per
  * row, the host factorization's operation order (the factor is the same bit
  * for bit), written into the existing sweeps (graphs captured with them stay

@@ -15,7 +15,7 @@ PP_OK, PP_ERR_INVALID, PP_ERR_HIP, PP_ERR_NOT_SPD, PP_ERR_ZERO_PIVOT, PP_ERR_ZER
 PP_ERR_NOT_READY, PP_ERR_BREAKDOWN, PP_ERR_NOT_FINITE, PP_ERR_NOMEM, PP_ERR_UNSUPPORTED = 6, 7, 8, 9, 10
 PP_F64, PP_F32 = 0, 1
 PP_PC_NONE, PP_PC_ICT, PP_PC_ILUT, PP_PC_ILDLT, PP_PC_IC0, PP_PC_ILU0, PP_PC_AMSED, PP_PC_GMSLR = range(8)
-PP_TRSV_AUTO, PP_TRSV_LEVELS, PP_TRSV_SYNCFREE, PP_TRSV_BLOCK = range(4)
+PP_TRSV_AUTO, PP_TRSV_LEVELS, PP_TRSV_SYNCFREE, PP_TRSV_BLOCK, PP_TRSV_PIPE = range(5)
 PP_SOLVER_CG, PP_SOLVER_GMRES = 0, 1
 PP_FP64, PP_MIXED = 0, 1
 PP_LAUNCH_GRAPH, PP_LAUNCH_STREAM = 0, 1

@@ -583,8 +583,11 @@ int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes) {
     if (!A || !bytes) return PP_ERR_INVALID;
     const uint64_t esz = dtype == PP_F64 ? 8 : 4;
     if ((dtype == PP_F64 ? A->spmv_variant : A->spmv_variant32) == kVarEll) {
-        // padded values and 16-bit offsets, x read, y written
-        *bytes = (esz + 2) * A->ell.width * (uint64_t)A->ell.n + esz * ((uint64_t)A->ncol + A->nrow);
+        // padded values and 16-bit offsets (or one pattern byte per row and
+        // the pattern table), x read, y written
+        const uint64_t wn = A->ell.width * (uint64_t)A->ell.n;
+        const uint64_t idx = A->ell.pid ? A->ell.n + 16ull * A->ell.npat : 2 * wn;
+        *bytes = esz * wn + idx + esz * ((uint64_t)A->ncol + A->nrow);
         return PP_OK;
     }
     uint64_t b = esz * A->nnz;                                    // values

@@ -295,7 +295,10 @@ struct EllOp {
     bool ok = false;
     uint32_t n = 0, width = 0;
     int64_t cbase = 0, dbase = 0;
-    int16_t *off = nullptr;      // [width * n]
+    int16_t *off = nullptr;      // [width * n], or nullptr when the rows use patterns:
+    uint8_t *pid = nullptr;      // [n] the row's offset pattern (stencil-like matrices:
+    int16_t *ptab = nullptr;     //     at most 256 distinct offset rows) and [npat * 8] the patterns
+    uint32_t npat = 0;
     double *v64 = nullptr;       // [width * n]
     float *v32 = nullptr;
     double *dg64 = nullptr;      // [n] or nullptr
@@ -607,6 +610,14 @@ void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x)
 template <class T>
 void launch_sweep_pair_chunks(pp_ctx *ctx, const ChunkSet &C, const DevChunkSweep<T> &lo, int mlo,
                               const DevChunkSweep<T> &up, int mup, const T *x, T *y);
+// The same sweeps, one wavefront computing the levels while the rest of the
+// workgroup stages their data in LDS (pp_sweep_pipe.hip); sweep_pipe_rings:
+// the LDS ring sizes, false when the chunks leave too little LDS for them.
+template <class T>
+void launch_sweep_pair_pipe(pp_ctx *ctx, const ChunkSet &C, const DevChunkSweep<T> &lo, int mlo,
+                            const DevChunkSweep<T> &up, int mup, const T *x, T *y);
+template <class T>
+bool sweep_pipe_rings(pp_ctx *ctx, const ChunkSet &C, uint32_t &re, uint32_t &rr);
 template <class T>
 void launch_sweep_pair(pp_ctx *ctx, const DevSweep<T> &lo, int mlo, const DevSweep<T> &up, int mup, bool fuse,
                        const T *x, T *y, uint32_t first_level = 0);

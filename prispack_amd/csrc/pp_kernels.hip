@@ -283,11 +283,29 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
 // a test (measured 22% slower on the 128^3 7-point SpMV, 43.1 against
 // 35.3 us).  ell_width_dispatch picks the exact form for the widths of the
 // 7- and 5-point stencils (natural and two-colour orders).
+// The offsets of a row from its pattern (EllOp::pid / ptab): one 16-byte
+// table entry, eight 16-bit slots.
+template <int MAXW>
+__device__ inline void ell_pattern(const int16_t *__restrict__ ptab, uint32_t p, int16_t *o) {
+    const int4 t = *reinterpret_cast<const int4 *>(ptab + 8 * (size_t)p);
+    const uint32_t w[4] = {(uint32_t)t.x, (uint32_t)t.y, (uint32_t)t.z, (uint32_t)t.w};
+#pragma unroll
+    for (int j = 0; j < MAXW; ++j) o[j] = (int16_t)((w[j / 2] >> (16 * (j & 1))) & 0xffffu);
+}
+
 template <int W>
 struct EllWidth {
     static constexpr int kMax = W > 0 ? W : -W;
     __device__ static inline bool has(int j, uint32_t width) { return W > 0 || j < (int)width; }
 };
+// The same for matrices whose rows use offset patterns (widths 4 to 7 only).
+#define PP_ELL_PAT_WIDTHS(w, CALL)             \
+    switch (w) {                               \
+        case 4: CALL(4); break;                \
+        case 5: CALL(5); break;                \
+        case 6: CALL(6); break;                \
+        default: CALL(7); break;               \
+    }
 #define PP_ELL_WIDTHS(w, CALL)                 \
     switch (w) {                               \
         case 4: CALL(4); break;                \
@@ -309,8 +327,9 @@ struct EllWidth {
 // may be one vector when the rows write where no row reads (the reduced
 // CG's KA: first colour from second).  DOT: the same sums as k_spmv.
 // ---------------------------------------------------------------------------
-template <class T, bool DOT, int W>
+template <class T, bool DOT, int W, bool PAT>
 __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const int16_t *__restrict__ off,
+                                             const uint8_t *__restrict__ pid, const int16_t *__restrict__ ptab,
                                              const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
                                              int64_t dbase, const T *x, T *y, T alpha, T beta, int beta_zero,
                                              const T *__restrict__ dotw, const T *__restrict__ aux,
@@ -326,11 +345,12 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const i
             constexpr int MAXW = EllWidth<W>::kMax;
             int16_t o[MAXW];
             T v[MAXW], xv[MAXW];
+            if constexpr (PAT) ell_pattern<MAXW>(ptab, pid[i], o);
 #pragma unroll
             for (int j = 0; j < MAXW; ++j)
                 if (EllWidth<W>::has(j, width)) {
                     const size_t e = (size_t)j * n + i;
-                    o[j] = off[e];
+                    if constexpr (!PAT) o[j] = off[e];
                     v[j] = val[e];
                 }
             T dgi = 0, xd = 0, yold = 0, u = 0, ax = 0, wg = 1;
@@ -386,8 +406,9 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const i
 // row's operation sequence is the one above, so y is the same bit for bit.
 // Needs an even row count and y aligned to two elements (checked by the
 // launcher).
-template <class T, int W>
+template <class T, int W, bool PAT>
 __global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const int16_t *__restrict__ off,
+                                              const uint8_t *__restrict__ pid, const int16_t *__restrict__ ptab,
                                               const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
                                               int64_t dbase, const T *x, T *y, T alpha, T beta, int beta_zero,
                                               const int *stop) {
@@ -403,13 +424,20 @@ __global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const 
         constexpr int MAXW = EllWidth<W>::kMax;
         int16_t o0[MAXW], o1[MAXW];
         T v0[MAXW], v1[MAXW], x0[MAXW], x1[MAXW];
+        if constexpr (PAT) {
+            const uint32_t pp = *reinterpret_cast<const uint16_t *>(pid + i);
+            ell_pattern<MAXW>(ptab, pp & 0xffu, o0);
+            ell_pattern<MAXW>(ptab, pp >> 8, o1);
+        }
 #pragma unroll
         for (int j = 0; j < MAXW; ++j)
             if (EllWidth<W>::has(j, width)) {
                 const size_t e = (size_t)j * n + i;
-                const uint32_t ow = *reinterpret_cast<const uint32_t *>(off + e);
-                o0[j] = (int16_t)(ow & 0xffffu);
-                o1[j] = (int16_t)(ow >> 16);
+                if constexpr (!PAT) {
+                    const uint32_t ow = *reinterpret_cast<const uint32_t *>(off + e);
+                    o0[j] = (int16_t)(ow & 0xffffu);
+                    o1[j] = (int16_t)(ow >> 16);
+                }
                 const T2 vv = *reinterpret_cast<const T2 *>(val + e);
                 v0[j] = vv.x;
                 v1[j] = vv.y;
@@ -478,21 +506,29 @@ void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void
         const uint32_t nb2 = (E.n / 2 + kBlock - 1) / kBlock;
         const unsigned g2 = xcd_grid(nb2, 1u << 20);
         const int bz2 = beta == 0.0;
-#define PP_ELL2_64(W)                                                                                     \
-    k_ell2<double, W><<<g2, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, E.v64, E.cbase, E.dg64, E.dbase, \
-                                                       (const double *)x, (double *)y, alpha, beta, bz2,     \
-                                                       ctx->stop_word)
-#define PP_ELL2_32(W)                                                                                   \
-    k_ell2<float, W><<<g2, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, E.v32, E.cbase, E.dg32, E.dbase, \
-                                                      (const float *)x, (float *)y, (float)alpha, (float)beta, \
-                                                      bz2, ctx->stop_word)
-        if (dtype == PP_F64) {
+#define PP_ELL2(T, W, P, VALS, DG)                                                                             \
+    k_ell2<T, W, P><<<g2, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, E.pid, E.ptab, VALS, E.cbase, DG, E.dbase, \
+                                                     (const T *)x, (T *)y, (T)alpha, (T)beta, bz2, ctx->stop_word)
+#define PP_ELL2_64(W) PP_ELL2(double, W, false, E.v64, E.dg64)
+#define PP_ELL2_32(W) PP_ELL2(float, W, false, E.v32, E.dg32)
+#define PP_ELL2_64P(W) PP_ELL2(double, W, true, E.v64, E.dg64)
+#define PP_ELL2_32P(W) PP_ELL2(float, W, true, E.v32, E.dg32)
+        if (E.pid) {
+            if (dtype == PP_F64) {
+                PP_ELL_PAT_WIDTHS(E.width, PP_ELL2_64P)
+            } else {
+                PP_ELL_PAT_WIDTHS(E.width, PP_ELL2_32P)
+            }
+        } else if (dtype == PP_F64) {
             PP_ELL_WIDTHS(E.width, PP_ELL2_64)
         } else {
             PP_ELL_WIDTHS(E.width, PP_ELL2_32)
         }
 #undef PP_ELL2_64
 #undef PP_ELL2_32
+#undef PP_ELL2_64P
+#undef PP_ELL2_32P
+#undef PP_ELL2
         PP_HIP(hipGetLastError());
         return;
     }
@@ -500,16 +536,34 @@ void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void
     if (grid == 0) return;
     double *part = dot ? ctx->red.partials + (size_t)dot_slot * kMaxPartials : nullptr;
     const int bz = beta == 0.0;
-#define PP_ELL(T, DOTV, W, VALS, DG)                                                                              \
-    k_ell<T, DOTV, W><<<grid, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, VALS, E.cbase, DG, E.dbase,         \
-                                                       (const T *)x, (T *)y, (T)alpha, (T)beta, bz,           \
-                                                       (const T *)dotw, (const T *)aux, (const T *)wgt, part, \
-                                                       ctx->stop_word)
-#define PP_ELL_64D(W) PP_ELL(double, true, W, E.v64, E.dg64)
-#define PP_ELL_64N(W) PP_ELL(double, false, W, E.v64, E.dg64)
-#define PP_ELL_32D(W) PP_ELL(float, true, W, E.v32, E.dg32)
-#define PP_ELL_32N(W) PP_ELL(float, false, W, E.v32, E.dg32)
-    if (dtype == PP_F64) {
+#define PP_ELL(T, DOTV, W, P, VALS, DG)                                                                       \
+    k_ell<T, DOTV, W, P><<<grid, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, E.pid, E.ptab, VALS, E.cbase, DG, \
+                                                          E.dbase, (const T *)x, (T *)y, (T)alpha, (T)beta, bz, \
+                                                          (const T *)dotw, (const T *)aux, (const T *)wgt, part, \
+                                                          ctx->stop_word)
+#define PP_ELL_64D(W) PP_ELL(double, true, W, false, E.v64, E.dg64)
+#define PP_ELL_64N(W) PP_ELL(double, false, W, false, E.v64, E.dg64)
+#define PP_ELL_32D(W) PP_ELL(float, true, W, false, E.v32, E.dg32)
+#define PP_ELL_32N(W) PP_ELL(float, false, W, false, E.v32, E.dg32)
+#define PP_ELL_64DP(W) PP_ELL(double, true, W, true, E.v64, E.dg64)
+#define PP_ELL_64NP(W) PP_ELL(double, false, W, true, E.v64, E.dg64)
+#define PP_ELL_32DP(W) PP_ELL(float, true, W, true, E.v32, E.dg32)
+#define PP_ELL_32NP(W) PP_ELL(float, false, W, true, E.v32, E.dg32)
+    if (E.pid) {
+        if (dtype == PP_F64) {
+            if (dot) {
+                PP_ELL_PAT_WIDTHS(E.width, PP_ELL_64DP)
+            } else {
+                PP_ELL_PAT_WIDTHS(E.width, PP_ELL_64NP)
+            }
+        } else {
+            if (dot) {
+                PP_ELL_PAT_WIDTHS(E.width, PP_ELL_32DP)
+            } else {
+                PP_ELL_PAT_WIDTHS(E.width, PP_ELL_32NP)
+            }
+        }
+    } else if (dtype == PP_F64) {
         if (dot) {
             PP_ELL_WIDTHS(E.width, PP_ELL_64D)
         } else {
@@ -526,6 +580,10 @@ void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void
 #undef PP_ELL_64N
 #undef PP_ELL_32D
 #undef PP_ELL_32N
+#undef PP_ELL_64DP
+#undef PP_ELL_64NP
+#undef PP_ELL_32DP
+#undef PP_ELL_32NP
 #undef PP_ELL
     PP_HIP(hipGetLastError());
 }
@@ -1098,6 +1156,26 @@ __device__ inline void chunk_ahead(ChunkAhead<T> &q, uint32_t L, uint32_t Lend, 
     q.d = S.pdiag ? S.pdiag[ks] : T(1);
 }
 
+// s - p[a] - p[a+1] - ... - p[b-1] (p in LDS), each subtraction rounded,
+// in index order.  The reads go out eight at a time before the first of
+// their subtractions, so a row waits on LDS once per eight entries instead
+// of once per entry (the row sums are the chunked sweep's critical path: a
+// few rows per level, a dozen or more entries per row).  Reads past b
+// repeat p[b-1] and subtract +0 instead, which leaves s unchanged (x - (+0)
+// is x for every x, -0 and NaN included).
+template <class T>
+__device__ inline T sub_run(T s, const T *p, uint32_t a, uint32_t b) {
+#pragma clang fp contract(off)
+    for (uint32_t e = a; e < b; e += 8) {
+        T q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) q[u] = p[min(e + u, b - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s = s - (e + u < b ? q[u] : T(0));
+    }
+    return s;
+}
+
 template <class T>
 __device__ inline void chunk_row_finish(T s, uint32_t i, T d, int mode, T *xs) {
 #pragma clang fp contract(off)
@@ -1134,7 +1212,7 @@ __device__ inline void chunk_level(ChunkAhead<T> &cur, ChunkAhead<T> &nxt, const
         if (cur.rok) {
             T s = xs[cur.i];
             if (mode == 3) s = s / cur.d;
-            for (uint32_t e = cur.e0; e < cur.e1; ++e) s = s - prod[e - E0];
+            s = sub_run(s, prod, cur.e0 - E0, cur.e1 - E0);
             chunk_row_finish(s, cur.i, cur.d, mode, xs);
         }
         for (uint32_t k = L0 + threadIdx.x + blockDim.x; k < L1; k += blockDim.x) {
@@ -1142,7 +1220,7 @@ __device__ inline void chunk_level(ChunkAhead<T> &cur, ChunkAhead<T> &nxt, const
             const T d = S.pdiag ? S.pdiag[k] : T(1);
             T s = xs[i];
             if (mode == 3) s = s / d;
-            for (uint32_t e = S.rp[k], e1 = S.rp[k + 1]; e < e1; ++e) s = s - prod[e - E0];
+            s = sub_run(s, prod, S.rp[k] - E0, S.rp[k + 1] - E0);
             chunk_row_finish(s, i, d, mode, xs);
         }
     } else {

@@ -888,6 +888,11 @@ void ml_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y) {
     PP_HIP(hipGetLastError());
 }
 
+const pp_precond *ml_block_factor(const pp_precond *M) {
+    const auto *S = M ? (const MlState *)M->multilevel : nullptr;
+    return S && !S->lev.empty() ? S->lev[0].Bpc : nullptr;
+}
+
 }  // namespace pp
 
 extern "C" int pp_ml_info(const pp_precond *M, uint32_t *nlev, uint32_t *level_sizes, uint32_t *ranks,

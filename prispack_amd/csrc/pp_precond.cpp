@@ -12,8 +12,9 @@
 namespace pp {
 // reverse Cuthill-McKee order of a CSR pattern (pp_io.cpp)
 std::vector<uint32_t> rcm_order(uint32_t n, const uint32_t *rp, const uint32_t *ci);
-// multilevel preconditioners (pp_multilevel.cpp)
+// multilevel preconditioners (pp_multilevel.hip)
 void ml_setup(pp_ctx *ctx, pp_precond *M);
+const pp_precond *ml_block_factor(const pp_precond *M);
 void ml_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);
 void ml_free(pp_precond *M);
 void ml_need_f32(pp_ctx *ctx, pp_precond *M);
@@ -673,7 +674,9 @@ static void lower_levels(uint32_t n, const std::vector<uint32_t> &rp, const std:
 // diagonal, the one-workgroup schedule).
 static bool devref_build(pp_ctx *ctx, pp_precond *M) {
     const int kind = M->opts.kind;
-    if ((kind != PP_PC_IC0 && kind != PP_PC_ILU0) || M->trsv == PP_TRSV_BLOCK || M->multilevel) return false;
+    if ((kind != PP_PC_IC0 && kind != PP_PC_ILU0) || M->trsv == PP_TRSV_BLOCK || M->trsv == PP_TRSV_PIPE ||
+        M->multilevel)
+        return false;
     const bool ordered = !M->order.empty();
     if (ordered && !M->Aperm) return false;
     const pp_csr *F = ordered ? M->Aperm : M->A;  // the matrix the factor was computed from
@@ -1020,14 +1023,15 @@ void precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y
         }
         return;
     }
-    if (M->trsv == PP_TRSV_BLOCK) {
+    if (M->trsv == PP_TRSV_BLOCK || M->trsv == PP_TRSV_PIPE) {
+        const bool pipe = M->trsv == PP_TRSV_PIPE;
         if (dtype == PP_F64) {
-            launch_sweep_pair_chunks<double>(ctx, M->chunks, M->clow64, M->hlow.mode, M->cup64, M->hup.mode,
-                                             (const double *)x, (double *)y);
+            (pipe ? launch_sweep_pair_pipe<double> : launch_sweep_pair_chunks<double>)(
+                ctx, M->chunks, M->clow64, M->hlow.mode, M->cup64, M->hup.mode, (const double *)x, (double *)y);
         } else {
             precond_need_f32(M);
-            launch_sweep_pair_chunks<float>(ctx, M->chunks, M->clow32, M->hlow.mode, M->cup32, M->hup.mode,
-                                            (const float *)x, (float *)y);
+            (pipe ? launch_sweep_pair_pipe<float> : launch_sweep_pair_chunks<float>)(
+                ctx, M->chunks, M->clow32, M->hlow.mode, M->cup32, M->hup.mode, (const float *)x, (float *)y);
         }
         return;
     }
@@ -1103,39 +1107,61 @@ static void tune_sweeps(pp_ctx *ctx, pp_precond *M) {
     M->sweep_variant = tune_sweep_pair<double>(ctx, M, M->low64, M->up64, M->sweep_variant);
 }
 
-// PP_TRSV_AUTO: the chunked one-workgroup schedule when the rows chunk
-// within LDS and an apply with it is faster than one launch per level (timed
-// here, a few applies each), else one launch per level.  PP_TRSV_BLOCK asked
-// for explicitly falls back to levels when the rows do not chunk.
-static int choose_schedule(pp_ctx *ctx, pp_precond *M, bool force_block) {
+// PP_TRSV_AUTO: the fastest of one launch per level and, when the rows chunk
+// within LDS, the chunked one-workgroup schedules (barrier per level, and
+// the pipelined form when its rings fit), timed here over a few applies
+// each.  PP_TRSV_BLOCK / PP_TRSV_PIPE asked for explicitly (force) fall
+// back to levels when the rows do not chunk, and PIPE to BLOCK when the
+// rings do not fit.
+static int choose_schedule(pp_ctx *ctx, pp_precond *M, int force) {
     const uint32_t nlev = M->low64.nlev + M->up64.nlev;
-    if ((!force_block && nlev < 8) || !build_block_sweeps(ctx, M)) return PP_TRSV_LEVELS;
-    if (force_block) return PP_TRSV_BLOCK;
+    if ((!force && nlev < 8) || !build_block_sweeps(ctx, M)) return PP_TRSV_LEVELS;
+    uint32_t re = 0, rr = 0;
+    const bool pipe_ok = sweep_pipe_rings<double>(ctx, M->chunks, re, rr);
+    if (force == PP_TRSV_BLOCK || (force == PP_TRSV_PIPE && !pipe_ok)) return PP_TRSV_BLOCK;
+    if (force == PP_TRSV_PIPE) return PP_TRSV_PIPE;
     double *v = dalloc<double>(M->n);
-    float best[2] = {0.f, 0.f};
+    const int cand[3] = {PP_TRSV_LEVELS, PP_TRSV_BLOCK, PP_TRSV_PIPE};
+    float ms[3] = {0.f, 0.f, 0.f};
+    const int ncand = pipe_ok ? 3 : 2;
     try {
         launch_fill_zero(ctx, v, (size_t)M->n * sizeof(double));
-        for (int k = 0; k < 2; ++k) {
+        launch_fill_zero(ctx, ctx->ctl, C_COUNT * sizeof(int));
+        for (int k = 0; k < ncand; ++k) {
             auto apply = [&] {
-                if (k == 0)
+                if (cand[k] == PP_TRSV_LEVELS)
                     launch_sweep_pair<double>(ctx, M->low64, M->hlow.mode, M->up64, M->hup.mode, M->fuse_mid, v, v);
-                else
+                else if (cand[k] == PP_TRSV_BLOCK)
                     launch_sweep_pair_chunks<double>(ctx, M->chunks, M->clow64, M->hlow.mode, M->cup64, M->hup.mode,
                                                      v, v);
+                else
+                    launch_sweep_pair_pipe<double>(ctx, M->chunks, M->clow64, M->hlow.mode, M->cup64, M->hup.mode,
+                                                   v, v);
             };
             apply();
             PP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
             for (int r = 0; r < 3; ++r) apply();
             PP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
             PP_HIP(hipEventSynchronize(ctx->ev1));
-            PP_HIP(hipEventElapsedTime(&best[k], ctx->ev0, ctx->ev1));
+            PP_HIP(hipEventElapsedTime(&ms[k], ctx->ev0, ctx->ev1));
+        }
+        if (ncand == 3) {  // a pipelined sweep that gave up waiting is not a candidate
+            int ctl[C_COUNT];
+            read_ctl(ctx, ctl);
+            if (ctl[C_SPIN]) {
+                ms[2] = 1e30f;
+                launch_fill_zero(ctx, ctx->ctl, C_COUNT * sizeof(int));
+            }
         }
     } catch (...) {
         dfree(v);
         throw;
     }
     dfree(v);
-    if (best[1] < best[0]) return PP_TRSV_BLOCK;
+    int best = 0;
+    for (int k = 1; k < ncand; ++k)
+        if (ms[k] < ms[best]) best = k;
+    if (cand[best] != PP_TRSV_LEVELS) return cand[best];
     M->chunks.free_all();  // not used: give the memory back
     M->clow64.free_all();
     M->cup64.free_all();
@@ -1184,9 +1210,10 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
             auto tp = std::chrono::steady_clock::now();
             M->phase_ms[2] = (float)(std::chrono::duration<double, std::milli>(tp - t0).count() - M->phase_ms[0] -
                                      M->phase_ms[1]);
-            M->trsv = M->opts.trsv == PP_TRSV_AUTO    ? choose_schedule(ctx, M, false)
-                      : M->opts.trsv == PP_TRSV_BLOCK ? choose_schedule(ctx, M, true)
-                                                      : M->opts.trsv;
+            const int want = M->opts.trsv;
+            M->trsv = want == PP_TRSV_AUTO || want == PP_TRSV_BLOCK || want == PP_TRSV_PIPE
+                          ? choose_schedule(ctx, M, want)
+                          : want;
             if (M->trsv == PP_TRSV_LEVELS) tune_sweeps(ctx, M);
             // the sync-free apply's buffer between its two sweeps: allocated here, so that no
             // apply allocates (a solve may capture its first apply into a graph)
@@ -1245,7 +1272,7 @@ int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void 
         if (dtype == PP_F32) precond_need_f32(M);  // FP32 copies, once, outside any capture
         launch_fill_zero(ctx, ctx->ctl, C_COUNT * sizeof(int));  // clear stop / error words (graph-safe)
         precond_apply(ctx, M, dtype, x, y);
-        if (M->trsv == PP_TRSV_SYNCFREE) {
+        if (M->trsv == PP_TRSV_SYNCFREE || M->trsv == PP_TRSV_PIPE) {
             int ctl[C_COUNT];
             read_ctl(ctx, ctl);
             PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
@@ -1381,11 +1408,17 @@ int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {
     info->fill_ratio = M->A->nnz ? (double)M->mnnz / (double)M->A->nnz : 0.0;
     info->levels_lower = M->low64.nlev;
     info->levels_upper = M->up64.nlev;
+    info->trsv = M->trsv;
+    if (const pp_precond *B = ml_block_factor(M)) {
+        // multilevel: the sweeps of the first level's block factor (the subdomains)
+        info->levels_lower = B->low64.nlev;
+        info->levels_upper = B->up64.nlev;
+        info->trsv = B->trsv;
+    }
     info->setup_ms = M->setup_ms;
     info->fail_row = M->fail_row;
     info->sweep_variant = M->sweep_variant;
     info->fused_levels = M->fuse_mid ? 1 : 0;
-    info->trsv = M->trsv;
     for (int k = 0; k < 4; ++k) info->phase_ms[k] = M->phase_ms[k];
     info->device_refactors = M->device_refactors;
     return PP_OK;

@@ -9,6 +9,9 @@
 #include <memory>
 #include <mutex>
 
+#include <array>
+#include <map>
+
 #include "pp_internal.h"
 
 namespace pp {
@@ -157,6 +160,11 @@ bool reduced_host(uint32_t n, const std::vector<uint32_t> &rp, const std::vector
 
 void EllOp::free_all() {
     dfree(off);
+    dfree(pid);
+    dfree(ptab);
+    pid = nullptr;
+    ptab = nullptr;
+    npat = 0;
     dfree(v64);
     dfree(v32);
     dfree(dg64);
@@ -219,9 +227,72 @@ bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
     E.width = width;
     E.cbase = cbase;
     E.dbase = dbase;
-    E.off = dalloc<int16_t>(off.size());
+    // Row patterns: when the rows' offset tuples take at most 256 distinct
+    // values (a stencil: interior rows share one, the boundary rows a few),
+    // a row stores a one-byte pattern number instead of width 16-bit
+    // offsets, and the kernels read the row's offsets from a 16-byte table
+    // entry (cached): 2 width - 1 bytes less per row.  For the widths the
+    // kernels are instantiated at exactly (4 to 7).
+    std::vector<uint8_t> pid;
+    std::vector<int16_t> ptab;
+    if (width >= 4 && width <= 7) {
+        // a row's offsets packed into one 64-bit key: 7 slots of 16 bits do
+        // not fit, so the key is the tuple hashed and the table compared
+        auto key_of = [&](uint32_t i, int16_t *k) {
+            for (uint32_t j = 0; j < 8; ++j) k[j] = j < width ? off[(size_t)j * n + i] : 0;
+        };
+        auto hash_of = [](const int16_t *k) {
+            uint64_t h = 1469598103934665603ull;
+            for (int j = 0; j < 8; ++j) h = (h ^ (uint16_t)k[j]) * 1099511628211ull;
+            return h;
+        };
+        std::unordered_multimap<uint64_t, uint8_t> seen;
+        pid.resize(n);
+        bool ok = true;
+        int16_t key[8], last[8] = {};
+        int last_id = -1;
+        for (uint32_t i = 0; i < n && ok; ++i) {
+            key_of(i, key);
+            if (last_id >= 0 && std::equal(key, key + 8, last)) {  // neighbouring rows mostly repeat
+                pid[i] = (uint8_t)last_id;
+                continue;
+            }
+            const uint64_t h = hash_of(key);
+            int id = -1;
+            for (auto [it, end] = seen.equal_range(h); it != end; ++it)
+                if (std::equal(key, key + 8, ptab.begin() + 8 * (size_t)it->second)) {
+                    id = it->second;
+                    break;
+                }
+            if (id < 0) {
+                if (seen.size() == 256) {
+                    ok = false;
+                    break;
+                }
+                id = (int)seen.size();
+                seen.emplace(h, (uint8_t)id);
+                ptab.insert(ptab.end(), key, key + 8);
+            }
+            pid[i] = (uint8_t)id;
+            std::copy(key, key + 8, last);
+            last_id = id;
+        }
+        if (!ok) {
+            pid.clear();
+            ptab.clear();
+        }
+    }
+    if (!pid.empty()) {
+        E.npat = (uint32_t)(ptab.size() / 8);
+        E.pid = dalloc<uint8_t>(n + 1);  // one more byte: a two-row load at the last even row stays inside
+        E.ptab = dalloc<int16_t>(ptab.size());
+        PP_HIP(hipMemcpy(E.pid, pid.data(), n, hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(E.ptab, ptab.data(), ptab.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+    } else {
+        E.off = dalloc<int16_t>(off.size());
+        PP_HIP(hipMemcpy(E.off, off.data(), off.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+    }
     E.v64 = dalloc<double>(val.size());
-    PP_HIP(hipMemcpy(E.off, off.data(), off.size() * sizeof(int16_t), hipMemcpyHostToDevice));
     PP_HIP(hipMemcpy(E.v64, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice));
     if (diag_last) {
         E.dg64 = dalloc<double>(n);

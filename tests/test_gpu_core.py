@@ -182,6 +182,66 @@ def test_spmv_variants_and_solves_agree(pp, torch, name):
         assert np.linalg.norm(xs - runs[0][2]) <= 1e-7 * np.linalg.norm(runs[0][2]), var
 
 
+def random_band(n, per_row, span, seed):
+    # per_row distinct sorted columns within +-span of each row (diagonal
+    # included): offsets that almost never repeat from row to row.  Row i's
+    # other columns come one from each of per_row - 1 equal segments of its
+    # window (so they are distinct), moved off the diagonal within a segment.
+    rng = np.random.default_rng(seed)
+    k = per_row - 1
+    i = np.arange(n, dtype=np.int64)
+    lo, hi = np.maximum(0, i - span), np.minimum(n, i + span + 1)
+    w = (hi - lo) // k
+    seg = lo[:, None] + np.arange(k)[None, :] * w[:, None]
+    c = seg + (rng.random((n, k)) * w[:, None]).astype(np.int64)
+    hit = c == i[:, None]
+    c[hit] = np.where(c[hit] + 1 < (seg + w[:, None])[hit], c[hit] + 1, c[hit] - 1)
+    ci = np.sort(np.concatenate([c, i[:, None]], axis=1), axis=1).astype(np.uint32).ravel()
+    rp = np.arange(n + 1, dtype=np.uint32) * per_row
+    v = rng.uniform(-1.0, 1.0, n * per_row)
+    return rp, ci, v
+
+
+@pytest.mark.parametrize("name", ["lap3d_40", "lap3d_41", "random_7"])
+def test_spmv_sliced_column_offset_forms(pp, torch, name):
+    # the sliced-column copy stores a stencil's offsets as one pattern byte
+    # per row (at most 256 distinct offset rows) and anything else as 16-bit
+    # offsets per entry; both give the oracle's y bit for bit, with two rows
+    # per thread (even n) and one (odd n)
+    import ctypes as C
+    from prispack_amd import _capi
+    L = _capi.lib()
+    if name == "random_7":
+        A = random_band(40000, 7, 20000, 5)
+    else:
+        g = int(name[-2:])
+        A = ppo.laplacian(g, g, g)
+    n = len(A[0]) - 1
+    print(f"[ell forms] {name} n={n}", flush=True)
+    M = pp.CSRMatrix(*A)
+    for dt in (0, 1):
+        assert L.pp_csr_set_spmv_variant(M.h, dt, 8) == 0
+    nb = C.c_uint64()
+    assert L.pp_csr_spmv_bytes(M.h, 0, C.byref(nb)) == 0
+    dense_offsets = 10 * 7 * n + 16 * n
+    if name == "random_7":
+        assert nb.value == dense_offsets
+    else:
+        assert nb.value < 8 * 7 * n + 2 * n + 16 * n  # a byte per row, a small table
+    x = ppo.uniform(n, 41)
+    want = ppo.spmv(A, x)
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    M.MatPVec(dev(torch, x), y)
+    assert np.array_equal(host(y), want)
+    b = ppo.uniform(n, 42)
+    yb = dev(torch, b)
+    M.MatPVec(dev(torch, x), yb, alpha=-1.0, beta=1.0)
+    np.testing.assert_allclose(host(yb), b - want, rtol=0, atol=1e-13 * (1 + np.abs(want).max()))
+    y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+    M.MatPVec(dev(torch, x, torch.float32), y32)
+    np.testing.assert_allclose(host(y32), want, rtol=0, atol=1e-5 * (1 + np.abs(want).max()))
+
+
 def test_sliced_column_copy_follows_value_updates(pp, torch):
     # new values on the same pattern (on the host, then on the device) reach
     # the sliced-column copy in both precisions
@@ -283,10 +343,10 @@ def make_prec(pp, A, kind, args, trsv=0):
     return P
 
 
-TRSV = {"levels": 1, "syncfree": 2, "block": 3}
+TRSV = {"levels": 1, "syncfree": 2, "block": 3, "pipe": 4}
 
 
-@pytest.mark.parametrize("trsv", ["levels", "syncfree", "block"])
+@pytest.mark.parametrize("trsv", ["levels", "syncfree", "block", "pipe"])
 @pytest.mark.parametrize("kind,args,mat", PRECS)
 def test_precond_apply_bit_exact(pp, torch, kind, args, mat, trsv):
     A = prec_matrix(mat)
@@ -522,10 +582,10 @@ def test_block_schedule_solves(pp, torch, kind, args):
     print(f"[block] {kind} n={n}", flush=True)
     M = pp.CSRMatrix(*A)
     out = {}
-    for trsv in (1, 3, 0):
+    for trsv in (1, 3, 4, 0):
         P = make_prec(pp, A, kind, args, trsv)
         info = P.info()
-        assert info.trsv == (trsv if trsv else info.trsv) and info.trsv in (1, 3)
+        assert info.trsv == (trsv if trsv else info.trsv) and info.trsv in (1, 3, 4)
         if kind == "ilut":
             s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000)
             s.setRestart(True, 30)
@@ -535,19 +595,21 @@ def test_block_schedule_solves(pp, torch, kind, args):
         s.solve(M, dev(torch, b), x)
         assert s.getConvergence() and s.report.true_relres < 1e-8
         out[trsv] = (s.getNumIter(), host(x))
-    print(f"[block] {kind}: levels {out[1][0]} it, block {out[3][0]} it", flush=True)
-    assert out[1][0] == out[3][0] == out[0][0]
-    assert np.array_equal(out[1][1], out[3][1]) and np.array_equal(out[1][1], out[0][1])
+    print(f"[block] {kind}: levels {out[1][0]} it, block {out[3][0]} it, pipelined {out[4][0]} it", flush=True)
+    assert out[1][0] == out[3][0] == out[4][0] == out[0][0]
+    for t in (3, 4, 0):
+        assert np.array_equal(out[1][1], out[t][1]), t
     # mixed precision runs the FP32 one-workgroup sweeps
-    P = make_prec(pp, A, kind, args, 3)
-    if kind == "ilut":
-        s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000, precision="mixed")
-        s.setRestart(True, 30)
-    else:
-        s = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision="mixed")
-    x = torch.zeros(n, dtype=torch.float64, device="cuda")
-    s.solve(M, dev(torch, b), x)
-    assert s.getConvergence() and s.report.true_relres < 1e-8
+    for trsv in (3, 4):
+        P = make_prec(pp, A, kind, args, trsv)
+        if kind == "ilut":
+            s = pp.GMRES(n, P, 1e-8, pp.FlexibleGMRES, maxIter=3000, precision="mixed")
+            s.setRestart(True, 30)
+        else:
+            s = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision="mixed")
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        s.solve(M, dev(torch, b), x)
+        assert s.getConvergence() and s.report.true_relres < 1e-8, trsv
 
 
 
@@ -572,18 +634,57 @@ def test_block_schedule_chunks(pp, torch, kind, args):
     n = len(A[0]) - 1
     print(f"[chunks] {kind} n={n}", flush=True)
     P1 = make_prec(pp, A, kind, args, 1)
-    P3 = make_prec(pp, A, kind, args, 3)
-    assert P3.info().trsv == 3
     x = ppo.uniform(n, 8)
-    t1, t3 = dev(torch, x), dev(torch, x)
+    t1, f1 = dev(torch, x), dev(torch, x, torch.float32)
     P1.MInvSolve(t1)
-    P3.MInvSolve(t3)
-    assert np.array_equal(host(t1), host(t3))
-    f1, f3 = dev(torch, x, torch.float32), dev(torch, x, torch.float32)
     P1.MInvSolve(f1)
-    P3.MInvSolve(f3)
-    assert np.array_equal(host(f1), host(f3))
-    # one connected block larger than the LDS holds: PP_TRSV_BLOCK falls back
+    for trsv in (3, 4):
+        P = make_prec(pp, A, kind, args, trsv)
+        assert P.info().trsv == trsv
+        t, f = dev(torch, x), dev(torch, x, torch.float32)
+        P.MInvSolve(t)
+        P.MInvSolve(f)
+        assert np.array_equal(host(t1), host(t)), trsv
+        assert np.array_equal(host(f1), host(f)), trsv
+    # one connected block larger than the LDS holds: PP_TRSV_BLOCK / PIPE fall back
     B = ppo.laplacian(160, 160, 1)
-    PB = make_prec(pp, B, kind if kind != "ilut" else "ict", (1e-3,), 3)
-    assert PB.info().trsv == 1
+    for trsv in (3, 4):
+        PB = make_prec(pp, B, kind if kind != "ilut" else "ict", (1e-3,), trsv)
+        assert PB.info().trsv == 1
+
+
+@pytest.mark.parametrize("case", ["wide_levels", "many_rows", "long_rows", "two_colour"])
+def test_pipe_sweep_shapes(pp, torch, case):
+    # the pipelined one-workgroup sweep (PP_TRSV_PIPE) on the shapes its
+    # paths serve: levels of more than 64 rows staged in the ring (200
+    # independent 1D blocks), levels too large for the ring read straight
+    # from memory (400 blocks: 400-row levels), rows longer than the
+    # registers hold (ICT(1e-5) fill), and the two levels of a two-colour
+    # IC(0); every apply bit-identical to the level schedule, FP64 and FP32
+    if case in ("wide_levels", "many_rows"):
+        nb = 200 if case == "wide_levels" else 400
+        A = block_diag(*[ppo.laplacian(20, 1, 1) for _ in range(nb)])
+        kind, args = "ict", (1e-3,)
+    elif case == "long_rows":
+        A = ppo.laplacian(48, 48, 1)
+        kind, args = "ict", (1e-5,)
+    else:
+        A = ppo.laplacian(40, 40, 1)
+        kind, args = "ic0", ()
+    n = len(A[0]) - 1
+    print(f"[pipe] {case} n={n}", flush=True)
+    if case == "two_colour":
+        P1, P4 = (pp.IC0(pp.CSRMatrix(*A), ordering=1, trsv=t) for t in (1, 4))
+        P1.setup()
+        P4.setup()
+    else:
+        P1 = make_prec(pp, A, kind, args, 1)
+        P4 = make_prec(pp, A, kind, args, 4)
+    assert P4.info().trsv == 4
+    for seed in (3, 4):
+        x = ppo.uniform(n, seed)
+        for dt in (torch.float64, torch.float32):
+            a, b = dev(torch, x, dt), dev(torch, x, dt)
+            P1.MInvSolve(a)
+            P4.MInvSolve(b)
+            assert np.array_equal(host(a), host(b)), (seed, dt)

@@ -249,12 +249,13 @@ class Ref:
     """The reference's own C++ hot path, compiled from its sources.
 
     variant 'strict' = serial, strict IEEE (golden vectors);
-    variant 'omp'    = the reference release flags + 4 OpenMP threads (timing).
+    variant 'omp'    = the reference release flags + 4 OpenMP threads (timing);
+    variant 'all'    = the same with 16 OpenMP threads (threads_override.h).
     Only present where oracle/_ref was built.
     """
 
     def __init__(self, variant="strict"):
-        name = "libppref.so" if variant == "strict" else "libppref_omp.so"
+        name = {"strict": "libppref.so", "omp": "libppref_omp.so", "all": "libppref_all.so"}[variant]
         path = os.path.join(HERE, "_ref", name)
         if not os.path.exists(path):
             raise FileNotFoundError(path)

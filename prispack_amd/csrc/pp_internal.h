@@ -646,6 +646,18 @@ void red_update(pp_ctx *ctx, uint32_t n, const T *w, T *p, T *s, T *x, T *r);
 // is written).  Iteration bodies hold an even number of these.
 template <class T>
 void red_step(pp_ctx *ctx, unsigned np, int parity, bool publish, uint32_t n, const T *w, T *p, T *s, T *x, T *r);
+// The reduced-CG iteration in two launches (k_red_ka, k_red_kb): red_ka =
+// the previous iteration's check and step lengths, then y0 = KA r_new with
+// the residual update folded into the gathers; red_kb = the update of p, s,
+// x, r in place, then w = KB [y0; r_new] with the three inner products (grid:
+// the partial count, as the DOT sliced-column launch).  Available when KA
+// and KB have exact-width (4-7) sliced-column forms (red_fused_ok).
+bool red_fused_ok(const EllOp &ka, const EllOp &kb, int dtype);
+template <class T>
+void red_ka(pp_ctx *ctx, const EllOp &E, const T *z, T *y, uint32_t rbase, const T *w, const T *s, unsigned np,
+            int parity, bool publish);
+template <class T>
+void red_kb(pp_ctx *ctx, const EllOp &E, T *z, uint32_t rbase, T *w, T *p, T *s, T *x, const T *wt, unsigned grid);
 // State block 0 for a fresh reduced solve, or (resume) for one that
 // continues from S_CG_GOLD / S_CG_ALPHA / S_CG_FIRST, gamma scaled by gscale.
 void red_init(pp_ctx *ctx, bool resume, double gscale, bool last);

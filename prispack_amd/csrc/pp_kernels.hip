@@ -1709,41 +1709,17 @@ void red_update(pp_ctx *ctx, uint32_t n, const T *w, T *p, T *s, T *x, T *r) {
 template void red_update<double>(pp_ctx *, uint32_t, const double *, double *, double *, double *, double *);
 template void red_update<float>(pp_ctx *, uint32_t, const float *, float *, float *, float *, float *);
 
-// Fused finish + update of the reduced CG (see red_step in pp_internal.h).
-// The test mirrors pcg_check and k_cgsr_finish: a pending last check stops
-// with m = maxit; a non-finite norm or norm < tol stops with m = the check
-// index; a zero or non-finite denominator is a breakdown.
-template <class T>
-__global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ partials, unsigned np, double *scal,
-                                                   int *ctl, int *ring, int parity, uint32_t n,
-                                                   const T *__restrict__ w, T *__restrict__ p, T *__restrict__ s,
-                                                   T *__restrict__ x, T *__restrict__ r) {
+// The convergence test and step lengths of one reduced-CG iteration from
+// the three inner products' partials (see red_step in pp_internal.h); every
+// workgroup of kFinishThreads threads computes the same, workgroup 0 also
+// writes the iteration state.  The test mirrors pcg_check and
+// k_cgsr_finish: a pending last check stops with m = maxit; a non-finite
+// norm or norm < tol stops with m = the check index; a zero or non-finite
+// denominator is a breakdown.  Returns false when the iteration stops.
+__device__ inline bool red_check(const double *__restrict__ partials, unsigned np, double *scal, int *ctl, int *ring,
+                                 int parity, double *sh, double *bc, int *go, double &alpha_out,
+                                 double &beta_out) {
 #pragma clang fp contract(off)
-    __shared__ double sh[3 * (kFinishThreads / 64)];
-    __shared__ double bc[2];
-    __shared__ int go;
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (ctl[C_STOP]) {  // an earlier check stopped (or this launch's first workgroup already did)
-        if (ring && lead) publish_ctl(ctl, ring);
-        return;
-    }
-    // the first kU elements of this thread are loaded before the reduction
-    // (they do not depend on alpha / beta), so their latency overlaps it
-    constexpr int kU = 4;
-    const uint32_t stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    T vr[kU], vp[kU], vw[kU], vs[kU], vx[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-        const uint32_t i = i0 + u * stride;
-        vr[u] = vp[u] = vw[u] = vs[u] = vx[u] = 0;
-        if (i < n) {
-            vr[u] = r[i];
-            vp[u] = p[i];
-            vw[u] = w[i];
-            vs[u] = s[i];
-            vx[u] = x[i];
-        }
-    }
     double t[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) t[q] = sum_partials(partials + (size_t)q * kMaxPartials, np);
@@ -1786,7 +1762,7 @@ __global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ pa
         }
         bc[0] = alpha;
         bc[1] = beta;
-        go = !stop;
+        *go = !stop;
         if (blockIdx.x == 0) {
             scal[S_CG_DELTA] = delta;
             scal[S_CG_GAMMA] = gamma;
@@ -1820,8 +1796,46 @@ __global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ pa
         }
     }
     __syncthreads();
-    if (!go) return;
-    const T alpha = (T)bc[0], beta = (T)bc[1];
+    alpha_out = bc[0];
+    beta_out = bc[1];
+    return *go != 0;
+}
+
+// Fused finish + update of the reduced CG (see red_step in pp_internal.h).
+template <class T>
+__global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ partials, unsigned np, double *scal,
+                                                   int *ctl, int *ring, int parity, uint32_t n,
+                                                   const T *__restrict__ w, T *__restrict__ p, T *__restrict__ s,
+                                                   T *__restrict__ x, T *__restrict__ r) {
+#pragma clang fp contract(off)
+    __shared__ double sh[3 * (kFinishThreads / 64)];
+    __shared__ double bc[2];
+    __shared__ int go;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (ctl[C_STOP]) {  // an earlier check stopped (or this launch's first workgroup already did)
+        if (ring && lead) publish_ctl(ctl, ring);
+        return;
+    }
+    // the first kU elements of this thread are loaded before the reduction
+    // (they do not depend on alpha / beta), so their latency overlaps it
+    constexpr int kU = 4;
+    const uint32_t stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    T vr[kU], vp[kU], vw[kU], vs[kU], vx[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const uint32_t i = i0 + u * stride;
+        vr[u] = vp[u] = vw[u] = vs[u] = vx[u] = 0;
+        if (i < n) {
+            vr[u] = r[i];
+            vp[u] = p[i];
+            vw[u] = w[i];
+            vs[u] = s[i];
+            vx[u] = x[i];
+        }
+    }
+    double a64, b64;
+    if (!red_check(partials, np, scal, ctl, ring, parity, sh, bc, &go, a64, b64)) return;
+    const T alpha = (T)a64, beta = (T)b64;
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
         const uint32_t i = i0 + u * stride;
@@ -1845,6 +1859,130 @@ __global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ pa
     }
 }
 
+// The reduced-CG iteration in two launches instead of three (red_step split
+// over the two products, sliced-column forms of KA and KB with exact
+// widths).  k_red_ka: the check and step lengths of the previous iteration
+// (red_check, as k_red_step), then y0 = KA r_new over the first colour,
+// with the previous iteration's residual update folded into the gathers:
+// r_new[j] = r[j] - alpha (w[j] + beta s[j]), the operations of k_red_step
+// in its order, so the products see the bits the separate update stores.
+// Nothing of the second colour is written here (other workgroups still
+// gather the old r, w, s).  The block's first rows' matrix entries are
+// loaded before the check, so its latency overlaps them.
+template <class T, int W, bool PAT>
+__global__ __launch_bounds__(1024) void k_red_ka(uint32_t n0, const int16_t *__restrict__ off,
+                                                 const uint8_t *__restrict__ pid, const int16_t *__restrict__ ptab,
+                                                 const T *__restrict__ val, int64_t cbase, const T *z, T *y,
+                                                 uint32_t rbase, const T *__restrict__ w, const T *__restrict__ s,
+                                                 const double *__restrict__ partials, unsigned np, double *scal,
+                                                 int *ctl, int *ring, int parity) {
+#pragma clang fp contract(off)
+    __shared__ double sh[3 * (kFinishThreads / 64)];
+    __shared__ double bc[2];
+    __shared__ int go;
+    if (ctl[C_STOP]) {
+        if (ring && blockIdx.x == 0 && threadIdx.x == 0) publish_ctl(ctl, ring);
+        return;
+    }
+    // XCD-aware rows: the workgroups of XCD q (workgroup b runs on XCD b % 8)
+    // take the contiguous eighth q of the rows, so the gathered neighbours
+    // stay in that XCD's L2
+    uint32_t lo = 0, hi = n0, stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gridDim.x % kXcds == 0) {
+        const uint32_t q = blockIdx.x % kXcds;
+        lo = (uint32_t)((uint64_t)n0 * q / kXcds);
+        hi = (uint32_t)((uint64_t)n0 * (q + 1) / kXcds);
+        stride = gridDim.x / kXcds * blockDim.x;
+        i0 = lo + blockIdx.x / kXcds * blockDim.x + threadIdx.x;
+    }
+    int16_t o[W];
+    T v[W];
+    auto load_row = [&](uint32_t i) {
+        if constexpr (PAT) ell_pattern<W>(ptab, pid[i], o);
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            const size_t e = (size_t)j * n0 + i;
+            if constexpr (!PAT) o[j] = off[e];
+            v[j] = val[e];
+        }
+    };
+    if (i0 < hi) load_row(i0);
+    double a64, b64;
+    if (!red_check(partials, np, scal, ctl, ring, parity, sh, bc, &go, a64, b64)) return;
+    const T alpha = (T)a64, beta = (T)b64;
+    for (uint32_t i = i0; i < hi; i += stride) {
+        if (i != i0) load_row(i);
+        T xv[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            const int64_t g = (int64_t)i + cbase + o[j];  // in z; the second colour starts at rbase
+            const T sn = w[g - rbase] + beta * s[g - rbase];
+            xv[j] = z[g] - alpha * sn;
+        }
+        T sum = 0;
+#pragma unroll
+        for (int j = 0; j < W; ++j) sum += v[j] * xv[j];
+        y[i] = sum;
+    }
+}
+
+// k_red_kb: the previous iteration's update of the second colour (p, s, x,
+// r in place, k_red_step's operations; alpha and beta as k_red_ka left them
+// in scal), then w = KB [y0; r_new] and the three inner products of the
+// next check, exactly as k_ell with DOT computes them.
+template <class T, int W, bool PAT>
+__global__ __launch_bounds__(256) void k_red_kb(uint32_t n1, const int16_t *__restrict__ off,
+                                                const uint8_t *__restrict__ pid, const int16_t *__restrict__ ptab,
+                                                const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
+                                                T *z, uint32_t rbase, T *__restrict__ w, T *__restrict__ p,
+                                                T *__restrict__ s, T *__restrict__ x, const T *__restrict__ wgt,
+                                                const double *scal, double *partials, const int *stop) {
+#pragma clang fp contract(off)
+    __shared__ double sh[3 * (kBlock / 64)];
+    if (stop && *stop) return;
+    const T alpha = (T)scal[S_CG_ALPHA], beta = (T)scal[S_CG_BETA];
+    double dacc = 0.0, dacc2 = 0.0, dacc3 = 0.0;
+    const uint32_t nb = (n1 + kBlock - 1) / kBlock;
+    const XcdRange xr = xcd_range(0, nb);
+    for (uint32_t b = xr.first; b < xr.end; b += xr.step) {
+        const uint32_t i = b * kBlock + threadIdx.x;
+        if (i < n1) {
+            int16_t o[W];
+            T v[W], xv[W];
+            if constexpr (PAT) ell_pattern<W>(ptab, pid[i], o);
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                const size_t e = (size_t)j * n1 + i;
+                if constexpr (!PAT) o[j] = off[e];
+                v[j] = val[e];
+            }
+            const T dgi = dg[i], ri = z[rbase + i], pi = p[i], wi = w[i], si = s[i], xi = x[i], wg = wgt[i];
+#pragma unroll
+            for (int j = 0; j < W; ++j) xv[j] = z[(int64_t)i + cbase + o[j]];
+            const T pn = ri + beta * pi;
+            const T sn = wi + beta * si;
+            const T rn = ri - alpha * sn;
+            p[i] = pn;
+            s[i] = sn;
+            x[i] = xi + alpha * pn;
+            z[rbase + i] = rn;
+            T sum = 0;
+#pragma unroll
+            for (int j = 0; j < W; ++j) sum += v[j] * xv[j];
+            sum += dgi * rn;
+            const T out = T(1) * sum;
+            w[i] = out;
+            dacc += (double)rn * (double)out;
+            dacc2 += (double)rn * (double)rn;
+            dacc3 += (double)rn * (double)rn * (double)wg;
+        }
+    }
+    block_sum3(dacc, dacc2, dacc3, sh);
+    publish_partial(dacc, partials);
+    publish_partial(dacc2, partials + kMaxPartials);
+    publish_partial(dacc3, partials + 2 * kMaxPartials);
+}
+
 template <class T>
 void red_step(pp_ctx *ctx, unsigned np, int parity, bool publish, uint32_t n, const T *w, T *p, T *s, T *x, T *r) {
     // one 1024-thread workgroup per CU: each sums the partials once
@@ -1859,6 +1997,66 @@ template void red_step<double>(pp_ctx *, unsigned, int, bool, uint32_t, const do
                                double *, double *);
 template void red_step<float>(pp_ctx *, unsigned, int, bool, uint32_t, const float *, float *, float *, float *,
                               float *);
+
+bool red_fused_ok(const EllOp &ka, const EllOp &kb, int dtype) {
+    const bool vals = dtype == PP_F64 ? (ka.v64 && kb.v64 && kb.dg64) : (ka.v32 && kb.v32 && kb.dg32);
+    return ka.ok && kb.ok && vals && ka.width >= 4 && ka.width <= 7 && kb.width >= 4 && kb.width <= 7 &&
+           !ka.dg64 && !ka.dg32 && ka.n > 0 && kb.n > 0;
+}
+
+template <class T>
+void red_ka(pp_ctx *ctx, const EllOp &E, const T *z, T *y, uint32_t rbase, const T *w, const T *s, unsigned np,
+            int parity, bool publish) {
+    const unsigned want = (E.n + kFinishThreads - 1) / kFinishThreads;
+    unsigned g = std::max(1u, std::min<unsigned>((unsigned)ctx->cu_count, want));
+    if (g >= kXcds) g -= g % kXcds;
+    const T *val = sizeof(T) == 8 ? (const T *)E.v64 : (const T *)E.v32;
+    double *part = ctx->red.partials + (size_t)S_CG_DELTA * kMaxPartials;
+    int *ring = publish ? ctx->ring_dev : nullptr;
+#define PP_RKA(W, P)                                                                                              \
+    k_red_ka<T, W, P><<<g, kFinishThreads, 0, ctx->stream>>>(E.n, E.off, E.pid, E.ptab, val, E.cbase, z, y, rbase, w, \
+                                                            s, part, np, ctx->scal, ctx->ctl, ring, parity)
+#define PP_RKA_P(W) PP_RKA(W, true)
+#define PP_RKA_N(W) PP_RKA(W, false)
+    if (E.pid) {
+        PP_ELL_PAT_WIDTHS(E.width, PP_RKA_P)
+    } else {
+        PP_ELL_PAT_WIDTHS(E.width, PP_RKA_N)
+    }
+#undef PP_RKA_P
+#undef PP_RKA_N
+#undef PP_RKA
+    PP_HIP(hipGetLastError());
+}
+
+template <class T>
+void red_kb(pp_ctx *ctx, const EllOp &E, T *z, uint32_t rbase, T *w, T *p, T *s, T *x, const T *wt, unsigned grid) {
+    const T *val = sizeof(T) == 8 ? (const T *)E.v64 : (const T *)E.v32;
+    const T *dg = sizeof(T) == 8 ? (const T *)E.dg64 : (const T *)E.dg32;
+    double *part = ctx->red.partials + (size_t)S_CG_DELTA * kMaxPartials;
+#define PP_RKB(W, P)                                                                                              \
+    k_red_kb<T, W, P><<<grid, kBlock, 0, ctx->stream>>>(E.n, E.off, E.pid, E.ptab, val, E.cbase, dg, z, rbase, w, p, \
+                                                       s, x, wt, ctx->scal, part, ctx->stop_word)
+#define PP_RKB_P(W) PP_RKB(W, true)
+#define PP_RKB_N(W) PP_RKB(W, false)
+    if (E.pid) {
+        PP_ELL_PAT_WIDTHS(E.width, PP_RKB_P)
+    } else {
+        PP_ELL_PAT_WIDTHS(E.width, PP_RKB_N)
+    }
+#undef PP_RKB_P
+#undef PP_RKB_N
+#undef PP_RKB
+    PP_HIP(hipGetLastError());
+}
+template void red_ka<double>(pp_ctx *, const EllOp &, const double *, double *, uint32_t, const double *,
+                             const double *, unsigned, int, bool);
+template void red_ka<float>(pp_ctx *, const EllOp &, const float *, float *, uint32_t, const float *, const float *,
+                            unsigned, int, bool);
+template void red_kb<double>(pp_ctx *, const EllOp &, double *, uint32_t, double *, double *, double *, double *,
+                             const double *, unsigned);
+template void red_kb<float>(pp_ctx *, const EllOp &, float *, uint32_t, float *, float *, float *, float *,
+                            const float *, unsigned);
 
 __global__ void k_red_init(double *scal, int resume, double gscale, int last) {
     double *st = scal + S_RED0;

@@ -335,12 +335,26 @@ PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, do
     // KA and KB in sliced-column form where they were built so, else CSR
     const bool ell = R->ka.ok && R->kb.ok;
     const unsigned np = ell ? ell_grid(R->kb, true) : spmv_grid(R->KB);
-    // the fused finish-and-update alternates its state blocks: an even
-    // number of iterations per body (the device still tests every one)
+    // two launches per iteration where KA and KB allow it: the first
+    // iteration's products here, then each body iteration's k_red_ka checks
+    // the previous one and forms the next KA product with the update folded
+    // in, k_red_kb applies the update and forms KB with its inner products
+    const bool fused = ell && red_fused_ok(R->ka, R->kb, DT);
+    if (fused) {
+        launch_ell(ctx, R->ka, DT, 1.0, z, 0.0, z, -1, nullptr, nullptr, nullptr);
+        launch_ell(ctx, R->kb, DT, 1.0, z, 0.0, w, S_CG_DELTA, r, r, wt);
+    }
+    // the check alternates its state blocks: an even number of iterations
+    // per body (the device still tests every one)
     const uint32_t every = std::max<uint32_t>(2, check_every + (check_every & 1));
     auto body = [&] {
         StopScope scope(ctx);
         for (uint32_t k = 0; k < every; ++k) {
+            if (fused) {
+                red_ka<T>(ctx, R->ka, z, z, n0, w, s, np, (int)(k & 1), k + 1 == every);
+                red_kb<T>(ctx, R->kb, z, n0, w, p, s, x, wt, np);
+                continue;
+            }
             if (ell) {
                 launch_ell(ctx, R->ka, DT, 1.0, z, 0.0, z, -1, nullptr, nullptr, nullptr);  // z[0, n0) = B r
                 launch_ell(ctx, R->kb, DT, 1.0, z, 0.0, w, S_CG_DELTA, r, r, wt);  // w = Sb r, (r,w), (r,r)
@@ -364,7 +378,8 @@ PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, do
                                        (uint64_t)p,
                                        (uint64_t)s,
                                        (uint64_t)x,
-                                       variant_key(R->KA, nullptr) << 16 | variant_key(R->KB, nullptr)};
+                                       variant_key(R->KA, nullptr) << 16 | variant_key(R->KB, nullptr),
+                                       (uint64_t)fused};
     drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false, key);
     PcgResult res = collect_pcg(ctx, ctl, maxit, false);
     red_post<T>(ctx, n0, n1, x, s1, rin, z, e);

@@ -1887,42 +1887,60 @@ __global__ __launch_bounds__(1024) void k_red_ka(uint32_t n0, const int16_t *__r
     // XCD-aware rows: the workgroups of XCD q (workgroup b runs on XCD b % 8)
     // take the contiguous eighth q of the rows, so the gathered neighbours
     // stay in that XCD's L2
-    uint32_t lo = 0, hi = n0, stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t hi = n0, stride = gridDim.x * blockDim.x, i0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (gridDim.x % kXcds == 0) {
         const uint32_t q = blockIdx.x % kXcds;
-        lo = (uint32_t)((uint64_t)n0 * q / kXcds);
+        const uint32_t lo = (uint32_t)((uint64_t)n0 * q / kXcds);
         hi = (uint32_t)((uint64_t)n0 * (q + 1) / kXcds);
         stride = gridDim.x / kXcds * blockDim.x;
         i0 = lo + blockIdx.x / kXcds * blockDim.x + threadIdx.x;
     }
-    int16_t o[W];
-    T v[W];
-    auto load_row = [&](uint32_t i) {
+    // the thread's first kU rows: matrix entries and every gathered r, w, s
+    // loaded before the check (none of them depends on alpha / beta), so
+    // the check's latency overlaps them (one row for the wide FP64 forms,
+    // whose two rows would not fit the registers of 1024-thread groups)
+    constexpr int kU = sizeof(T) == 8 && W >= 6 ? 1 : 2;
+    T v[kU][W], gr[kU][W], gw[kU][W], gs[kU][W];
+    auto load_row = [&](uint32_t i, T *vv, T *rr, T *ww, T *ss) {
+        int16_t o[W];
         if constexpr (PAT) ell_pattern<W>(ptab, pid[i], o);
 #pragma unroll
         for (int j = 0; j < W; ++j) {
             const size_t e = (size_t)j * n0 + i;
             if constexpr (!PAT) o[j] = off[e];
-            v[j] = val[e];
+            vv[j] = val[e];
         }
-    };
-    if (i0 < hi) load_row(i0);
-    double a64, b64;
-    if (!red_check(partials, np, scal, ctl, ring, parity, sh, bc, &go, a64, b64)) return;
-    const T alpha = (T)a64, beta = (T)b64;
-    for (uint32_t i = i0; i < hi; i += stride) {
-        if (i != i0) load_row(i);
-        T xv[W];
 #pragma unroll
         for (int j = 0; j < W; ++j) {
             const int64_t g = (int64_t)i + cbase + o[j];  // in z; the second colour starts at rbase
-            const T sn = w[g - rbase] + beta * s[g - rbase];
-            xv[j] = z[g] - alpha * sn;
+            rr[j] = z[g];
+            ww[j] = w[g - rbase];
+            ss[j] = s[g - rbase];
         }
+    };
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+        if (i0 + u * stride < hi) load_row(i0 + u * stride, v[u], gr[u], gw[u], gs[u]);
+    double a64, b64;
+    if (!red_check(partials, np, scal, ctl, ring, parity, sh, bc, &go, a64, b64)) return;
+    const T alpha = (T)a64, beta = (T)b64;
+    auto row = [&](uint32_t i, const T *vv, const T *rr, const T *ww, const T *ss) {
         T sum = 0;
 #pragma unroll
-        for (int j = 0; j < W; ++j) sum += v[j] * xv[j];
+        for (int j = 0; j < W; ++j) {
+            const T sn = ww[j] + beta * ss[j];
+            const T rn = rr[j] - alpha * sn;
+            sum += vv[j] * rn;
+        }
         y[i] = sum;
+    };
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+        if (i0 + u * stride < hi) row(i0 + u * stride, v[u], gr[u], gw[u], gs[u]);
+    for (uint32_t i = i0 + kU * stride; i < hi; i += stride) {
+        T vv[W], rr[W], ww[W], ss[W];
+        load_row(i, vv, rr, ww, ss);
+        row(i, vv, rr, ww, ss);
     }
 }
 
@@ -2007,6 +2025,8 @@ bool red_fused_ok(const EllOp &ka, const EllOp &kb, int dtype) {
 template <class T>
 void red_ka(pp_ctx *ctx, const EllOp &E, const T *z, T *y, uint32_t rbase, const T *w, const T *s, unsigned np,
             int parity, bool publish) {
+    // one 1024-thread workgroup per CU (each sums the partials once; all
+    // resident at once, so the check runs in one round)
     const unsigned want = (E.n + kFinishThreads - 1) / kFinishThreads;
     unsigned g = std::max(1u, std::min<unsigned>((unsigned)ctx->cu_count, want));
     if (g >= kXcds) g -= g % kXcds;

@@ -32,27 +32,33 @@ double *gmres_coeff_buffer(pp_ctx *ctx) {
 // column is summed in the same order as a column-at-a-time pass (the
 // thread's rows in stride order, then the wave and the four waves), so the
 // result does not depend on the grouping.
-constexpr int kGemvGroup = 16;
+template <int NG>
 __global__ __launch_bounds__(256) void k_gemv_t_partial(uint32_t n, uint32_t ncol, const double *__restrict__ V,
                                                         size_t ld, const double *__restrict__ w, double *partial,
                                                         const int *stop) {
-    __shared__ double sh[kGemvGroup][4];
+    __shared__ double sh[NG][4];
     if (stop && *stop) return;
     const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const uint32_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
-    for (uint32_t c0 = 0; c0 < ncol; c0 += kGemvGroup) {
-        const uint32_t nc = min((uint32_t)kGemvGroup, ncol - c0);
-        double acc[kGemvGroup];
+    for (uint32_t c0 = 0; c0 < ncol; c0 += NG) {
+        const uint32_t nc = min((uint32_t)NG, ncol - c0);
+        double acc[NG];
 #pragma unroll
-        for (int c = 0; c < kGemvGroup; ++c) acc[c] = 0.0;
+        for (int c = 0; c < NG; ++c) acc[c] = 0.0;
         for (uint32_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
             const double wr = w[r];
+            // every slot's load is unconditional (slots past the group's
+            // last column reread that column, a cache hit, and their sums are
+            // never used), so the loads issue back to back instead of one
+            // behind each branch
+            double vv[NG];
 #pragma unroll
-            for (int c = 0; c < kGemvGroup; ++c)
-                if ((uint32_t)c < nc) acc[c] += V[ld * (c0 + c) + r] * wr;
+            for (int c = 0; c < NG; ++c) vv[c] = V[ld * (c0 + min((uint32_t)c, nc - 1)) + r];
+#pragma unroll
+            for (int c = 0; c < NG; ++c) acc[c] += vv[c] * wr;
         }
 #pragma unroll
-        for (int c = 0; c < kGemvGroup; ++c)
+        for (int c = 0; c < NG; ++c)
             if ((uint32_t)c < nc) {
                 const double v = wsum(acc[c]);
                 if ((threadIdx.x & 63) == 0) sh[c][threadIdx.x >> 6] = v;
@@ -84,7 +90,11 @@ void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size
     PP_CHECK(ncol <= kMaxCols, PP_ERR_UNSUPPORTED, "too many columns");
     double *buf = gmres_coeff_buffer(ctx);
     double *partial = buf + 640;
-    k_gemv_t_partial<<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
+    // the column group no wider than the columns (4, 8 or 16 slots)
+    if (ncol <= 4) k_gemv_t_partial<4><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
+    else if (ncol <= 8)
+        k_gemv_t_partial<8><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
+    else k_gemv_t_partial<16><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
     k_gemv_t_reduce<<<ncol, 256, 0, ctx->stream>>>(kGemvBlocks, partial, dest ? dest : buf, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
@@ -94,9 +104,19 @@ __global__ __launch_bounds__(256) void k_gemv_sub(uint32_t n, uint32_t ncol, con
                                                   size_t ld, const double *__restrict__ h, double *__restrict__ w,
                                                   const int *stop) {
     if (stop && *stop) return;
+    constexpr int kU = 4;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
         double acc = 0.0;
-        for (uint32_t c = 0; c < ncol; ++c) acc += V[ld * c + r] * h[c];
+        // the columns four at a time: the four loads first (past the last
+        // column they reread it), then the products in column order
+        for (uint32_t c0 = 0; c0 < ncol; c0 += kU) {
+            double vv[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) vv[u] = V[ld * min(c0 + u, ncol - 1) + r];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (c0 + u < ncol) acc += vv[u] * h[c0 + u];
+        }
         w[r] -= acc;
     }
 }
@@ -115,33 +135,35 @@ void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, si
 // fewer.  Each thread keeps its row's ncol basis entries in registers
 // between the two uses.
 constexpr int kFuseCols = 32;
+template <int NC>
 __global__ __launch_bounds__(256) void k_gemv_sub_t(uint32_t n, uint32_t ncol, const double *__restrict__ V,
                                                     size_t ld, const double *__restrict__ h,
                                                     double *__restrict__ w, double *partial, const int *stop) {
-    __shared__ double sh[kFuseCols][4];
+    __shared__ double sh[NC][4];
     if (stop && *stop) return;
     const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const uint32_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
-    double acc[kFuseCols];
+    double acc[NC];
 #pragma unroll
-    for (int c = 0; c < kFuseCols; ++c) acc[c] = 0.0;
+    for (int c = 0; c < NC; ++c) acc[c] = 0.0;
     for (uint32_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        double v[kFuseCols];
+        double v[NC];
         double a = 0.0;
+        // loads unconditional (past the last column they reread it), so they
+        // issue back to back; the products only over the real columns
 #pragma unroll
-        for (int c = 0; c < kFuseCols; ++c)
-            if ((uint32_t)c < ncol) {
-                v[c] = V[ld * c + r];
-                a += v[c] * h[c];
-            }
+        for (int c = 0; c < NC; ++c) v[c] = V[ld * min((uint32_t)c, ncol - 1) + r];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if ((uint32_t)c < ncol) a += v[c] * h[c];
         const double wr = w[r] - a;
         w[r] = wr;
 #pragma unroll
-        for (int c = 0; c < kFuseCols; ++c)
+        for (int c = 0; c < NC; ++c)
             if ((uint32_t)c < ncol) acc[c] += v[c] * wr;
     }
 #pragma unroll
-    for (int c = 0; c < kFuseCols; ++c)
+    for (int c = 0; c < NC; ++c)
         if ((uint32_t)c < ncol) {
             const double t = wsum(acc[c]);
             if ((threadIdx.x & 63) == 0) sh[c][threadIdx.x >> 6] = t;
@@ -162,7 +184,12 @@ void launch_gemv_sub_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, 
     }
     double *buf = gmres_coeff_buffer(ctx);
     double *partial = buf + 640;
-    k_gemv_sub_t<<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
+    // register slots for the columns: 8, 16 or 32
+    if (ncol <= 8)
+        k_gemv_sub_t<8><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
+    else if (ncol <= 16)
+        k_gemv_sub_t<16><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
+    else k_gemv_sub_t<32><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
     k_gemv_t_reduce<<<ncol, 256, 0, ctx->stream>>>(kGemvBlocks, partial, dest ? dest : buf, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }

@@ -504,13 +504,16 @@ void csr_free(pp_csr *A);
 // ---- tall-skinny GEMVs and GMRES helpers (pp_kernels_krylov.hip) ----
 // device coefficient buffer: [0,320) h, [320,640) y or a copy of h, then partials
 double *gmres_coeff_buffer(pp_ctx *ctx);
-// dest[c] = sum_r V[r, c] w[r] for c < ncol (dest NULL: the coefficient buffer)
-void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w,
-                   double *dest);
-void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h, double *w);
+// dest[c] = sum_r V[r, c] w[r] for c < ncol (dest NULL: the coefficient
+// buffer).  V and w in FP64 or FP32 (the mixed GMRES basis); sums in FP64.
+template <class TV>
+void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const TV *V, size_t ld, const TV *w, double *dest);
+template <class TV>
+void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const TV *V, size_t ld, const double *h, TV *w);
 // w -= V h1 followed by dest = V^T w, in one pass over V where it can
-void launch_gemv_sub_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h1,
-                       double *w, double *dest);
+template <class TV>
+void launch_gemv_sub_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const TV *V, size_t ld, const double *h1, TV *w,
+                       double *dest);
 void launch_gemv_add(pp_ctx *ctx, uint32_t n, uint32_t ncol, const float *Z32, const double *Z64, size_t ld,
                      const double *y, double *x);
 void read_slots(pp_ctx *ctx, int first, int count, double *out);
@@ -530,7 +533,8 @@ void launch_gmres_cycle_init(pp_ctx *ctx, const GmresDev &st, double beta, doubl
                              uint32_t hist_base);
 void launch_gmres_step(pp_ctx *ctx, const GmresDev &st, uint32_t j, int quirks, double *hist, uint32_t hist_idx,
                        uint32_t hist_cap);
-void launch_gmres_next_basis(pp_ctx *ctx, const GmresDev &st, uint32_t n, const double *w, double *vnext);
+template <class TV>
+void launch_gmres_next_basis(pp_ctx *ctx, const GmresDev &st, uint32_t n, const TV *w, TV *vnext);
 void launch_gmres_backsolve(pp_ctx *ctx, const GmresDev &st, int quirks, double *y);
 void launch_permute(pp_ctx *ctx, uint32_t n, const uint32_t *perm, int dtype, const void *src, void *dst,
                     bool gather);

@@ -32,9 +32,9 @@ double *gmres_coeff_buffer(pp_ctx *ctx) {
 // column is summed in the same order as a column-at-a-time pass (the
 // thread's rows in stride order, then the wave and the four waves), so the
 // result does not depend on the grouping.
-template <int NG>
-__global__ __launch_bounds__(256) void k_gemv_t_partial(uint32_t n, uint32_t ncol, const double *__restrict__ V,
-                                                        size_t ld, const double *__restrict__ w, double *partial,
+template <int NG, class TV>
+__global__ __launch_bounds__(256) void k_gemv_t_partial(uint32_t n, uint32_t ncol, const TV *__restrict__ V,
+                                                        size_t ld, const TV *__restrict__ w, double *partial,
                                                         const int *stop) {
     __shared__ double sh[NG][4];
     if (stop && *stop) return;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_gemv_t_partial(uint32_t n, uint32_t nco
             // behind each branch
             double vv[NG];
 #pragma unroll
-            for (int c = 0; c < NG; ++c) vv[c] = V[ld * (c0 + min((uint32_t)c, nc - 1)) + r];
+            for (int c = 0; c < NG; ++c) vv[c] = (double)V[ld * (c0 + min((uint32_t)c, nc - 1)) + r];
 #pragma unroll
             for (int c = 0; c < NG; ++c) acc[c] += vv[c] * wr;
         }
@@ -85,23 +85,25 @@ __global__ __launch_bounds__(256) void k_gemv_t_reduce(uint32_t nb, const double
     if (threadIdx.x == 0) h[c] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *w,
-                   double *dest) {
+template <class TV>
+void launch_gemv_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const TV *V, size_t ld, const TV *w, double *dest) {
     PP_CHECK(ncol <= kMaxCols, PP_ERR_UNSUPPORTED, "too many columns");
     double *buf = gmres_coeff_buffer(ctx);
     double *partial = buf + 640;
     // the column group no wider than the columns (4, 8 or 16 slots)
-    if (ncol <= 4) k_gemv_t_partial<4><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
+    if (ncol <= 4)
+        k_gemv_t_partial<4, TV><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
     else if (ncol <= 8)
-        k_gemv_t_partial<8><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
-    else k_gemv_t_partial<16><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
+        k_gemv_t_partial<8, TV><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
+    else k_gemv_t_partial<16, TV><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, w, partial, ctx->stop_word);
     k_gemv_t_reduce<<<ncol, 256, 0, ctx->stream>>>(kGemvBlocks, partial, dest ? dest : buf, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
 
-// w[r] -= sum_c V[r, c] h[c]
-__global__ __launch_bounds__(256) void k_gemv_sub(uint32_t n, uint32_t ncol, const double *__restrict__ V,
-                                                  size_t ld, const double *__restrict__ h, double *__restrict__ w,
+// w[r] -= sum_c V[r, c] h[c] (the sum in FP64 for either storage type)
+template <class TV>
+__global__ __launch_bounds__(256) void k_gemv_sub(uint32_t n, uint32_t ncol, const TV *__restrict__ V,
+                                                  size_t ld, const double *__restrict__ h, TV *__restrict__ w,
                                                   const int *stop) {
     if (stop && *stop) return;
     constexpr int kU = 4;
@@ -112,18 +114,19 @@ __global__ __launch_bounds__(256) void k_gemv_sub(uint32_t n, uint32_t ncol, con
         for (uint32_t c0 = 0; c0 < ncol; c0 += kU) {
             double vv[kU];
 #pragma unroll
-            for (int u = 0; u < kU; ++u) vv[u] = V[ld * min(c0 + u, ncol - 1) + r];
+            for (int u = 0; u < kU; ++u) vv[u] = (double)V[ld * min(c0 + u, ncol - 1) + r];
 #pragma unroll
             for (int u = 0; u < kU; ++u)
                 if (c0 + u < ncol) acc += vv[u] * h[c0 + u];
         }
-        w[r] -= acc;
+        w[r] = (TV)((double)w[r] - acc);
     }
 }
 
-void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h, double *w) {
+template <class TV>
+void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const TV *V, size_t ld, const double *h, TV *w) {
     unsigned g = std::min<uint32_t>((n + 255) / 256, 8192);
-    k_gemv_sub<<<g, 256, 0, ctx->stream>>>(n, ncol, V, ld, h, w, ctx->stop_word);
+    k_gemv_sub<TV><<<g, 256, 0, ctx->stream>>>(n, ncol, V, ld, h, w, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
 
@@ -135,10 +138,10 @@ void launch_gemv_sub(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, si
 // fewer.  Each thread keeps its row's ncol basis entries in registers
 // between the two uses.
 constexpr int kFuseCols = 32;
-template <int NC>
-__global__ __launch_bounds__(256) void k_gemv_sub_t(uint32_t n, uint32_t ncol, const double *__restrict__ V,
+template <int NC, class TV>
+__global__ __launch_bounds__(256) void k_gemv_sub_t(uint32_t n, uint32_t ncol, const TV *__restrict__ V,
                                                     size_t ld, const double *__restrict__ h,
-                                                    double *__restrict__ w, double *partial, const int *stop) {
+                                                    TV *__restrict__ w, double *partial, const int *stop) {
     __shared__ double sh[NC][4];
     if (stop && *stop) return;
     const uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
@@ -152,12 +155,13 @@ __global__ __launch_bounds__(256) void k_gemv_sub_t(uint32_t n, uint32_t ncol, c
         // loads unconditional (past the last column they reread it), so they
         // issue back to back; the products only over the real columns
 #pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] = V[ld * min((uint32_t)c, ncol - 1) + r];
+        for (int c = 0; c < NC; ++c) v[c] = (double)V[ld * min((uint32_t)c, ncol - 1) + r];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
             if ((uint32_t)c < ncol) a += v[c] * h[c];
-        const double wr = w[r] - a;
-        w[r] = wr;
+        const TV wt = (TV)((double)w[r] - a);
+        w[r] = wt;
+        const double wr = (double)wt;
 #pragma unroll
         for (int c = 0; c < NC; ++c)
             if ((uint32_t)c < ncol) acc[c] += v[c] * wr;
@@ -175,8 +179,9 @@ __global__ __launch_bounds__(256) void k_gemv_sub_t(uint32_t n, uint32_t ncol, c
 }
 
 // w -= V h1, then dest = V^T w: fused where the columns fit the registers
-void launch_gemv_sub_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, size_t ld, const double *h1,
-                       double *w, double *dest) {
+template <class TV>
+void launch_gemv_sub_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const TV *V, size_t ld, const double *h1, TV *w,
+                       double *dest) {
     if (ncol > (uint32_t)kFuseCols) {
         launch_gemv_sub(ctx, n, ncol, V, ld, h1, w);
         launch_gemv_t(ctx, n, ncol, V, ld, w, dest);
@@ -186,13 +191,22 @@ void launch_gemv_sub_t(pp_ctx *ctx, uint32_t n, uint32_t ncol, const double *V, 
     double *partial = buf + 640;
     // register slots for the columns: 8, 16 or 32
     if (ncol <= 8)
-        k_gemv_sub_t<8><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
+        k_gemv_sub_t<8, TV><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
     else if (ncol <= 16)
-        k_gemv_sub_t<16><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
-    else k_gemv_sub_t<32><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
+        k_gemv_sub_t<16, TV><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
+    else k_gemv_sub_t<32, TV><<<kGemvBlocks, 256, 0, ctx->stream>>>(n, ncol, V, ld, h1, w, partial, ctx->stop_word);
     k_gemv_t_reduce<<<ncol, 256, 0, ctx->stream>>>(kGemvBlocks, partial, dest ? dest : buf, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
+template void launch_gemv_t<double>(pp_ctx *, uint32_t, uint32_t, const double *, size_t, const double *, double *);
+template void launch_gemv_t<float>(pp_ctx *, uint32_t, uint32_t, const float *, size_t, const float *, double *);
+template void launch_gemv_sub<double>(pp_ctx *, uint32_t, uint32_t, const double *, size_t, const double *,
+                                      double *);
+template void launch_gemv_sub<float>(pp_ctx *, uint32_t, uint32_t, const float *, size_t, const double *, float *);
+template void launch_gemv_sub_t<double>(pp_ctx *, uint32_t, uint32_t, const double *, size_t, const double *,
+                                        double *, double *);
+template void launch_gemv_sub_t<float>(pp_ctx *, uint32_t, uint32_t, const float *, size_t, const double *, float *,
+                                       double *);
 
 
 // x[r] += sum_c Z[r, c] y[c] (Z in FP32 or FP64), columns in ascending order
@@ -362,15 +376,16 @@ __global__ void k_gmres_step(GmresDev st, uint32_t j, int quirks, const double *
 }
 
 // v_{j+1} = (1/h_{j+1,j}) w, evaluated like HostVector::add with b = 0
-__global__ void k_gmres_next_basis(uint32_t n, const double *misc, const double *__restrict__ w,
-                                   double *__restrict__ v, const int *stop) {
+template <class TV>
+__global__ void k_gmres_next_basis(uint32_t n, const double *misc, const TV *__restrict__ w, TV *__restrict__ v,
+                                   const int *stop) {
 #pragma clang fp contract(off)
     if (stop && *stop) return;
     const double a = misc[GM_INVH];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double t1 = a * w[i];
-        const double t2 = 0.0 * w[i];
-        v[i] = t1 + t2;
+        const double t1 = a * (double)w[i];
+        const double t2 = 0.0 * (double)w[i];
+        v[i] = (TV)(t1 + t2);
     }
 }
 
@@ -402,11 +417,14 @@ void launch_gmres_step(pp_ctx *ctx, const GmresDev &st, uint32_t j, int quirks, 
     PP_HIP(hipGetLastError());
 }
 
-void launch_gmres_next_basis(pp_ctx *ctx, const GmresDev &st, uint32_t n, const double *w, double *vnext) {
+template <class TV>
+void launch_gmres_next_basis(pp_ctx *ctx, const GmresDev &st, uint32_t n, const TV *w, TV *vnext) {
     const unsigned g = std::min<uint32_t>((n + 255) / 256, 4096);
-    k_gmres_next_basis<<<g, 256, 0, ctx->stream>>>(n, st.misc, w, vnext, ctx->stop_word);
+    k_gmres_next_basis<TV><<<g, 256, 0, ctx->stream>>>(n, st.misc, w, vnext, ctx->stop_word);
     PP_HIP(hipGetLastError());
 }
+template void launch_gmres_next_basis<double>(pp_ctx *, const GmresDev &, uint32_t, const double *, double *);
+template void launch_gmres_next_basis<float>(pp_ctx *, const GmresDev &, uint32_t, const float *, float *);
 
 void launch_gmres_backsolve(pp_ctx *ctx, const GmresDev &st, int quirks, double *y) {
     k_gmres_backsolve<<<1, 64, 0, ctx->stream>>>(st, quirks, y);

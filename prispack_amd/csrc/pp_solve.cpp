@@ -596,19 +596,28 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
     PP_CHECK(mm >= 1, PP_ERR_INVALID, "GMRES needs n >= 2");
     PP_CHECK(mm <= 256, PP_ERR_UNSUPPORTED, "restart > 256");
     const size_t ld = n;
-    double *V = (double *)ws_get(ctx, W_VB, ld * (mm + 1) * sizeof(double));
-    double *Z = nullptr;
-    float *Z32 = nullptr;
+    // mixed: the basis V, the preconditioned vectors Z and w in FP32 (half
+    // the bytes of the orthogonalisation, which dominates a cycle), the CGS2
+    // sums, the Hessenberg column and the least squares in FP64; every
+    // restart forms the true residual in FP64, which makes it the
+    // refinement step
+    double *V = nullptr, *Z = nullptr;
+    float *V32 = nullptr, *Z32 = nullptr;
     if (mixed) {
         csr_need_f32(ctx, A);
         if (M) precond_need_f32(M);
+        V32 = (float *)ws_get(ctx, W_VB, ld * (mm + 1) * sizeof(float));
+        launch_fill_zero(ctx, V32, ld * (mm + 1) * sizeof(float));
         Z32 = (float *)ws_get(ctx, W_ZB32, ld * (mm + 1) * sizeof(float));
         launch_fill_zero(ctx, Z32, ld * (mm + 1) * sizeof(float));
-    } else if (flexible) {
-        Z = (double *)ws_get(ctx, W_ZB, ld * (mm + 1) * sizeof(double));
-        launch_fill_zero(ctx, Z, ld * (mm + 1) * sizeof(double));
+    } else {
+        V = (double *)ws_get(ctx, W_VB, ld * (mm + 1) * sizeof(double));
+        launch_fill_zero(ctx, V, ld * (mm + 1) * sizeof(double));
+        if (flexible) {
+            Z = (double *)ws_get(ctx, W_ZB, ld * (mm + 1) * sizeof(double));
+            launch_fill_zero(ctx, Z, ld * (mm + 1) * sizeof(double));
+        }
     }
-    launch_fill_zero(ctx, V, ld * (mm + 1) * sizeof(double));
     double *w = (double *)ws_get(ctx, W_W, (size_t)n * sizeof(double));
     double *r = (double *)ws_get(ctx, W_R, (size_t)n * sizeof(double));
     float *w32 = mixed ? (float *)ws_get(ctx, W_W32, (size_t)n * sizeof(float)) : nullptr;
@@ -653,19 +662,27 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
         // (k_gmres_step) and the stop word ends the cycle's remaining kernels.
         ctl_init(ctx, 1, 0);
         launch_gmres_cycle_init(ctx, st, beta, bnorm, o.tol, nrec);
-        launch_axpby(ctx, PP_F64, n, 1.0 / beta, r, 0.0, r, V);  // v0 = r / beta
+        if (mixed) launch_scale_convert(ctx, n, r, 1.0 / beta, V32);  // v0 = r / beta
+        else launch_axpby(ctx, PP_F64, n, 1.0 / beta, r, 0.0, r, V);
         auto cycle = [&] {
             StopScope scope(ctx);
             for (uint32_t j = 0; j < m; ++j) {
+                if (mixed) {
+                    // z_j = M^{-1} v_j ; w = A z_j ; CGS2 and the next basis vector on the FP32 basis
+                    float *vj = V32 + ld * j, *zj = Z32 + ld * j;
+                    if (flexible && M) precond_apply(ctx, M, PP_F32, vj, zj);
+                    launch_spmv(ctx, A, PP_F32, 1.0, flexible && M ? zj : vj, 0.0, w32, -1, nullptr, false);
+                    launch_gemv_t(ctx, n, j + 1, V32, ld, w32, st.h1);
+                    launch_gemv_sub_t(ctx, n, j + 1, V32, ld, st.h1, w32, st.h2);
+                    launch_gemv_sub(ctx, n, j + 1, V32, ld, st.h2, w32);
+                    launch_dot(ctx, PP_F32, n, w32, w32, S_TMP2);
+                    launch_gmres_step(ctx, st, j, quirks, hist_on ? ctx->hist : nullptr, j, hist_cap);
+                    launch_gmres_next_basis(ctx, st, n, w32, V32 + ld * (j + 1));
+                    continue;
+                }
                 double *vj = V + ld * j;
                 // z_j = M^{-1} v_j ; w = A z_j
-                if (mixed) {
-                    float *zj = Z32 + ld * j;
-                    launch_convert(ctx, n, PP_F64, vj, PP_F32, zj);
-                    if (flexible && M) precond_apply(ctx, M, PP_F32, zj, zj);
-                    launch_spmv(ctx, A, PP_F32, 1.0, zj, 0.0, w32, -1, nullptr, false);
-                    launch_convert(ctx, n, PP_F32, w32, PP_F64, w);
-                } else if (flexible) {
+                if (flexible) {
                     double *zj = Z + ld * j;
                     if (M) precond_apply(ctx, M, PP_F64, vj, zj);
                     else launch_convert(ctx, n, PP_F64, vj, PP_F64, zj);
@@ -688,7 +705,8 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
             const std::vector<uint64_t> key = {4, m, (uint64_t)mixed, (uint64_t)flexible, (uint64_t)quirks, A->uid,
                                                (uint64_t)A->v32, (uint64_t)A->spmv_variant << 8 | A->spmv_variant32,
                                                M ? M->uid : 0, M ? (uint64_t)M->io_permuted : 0,
-                                               M ? (uint64_t)M->have32 : 0, (uint64_t)V, (uint64_t)Z, (uint64_t)Z32,
+                                               M ? (uint64_t)M->have32 : 0, (uint64_t)V, (uint64_t)V32,
+                                               (uint64_t)Z, (uint64_t)Z32,
                                                (uint64_t)w, (uint64_t)w32, (uint64_t)st.misc,
                                                (uint64_t)(hist_on ? ctx->hist : nullptr), hist_cap, ld,
                                                variant_key(A, M)};
@@ -718,7 +736,8 @@ PcgResult gmres_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const pp_solve_opt
         // residual estimate, left in place
         double *y = hdev + 320;
         launch_gmres_backsolve(ctx, st, quirks, y);
-        launch_gemv_add(ctx, n, mm + 1, Z32, mixed ? nullptr : (flexible ? Z : V), ld, y, x);
+        launch_gemv_add(ctx, n, mm + 1, mixed ? (flexible && M ? Z32 : V32) : nullptr,
+                        mixed ? nullptr : (flexible ? Z : V), ld, y, x);
         conv = relerr < o.tol;
         PP_HIP(hipMemcpyAsync(r, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         launch_spmv(ctx, A, PP_F64, -1.0, x, 1.0, r, -1, nullptr, false);

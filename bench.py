@@ -57,13 +57,13 @@ def parse():
     return ap.parse_args()
 
 
-# HBM bytes per launch of k_spmv<double> read from two rocprofv3 --pmc passes
+# HBM bytes per launch of the FP64 SpMV read from two rocprofv3 --pmc passes
 # (FETCH_SIZE, WRITE_SIZE, KiB per dispatch) over this script, each scaled by
-# what the same counter reads for a device-to-device copy of a known size
-# (--copy-probe).  Keyed by (grid side, SpMV storage variant); the command and
-# the raw per-dispatch figures are in profiles/k_spmv_poisson128_pmc.md.
+# what the same counter reads for the streaming copy of a known size
+# (--copy-probe).  Keyed by (grid side, SpMV storage variant); the commands and
+# the raw per-dispatch figures are in profiles/spmv_ell2_poisson128_pmc.md.
 PMC_TRAFFIC = {(128, v): b for v, b in enumerate((224627870, 226711825, 194751552, 197276754, 219112328, 229056082,
-                                                        189937730, 192875293, 154991311))}
+                                                        189937730, 192875293, 154997760))}
 
 
 ELL_VARIANT = 8  # the SpMV variant that runs the sliced-column form
@@ -385,6 +385,34 @@ def main():
             f"({repmx.outer_iters} outer / {repmx.inner_iters} inner, {inner_variant}); same iteration in fp64 "
             f"{same['fp64_same'][0]:.4f} s ({same['fp64_same'][1].iters} it); fp64 single-reduction "
             f"{np.median(tsr):.4f} s ({repsr.iters} it); mixed with single-reduction inner {same['mixed_sr'][0]:.4f} s")
+        # new values on the same pattern (A scaled by 1.5, then 2): the
+        # device re-factorization (the first call also builds its symbolic
+        # maps on the host) and the next mixed solve, whose reduced form is
+        # refreshed on the device; then the original values again
+        v0 = np.asarray(A[2], np.float64)
+        resetup = {}
+        for scale in (1.5, 2.0):
+            M.set_values(v0 * scale)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            P.refactor()
+            torch.cuda.synchronize()
+            t_ref = time.perf_counter() - t
+            dt_s, rp_s, _ = solve("mixed")
+            resetup[scale] = (t_ref, dt_s, rp_s)
+        M.set_values(v0)
+        P.refactor()
+        resetup_rec = {"workload": "the same matrix with its values scaled (1.5, then 2): pp_precond_refactor, "
+                                   "then the first mixed solve on the new values (its reduced form refreshed on "
+                                   "the device)",
+                       "first_refactor_s": round(resetup[1.5][0], 4),
+                       "refactor_s": round(resetup[2.0][0], 4), "first_mixed_solve_s": round(resetup[2.0][1], 4),
+                       "total_s": round(resetup[2.0][0] + resetup[2.0][1], 4),
+                       "inner_iters": resetup[2.0][2].inner_iters,
+                       "converged": bool(resetup[2.0][2].converged)}
+        log(f"[bench] new values: refactor {resetup_rec['refactor_s'] * 1e3:.1f} ms (first "
+            f"{resetup_rec['first_refactor_s'] * 1e3:.0f} ms, maps built) + first mixed solve "
+            f"{resetup_rec['first_mixed_solve_s'] * 1e3:.1f} ms")
 
     # ---- kernel throughput on the library's stream
     stream = L.pp_ctx_stream(ctx.h)
@@ -486,6 +514,12 @@ def main():
                 f"{sorted(copy_by_groups)} workgroups: {copy_by_groups}",
                 "torch_copy_GBs": round(torch_copy_gbs, 1),
                 "frac_of_copy": round(gbs(alg_spmv, spmv_ms) / copy_gbs, 4),
+                # the same launch priced on the HBM bytes the counters measured
+                # (the sliced-column form stores one pattern byte per row, so it
+                # moves fewer bytes than the 32-bit-index algorithmic figure)
+                "traffic_GBs": round(gbs(traffic, spmv_ms), 1) if traffic else None,
+                "traffic_frac": round(gbs(traffic, spmv_ms) / HBM_PEAK_GBS, 4) if traffic else None,
+                "traffic_frac_of_copy": round(gbs(traffic, spmv_ms) / copy_gbs, 4) if traffic else None,
                 "streamed_GBs": round(gbs(spmv_bytes, spmv_ms), 1), "variant": spmv_var.value}
     apply_rec = {"kernel": f"IC(0) apply = forward + backward sweep ({sched} schedule, {info.levels_lower} levels)",
                  "bound": "hbm", "achieved": round(gbs(alg_apply, trsv_ms), 1), "peak": HBM_PEAK_GBS,
@@ -550,13 +584,17 @@ def main():
         "levels": [info.levels_lower, info.levels_upper],
         "setup_phases_ms": [round(float(v), 1) for v in info.phase_ms],
         "first_mixed_solve_s": None if first_mixed_s is None else round(first_mixed_s, 4),
+        "new_values": resetup_rec,
         "roofline": dominant,
         "roofline_apply": apply_rec,
-        "roofline_spmv_f32": {"kernel": "k_ell<float>" if spmv32_var.value == ELL_VARIANT else "k_spmv<float>",
+        "roofline_spmv_f32": {"kernel": ("k_ell2<float> (sliced-column, row patterns)" if spmv32_var.value == ELL_VARIANT
+                                         else "k_spmv<float>"),
                               "variant": spmv32_var.value, "achieved": round(gbs(alg_spmv32, spmv32_ms), 1),
                               "unit": "GB/s", "frac": round(gbs(alg_spmv32, spmv32_ms) / HBM_PEAK_GBS, 4),
                               "algorithmic_bytes": alg_spmv32, "avg_us": round(spmv32_ms * 1e3, 2),
-                              "bytes_streamed": spmv32_bytes},
+                              "bytes_streamed": spmv32_bytes,
+                              "streamed_GBs": round(gbs(spmv32_bytes, spmv32_ms), 1),
+                              "streamed_frac": round(gbs(spmv32_bytes, spmv32_ms) / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "secondary": extra,
     }

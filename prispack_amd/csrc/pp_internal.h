@@ -372,6 +372,8 @@ struct ReducedForm {
     double anorm = 0.0;           // max absolute row sum of the colour-ordered matrix
     uint64_t src_uid = 0;  // built from the matrix of this uid at src_version
     uint64_t src_version = 0;
+    int *dfail = nullptr;     // device word of reduced_refresh's pivot test
+    double *dwork = nullptr;  // its diagonal of the matrix (n)
 };
 // Host arrays of the reduced form of one colour-ordered matrix: KA (n0
 // rows), KB and KF (n1 rows), KE (n0 rows), columns in the matrix's
@@ -389,6 +391,10 @@ bool reduced_host(uint32_t n, const std::vector<uint32_t> &rp, const std::vector
 // The reduced form of M for the colour-ordered matrix As (built once and
 // cached in M), or nullptr when it does not apply.
 ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As);
+// New values of an existing reduced form from As's current values (same
+// pattern as it was built from), on the device, bit-identical to a host
+// rebuild; false when a pivot is not positive (pp_reduced_dev.hip).
+bool reduced_refresh(pp_ctx *ctx, ReducedForm *R, pp_csr *As);
 void reduced_need_f32(pp_ctx *ctx, ReducedForm *R);
 void reduced_free(pp_precond *M);
 }  // namespace pp
@@ -598,6 +604,8 @@ unsigned ell_grid(const EllOp &E, bool dot);
 template <class T>
 void ell_fill_rows(uint32_t n, uint32_t width, const uint32_t *rp, const T *v, T *out);
 // E's values (dtype) from the CSR values v of the matrix it was built from
+// (a form with a separate diagonal takes each row's last entry from dg, not
+// from the slots)
 void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp, const void *v);
 // host form -> EllOp (FP64 values on the device); false (nothing kept) when
 // a row is empty or an offset does not fit 16 bits

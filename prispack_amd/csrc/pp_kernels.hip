@@ -590,12 +590,12 @@ void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void
 
 template <class T>
 __global__ void k_ell_fill(uint32_t n, uint32_t width, const uint32_t *__restrict__ rp, const T *__restrict__ v,
-                           T *__restrict__ out) {
+                           T *__restrict__ out, uint32_t tail) {
     const size_t total = (size_t)width * n;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
         const uint32_t j = (uint32_t)(e / n), i = (uint32_t)(e - (size_t)j * n);
         const uint32_t k = rp[i] + j;
-        out[e] = k < rp[i + 1] ? v[k] : (T)0;
+        out[e] = k < rp[i + 1] - tail ? v[k] : (T)0;
     }
 }
 
@@ -605,7 +605,7 @@ template <class T>
 void ell_fill_rows(uint32_t n, uint32_t width, const uint32_t *rp, const T *v, T *out) {
     const uint64_t total = (uint64_t)width * n;
     const unsigned g = (unsigned)std::min<uint64_t>((total + kBlock * 4 - 1) / (kBlock * 4), 8192);
-    k_ell_fill<T><<<std::max(1u, g), kBlock, 0, 0>>>(n, width, rp, v, out);
+    k_ell_fill<T><<<std::max(1u, g), kBlock, 0, 0>>>(n, width, rp, v, out, 0u);
     PP_HIP(hipGetLastError());
     PP_HIP(hipStreamSynchronize(0));
 }
@@ -613,11 +613,13 @@ template void ell_fill_rows<double>(uint32_t, uint32_t, const uint32_t *, const 
 template void ell_fill_rows<float>(uint32_t, uint32_t, const uint32_t *, const float *, float *);
 
 void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp, const void *v) {
+    // a form with a separate diagonal (dg) leaves each row's last entry out of the slots
+    const uint32_t tail = E.dg64 ? 1 : 0;
     const unsigned g = grid_for(ctx, (uint64_t)E.width * E.n, 4);
     if (dtype == PP_F64)
-        k_ell_fill<double><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const double *)v, E.v64);
+        k_ell_fill<double><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const double *)v, E.v64, tail);
     else
-        k_ell_fill<float><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const float *)v, E.v32);
+        k_ell_fill<float><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const float *)v, E.v32, tail);
     PP_HIP(hipGetLastError());
 }
 

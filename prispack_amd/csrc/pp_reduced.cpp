@@ -8,9 +8,7 @@
 #include <cmath>
 #include <memory>
 #include <mutex>
-
-#include <array>
-#include <map>
+#include <unordered_map>
 
 #include "pp_internal.h"
 
@@ -307,6 +305,12 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
     // keyed on the matrix's uid (never reused, unlike an address) and its value version
     if (M->reduced && M->reduced->src_uid == As->uid && M->reduced->src_version == As->version)
         return M->reduced->ok ? M->reduced : nullptr;
+    // the same matrix with new values on its pattern (a re-factorization):
+    // the operators' values recomputed in place on the device
+    if (M->reduced && M->reduced->ok && M->reduced->src_uid == As->uid && reduced_refresh(ctx, M->reduced, As)) {
+        M->reduced->src_version = As->version;
+        return M->reduced;
+    }
     reduced_free(M);
     std::unique_ptr<ReducedForm> R(new ReducedForm);
     R->src_uid = As->uid;
@@ -394,6 +398,8 @@ void reduced_free(pp_precond *M) {
     dfree(R->dt_32);
     R->ka.free_all();
     R->kb.free_all();
+    dfree(R->dfail);
+    dfree(R->dwork);
     delete R;
     M->reduced = nullptr;
 }

@@ -147,3 +147,60 @@ def test_refactor_at_128_cubed(pp, torch):
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         cg.solve(M, torch.tensor(b, device="cuda"), x)
         assert cg.getConvergence() and cg.report.true_relres <= 1e-8, prec
+    # once the reduced form exists, new values cost a device re-factorization
+    # and a device refresh of the reduced form inside the next mixed solve
+    M.set_values(np.asarray(A[2], np.float64) * 3.0)
+    b3 = ppo.spmv((A[0], A[1], np.asarray(A[2]) * 3.0), ppo.normal(n, 0))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    P.refactor()
+    cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision="mixed")
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    cg.solve(M, torch.tensor(b3, device="cuda"), x)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t
+    print(f"[refactor] 128^3: refactor + first mixed solve on new values {total * 1e3:.1f} ms", flush=True)
+    assert cg.getConvergence() and cg.report.true_relres <= 1e-8
+
+
+def test_refactor_refreshes_the_reduced_form(pp, torch):
+    # a mixed solve builds the reduced form of the two-colour IC(0); after new
+    # values and a device re-factorization the next mixed solve recomputes
+    # that form's values on the device: iterations and x equal, bit for bit,
+    # a mixed solve through a preconditioner set up from scratch on the new
+    # values (whose reduced form is built on the host)
+    # 64^3: the reduced operators are large enough for their sliced-column
+    # forms (at most 10% padding), so the two-launch iteration runs too
+    g = 64
+    A = ppo.laplacian(g, g, g)
+    n = len(A[0]) - 1
+    rows = np.repeat(np.arange(n), np.diff(A[0]))
+    cols = np.asarray(A[1], np.int64)
+    v1 = np.asarray(A[2], np.float64)
+    rng = np.random.default_rng(11)
+    # new values D A D + I/2 with a positive diagonal D: still SPD, new pivots
+    dsc = 1.0 + 0.3 * rng.uniform(-1, 1, n)
+    v2 = v1 * dsc[rows] * dsc[cols]
+    v2[A[1] == rows] += 0.5
+    b1 = ppo.spmv(A, ppo.normal(n, 2))
+    b2 = ppo.spmv((A[0], A[1], v2), ppo.normal(n, 3))
+    M = pp.CSRMatrix(*A)
+    P = make(pp, M, "ic0", 1, 0)
+
+    def mixed(Pc, Mc, b):
+        cg = pp.CG(n, Pc, pp.FlexibleCG, tol=1e-9, maxIter=3000, precision="mixed")
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        cg.solve(Mc, torch.tensor(b, device="cuda"), x)
+        assert cg.getConvergence() and cg.report.cg_variant == 3  # the reduced inner solve ran (PP_CG_REDUCED)
+        return cg.report.outer_iters, cg.report.inner_iters, x.cpu().numpy()
+
+    mixed(P, M, b1)  # builds the reduced form for v1
+    M.set_values(v2)
+    P.refactor()
+    got = mixed(P, M, b2)
+    M2 = pp.CSRMatrix(A[0], A[1], v2)
+    F = make(pp, M2, "ic0", 1, 0)
+    want = mixed(F, M2, b2)
+    print(f"[refactor] reduced form refreshed: {got[0]} outer / {got[1]} inner iterations", flush=True)
+    assert got[0] == want[0] and got[1] == want[1]
+    np.testing.assert_array_equal(got[2], want[2])

@@ -244,10 +244,10 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M);
 int pp_precond_refactor(pp_ctx *ctx, pp_precond *M);
 /* Pins the level-sweep variant of both precisions instead of the timed choice
  * (0..7 the staged kernel's loads, 8 the sliced-column form of the wide
- * contiguous levels, the other levels staged with the default loads); every
- * variant gives the same result bit for bit.  PP_ERR_INVALID outside 0..8 or
- * for 8 when no level has that form; PP_ERR_NOT_READY before setup.  No
- * reference counterpart. */
+ * contiguous levels, kept by factors of at least 2^20 rows, the other levels
+ * staged with the default loads); every variant gives the same result bit for
+ * bit.  PP_ERR_INVALID outside 0..8 or for 8 when no level has that form;
+ * PP_ERR_NOT_READY before setup.  No reference counterpart. */
 int pp_precond_set_sweep_variant(pp_precond *M, int variant);
 /* y = M^{-1} x (MInvSolve in->out, BasePreconditon.h:113-128); y may equal x */
 int pp_precond_apply(pp_ctx *ctx, pp_precond *M, int dtype, const void *x, void *y);

@@ -36,6 +36,8 @@ struct Error : std::runtime_error {
 constexpr int kBlock = 256;             // threads per workgroup of the streaming kernels
 constexpr uint32_t kSpmvNnzBlock = 2048; // nnz staged in LDS per SpMV workgroup
 constexpr uint32_t kStreamLevelRows = 4096; // sweep levels this wide use the LDS-staged kernel
+constexpr uint32_t kTuneSweepRows = 1u << 20; // factors this large time their sweep variants at setup
+                                              // (and keep the sliced-column copies the timing weighs)
 constexpr uint32_t kNoBase = 0xFFFFFFFFu;   // row block without 16-bit column offsets
 // 16-bit column offsets of a row block: bit 15 selects one of the block's two
 // bases, bits 0-14 the distance from it (two windows of 32768 columns each,

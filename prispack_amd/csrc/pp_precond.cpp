@@ -170,9 +170,11 @@ void upload_sweep(const HostSweep &s, DevSweep<T> &d) {
         d.diag = dalloc<T>(n);
         PP_HIP(hipMemcpy(d.diag, dg.data(), n * sizeof(T), hipMemcpyHostToDevice));
     }
-    // sliced-column form of the wide levels whose rows are contiguous
+    // sliced-column form of the wide levels whose rows are contiguous, for
+    // factors large enough for the setup to time it against the staged
+    // kernel (tune_sweeps); smaller ones would never run it
     d.ell.assign(d.nlev, typename DevSweep<T>::EllLevel());
-    for (uint32_t l = 0; l < d.nlev; ++l) {
+    for (uint32_t l = 0; l < d.nlev && n >= kTuneSweepRows; ++l) {
         const uint32_t k0 = lp[l], k1 = lp[l + 1], nr = k1 - k0;
         if (nr < kStreamLevelRows || d.level_off[l] == DevSweep<T>::kNoOffset) continue;
         const int64_t row0 = (int64_t)k0 + d.level_off[l];
@@ -930,7 +932,7 @@ void precond_need_f32(pp_precond *M) {
     derive_sweep_f32(M->ctx, M->low64, M->low32);
     derive_sweep_f32(M->ctx, M->up64, M->up32);
     M->low32.variant = M->up32.variant = M->sweep_variant;
-    if (M->trsv == PP_TRSV_LEVELS && M->n >= (1u << 20) && (M->low32.blocks || M->up32.blocks) && !M->sweep_pinned)
+    if (M->trsv == PP_TRSV_LEVELS && M->n >= kTuneSweepRows && (M->low32.blocks || M->up32.blocks) && !M->sweep_pinned)
         tune_sweep_pair<float>(M->ctx, M, M->low32, M->up32, M->sweep_variant);
     if (M->chunks.nchunk) {
         upload_chunk_sweep(M->hlow, M->chunks, M->clow32);
@@ -1103,7 +1105,7 @@ int pp_precond_create(pp_ctx *ctx, pp_csr *A, const pp_precond_opts *opts, pp_pr
 // FP32 sweeps use the same choice.  Only when a level is wide enough to use
 // the staged kernel and the factor is large.
 static void tune_sweeps(pp_ctx *ctx, pp_precond *M) {
-    if (M->n < (1u << 20) || (!M->low64.blocks && !M->up64.blocks)) return;
+    if (M->n < kTuneSweepRows || (!M->low64.blocks && !M->up64.blocks)) return;
     M->sweep_variant = tune_sweep_pair<double>(ctx, M, M->low64, M->up64, M->sweep_variant);
 }
 

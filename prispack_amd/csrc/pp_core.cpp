@@ -132,6 +132,7 @@ void csr_need_f32(pp_ctx *ctx, pp_csr *A) {
         csr_upload_values_f32(ctx, A);
         if (A->ell.ok) {
             if (!A->ell.v32) A->ell.v32 = dalloc<float>((size_t)A->ell.width * A->ell.n);
+            if (A->ell.dg64 && !A->ell.dg32) A->ell.dg32 = dalloc<float>(A->ell.n);
             ell_fill_values(ctx, A->ell, PP_F32, A->rp, A->v32);
         }
     }
@@ -288,7 +289,15 @@ pp_csr *csr_from_vectors(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, std::vector<
     A->tune_warm = tune == 2;
     // the sliced-column copy for matrices timed cold (the reduced CG's
     // operators, timed warm, keep their own)
-    if (tune == 1 && nrow >= (1u << 14)) ell_build(A->h_rp, A->h_ci, A->h_v, 0, false, 0, A->ell, 1.1);
+    if (tune == 1 && nrow >= (1u << 14) && !ell_build(A->h_rp, A->h_ci, A->h_v, 0, false, 0, A->ell, 1.1) &&
+        nrow == ncol) {
+        // a colour-ordered matrix (columns about n/2 away from their rows):
+        // the two-colour form, the split at the first row with an entry left
+        // of its diagonal
+        uint32_t split = 0;
+        while (split < nrow && (A->h_rp[split] == A->h_rp[split + 1] || A->h_ci[A->h_rp[split]] >= split)) ++split;
+        ell_build_two_colour(A->h_rp, A->h_ci, A->h_v, split, A->ell, 1.1);
+    }
     if (tune) csr_tune_spmv(ctx, A.get(), PP_F64);
     else A->tuned32 = true;  // neither precision is timed
     return A.release();
@@ -329,6 +338,7 @@ static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A, int dtype) {
         if (A->ell.ok) {
             if (dtype == PP_F32 && !A->ell.v32) {
                 A->ell.v32 = dalloc<float>((size_t)A->ell.width * A->ell.n);
+                if (A->ell.dg64 && !A->ell.dg32) A->ell.dg32 = dalloc<float>(A->ell.n);
                 ell_fill_values(ctx, A->ell, PP_F32, A->rp, A->v32);
             }
             uint64_t csr_bytes = 0, ell_bytes = 0;
@@ -567,6 +577,7 @@ int pp_csr_set_spmv_variant(pp_csr *A, int dtype, int variant) {
     if (variant != kVarEll && (variant & kVarIdx16) && !A->ci16) return PP_ERR_INVALID;
     if (variant == kVarEll && dtype == PP_F32 && !A->ell.v32 && A->v32 && !A->f32_dirty) {
         A->ell.v32 = dalloc<float>((size_t)A->ell.width * A->ell.n);
+        if (A->ell.dg64 && !A->ell.dg32) A->ell.dg32 = dalloc<float>(A->ell.n);
         ell_fill_values(A->ctx, A->ell, PP_F32, A->rp, A->v32);
         PP_HIP(hipStreamSynchronize(A->ctx->stream));
     }
@@ -587,7 +598,8 @@ int pp_csr_spmv_bytes(const pp_csr *A, int dtype, uint64_t *bytes) {
         // the pattern table), x read, y written
         const uint64_t wn = A->ell.width * (uint64_t)A->ell.n;
         const uint64_t idx = A->ell.pid ? A->ell.n + 16ull * A->ell.npat : 2 * wn;
-        *bytes = esz * wn + idx + esz * ((uint64_t)A->ncol + A->nrow);
+        const uint64_t diag = A->ell.dg64 ? esz * A->ell.n : 0;  // a separate diagonal vector
+        *bytes = esz * wn + idx + diag + esz * ((uint64_t)A->ncol + A->nrow);
         return PP_OK;
     }
     uint64_t b = esz * A->nnz;                                    // values

@@ -297,6 +297,11 @@ struct EllOp {
     bool ok = false;
     uint32_t n = 0, width = 0;
     int64_t cbase = 0, dbase = 0;
+    // two-colour forms (ell_build_two_colour): rows from csplit on take their
+    // columns from row + cbase1 instead of row + cbase, and rows below dsplit
+    // add their diagonal (dg) first instead of last
+    uint32_t csplit = UINT32_MAX, dsplit = 0;
+    int64_t cbase1 = 0;
     int16_t *off = nullptr;      // [width * n], or nullptr when the rows use patterns:
     uint8_t *pid = nullptr;      // [n] the row's offset pattern (stencil-like matrices:
     int16_t *ptab = nullptr;     //     at most 256 distinct offset rows) and [npat * 8] the patterns
@@ -613,6 +618,12 @@ void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp,
 // a row is empty or an offset does not fit 16 bits
 bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci, const std::vector<double> &v,
                int64_t cbase, bool diag_last, int64_t dbase, EllOp &E, double max_fill = 1.25);
+// The sliced-column form of a two-colour matrix (rows [0, split) couple only
+// to [split, n) and back, every row holding its diagonal; a colour-ordered
+// stencil): per-colour column bases, the diagonal in dg (first in the rows'
+// sums below split, last from split on, as the CSR rows order it).
+bool ell_build_two_colour(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
+                          const std::vector<double> &v, uint32_t split, EllOp &E, double max_fill = 1.1);
 void launch_outer_on_pattern(pp_ctx *ctx, const pp_csr *A, double scale, const double *lam,
                              const double *x, double *g);
 void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks);

@@ -331,7 +331,8 @@ template <class T, bool DOT, int W, bool PAT>
 __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const int16_t *__restrict__ off,
                                              const uint8_t *__restrict__ pid, const int16_t *__restrict__ ptab,
                                              const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
-                                             int64_t dbase, const T *x, T *y, T alpha, T beta, int beta_zero,
+                                             int64_t dbase, uint32_t csplit, int64_t cbase1, uint32_t dsplit,
+                                             const T *x, T *y, T alpha, T beta, int beta_zero,
                                              const T *__restrict__ dotw, const T *__restrict__ aux,
                                              const T *__restrict__ wgt, double *partials, const int *stop) {
     __shared__ double sh[3 * (kBlock / 64)];
@@ -364,16 +365,19 @@ __global__ __launch_bounds__(256) void k_ell(uint32_t n, uint32_t width, const i
                 if (aux) ax = aux[i];
                 if (wgt) wg = wgt[i];
             }
+            const int64_t cb = i < csplit ? cbase : cbase1;
+            const bool dfirst = i < dsplit;
 #pragma unroll
             for (int j = 0; j < MAXW; ++j)
-                if (EllWidth<W>::has(j, width)) xv[j] = x[(int64_t)i + cbase + o[j]];
+                if (EllWidth<W>::has(j, width)) xv[j] = x[(int64_t)i + cb + o[j]];
             T sum = 0;
             {
 #pragma clang fp contract(off)
+                if (dg && dfirst) sum += dgi * xd;
 #pragma unroll
                 for (int j = 0; j < MAXW; ++j)
                     if (EllWidth<W>::has(j, width)) sum += v[j] * xv[j];
-                if (dg) sum += dgi * xd;
+                if (dg && !dfirst) sum += dgi * xd;
             }
             T out = alpha * sum;
             if (!beta_zero) out += beta * yold;
@@ -410,8 +414,8 @@ template <class T, int W, bool PAT>
 __global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const int16_t *__restrict__ off,
                                               const uint8_t *__restrict__ pid, const int16_t *__restrict__ ptab,
                                               const T *__restrict__ val, int64_t cbase, const T *__restrict__ dg,
-                                              int64_t dbase, const T *x, T *y, T alpha, T beta, int beta_zero,
-                                              const int *stop) {
+                                              int64_t dbase, uint32_t csplit, int64_t cbase1, uint32_t dsplit,
+                                              const T *x, T *y, T alpha, T beta, int beta_zero, const int *stop) {
     using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     if (stop && *stop) return;
     const uint32_t npair = n / 2;
@@ -454,15 +458,21 @@ __global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const 
             yo0 = yy.x;
             yo1 = yy.y;
         }
+        const int64_t cb0 = i < csplit ? cbase : cbase1, cb1 = i + 1 < csplit ? cbase : cbase1;
+        const bool df0 = i < dsplit, df1 = i + 1 < dsplit;
 #pragma unroll
         for (int j = 0; j < MAXW; ++j)
             if (EllWidth<W>::has(j, width)) {
-                x0[j] = x[(int64_t)i + cbase + o0[j]];
-                x1[j] = x[(int64_t)i + 1 + cbase + o1[j]];
+                x0[j] = x[(int64_t)i + cb0 + o0[j]];
+                x1[j] = x[(int64_t)i + 1 + cb1 + o1[j]];
             }
         T s0 = 0, s1 = 0;
         {
 #pragma clang fp contract(off)
+            if (dg) {  // a diagonal that comes first in its row
+                if (df0) s0 += dg0 * xd0;
+                if (df1) s1 += dg1 * xd1;
+            }
 #pragma unroll
             for (int j = 0; j < MAXW; ++j)
                 if (EllWidth<W>::has(j, width)) {
@@ -470,8 +480,8 @@ __global__ __launch_bounds__(256) void k_ell2(uint32_t n, uint32_t width, const 
                     s1 += v1[j] * x1[j];
                 }
             if (dg) {
-                s0 += dg0 * xd0;
-                s1 += dg1 * xd1;
+                if (!df0) s0 += dg0 * xd0;
+                if (!df1) s1 += dg1 * xd1;
             }
         }
         T r0 = alpha * s0, r1 = alpha * s1;
@@ -508,7 +518,8 @@ void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void
         const int bz2 = beta == 0.0;
 #define PP_ELL2(T, W, P, VALS, DG)                                                                             \
     k_ell2<T, W, P><<<g2, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, E.pid, E.ptab, VALS, E.cbase, DG, E.dbase, \
-                                                     (const T *)x, (T *)y, (T)alpha, (T)beta, bz2, ctx->stop_word)
+                                                     E.csplit, E.cbase1, E.dsplit, (const T *)x, (T *)y, (T)alpha, \
+                                                     (T)beta, bz2, ctx->stop_word)
 #define PP_ELL2_64(W) PP_ELL2(double, W, false, E.v64, E.dg64)
 #define PP_ELL2_32(W) PP_ELL2(float, W, false, E.v32, E.dg32)
 #define PP_ELL2_64P(W) PP_ELL2(double, W, true, E.v64, E.dg64)
@@ -538,9 +549,9 @@ void launch_ell(pp_ctx *ctx, const EllOp &E, int dtype, double alpha, const void
     const int bz = beta == 0.0;
 #define PP_ELL(T, DOTV, W, P, VALS, DG)                                                                       \
     k_ell<T, DOTV, W, P><<<grid, kBlock, 0, ctx->stream>>>(E.n, E.width, E.off, E.pid, E.ptab, VALS, E.cbase, DG, \
-                                                          E.dbase, (const T *)x, (T *)y, (T)alpha, (T)beta, bz, \
-                                                          (const T *)dotw, (const T *)aux, (const T *)wgt, part, \
-                                                          ctx->stop_word)
+                                                          E.dbase, E.csplit, E.cbase1, E.dsplit, (const T *)x,  \
+                                                          (T *)y, (T)alpha, (T)beta, bz, (const T *)dotw,       \
+                                                          (const T *)aux, (const T *)wgt, part, ctx->stop_word)
 #define PP_ELL_64D(W) PP_ELL(double, true, W, false, E.v64, E.dg64)
 #define PP_ELL_64N(W) PP_ELL(double, false, W, false, E.v64, E.dg64)
 #define PP_ELL_32D(W) PP_ELL(float, true, W, false, E.v32, E.dg32)
@@ -599,6 +610,21 @@ __global__ void k_ell_fill(uint32_t n, uint32_t width, const uint32_t *__restric
     }
 }
 
+// The same for a form with a separate diagonal: rows below dsplit hold it
+// first (the slots are their other entries), the rest last; dg refilled too
+template <class T>
+__global__ void k_ell_fill_dg(uint32_t n, uint32_t width, uint32_t dsplit, const uint32_t *__restrict__ rp,
+                              const T *__restrict__ v, T *__restrict__ out, T *__restrict__ dg) {
+    const size_t total = (size_t)width * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t j = (uint32_t)(e / n), i = (uint32_t)(e - (size_t)j * n);
+        const bool first = i < dsplit;
+        const uint32_t k = rp[i] + (first ? 1 : 0) + j, k1 = rp[i + 1] - (first ? 0 : 1);
+        out[e] = k < k1 ? v[k] : (T)0;
+        if (j == 0) dg[i] = v[first ? rp[i] : rp[i + 1] - 1];
+    }
+}
+
 // out[j * n + i] from the CSR values v of rows i (row pointers rp), on the
 // null stream, waited for (setup only)
 template <class T>
@@ -613,13 +639,19 @@ template void ell_fill_rows<double>(uint32_t, uint32_t, const uint32_t *, const 
 template void ell_fill_rows<float>(uint32_t, uint32_t, const uint32_t *, const float *, float *);
 
 void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp, const void *v) {
-    // a form with a separate diagonal (dg) leaves each row's last entry out of the slots
-    const uint32_t tail = E.dg64 ? 1 : 0;
     const unsigned g = grid_for(ctx, (uint64_t)E.width * E.n, 4);
-    if (dtype == PP_F64)
-        k_ell_fill<double><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const double *)v, E.v64, tail);
-    else
-        k_ell_fill<float><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const float *)v, E.v32, tail);
+    if (E.dg64) {  // a separate diagonal: out of the slots, into dg
+        if (dtype == PP_F64)
+            k_ell_fill_dg<double><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, E.dsplit, rp, (const double *)v, E.v64,
+                                                                 E.dg64);
+        else
+            k_ell_fill_dg<float><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, E.dsplit, rp, (const float *)v, E.v32,
+                                                                E.dg32);
+    } else if (dtype == PP_F64) {
+        k_ell_fill<double><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const double *)v, E.v64, 0u);
+    } else {
+        k_ell_fill<float><<<g, kBlock, 0, ctx->stream>>>(E.n, E.width, rp, (const float *)v, E.v32, 0u);
+    }
     PP_HIP(hipGetLastError());
 }
 

@@ -3,6 +3,7 @@
 // Host-side construction from the colour-ordered matrix, once per matrix
 // version; the device work is SpMV (k_spmv) on the four operators.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <climits>
 #include <cmath>
@@ -172,7 +173,116 @@ void EllOp::free_all() {
     v32 = dg32 = nullptr;
     ok = false;
     n = width = 0;
+    csplit = UINT32_MAX;
+    dsplit = 0;
+    cbase1 = 0;
 }
+
+namespace {
+
+// The offsets' pattern table and the upload of a sliced-column form whose
+// slots (off, val: entry j of row i at j n + i) and separate diagonal (dg,
+// empty for none) are built: E's geometry (n, width, bases, splits) is set
+// by the caller.
+void ell_finish(EllOp &E, uint32_t n, uint32_t width, const std::vector<int16_t> &off,
+                const std::vector<double> &val, const std::vector<double> &dg) {
+    // Row patterns: when the rows' offset tuples take at most 256 distinct
+    // values (a stencil: interior rows share one, the boundary rows a few),
+    // a row stores a one-byte pattern number instead of width 16-bit
+    // offsets, and the kernels read the row's offsets from a 16-byte table
+    // entry (cached): 2 width - 1 bytes less per row.  For the widths the
+    // kernels are instantiated at exactly (4 to 7).
+    std::vector<uint8_t> pid;
+    std::vector<int16_t> ptab;
+    if (width >= 4 && width <= 7) {
+        // Numbered in order of first appearance, found in parallel: each
+        // chunk of rows lists its own patterns (consecutive rows mostly
+        // repeat one: a 16-byte compare), the lists are merged in chunk
+        // order (which keeps the first-appearance order of a serial pass),
+        // and each row's chunk-local number is mapped to the merged one.
+        using Key = std::array<int16_t, 8>;
+        auto key_of = [&](uint32_t i) {
+            Key k{};
+            for (uint32_t j = 0; j < width; ++j) k[j] = off[(size_t)j * n + i];
+            return k;
+        };
+        const uint32_t cs = 1u << 16, nchunk = (n + cs - 1) / cs;
+        std::vector<std::vector<Key>> local(nchunk);
+        pid.resize(n);
+        std::atomic<bool> ok{true};
+        parallel_for(nchunk, 1, [&](size_t c0, size_t c1) {
+            for (size_t c = c0; c < c1 && ok; ++c) {
+                std::vector<Key> &pats = local[c];
+                int last = -1;
+                for (uint32_t i = (uint32_t)c * cs, e = std::min(n, (uint32_t)(c + 1) * cs); i < e; ++i) {
+                    const Key k = key_of(i);
+                    if (last >= 0 && k == pats[last]) {
+                        pid[i] = (uint8_t)last;
+                        continue;
+                    }
+                    last = -1;
+                    for (size_t q = 0; q < pats.size(); ++q)
+                        if (pats[q] == k) {
+                            last = (int)q;
+                            break;
+                        }
+                    if (last < 0) {
+                        if (pats.size() == 256) {
+                            ok = false;
+                            return;
+                        }
+                        last = (int)pats.size();
+                        pats.push_back(k);
+                    }
+                    pid[i] = (uint8_t)last;
+                }
+            }
+        });
+        std::vector<std::vector<uint8_t>> remap(nchunk);
+        std::vector<Key> all;
+        for (uint32_t c = 0; c < nchunk && ok; ++c)
+            for (const Key &k : local[c]) {
+                size_t q = std::find(all.begin(), all.end(), k) - all.begin();
+                if (q == all.size()) {
+                    if (all.size() == 256) {
+                        ok = false;
+                        break;
+                    }
+                    all.push_back(k);
+                }
+                remap[c].push_back((uint8_t)q);
+            }
+        if (ok) {
+            parallel_for(n, cs, [&](size_t i0, size_t i1) {
+                for (size_t i = i0; i < i1; ++i) pid[i] = remap[i / cs][pid[i]];
+            });
+            for (const Key &k : all) ptab.insert(ptab.end(), k.begin(), k.end());
+        } else {
+            pid.clear();
+        }
+    }
+    if (!pid.empty()) {
+        E.npat = (uint32_t)(ptab.size() / 8);
+        E.pid = dalloc<uint8_t>(n + 1);  // one more byte: a two-row load at the last even row stays inside
+        E.ptab = dalloc<int16_t>(ptab.size());
+        PP_HIP(hipMemcpy(E.pid, pid.data(), n, hipMemcpyHostToDevice));
+        PP_HIP(hipMemcpy(E.ptab, ptab.data(), ptab.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+    } else {
+        E.off = dalloc<int16_t>(off.size());
+        PP_HIP(hipMemcpy(E.off, off.data(), off.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+    }
+    E.v64 = dalloc<double>(val.size());
+    PP_HIP(hipMemcpy(E.v64, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!dg.empty()) {
+        E.dg64 = dalloc<double>(n);
+        PP_HIP(hipMemcpy(E.dg64, dg.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    E.ok = true;
+}
+
+}  // namespace
+
+
 
 // diag_last: the last entry of every row is its diagonal term, at column
 // dbase + row; it goes to dg instead of the sliced columns.
@@ -225,78 +335,73 @@ bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
     E.width = width;
     E.cbase = cbase;
     E.dbase = dbase;
-    // Row patterns: when the rows' offset tuples take at most 256 distinct
-    // values (a stencil: interior rows share one, the boundary rows a few),
-    // a row stores a one-byte pattern number instead of width 16-bit
-    // offsets, and the kernels read the row's offsets from a 16-byte table
-    // entry (cached): 2 width - 1 bytes less per row.  For the widths the
-    // kernels are instantiated at exactly (4 to 7).
-    std::vector<uint8_t> pid;
-    std::vector<int16_t> ptab;
-    if (width >= 4 && width <= 7) {
-        // a row's offsets packed into one 64-bit key: 7 slots of 16 bits do
-        // not fit, so the key is the tuple hashed and the table compared
-        auto key_of = [&](uint32_t i, int16_t *k) {
-            for (uint32_t j = 0; j < 8; ++j) k[j] = j < width ? off[(size_t)j * n + i] : 0;
-        };
-        auto hash_of = [](const int16_t *k) {
-            uint64_t h = 1469598103934665603ull;
-            for (int j = 0; j < 8; ++j) h = (h ^ (uint16_t)k[j]) * 1099511628211ull;
-            return h;
-        };
-        std::unordered_multimap<uint64_t, uint8_t> seen;
-        pid.resize(n);
-        bool ok = true;
-        int16_t key[8], last[8] = {};
-        int last_id = -1;
-        for (uint32_t i = 0; i < n && ok; ++i) {
-            key_of(i, key);
-            if (last_id >= 0 && std::equal(key, key + 8, last)) {  // neighbouring rows mostly repeat
-                pid[i] = (uint8_t)last_id;
-                continue;
+    ell_finish(E, n, width, off, val, dg);
+    return true;
+}
+
+// A two-colour matrix (rows [0, split) couple only to [split, n) and back,
+// each row holding its diagonal): rows below split take their columns from
+// row + split (cbase), the others from row - split (cbase1); the diagonal
+// goes to dg, first in its row's sum below split (where it is the row's
+// first entry) and last from split on, as the CSR rows order them.
+bool ell_build_two_colour(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci,
+                          const std::vector<double> &v, uint32_t split, EllOp &E, double max_fill) {
+    E.free_all();
+    const uint32_t n = (uint32_t)rp.size() - 1;
+    if (split == 0 || split >= n) return false;
+    uint32_t width = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t len = rp[i + 1] - rp[i];
+        if (len < 2) return false;  // a row with no sliced entry
+        width = std::max(width, len - 1);
+    }
+    if (width > 16 || (uint64_t)width * n >= (1ull << 32)) return false;
+    if ((double)width * n > max_fill * (double)(rp[n] - (uint64_t)n)) return false;
+    const int64_t cb0 = (int64_t)split, cb1 = -(int64_t)split;
+    std::vector<int16_t> off((size_t)width * n);
+    std::vector<double> val((size_t)width * n), dg(n);
+    std::atomic<bool> fits{true};
+    parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1 && fits; ++i) {
+            const bool lo = i < split;
+            // the diagonal: first entry below split, last from split on
+            const uint32_t kd = lo ? rp[i] : rp[i + 1] - 1;
+            if (ci[kd] != i) {
+                fits = false;
+                return;
             }
-            const uint64_t h = hash_of(key);
-            int id = -1;
-            for (auto [it, end] = seen.equal_range(h); it != end; ++it)
-                if (std::equal(key, key + 8, ptab.begin() + 8 * (size_t)it->second)) {
-                    id = it->second;
-                    break;
+            dg[i] = v[kd];
+            const uint32_t k0 = lo ? rp[i] + 1 : rp[i], k1 = lo ? rp[i + 1] : rp[i + 1] - 1;
+            const int64_t cb = lo ? cb0 : cb1;
+            int16_t last = 0;
+            for (uint32_t j = 0; j < width; ++j) {
+                const size_t e = (size_t)j * n + i;
+                if (k0 + j < k1) {
+                    const uint32_t c = ci[k0 + j];
+                    const int64_t o = (int64_t)c - (int64_t)i - cb;
+                    if ((c < split) == lo || o < INT16_MIN || o > INT16_MAX) {  // a coupling inside a colour
+                        fits = false;
+                        return;
+                    }
+                    last = (int16_t)o;
+                    off[e] = last;
+                    val[e] = v[k0 + j];
+                } else {  // padding: a zero on the row's last column
+                    off[e] = last;
+                    val[e] = 0.0;
                 }
-            if (id < 0) {
-                if (seen.size() == 256) {
-                    ok = false;
-                    break;
-                }
-                id = (int)seen.size();
-                seen.emplace(h, (uint8_t)id);
-                ptab.insert(ptab.end(), key, key + 8);
             }
-            pid[i] = (uint8_t)id;
-            std::copy(key, key + 8, last);
-            last_id = id;
         }
-        if (!ok) {
-            pid.clear();
-            ptab.clear();
-        }
-    }
-    if (!pid.empty()) {
-        E.npat = (uint32_t)(ptab.size() / 8);
-        E.pid = dalloc<uint8_t>(n + 1);  // one more byte: a two-row load at the last even row stays inside
-        E.ptab = dalloc<int16_t>(ptab.size());
-        PP_HIP(hipMemcpy(E.pid, pid.data(), n, hipMemcpyHostToDevice));
-        PP_HIP(hipMemcpy(E.ptab, ptab.data(), ptab.size() * sizeof(int16_t), hipMemcpyHostToDevice));
-    } else {
-        E.off = dalloc<int16_t>(off.size());
-        PP_HIP(hipMemcpy(E.off, off.data(), off.size() * sizeof(int16_t), hipMemcpyHostToDevice));
-    }
-    E.v64 = dalloc<double>(val.size());
-    PP_HIP(hipMemcpy(E.v64, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice));
-    if (diag_last) {
-        E.dg64 = dalloc<double>(n);
-        PP_HIP(hipMemcpy(E.dg64, dg.data(), n * sizeof(double), hipMemcpyHostToDevice));
-    }
-    E.ok = true;
+    });
+    if (!fits) return false;
+    E.n = n;
+    E.width = width;
+    E.cbase = cb0;
+    E.cbase1 = cb1;
+    E.csplit = split;
+    E.dsplit = split;
+    E.dbase = 0;
+    ell_finish(E, n, width, off, val, dg);
     return true;
 }
 

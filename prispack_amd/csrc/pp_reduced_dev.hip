@@ -92,11 +92,6 @@ __global__ void k_rr_first(uint32_t n0, const uint32_t *__restrict__ rp, const u
     }
 }
 
-// KB's diagonal (its rows' last entries) into the sliced-column copy's vector
-__global__ void k_rr_dg(uint32_t n1, const uint32_t *__restrict__ kb_rp, const double *__restrict__ kb_v, double *dg) {
-    for (uint32_t q = blockIdx.x * kB + threadIdx.x; q < n1; q += gridDim.x * kB) dg[q] = kb_v[kb_rp[q + 1] - 1];
-}
-
 }  // namespace
 
 bool reduced_refresh(pp_ctx *ctx, ReducedForm *R, pp_csr *As) {
@@ -119,10 +114,7 @@ bool reduced_refresh(pp_ctx *ctx, ReducedForm *R, pp_csr *As) {
     if (bad) return false;
     // the copies: sliced-column forms, FP32 forms, Dt and Dt^-1/2 in FP32
     if (R->ka.ok) ell_fill_values(ctx, R->ka, PP_F64, R->KA->rp, R->KA->v64);
-    if (R->kb.ok) {
-        ell_fill_values(ctx, R->kb, PP_F64, R->KB->rp, R->KB->v64);
-        if (R->kb.dg64) k_rr_dg<<<grid_of(n1), kB, 0, ctx->stream>>>(n1, R->KB->rp, R->KB->v64, R->kb.dg64);
-    }
+    if (R->kb.ok) ell_fill_values(ctx, R->kb, PP_F64, R->KB->rp, R->KB->v64);  // its diagonal too
     for (pp_csr *K : {R->KA, R->KB, R->KF, R->KE}) {
         if (K->ell.ok) ell_fill_values(ctx, K->ell, PP_F64, K->rp, K->v64);
         K->f32_dirty = true;  // the next FP32 use converts again (csr_need_f32)

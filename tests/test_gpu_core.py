@@ -202,7 +202,7 @@ def random_band(n, per_row, span, seed):
     return rp, ci, v
 
 
-@pytest.mark.parametrize("name", ["lap3d_40", "lap3d_41", "random_7"])
+@pytest.mark.parametrize("name", ["lap3d_40", "lap3d_41", "random_7", "rb_lap3d_40", "rb_upwind_33"])
 def test_spmv_sliced_column_offset_forms(pp, torch, name):
     # the sliced-column copy stores a stencil's offsets as one pattern byte
     # per row (at most 256 distinct offset rows) and anything else as 16-bit
@@ -213,6 +213,13 @@ def test_spmv_sliced_column_offset_forms(pp, torch, name):
     L = _capi.lib()
     if name == "random_7":
         A = random_band(40000, 7, 20000, 5)
+    elif name.startswith("rb_"):
+        # red-black (colour) order: every column about n/2 from its row, the
+        # two-colour form with per-colour bases and the diagonal apart
+        from mc_order import parity_order, permute_fast
+        g = int(name[-2:])
+        B = ppo.laplacian(g, g, g) if "lap" in name else ppo.upwind(g, g, g, 20.0, 10.0, 5.0)
+        A = permute_fast(B, parity_order(g, g, g))
     else:
         g = int(name[-2:])
         A = ppo.laplacian(g, g, g)
@@ -226,6 +233,8 @@ def test_spmv_sliced_column_offset_forms(pp, torch, name):
     dense_offsets = 10 * 7 * n + 16 * n
     if name == "random_7":
         assert nb.value == dense_offsets
+    elif name.startswith("rb_"):
+        assert nb.value < 8 * 6 * n + 8 * n + 2 * n + 16 * n  # six slots, the diagonal, a byte per row
     else:
         assert nb.value < 8 * 7 * n + 2 * n + 16 * n  # a byte per row, a small table
     x = ppo.uniform(n, 41)
@@ -240,6 +249,12 @@ def test_spmv_sliced_column_offset_forms(pp, torch, name):
     y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
     M.MatPVec(dev(torch, x, torch.float32), y32)
     np.testing.assert_allclose(host(y32), want, rtol=0, atol=1e-5 * (1 + np.abs(want).max()))
+    # new values on the same pattern refill the copy (its diagonal included)
+    v2 = np.asarray(A[2], np.float64) * (1.0 + 0.5 * ppo.uniform(len(A[2]), 43))
+    M.set_values(v2)
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    M.MatPVec(dev(torch, x), y)
+    assert np.array_equal(host(y), ppo.spmv((A[0], A[1], v2), x))
 
 
 def test_sliced_column_copy_follows_value_updates(pp, torch):

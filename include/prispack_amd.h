@@ -327,9 +327,11 @@ typedef struct {
     double inner_tol;    /* mixed: relative reduction asked of each inner solve */
     uint32_t max_outer;  /* mixed: refinement steps */
     int record_history;  /* keep per-iteration ||r||/||b|| */
-    int launch;          /* PP_LAUNCH_GRAPH: the iteration body is captured once in a
-                            hipGraph and replayed; PP_LAUNCH_STREAM: its kernels are
-                            launched on the stream each iteration */
+    int launch;          /* PP_LAUNCH_STREAM (pp_solve_opts_default): the iteration
+                            body's kernels are launched on the stream each time;
+                            PP_LAUNCH_GRAPH: the body is captured once in a hipGraph
+                            and replayed (consecutive graph launches leave 8-14 us
+                            between bodies on MI355X, stream launches none) */
     int cg_variant;      /* PP_CG_* */
     uint32_t reserved[6];
 } pp_solve_opts;

@@ -897,6 +897,7 @@ void pp_solve_opts_default(pp_solve_opts *o, int solver) {
     o->restart = 30;
     o->reference_quirks = 0;
     o->check_every = 4;
+    o->launch = PP_LAUNCH_STREAM;
     o->inner_tol = 1e-6;
     o->max_outer = 20;
     o->record_history = 1;

@@ -406,10 +406,12 @@ def _golden(name):
     return dict(np.load(os.path.join(GOLD, name + ".npz")))
 
 
+@pytest.mark.parametrize("launch", ["stream", "graph"])
 @pytest.mark.parametrize("trsv", ["levels", "syncfree"])
 @pytest.mark.parametrize("name", ["cg_128x128x1_tau0.001", "cg_128x128x1_tau0.01", "cg_12x12x12_tau0.001",
                                   "cg_20x20x20_tau0.01", "cg_16x16x1_tau0.001"])
-def test_cg_parity_with_reference(pp, torch, name, trsv):
+def test_cg_parity_with_reference(pp, torch, name, trsv, launch):
+    from prispack_amd._capi import PP_LAUNCH_GRAPH, PP_LAUNCH_STREAM
     g = _golden(name)
     dims = tuple(int(t) for t in g["dims"])
     A = ppo.laplacian(*dims)
@@ -420,6 +422,7 @@ def test_cg_parity_with_reference(pp, torch, name, trsv):
     M = pp.CSRMatrix(*A)
     P = pp.IncompleteCholesky(M, float(g["tau"]), trsv=TRSV[trsv])
     cg = pp.CG(n, P, pp.FlexibleCG, tol=float(g["tol"]), maxIter=int(g["maxit"]))
+    cg.opts.launch = PP_LAUNCH_GRAPH if launch == "graph" else PP_LAUNCH_STREAM
     x = torch.zeros(n, dtype=torch.float64, device="cuda")
     cg.solve(M, dev(torch, b), x)
     xs = host(x)
@@ -496,11 +499,13 @@ def test_mixed_precision_cg(pp, torch, dims, trsv):
     assert err <= 1e-6
 
 
+@pytest.mark.parametrize("launch", ["stream", "graph"])
 @pytest.mark.parametrize("trsv", ["levels", "syncfree"])
 @pytest.mark.parametrize("name", ["gmres_lapconv_12x12x12", "gmres_upwind_16x16x16", "gmres_upwind_10x10x10_l3"])
-def test_gmres_parity_with_reference(pp, torch, name, trsv):
-    # the FGMRES cycle is captured into a graph before its first apply: the
+def test_gmres_parity_with_reference(pp, torch, name, trsv, launch):
+    # graph launches capture the FGMRES cycle before its first apply: the
     # sync-free schedule's buffers must exist from setup on
+    from prispack_amd._capi import PP_LAUNCH_GRAPH, PP_LAUNCH_STREAM
     g = _golden(name)
     A = (g["rp"], g["ci"], g["v"])
     n = len(A[0]) - 1
@@ -512,6 +517,7 @@ def test_gmres_parity_with_reference(pp, torch, name, trsv):
     for quirks in (True, False):
         gm = pp.GMRES(n, P, float(g["tol"]), pp.FlexibleGMRES, reference_quirks=quirks)
         gm.setRestart(True, int(g["restart"]))
+        gm.opts.launch = PP_LAUNCH_GRAPH if launch == "graph" else PP_LAUNCH_STREAM
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         gm.solve(M, dev(torch, b), x)
         xs = host(x)
@@ -541,6 +547,7 @@ def test_gmres_mixed(pp, torch):
 
 def test_single_reduction_fp64_syncfree(pp, torch):
     # the single-reduction body's first apply is inside the captured graph
+    from prispack_amd._capi import PP_LAUNCH_GRAPH
     A = ppo.laplacian(16, 16, 12)
     n = len(A[0]) - 1
     b = ppo.spmv(A, ppo.normal(n, 2))
@@ -549,6 +556,7 @@ def test_single_reduction_fp64_syncfree(pp, torch):
     for trsv in ("levels", "syncfree"):
         P = pp.IncompleteCholesky(M, 1e-3, trsv=TRSV[trsv])
         cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-10, maxIter=2000).setCGVariant("single_reduction")
+        cg.opts.launch = PP_LAUNCH_GRAPH
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         cg.solve(M, dev(torch, b), x)
         assert cg.getConvergence() and cg.report.true_relres < 1e-10, trsv

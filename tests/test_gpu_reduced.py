@@ -32,10 +32,12 @@ def pp(torch):
     return prispack_amd
 
 
-def solve(pp, torch, M, P, b, precision, variant, tol=1e-8, check_every=4):
+def solve(pp, torch, M, P, b, precision, variant, tol=1e-8, check_every=4, launch=None):
     n = len(b)
     cg = pp.CG(n, P, pp.FlexibleCG, tol=tol, maxIter=3000, precision=precision).setCGVariant(variant)
     cg.opts.check_every = check_every
+    if launch is not None:
+        cg.opts.launch = launch
     x = torch.zeros(n, dtype=torch.float64, device="cuda")
     cg.solve(M, torch.tensor(b, device="cuda"), x)
     return cg, x.cpu().numpy()
@@ -122,6 +124,7 @@ def test_reduced_mixed_any_rhs(pp, torch, rhs):
 def test_reduced_repeated_and_graph_reuse(pp, torch):
     # the reduced form and its iteration graphs are built once and reused;
     # a second right-hand side gives its own solution
+    from prispack_amd._capi import PP_LAUNCH_GRAPH
     A = ppo.laplacian(24, 24, 24)
     n = len(A[0]) - 1
     M = pp.CSRMatrix(*A)
@@ -130,7 +133,7 @@ def test_reduced_repeated_and_graph_reuse(pp, torch):
         xt = ppo.normal(n, seed)
         b = ppo.spmv(A, xt)
         for prec in ("mixed", "fp64"):
-            cg, x = solve(pp, torch, M, P, b, prec, "reduced")
+            cg, x = solve(pp, torch, M, P, b, prec, "reduced", launch=PP_LAUNCH_GRAPH)
             assert cg.getConvergence() and cg.report.true_relres < 1e-8, (seed, prec)
             assert np.linalg.norm(x - xt) / np.linalg.norm(xt) <= 1e-5, (seed, prec)
 
@@ -217,7 +220,7 @@ def test_reduced_edge_cases(pp, torch, precision):
 def test_reduced_stream_launches_match_graphs(pp, torch):
     # the iteration bodies launched on the stream (no graph capture) give
     # the same iterates as the captured, replayed bodies, in both precisions
-    from prispack_amd._capi import PP_LAUNCH_STREAM
+    from prispack_amd._capi import PP_LAUNCH_GRAPH, PP_LAUNCH_STREAM
     A = ppo.laplacian(30, 29, 28)
     n = len(A[0]) - 1
     b = ppo.spmv(A, ppo.normal(n, 12))
@@ -227,8 +230,7 @@ def test_reduced_stream_launches_match_graphs(pp, torch):
         out = []
         for stream in (False, True):
             cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=3000, precision=prec).setCGVariant("reduced")
-            if stream:
-                cg.opts.launch = PP_LAUNCH_STREAM
+            cg.opts.launch = PP_LAUNCH_STREAM if stream else PP_LAUNCH_GRAPH
             x = torch.zeros(n, dtype=torch.float64, device="cuda")
             cg.solve(M, torch.tensor(b, device="cuda"), x)
             assert cg.getConvergence() and cg.report.true_relres < 1e-8

@@ -349,12 +349,16 @@ def main():
         return dt, cg.report, x
 
     first_mixed_s = None
+    reduced_rec = None
     if not args.kernels_only:
         for w in range(args.warmup):
             solve("fp64")
             dt_w, _, _ = solve("mixed")
             if w == 0:
                 first_mixed_s = dt_w  # includes the colour-ordered matrix and reduced form built on first use
+                ri = P.info()
+                reduced_rec = {"built": {0: None, 1: "host", 2: "device", 3: "refreshed"}[ri.reduced_form],
+                               "ms": round(float(ri.reduced_form_ms), 2)}
         t64, tmx, rep64, repmx = [], [], None, None
         torch.cuda.synchronize()
         for _ in range(args.steps):
@@ -584,6 +588,7 @@ def main():
         "levels": [info.levels_lower, info.levels_upper],
         "setup_phases_ms": [round(float(v), 1) for v in info.phase_ms],
         "first_mixed_solve_s": None if first_mixed_s is None else round(first_mixed_s, 4),
+        "reduced_form": reduced_rec,
         "new_values": resetup_rec,
         "roofline": dominant,
         "roofline_apply": apply_rec,

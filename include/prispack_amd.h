@@ -224,6 +224,11 @@ typedef struct {
     float phase_ms[4];      /* setup phases: ordering, factorization, sweep analysis + upload,
                                schedule choice + load-variant timing */
     uint32_t device_refactors; /* pp_precond_refactor calls that factored on the device */
+    int32_t reduced_form;   /* operators of the reduced CG (IC(0) in a two-colour order,
+                               built by the first solve that runs it): 0 none, 1 built
+                               on the host, 2 built on the device, 3 refreshed on the
+                               device after new values */
+    float reduced_form_ms;  /* the last build or refresh of those operators */
 } pp_precond_info;
 
 void pp_precond_opts_default(pp_precond_opts *o);

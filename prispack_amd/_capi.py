@@ -46,7 +46,8 @@ class PrecondInfo(C.Structure):
                 ("mnnz", C.c_uint32), ("fill_ratio", C.c_double), ("levels_lower", C.c_uint32),
                 ("levels_upper", C.c_uint32), ("setup_ms", C.c_double), ("fail_row", C.c_uint32),
                 ("sweep_variant", C.c_int32), ("fused_levels", C.c_int32), ("trsv", C.c_int32),
-                ("phase_ms", C.c_float * 4), ("device_refactors", C.c_uint32)]
+                ("phase_ms", C.c_float * 4), ("device_refactors", C.c_uint32), ("reduced_form", C.c_int32),
+                ("reduced_form_ms", C.c_float)]
 
 
 class SolveOpts(C.Structure):

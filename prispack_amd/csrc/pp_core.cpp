@@ -303,6 +303,39 @@ pp_csr *csr_from_vectors(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, std::vector<
     return A.release();
 }
 
+pp_csr *csr_from_device(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, uint32_t *rp, uint32_t *ci, double *v,
+                        std::vector<uint32_t> &&h_rp) {
+    pp_csr *A = new pp_csr;
+    A->rp = rp;
+    A->ci = ci;
+    A->v64 = v;
+    try {
+        PP_CHECK(h_rp.size() == (size_t)nrow + 1 && h_rp[0] == 0, PP_ERR_INVALID, "rowptr size");
+        A->ctx = ctx;
+        A->uid = next_uid();
+        A->nrow = nrow;
+        A->ncol = ncol;
+        A->nnz = h_rp[nrow];
+        A->h_rp = std::move(h_rp);
+        std::vector<uint32_t> blocks;
+        build_row_blocks(A->h_rp, blocks);
+        A->nblocks = (uint32_t)blocks.size() - 1;
+        A->row_blocks = dalloc<uint32_t>(blocks.size());
+        const std::vector<BlockDesc> ds = block_descs(blocks, A->h_rp, std::vector<uint32_t>());
+        A->blk_desc = dalloc<BlockDesc>(ds.size());
+        PP_HIP(hipMemcpyAsync(A->row_blocks, blocks.data(), blocks.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+        PP_HIP(hipMemcpyAsync(A->blk_desc, ds.data(), ds.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
+                              ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));  // blocks, ds leave scope
+        A->tuned32 = true;                         // neither precision is timed
+    } catch (...) {
+        csr_free(A);
+        throw;
+    }
+    return A;
+}
+
 pp_csr *csr_new(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, const uint32_t *rp, const uint32_t *ci,
                 const double *v) {
     PP_CHECK(rp != nullptr, PP_ERR_INVALID, "rowptr is NULL");

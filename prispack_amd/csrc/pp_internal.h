@@ -381,6 +381,8 @@ struct ReducedForm {
     uint64_t src_version = 0;
     int *dfail = nullptr;     // device word of reduced_refresh's pivot test
     double *dwork = nullptr;  // its diagonal of the matrix (n)
+    int built = 0;            // 1 host construction, 2 device, 3 refreshed on the device
+    float build_ms = 0.f;     // that construction's wall time
 };
 // Host arrays of the reduced form of one colour-ordered matrix: KA (n0
 // rows), KB and KF (n1 rows), KE (n0 rows), columns in the matrix's
@@ -398,6 +400,11 @@ bool reduced_host(uint32_t n, const std::vector<uint32_t> &rp, const std::vector
 // The reduced form of M for the colour-ordered matrix As (built once and
 // cached in M), or nullptr when it does not apply.
 ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As);
+// The reduced form built on the device from As's device arrays (R empty on
+// entry): 1 built, 0 the structure is absent or a pivot is not positive
+// (R->ok stays false), -1 As has no two-colour sliced-column copy of the
+// operators' width (the host construction applies).
+int reduced_build_device(pp_ctx *ctx, pp_precond *M, pp_csr *As, ReducedForm *R);
 // New values of an existing reduced form from As's current values (same
 // pattern as it was built from), on the device, bit-identical to a host
 // rebuild; false when a pivot is not positive (pp_reduced_dev.hip).
@@ -627,6 +634,12 @@ bool ell_build_two_colour(const std::vector<uint32_t> &rp, const std::vector<uin
 void launch_outer_on_pattern(pp_ctx *ctx, const pp_csr *A, double scale, const double *lam,
                              const double *x, double *g);
 void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks);
+// The CSR object of device arrays it takes over (freed with it, also when
+// this throws), with the host copy of the row pointers: row blocks for the
+// CSR-stream SpMV, no 16-bit column form, no variant timing, no host copy
+// of the columns or values.
+pp_csr *csr_from_device(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, uint32_t *rp, uint32_t *ci, double *v,
+                        std::vector<uint32_t> &&h_rp);
 
 template <class T>
 void launch_sweep(pp_ctx *ctx, const DevSweep<T> &s, int mode, const T *b, T *x);

@@ -1423,6 +1423,10 @@ int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {
     info->fused_levels = M->fuse_mid ? 1 : 0;
     for (int k = 0; k < 4; ++k) info->phase_ms[k] = M->phase_ms[k];
     info->device_refactors = M->device_refactors;
+    if (M->reduced && M->reduced->ok) {
+        info->reduced_form = M->reduced->built;
+        info->reduced_form_ms = M->reduced->build_ms;
+    }
     return PP_OK;
 }
 

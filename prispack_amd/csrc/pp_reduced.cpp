@@ -5,8 +5,10 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
 #include <mutex>
 #include <unordered_map>
@@ -412,14 +414,37 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
         return M->reduced->ok ? M->reduced : nullptr;
     // the same matrix with new values on its pattern (a re-factorization):
     // the operators' values recomputed in place on the device
+    const auto t0 = std::chrono::steady_clock::now();
+    auto ms = [&] {
+        return (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
     if (M->reduced && M->reduced->ok && M->reduced->src_uid == As->uid && reduced_refresh(ctx, M->reduced, As)) {
         M->reduced->src_version = As->version;
+        M->reduced->built = 3;
+        M->reduced->build_ms = ms();
         return M->reduced;
     }
     reduced_free(M);
     std::unique_ptr<ReducedForm> R(new ReducedForm);
     R->src_uid = As->uid;
     R->src_version = As->version;
+    // on the device from the matrix's device arrays when its sliced-column
+    // copy serves the operators too, else on the host (always on the host
+    // with PP_REDUCED_HOST_BUILD=1 in the environment: the reference
+    // construction the device one is tested against)
+    const char *host_only = std::getenv("PP_REDUCED_HOST_BUILD");
+    try {
+        if (!(host_only && host_only[0] == '1') && reduced_build_device(ctx, M, As, R.get()) >= 0) {
+            R->built = 2;
+            R->build_ms = ms();
+            M->reduced = R.release();
+            return M->reduced->ok ? M->reduced : nullptr;
+        }
+    } catch (...) {
+        M->reduced = R.release();
+        reduced_free(M);
+        throw;
+    }
     ReducedHost H;
     // the factor's first level must be exactly the first colour
     const bool ok = As->nrow == As->ncol && reduced_host(As->nrow, As->h_rp, As->h_ci, As->h_v, H) &&
@@ -453,6 +478,8 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
         R->dt_64 = dalloc<double>(n1);
         PP_HIP(hipMemcpy(R->dt_64, H.dt.data(), n1 * sizeof(double), hipMemcpyHostToDevice));
         R->ok = true;
+        R->built = 1;
+        R->build_ms = ms();
     } catch (...) {
         M->reduced = R.release();
         reduced_free(M);
@@ -467,14 +494,10 @@ void reduced_need_f32(pp_ctx *ctx, ReducedForm *R) {
     csr_need_f32(ctx, R->KB);
     csr_need_f32(ctx, R->KF);
     csr_need_f32(ctx, R->KE);
-    auto to32 = [&](const double *src, float *&dst) {
+    auto to32 = [&](const double *src, float *&dst) {  // rounded to nearest, as a host cast
         if (dst) return;
-        std::vector<double> h(R->n1);
-        std::vector<float> f(R->n1);
-        PP_HIP(hipMemcpy(h.data(), src, R->n1 * sizeof(double), hipMemcpyDeviceToHost));
-        for (uint32_t i = 0; i < R->n1; ++i) f[i] = (float)h[i];
         dst = dalloc<float>(R->n1);
-        PP_HIP(hipMemcpy(dst, f.data(), R->n1 * sizeof(float), hipMemcpyHostToDevice));
+        launch_convert(ctx, R->n1, PP_F64, src, PP_F32, dst);
     };
     to32(R->s1_64, R->s1_32);
     to32(R->dt_64, R->dt_32);

@@ -14,7 +14,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "pp_internal.h"
 
@@ -92,7 +94,272 @@ __global__ void k_rr_first(uint32_t n0, const uint32_t *__restrict__ rp, const u
     }
 }
 
+// n0: the first row holding an entry left of its diagonal
+__global__ void k_rd_split(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+                           uint32_t *n0) {
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
+        for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] < i) {
+                atomicMin(n0, i);
+                break;
+            }
+}
+
+// the two-colour structure: one diagonal entry per row, every other entry
+// in the other colour
+__global__ void k_rd_check(uint32_t n, uint32_t n0, const uint32_t *__restrict__ rp,
+                           const uint32_t *__restrict__ ci, int *fail) {
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+        uint32_t diag = 0;
+        bool bad = false;
+        for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            const uint32_t j = ci[k];
+            if (j == i) ++diag;
+            else bad |= (i < n0) == (j < n0);
+        }
+        if (bad || diag != 1) atomicOr(fail, 1);
+    }
+}
+
+// The operators' patterns (every row of the matrix holds one diagonal):
+// KA row i = row i's off-diagonal columns, KE row i = i then those, KB and
+// KF row q = row n0 + q's off-diagonal columns then n0 + q; their row
+// pointers follow from the matrix's (KA: rp[i] - i, KE: rp[i], KB and KF:
+// rp[n0 + q] - rp[n0]).
+__global__ void k_rd_patterns(uint32_t n, uint32_t n0, const uint32_t *__restrict__ rp,
+                              const uint32_t *__restrict__ ci, uint32_t *ka_rp, uint32_t *ka_ci, uint32_t *ke_rp,
+                              uint32_t *ke_ci, uint32_t *kb_rp, uint32_t *kb_ci, uint32_t *kf_rp, uint32_t *kf_ci) {
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i <= n; i += gridDim.x * kB) {
+        if (i <= n0) {
+            ka_rp[i] = rp[i] - i;
+            ke_rp[i] = rp[i];
+        }
+        if (i >= n0) kb_rp[i - n0] = kf_rp[i - n0] = rp[i] - rp[n0];
+        if (i == n) continue;
+        if (i < n0) {
+            uint32_t pa = rp[i] - i, pe = rp[i];
+            ke_ci[pe++] = i;
+            for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const uint32_t j = ci[k];
+                if (j == i) continue;
+                ka_ci[pa++] = j;
+                ke_ci[pe++] = j;
+            }
+        } else {
+            uint32_t pb = rp[i] - rp[n0];
+            for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+                const uint32_t j = ci[k];
+                if (j == i) continue;
+                kb_ci[pb] = j;
+                kf_ci[pb++] = j;
+            }
+            kb_ci[pb] = i;
+            kf_ci[pb] = i;
+        }
+    }
+}
+
+// the largest absolute row sum (each row summed in order; NaN rows skipped,
+// as std::max skips them); non-negative doubles order as their bit patterns
+__global__ void k_rd_anorm(uint32_t n, const uint32_t *__restrict__ rp, const double *__restrict__ v,
+                           unsigned long long *out) {
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
+        double t = 0.0;
+        for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) t += fabs(v[k]);
+        if (t == t) atomicMax(out, (unsigned long long)__double_as_longlong(t));
+    }
+}
+
+// the highest pattern number in pid[0, n)
+__global__ void k_rd_max_pid(uint32_t n, const uint8_t *__restrict__ pid, uint32_t *out) {
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) atomicMax(out, (uint32_t)pid[i]);
+}
+
+// first[p] = the first row of pid[0, n) with pattern p
+__global__ void k_rd_first_seen(uint32_t n, const uint8_t *__restrict__ pid, uint32_t *first) {
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) atomicMin(first + pid[i], i);
+}
+
+__global__ void k_rd_remap(uint32_t n, const uint8_t *__restrict__ pid, const uint8_t *__restrict__ map,
+                           uint8_t *out) {
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) out[i] = map[pid[i]];
+}
+
 }  // namespace
+
+int reduced_build_device(pp_ctx *ctx, pp_precond *M, pp_csr *As, ReducedForm *R) {
+    const uint32_t n = As->nrow;
+    if (n != As->ncol || n < 2) return 0;
+    uint32_t *u = dalloc<uint32_t>(256 + 2);  // n0, the highest pattern, first rows of the patterns
+    int *fail = nullptr;
+    auto cleanup = [&] {
+        dfree(u);
+        dfree(fail);
+    };
+    try {
+        fail = dalloc<int>(1);
+        PP_HIP(hipMemsetAsync(u, 0xff, sizeof(uint32_t), ctx->stream));
+        PP_HIP(hipMemsetAsync(fail, 0, sizeof(int), ctx->stream));
+        k_rd_split<<<grid_of(n), kB, 0, ctx->stream>>>(n, As->rp, As->ci, u);
+        PP_HIP(hipGetLastError());
+        uint32_t n0 = 0;
+        PP_HIP(hipMemcpyAsync(&n0, u, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        if (n0 == 0 || n0 >= n) {
+            cleanup();
+            return 0;
+        }
+        k_rd_check<<<grid_of(n), kB, 0, ctx->stream>>>(n, n0, As->rp, As->ci, fail);
+        PP_HIP(hipGetLastError());
+        int bad = 0;
+        PP_HIP(hipMemcpyAsync(&bad, fail, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        // the factor's first level must be exactly the first colour
+        if (bad || M->low64.nlev != 2 || M->up64.nlev != 2 || !M->lvl0_contiguous || M->lvl0_lo != 0 ||
+            M->lvl0_hi != n0) {
+            cleanup();
+            return 0;
+        }
+        const uint32_t n1 = n - n0;
+        // the row pointers on the host (row blocks, widths)
+        std::vector<uint32_t> hrp;
+        if (As->h_rp.size() != (size_t)n + 1) {
+            hrp.resize((size_t)n + 1);
+            PP_HIP(hipMemcpy(hrp.data(), As->rp, ((size_t)n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        }
+        const std::vector<uint32_t> &rp = hrp.empty() ? As->h_rp : hrp;
+        // KA and KB take the sliced-column form of the matrix's own
+        // two-colour copy (the same offsets row for row, its pattern numbers
+        // renumbered for the second colour) when the widths agree; else the
+        // host construction runs
+        const EllOp &S = As->ell;
+        uint32_t wa = 0, wb = 0;
+        for (uint32_t i = 0; i < n0; ++i) wa = std::max(wa, rp[i + 1] - rp[i] - 1);
+        for (uint32_t i = n0; i < n; ++i) wb = std::max(wb, rp[i + 1] - rp[i] - 1);
+        const bool fits = S.ok && S.pid && S.csplit == n0 && S.dsplit == n0 && S.cbase == (int64_t)n0 &&
+                          S.cbase1 == -(int64_t)n0 && S.dbase == 0 && wa == S.width && wb == S.width &&
+                          (double)wa * n0 <= 1.25 * (double)(rp[n0] - (uint64_t)n0) &&
+                          (double)wb * n1 <= 1.25 * (double)(rp[n] - (uint64_t)rp[n0] - n1);
+        if (!fits) {
+            cleanup();
+            return -1;
+        }
+        R->n0 = n0;
+        R->n1 = n1;
+        // each operator's arrays go to its CSR object as soon as they exist
+        auto make = [&](pp_csr *&K, uint32_t rows, std::vector<uint32_t> &&h) {
+            uint32_t *krp = nullptr, *kci = nullptr;
+            double *kv = nullptr;
+            try {
+                krp = dalloc<uint32_t>((size_t)rows + 1);
+                kci = dalloc<uint32_t>(h[rows]);
+                kv = dalloc<double>(h[rows]);
+            } catch (...) {
+                dfree(krp);
+                dfree(kci);
+                dfree(kv);
+                throw;
+            }
+            K = csr_from_device(ctx, rows, n, krp, kci, kv, std::move(h));
+        };
+        std::vector<uint32_t> h_ka(n0 + 1), h_ke(rp.begin(), rp.begin() + n0 + 1), h_kb(n1 + 1);
+        for (uint32_t i = 0; i <= n0; ++i) h_ka[i] = rp[i] - i;
+        for (uint32_t q = 0; q <= n1; ++q) h_kb[q] = rp[n0 + q] - rp[n0];
+        std::vector<uint32_t> h_kf(h_kb);
+        make(R->KA, n0, std::move(h_ka));
+        make(R->KE, n0, std::move(h_ke));
+        make(R->KB, n1, std::move(h_kb));
+        make(R->KF, n1, std::move(h_kf));
+        R->s1_64 = dalloc<double>(n1);
+        R->dt_64 = dalloc<double>(n1);
+        if (!R->dwork) R->dwork = dalloc<double>(n);
+        if (!R->dfail) R->dfail = dalloc<int>(1);
+        k_rd_patterns<<<grid_of(n + 1), kB, 0, ctx->stream>>>(n, n0, As->rp, As->ci, R->KA->rp, R->KA->ci,
+                                                              R->KE->rp, R->KE->ci, R->KB->rp, R->KB->ci,
+                                                              R->KF->rp, R->KF->ci);
+        PP_HIP(hipMemsetAsync(R->dfail, 0, sizeof(int), ctx->stream));
+        PP_HIP(hipMemsetAsync(u, 0, sizeof(unsigned long long), ctx->stream));
+        double *d = R->dwork;
+        k_rr_diag<<<grid_of(n), kB, 0, ctx->stream>>>(n, n0, As->rp, As->ci, As->v64, d, R->dfail);
+        k_rr_second<<<grid_of(n1), kB, 0, ctx->stream>>>(n0, n1, As->rp, As->ci, As->v64, d, R->KB->rp,
+                                                         R->KB->v64, R->KF->rp, R->KF->v64, R->dt_64, R->s1_64,
+                                                         R->dfail);
+        k_rr_first<<<grid_of(n0), kB, 0, ctx->stream>>>(n0, As->rp, As->ci, As->v64, d, R->dt_64, R->KA->rp,
+                                                        R->KA->v64, R->KE->rp, R->KE->v64);
+        k_rd_anorm<<<grid_of(n), kB, 0, ctx->stream>>>(n, As->rp, As->v64, (unsigned long long *)u);
+        PP_HIP(hipGetLastError());
+        unsigned long long an = 0;
+        PP_HIP(hipMemcpyAsync(&bad, R->dfail, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipMemcpyAsync(&an, u, sizeof(an), hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        if (bad) {  // a pivot that is not positive: no reduced form
+            cleanup();
+            return 0;
+        }
+        std::memcpy(&R->anorm, &an, sizeof(double));
+        // KA: rows [0, n0) of S, its pattern numbers unchanged (numbered by
+        // first appearance, the first colour's patterns are S's first ones)
+        EllOp &ka = R->ka, &kb = R->kb;
+        ka.n = n0;
+        ka.width = S.width;
+        ka.cbase = n0;
+        ka.dbase = 0;
+        PP_HIP(hipMemsetAsync(u + 1, 0, sizeof(uint32_t), ctx->stream));
+        k_rd_max_pid<<<grid_of(n0), kB, 0, ctx->stream>>>(n0, S.pid, u + 1);
+        PP_HIP(hipMemsetAsync(u + 2, 0xff, 256 * sizeof(uint32_t), ctx->stream));
+        k_rd_first_seen<<<grid_of(n1), kB, 0, ctx->stream>>>(n1, S.pid + n0, u + 2);
+        PP_HIP(hipGetLastError());
+        std::vector<uint32_t> hu(258);
+        std::vector<int16_t> ptab((size_t)S.npat * 8);
+        PP_HIP(hipMemcpyAsync(hu.data(), u, hu.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipMemcpyAsync(ptab.data(), S.ptab, ptab.size() * sizeof(int16_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        ka.npat = hu[1] + 1;
+        ka.pid = dalloc<uint8_t>(n0 + 1);
+        ka.ptab = dalloc<int16_t>((size_t)ka.npat * 8);
+        PP_HIP(hipMemcpyAsync(ka.pid, S.pid, n0, hipMemcpyDeviceToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(ka.ptab, S.ptab, (size_t)ka.npat * 8 * sizeof(int16_t), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        ka.v64 = dalloc<double>((size_t)ka.width * n0);
+        ka.ok = true;
+        // KB: rows [n0, n) of S, patterns renumbered by first appearance there
+        std::vector<std::pair<uint32_t, uint32_t>> seen;
+        for (uint32_t p = 0; p < S.npat; ++p)
+            if (hu[2 + p] != UINT32_MAX) seen.emplace_back(hu[2 + p], p);
+        std::sort(seen.begin(), seen.end());
+        std::vector<uint8_t> map(256, 0);
+        std::vector<int16_t> tb;
+        for (size_t r = 0; r < seen.size(); ++r) {
+            map[seen[r].second] = (uint8_t)r;
+            tb.insert(tb.end(), ptab.begin() + (size_t)seen[r].second * 8, ptab.begin() + (size_t)seen[r].second * 8 + 8);
+        }
+        kb.n = n1;
+        kb.width = S.width;
+        kb.cbase = 0;
+        kb.dbase = n0;
+        kb.npat = (uint32_t)seen.size();
+        kb.pid = dalloc<uint8_t>(n1 + 1);
+        kb.ptab = dalloc<int16_t>(tb.size());
+        uint8_t *dmap = (uint8_t *)(u + 2);  // the first rows are read: the table's room holds the map
+        PP_HIP(hipMemcpyAsync(dmap, map.data(), 256, hipMemcpyHostToDevice, ctx->stream));
+        PP_HIP(hipMemcpyAsync(kb.ptab, tb.data(), tb.size() * sizeof(int16_t), hipMemcpyHostToDevice, ctx->stream));
+        k_rd_remap<<<grid_of(n1), kB, 0, ctx->stream>>>(n1, S.pid + n0, dmap, kb.pid);
+        PP_HIP(hipGetLastError());
+        kb.v64 = dalloc<double>((size_t)kb.width * n1);
+        kb.dg64 = dalloc<double>(n1);
+        kb.ok = true;
+        ell_fill_values(ctx, ka, PP_F64, R->KA->rp, R->KA->v64);
+        ell_fill_values(ctx, kb, PP_F64, R->KB->rp, R->KB->v64);
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        R->ok = true;
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+    return 1;
+}
 
 bool reduced_refresh(pp_ctx *ctx, ReducedForm *R, pp_csr *As) {
     const uint32_t n = As->nrow, n0 = R->n0, n1 = R->n1;

@@ -264,3 +264,39 @@ def test_reduced_form_follows_value_updates(pp, torch):
     assert cg1.getConvergence() and cg1.report.true_relres < 1e-8
     assert (cg1.report.outer_iters, cg1.report.inner_iters) == (cg2.report.outer_iters, cg2.report.inner_iters)
     np.testing.assert_array_equal(x1, x2)
+
+
+def test_reduced_form_device_build_matches_host(pp, torch, monkeypatch):
+    # the reduced form built on the device from the colour-ordered matrix's
+    # device arrays (64^3 class: its sliced-column copy exists) gives the
+    # same mixed and FP64 reduced solves, bit for bit, as the host
+    # construction (PP_REDUCED_HOST_BUILD=1); variable values D A D + I/4
+    A = ppo.laplacian(64, 63, 62)
+    n = len(A[0]) - 1
+    rows = np.repeat(np.arange(n), np.diff(A[0]))
+    cols = np.asarray(A[1], np.int64)
+    rng = np.random.default_rng(5)
+    dsc = 1.0 + 0.4 * rng.uniform(-1, 1, n)
+    v = np.asarray(A[2], np.float64) * dsc[rows] * dsc[cols]
+    v[A[1] == rows] += 0.25
+    B = (A[0], A[1], v)
+    b = ppo.spmv(B, ppo.normal(n, 21))
+    out = {}
+    for where in ("host", "device"):
+        if where == "host":
+            monkeypatch.setenv("PP_REDUCED_HOST_BUILD", "1")
+        else:
+            monkeypatch.delenv("PP_REDUCED_HOST_BUILD", raising=False)
+        M = pp.CSRMatrix(*B)
+        P = pp.IC0(M, ordering=1, make_f32=True)
+        res = []
+        for prec, var in (("mixed", "auto"), ("fp64", "reduced")):
+            cg, x = solve(pp, torch, M, P, b, prec, var)
+            assert cg.getConvergence() and cg.report.cg_variant == 3, (where, prec)
+            res.append((cg.report.outer_iters, cg.report.inner_iters, cg.getNumIter(), x))
+        assert P.info().reduced_form == (1 if where == "host" else 2), where
+        out[where] = res
+    print(f"[reduced] device build: {out['device'][0][:3]} / {out['device'][1][:3]}", flush=True)
+    for h, d in zip(out["host"], out["device"]):
+        assert h[:3] == d[:3]
+        np.testing.assert_array_equal(h[3], d[3])

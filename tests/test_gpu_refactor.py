@@ -168,7 +168,7 @@ def test_refactor_refreshes_the_reduced_form(pp, torch):
     # values and a device re-factorization the next mixed solve recomputes
     # that form's values on the device: iterations and x equal, bit for bit,
     # a mixed solve through a preconditioner set up from scratch on the new
-    # values (whose reduced form is built on the host)
+    # values (whose reduced form is built anew)
     # 64^3: the reduced operators are large enough for their sliced-column
     # forms (at most 10% padding), so the two-launch iteration runs too
     g = 64
@@ -198,6 +198,7 @@ def test_refactor_refreshes_the_reduced_form(pp, torch):
     M.set_values(v2)
     P.refactor()
     got = mixed(P, M, b2)
+    assert P.info().reduced_form == 3  # refreshed in place
     M2 = pp.CSRMatrix(A[0], A[1], v2)
     F = make(pp, M2, "ic0", 1, 0)
     want = mixed(F, M2, b2)

@@ -66,6 +66,7 @@ template unsigned char *dalloc<unsigned char>(size_t);
 template BlockDesc *dalloc<BlockDesc>(size_t);
 template uint4 *dalloc<uint4>(size_t);
 template int16_t *dalloc<int16_t>(size_t);
+template unsigned long long *dalloc<unsigned long long>(size_t);
 
 void dfree(void *p) {
     if (p) (void)hipFree(p);
@@ -289,14 +290,15 @@ pp_csr *csr_from_vectors(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, std::vector<
     A->tune_warm = tune == 2;
     // the sliced-column copy for matrices timed cold (the reduced CG's
     // operators, timed warm, keep their own)
-    if (tune == 1 && nrow >= (1u << 14) && !ell_build(A->h_rp, A->h_ci, A->h_v, 0, false, 0, A->ell, 1.1) &&
+    // (built on the device from the arrays just uploaded)
+    if (tune == 1 && nrow >= (1u << 14) && !ell_build_device(ctx, A.get(), false, 0, A->ell, 1.1) &&
         nrow == ncol) {
         // a colour-ordered matrix (columns about n/2 away from their rows):
         // the two-colour form, the split at the first row with an entry left
         // of its diagonal
         uint32_t split = 0;
         while (split < nrow && (A->h_rp[split] == A->h_rp[split + 1] || A->h_ci[A->h_rp[split]] >= split)) ++split;
-        ell_build_two_colour(A->h_rp, A->h_ci, A->h_v, split, A->ell, 1.1);
+        ell_build_device(ctx, A.get(), true, split, A->ell, 1.1);
     }
     if (tune) csr_tune_spmv(ctx, A.get(), PP_F64);
     else A->tuned32 = true;  // neither precision is timed

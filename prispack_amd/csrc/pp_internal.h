@@ -625,6 +625,11 @@ void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp,
 // a row is empty or an offset does not fit 16 bits
 bool ell_build(const std::vector<uint32_t> &rp, const std::vector<uint32_t> &ci, const std::vector<double> &v,
                int64_t cbase, bool diag_last, int64_t dbase, EllOp &E, double max_fill = 1.25);
+// The same forms built on the device from A's device arrays (A->h_rp on the
+// host for the row-length conditions): the natural form (no base, no
+// separate diagonal) or, two_colour, the two-colour form split at `split`;
+// the result equals the host construction's, pattern numbers included.
+bool ell_build_device(pp_ctx *ctx, const pp_csr *A, bool two_colour, uint32_t split, EllOp &E, double max_fill);
 // The sliced-column form of a two-colour matrix (rows [0, split) couple only
 // to [split, n) and back, every row holding its diagonal; a colour-ordered
 // stencil): per-colour column bases, the diagonal in dg (first in the rows'

@@ -47,7 +47,7 @@ class PrecondInfo(C.Structure):
                 ("levels_upper", C.c_uint32), ("setup_ms", C.c_double), ("fail_row", C.c_uint32),
                 ("sweep_variant", C.c_int32), ("fused_levels", C.c_int32), ("trsv", C.c_int32),
                 ("phase_ms", C.c_float * 4), ("device_refactors", C.c_uint32), ("reduced_form", C.c_int32),
-                ("reduced_form_ms", C.c_float)]
+                ("reduced_form_ms", C.c_float), ("setup_on_device", C.c_int32)]
 
 
 class SolveOpts(C.Structure):

@@ -305,6 +305,21 @@ pp_csr *csr_from_vectors(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, std::vector<
     return A.release();
 }
 
+void csr_tune(pp_ctx *ctx, pp_csr *A) { csr_tune_spmv(ctx, A, PP_F64); }
+
+void csr_ensure_host(pp_csr *A) {
+    if (A->h_ci.size() == A->nnz && A->h_v.size() == A->nnz && A->h_rp.size() == (size_t)A->nrow + 1) return;
+    PP_HIP(hipStreamSynchronize(A->ctx->stream));
+    A->h_rp.resize((size_t)A->nrow + 1);
+    A->h_ci.resize(A->nnz);
+    A->h_v.resize(A->nnz);
+    PP_HIP(hipMemcpy(A->h_rp.data(), A->rp, ((size_t)A->nrow + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (A->nnz) {
+        PP_HIP(hipMemcpy(A->h_ci.data(), A->ci, (size_t)A->nnz * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        PP_HIP(hipMemcpy(A->h_v.data(), A->v64, (size_t)A->nnz * sizeof(double), hipMemcpyDeviceToHost));
+    }
+}
+
 pp_csr *csr_from_device(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, uint32_t *rp, uint32_t *ci, double *v,
                         std::vector<uint32_t> &&h_rp) {
     pp_csr *A = new pp_csr;
@@ -323,10 +338,15 @@ pp_csr *csr_from_device(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, uint32_t *rp,
         build_row_blocks(A->h_rp, blocks);
         A->nblocks = (uint32_t)blocks.size() - 1;
         A->row_blocks = dalloc<uint32_t>(blocks.size());
-        const std::vector<BlockDesc> ds = block_descs(blocks, A->h_rp, std::vector<uint32_t>());
-        A->blk_desc = dalloc<BlockDesc>(ds.size());
         PP_HIP(hipMemcpyAsync(A->row_blocks, blocks.data(), blocks.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                               ctx->stream));
+        // 16-bit two-base column offsets where a block's columns fit two windows
+        std::vector<uint32_t> base;
+        A->nnz16 = two_base_device(ctx, A->nblocks, A->row_blocks, A->rp, A->ci, A->nnz, A->ci16, A->blk_base, base);
+        A->nblocks16 = 0;
+        for (uint32_t bi = 0; bi < A->nblocks; ++bi) A->nblocks16 += base[2 * bi] != kNoBase;
+        const std::vector<BlockDesc> ds = block_descs(blocks, A->h_rp, base);
+        A->blk_desc = dalloc<BlockDesc>(ds.size());
         PP_HIP(hipMemcpyAsync(A->blk_desc, ds.data(), ds.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
                               ctx->stream));
         PP_HIP(hipStreamSynchronize(ctx->stream));  // blocks, ds leave scope

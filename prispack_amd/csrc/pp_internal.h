@@ -456,6 +456,10 @@ struct pp_precond {
     pp::DevRefactor *dref = nullptr;
     uint32_t device_refactors = 0;
     bool host_factor_stale = false;  // the host copy of the factor predates a device re-factorization
+    // set up on the device: the host structures (factor, sweeps, P A P^T's
+    // columns and values) are built only when a caller needs them
+    bool host_pending = false;
+    bool dev_setup = false;  // the last setup ran on the device
     uint64_t Aperm_src_uid = 0;
     uint64_t Aperm_version = 0;
     bool io_permuted = false;
@@ -639,10 +643,29 @@ bool ell_build_two_colour(const std::vector<uint32_t> &rp, const std::vector<uin
 void launch_outer_on_pattern(pp_ctx *ctx, const pp_csr *A, double scale, const double *lam,
                              const double *x, double *g);
 void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks);
+// The SpMV load variants of A timed now (FP64; FP32 at its first use), as
+// for a matrix created with timing.
+void csr_tune(pp_ctx *ctx, pp_csr *A);
+// The host copies of A's columns and values, downloaded when A was built on
+// the device without them.
+void csr_ensure_host(pp_csr *A);
+// The 16-bit two-base column offsets of nblk row blocks (device arrays:
+// block bounds, row pointers, columns) computed on the device, as
+// two_base_blocks does on the host: the entries in those blocks, off16
+// [nnz] and dbase [2 nblk] on the device (both null when no block fits) and
+// the bases on the host.
+uint64_t two_base_device(pp_ctx *ctx, uint32_t nblk, const uint32_t *blocks, const uint32_t *rp, const uint32_t *ci,
+                         uint32_t nnz, uint16_t *&off16, uint32_t *&dbase, std::vector<uint32_t> &base);
+// IC(0) in multicolour order set up on the device (pp_setup_dev.hip): the
+// order, P A P^T, the factor, both FP64 sweeps and the re-factorization
+// state, the host's results without its host copies; false (M unchanged)
+// when the matrix is outside that path (unsorted or unsymmetric pattern,
+// long rows or chains, more than 64 colours, a failing pivot).
+bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M);
 // The CSR object of device arrays it takes over (freed with it, also when
-// this throws), with the host copy of the row pointers: row blocks for the
-// CSR-stream SpMV, no 16-bit column form, no variant timing, no host copy
-// of the columns or values.
+// this throws), with the host copy of the row pointers: row blocks and
+// their 16-bit column form for the CSR-stream SpMV (computed on the
+// device), no variant timing, no host copy of the columns or values.
 pp_csr *csr_from_device(pp_ctx *ctx, uint32_t nrow, uint32_t ncol, uint32_t *rp, uint32_t *ci, double *v,
                         std::vector<uint32_t> &&h_rp);
 

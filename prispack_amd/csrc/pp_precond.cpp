@@ -5,6 +5,7 @@
 #include <atomic>
 #include <chrono>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 
 #include "pp_internal.h"
@@ -390,7 +391,10 @@ static void upload_chunk_sweep(const HostSweep &s, const ChunkSet &C, DevChunkSw
 
 // PP_TRSV_BLOCK storage for M (FP64; FP32 too when the FP32 sweeps exist).
 // false (nothing kept) when the rows do not chunk within the LDS size.
+static void ensure_host_setup(pp_precond *M);
+
 static bool build_block_sweeps(pp_ctx *ctx, pp_precond *M) {
+    ensure_host_setup(M);  // the chunks are cut from the host sweeps
     M->chunks.free_all();
     M->clow64.free_all();
     M->cup64.free_all();
@@ -496,6 +500,22 @@ static double ms_since(std::chrono::steady_clock::time_point &t) {
     return ms;
 }
 
+// The first level's rows (when contiguous) and the FP32 sweeps, once the
+// FP64 sweeps are on the device.
+static void finish_device_sweeps(pp_precond *M) {
+    M->lvl0_contiguous = false;
+    if (M->low64.nlev > 0 && M->low64.level_off[0] != DevSweep<double>::kNoOffset) {
+        M->lvl0_contiguous = true;
+        M->lvl0_lo = (uint32_t)((int64_t)M->low64.level_ptr[0] + M->low64.level_off[0]);
+        M->lvl0_hi = (uint32_t)((int64_t)M->low64.level_ptr[1] + M->low64.level_off[0]);
+    }
+    if (M->opts.make_f32) {
+        derive_sweep_f32(M->ctx, M->low64, M->low32);
+        derive_sweep_f32(M->ctx, M->up64, M->up32);
+        M->have32 = true;
+    }
+}
+
 // perm_out (multicolour order): receives P A P^T, which the factor was
 // computed from.
 static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload, HostCsr *perm_out = nullptr) {
@@ -590,18 +610,36 @@ static void setup_triangular(pp_precond *M, const HostCsr &A0, bool upload, Host
     }
     upload_sweep(M->hlow, M->low64);
     upload_sweep(M->hup, M->up64);
-    M->lvl0_contiguous = false;
-    if (M->low64.nlev > 0 && M->low64.level_off[0] != DevSweep<double>::kNoOffset) {
-        M->lvl0_contiguous = true;
-        M->lvl0_lo = (uint32_t)((int64_t)M->low64.level_ptr[0] + M->low64.level_off[0]);
-        M->lvl0_hi = (uint32_t)((int64_t)M->low64.level_ptr[1] + M->low64.level_off[0]);
-    }
-    if (M->opts.make_f32) {
-        derive_sweep_f32(M->ctx, M->low64, M->low32);
-        derive_sweep_f32(M->ctx, M->up64, M->up32);
-        M->have32 = true;
-    }
+    finish_device_sweeps(M);
     if (perm_out && Ap == &Aperm) *perm_out = std::move(Aperm);
+}
+
+// The host structures of a preconditioner set up on the device (the
+// factor, its sweeps), from the matrix's current values: the same results
+// the device path computed.
+static void ensure_host_setup(pp_precond *M) {
+    if (!M->host_pending) return;
+    float saved[4];
+    std::copy(M->phase_ms, M->phase_ms + 4, saved);
+    HostCsr A;
+    A.n = M->A->nrow;
+    A.rp.swap(M->A->h_rp);
+    A.ci.swap(M->A->h_ci);
+    A.v.swap(M->A->h_v);
+    try {
+        setup_triangular(M, A, false);
+    } catch (...) {
+        A.rp.swap(M->A->h_rp);
+        A.ci.swap(M->A->h_ci);
+        A.v.swap(M->A->h_v);
+        throw;
+    }
+    A.rp.swap(M->A->h_rp);
+    A.ci.swap(M->A->h_ci);
+    A.v.swap(M->A->h_v);
+    std::copy(saved, saved + 4, M->phase_ms);
+    M->host_pending = false;
+    M->host_factor_stale = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -829,8 +867,9 @@ static void devref_run(pp_ctx *ctx, pp_precond *M) {
         pp_csr *P = M->Aperm;
         launch_gather_map(ctx, R.aperm_nnz, R.aperm_map, M->A->v64, PP_F64, P->v64);
         if (P->ell.ok) ell_fill_values(ctx, P->ell, PP_F64, P->rp, P->v64);
-        PP_HIP(hipMemcpyAsync(P->h_v.data(), P->v64, (size_t)R.aperm_nnz * sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
+        if (P->h_v.size() == R.aperm_nnz)  // a matrix built on the device keeps no host values
+            PP_HIP(hipMemcpyAsync(P->h_v.data(), P->v64, (size_t)R.aperm_nnz * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
         P->f32_dirty = true;
         ++P->version;
         if (P->trans) {
@@ -1170,6 +1209,17 @@ static int choose_schedule(pp_ctx *ctx, pp_precond *M, int force) {
     return PP_TRSV_LEVELS;
 }
 
+// IC(0) in multicolour order on a matrix large enough for the device work
+// to pay (and not PP_SETUP_HOST=1 in the environment: the host setup the
+// device one is tested against)
+static bool device_setup_applies(const pp_precond *M) {
+    const char *host = std::getenv("PP_SETUP_HOST");
+    const int t = M->opts.trsv;
+    return M->opts.kind == PP_PC_IC0 && M->opts.ordering == PP_ORDER_MULTICOLOR &&
+           (t == PP_TRSV_AUTO || t == PP_TRSV_LEVELS || t == PP_TRSV_SYNCFREE) && M->A->nrow >= (1u << 15) &&
+           !(host && host[0] == '1');
+}
+
 extern "C" {
 
 int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
@@ -1182,7 +1232,35 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
             M->mnnz = 0;
         } else if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) {
             ml_setup(ctx, M);
+        } else if (device_setup_applies(M) && setup_ic0_mc_device(ctx, M)) {
+            // on the device (pp_setup_dev.hip): the host structures wait for a caller that needs them
+            M->host_pending = true;
+            M->dev_setup = true;
+            M->hlow = HostSweep();
+            M->hup = HostSweep();
+            M->hup.lower = false;
+            M->lt_rp.clear();
+            M->lt_ci.clear();
+            M->lt_v.clear();
+            M->d.clear();
+            M->fail_row = 0;
+            finish_device_sweeps(M);
+            auto tp = std::chrono::steady_clock::now();
+            M->phase_ms[2] = (float)(std::chrono::duration<double, std::milli>(tp - t0).count() - M->phase_ms[0] -
+                                     M->phase_ms[1]);
+            const int want = M->opts.trsv;
+            M->trsv = want == PP_TRSV_AUTO ? choose_schedule(ctx, M, want) : want;
+            if (M->trsv == PP_TRSV_BLOCK || M->trsv == PP_TRSV_PIPE) {
+                delete M->dref;  // the re-factorization does not refill the chunked sweeps
+                M->dref = nullptr;
+            }
+            if (M->trsv == PP_TRSV_LEVELS) tune_sweeps(ctx, M);
+            if (M->trsv == PP_TRSV_SYNCFREE && !M->tmp64) M->tmp64 = dalloc<double>(M->n);
+            PP_HIP(hipStreamSynchronize(ctx->stream));
+            M->phase_ms[3] = (float)ms_since(tp);
         } else {
+            M->host_pending = false;
+            M->dev_setup = false;
             // the matrix's host arrays are lent to the factorization, not copied
             HostCsr A, Ap;
             A.n = M->A->nrow;
@@ -1423,6 +1501,7 @@ int pp_precond_info_get(const pp_precond *M, pp_precond_info *info) {
     info->fused_levels = M->fuse_mid ? 1 : 0;
     for (int k = 0; k < 4; ++k) info->phase_ms[k] = M->phase_ms[k];
     info->device_refactors = M->device_refactors;
+    info->setup_on_device = M->dev_setup ? 1 : 0;
     if (M->reduced && M->reduced->ok) {
         info->reduced_form = M->reduced->built;
         info->reduced_form_ms = M->reduced->build_ms;
@@ -1470,10 +1549,11 @@ static void refresh_host_factor(pp_precond *M) {
 int pp_precond_factor(const pp_precond *M, int which, uint32_t *nnz, uint32_t *rowptr, uint32_t *colidx,
                       double *values, double *d) {
     if (!M || !nnz) return PP_ERR_INVALID;
-    if (M->host_factor_stale && M->dref) {
+    if (M->host_pending || (M->host_factor_stale && M->dref)) {
         try {
             PP_HIP(hipSetDevice(M->ctx->device));
-            refresh_host_factor(const_cast<pp_precond *>(M));
+            if (M->host_pending) ensure_host_setup(const_cast<pp_precond *>(M));
+            else refresh_host_factor(const_cast<pp_precond *>(M));
         } catch (const Error &e) {
             set_host_error(e.what());
             return e.status;

@@ -446,6 +446,7 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
         throw;
     }
     ReducedHost H;
+    csr_ensure_host(As);  // a matrix built on the device: its host copy first
     // the factor's first level must be exactly the first colour
     const bool ok = As->nrow == As->ncol && reduced_host(As->nrow, As->h_rp, As->h_ci, As->h_v, H) &&
                     M->low64.nlev == 2 && M->up64.nlev == 2 && M->lvl0_contiguous && M->lvl0_lo == 0 &&

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "pp_internal.h"
@@ -246,48 +247,68 @@ int reduced_build_device(pp_ctx *ctx, pp_precond *M, pp_csr *As, ReducedForm *R)
         }
         R->n0 = n0;
         R->n1 = n1;
-        // each operator's arrays go to its CSR object as soon as they exist
-        auto make = [&](pp_csr *&K, uint32_t rows, std::vector<uint32_t> &&h) {
-            uint32_t *krp = nullptr, *kci = nullptr;
-            double *kv = nullptr;
-            try {
-                krp = dalloc<uint32_t>((size_t)rows + 1);
-                kci = dalloc<uint32_t>(h[rows]);
-                kv = dalloc<double>(h[rows]);
-            } catch (...) {
-                dfree(krp);
-                dfree(kci);
-                dfree(kv);
-                throw;
-            }
-            K = csr_from_device(ctx, rows, n, krp, kci, kv, std::move(h));
-        };
+        // the operators' arrays, filled before their CSR objects take them
+        // over (those compute their row blocks' column forms from them)
         std::vector<uint32_t> h_ka(n0 + 1), h_ke(rp.begin(), rp.begin() + n0 + 1), h_kb(n1 + 1);
         for (uint32_t i = 0; i <= n0; ++i) h_ka[i] = rp[i] - i;
         for (uint32_t q = 0; q <= n1; ++q) h_kb[q] = rp[n0 + q] - rp[n0];
         std::vector<uint32_t> h_kf(h_kb);
-        make(R->KA, n0, std::move(h_ka));
-        make(R->KE, n0, std::move(h_ke));
-        make(R->KB, n1, std::move(h_kb));
-        make(R->KF, n1, std::move(h_kf));
-        R->s1_64 = dalloc<double>(n1);
-        R->dt_64 = dalloc<double>(n1);
-        if (!R->dwork) R->dwork = dalloc<double>(n);
-        if (!R->dfail) R->dfail = dalloc<int>(1);
-        k_rd_patterns<<<grid_of(n + 1), kB, 0, ctx->stream>>>(n, n0, As->rp, As->ci, R->KA->rp, R->KA->ci,
-                                                              R->KE->rp, R->KE->ci, R->KB->rp, R->KB->ci,
-                                                              R->KF->rp, R->KF->ci);
+        struct Raw {
+            uint32_t *rp = nullptr, *ci = nullptr;
+            double *v = nullptr;
+        } ra, re, rb, rf;
+        auto raw_free = [&] {
+            for (Raw *r : {&ra, &re, &rb, &rf}) {
+                dfree(r->rp);
+                dfree(r->ci);
+                dfree(r->v);
+                *r = Raw();
+            }
+        };
+        try {
+            for (auto [r, rows, nnz] : {std::make_tuple(&ra, n0, h_ka[n0]), std::make_tuple(&re, n0, h_ke[n0]),
+                                        std::make_tuple(&rb, n1, h_kb[n1]), std::make_tuple(&rf, n1, h_kf[n1])}) {
+                r->rp = dalloc<uint32_t>((size_t)rows + 1);
+                r->ci = dalloc<uint32_t>(std::max<uint32_t>(nnz, 1));
+                r->v = dalloc<double>(std::max<uint32_t>(nnz, 1));
+            }
+            R->s1_64 = dalloc<double>(n1);
+            R->dt_64 = dalloc<double>(n1);
+            if (!R->dwork) R->dwork = dalloc<double>(n);
+            if (!R->dfail) R->dfail = dalloc<int>(1);
+        } catch (...) {
+            raw_free();
+            throw;
+        }
+        k_rd_patterns<<<grid_of(n + 1), kB, 0, ctx->stream>>>(n, n0, As->rp, As->ci, ra.rp, ra.ci, re.rp, re.ci,
+                                                              rb.rp, rb.ci, rf.rp, rf.ci);
         PP_HIP(hipMemsetAsync(R->dfail, 0, sizeof(int), ctx->stream));
         PP_HIP(hipMemsetAsync(u, 0, sizeof(unsigned long long), ctx->stream));
         double *d = R->dwork;
         k_rr_diag<<<grid_of(n), kB, 0, ctx->stream>>>(n, n0, As->rp, As->ci, As->v64, d, R->dfail);
-        k_rr_second<<<grid_of(n1), kB, 0, ctx->stream>>>(n0, n1, As->rp, As->ci, As->v64, d, R->KB->rp,
-                                                         R->KB->v64, R->KF->rp, R->KF->v64, R->dt_64, R->s1_64,
-                                                         R->dfail);
-        k_rr_first<<<grid_of(n0), kB, 0, ctx->stream>>>(n0, As->rp, As->ci, As->v64, d, R->dt_64, R->KA->rp,
-                                                        R->KA->v64, R->KE->rp, R->KE->v64);
+        k_rr_second<<<grid_of(n1), kB, 0, ctx->stream>>>(n0, n1, As->rp, As->ci, As->v64, d, rb.rp, rb.v, rf.rp,
+                                                         rf.v, R->dt_64, R->s1_64, R->dfail);
+        k_rr_first<<<grid_of(n0), kB, 0, ctx->stream>>>(n0, As->rp, As->ci, As->v64, d, R->dt_64, ra.rp, ra.v, re.rp,
+                                                        re.v);
         k_rd_anorm<<<grid_of(n), kB, 0, ctx->stream>>>(n, As->rp, As->v64, (unsigned long long *)u);
-        PP_HIP(hipGetLastError());
+        const hipError_t launched = hipGetLastError();
+        if (launched != hipSuccess) raw_free();
+        PP_HIP(launched);
+        // each CSR object owns its arrays from here on (and frees them if it throws)
+        auto take = [&](pp_csr *&K, uint32_t rows, Raw &r, std::vector<uint32_t> &&h) {
+            Raw t = r;
+            r = Raw();
+            K = csr_from_device(ctx, rows, n, t.rp, t.ci, t.v, std::move(h));
+        };
+        try {
+            take(R->KA, n0, ra, std::move(h_ka));
+            take(R->KE, n0, re, std::move(h_ke));
+            take(R->KB, n1, rb, std::move(h_kb));
+            take(R->KF, n1, rf, std::move(h_kf));
+        } catch (...) {
+            raw_free();
+            throw;
+        }
         unsigned long long an = 0;
         PP_HIP(hipMemcpyAsync(&bad, R->dfail, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         PP_HIP(hipMemcpyAsync(&an, u, sizeof(an), hipMemcpyDeviceToHost, ctx->stream));
@@ -399,25 +420,15 @@ bool reduced_refresh(pp_ctx *ctx, ReducedForm *R, pp_csr *As) {
     if (R->s1_32) launch_convert(ctx, n1, PP_F64, R->s1_64, PP_F32, R->s1_32);
     if (R->dt_32) launch_convert(ctx, n1, PP_F64, R->dt_64, PP_F32, R->dt_32);
     PP_HIP(hipGetLastError());
-    // the largest absolute row sum, from the matrix's host copy of its values
-    double anorm = 0.0;
-    {
-        std::mutex mu;
-        const uint32_t *rp = As->h_rp.data();
-        const double *hv = As->h_v.data();
-        parallel_for(n, 1u << 15, [&](size_t i0, size_t i1) {
-            double m = 0.0;
-            for (size_t i = i0; i < i1; ++i) {
-                double t = 0.0;
-                for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) t += std::fabs(hv[k]);
-                m = std::max(m, t);
-            }
-            std::lock_guard<std::mutex> g(mu);
-            anorm = std::max(anorm, m);
-        });
-    }
-    R->anorm = anorm;
+    // the largest absolute row sum of the new values
+    unsigned long long *an = (unsigned long long *)R->dwork;  // d is no longer read
+    PP_HIP(hipMemsetAsync(an, 0, sizeof(unsigned long long), ctx->stream));
+    k_rd_anorm<<<grid_of(n), kB, 0, ctx->stream>>>(n, As->rp, As->v64, an);
+    PP_HIP(hipGetLastError());
+    unsigned long long bits = 0;
+    PP_HIP(hipMemcpyAsync(&bits, an, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
     PP_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(&R->anorm, &bits, sizeof(double));
     return true;
 }
 

@@ -205,3 +205,70 @@ def test_refactor_refreshes_the_reduced_form(pp, torch):
     print(f"[refactor] reduced form refreshed: {got[0]} outer / {got[1]} inner iterations", flush=True)
     assert got[0] == want[0] and got[1] == want[1]
     np.testing.assert_array_equal(got[2], want[2])
+
+
+def test_device_setup_matches_host_setup(pp, torch, monkeypatch):
+    # IC(0) in multicolour order set up on the device (colouring, P A P^T,
+    # factor, sweeps) against the host setup (PP_SETUP_HOST=1) on variable
+    # values D A D + I/4 of a 40 x 41 x 42 grid: the same applies in both
+    # precisions, the same FP64 and mixed solves, the same factor export,
+    # and a re-factorization on new values that still agrees
+    A = ppo.laplacian(40, 41, 42)
+    n = len(A[0]) - 1
+    rows = np.repeat(np.arange(n), np.diff(A[0]))
+    cols = np.asarray(A[1], np.int64)
+    rng = np.random.default_rng(8)
+    dsc = 1.0 + 0.4 * rng.uniform(-1, 1, n)
+    v = np.asarray(A[2], np.float64) * dsc[rows] * dsc[cols]
+    v[A[1] == rows] += 0.25
+    v2 = v * (1.0 + 0.1 * rng.uniform(-1, 1, len(v)))
+    v2 = 0.5 * (v2 + v2[_transpose_positions(A)])  # symmetric again
+    b = ppo.spmv((A[0], A[1], v), ppo.normal(n, 4))
+    x0 = torch.tensor(ppo.normal(n, 5), device="cuda")
+    out = {}
+    for where in ("host", "device"):
+        if where == "host":
+            monkeypatch.setenv("PP_SETUP_HOST", "1")
+        else:
+            monkeypatch.delenv("PP_SETUP_HOST", raising=False)
+        print(f"[setup] {where} n={n}", flush=True)
+        M = pp.CSRMatrix(A[0], A[1], v)
+        P = pp.IC0(M, ordering=1, make_f32=True)
+        P.setup()
+        info = P.info()
+        assert info.setup_on_device == (where == "device")
+        rec = {"levels": (info.levels_lower, info.levels_upper, info.fused_levels, info.mnnz)}
+        rec["apply64"] = P.MInvSolve(x0.clone()).cpu().numpy()
+        rec["apply32"] = P.MInvSolve(x0.clone().float()).cpu().numpy()
+        res = []
+        for prec in ("fp64", "mixed"):
+            cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-9, maxIter=3000, precision=prec)
+            x = torch.zeros(n, dtype=torch.float64, device="cuda")
+            cg.solve(M, torch.tensor(b, device="cuda"), x)
+            assert cg.getConvergence(), (where, prec)
+            res.append((cg.getNumIter(), cg.report.inner_iters, x.cpu().numpy()))
+        rec["solves"] = res
+        M.set_values(v2)
+        P.refactor()
+        rec["refactor"] = P.MInvSolve(x0.clone()).cpu().numpy()
+        rec["factor"] = P.factor(0)  # the host copy (built on demand after a device setup)
+        out[where] = rec
+    h, d = out["host"], out["device"]
+    assert h["levels"] == d["levels"]
+    for key in ("apply64", "apply32", "refactor"):
+        np.testing.assert_array_equal(h[key], d[key], err_msg=key)
+    for (ih, jh, xh), (idv, jd, xd) in zip(h["solves"], d["solves"]):
+        assert (ih, jh) == (idv, jd)
+        np.testing.assert_array_equal(xh, xd)
+    for a, c in zip(h["factor"], d["factor"]):
+        np.testing.assert_array_equal(a, c)
+
+
+def _transpose_positions(A):
+    # position of entry (j, i) for every entry (i, j) of a symmetric pattern
+    rp, ci = np.asarray(A[0], np.int64), np.asarray(A[1], np.int64)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    key = rows * (len(rp) - 1) + ci
+    tkey = ci * (len(rp) - 1) + rows
+    order = np.argsort(key)
+    return order[np.searchsorted(key[order], tkey)]

@@ -110,30 +110,52 @@ __global__ void k_sd_iota(uint32_t n, uint32_t *a) {
     for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) a[i] = i;
 }
 
-// one round of the greedy colouring: a row whose lower neighbours all have
-// their colours takes the smallest one none of them has (fail bit 16: all
-// 64 colours taken)
-__global__ void k_sd_colour(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
-                            int *colour, int *fail) {
-    volatile int *vc = colour;
-    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
-        if (vc[i] >= 0) continue;
-        unsigned long long m = 0;
-        bool ready = true;
-        for (uint32_t k = rp[i]; k < rp[i + 1] && ready; ++k) {
-            const uint32_t j = ci[k];
-            if (j >= i) continue;
-            const int c = vc[j];
-            if (c < 0) ready = false;
-            else m |= 1ull << c;
+// The greedy colouring in one launch, sync-free: one thread per row (no
+// grid stride, so a row waits only on rows of its own or earlier
+// workgroups, which the in-order dispatch has started), each reading its
+// lower neighbours' colours as they appear (agent-scope atomics, visible
+// across the XCDs' caches) and taking the smallest colour none of them
+// has.  fail bit 16: all 64 colours taken; bit 128: a wait outlasted the
+// deadline (the host path then colours).
+__global__ __launch_bounds__(kB) void k_sd_colour(uint32_t n, const uint32_t *__restrict__ rp,
+                                                  const uint32_t *__restrict__ ci, int *colour, long long deadline,
+                                                  int *fail) {
+    const uint32_t i = blockIdx.x * kB + threadIdx.x;
+    bool done = i >= n;
+    uint32_t e = 0, e1 = 0;
+    unsigned long long m = 0;
+    if (!done) {
+        e = rp[i];
+        e1 = rp[i + 1];
+    }
+    const long long t0 = wall_clock64();
+    for (unsigned iter = 0;; ++iter) {
+        if (!done) {
+            while (e < e1) {
+                const uint32_t j = ci[e];
+                if (j < i) {
+                    const int c = __hip_atomic_load(&colour[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (c < 0) break;
+                    m |= 1ull << c;
+                }
+                ++e;
+            }
+            if (e == e1) {
+                int c = 0;
+                if (m == ~0ull) atomicOr(fail, 16);
+                else c = __ffsll((long long)~m) - 1;
+                __hip_atomic_store(&colour[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                done = true;
+            }
         }
-        if (!ready) continue;
-        if (m == ~0ull) {
-            atomicOr(fail, 16);
-            vc[i] = 0;
-        } else {
-            vc[i] = __ffsll((long long)~m) - 1;
+        if (__all(done)) break;
+        if (wall_clock64() - t0 > deadline) {
+            if (!done) atomicOr(fail, 128);
+            break;
         }
+        // another wave already gave up: its dependents would wait in vain
+        if ((iter & 63) == 63 && (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 128)) break;
+        __builtin_amdgcn_s_sleep(1);
     }
 }
 
@@ -298,9 +320,13 @@ __global__ void k_sd_diag_map(uint32_t n, uint32_t nnz, uint32_t *map) {
     for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) map[i] = nnz + i;
 }
 
-// count[lev[i]]++
-__global__ void k_sd_hist(uint32_t n, const int *__restrict__ lev, uint32_t *count) {
-    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) atomicAdd(count + lev[i], 1u);
+// level bounds from the sorted keys: start[l] = the first position of a key
+// >= l (start[nlev] = n), written where the key steps up
+__global__ void k_sd_bounds(uint32_t n, uint32_t nlev, const uint32_t *__restrict__ key, uint32_t *start) {
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k <= n; k += gridDim.x * kB) {
+        const uint32_t lo = k == 0 ? 0 : key[k - 1] + 1, hi = k == n ? nlev : key[k];
+        for (uint32_t l = lo; l <= hi && l <= nlev; ++l) start[l] = k;
+    }
 }
 
 // the first row whose first entry lies left of the diagonal (the colour split)
@@ -453,15 +479,12 @@ void level_order(pp_ctx *ctx, Arena &ar, uint32_t n, int *lev, uint32_t nlev, co
                  std::vector<uint32_t> &lptr) {
     uint32_t *kout = ar.get<uint32_t>(n);
     sort_pairs(ctx, (const uint32_t *)lev, kout, iota, perm, n, bits_for(nlev));
-    uint32_t *cnt = ar.get<uint32_t>(nlev);
-    PP_HIP(hipMemsetAsync(cnt, 0, nlev * sizeof(uint32_t), ctx->stream));
-    k_sd_hist<<<grid_of(n), kB, 0, ctx->stream>>>(n, lev, cnt);
+    uint32_t *start = ar.get<uint32_t>(nlev + 1);
+    k_sd_bounds<<<grid_of(n + 1), kB, 0, ctx->stream>>>(n, nlev, kout, start);
     PP_HIP(hipGetLastError());
-    std::vector<uint32_t> h(nlev);
-    PP_HIP(hipMemcpyAsync(h.data(), cnt, nlev * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    PP_HIP(hipStreamSynchronize(ctx->stream));
     lptr.assign(nlev + 1, 0);
-    for (uint32_t l = 0; l < nlev; ++l) lptr[l + 1] = lptr[l] + h[l];
+    PP_HIP(hipMemcpyAsync(lptr.data(), start, (nlev + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 // The level-ordered sweep d from its device arrays (perm, rp, ci taken
@@ -625,11 +648,22 @@ bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M) {
     // colouring and the colour order
     int *colour = ar.get<int>(n);
     k_sd_fill<<<grid_of(n), kB, 0, ctx->stream>>>(n, colour, -1);
+    k_sd_colour<<<(n + kB - 1) / kB, kB, 0, ctx->stream>>>(n, A->rp, A->ci, colour, ctx->spin_deadline, fail);
+    PP_HIP(hipGetLastError());
+    // more than 64 colours (the host's other colouring) or a wait that gave up
+    if (read_int(ctx, fail)) return false;
     int maxc = 0;
-    if (!rounds(ctx, n, colour, st,
-                [&] { k_sd_colour<<<grid_of(n), kB, 0, ctx->stream>>>(n, A->rp, A->ci, colour, fail); }, maxc))
-        return false;
-    if (read_int(ctx, fail)) return false;  // more than 64 colours: the host's other colouring
+    PP_HIP(hipMemsetAsync(st, 0, sizeof(int), ctx->stream));
+    PP_HIP(hipMemsetAsync(st + 1, 0xff, sizeof(int), ctx->stream));
+    k_sd_scan_state<<<grid_of(n), kB, 0, ctx->stream>>>(n, colour, st, st + 1);
+    PP_HIP(hipGetLastError());
+    {
+        int h[2];
+        PP_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        if (h[0]) return false;  // a row left without a colour
+        maxc = h[1];
+    }
     const uint32_t ncolors = (uint32_t)maxc + 1;
     uint32_t *iota = ar.get<uint32_t>(n);
     k_sd_iota<<<grid_of(n), kB, 0, ctx->stream>>>(n, iota);

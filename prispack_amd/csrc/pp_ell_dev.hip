@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <vector>
 
@@ -160,12 +161,23 @@ bool ell_build_device(pp_ctx *ctx, const pp_csr *A, bool two_colour, uint32_t sp
     // the host construction's conditions on the row lengths
     if (two_colour && (split == 0 || split >= n)) return false;
     const uint32_t tail = two_colour ? 1 : 0;
-    uint32_t width = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t len = rp[i + 1] - rp[i];
-        if (len <= tail) return false;  // a row with no sliced entry
-        width = std::max(width, len - tail);
-    }
+    std::atomic<uint32_t> wmax{0};
+    std::atomic<bool> empty_row{false};  // a row with no sliced entry
+    parallel_for(n, 1u << 16, [&](size_t i0, size_t i1) {
+        uint32_t m = 0;
+        bool e = false;
+        for (size_t i = i0; i < i1; ++i) {
+            const uint32_t len = rp[i + 1] - rp[i];
+            e |= len <= tail;
+            m = std::max(m, len);
+        }
+        if (e) empty_row = true;
+        uint32_t cur = wmax.load();
+        while (m > cur && !wmax.compare_exchange_weak(cur, m)) {
+        }
+    });
+    if (empty_row.load()) return false;
+    const uint32_t width = wmax.load() - tail;
     if (width > 16 || (uint64_t)width * n >= (1ull << 32)) return false;
     if ((double)width * n > max_fill * (double)(rp[n] - (uint64_t)tail * n)) return false;
     const Form f{n, width, two_colour ? split : 0u, two_colour ? 1 : 0};

@@ -5,6 +5,7 @@
 #include <type_traits>
 
 #include <algorithm>
+#include <atomic>
 
 #include "pp_internal.h"
 
@@ -658,6 +659,22 @@ void ell_fill_values(pp_ctx *ctx, const EllOp &E, int dtype, const uint32_t *rp,
 void build_row_blocks(const std::vector<uint32_t> &rp, std::vector<uint32_t> &blocks) {
     blocks.clear();
     const uint32_t n = (uint32_t)rp.size() - 1;
+    // rows of at most kSpmvNnzBlock / kSpmvRowsBlock entries: every block
+    // takes kSpmvRowsBlock rows (the rule below, without its serial pass)
+    std::atomic<uint32_t> longest{0};
+    parallel_for(n, 1u << 16, [&](size_t i0, size_t i1) {
+        uint32_t m = 0;
+        for (size_t i = i0; i < i1; ++i) m = std::max(m, rp[i + 1] - rp[i]);
+        uint32_t cur = longest.load();
+        while (m > cur && !longest.compare_exchange_weak(cur, m)) {
+        }
+    });
+    if (longest.load() <= kSpmvNnzBlock / kSpmvRowsBlock) {
+        blocks.reserve(n / kSpmvRowsBlock + 2);
+        for (uint32_t r = 0; r < n; r += kSpmvRowsBlock) blocks.push_back(r);
+        blocks.push_back(n);
+        return;
+    }
     uint32_t r = 0;
     blocks.push_back(0);
     while (r < n) {

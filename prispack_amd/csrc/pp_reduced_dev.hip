@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -234,9 +235,22 @@ int reduced_build_device(pp_ctx *ctx, pp_precond *M, pp_csr *As, ReducedForm *R)
         // renumbered for the second colour) when the widths agree; else the
         // host construction runs
         const EllOp &S = As->ell;
-        uint32_t wa = 0, wb = 0;
-        for (uint32_t i = 0; i < n0; ++i) wa = std::max(wa, rp[i + 1] - rp[i] - 1);
-        for (uint32_t i = n0; i < n; ++i) wb = std::max(wb, rp[i + 1] - rp[i] - 1);
+        std::atomic<uint32_t> wmax[2];
+        wmax[0] = 0;
+        wmax[1] = 0;
+        parallel_for(n, 1u << 16, [&](size_t i0, size_t i1) {
+            uint32_t m[2] = {0, 0};
+            for (size_t i = i0; i < i1; ++i) {
+                uint32_t &w = m[i < n0 ? 0 : 1];
+                w = std::max(w, rp[i + 1] - rp[i] - 1);
+            }
+            for (int c = 0; c < 2; ++c) {
+                uint32_t cur = wmax[c].load();
+                while (m[c] > cur && !wmax[c].compare_exchange_weak(cur, m[c])) {
+                }
+            }
+        });
+        const uint32_t wa = wmax[0].load(), wb = wmax[1].load();
         const bool fits = S.ok && S.pid && S.csplit == n0 && S.dsplit == n0 && S.cbase == (int64_t)n0 &&
                           S.cbase1 == -(int64_t)n0 && S.dbase == 0 && wa == S.width && wb == S.width &&
                           (double)wa * n0 <= 1.25 * (double)(rp[n0] - (uint64_t)n0) &&
@@ -249,10 +263,13 @@ int reduced_build_device(pp_ctx *ctx, pp_precond *M, pp_csr *As, ReducedForm *R)
         R->n1 = n1;
         // the operators' arrays, filled before their CSR objects take them
         // over (those compute their row blocks' column forms from them)
-        std::vector<uint32_t> h_ka(n0 + 1), h_ke(rp.begin(), rp.begin() + n0 + 1), h_kb(n1 + 1);
-        for (uint32_t i = 0; i <= n0; ++i) h_ka[i] = rp[i] - i;
-        for (uint32_t q = 0; q <= n1; ++q) h_kb[q] = rp[n0 + q] - rp[n0];
-        std::vector<uint32_t> h_kf(h_kb);
+        std::vector<uint32_t> h_ka(n0 + 1), h_ke(rp.begin(), rp.begin() + n0 + 1), h_kb(n1 + 1), h_kf(n1 + 1);
+        parallel_for((size_t)n + 2, 1u << 16, [&](size_t t0, size_t t1) {
+            for (size_t t = t0; t < t1; ++t) {
+                if (t <= n0) h_ka[t] = rp[t] - (uint32_t)t;
+                else h_kb[t - n0 - 1] = h_kf[t - n0 - 1] = rp[t - 1] - rp[n0];
+            }
+        });
         struct Raw {
             uint32_t *rp = nullptr, *ci = nullptr;
             double *v = nullptr;

@@ -422,6 +422,63 @@ __global__ __launch_bounds__(kTB) void k_two_base(const uint32_t *__restrict__ b
     }
 }
 
+// statistics of one level (positions [k0, k1)): st[0] the longest row,
+// st[1] the shortest, st[2] set when the rows are not consecutive in the
+// original numbering
+__global__ void k_sd_level_stats(uint32_t k0, uint32_t k1, const uint32_t *__restrict__ perm,
+                                 const uint32_t *__restrict__ rp, uint32_t *st) {
+    uint32_t mx = 0, mn = UINT32_MAX, gap = 0;
+    for (uint32_t k = k0 + blockIdx.x * kB + threadIdx.x; k < k1; k += gridDim.x * kB) {
+        const uint32_t len = rp[k + 1] - rp[k];
+        mx = max(mx, len);
+        mn = min(mn, len);
+        if (k > k0 && perm[k] != perm[k - 1] + 1) gap = 1;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, o));
+        gap |= (uint32_t)__shfl_xor((int)gap, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(st, mx);
+        atomicMin(st + 1, mn);
+        if (gap) atomicOr(st + 2, 1u);
+    }
+}
+
+// per level l: heads[3l] = perm[lp[l]], heads[3l+1] = rp[lp[l]],
+// heads[3l+2] = ci[rp[lp[l]]] (when the level has entries); heads[3 nlev] = rp[n]
+__global__ void k_sd_level_heads(uint32_t nlev, uint32_t n, const uint32_t *__restrict__ lp,
+                                 const uint32_t *__restrict__ perm, const uint32_t *__restrict__ rp,
+                                 const uint32_t *__restrict__ ci, uint32_t *heads) {
+    const uint32_t l = blockIdx.x * kB + threadIdx.x;
+    if (l > nlev) return;
+    if (l == nlev) {
+        heads[3 * nlev] = rp[n];
+        return;
+    }
+    const uint32_t k = lp[l];
+    heads[3 * l] = k < n ? perm[k] : 0;
+    heads[3 * l + 1] = rp[k];
+    heads[3 * l + 2] = k < lp[l + 1] && rp[k] < rp[lp[l + 1]] ? ci[rp[k]] : 0;
+}
+
+// descriptors of nb row blocks (bounds blk [nb + 1]): first position, first
+// entry, the 16-bit bases (kNoBase without them); the last closes the final block
+__global__ void k_sd_descs(uint32_t nb, const uint32_t *__restrict__ blk, const uint32_t *__restrict__ rp,
+                           const uint32_t *__restrict__ base, BlockDesc *desc) {
+    for (uint32_t j = blockIdx.x * kB + threadIdx.x; j <= nb; j += gridDim.x * kB) {
+        const bool has = j < nb && base;
+        desc[j] = {blk[j], rp[blk[j]], has ? base[2 * j] : kNoBase, has ? base[2 * j + 1] : kNoBase};
+    }
+}
+
+// flag when a[0, cnt) and b[0, cnt) differ
+__global__ void k_sd_differ(uint32_t cnt, const uint32_t *__restrict__ a, const uint32_t *__restrict__ b, int *flag) {
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < cnt; k += gridDim.x * kB)
+        if (a[k] != b[k]) atomicOr(flag, 1);
+}
+
 // ---- owned device arrays, freed unless handed over
 struct Arena {
     std::vector<void *> p;
@@ -489,48 +546,75 @@ void level_order(pp_ctx *ctx, Arena &ar, uint32_t n, int *lev, uint32_t nlev, co
 
 // The level-ordered sweep d from its device arrays (perm, rp, ci taken
 // over; values gathered through map from the factor array): the host
-// upload_sweep's layout.
-std::vector<uint32_t> finish_sweep(pp_ctx *ctx, Arena &ar, DevSweep<double> &d, uint32_t n,
-                                   const std::vector<uint32_t> &lp, uint32_t *perm, uint32_t *rp, uint32_t *ci,
-                                   const uint32_t *map, const double *fval, int *fail) {
+// upload_sweep's layout, its per-level facts reduced on the device (the
+// row pointers cross to the host only for a wide level whose rows do not
+// all fit the uniform row blocks).
+void finish_sweep(pp_ctx *ctx, Arena &ar, DevSweep<double> &d, uint32_t n, const std::vector<uint32_t> &lp,
+                  uint32_t *perm, uint32_t *rp, uint32_t *ci, const uint32_t *map, const double *fval, int *fail) {
     d.free_all();
     d.n = n;
     d.nlev = (uint32_t)lp.size() - 1;
     d.level_ptr = lp;
-    std::vector<uint32_t> hperm(n), hrp(n + 1);
-    PP_HIP(hipMemcpyAsync(hperm.data(), perm, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    PP_HIP(hipMemcpyAsync(hrp.data(), rp, (n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    PP_HIP(hipStreamSynchronize(ctx->stream));
-    d.nnz = hrp[n];
+    const uint32_t nlev = d.nlev;
     d.perm = ar.keep(perm);
     d.rp = ar.keep(rp);
     d.ci = ar.keep(ci);
+    // per-level statistics and heads
+    uint32_t *st = ar.get<uint32_t>(3 * (size_t)nlev), *dlp = ar.get<uint32_t>(nlev + 1);
+    uint32_t *heads = ar.get<uint32_t>(3 * (size_t)nlev + 1);
+    std::vector<uint32_t> init(3 * (size_t)nlev);
+    for (uint32_t l = 0; l < nlev; ++l) {
+        init[3 * l] = 0;
+        init[3 * l + 1] = UINT32_MAX;
+        init[3 * l + 2] = 0;
+    }
+    PP_HIP(hipMemcpyAsync(st, init.data(), init.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    PP_HIP(hipMemcpyAsync(dlp, lp.data(), lp.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    for (uint32_t l = 0; l < nlev; ++l)
+        if (lp[l + 1] > lp[l])
+            k_sd_level_stats<<<std::min<uint32_t>((lp[l + 1] - lp[l] + kB - 1) / kB, 2048), kB, 0, ctx->stream>>>(
+                lp[l], lp[l + 1], perm, rp, st + 3 * l);
+    k_sd_level_heads<<<(nlev + kB) / kB, kB, 0, ctx->stream>>>(nlev, n, dlp, perm, rp, ci, heads);
+    PP_HIP(hipGetLastError());
+    std::vector<uint32_t> hst(3 * (size_t)nlev), hh(3 * (size_t)nlev + 1);
+    PP_HIP(hipMemcpyAsync(hst.data(), st, hst.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipMemcpyAsync(hh.data(), heads, hh.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));  // init leaves scope
+    d.nnz = hh[3 * nlev];
+    auto level_entries = [&](uint32_t l) { return (l + 1 < nlev ? hh[3 * (l + 1) + 1] : d.nnz) - hh[3 * l + 1]; };
     d.val = dalloc<double>(std::max<uint32_t>(d.nnz, 1));
     launch_gather_map(ctx, d.nnz, map, fval, PP_F64, d.val);
     d.max_level_rows = 0;
-    for (uint32_t l = 0; l < d.nlev; ++l) d.max_level_rows = std::max(d.max_level_rows, lp[l + 1] - lp[l]);
-    d.level_off.assign(d.nlev, DevSweep<double>::kNoOffset);
-    for (uint32_t l = 0; l < d.nlev; ++l) {
-        bool contiguous = true;
-        for (uint32_t k = lp[l] + 1; k < lp[l + 1] && contiguous; ++k) contiguous = hperm[k] == hperm[k - 1] + 1;
-        if (contiguous && lp[l + 1] > lp[l]) d.level_off[l] = (int64_t)hperm[lp[l]] - (int64_t)lp[l];
-    }
-    // row blocks of the wide levels (the host's rule)
-    std::vector<uint32_t> blk;
-    d.level_blk.assign(2 * (size_t)d.nlev, 0);
-    for (uint32_t l = 0; l < d.nlev; ++l) {
+    for (uint32_t l = 0; l < nlev; ++l) d.max_level_rows = std::max(d.max_level_rows, lp[l + 1] - lp[l]);
+    d.level_off.assign(nlev, DevSweep<double>::kNoOffset);
+    for (uint32_t l = 0; l < nlev; ++l)
+        if (lp[l + 1] > lp[l] && !hst[3 * l + 2]) d.level_off[l] = (int64_t)hh[3 * l] - (int64_t)lp[l];
+    // row blocks of the wide levels (the host's rule; uniform when every row
+    // of the level fits kSpmvNnzBlock / kSpmvRowsBlock entries)
+    std::vector<uint32_t> blk, hrp;
+    d.level_blk.assign(2 * (size_t)nlev, 0);
+    for (uint32_t l = 0; l < nlev; ++l) {
         if (lp[l + 1] - lp[l] < kStreamLevelRows) continue;
         uint32_t k = lp[l];
         if (blk.empty() || blk.back() != k) blk.push_back(k);
         d.level_blk[2 * l] = (uint32_t)blk.size() - 1;
-        while (k < lp[l + 1]) {
-            const uint32_t start = k, e0 = hrp[k];
-            if (hrp[k + 1] - e0 > kSpmvNnzBlock) {
-                ++k;
-            } else {
-                while (k < lp[l + 1] && k - start < kSpmvRowsBlock && hrp[k + 1] - e0 <= kSpmvNnzBlock) ++k;
+        if (hst[3 * l] <= kSpmvNnzBlock / kSpmvRowsBlock) {
+            for (k += kSpmvRowsBlock; k < lp[l + 1]; k += kSpmvRowsBlock) blk.push_back(k);
+            blk.push_back(lp[l + 1]);
+        } else {
+            if (hrp.empty()) {
+                hrp.resize((size_t)n + 1);
+                PP_HIP(hipMemcpy(hrp.data(), rp, ((size_t)n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
             }
-            blk.push_back(k);
+            while (k < lp[l + 1]) {
+                const uint32_t start = k, e0 = hrp[k];
+                if (hrp[k + 1] - e0 > kSpmvNnzBlock) {
+                    ++k;
+                } else {
+                    while (k < lp[l + 1] && k - start < kSpmvRowsBlock && hrp[k + 1] - e0 <= kSpmvNnzBlock) ++k;
+                }
+                blk.push_back(k);
+            }
         }
         d.level_blk[2 * l + 1] = (uint32_t)blk.size() - 1;
     }
@@ -547,34 +631,25 @@ std::vector<uint32_t> finish_sweep(pp_ctx *ctx, Arena &ar, DevSweep<double> &d, 
             d.ci16 = off16;
             d.blk_base = dbase;
         }
-        const std::vector<BlockDesc> ds = block_descs(blk, hrp, base);
-        d.desc = dalloc<BlockDesc>(ds.size());
-        PP_HIP(hipMemcpyAsync(d.desc, ds.data(), ds.size() * sizeof(BlockDesc), hipMemcpyHostToDevice,
-                              ctx->stream));
-        PP_HIP(hipStreamSynchronize(ctx->stream));  // blk, ds leave scope
+        d.desc = dalloc<BlockDesc>(blk.size());
+        k_sd_descs<<<grid_of(blk.size()), kB, 0, ctx->stream>>>((uint32_t)d.nblk, d.blocks, d.rp, d.blk_base, d.desc);
+        PP_HIP(hipGetLastError());
+        PP_HIP(hipStreamSynchronize(ctx->stream));  // blk leaves scope
     }
-    for (uint32_t l = 0; l < d.nlev; ++l)
+    for (uint32_t l = 0; l < nlev; ++l)
         if (d.level_off[l] == DevSweep<double>::kNoOffset) d.perm_rows += lp[l + 1] - lp[l];
     // the pivots (original numbering), gathered by the caller
     d.diag = dalloc<double>(n);
     // sliced-column levels (factors large enough for the setup to time them)
-    d.ell.assign(d.nlev, DevSweep<double>::EllLevel());
-    for (uint32_t l = 0; l < d.nlev && n >= kTuneSweepRows; ++l) {
+    d.ell.assign(nlev, DevSweep<double>::EllLevel());
+    for (uint32_t l = 0; l < nlev && n >= kTuneSweepRows; ++l) {
         const uint32_t k0 = lp[l], k1 = lp[l + 1], nr = k1 - k0;
         if (nr < kStreamLevelRows || d.level_off[l] == DevSweep<double>::kNoOffset) continue;
         const int64_t row0 = (int64_t)k0 + d.level_off[l];
-        uint32_t w = 0;
-        bool ok = true;
-        for (uint32_t k = k0; k < k1 && ok; ++k) {
-            w = std::max(w, hrp[k + 1] - hrp[k]);
-            ok = hrp[k + 1] > hrp[k];
-        }
-        if (!ok || w > 16 || (double)w * nr > 1.25 * (double)(hrp[k1] - hrp[k0])) continue;
-        uint32_t c0 = 0;
-        PP_HIP(hipMemcpyAsync(&c0, d.ci + hrp[k0], sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        const uint32_t w = hst[3 * l], entries = level_entries(l);
+        if (hst[3 * l + 1] == 0 || w > 16 || (double)w * nr > 1.25 * (double)entries) continue;
+        const int64_t cbase = (int64_t)hh[3 * l + 2] - row0;
         PP_HIP(hipMemsetAsync(fail, 0, sizeof(int), ctx->stream));
-        PP_HIP(hipStreamSynchronize(ctx->stream));
-        const int64_t cbase = (int64_t)c0 - row0;
         int16_t *off = dalloc<int16_t>((size_t)w * nr);
         k_sd_ell_level<<<grid_of(nr), kB, 0, ctx->stream>>>(nr, w, d.rp + k0, d.ci, row0, cbase, off, fail);
         PP_HIP(hipGetLastError());
@@ -584,7 +659,7 @@ std::vector<uint32_t> finish_sweep(pp_ctx *ctx, Arena &ar, DevSweep<double> &d, 
         }
         auto &E = d.ell[l];
         E.nrows = nr;
-        E.entries = hrp[k1] - hrp[k0];
+        E.entries = entries;
         E.width = w;
         E.row0 = row0;
         E.cbase = cbase;
@@ -592,7 +667,6 @@ std::vector<uint32_t> finish_sweep(pp_ctx *ctx, Arena &ar, DevSweep<double> &d, 
         E.val = dalloc<double>((size_t)w * nr);
         E.ok = true;
     }
-    return hperm;
 }
 
 }  // namespace
@@ -746,10 +820,8 @@ bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M) {
     k_sd_diag_map<<<grid_of(n), kB, 0, ctx->stream>>>(n, nl, dmap);
     PP_HIP(hipGetLastError());
     // everything that can fail has run: hand the arrays to the preconditioner
-    const std::vector<uint32_t> hp_lo = finish_sweep(ctx, ar, M->low64, n, lp_lo, perm_lo, srp_lo, sci_lo, map_lo,
-                                                     fval, fail);
-    const std::vector<uint32_t> hp_up = finish_sweep(ctx, ar, M->up64, n, lp_up, perm_up, srp_up, sci_up, map_up,
-                                                     fval, fail);
+    finish_sweep(ctx, ar, M->low64, n, lp_lo, perm_lo, srp_lo, sci_lo, map_lo, fval, fail);
+    finish_sweep(ctx, ar, M->up64, n, lp_up, perm_up, srp_up, sci_up, map_up, fval, fail);
     for (DevSweep<double> *d : {&M->low64, &M->up64}) launch_gather_map(ctx, n, dmap, fval, PP_F64, d->diag);
     PP_HIP(hipStreamSynchronize(ctx->stream));  // ell_fill_rows runs on the null stream
     for (DevSweep<double> *d : {&M->low64, &M->up64})
@@ -763,9 +835,15 @@ bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M) {
     M->fuse_rows = 0;
     if (lp_lo.size() >= 2 && lp_up.size() >= 2) {
         const uint32_t a0 = lp_lo[lp_lo.size() - 2], a1 = lp_lo.back(), b0 = lp_up[0], b1 = lp_up[1];
-        if (a1 - a0 == b1 - b0 && std::equal(hp_lo.begin() + a0, hp_lo.begin() + a1, hp_up.begin() + b0)) {
-            M->fuse_mid = true;
-            M->fuse_rows = a1 - a0;
+        if (a1 - a0 == b1 - b0) {
+            PP_HIP(hipMemsetAsync(fail, 0, sizeof(int), ctx->stream));
+            k_sd_differ<<<grid_of(a1 - a0), kB, 0, ctx->stream>>>(a1 - a0, M->low64.perm + a0, M->up64.perm + b0,
+                                                                   fail);
+            PP_HIP(hipGetLastError());
+            if (!read_int(ctx, fail)) {
+                M->fuse_mid = true;
+                M->fuse_rows = a1 - a0;
+            }
         }
     }
     M->mnnz = (uint32_t)(2 * ((uint64_t)nl + n) + n);  // the host factor's count (L^T with its pivots, twice, + n)

@@ -99,12 +99,15 @@ __global__ void k_rr_first(uint32_t n0, const uint32_t *__restrict__ rp, const u
 // n0: the first row holding an entry left of its diagonal
 __global__ void k_rd_split(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
                            uint32_t *n0) {
+    uint32_t m = UINT32_MAX;
     for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
         for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
             if (ci[k] < i) {
-                atomicMin(n0, i);
+                m = min(m, i);
                 break;
             }
+    for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0 && m != UINT32_MAX) atomicMin(n0, m);
 }
 
 // the two-colour structure: one diagonal entry per row, every other entry
@@ -165,21 +168,46 @@ __global__ void k_rd_patterns(uint32_t n, uint32_t n0, const uint32_t *__restric
 // as std::max skips them); non-negative doubles order as their bit patterns
 __global__ void k_rd_anorm(uint32_t n, const uint32_t *__restrict__ rp, const double *__restrict__ v,
                            unsigned long long *out) {
+    unsigned long long m = 0;  // the bits of +0.0
     for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
         double t = 0.0;
         for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) t += fabs(v[k]);
-        if (t == t) atomicMax(out, (unsigned long long)__double_as_longlong(t));
+        if (t == t) m = max(m, (unsigned long long)__double_as_longlong(t));
     }
+    for (int o = 32; o > 0; o >>= 1) {  // one atomic per wavefront
+        const unsigned long long h = (unsigned long long)(uint32_t)__shfl_xor((int)(uint32_t)(m >> 32), o) << 32;
+        m = max(m, h | (uint32_t)__shfl_xor((int)(uint32_t)m, o));
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
-// the highest pattern number in pid[0, n)
+// the highest pattern number in pid[0, n) (one atomic per wavefront: the
+// rows mostly share one pattern, so per-row atomics would queue on one word)
 __global__ void k_rd_max_pid(uint32_t n, const uint8_t *__restrict__ pid, uint32_t *out) {
-    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) atomicMax(out, (uint32_t)pid[i]);
+    uint32_t m = 0;
+    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) m = max(m, (uint32_t)pid[i]);
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
-// first[p] = the first row of pid[0, n) with pattern p
+// first[p] = the first row of pid[0, n) with pattern p: the lanes of a
+// wavefront sharing a pattern are represented by their lowest lane (its row
+// is their smallest), one atomic per pattern and wavefront
 __global__ void k_rd_first_seen(uint32_t n, const uint8_t *__restrict__ pid, uint32_t *first) {
-    for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) atomicMin(first + pid[i], i);
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = blockIdx.x * kB + (threadIdx.x & ~63u); base < n; base += gridDim.x * kB) {
+        const uint32_t i = base + lane;
+        const bool valid = i < n;
+        const int p = valid ? (int)pid[i] : -1;
+        unsigned long long pending = __ballot(valid);
+        while (pending) {
+            const int leader = __ffsll((long long)pending) - 1;
+            const int lp = __shfl(p, leader);
+            const unsigned long long same = __ballot(valid && p == lp) & pending;
+            if ((int)lane == leader) atomicMin(first + lp, i);
+            pending &= ~same;
+        }
+    }
 }
 
 __global__ void k_rd_remap(uint32_t n, const uint8_t *__restrict__ pid, const uint8_t *__restrict__ map,

@@ -189,8 +189,14 @@ __global__ void k_sd_scan_state(uint32_t n, const int *__restrict__ a, int *flag
         neg |= v < 0;
         m = max(m, v);
     }
-    if (neg) atomicOr(flag, 1);
-    atomicMax(mx, m);
+    for (int o = 32; o > 0; o >>= 1) {  // one atomic per wavefront
+        m = max(m, __shfl_xor(m, o));
+        neg |= __shfl_xor((int)neg, o) != 0;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (neg) atomicOr(flag, 1);
+        atomicMax(mx, m);
+    }
 }
 
 __global__ void k_sd_inverse(uint32_t n, const uint32_t *__restrict__ order, uint32_t *inv) {
@@ -331,8 +337,11 @@ __global__ void k_sd_bounds(uint32_t n, uint32_t nlev, const uint32_t *__restric
 
 // the first row whose first entry lies left of the diagonal (the colour split)
 __global__ void k_sd_split(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci, uint32_t *s) {
+    uint32_t m = UINT32_MAX;
     for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
-        if (rp[i] < rp[i + 1] && ci[rp[i]] < i) atomicMin(s, i);
+        if (rp[i] < rp[i + 1] && ci[rp[i]] < i) m = min(m, i);
+    for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o));
+    if ((threadIdx.x & 63) == 0 && m != UINT32_MAX) atomicMin(s, m);
 }
 
 // offsets of one sliced-column sweep level (the host's upload_sweep loop):

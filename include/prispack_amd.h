@@ -229,8 +229,8 @@ typedef struct {
                                on the host, 2 built on the device, 3 refreshed on the
                                device after new values */
     float reduced_form_ms;  /* the last build or refresh of those operators */
-    int32_t setup_on_device; /* 1: setup() ran on the device (IC(0) in multicolour order;
-                               the host copy of the factor is built when asked for) */
+    int32_t setup_on_device; /* 1: setup() ran on the device (IC(0) / ILU(0) in multicolour
+                               order; the host copy of the factor is built when asked for) */
 } pp_precond_info;
 
 void pp_precond_opts_default(pp_precond_opts *o);

@@ -656,12 +656,12 @@ void csr_ensure_host(pp_csr *A);
 // the bases on the host.
 uint64_t two_base_device(pp_ctx *ctx, uint32_t nblk, const uint32_t *blocks, const uint32_t *rp, const uint32_t *ci,
                          uint32_t nnz, uint16_t *&off16, uint32_t *&dbase, std::vector<uint32_t> &base);
-// IC(0) in multicolour order set up on the device (pp_setup_dev.hip): the
-// order, P A P^T, the factor, both FP64 sweeps and the re-factorization
-// state, the host's results without its host copies; false (M unchanged)
-// when the matrix is outside that path (unsorted or unsymmetric pattern,
-// long rows or chains, more than 64 colours, a failing pivot).
-bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M);
+// IC(0) or ILU(0) in multicolour order set up on the device
+// (pp_setup_dev.hip): the order, P A P^T, the factor, both FP64 sweeps and
+// the re-factorization state, the host's results without its host copies;
+// false (M unchanged) when the matrix is outside that path (unsorted or
+// unsymmetric pattern, long rows, more than 64 colours, a failing pivot).
+bool setup_mc_device(pp_ctx *ctx, pp_precond *M);
 // The CSR object of device arrays it takes over (freed with it, also when
 // this throws), with the host copy of the row pointers: row blocks and
 // their 16-bit column form for the CSR-stream SpMV (computed on the

@@ -1209,13 +1209,13 @@ static int choose_schedule(pp_ctx *ctx, pp_precond *M, int force) {
     return PP_TRSV_LEVELS;
 }
 
-// IC(0) in multicolour order on a matrix large enough for the device work
-// to pay (and not PP_SETUP_HOST=1 in the environment: the host setup the
-// device one is tested against)
+// IC(0) or ILU(0) in multicolour order on a matrix large enough for the
+// device work to pay (and not PP_SETUP_HOST=1 in the environment: the host
+// setup the device one is tested against)
 static bool device_setup_applies(const pp_precond *M) {
     const char *host = std::getenv("PP_SETUP_HOST");
     const int t = M->opts.trsv;
-    return M->opts.kind == PP_PC_IC0 && M->opts.ordering == PP_ORDER_MULTICOLOR &&
+    return (M->opts.kind == PP_PC_IC0 || M->opts.kind == PP_PC_ILU0) && M->opts.ordering == PP_ORDER_MULTICOLOR &&
            (t == PP_TRSV_AUTO || t == PP_TRSV_LEVELS || t == PP_TRSV_SYNCFREE) && M->A->nrow >= (1u << 15) &&
            !(host && host[0] == '1');
 }
@@ -1232,17 +1232,18 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
             M->mnnz = 0;
         } else if (M->opts.kind == PP_PC_AMSED || M->opts.kind == PP_PC_GMSLR) {
             ml_setup(ctx, M);
-        } else if (device_setup_applies(M) && setup_ic0_mc_device(ctx, M)) {
+        } else if (device_setup_applies(M) && setup_mc_device(ctx, M)) {
             // on the device (pp_setup_dev.hip): the host structures wait for a caller that needs them
+            const bool ic = M->opts.kind == PP_PC_IC0;
             M->host_pending = true;
             M->dev_setup = true;
             M->hlow = HostSweep();
+            M->hlow.mode = ic ? HostSweep::DIVIDE : HostSweep::UNIT;
             M->hup = HostSweep();
             M->hup.lower = false;
-            M->lt_rp.clear();
-            M->lt_ci.clear();
-            M->lt_v.clear();
-            M->d.clear();
+            M->hup.mode = ic ? HostSweep::DIVIDE : HostSweep::MULTIPLY;
+            for (auto *v : {&M->lt_rp, &M->lt_ci, &M->l_rp, &M->l_ci, &M->u_rp, &M->u_ci}) v->clear();
+            for (auto *v : {&M->lt_v, &M->l_v, &M->u_v, &M->d}) v->clear();
             M->fail_row = 0;
             finish_device_sweeps(M);
             auto tp = std::chrono::steady_clock::now();

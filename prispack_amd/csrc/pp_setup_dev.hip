@@ -1,4 +1,4 @@
-// pp_setup_dev.hip -- IC(0) in multicolour order set up on the device.
+// pp_setup_dev.hip -- IC(0) and ILU(0) in multicolour order set up on the device.
 //
 // The host setup (pp_precond.cpp: setup_triangular, upload_sweep) colours
 // the matrix, forms P A P^T, factors it and lays the two sweeps out in
@@ -12,10 +12,11 @@
 //   order       rows stably sorted by colour (hipCUB radix sort);
 //   P A P^T     rows gathered, columns renumbered and sorted;
 //   factor      the strict lower pattern, its dependency levels (rounds
-//               again), the numeric IC(0) level by level (k_ic0_level, the
-//               host's operation order);
-//   sweeps      forward = L's rows, backward = L^T's rows (the matrix's
-//               upper rows, values found in L by binary search), each in
+//               again), the numeric IC(0) or ILU(0) level by level
+//               (k_ic0_level, k_ilu0_level: the host's operation order);
+//   sweeps      forward = L's rows, backward = L^T's rows (IC(0): the
+//               matrix's upper rows, values found in L by binary search) or
+//               U's rows (ILU(0): the upper entries themselves), each in
 //               level order with the row blocks of the wide levels and the
 //               sliced-column levels; the values reach them through maps
 //               (factor entry per slot) that the re-factorization reuses.
@@ -321,6 +322,28 @@ __global__ void k_sd_gather_upper(uint32_t n, const uint32_t *__restrict__ perm,
     }
 }
 
+// ILU(0)'s backward sweep in level order: position k holds U's row j =
+// perm[k], the columns i > j of the matrix's row j with their own entries
+// (the factor is computed in place on the matrix's pattern)
+__global__ void k_sd_gather_upper_lu(uint32_t n, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ prp,
+                                     const uint32_t *__restrict__ pci, const uint32_t *__restrict__ rp, uint32_t *ci,
+                                     uint32_t *map) {
+    for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) {
+        const uint32_t j = perm[k];
+        uint32_t q = rp[k];
+        for (uint32_t e = prp[j]; e < prp[j + 1]; ++e)
+            if (pci[e] > j) {
+                ci[q] = pci[e];
+                map[q++] = e;
+            }
+    }
+}
+
+// map[q] = via[map[q]]
+__global__ void k_sd_compose(uint32_t cnt, const uint32_t *__restrict__ via, uint32_t *map) {
+    for (uint32_t q = blockIdx.x * kB + threadIdx.x; q < cnt; q += gridDim.x * kB) map[q] = via[map[q]];
+}
+
 // the pivots' positions in the factor array: nnz + row
 __global__ void k_sd_diag_map(uint32_t n, uint32_t nnz, uint32_t *map) {
     for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) map[i] = nnz + i;
@@ -559,7 +582,8 @@ void level_order(pp_ctx *ctx, Arena &ar, uint32_t n, int *lev, uint32_t nlev, co
 // row pointers cross to the host only for a wide level whose rows do not
 // all fit the uniform row blocks).
 void finish_sweep(pp_ctx *ctx, Arena &ar, DevSweep<double> &d, uint32_t n, const std::vector<uint32_t> &lp,
-                  uint32_t *perm, uint32_t *rp, uint32_t *ci, const uint32_t *map, const double *fval, int *fail) {
+                  uint32_t *perm, uint32_t *rp, uint32_t *ci, const uint32_t *map, const double *fval, int *fail,
+                  bool with_diag) {
     d.free_all();
     d.n = n;
     d.nlev = (uint32_t)lp.size() - 1;
@@ -647,8 +671,9 @@ void finish_sweep(pp_ctx *ctx, Arena &ar, DevSweep<double> &d, uint32_t n, const
     }
     for (uint32_t l = 0; l < nlev; ++l)
         if (d.level_off[l] == DevSweep<double>::kNoOffset) d.perm_rows += lp[l + 1] - lp[l];
-    // the pivots (original numbering), gathered by the caller
-    d.diag = dalloc<double>(n);
+    // the pivots (original numbering), gathered by the caller; a unit
+    // diagonal keeps none
+    if (with_diag) d.diag = dalloc<double>(n);
     // sliced-column levels (factors large enough for the setup to time them)
     d.ell.assign(nlev, DevSweep<double>::EllLevel());
     for (uint32_t l = 0; l < nlev && n >= kTuneSweepRows; ++l) {
@@ -716,7 +741,8 @@ uint64_t two_base_device(pp_ctx *ctx, uint32_t nblk, const uint32_t *blocks, con
     return n16;
 }
 
-bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M) {
+bool setup_mc_device(pp_ctx *ctx, pp_precond *M) {
+    const bool ic = M->opts.kind == PP_PC_IC0;
     pp_csr *A = M->A;
     const uint32_t n = A->nrow;
     if (n != A->ncol || n == 0 || A->h_rp.size() != (size_t)n + 1) return false;
@@ -796,23 +822,33 @@ bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M) {
     std::vector<uint32_t> lp_lo, lp_up;
     level_order(ctx, ar, n, lev_lo, nlev_lo, iota, perm_lo, lp_lo);
     level_order(ctx, ar, n, lev_up, nlev_up, iota, perm_up, lp_up);
-    // numeric IC(0), level by level (the forward sweep's levels are L's)
-    double *fval = ar.get<double>((size_t)nl + n);
+    // the numeric factor, level by level (the forward sweep's levels are
+    // L's): IC(0) into L's strict lower part then the pivots l_ii; ILU(0)
+    // in place on the matrix's pattern then the inverse pivots 1/u_ii
+    const uint32_t nf = ic ? nl : nnz;  // factor entries before the pivots
+    double *fval = ar.get<double>((size_t)nf + n);
     uint32_t *rows = ar.get<uint32_t>(n);
     PP_HIP(hipMemcpyAsync(rows, perm_lo, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
     const int big = INT32_MAX;
     int *ffail = ar.get<int>(1);
     PP_HIP(hipMemcpyAsync(ffail, &big, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-    for (uint32_t l = 0; l < nlev_lo; ++l)
-        launch_ic0_level(ctx, rows + lp_lo[l], lp_lo[l + 1] - lp_lo[l], lrp, lci, lsrc, dsrc, pv, fval, fval + nl,
-                         ffail);
-    if (read_int(ctx, ffail) != big) return false;  // a pivot that is not positive: the host path reports it
+    for (uint32_t l = 0; l < nlev_lo; ++l) {
+        if (ic)
+            launch_ic0_level(ctx, rows + lp_lo[l], lp_lo[l + 1] - lp_lo[l], lrp, lci, lsrc, dsrc, pv, fval,
+                             fval + nf, ffail);
+        else
+            launch_ilu0_level(ctx, rows + lp_lo[l], lp_lo[l + 1] - lp_lo[l], prp, pci, dsrc, pv, fval, fval + nf,
+                              ffail);
+    }
+    if (read_int(ctx, ffail) != big) return false;  // a failing pivot: the host path reports it
     const float ms_factor = ms();
     // the sweeps
     uint32_t *srp_lo = ar.get<uint32_t>(n + 1), *sci_lo = ar.get<uint32_t>(nl), *map_lo = ar.get<uint32_t>(nl);
     k_sd_row_len<<<grid_of(n + 1), kB, 0, ctx->stream>>>(n, perm_lo, lrp, len);
     exclusive_scan(ctx, len, srp_lo, n + 1);
     k_sd_gather_lower<<<grid_of(n), kB, 0, ctx->stream>>>(n, perm_lo, lrp, lci, srp_lo, sci_lo, map_lo);
+    if (!ic)  // ILU(0): L's entries are the matrix's own
+        k_sd_compose<<<grid_of(nl), kB, 0, ctx->stream>>>(nl, lsrc, map_lo);
     PP_HIP(hipGetLastError());
     uint32_t *ucnt = ar.get<uint32_t>(n + 1), *srp_up = ar.get<uint32_t>(n + 1);
     uint32_t *sci_up = ar.get<uint32_t>(nl), *map_up = ar.get<uint32_t>(nl);
@@ -821,17 +857,23 @@ bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M) {
     PP_HIP(hipGetLastError());
     exclusive_scan(ctx, len, srp_up, n + 1);
     PP_HIP(hipMemsetAsync(fail, 0, sizeof(int), ctx->stream));
-    k_sd_gather_upper<<<grid_of(n), kB, 0, ctx->stream>>>(n, perm_up, prp, pci, lrp, lci, srp_up, sci_up, map_up,
-                                                          fail);
+    if (ic)
+        k_sd_gather_upper<<<grid_of(n), kB, 0, ctx->stream>>>(n, perm_up, prp, pci, lrp, lci, srp_up, sci_up,
+                                                              map_up, fail);
+    else
+        k_sd_gather_upper_lu<<<grid_of(n), kB, 0, ctx->stream>>>(n, perm_up, prp, pci, srp_up, sci_up, map_up);
     PP_HIP(hipGetLastError());
     if (read_int(ctx, fail)) return false;
     uint32_t *dmap = ar.get<uint32_t>(n);
-    k_sd_diag_map<<<grid_of(n), kB, 0, ctx->stream>>>(n, nl, dmap);
+    k_sd_diag_map<<<grid_of(n), kB, 0, ctx->stream>>>(n, nf, dmap);
     PP_HIP(hipGetLastError());
     // everything that can fail has run: hand the arrays to the preconditioner
-    finish_sweep(ctx, ar, M->low64, n, lp_lo, perm_lo, srp_lo, sci_lo, map_lo, fval, fail);
-    finish_sweep(ctx, ar, M->up64, n, lp_up, perm_up, srp_up, sci_up, map_up, fval, fail);
-    for (DevSweep<double> *d : {&M->low64, &M->up64}) launch_gather_map(ctx, n, dmap, fval, PP_F64, d->diag);
+    // IC(0): forward L (divide by l_ii), backward L^T (divide); ILU(0):
+    // forward L (unit), backward U (multiply by 1/u_ii)
+    finish_sweep(ctx, ar, M->low64, n, lp_lo, perm_lo, srp_lo, sci_lo, map_lo, fval, fail, ic);
+    finish_sweep(ctx, ar, M->up64, n, lp_up, perm_up, srp_up, sci_up, map_up, fval, fail, true);
+    for (DevSweep<double> *d : {&M->low64, &M->up64})
+        if (d->diag) launch_gather_map(ctx, n, dmap, fval, PP_F64, d->diag);
     PP_HIP(hipStreamSynchronize(ctx->stream));  // ell_fill_rows runs on the null stream
     for (DevSweep<double> *d : {&M->low64, &M->up64})
         for (uint32_t l = 0; l < d->nlev; ++l)
@@ -855,7 +897,8 @@ bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M) {
             }
         }
     }
-    M->mnnz = (uint32_t)(2 * ((uint64_t)nl + n) + n);  // the host factor's count (L^T with its pivots, twice, + n)
+    // the host factor's count: IC(0) L^T with its pivots twice, + n; ILU(0) L + U + n
+    M->mnnz = ic ? (uint32_t)(2 * ((uint64_t)nl + n) + n) : nnz;
     const float ms_sweeps = ms();
     // P A P^T as the solves' device matrix (its row pointers on the host)
     std::vector<uint32_t> hprp(n + 1);
@@ -882,22 +925,31 @@ bool setup_ic0_mc_device(pp_ctx *ctx, pp_precond *M) {
     // the re-factorization state: the pattern, levels and maps above
     std::unique_ptr<DevRefactor> R(new DevRefactor);
     R->n = n;
-    R->nnz = nl;
-    R->rp = ar.keep(lrp);
-    R->ci = ar.keep(lci);
-    R->src = ar.keep(lsrc);
+    R->nnz = nf;
+    if (ic) {
+        R->rp = ar.keep(lrp);
+        R->ci = ar.keep(lci);
+        R->src = ar.keep(lsrc);
+    } else {  // ILU(0) factors on the matrix's own pattern (its own copy: P A P^T keeps its arrays)
+        R->rp = dalloc<uint32_t>((size_t)n + 1);
+        R->ci = dalloc<uint32_t>(std::max<uint32_t>(nnz, 1));
+        PP_HIP(hipMemcpyAsync(R->rp, M->Aperm->rp, ((size_t)n + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        PP_HIP(hipMemcpyAsync(R->ci, M->Aperm->ci, (size_t)nnz * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    }
     R->dsrc = ar.keep(dsrc);
     R->val = ar.keep(fval);
     R->rows = ar.keep(rows);
     R->lptr = lp_lo;
     R->lo_map = ar.keep(map_lo);
     R->lo_nnz = nl;
-    R->lo_dmap = ar.keep(dmap);
-    R->lo_diag = true;
+    R->lo_diag = ic;
+    if (ic) R->lo_dmap = ar.keep(dmap);
     R->up_map = ar.keep(map_up);
     R->up_nnz = nl;
     R->up_dmap = dalloc<uint32_t>(n);
-    PP_HIP(hipMemcpyAsync(R->up_dmap, R->lo_dmap, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+    PP_HIP(hipMemcpyAsync(R->up_dmap, dmap, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
     R->up_diag = true;
     R->aperm_map = ar.keep(asrc);
     R->aperm_nnz = nnz;

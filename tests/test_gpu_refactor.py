@@ -207,22 +207,31 @@ def test_refactor_refreshes_the_reduced_form(pp, torch):
     np.testing.assert_array_equal(got[2], want[2])
 
 
-def test_device_setup_matches_host_setup(pp, torch, monkeypatch):
-    # IC(0) in multicolour order set up on the device (colouring, P A P^T,
-    # factor, sweeps) against the host setup (PP_SETUP_HOST=1) on variable
-    # values D A D + I/4 of a 40 x 41 x 42 grid: the same applies in both
-    # precisions, the same FP64 and mixed solves, the same factor export,
-    # and a re-factorization on new values that still agrees
-    A = ppo.laplacian(40, 41, 42)
-    n = len(A[0]) - 1
-    rows = np.repeat(np.arange(n), np.diff(A[0]))
-    cols = np.asarray(A[1], np.int64)
-    rng = np.random.default_rng(8)
-    dsc = 1.0 + 0.4 * rng.uniform(-1, 1, n)
-    v = np.asarray(A[2], np.float64) * dsc[rows] * dsc[cols]
-    v[A[1] == rows] += 0.25
-    v2 = v * (1.0 + 0.1 * rng.uniform(-1, 1, len(v)))
-    v2 = 0.5 * (v2 + v2[_transpose_positions(A)])  # symmetric again
+@pytest.mark.parametrize("kind", ["ic0", "ilu0"])
+def test_device_setup_matches_host_setup(pp, torch, monkeypatch, kind):
+    # IC(0) / ILU(0) in multicolour order set up on the device (colouring,
+    # P A P^T, factor, sweeps) against the host setup (PP_SETUP_HOST=1) on a
+    # 40 x 41 x 42 grid -- IC(0): variable values D A D + I/4 of the
+    # Laplacian, CG; ILU(0): the upwind operator, FGMRES(30) -- the same
+    # applies in both precisions, the same FP64 and mixed solves, the same
+    # factor export, and a re-factorization on new values that still agrees
+    if kind == "ic0":
+        A = ppo.laplacian(40, 41, 42)
+        n = len(A[0]) - 1
+        rows = np.repeat(np.arange(n), np.diff(A[0]))
+        cols = np.asarray(A[1], np.int64)
+        rng = np.random.default_rng(8)
+        dsc = 1.0 + 0.4 * rng.uniform(-1, 1, n)
+        v = np.asarray(A[2], np.float64) * dsc[rows] * dsc[cols]
+        v[A[1] == rows] += 0.25
+        v2 = v * (1.0 + 0.1 * rng.uniform(-1, 1, len(v)))
+        v2 = 0.5 * (v2 + v2[_transpose_positions(A)])  # symmetric again
+    else:
+        A = ppo.upwind(40, 41, 42, 20.0, 10.0, 5.0)
+        n = len(A[0]) - 1
+        rng = np.random.default_rng(9)
+        v = np.asarray(A[2], np.float64)
+        v2 = v * (1.0 + 0.1 * rng.uniform(-1, 1, len(v)))
     b = ppo.spmv((A[0], A[1], v), ppo.normal(n, 4))
     x0 = torch.tensor(ppo.normal(n, 5), device="cuda")
     out = {}
@@ -231,9 +240,9 @@ def test_device_setup_matches_host_setup(pp, torch, monkeypatch):
             monkeypatch.setenv("PP_SETUP_HOST", "1")
         else:
             monkeypatch.delenv("PP_SETUP_HOST", raising=False)
-        print(f"[setup] {where} n={n}", flush=True)
+        print(f"[setup] {kind} {where} n={n}", flush=True)
         M = pp.CSRMatrix(A[0], A[1], v)
-        P = pp.IC0(M, ordering=1, make_f32=True)
+        P = (pp.IC0 if kind == "ic0" else pp.ILU0)(M, ordering=1, make_f32=True)
         P.setup()
         info = P.info()
         assert info.setup_on_device == (where == "device")
@@ -242,16 +251,21 @@ def test_device_setup_matches_host_setup(pp, torch, monkeypatch):
         rec["apply32"] = P.MInvSolve(x0.clone().float()).cpu().numpy()
         res = []
         for prec in ("fp64", "mixed"):
-            cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-9, maxIter=3000, precision=prec)
+            if kind == "ic0":
+                s = pp.CG(n, P, pp.FlexibleCG, tol=1e-9, maxIter=3000, precision=prec)
+            else:
+                s = pp.GMRES(n, P, 1e-9, pp.FlexibleGMRES, maxIter=3000, precision=prec)
+                s.setRestart(True, 30)
             x = torch.zeros(n, dtype=torch.float64, device="cuda")
-            cg.solve(M, torch.tensor(b, device="cuda"), x)
-            assert cg.getConvergence(), (where, prec)
-            res.append((cg.getNumIter(), cg.report.inner_iters, x.cpu().numpy()))
+            s.solve(M, torch.tensor(b, device="cuda"), x)
+            assert s.getConvergence(), (where, prec)
+            res.append((s.getNumIter(), s.report.inner_iters, x.cpu().numpy()))
         rec["solves"] = res
         M.set_values(v2)
         P.refactor()
         rec["refactor"] = P.MInvSolve(x0.clone()).cpu().numpy()
-        rec["factor"] = P.factor(0)  # the host copy (built on demand after a device setup)
+        # the host copy (built on demand after a device setup)
+        rec["factor"] = [P.factor(0)] if kind == "ic0" else [P.factor(1), P.factor(2)]
         out[where] = rec
     h, d = out["host"], out["device"]
     assert h["levels"] == d["levels"]
@@ -260,8 +274,9 @@ def test_device_setup_matches_host_setup(pp, torch, monkeypatch):
     for (ih, jh, xh), (idv, jd, xd) in zip(h["solves"], d["solves"]):
         assert (ih, jh) == (idv, jd)
         np.testing.assert_array_equal(xh, xd)
-    for a, c in zip(h["factor"], d["factor"]):
-        np.testing.assert_array_equal(a, c)
+    for fh, fd in zip(h["factor"], d["factor"]):
+        for a, c in zip(fh, fd):
+            np.testing.assert_array_equal(a, c)
 
 
 def _transpose_positions(A):

@@ -47,7 +47,9 @@ def parse():
                          "the known-size reference for scaling FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only build the system and time the SpMV and preconditioner-apply launches (no solves)")
-    ap.add_argument("--launch", type=int, default=0, help="PCG body: 0 hipGraph replay, 1 stream launches")
+    ap.add_argument("--launch", type=int, default=None,
+                    help="iteration bodies: the library's default (stream launches) unless 0 hipGraph replay, "
+                         "1 stream launches")
     ap.add_argument("--check-every", type=int, default=4, help="iterations per graph body (per host convergence poll)")
     ap.add_argument("--spmv-variant", type=int, default=-1,
                     help="pin the SpMV load variant of the timed launches (0-7 CSR-stream, 8 sliced-column; "
@@ -338,7 +340,8 @@ def main():
         cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-8, maxIter=2000, precision=precision)
         cg.setCGVariant(variant)
         cg.opts.record_history = 0
-        cg.opts.launch = args.launch
+        if args.launch is not None:
+            cg.opts.launch = args.launch
         cg.opts.check_every = args.check_every
         x = torch.zeros(n, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()

@@ -76,6 +76,22 @@ int bits_for(uint32_t m) {  // bits to hold 0..m
 
 // ---- kernels (one thread per row unless stated)
 
+// Reductions over a workgroup of kB threads: the wavefronts combine by
+// shuffles, then the workgroup through LDS, so one atomic per workgroup
+// reaches the global word (per-row or per-wavefront atomics on one word
+// queue behind each other).
+template <class T, class Op>
+__device__ T block_reduce(T v, Op op, T *lds) {
+    for (int o = 32; o > 0; o >>= 1) v = op(v, (T)__shfl_xor(v, o));
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 1; k < kB / 64; ++k) v = op(v, lds[k]);
+    return v;  // complete in thread 0
+}
+
 // fail bits: 1 columns not strictly ascending, 2 pattern not symmetric,
 // 4 no diagonal, 8 a row longer than kMaxRow
 __global__ void k_sd_check(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci, int *fail) {
@@ -183,18 +199,16 @@ __global__ void k_sd_level(uint32_t n, const uint32_t *__restrict__ rp, const ui
 
 // flag: a negative entry remains; mx: the largest entry
 __global__ void k_sd_scan_state(uint32_t n, const int *__restrict__ a, int *flag, int *mx) {
-    int m = -1;
-    bool neg = false;
+    __shared__ int lds[kB / 64];
+    int m = -1, neg = 0;
     for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB) {
         const int v = a[i];
         neg |= v < 0;
         m = max(m, v);
     }
-    for (int o = 32; o > 0; o >>= 1) {  // one atomic per wavefront
-        m = max(m, __shfl_xor(m, o));
-        neg |= __shfl_xor((int)neg, o) != 0;
-    }
-    if ((threadIdx.x & 63) == 0) {
+    m = block_reduce(m, [](int x, int y) { return max(x, y); }, lds);
+    neg = block_reduce(neg, [](int x, int y) { return x | y; }, lds);
+    if (threadIdx.x == 0) {
         if (neg) atomicOr(flag, 1);
         atomicMax(mx, m);
     }
@@ -229,8 +243,36 @@ __global__ void k_sd_permute(uint32_t n, const uint32_t *__restrict__ order, con
                              const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
                              const double *__restrict__ v, const uint32_t *__restrict__ prp, uint32_t *pci,
                              double *pv, uint32_t *src) {
+    constexpr uint32_t kReg = 8;  // rows this short are sorted in registers
     for (uint32_t k = blockIdx.x * kB + threadIdx.x; k < n; k += gridDim.x * kB) {
         const uint32_t i = order[k], a0 = rp[i], len = rp[i + 1] - a0, p0 = prp[k];
+        if (len <= kReg) {
+            uint32_t c[kReg], e[kReg];
+#pragma unroll
+            for (uint32_t t = 0; t < kReg; ++t) {
+                c[t] = t < len ? inv[ci[a0 + t]] : UINT32_MAX;
+                e[t] = a0 + t;
+            }
+#pragma unroll
+            for (uint32_t t = 1; t < kReg; ++t)  // insertion sort over a fixed network of compares
+#pragma unroll
+                for (uint32_t q = t; q > 0; --q)
+                    if (c[q - 1] > c[q]) {
+                        const uint32_t tc = c[q - 1], te = e[q - 1];
+                        c[q - 1] = c[q];
+                        e[q - 1] = e[q];
+                        c[q] = tc;
+                        e[q] = te;
+                    }
+#pragma unroll
+            for (uint32_t t = 0; t < kReg; ++t)
+                if (t < len) {
+                    pci[p0 + t] = c[t];
+                    pv[p0 + t] = v[e[t]];
+                    src[p0 + t] = e[t];
+                }
+            continue;
+        }
         for (uint32_t t = 0; t < len; ++t) {  // insertion into the sorted prefix
             const uint32_t c = inv[ci[a0 + t]];
             uint32_t q = t;
@@ -360,11 +402,12 @@ __global__ void k_sd_bounds(uint32_t n, uint32_t nlev, const uint32_t *__restric
 
 // the first row whose first entry lies left of the diagonal (the colour split)
 __global__ void k_sd_split(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci, uint32_t *s) {
+    __shared__ uint32_t lds[kB / 64];
     uint32_t m = UINT32_MAX;
     for (uint32_t i = blockIdx.x * kB + threadIdx.x; i < n; i += gridDim.x * kB)
         if (rp[i] < rp[i + 1] && ci[rp[i]] < i) m = min(m, i);
-    for (int o = 32; o > 0; o >>= 1) m = min(m, (uint32_t)__shfl_xor((int)m, o));
-    if ((threadIdx.x & 63) == 0 && m != UINT32_MAX) atomicMin(s, m);
+    m = block_reduce(m, [](uint32_t x, uint32_t y) { return min(x, y); }, lds);
+    if (threadIdx.x == 0 && m != UINT32_MAX) atomicMin(s, m);
 }
 
 // offsets of one sliced-column sweep level (the host's upload_sweep loop):
@@ -466,12 +509,11 @@ __global__ void k_sd_level_stats(uint32_t k0, uint32_t k1, const uint32_t *__res
         mn = min(mn, len);
         if (k > k0 && perm[k] != perm[k - 1] + 1) gap = 1;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        mx = max(mx, (uint32_t)__shfl_xor((int)mx, o));
-        mn = min(mn, (uint32_t)__shfl_xor((int)mn, o));
-        gap |= (uint32_t)__shfl_xor((int)gap, o);
-    }
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ uint32_t lds[kB / 64];
+    mx = block_reduce(mx, [](uint32_t x, uint32_t y) { return max(x, y); }, lds);
+    mn = block_reduce(mn, [](uint32_t x, uint32_t y) { return min(x, y); }, lds);
+    gap = block_reduce(gap, [](uint32_t x, uint32_t y) { return x | y; }, lds);
+    if (threadIdx.x == 0) {
         atomicMax(st, mx);
         atomicMin(st + 1, mn);
         if (gap) atomicOr(st + 2, 1u);

@@ -1948,9 +1948,10 @@ __global__ __launch_bounds__(1024) void k_red_ka(uint32_t n0, const int16_t *__r
     }
     // the thread's first kU rows: matrix entries and every gathered r, w, s
     // loaded before the check (none of them depends on alpha / beta), so
-    // the check's latency overlaps them (one row for the wide FP64 forms,
-    // whose two rows would not fit the registers of 1024-thread groups)
-    constexpr int kU = sizeof(T) == 8 && W >= 6 ? 1 : 2;
+    // the check's latency overlaps them (as many rows as the registers of
+    // 1024-thread groups hold without spilling: one for the wide FP64
+    // forms, three for FP32 up to six entries a row, two otherwise)
+    constexpr int kU = sizeof(T) == 8 && W >= 6 ? 1 : (sizeof(T) == 4 && W <= 6 ? 3 : 2);
     T v[kU][W], gr[kU][W], gw[kU][W], gs[kU][W];
     auto load_row = [&](uint32_t i, T *vv, T *rr, T *ww, T *ss) {
         int16_t o[W];

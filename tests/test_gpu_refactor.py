@@ -287,3 +287,68 @@ def _transpose_positions(A):
     tkey = ci * (len(rp) - 1) + rows
     order = np.argsort(key)
     return order[np.searchsorted(key[order], tkey)]
+
+
+def _stencil27(nx, ny, nz):
+    # the 27-point stencil (26 on the diagonal, -1 to every neighbour in the
+    # 3x3x3 box): SPD, eight colours in row order
+    import scipy.sparse as sp
+    def line(m):
+        return sp.diags([1.0, 1.0, 1.0], [-1, 0, 1], shape=(m, m))
+    B = sp.kron(sp.kron(line(nz), line(ny)), line(nx)).tocsr()
+    A = (27.0 * sp.identity(nx * ny * nz) - B).tocsr()
+    A.sort_indices()
+    return (A.indptr.astype(np.uint32), A.indices.astype(np.uint32), A.data.astype(np.float64))
+
+
+@pytest.mark.parametrize("case", ["27pt", "27pt_syncfree", "unsorted", "unsymmetric"])
+def test_device_setup_paths(pp, torch, monkeypatch, case):
+    # the device setup on an eight-colour factor (the chunked schedules cut
+    # from host structures built on demand) and with the sync-free schedule;
+    # matrices outside it (unsorted rows, an unsymmetric pattern) set up on
+    # the host.  Applies and solves agree with a host setup bit for bit.
+    A = _stencil27(34, 34, 34)
+    n = len(A[0]) - 1
+    trsv = 2 if case == "27pt_syncfree" else 0
+    if case == "unsorted":  # each row's columns reversed: same matrix, unsorted storage
+        rp, ci, v = (np.array(a) for a in A)
+        for i in range(n):
+            ci[rp[i]:rp[i + 1]] = ci[rp[i]:rp[i + 1]][::-1].copy()
+            v[rp[i]:rp[i + 1]] = v[rp[i]:rp[i + 1]][::-1].copy()
+        A = (rp, ci, v)
+    elif case == "unsymmetric":  # explicit zeros at (i, i + 2) in every tenth row: same values, unsymmetric pattern
+        import scipy.sparse as sp
+        C = sp.csr_matrix((A[2], A[1], A[0]), shape=(n, n))
+        rows = np.arange(0, n - 2, 10)
+        Z = sp.csr_matrix((np.full(len(rows), 1.0), (rows, rows + 2)), shape=(n, n))
+        C = (C + Z).tocsr()
+        C.sort_indices()
+        C.data[np.isin(np.repeat(np.arange(n), np.diff(C.indptr)) * n + C.indices, rows * n + rows + 2)] = 0.0
+        A = (C.indptr.astype(np.uint32), C.indices.astype(np.uint32), C.data.astype(np.float64))
+    b = ppo.spmv(A, ppo.normal(n, 3))
+    x0 = torch.tensor(ppo.normal(n, 4), device="cuda")
+    out = {}
+    for where in ("host", "device"):
+        if where == "host":
+            monkeypatch.setenv("PP_SETUP_HOST", "1")
+        else:
+            monkeypatch.delenv("PP_SETUP_HOST", raising=False)
+        print(f"[setup] {case} {where} n={n}", flush=True)
+        M = pp.CSRMatrix(*A)
+        P = pp.IC0(M, ordering=1, make_f32=True, trsv=trsv)
+        P.setup()
+        info = P.info()
+        expect_device = where == "device" and case in ("27pt", "27pt_syncfree")
+        assert info.setup_on_device == expect_device, (case, where)
+        rec = [(info.levels_lower, info.levels_upper, info.mnnz),
+               P.MInvSolve(x0.clone()).cpu().numpy(), P.MInvSolve(x0.clone().float()).cpu().numpy()]
+        cg = pp.CG(n, P, pp.FlexibleCG, tol=1e-9, maxIter=2000)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        cg.solve(M, torch.tensor(b, device="cuda"), x)
+        assert cg.getConvergence(), (case, where)
+        rec += [cg.getNumIter(), x.cpu().numpy()]
+        out[where] = rec
+    h, d = out["host"], out["device"]
+    assert h[0] == d[0] and h[3] == d[3], (h[0], d[0], h[3], d[3])
+    for k in (1, 2, 4):
+        np.testing.assert_array_equal(h[k], d[k])

@@ -287,11 +287,14 @@ __global__ __launch_bounds__(256) void k_spmv(const BlockDesc *__restrict__ desc
 // The offsets of a row from its pattern (EllOp::pid / ptab): one 16-byte
 // table entry, eight 16-bit slots.
 template <int MAXW>
-__device__ inline void ell_pattern(const int16_t *__restrict__ ptab, uint32_t p, int16_t *o) {
-    const int4 t = *reinterpret_cast<const int4 *>(ptab + 8 * (size_t)p);
+__device__ inline void ell_unpack(const int4 t, int16_t *o) {
     const uint32_t w[4] = {(uint32_t)t.x, (uint32_t)t.y, (uint32_t)t.z, (uint32_t)t.w};
 #pragma unroll
     for (int j = 0; j < MAXW; ++j) o[j] = (int16_t)((w[j / 2] >> (16 * (j & 1))) & 0xffffu);
+}
+template <int MAXW>
+__device__ inline void ell_pattern(const int16_t *__restrict__ ptab, uint32_t p, int16_t *o) {
+    ell_unpack<MAXW>(*reinterpret_cast<const int4 *>(ptab + 8 * (size_t)p), o);
 }
 
 template <int W>
@@ -1766,17 +1769,19 @@ template void red_update<float>(pp_ctx *, uint32_t, const float *, float *, floa
 // writes the iteration state.  The test mirrors pcg_check and
 // k_cgsr_finish: a pending last check stops with m = maxit; a non-finite
 // norm or norm < tol stops with m = the check index; a zero or non-finite
-// denominator is a breakdown.  Returns false when the iteration stops.
-__device__ inline bool red_check(const double *__restrict__ partials, unsigned np, double *scal, int *ctl, int *ring,
-                                 int parity, double *sh, double *bc, int *go, double &alpha_out,
-                                 double &beta_out) {
+// denominator is a breakdown.  t: this thread's sums of the three partial
+// arrays (sum_partials); prior_stop: ctl[C_STOP] as read at the kernel's
+// start -- an earlier check stopped, so this one only republishes the
+// control words.  Returns false when the iteration stops.
+__device__ inline bool red_check(const double t[3], int prior_stop, double *scal, int *ctl, int *ring, int parity,
+                                 double *sh, double *bc, int *go, double &alpha_out, double &beta_out) {
 #pragma clang fp contract(off)
-    double t[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) t[q] = sum_partials(partials + (size_t)q * kMaxPartials, np);
     double v[3] = {t[0], t[1], t[2]};
     block_sum3(v[0], v[1], v[2], sh);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && prior_stop) {
+        *go = 0;
+        if (ring && blockIdx.x == 0) publish_ctl(ctl, ring);
+    } else if (threadIdx.x == 0) {
         const double *st = scal + (parity ? S_RED1 : S_RED0);
         double *nx = scal + (parity ? S_RED0 : S_RED1);
         const double delta = v[0], gamma = v[1], rho = v[2];
@@ -1884,8 +1889,10 @@ __global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ pa
             vx[u] = x[i];
         }
     }
-    double a64, b64;
-    if (!red_check(partials, np, scal, ctl, ring, parity, sh, bc, &go, a64, b64)) return;
+    double t[3], a64, b64;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) t[q] = sum_partials(partials + (size_t)q * kMaxPartials, np);
+    if (!red_check(t, 0, scal, ctl, ring, parity, sh, bc, &go, a64, b64)) return;
     const T alpha = (T)a64, beta = (T)b64;
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
@@ -1918,9 +1925,21 @@ __global__ __launch_bounds__(1024) void k_red_step(const double *__restrict__ pa
 // r_new[j] = r[j] - alpha (w[j] + beta s[j]), the operations of k_red_step
 // in its order, so the products see the bits the separate update stores.
 // Nothing of the second colour is written here (other workgroups still
-// gather the old r, w, s).  The block's first rows' matrix entries are
-// loaded before the check, so its latency overlaps them.
-template <class T, int W, bool PAT>
+// gather the old r, w, s).  Every load the check and the thread's first
+// rows need is issued before the first wait -- the stop flag, the partials,
+// then the rows' matrix entries and gathers -- so the check's latency and
+// the launch's start overlap the row traffic.  Addresses are 32-bit byte
+// offsets from uniform bases (red_fused_ok: every vector under 4 GiB), one
+// register per gathered neighbour for r, w and s together.
+// k_red_ka's rows loaded ahead of the check per thread: as many as the
+// registers of 1024-thread groups hold without spilling -- one for the wide
+// FP64 forms, three for FP32 up to six entries a row, two otherwise (four
+// FP32 rows of six spill, and measured no faster than three)
+template <class T, int W>
+constexpr int red_ka_rows() {
+    return sizeof(T) == 8 && W >= 6 ? 1 : (sizeof(T) == 4 && W <= 6 ? 3 : 2);
+}
+template <class T, int W, bool PAT, int kU>
 __global__ __launch_bounds__(1024) void k_red_ka(uint32_t n0, const int16_t *__restrict__ off,
                                                  const uint8_t *__restrict__ pid, const int16_t *__restrict__ ptab,
                                                  const T *__restrict__ val, int64_t cbase, const T *z, T *y,
@@ -1931,10 +1950,13 @@ __global__ __launch_bounds__(1024) void k_red_ka(uint32_t n0, const int16_t *__r
     __shared__ double sh[3 * (kFinishThreads / 64)];
     __shared__ double bc[2];
     __shared__ int go;
-    if (ctl[C_STOP]) {
-        if (ring && blockIdx.x == 0 && threadIdx.x == 0) publish_ctl(ctl, ring);
-        return;
-    }
+    // read by the one thread that tests it, as a vector load issued here (a
+    // scalar or wave-uniform read would be waited for at once)
+    int prior_stop = 0;
+    if (threadIdx.x == 0) prior_stop = __hip_atomic_load(ctl + C_STOP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double t[3];  // at most one partial per thread (np <= kFinishThreads, red_ka)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) t[q] = threadIdx.x < np ? partials[(size_t)q * kMaxPartials + threadIdx.x] : 0.0;
     // XCD-aware rows: the workgroups of XCD q (workgroup b runs on XCD b % 8)
     // take the contiguous eighth q of the rows, so the gathered neighbours
     // stay in that XCD's L2
@@ -1946,35 +1968,72 @@ __global__ __launch_bounds__(1024) void k_red_ka(uint32_t n0, const int16_t *__r
         stride = gridDim.x / kXcds * blockDim.x;
         i0 = lo + blockIdx.x / kXcds * blockDim.x + threadIdx.x;
     }
-    // the thread's first kU rows: matrix entries and every gathered r, w, s
-    // loaded before the check (none of them depends on alpha / beta), so
-    // the check's latency overlaps them (as many rows as the registers of
-    // 1024-thread groups hold without spilling: one for the wide FP64
-    // forms, three for FP32 up to six entries a row, two otherwise)
-    constexpr int kU = sizeof(T) == 8 && W >= 6 ? 1 : (sizeof(T) == 4 && W <= 6 ? 3 : 2);
+    // the thread's first kU rows (red_ka_rows): matrix entries and every
+    // gathered r, w, s loaded before the check (none of them depends on
+    // alpha / beta).  The loads go out in three waves, for
+    // all kU rows at once: pattern ids (or offsets), then the patterns and
+    // the matrix entries, then the gathers -- so a thread waits on two
+    // dependent latencies in all, not on two for every row (memory loads
+    // complete in issue order).  Rows past the end load row n0 - 1 and
+    // store nothing.
+    constexpr uint32_t kB = sizeof(T);
+    const char *zb = (const char *)z;  // z, w and s at the same byte offset of a second-colour index
+    const char *wb = (const char *)w - (size_t)rbase * kB;
+    const char *sb = (const char *)s - (size_t)rbase * kB;
+    const uint32_t cb = (uint32_t)cbase;
     T v[kU][W], gr[kU][W], gw[kU][W], gs[kU][W];
     auto load_row = [&](uint32_t i, T *vv, T *rr, T *ww, T *ss) {
         int16_t o[W];
         if constexpr (PAT) ell_pattern<W>(ptab, pid[i], o);
+        const uint32_t ib = i * kB;
 #pragma unroll
         for (int j = 0; j < W; ++j) {
-            const size_t e = (size_t)j * n0 + i;
-            if constexpr (!PAT) o[j] = off[e];
-            vv[j] = val[e];
+            if constexpr (!PAT) o[j] = off[(size_t)j * n0 + i];
+            vv[j] = *(const T *)((const char *)val + (size_t)j * n0 * kB + ib);
         }
 #pragma unroll
         for (int j = 0; j < W; ++j) {
-            const int64_t g = (int64_t)i + cbase + o[j];  // in z; the second colour starts at rbase
-            rr[j] = z[g];
-            ww[j] = w[g - rbase];
-            ss[j] = s[g - rbase];
+            const uint32_t gb = (i + cb + (uint32_t)(int32_t)o[j]) * kB;  // in z; the second colour starts at rbase
+            rr[j] = *(const T *)(zb + gb);
+            ww[j] = *(const T *)(wb + gb);
+            ss[j] = *(const T *)(sb + gb);
         }
     };
+    uint32_t ri[kU];
+    int16_t o[kU][W];
+    uint32_t pv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        ri[u] = min(i0 + u * stride, n0 - 1);
+        if constexpr (PAT) pv[u] = pid[ri[u]];
+        else
+#pragma unroll
+            for (int j = 0; j < W; ++j) o[u][j] = off[(size_t)j * n0 + ri[u]];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    int4 pt[kU];
+    if constexpr (PAT)
+#pragma unroll
+        for (int u = 0; u < kU; ++u) pt[u] = *reinterpret_cast<const int4 *>(ptab + 8 * (size_t)pv[u]);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < kU; ++u)
-        if (i0 + u * stride < hi) load_row(i0 + u * stride, v[u], gr[u], gw[u], gs[u]);
+#pragma unroll
+        for (int j = 0; j < W; ++j) v[u][j] = *(const T *)((const char *)val + (size_t)j * n0 * kB + ri[u] * kB);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        if constexpr (PAT) ell_unpack<W>(pt[u], o[u]);
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            const uint32_t gb = (ri[u] + cb + (uint32_t)(int32_t)o[u][j]) * kB;
+            gr[u][j] = *(const T *)(zb + gb);
+            gw[u][j] = *(const T *)(wb + gb);
+            gs[u][j] = *(const T *)(sb + gb);
+        }
+    }
     double a64, b64;
-    if (!red_check(partials, np, scal, ctl, ring, parity, sh, bc, &go, a64, b64)) return;
+    if (!red_check(t, prior_stop, scal, ctl, ring, parity, sh, bc, &go, a64, b64)) return;
     const T alpha = (T)a64, beta = (T)b64;
     auto row = [&](uint32_t i, const T *vv, const T *rr, const T *ww, const T *ss) {
         T sum = 0;
@@ -2070,7 +2129,9 @@ template void red_step<float>(pp_ctx *, unsigned, int, bool, uint32_t, const flo
 
 bool red_fused_ok(const EllOp &ka, const EllOp &kb, int dtype) {
     const bool vals = dtype == PP_F64 ? (ka.v64 && kb.v64 && kb.dg64) : (ka.v32 && kb.v32 && kb.dg32);
-    return ka.ok && kb.ok && vals && ka.width >= 4 && ka.width <= 7 && kb.width >= 4 && kb.width <= 7 &&
+    // k_red_ka addresses the vectors (n0 + n1 entries) by 32-bit byte offsets
+    const bool small = ((uint64_t)ka.n + kb.n) * (dtype == PP_F64 ? 8 : 4) < (1ull << 32);
+    return ka.ok && kb.ok && vals && small && ka.width >= 4 && ka.width <= 7 && kb.width >= 4 && kb.width <= 7 &&
            !ka.dg64 && !ka.dg32 && ka.n > 0 && kb.n > 0;
 }
 
@@ -2079,6 +2140,7 @@ void red_ka(pp_ctx *ctx, const EllOp &E, const T *z, T *y, uint32_t rbase, const
             int parity, bool publish) {
     // one 1024-thread workgroup per CU (each sums the partials once; all
     // resident at once, so the check runs in one round)
+    PP_CHECK(np <= (unsigned)kFinishThreads, PP_ERR_INVALID, "reduced CG: more partials than k_red_ka sums");
     const unsigned want = (E.n + kFinishThreads - 1) / kFinishThreads;
     unsigned g = std::max(1u, std::min<unsigned>((unsigned)ctx->cu_count, want));
     if (g >= kXcds) g -= g % kXcds;
@@ -2086,8 +2148,8 @@ void red_ka(pp_ctx *ctx, const EllOp &E, const T *z, T *y, uint32_t rbase, const
     double *part = ctx->red.partials + (size_t)S_CG_DELTA * kMaxPartials;
     int *ring = publish ? ctx->ring_dev : nullptr;
 #define PP_RKA(W, P)                                                                                              \
-    k_red_ka<T, W, P><<<g, kFinishThreads, 0, ctx->stream>>>(E.n, E.off, E.pid, E.ptab, val, E.cbase, z, y, rbase, w, \
-                                                            s, part, np, ctx->scal, ctx->ctl, ring, parity)
+    k_red_ka<T, W, P, red_ka_rows<T, W>()><<<g, kFinishThreads, 0, ctx->stream>>>(                                \
+        E.n, E.off, E.pid, E.ptab, val, E.cbase, z, y, rbase, w, s, part, np, ctx->scal, ctx->ctl, ring, parity)
 #define PP_RKA_P(W) PP_RKA(W, true)
 #define PP_RKA_N(W) PP_RKA(W, false)
     if (E.pid) {

@@ -65,7 +65,7 @@ def parse():
 # (--copy-probe).  Keyed by (grid side, SpMV storage variant); the commands and
 # the raw per-dispatch figures are in profiles/spmv_ell2_poisson128_pmc.md.
 PMC_TRAFFIC = {(128, v): b for v, b in enumerate((224627870, 226711825, 194751552, 197276754, 219112328, 229056082,
-                                                        189937730, 192875293, 154997760))}
+                                                        189937730, 192875293, 154977738))}
 
 
 ELL_VARIANT = 8  # the SpMV variant that runs the sliced-column form

@@ -102,6 +102,9 @@ enum Ctl : int {
     C_SPIN,       // a sync-free sweep hit its wait deadline
     C_PUB,        // iteration bodies that have published their control words
     C_MAXIT,      // iteration cap of the running PCG (read by its check)
+    C_NORM_SET,   // a check of this solve has written its norm (S_TMP0) ...
+    C_NORM_LO,    // ... whose bits are these two words, so that the host
+    C_NORM_HI,    //     reads it with the control words from the ring
     C_COUNT = 16
 };
 

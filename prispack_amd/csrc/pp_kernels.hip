@@ -1541,6 +1541,16 @@ __global__ __launch_bounds__(256) void k_pcg_wv(uint32_t n, const T *__restrict_
     publish_partial(block_sum((double)acc, sh), partials);
 }
 
+// The relative residual norm of a check: scal[S_TMP0] and its bits in the
+// control words (published with them to the host's ring).
+__device__ inline void record_norm(double *scal, int *ctl, double norm) {
+    scal[S_TMP0] = norm;
+    const unsigned long long b = (unsigned long long)__double_as_longlong(norm);
+    ctl[C_NORM_LO] = (int)(unsigned)(b & 0xffffffffull);
+    ctl[C_NORM_HI] = (int)(unsigned)(b >> 32);
+    ctl[C_NORM_SET] = 1;
+}
+
 // Convergence check of CG.cpp:57-63 (run by k_finish on (r, r)): the check
 // index c counts the reference loop index i.
 __device__ void pcg_check(double rr, const double *scal, int *ctl, double *hist, uint32_t hist_cap) {
@@ -1554,8 +1564,7 @@ __device__ void pcg_check(double rr, const double *scal, int *ctl, double *hist,
     const int c = ctl[C_CHECK];
     const double norm = sqrt(rr) / sqrt(scal[S_BB]);
     if (hist && (uint32_t)c - 1u < hist_cap) hist[c - 1] = norm;
-    double *s = const_cast<double *>(scal);
-    s[S_TMP0] = norm;
+    record_norm(const_cast<double *>(scal), ctl, norm);
     if (!(norm == norm) || isinf(norm)) {
         ctl[C_STATUS] = PP_ERR_NOT_FINITE;
         ctl[C_STOP] = 1;
@@ -1823,7 +1832,7 @@ __device__ inline bool red_check(const double t[3], int prior_stop, double *scal
             scal[S_CG_DELTA] = delta;
             scal[S_CG_GAMMA] = gamma;
             scal[S_CG_RHO] = rho;
-            if (!last) scal[S_TMP0] = norm;
+            if (!last) record_norm(scal, ctl, norm);
             const bool next_last = !stop && c >= maxit - 1;
             if (stop) {
                 // the state a resumed solve continues from

@@ -66,6 +66,10 @@ struct PcgResult {
 // key: everything the body's kernels are launched with; an instantiated
 // graph with the same key is reused (capture and instantiation cost more
 // than a few iterations).
+// On return ctl holds the final control words and *norm the last check's
+// relative norm (scal[S_TMP0]): where a body published the stop with its
+// norm (C_NORM_SET), both come from the ring and the host waits once, for
+// the bodies still in flight to drain; otherwise they are read after it.
 // The cached graph of `body` under `key` (captured on first use, as in
 // drive_bodies), launched once on the context's stream.
 template <class Body>
@@ -95,7 +99,7 @@ void run_graph(pp_ctx *ctx, const std::vector<uint64_t> &key, Body &body) {
 
 template <class Body>
 void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays, int *ctl, bool check_first,
-                  const std::vector<uint64_t> &key) {
+                  const std::vector<uint64_t> &key, double *norm) {
     Graph *G = nullptr;
     std::unique_ptr<Graph> fresh;
     if (use_graph) {
@@ -142,7 +146,7 @@ void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays,
             }
         }
     };
-    bool run = true;
+    bool run = true, published = false;
     if (check_first) {
         read_ctl(ctx, ctl);
         PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
@@ -155,21 +159,31 @@ void drive_bodies(pp_ctx *ctx, Body &body, bool use_graph, uint64_t max_replays,
             for (int i = 0; i < C_COUNT; ++i) ctl[i] = slots[(checked % R) * C_COUNT + i];
             ++checked;
             PP_CHECK(!ctl[C_SPIN], PP_ERR_HIP, "sync-free triangular sweep timed out waiting for a dependency");
+            if (ctl[C_STOP]) published = true;  // bodies after the stop leave the words as they are
             if (ctl[C_STOP] || checked >= max_replays) break;
         }
-        PP_HIP(hipStreamSynchronize(ctx->stream));
     }
-    read_ctl(ctx, ctl);
+    if (published && ctl[C_NORM_SET]) {
+        PP_HIP(hipStreamSynchronize(ctx->stream));  // no later body may publish into the next solve's ring
+        const uint64_t bits = (uint64_t)(uint32_t)ctl[C_NORM_HI] << 32 | (uint32_t)ctl[C_NORM_LO];
+        std::memcpy(norm, &bits, sizeof(double));
+        return;
+    }
+    PP_HIP(hipMemcpyAsync(ctx->pinned_ctl, ctx->ctl, C_COUNT * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipMemcpyAsync(ctx->pinned, ctx->scal + S_TMP0, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    PP_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(ctl, ctx->pinned_ctl, C_COUNT * sizeof(int));
+    *norm = ctx->pinned[0];
 }
 
-// Result of a device PCG from its final control words.
-PcgResult collect_pcg(pp_ctx *ctx, const int *ctl, uint32_t maxit, bool hist_on) {
+// Result of a device PCG from its final control words and last norm.
+PcgResult collect_pcg(pp_ctx *ctx, const int *ctl, double norm, uint32_t maxit, bool hist_on) {
     PcgResult res;
     res.conv = ctl[C_CONV];
     res.status = ctl[C_STATUS];
     const int m = ctl[C_M];
     res.iters = m + 1;  // CG.cpp:81
-    res.final_error = read_slot(ctx, S_TMP0);
+    res.final_error = norm;
     if (ctl[C_CHECK] <= 1 && !res.conv && maxit <= 1) res.final_error = -1;
     res.nhist = std::max(0, std::min<int>(m, (int)maxit));
     if (hist_on) {
@@ -228,8 +242,9 @@ PcgResult pcg_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, bool flexible, const
                                        (uint64_t)flexible, every, (uint64_t)hist_on, (uint64_t)ctx->hist,
                                        ctx->hist_cap, (uint64_t)r, (uint64_t)w, (uint64_t)d, (uint64_t)v,
                                        (uint64_t)x, (uint64_t)b, variant_key(A, M)};
-    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, true, key);
-    return collect_pcg(ctx, ctl, maxit, hist_on);
+    double norm;
+    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, true, key, &norm);
+    return collect_pcg(ctx, ctl, norm, maxit, hist_on);
 }
 
 // Single-reduction PCG (k_cgsr_finish / k_cgsr_update): per iteration the
@@ -289,8 +304,9 @@ PcgResult pcg_sr_device(pp_ctx *ctx, pp_csr *A, pp_precond *M, const T *b, T *x,
                                        (uint64_t)l0, every, (uint64_t)hist_on, (uint64_t)ctx->hist, ctx->hist_cap,
                                        (uint64_t)r, (uint64_t)u, (uint64_t)w, (uint64_t)p, (uint64_t)s,
                                        (uint64_t)x, (uint64_t)b, variant_key(A, M)};
-    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false, key);
-    return collect_pcg(ctx, ctl, maxit, hist_on);
+    double norm;
+    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false, key, &norm);
+    return collect_pcg(ctx, ctl, norm, maxit, hist_on);
 }
 
 // CG on the reduced system of a two-colour IC(0) (ReducedForm): solves
@@ -380,8 +396,9 @@ PcgResult pcg_reduced_device(pp_ctx *ctx, ReducedForm *R, const T *rin, T *e, do
                                        (uint64_t)x,
                                        variant_key(R->KA, nullptr) << 16 | variant_key(R->KB, nullptr),
                                        (uint64_t)fused};
-    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false, key);
-    PcgResult res = collect_pcg(ctx, ctl, maxit, false);
+    double norm;
+    drive_bodies(ctx, body, use_graph, (uint64_t)maxit / every + 2, ctl, false, key, &norm);
+    PcgResult res = collect_pcg(ctx, ctl, norm, maxit, false);
     red_post<T>(ctx, n0, n1, x, s1, rin, z, e);
     launch_spmv(ctx, R->KE, DT, 1.0, z, 0.0, e, -1, nullptr, false);
     return res;

@@ -50,10 +50,47 @@ namespace pp {
 
 void set_device(pp_ctx *ctx) { PP_HIP(hipSetDevice(ctx->device)); }
 
+static thread_local pp_ctx *tl_pool = nullptr;
+PoolScope::PoolScope(pp_ctx *ctx) : prev(tl_pool) { tl_pool = ctx; }
+PoolScope::~PoolScope() { tl_pool = prev; }
+
+static void *pool_take(pp_ctx *ctx, size_t bytes) {
+    auto it = ctx->pool.lower_bound(bytes);
+    if (it == ctx->pool.end() || it->first > bytes + bytes / 4) return nullptr;
+    if (ctx->pool_dirty) {
+        PP_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->pool_dirty = false;
+    }
+    void *p = it->second;
+    ctx->pool_bytes -= it->first;
+    ctx->pool.erase(it);
+    return p;
+}
+
+void pool_put(pp_ctx *ctx, void *p, size_t bytes) {
+    if (!p) return;
+    if (ctx->pool_bytes + bytes > kPoolCap) {
+        dfree(p);
+        return;
+    }
+    ctx->pool.emplace(bytes, p);
+    ctx->pool_bytes += bytes;
+    ctx->pool_dirty = true;
+}
+
+void pool_release(pp_ctx *ctx) {
+    for (auto &e : ctx->pool) dfree(e.second);
+    ctx->pool.clear();
+    ctx->pool_bytes = 0;
+    ctx->pool_dirty = false;
+}
+
 template <class T>
 T *dalloc(size_t n) {
     void *p = nullptr;
     if (n == 0) n = 1;
+    if (tl_pool)
+        if (void *q = pool_take(tl_pool, n * sizeof(T))) return (T *)q;
     hipError_t e = hipMalloc(&p, n * sizeof(T));
     if (e != hipSuccess) throw Error(PP_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     return (T *)p;
@@ -520,6 +557,9 @@ int pp_ctx_create(int device, pp_ctx **out) {
 
 int pp_ctx_set_stream(pp_ctx *ctx, void *s) {
     if (!ctx) return PP_ERR_INVALID;
+    // pooled blocks may still be in use by work queued on the old stream
+    if (ctx->pool_dirty && hipStreamSynchronize(ctx->stream) != hipSuccess) return PP_ERR_HIP;
+    ctx->pool_dirty = false;
     ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
     return PP_OK;
 }
@@ -534,6 +574,7 @@ void pp_ctx_destroy(pp_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    pool_release(ctx);
     dfree(ctx->red.partials);
     dfree(ctx->scal);
     dfree(ctx->ctl);

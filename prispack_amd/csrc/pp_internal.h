@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <string>
 #include <functional>
+#include <map>
 #include <vector>
 
 #include "../../include/prispack_amd.h"
@@ -136,6 +137,11 @@ struct pp_ctx {
     uint32_t hist_cap = 0;
     double *pinned = nullptr;     // pinned host mirror (scalars, 64 doubles)
     void *flush_buf = nullptr;    // cache-flush buffer of the variant timers (allocated on first use)
+    // device blocks of finished setup temporaries (size -> block), taken by
+    // later allocations inside a PoolScope on this context; freed with it
+    std::multimap<size_t, void *> pool;
+    size_t pool_bytes = 0;
+    bool pool_dirty = false;  // blocks returned since the stream's last synchronization
     int *pinned_ctl = nullptr;    // pinned host mirror of ctl
     int *ring_host = nullptr;     // mapped pinned slots the PCG bodies publish ctl into
     int *ring_dev = nullptr;      // the same memory as the device sees it
@@ -483,6 +489,20 @@ void set_device(pp_ctx *ctx);
 template <class T>
 T *dalloc(size_t n);
 void dfree(void *p);
+// Inside a PoolScope, dalloc first takes a block of the context's pool that
+// is at most a quarter larger than asked (after synchronizing the stream
+// once blocks were returned, so queued work is done with them).  pool_put
+// returns a block whose last users are queued on ctx->stream; beyond
+// kPoolCap bytes it is freed instead.  hipFree costs 0.1-0.3 ms per
+// megabyte-sized block, the setup's temporaries number dozens.
+struct PoolScope {
+    pp_ctx *prev;
+    explicit PoolScope(pp_ctx *ctx);
+    ~PoolScope();
+};
+constexpr size_t kPoolCap = size_t(4) << 30;
+void pool_put(pp_ctx *ctx, void *p, size_t bytes);
+void pool_release(pp_ctx *ctx);
 void ensure_hist(pp_ctx *ctx, uint32_t cap);
 void *ws_get(pp_ctx *ctx, int idx, size_t bytes);
 // ---- zero-allocation solves (pp_noalloc.hip) ----

@@ -1227,6 +1227,7 @@ int pp_precond_setup(pp_ctx *ctx, pp_precond *M) {
     return guarded_pc(ctx, [&] {
         PP_CHECK(M, PP_ERR_INVALID, "NULL preconditioner");
         if (M->ready) return;  // setup() is idempotent (IncompleteCholesky.cpp:105)
+        PoolScope pool(ctx);
         auto t0 = std::chrono::steady_clock::now();
         if (M->opts.kind == PP_PC_NONE) {
             M->mnnz = 0;

@@ -425,6 +425,7 @@ ReducedForm *reduced_form(pp_ctx *ctx, pp_precond *M, pp_csr *As) {
         return M->reduced;
     }
     reduced_free(M);
+    PoolScope pool(ctx);
     std::unique_ptr<ReducedForm> R(new ReducedForm);
     R->src_uid = As->uid;
     R->src_version = As->version;

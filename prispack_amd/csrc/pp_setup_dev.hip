@@ -554,22 +554,27 @@ __global__ void k_sd_differ(uint32_t cnt, const uint32_t *__restrict__ a, const 
 }
 
 // ---- owned device arrays, freed unless handed over
+// The setup's temporaries: returned to the context's pool at the end
+// (kept ones leave with their new owner)
 struct Arena {
-    std::vector<void *> p;
+    pp_ctx *ctx;
+    std::vector<std::pair<void *, size_t>> p;
+    explicit Arena(pp_ctx *c) : ctx(c) {}
     template <class T>
     T *get(size_t n) {
-        T *a = dalloc<T>(std::max<size_t>(n, 1));
-        p.push_back(a);
+        n = std::max<size_t>(n, 1);
+        T *a = dalloc<T>(n);
+        p.emplace_back(a, n * sizeof(T));
         return a;
     }
     template <class T>
     T *keep(T *a) {  // no longer freed here
         for (auto &q : p)
-            if (q == a) q = nullptr;
+            if (q.first == a) q.first = nullptr;
         return a;
     }
     ~Arena() {
-        for (void *q : p) dfree(q);
+        for (auto &q : p) pool_put(ctx, q.first, q.second);
     }
 };
 
@@ -788,7 +793,7 @@ bool setup_mc_device(pp_ctx *ctx, pp_precond *M) {
     pp_csr *A = M->A;
     const uint32_t n = A->nrow;
     if (n != A->ncol || n == 0 || A->h_rp.size() != (size_t)n + 1) return false;
-    Arena ar;
+    Arena ar(ctx);
     int *st = ar.get<int>(2);
     int *fail = ar.get<int>(1);
     PP_HIP(hipMemsetAsync(fail, 0, sizeof(int), ctx->stream));

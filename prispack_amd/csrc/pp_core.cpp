@@ -415,10 +415,28 @@ static void csr_tune_spmv(pp_ctx *ctx, pp_csr *A, int dtype) {
         launch_fill_zero(ctx, x, (size_t)A->ncol * sizeof(double));
         float best = 1e30f;
         int best_var = variant;
+        std::vector<std::pair<uint64_t, int>> cand;  // (bytes streamed, variant)
         for (int var = 0; var < kVarCount; ++var) {
             if ((var & kVarIdx16) && !A->ci16) continue;
             // non-temporal loads evict what the iteration's next launch reads
             if (A->tune_warm && (var & kVarNT)) continue;
+            uint64_t bytes = 0;
+            variant = var;
+            pp_csr_spmv_bytes(A, dtype, &bytes);
+            cand.emplace_back(bytes, var);
+        }
+        // against a sliced-column form only the two CSR variants streaming the
+        // fewest bytes are timed (the others move more through HBM, and each
+        // cold timing costs a cache flush)
+        if (A->ell.ok && cand.size() > 2) {
+            std::stable_sort(cand.begin(), cand.end(),
+                             [](const std::pair<uint64_t, int> &a, const std::pair<uint64_t, int> &b) {
+                                 return a.first < b.first;
+                             });
+            cand.resize(2);
+        }
+        for (const auto &c : cand) {
+            const int var = c.second;
             variant = var;
             const float ms = time_cold_ms(
                 ctx, [&] { launch_spmv(ctx, A, dtype, 1.0, x, 0.0, y, -1, nullptr, false); }, 3, !A->tune_warm);
